@@ -1,0 +1,36 @@
+/*
+ * fel_oracle.h -- CPU restatement of the per-timestep solver of
+ * toruneko1129/fel-multi-phase-flow (Fortran, srcs/ Fortran sources).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
+ * host driver, the Python package) includes, links or calls this code.  It is
+ * the checker that the GPU path is compared against in tests/, in
+ * __graft_entry__.smoke() and in the cpu_baseline leg of bench.py.
+ *
+ * Every routine keeps the argument list of the Fortran subroutine it
+ * restates (all arguments by reference, column-major arrays with bounds
+ * (-2:ni+3,-2:nj+3,-2:nk+3)), under the name orc_<subroutine>, so one test
+ * harness can call the oracle, the compiled reference and the GPU library
+ * with identical actual arguments.
+ */
+#ifndef FEL_ORACLE_H
+#define FEL_ORACLE_H
+
+#include <stddef.h>
+
+/* Flat offset of (i,j,k) in an array declared (-2:ni+3,-2:nj+3,-2:nk+3). */
+#define OIX(i, j, k) \
+  ((ptrdiff_t)((i) + 2) + (ptrdiff_t)sx * ((ptrdiff_t)((j) + 2) + (ptrdiff_t)sy * (ptrdiff_t)((k) + 2)))
+
+#define ODIMS(ni, nj, nk)              \
+  const ptrdiff_t sx = (ni) + 6;       \
+  const ptrdiff_t sy = (nj) + 6;       \
+  const ptrdiff_t sz = (nk) + 6;       \
+  const ptrdiff_t n3 = sx * sy * sz;   \
+  (void)n3
+
+/* Wall/periodic neighbour table of the reference (srcs/param.h:1-8). */
+enum { ORC_X_MINUS = 1, ORC_Y_MINUS = 2, ORC_Z_MINUS = 3,
+       ORC_X_PLUS = 4, ORC_Y_PLUS = 5, ORC_Z_PLUS = 6 };
+
+#endif
