@@ -1,0 +1,243 @@
+// felgpu.hpp -- shared definitions of the gfx950 implementation of the
+// per-timestep solver of toruneko1129/fel-multi-phase-flow.
+//
+// Arrays keep the reference's image: (-2:ni+3,-2:nj+3,-2:nk+3), x fastest,
+// FP64 (srcs/allocate.h:7-134).  Offsets are 32-bit: the largest grid that
+// fits (512x256x512 plus ghosts, 6 components) stays below 2^31 elements and
+// felgpu_check_dims() rejects anything larger.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+#define FEL_CHECK(call)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      std::fprintf(stderr, "felgpu: %s failed at %s:%d: %s\n", #call, __FILE__, __LINE__, \
+                   hipGetErrorString(e_));                                              \
+      std::abort();                                                                     \
+    }                                                                                   \
+  } while (0)
+
+namespace fel {
+
+// Grid extents and flat offsets of the padded image.
+struct Dims {
+  int ni, nj, nk;
+  int sx, sy, sz;  // padded extents ni+6, nj+6, nk+6
+  int n3;          // sx*sy*sz
+  __host__ __device__ __forceinline__ int ix(int i, int j, int k) const {
+    return (i + 2) + sx * ((j + 2) + sy * (k + 2));
+  }
+};
+
+Dims make_dims(int ni, int nj, int nk);
+
+// Inclusive index box [i0,i1]x[j0,j1]x[k0,k1].
+struct Box {
+  int i0, i1, j0, j1, k0, k1;
+  __host__ __device__ int nx() const { return i1 - i0 + 1; }
+  __host__ __device__ int ny() const { return j1 - j0 + 1; }
+  __host__ __device__ int nz() const { return k1 - k0 + 1; }
+};
+
+// ---------------------------------------------------------------------------
+// Runtime: stream, device mirrors of host arrays, scalar read-back.
+// ---------------------------------------------------------------------------
+hipStream_t stream();
+void ensure_init();
+
+// How an argument array is used by one drop-in call.
+enum Access { kRead = 1, kWrite = 2, kReadWrite = 3 };
+
+// Resolves the host array arguments of one drop-in call to device memory.
+// Arrays registered with felgpu_register_ are device-resident: the device
+// copy is the current one and no transfer happens.  Any other host pointer is
+// mirrored for the duration of the call: copied to the device before the
+// kernels (every access, so partial writes keep the untouched elements) and
+// back afterwards when written.  finish() completes those copies.
+class Call {
+ public:
+  explicit Call(const char* name);
+  ~Call();
+  double* arr(const void* host, size_t count, Access a, const char* what);
+  double* in(const void* host, size_t count, const char* what = "") {
+    return arr(host, count, kRead, what);
+  }
+  double* out(const void* host, size_t count, const char* what = "") {
+    return arr(host, count, kWrite, what);
+  }
+  double* io(const void* host, size_t count, const char* what = "") {
+    return arr(host, count, kReadWrite, what);
+  }
+  void finish();
+
+ private:
+  const char* name_;
+  struct Pending {
+    void* host;
+    double* dev;
+    size_t bytes;
+  };
+  Pending pend_[64];
+  int npend_ = 0;
+  bool finished_ = false;
+};
+
+// A deferred producer: the 45 viscous coefficient arrays of
+// cal_arith_coef_vis_ are a pure function of the viscosity.  When all of
+// them are device-resident the entry point only records where they come
+// from; the Gauss-Seidel solve, which is their only consumer in the
+// reference's step, recomputes the values on the fly from the viscosity.
+// Any other access to one of the arrays (an argument of another entry point,
+// a download), or a write to the viscosity, materialises them first.
+struct DeferredCoef {
+  bool active = false;
+  const void* out_host[45];
+  double* out_dev[45];
+  const void* src_host;
+  const double* src_dev;
+  Dims d;
+  double dxinv, dyinv, dzinv;
+};
+DeferredCoef& deferred_coef();
+bool deferred_coef_disabled();                    // FELGPU_EAGER_COEF=1 turns deferral off
+bool is_registered(const void* host, size_t count);  // whole range device-resident?
+void deferred_coef_flush();                       // materialise now, if pending
+void deferred_coef_check(const void* host, bool write);  // flush if host conflicts
+void coef_vis(const Dims d, double dxinv, double dyinv, double dzinv, const double* rmu,
+              double* const* out45);
+
+// Pinned host scratch for small device->host read-backs (reductions).
+double* pinned_scalars();  // 64 doubles
+void sync();
+
+// Device scratch reused across calls (grows, never shrinks).
+void* scratch(int slot, size_t bytes);
+
+// Launch helpers --------------------------------------------------------------
+
+// Kernel over an index box, one thread per cell, 64x4 thread blocks.
+template <class F>
+__global__ void __launch_bounds__(256) box_kernel(Box b, F f) {
+  const int i = b.i0 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = b.j0 + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = b.k0 + (int)blockIdx.z;
+  if (i <= b.i1 && j <= b.j1) f(i, j, k);
+}
+
+template <class F>
+inline void launch_box(const Box& b, F f) {
+  if (b.nx() <= 0 || b.ny() <= 0 || b.nz() <= 0) return;
+  dim3 grid((b.nx() + 63) / 64, (b.ny() + 3) / 4, b.nz());
+  box_kernel<F><<<grid, dim3(64, 4, 1), 0, stream()>>>(b, f);
+  FEL_CHECK(hipGetLastError());
+}
+
+// Kernel over a flat range [0,n), grid-stride.
+template <class F>
+__global__ void __launch_bounds__(256) flat_kernel(int n, F f) {
+  for (int m = (int)(blockIdx.x * 256 + threadIdx.x); m < n; m += (int)(gridDim.x * 256)) f(m);
+}
+
+template <class F>
+inline void launch_flat(int n, F f) {
+  if (n <= 0) return;
+  int blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  flat_kernel<F><<<dim3(blocks), dim3(256), 0, stream()>>>(n, f);
+  FEL_CHECK(hipGetLastError());
+}
+
+// Deterministic FP64 reductions over the interior cells [1,n]^3.  Stage 1:
+// fixed grid, each thread sums a fixed strided subset, fixed LDS tree per
+// block.  Stage 2: one block sums the block partials in fixed order and
+// hands the NV totals to g(total, out), which runs on the device, so a
+// following kernel can consume derived scalars without a host round trip.
+constexpr int kRedBlocks = 1024;
+
+template <int NV, class F>
+__global__ void __launch_bounds__(256) sum_stage1(Dims d, F f, double* part) {
+  __shared__ double sh[NV][256];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = 0.0;
+  const int nxy = d.ni * d.nj, n = nxy * d.nk;
+  for (int m = (int)(blockIdx.x * 256 + threadIdx.x); m < n; m += (int)(gridDim.x * 256)) {
+    const int k = m / nxy, r = m - k * nxy, j = r / d.ni, i = r - j * d.ni;
+    double v[NV];
+    f(i + 1, j + 1, k + 1, v);
+#pragma unroll
+    for (int c = 0; c < NV; c++) acc[c] += v[c];
+  }
+  const int t = (int)threadIdx.x;
+#pragma unroll
+  for (int c = 0; c < NV; c++) sh[c][t] = acc[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int c = 0; c < NV; c++) sh[c][t] += sh[c][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) part[blockIdx.x * NV + c] = sh[c][0];
+  }
+}
+
+template <int NV, class G>
+__global__ void __launch_bounds__(256) sum_stage2(int nb, const double* part, double* out, G g) {
+  __shared__ double sh[NV][256];
+  const int t = (int)threadIdx.x;
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = 0.0;
+  for (int b = t; b < nb; b += 256) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) acc[c] += part[b * NV + c];
+  }
+#pragma unroll
+  for (int c = 0; c < NV; c++) sh[c][t] = acc[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int c = 0; c < NV; c++) sh[c][t] += sh[c][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double tot[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) tot[c] = sh[c][0];
+    g(tot, out);
+  }
+}
+
+// Runs both stages; `out` is device memory.
+template <int NV, class F, class G>
+inline void interior_reduce(const Dims& d, F f, G g, double* out) {
+  double* part = static_cast<double*>(scratch(15, sizeof(double) * NV * kRedBlocks));
+  sum_stage1<NV, F><<<dim3(kRedBlocks), dim3(256), 0, stream()>>>(d, f, part);
+  FEL_CHECK(hipGetLastError());
+  sum_stage2<NV, G><<<dim3(1), dim3(256), 0, stream()>>>(kRedBlocks, (const double*)part, out, g);
+  FEL_CHECK(hipGetLastError());
+}
+
+// Small device buffer for reduction results (slot-indexed, 16 doubles each).
+double* dev_scalars(int slot);
+// Copies n doubles from device to the pinned buffer and waits for them.
+const double* read_scalars(const double* dev, int n);
+
+// Per-kernel event timer used by the benchmark (felgpu_prof_*).
+void prof_begin(int id);
+void prof_end(int id);
+
+}  // namespace fel

@@ -1,0 +1,164 @@
+// k_csf.hip -- continuum-surface-force surface tension: face gradients,
+// staggered unit normals, curvature and force.
+//
+// Reference: srcs/cal_grad_p2uvw.f, srcs/calrn.f, srcs/calrkap.f,
+// srcs/calfst.f (called from srcs/main.f90:545-570).
+#include <cmath>
+
+#include "felgpu.hpp"
+
+namespace fel {
+
+// srcs/cal_grad_p2uvw.f:25-43 (per-j dyinv from dyinv_array)
+void grad_p2uvw(const Dims d, int ID, double dxinv, const double* dyt_dev, double dyu, double dzinv,
+                const double* q, double* qx, double* qy, double* qz) {
+  launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    const double dyinv = dyt_dev ? dyt_dev[j + d.nj * ID + 2] : dyu;
+    const double q0 = q[c];
+    qx[c] = (-q0 + q[c + 1]) * dxinv;
+    qy[c] = (-q0 + q[c + d.sx]) * dyinv;
+    qz[c] = (-q0 + q[c + d.sx * d.sy]) * dzinv;
+  });
+}
+
+// srcs/calrn.f:64-135: one thread per target cell; targets outside the
+// shifted loop range keep the cleared value 0.
+void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const double* px, const double* py,
+           const double* pz, double* rn) {
+  const int n3 = d.n3;
+  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int ii, int jj, int kk) {
+    const int t = d.ix(ii, jj, kk);
+    if (ii < -2 + ip || jj < -2 + jp || kk < -2 + kp) {
+      rn[t] = 0.0;
+      rn[n3 + t] = 0.0;
+      rn[2 * n3 + t] = 0.0;
+      return;
+    }
+    const int X = ip, Y = jp * d.sx, Z = kp * d.sx * d.sy;
+    const int c = t - X - Y - Z;  // (i,j,k) = (ii-ip, jj-jp, kk-kp)
+    double r[3];
+    const double* a[3] = {px, py, pz};
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const double* p = a[m];
+      r[m] = (+p[c] + p[c + X] + p[c + Y] + p[c + X + Y] + p[c + Z] + p[c + X + Z] + p[c + Y + Z] +
+              p[c + X + Y + Z]) *
+             0.125;
+    }
+    const double arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
+    const double inv = 1.0 / arn;
+    rn[t] = r[0] * inv;
+    rn[n3 + t] = r[1] * inv;
+    rn[2 * n3 + t] = r[2] * inv;
+  });
+}
+
+// srcs/calrkap.f:163-240
+void calrkap(const Dims d, double dxinv, double dyinv, double dzinv, const double* rpp,
+             const double* r12, const double* r13, const double* r23, double* rk) {
+  const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    const int C2 = n3, C3 = 2 * n3;
+    double a = 0.0, b = 0.0, e = 0.0;
+    if (k >= -1 && j >= -1 && i <= ni + 2)
+      a = +(-rpp[c] + rpp[c + X]) * dxinv + (-r12[C2 + c - Y] + r12[C2 + c]) * dyinv +
+          (-r13[C3 + c - Z] + r13[C3 + c]) * dzinv;
+    if (k >= -1 && j <= nj + 2 && i >= -1)
+      b = +(-r12[c - X] + r12[c]) * dxinv + (-rpp[C2 + c] + rpp[C2 + c + Y]) * dyinv +
+          (-r23[C3 + c - Z] + r23[C3 + c]) * dzinv;
+    if (k <= nk + 2 && j >= -1 && i >= -1)
+      e = +(-r13[c - X] + r13[c]) * dxinv + (-r23[C2 + c - Y] + r23[C2 + c]) * dyinv +
+          (-rpp[C3 + c] + rpp[C3 + c + Z]) * dzinv;
+    rk[c] = a;
+    rk[C2 + c] = b;
+    rk[C3 + c] = e;
+  });
+}
+
+// srcs/calfst.f:270-290
+void calfst(const Dims d, double st, const double* px, const double* py, const double* pz,
+            const double* rk, double* fu, double* fv, double* fw) {
+  const int n3 = d.n3;
+  launch_flat(n3, [=] __device__(int m) {
+    fu[m] = -(st * rk[m] * px[m]);
+    fv[m] = -(st * rk[n3 + m] * py[m]);
+    fw[m] = -(st * rk[2 * n3 + m] * pz[m]);
+  });
+}
+
+}  // namespace fel
+
+using namespace fel;
+
+extern "C" {
+
+// srcs/cal_grad_p2uvw.f:1-3
+void cal_grad_p2uvw_(const int* ID, const int* svall, const int* ni, const int* nj, const int* nk,
+                     const double* dxinv, const double* dyinv_array, const double* dzinv,
+                     const double* q, double* qx_u, double* qy_v, double* qz_w) {
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_grad_p2uvw_");
+  bool uni = true;
+  const int cnt = *svall + 6;
+  for (int m = 0; m < cnt; m++)
+    if (dyinv_array[m] != dyinv_array[0]) uni = false;
+  const double* table = nullptr;
+  if (!uni) {
+    double* dev = static_cast<double*>(scratch(2, sizeof(double) * cnt));
+    FEL_CHECK(hipMemcpyAsync(dev, dyinv_array, sizeof(double) * cnt, hipMemcpyHostToDevice, stream()));
+    FEL_CHECK(hipStreamSynchronize(stream()));
+    table = dev;
+  }
+  const double* dq = c.in(q, d.n3);
+  double* x = c.io(qx_u, d.n3);
+  double* y = c.io(qy_v, d.n3);
+  double* z = c.io(qz_w, d.n3);
+  grad_p2uvw(d, *ID, *dxinv, table, dyinv_array[0], *dzinv, dq, x, y, z);
+  c.finish();
+}
+
+// srcs/calrn.f:1-2
+void calrn_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
+            const double* dzinv, const int* ip, const int* jp, const int* kp, const double* phix,
+            const double* phiy, const double* phiz, double* rn) {
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calrn_");
+  const double* x = c.in(phix, d.n3);
+  const double* y = c.in(phiy, d.n3);
+  const double* z = c.in(phiz, d.n3);
+  double* r = c.out(rn, 3 * (size_t)d.n3);
+  const double vs = std::sqrt(*dxinv * *dxinv + *dyinv * *dyinv + *dzinv * *dzinv) * 1.0e-30;
+  calrn(d, vs, *ip, *jp, *kp, x, y, z, r);
+  c.finish();
+}
+
+// srcs/calrkap.f:1-2
+void calrkap_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
+              const double* dzinv, const double* rn_pp, const double* rn_12, const double* rn_13,
+              const double* rn_23, double* rkap) {
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calrkap_");
+  const size_t n = 3 * (size_t)d.n3;
+  const double *a = c.in(rn_pp, n), *b = c.in(rn_12, n), *e = c.in(rn_13, n), *f = c.in(rn_23, n);
+  double* r = c.out(rkap, n);
+  calrkap(d, *dxinv, *dyinv, *dzinv, a, b, e, f, r);
+  c.finish();
+}
+
+// srcs/calfst.f:1-3
+void calfst_(const int* ni, const int* nj, const int* nk, const double* rhol, const double* rhog,
+             const double* surface_tension, const double* phix_u, const double* phiy_v,
+             const double* phiz_w, const double* rkap, double* fst_u, double* fst_v, double* fst_w) {
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calfst_");
+  const double *x = c.in(phix_u, d.n3), *y = c.in(phiy_v, d.n3), *z = c.in(phiz_w, d.n3);
+  const double* rk = c.in(rkap, 3 * (size_t)d.n3);
+  double *fu = c.out(fst_u, d.n3), *fv = c.out(fst_v, d.n3), *fw = c.out(fst_w, d.n3);
+  const double st = 2.0 * (*surface_tension) / (*rhol + *rhog);
+  calfst(d, st, x, y, z, rk, fu, fv, fw);
+  c.finish();
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// k_diag.hip -- per-step diagnostics: phase centroid with lap tracking and
+// phase-weighted velocity (device reductions, scalars returned to the host).
+//
+// Reference: srcs/cal_center.f:1-120, srcs/cal_vel.f:1-83 (called every step
+// from srcs/main.f90:663-666).
+#include <cmath>
+
+#include "felgpu.hpp"
+
+using namespace fel;
+
+extern "C" {
+
+// srcs/cal_center.f:1-3 (ipara must be 0)
+void cal_center_(const int* ipara, const int* ID, const int* svall, const int* ni, const int* nj,
+                 const int* nk, const double* xl_, const double* yl_, const double* zl_,
+                 const double* dt, const double* rhog, const double* phi, double* center,
+                 double* center_pre1, double* center_pre2, int* area_pre1, int* area_pre2, int* lap,
+                 double* velocity) {
+  (void)ipara; (void)zl_; (void)rhog;
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_center_");
+  const double* p = c.in(phi, d.n3);
+  const double xl = *xl_, yl = *yl_;
+  const double delta = yl / (double)(*svall);
+  const double s1 = xl * 0.25, s2 = xl * 0.75;
+  if (*area_pre1 == 1 && *area_pre2 == 3) *lap = *lap + 1;
+  const int ap1 = *area_pre1, lp = *lap, id = *ID, njl = *nj;
+  double* s = dev_scalars(6);
+  interior_reduce<4>(
+      d,
+      [=] __device__(int i, int j, int k, double* o) {
+        const double ph = p[d.ix(i, j, k)];
+        double lx = delta * ((double)i - 0.5);
+        const double ly = delta * ((double)(j + id * njl) - 0.5);
+        const double lz = delta * ((double)k - 0.5);
+        if (ap1 == 3 && lx < s1) lx = lx + xl;
+        if (ap1 == 1 && lx > s2) lx = lx - xl;
+        lx = lx + xl * (double)lp;
+        o[0] = ph;
+        o[1] = ph * lx;
+        o[2] = ph * ly;
+        o[3] = ph * lz;
+      },
+      [=] __device__(const double* t, double* out) {
+        for (int m = 0; m < 4; m++) out[m] = t[m];
+      },
+      s);
+  const double* r = read_scalars(s, 4);
+  c.finish();
+  center[0] = r[1] / r[0];
+  center[1] = r[2] / r[0];
+  center[2] = r[3] / r[0];
+  *velocity = (center[0] - *center_pre1) / *dt;
+  *center_pre2 = *center_pre1;
+  *center_pre1 = center[0];
+  double cc = center[0];
+  *area_pre2 = *area_pre1;
+  if (cc > xl) {
+    do {
+      cc = cc - xl;
+    } while (!(cc < xl));
+  }
+  if (cc < s1) *area_pre1 = 1;
+  if (cc > s2) *area_pre1 = 3;
+  if (s1 <= cc && cc <= s2) *area_pre1 = 2;
+}
+
+// srcs/cal_vel.f:121-122 (ipara must be 0)
+void cal_vel_(const int* ipara, const int* ni, const int* nj, const int* nk, const double* radius,
+              const double* rhol, const double* rmul, const double* u, const double* v,
+              const double* w, const double* phi, double* vel, double* re) {
+  (void)ipara;
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_vel_");
+  const double *p = c.in(phi, d.n3), *du = c.in(u, d.n3), *dv = c.in(v, d.n3), *dw = c.in(w, d.n3);
+  double* s = dev_scalars(7);
+  interior_reduce<4>(
+      d,
+      [=] __device__(int i, int j, int k, double* o) {
+        const int m = d.ix(i, j, k);
+        const double ph = p[m];
+        o[0] = ph;
+        o[1] = ph * du[m];
+        o[2] = ph * dv[m];
+        o[3] = ph * dw[m];
+      },
+      [=] __device__(const double* t, double* out) {
+        for (int m = 0; m < 4; m++) out[m] = t[m];
+      },
+      s);
+  const double* r = read_scalars(s, 4);
+  c.finish();
+  const double diameter = *radius * 2.0;
+  for (int m = 0; m < 3; m++) {
+    vel[m] = r[m + 1] / r[0];
+    re[m] = *rhol * vel[m] * diameter / *rmul;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+// k_sor.hip -- semi-implicit (Crank-Nicolson) viscous solve.
+//
+// Reference: srcs/solu_sor4.f:1-308.  Block Gauss-Seidel u -> v -> w; each
+// component is swept over 8 parity colours in the order
+// (111,222), (112,221), (121,212), (211,122), omega = 1, with the ghost
+// refresh bndu + bnd_periodic(component) after every second colour.  The two
+// colours of each pair differ in the parity of all three indices, so no cell
+// of one is a face neighbour of the other and updating both in one launch
+// gives the reference's values exactly; the refresh points are unchanged.
+// The stop test (err/err0 <= 1e-3, at most 21 sweeps) is evaluated on the
+// host from a deterministic device sum of du^2 per sweep.
+//
+// Coefficients: when cal_arith_coef_vis_ deferred them (device-resident
+// arguments), each update recomputes its 15 coefficients from the viscosity
+// with the same expressions (felgpu_visc.hpp), reading one array instead of
+// fifteen; otherwise the 45 arrays are read.
+#include <cmath>
+
+#include "felgpu.hpp"
+#include "felgpu_bnd.hpp"
+#include "felgpu_visc.hpp"
+
+namespace fel {
+
+struct SorArgs {
+  const double *rho, *rhon, *src[3], *rmu;
+  const double* cf[45];
+  double *un, *vn, *wn;
+  double dtinv;
+  VisScales sc;
+};
+
+// Colour-pair base parities (srcs/solu_sor4.f:94-120); the partner is the
+// complement in every index.
+static const int kIs[4] = {1, 1, 1, 2}, kJs[4] = {1, 1, 2, 1}, kKs[4] = {1, 2, 1, 1};
+
+template <int COMP, bool ONFLY>
+__global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is, int js, int ks,
+                                                       int jend, double* part) {
+  const int k = 1 + (int)blockIdx.z;
+  const int q = (k - ks) & 1;
+  const int i = 1 + ((is + q - 1) & 1) + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = 1 + ((js + q - 1) & 1) + 2 * (int)(blockIdx.y * 4 + threadIdx.y);
+  double e = 0.0;
+  if (i <= d.ni && j <= jend) {
+    const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+    const int c = d.ix(i, j, k);
+    double cf[15];
+    if (ONFLY) {
+      if (COMP == 0) coef_u(d, a.sc, a.rmu, c, cf);
+      if (COMP == 1) coef_v(d, a.sc, a.rmu, c, cf);
+      if (COMP == 2) coef_w(d, a.sc, a.rmu, c, cf);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 15; m++) cf[m] = a.cf[15 * COMP + m][c];
+    }
+    const double *un = a.un, *vn = a.vn, *wn = a.wn;
+    if (COMP == 0) {
+      const double r00 = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * 0.25;
+      const double visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
+                           cf[1] * un[c + X] + cf[3] * un[c + Y] + cf[5] * un[c + Z] +
+                           cf[7] * vn[c - Y] + cf[8] * vn[c + X - Y] + cf[9] * vn[c] +
+                           cf[10] * vn[c + X] + cf[11] * wn[c - Z] + cf[12] * wn[c + X - Z] +
+                           cf[13] * wn[c] + cf[14] * wn[c + X]) /
+                          r00;
+      const double du = (visn * 0.5 + a.src[0][c]) / (a.dtinv + cf[6] / r00 * 0.5) - un[c];
+      a.un[c] = un[c] + du;
+      e = du * du;
+    } else if (COMP == 1) {
+      const double r00 = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * 0.25;
+      const double visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
+                           cf[1] * vn[c + X] + cf[3] * vn[c + Y] + cf[5] * vn[c + Z] +
+                           cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Y] +
+                           cf[10] * un[c + Y] + cf[11] * wn[c - Z] + cf[12] * wn[c + Y - Z] +
+                           cf[13] * wn[c] + cf[14] * wn[c + Y]) /
+                          r00;
+      const double dv = (visn * 0.5 + a.src[1][c]) / (a.dtinv + cf[6] / r00 * 0.5) - vn[c];
+      a.vn[c] = vn[c] + dv;
+      e = dv * dv;
+    } else {
+      const double r00 = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * 0.25;
+      const double visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
+                           cf[1] * wn[c + X] + cf[3] * wn[c + Y] + cf[5] * wn[c + Z] +
+                           cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Z] +
+                           cf[10] * un[c + Z] + cf[11] * vn[c - Y] + cf[13] * vn[c] +
+                           cf[12] * vn[c - Y + Z] + cf[14] * vn[c + Z]) /
+                          r00;
+      const double dw = (visn * 0.5 + a.src[2][c]) / (a.dtinv + cf[6] / r00 * 0.5) - wn[c];
+      a.wn[c] = wn[c] + dw;
+      e = dw * dw;
+    }
+  }
+  // block sum of du^2 in a fixed order
+  __shared__ double sh[256];
+  const int t = (int)(threadIdx.y * 64 + threadIdx.x);
+  sh[t] = e;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = sh[0];
+}
+
+struct PartSlots {
+  int off[12], n[12];
+};
+
+__global__ void __launch_bounds__(256) sor_err_kernel(const double* part, PartSlots ps, double* out) {
+  __shared__ double sh[256];
+  const int t = (int)threadIdx.x;
+  double acc = 0.0;
+  for (int s = 0; s < 12; s++)
+    for (int b = t; b < ps.n[s]; b += 256) acc += part[ps.off[s] + b];
+  sh[t] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = sh[0];
+}
+
+struct SorResult {
+  int lp;
+  double err, err0;
+};
+
+SorResult sor4(const Dims d, SorArgs a, bool onfly, bool lo, bool hi, double uwall, double dy,
+               const double* l1, const double* l2, const double* phi, int ndiv) {
+  const int jen = hi ? d.nj - 1 : d.nj;
+  const int gx = ((d.ni + 1) / 2 + 63) / 64;  // cells of one parity per row
+  const int gy = ((d.nj + 1) / 2 + 3) / 4;    // rows of one parity per plane
+  const int nb = gx * gy * d.nk;
+  double* part = static_cast<double*>(scratch(3, sizeof(double) * (size_t)nb * 12));
+  PartSlots ps;
+  for (int s = 0; s < 12; s++) {
+    ps.off[s] = s * nb;
+    ps.n[s] = nb;
+  }
+  double* derr = dev_scalars(1);
+  double* comp_ptr[3] = {a.un, a.vn, a.wn};
+  SorResult res{0, 0.0, 1.0};
+  double err0 = 1.0, err = 0.0;
+  int lp;
+  for (lp = 0; lp <= 20; lp++) {
+    int slot = 0;
+    for (int comp = 0; comp < 3; comp++) {
+      const int jend = comp == 1 ? jen : d.nj;
+      for (int p = 0; p < 4; p++, slot++) {
+        dim3 grid(gx, gy, d.nk), block(64, 4, 1);
+        double* pp = part + ps.off[slot];
+        prof_begin(3);
+        if (onfly) {
+          if (comp == 0) sor_pair_kernel<0, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+          if (comp == 1) sor_pair_kernel<1, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+          if (comp == 2) sor_pair_kernel<2, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+        } else {
+          if (comp == 0) sor_pair_kernel<0, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+          if (comp == 1) sor_pair_kernel<1, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+          if (comp == 2) sor_pair_kernel<2, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+        }
+        FEL_CHECK(hipGetLastError());
+        prof_end(3);
+        prof_begin(4);
+        bndu(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi);
+        periodic(d, comp_ptr[comp]);
+        prof_end(4);
+      }
+    }
+    sor_err_kernel<<<1, 256, 0, stream()>>>(part, ps, derr);
+    FEL_CHECK(hipGetLastError());
+    const double sum = read_scalars(derr, 1)[0];
+    err = std::sqrt(sum / (double)(d.ni * d.nj * d.nk * ndiv)) + 1.0e-99;
+    if (lp == 0) err0 = err;
+    if (err / err0 <= 1.0e-3) break;
+  }
+  if (lp > 20) lp = 20;
+  res.lp = lp;
+  res.err = err;
+  res.err0 = err0;
+  return res;
+}
+
+static int g_last_lp = -1;
+static double g_last_ratio = 0.0;
+
+}  // namespace fel
+
+using namespace fel;
+
+extern "C" {
+
+// Iteration count and err/err0 of the last solu_sor4_ call (the reference
+// prints them every imon_t steps, srcs/solu_sor4.f:298-305).
+void felgpu_sor_last_(int* lp, double* ratio) {
+  *lp = g_last_lp;
+  *ratio = g_last_ratio;
+}
+
+// srcs/solu_sor4.f:1-17 (ipara must be 0)
+void solu_sor4_(const int* ipara, const int* ID, const int* nID, const int* ndiv, const int* ni,
+                const int* nj, const int* nk, const int* key, double* sendjb, double* recvjb,
+                const int* nstep, const int* imon_t, const double* dt, const double* rho,
+                const double* rhon, double* c0, double* c1, double* c2, double* c3, double* c4,
+                double* c5, double* c6, double* c7, double* c8, double* c9, double* c10,
+                double* c11, double* c12, double* c13, double* c14, double* c15, double* c16,
+                double* c17, double* c18, double* c19, double* c20, double* c21, double* c22,
+                double* c23, double* c24, double* c25, double* c26, double* c27, double* c28,
+                double* c29, double* c30, double* c31, double* c32, double* c33, double* c34,
+                double* c35, double* c36, double* c37, double* c38, double* c39, double* c40,
+                double* c41, double* c42, double* c43, double* c44, const double* src_u,
+                const double* src_v, const double* src_w, double* un, double* vn, double* wn,
+                const double* uwall, const double* dy, const double* l1, const double* l2,
+                const double* phi) {
+  (void)key; (void)sendjb; (void)recvjb;
+  if (*ipara != 0) {
+    std::fprintf(stderr, "felgpu: solu_sor4_: ipara=%d, only the single-domain case (0) is supported\n", *ipara);
+    std::abort();
+  }
+  const Dims d = make_dims(*ni, *nj, *nk);
+  double* h[45] = {c0,  c1,  c2,  c3,  c4,  c5,  c6,  c7,  c8,  c9,  c10, c11, c12, c13, c14,
+                   c15, c16, c17, c18, c19, c20, c21, c22, c23, c24, c25, c26, c27, c28, c29,
+                   c30, c31, c32, c33, c34, c35, c36, c37, c38, c39, c40, c41, c42, c43, c44};
+  // on-the-fly coefficients when they are a pending function of the viscosity
+  DeferredCoef& dc = deferred_coef();
+  bool onfly = dc.active && dc.d.ni == d.ni && dc.d.nj == d.nj && dc.d.nk == d.nk;
+  for (int m = 0; m < 45 && onfly; m++) onfly = dc.out_host[m] == (const void*)h[m];
+  Call c("solu_sor4_");
+  SorArgs a;
+  a.rho = c.in(rho, d.n3);
+  a.rhon = c.in(rhon, d.n3);
+  if (onfly) {
+    a.rmu = dc.src_dev;
+    a.sc = vis_scales(dc.dxinv, dc.dyinv, dc.dzinv);
+    for (int m = 0; m < 45; m++) a.cf[m] = nullptr;
+  } else {
+    a.rmu = nullptr;
+    a.sc = vis_scales(0.0, 0.0, 0.0);
+    for (int m = 0; m < 45; m++) a.cf[m] = c.in(h[m], d.n3);
+  }
+  a.src[0] = c.in(src_u, d.n3);
+  a.src[1] = c.in(src_v, d.n3);
+  a.src[2] = c.in(src_w, d.n3);
+  a.un = c.io(un, d.n3);
+  a.vn = c.io(vn, d.n3);
+  a.wn = c.io(wn, d.n3);
+  a.dtinv = 1.0 / *dt;
+  const double* dl1 = c.in(l1, d.n3);
+  const double* dl2 = c.in(l2, d.n3);
+  const double* dphi = c.in(phi, d.n3);
+  const SorResult r = sor4(d, a, onfly, nID[kYMinus] < 0, nID[kYPlus] < 0, *uwall, *dy, dl1, dl2, dphi, *ndiv);
+  c.finish();
+  g_last_lp = r.lp;
+  g_last_ratio = r.err / r.err0;
+  if (*imon_t > 0 && (*nstep % *imon_t) == 0 && *ID == 0)
+    std::printf("Err_solu %10d%10d%20.10E%20.10E%20.10E\n", *nstep, r.lp, r.err / r.err0, r.err, r.err0);
+}
+
+}  // extern "C"

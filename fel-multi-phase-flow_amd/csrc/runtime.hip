@@ -1,0 +1,425 @@
+// runtime.hip -- stream, device mirrors of the host arrays the Fortran host
+// passes by address, scratch memory, scalar read-back and the event timers.
+//
+// The reference allocates every field once on the host (srcs/allocate.h) and
+// passes base addresses (or slice addresses such as phi(-2,-2,-2,l)) to each
+// subroutine.  The drop-in entry points receive the same host addresses and
+// resolve them here.  Registered ranges (felgpu_register_) live on the device
+// for the whole run; a pointer anywhere inside a registered range resolves to
+// the same offset of its device copy.  Unregistered pointers are mirrored per
+// call (copy in, run, copy out), so every entry point also works, slowly, on
+// plain host memory.
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "felgpu.hpp"
+
+namespace fel {
+
+namespace {
+
+struct Range {
+  uintptr_t host;
+  size_t bytes;
+  double* dev;
+};
+
+struct Runtime {
+  bool init = false;
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::map<uintptr_t, Range> reg;         // registered, keyed by host start
+  std::map<uintptr_t, Range> transient;   // per-call mirrors, cached
+  std::vector<void*> scratch_ptr;
+  std::vector<size_t> scratch_size;
+  double* pinned = nullptr;
+  double* dscal = nullptr;
+  // event timers: per id, a list of recorded (begin, end) pairs that are
+  // only read back (and recycled) when the caller asks for the totals, so
+  // timing adds no host synchronisation to the timed region
+  struct Timer {
+    std::vector<hipEvent_t> b, e;
+    int used = 0;
+    bool open = false;
+    double ms = 0.0;
+    long n = 0;
+  };
+  std::vector<Timer> timers;
+  bool prof = false;
+};
+
+Runtime& R() {
+  static Runtime r;
+  return r;
+}
+
+const Range* find_range(const std::map<uintptr_t, Range>& m, uintptr_t p, size_t bytes) {
+  auto it = m.upper_bound(p);
+  if (it == m.begin()) return nullptr;
+  --it;
+  const Range& r = it->second;
+  if (p >= r.host && p + bytes <= r.host + r.bytes) return &r;
+  return nullptr;
+}
+
+}  // namespace
+
+void ensure_init() {
+  Runtime& r = R();
+  if (r.init) return;
+  const char* dev = std::getenv("FELGPU_DEVICE");
+  r.device = dev ? std::atoi(dev) : 0;
+  FEL_CHECK(hipSetDevice(r.device));
+  FEL_CHECK(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+  FEL_CHECK(hipHostMalloc((void**)&r.pinned, 64 * sizeof(double), hipHostMallocDefault));
+  FEL_CHECK(hipMalloc((void**)&r.dscal, 64 * 16 * sizeof(double)));
+  FEL_CHECK(hipMemsetAsync(r.dscal, 0, 64 * 16 * sizeof(double), r.s));
+  r.init = true;
+}
+
+hipStream_t stream() {
+  ensure_init();
+  return R().s;
+}
+
+void sync() { FEL_CHECK(hipStreamSynchronize(stream())); }
+
+Dims make_dims(int ni, int nj, int nk) {
+  if (ni < 4 || nj < 4 || nk < 4) {
+    std::fprintf(stderr, "felgpu: grid %dx%dx%d too small (each extent must be >= 4)\n", ni, nj, nk);
+    std::abort();
+  }
+  const long long n3 = (long long)(ni + 6) * (nj + 6) * (nk + 6);
+  if (n3 * 6 >= (1LL << 31)) {
+    std::fprintf(stderr, "felgpu: grid %dx%dx%d exceeds the 32-bit offset range\n", ni, nj, nk);
+    std::abort();
+  }
+  Dims d;
+  d.ni = ni; d.nj = nj; d.nk = nk;
+  d.sx = ni + 6; d.sy = nj + 6; d.sz = nk + 6;
+  d.n3 = (int)n3;
+  return d;
+}
+
+void* scratch(int slot, size_t bytes) {
+  Runtime& r = R();
+  ensure_init();
+  if ((int)r.scratch_ptr.size() <= slot) {
+    r.scratch_ptr.resize(slot + 1, nullptr);
+    r.scratch_size.resize(slot + 1, 0);
+  }
+  if (r.scratch_size[slot] < bytes) {
+    if (r.scratch_ptr[slot]) {
+      FEL_CHECK(hipStreamSynchronize(r.s));
+      FEL_CHECK(hipFree(r.scratch_ptr[slot]));
+    }
+    FEL_CHECK(hipMalloc(&r.scratch_ptr[slot], bytes));
+    r.scratch_size[slot] = bytes;
+  }
+  return r.scratch_ptr[slot];
+}
+
+double* pinned_scalars() {
+  ensure_init();
+  return R().pinned;
+}
+
+double* dev_scalars(int slot) {
+  ensure_init();
+  return R().dscal + 16 * slot;
+}
+
+const double* read_scalars(const double* dev, int n) {
+  Runtime& r = R();
+  FEL_CHECK(hipMemcpyAsync(r.pinned, dev, sizeof(double) * n, hipMemcpyDeviceToHost, r.s));
+  FEL_CHECK(hipStreamSynchronize(r.s));
+  return r.pinned;
+}
+
+// --------------------------------------------------------------------------
+// Deferred viscous coefficients
+// --------------------------------------------------------------------------
+
+DeferredCoef& deferred_coef() {
+  static DeferredCoef dc;
+  return dc;
+}
+
+bool deferred_coef_disabled() {
+  static const int off = [] {
+    const char* e = std::getenv("FELGPU_EAGER_COEF");
+    return (e && std::atoi(e) != 0) ? 1 : 0;
+  }();
+  return off != 0;
+}
+
+bool is_registered(const void* host, size_t count) {
+  return find_range(R().reg, reinterpret_cast<uintptr_t>(host), count * sizeof(double)) != nullptr;
+}
+
+void deferred_coef_flush() {
+  DeferredCoef& dc = deferred_coef();
+  if (!dc.active) return;
+  dc.active = false;
+  coef_vis(dc.d, dc.dxinv, dc.dyinv, dc.dzinv, dc.src_dev, dc.out_dev);
+}
+
+void deferred_coef_check(const void* host, bool write) {
+  DeferredCoef& dc = deferred_coef();
+  if (!dc.active) return;
+  const uintptr_t p = reinterpret_cast<uintptr_t>(host);
+  const size_t bytes = sizeof(double) * (size_t)dc.d.n3;
+  bool hit = write && p >= reinterpret_cast<uintptr_t>(dc.src_host) &&
+             p < reinterpret_cast<uintptr_t>(dc.src_host) + bytes;
+  for (int m = 0; m < 45 && !hit; m++) {
+    const uintptr_t o = reinterpret_cast<uintptr_t>(dc.out_host[m]);
+    if (p >= o && p < o + bytes) hit = true;
+  }
+  if (hit) deferred_coef_flush();
+}
+
+// --------------------------------------------------------------------------
+// Call: argument resolution
+// --------------------------------------------------------------------------
+
+Call::Call(const char* name) : name_(name) { ensure_init(); }
+
+Call::~Call() {
+  if (!finished_) finish();
+}
+
+double* Call::arr(const void* host, size_t count, Access a, const char* what) {
+  Runtime& r = R();
+  const uintptr_t p = reinterpret_cast<uintptr_t>(host);
+  const size_t bytes = count * sizeof(double);
+  if (host == nullptr) {
+    std::fprintf(stderr, "felgpu: %s: null array argument %s\n", name_, what);
+    std::abort();
+  }
+  deferred_coef_check(host, (a & kWrite) != 0);
+  if (const Range* g = find_range(r.reg, p, bytes)) {
+    return g->dev + (p - g->host) / sizeof(double);
+  }
+  // per-call mirror
+  double* dev = nullptr;
+  auto it = r.transient.find(p);
+  if (it != r.transient.end() && it->second.bytes >= bytes) {
+    dev = it->second.dev;
+  } else {
+    if (it != r.transient.end()) {
+      FEL_CHECK(hipStreamSynchronize(r.s));
+      FEL_CHECK(hipFree(it->second.dev));
+      r.transient.erase(it);
+    }
+    FEL_CHECK(hipMalloc((void**)&dev, bytes));
+    r.transient[p] = Range{p, bytes, dev};
+  }
+  FEL_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, r.s));
+  if (a & kWrite) {
+    if (npend_ >= 64) {
+      std::fprintf(stderr, "felgpu: %s: too many unregistered output arrays\n", name_);
+      std::abort();
+    }
+    pend_[npend_++] = Pending{const_cast<void*>(host), dev, bytes};
+  }
+  return dev;
+}
+
+void Call::finish() {
+  Runtime& r = R();
+  finished_ = true;
+  if (npend_ == 0) return;
+  for (int m = 0; m < npend_; m++)
+    FEL_CHECK(hipMemcpyAsync(pend_[m].host, pend_[m].dev, pend_[m].bytes, hipMemcpyDeviceToHost, r.s));
+  FEL_CHECK(hipStreamSynchronize(r.s));
+  npend_ = 0;
+}
+
+// --------------------------------------------------------------------------
+// Event timers (opt-in; the benchmark reads them)
+// --------------------------------------------------------------------------
+
+void prof_begin(int id) {
+  Runtime& r = R();
+  if (!r.prof) return;
+  if ((int)r.timers.size() <= id) r.timers.resize(id + 1);
+  Runtime::Timer& t = r.timers[id];
+  if (t.used == (int)t.b.size()) {
+    hipEvent_t a, e;
+    FEL_CHECK(hipEventCreate(&a));
+    FEL_CHECK(hipEventCreate(&e));
+    t.b.push_back(a);
+    t.e.push_back(e);
+  }
+  FEL_CHECK(hipEventRecord(t.b[t.used], r.s));
+  t.open = true;
+}
+
+void prof_end(int id) {
+  Runtime& r = R();
+  if (!r.prof || id >= (int)r.timers.size() || !r.timers[id].open) return;
+  Runtime::Timer& t = r.timers[id];
+  FEL_CHECK(hipEventRecord(t.e[t.used], r.s));
+  t.used++;
+  t.open = false;
+}
+
+static void prof_collect() {
+  Runtime& r = R();
+  if (!r.init) return;
+  FEL_CHECK(hipStreamSynchronize(r.s));
+  for (auto& t : r.timers) {
+    for (int m = 0; m < t.used; m++) {
+      float ms = 0.f;
+      FEL_CHECK(hipEventElapsedTime(&ms, t.b[m], t.e[m]));
+      t.ms += ms;
+      t.n += 1;
+    }
+    t.used = 0;
+  }
+}
+
+}  // namespace fel
+
+using namespace fel;
+
+extern "C" {
+
+// Selects the device and creates the library's stream.  Optional: every
+// entry point initialises on first use.
+void felgpu_init_(const int* device) {
+  if (device && *device >= 0 && !R().init) {
+    char buf[16];
+    std::snprintf(buf, sizeof buf, "%d", *device);
+    setenv("FELGPU_DEVICE", buf, 1);
+  }
+  ensure_init();
+}
+
+// Makes [host, host+nbytes) device-resident.  upload != 0 copies the host
+// contents; upload == 0 zero-fills the device copy (the state of a freshly
+// allocated, never-written host array on this platform: large allocations
+// come from zero pages).
+void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload) {
+  ensure_init();
+  Runtime& r = R();
+  const uintptr_t p = reinterpret_cast<uintptr_t>(host);
+  const size_t bytes = (size_t)*nbytes;
+  auto it = r.reg.find(p);
+  if (it != r.reg.end()) {
+    if (it->second.bytes != bytes) {
+      std::fprintf(stderr, "felgpu: host range %p registered twice with different sizes\n", host);
+      std::abort();
+    }
+  } else {
+    // overlap check
+    auto nx = r.reg.lower_bound(p);
+    if (nx != r.reg.end() && nx->second.host < p + bytes) {
+      std::fprintf(stderr, "felgpu: host range %p overlaps a registered range\n", host);
+      std::abort();
+    }
+    if (nx != r.reg.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->second.host + pv->second.bytes > p) {
+        std::fprintf(stderr, "felgpu: host range %p overlaps a registered range\n", host);
+        std::abort();
+      }
+    }
+    double* dev = nullptr;
+    FEL_CHECK(hipMalloc((void**)&dev, bytes));
+    it = r.reg.emplace(p, Range{p, bytes, dev}).first;
+  }
+  if (*upload)
+    FEL_CHECK(hipMemcpyAsync(it->second.dev, host, bytes, hipMemcpyHostToDevice, r.s));
+  else
+    FEL_CHECK(hipMemsetAsync(it->second.dev, 0, bytes, r.s));
+  FEL_CHECK(hipStreamSynchronize(r.s));
+}
+
+static const Range& must_find(const void* host, const char* who) {
+  const Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
+  if (!g) {
+    std::fprintf(stderr, "felgpu: %s: %p is not inside a registered range\n", who, host);
+    std::abort();
+  }
+  return *g;
+}
+
+// Host -> device copy of a whole registered range (after host-side writes).
+void felgpu_upload_(const void* host) {
+  deferred_coef_check(host, true);
+  const Range& g = must_find(host, "felgpu_upload_");
+  FEL_CHECK(hipMemcpyAsync(g.dev, (const void*)g.host, g.bytes, hipMemcpyHostToDevice, stream()));
+  FEL_CHECK(hipStreamSynchronize(stream()));
+}
+
+// Device -> host copy of a whole registered range (before host-side reads).
+void felgpu_download_(void* host) {
+  deferred_coef_check(host, false);
+  const Range& g = must_find(host, "felgpu_download_");
+  FEL_CHECK(hipMemcpyAsync((void*)g.host, g.dev, g.bytes, hipMemcpyDeviceToHost, stream()));
+  FEL_CHECK(hipStreamSynchronize(stream()));
+}
+
+void felgpu_unregister_(const void* host) {
+  deferred_coef_check(host, true);
+  Runtime& r = R();
+  auto it = r.reg.find(reinterpret_cast<uintptr_t>(host));
+  if (it == r.reg.end()) return;
+  FEL_CHECK(hipStreamSynchronize(r.s));
+  FEL_CHECK(hipFree(it->second.dev));
+  r.reg.erase(it);
+}
+
+void felgpu_unregister_all_() {
+  Runtime& r = R();
+  if (!r.init) return;
+  deferred_coef_flush();
+  FEL_CHECK(hipStreamSynchronize(r.s));
+  for (auto& kv : r.reg) FEL_CHECK(hipFree(kv.second.dev));
+  r.reg.clear();
+  for (auto& kv : r.transient) FEL_CHECK(hipFree(kv.second.dev));
+  r.transient.clear();
+}
+
+// Device address of a registered host address (for callers that launch their
+// own work on the library stream, e.g. the test harness).
+void* felgpu_device_ptr_(const void* host) {
+  const Range& g = must_find(host, "felgpu_device_ptr_");
+  return g.dev + (reinterpret_cast<uintptr_t>(host) - g.host) / sizeof(double);
+}
+
+void felgpu_sync_() { sync(); }
+
+void* felgpu_stream_() { return (void*)stream(); }
+
+// Event timers: enable/disable, read (total ms, count) of timer id, reset.
+void felgpu_prof_enable_(const int* on) {
+  ensure_init();
+  R().prof = (*on != 0);
+}
+
+void felgpu_prof_read_(const int* id, double* ms_total, int64_t* count) {
+  Runtime& r = R();
+  prof_collect();
+  if (*id >= 0 && *id < (int)r.timers.size()) {
+    *ms_total = r.timers[*id].ms;
+    *count = r.timers[*id].n;
+  } else {
+    *ms_total = 0.0;
+    *count = 0;
+  }
+}
+
+void felgpu_prof_reset_() {
+  Runtime& r = R();
+  prof_collect();
+  for (auto& t : r.timers) {
+    t.ms = 0.0;
+    t.n = 0;
+  }
+}
+
+}  // extern "C"

@@ -19,6 +19,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
 REF_SO = os.path.join(REPO, "oracle", "_ref", "libfelref.so")
+GPU_SO = os.path.join(REPO, "fel-multi-phase-flow_amd", "libfelgpu.so")
 
 _libs: dict[str, ctypes.CDLL] = {}
 
@@ -114,6 +115,16 @@ def call(lib: ctypes.CDLL, name: str, *args):
     fn = getattr(lib, name)
     fn.restype = None
     fn(*[_arg(a) for a in args])
+
+
+def gpu() -> ctypes.CDLL:
+    if not os.path.exists(GPU_SO):
+        raise RuntimeError(f"{GPU_SO} is not built (run __graft_entry__.build())")
+    return load(GPU_SO)
+
+
+def call_gpu(name, *args):
+    call(gpu(), name + "_", *args)
 
 
 def call_oracle(name, *args):
