@@ -1,0 +1,245 @@
+"""Run configurations and initial states.
+
+The reference hard-codes its parameters (srcs/main.f90:91-249) and is
+periodic in x (i) and z (k) with walls at the j faces (srcs/bndu.f:29,63;
+srcs/bnd_periodic.f:16-43).  The benchmark configurations are stated with
+walls at z, so a configuration's (x, y, z) maps to the reference's (i, k, j):
+the wall-normal axis is j.  Phase convention of srcs/cal_arith_mean.f:18-26:
+phi = 1 takes the "g" properties, so the ambient (light) phase is phi = 1 and
+a droplet or liquid stripe is phi = 0.  Dimensionless groups are turned into
+the reference's dimensional parameters with liquid scales rho_l = 1, U = 1
+and L = the droplet diameter (stripe width for the Couette case):
+mu_l = L/Re, sigma = L/We (or mu_l U / Ca), g = Bo sigma / L^2 acting along
+-j.  zeta (the GNBC friction) follows srcs/main.f90:172.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+
+class FelCfg(ctypes.Structure):
+    """Mirror of fel_cfg (host/fel_driver.f90) and of the oracle's orc_cfg."""
+
+    _fields_ = [
+        ("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int), ("imon_t", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+        ("bet_mthinc", ctypes.c_double), ("rhol", ctypes.c_double), ("rhog", ctypes.c_double),
+        ("rmul", ctypes.c_double), ("rmug", ctypes.c_double), ("surface_tension", ctypes.c_double),
+        ("uwall", ctypes.c_double), ("grvb", ctypes.c_double), ("grvp", ctypes.c_double),
+        ("xl", ctypes.c_double), ("yl", ctypes.c_double), ("zl", ctypes.c_double),
+        ("particle_radius", ctypes.c_double),
+    ]
+
+
+@dataclass
+class WallSide:
+    theta0: float          # static angle [deg]
+    slip: float            # slip length l1 = l2
+    theta_alt: float = 0.0  # second angle of a checkerboard pattern
+    patch: int = 0          # checkerboard patch size in cells (0: uniform wall)
+
+
+@dataclass
+class RunConfig:
+    name: str
+    ni: int
+    nj: int
+    nk: int
+    xl: float
+    yl: float
+    zl: float
+    dt: float
+    rhol: float
+    rhog: float
+    rmul: float
+    rmug: float
+    sigma: float
+    bottom: WallSide
+    top: WallSide
+    uwall: float = 0.0
+    grvp: float = 0.0
+    bet: float = 2.0
+    imon_t: int = 0
+    steps: int = 100
+    init: str = "droplet"
+    params: dict = field(default_factory=dict)
+    radius: float = 5e-4  # cal_vel's particle radius (diagnostic only)
+
+    @property
+    def dx(self):
+        return self.xl / self.ni
+
+    @property
+    def dy(self):
+        return self.yl / self.nj
+
+    @property
+    def dz(self):
+        return self.zl / self.nk
+
+    @property
+    def cells(self):
+        return self.ni * self.nj * self.nk
+
+    @property
+    def shape(self):
+        return (self.nk + 6, self.nj + 6, self.ni + 6)
+
+    def cstruct(self) -> FelCfg:
+        return FelCfg(self.ni, self.nj, self.nk, self.imon_t, self.dx, self.dy, self.dz, self.dt,
+                      self.bet, self.rhol, self.rhog, self.rmul, self.rmug, self.sigma, self.uwall,
+                      0.0, self.grvp, self.xl, self.yl, self.zl, self.radius)
+
+    def scaled(self, ni, nj, nk, **kw) -> "RunConfig":
+        """The same physics on another grid (same domain lengths)."""
+        return replace(self, ni=ni, nj=nj, nk=nk, **kw)
+
+
+def zeta_of(rmul, rmug, l1, l2):
+    """GNBC friction coefficient, srcs/main.f90:172."""
+    return 0.21 * 6.0 * (rmul / l1 + rmug / l2)
+
+
+# ---------------------------------------------------------------------------
+# The benchmark configurations, in reference orientation
+# ---------------------------------------------------------------------------
+
+def sessile_droplet() -> RunConfig:
+    """64^3 unit cube, walls at the j faces with GNBC (slip 0.02), cap of
+    radius 0.25 on the bottom wall at 90 deg, static angle 60 deg, density
+    and viscosity ratios 10, Re 100, We 10, beta 1, dt 1e-4, 100 steps."""
+    D = 0.5
+    rmul = D / 100.0
+    return RunConfig("sessile_droplet", 64, 64, 64, 1.0, 1.0, 1.0, dt=1e-4, rhol=1.0, rhog=0.1,
+                     rmul=rmul, rmug=rmul / 10.0, sigma=D / 10.0,
+                     bottom=WallSide(60.0, 0.02), top=WallSide(60.0, 0.02), bet=1.0, steps=100,
+                     init="cap", params=dict(r=0.25, cx=0.5, cz=0.5))
+
+
+def couette_stripe() -> RunConfig:
+    """256x64x128 (Lx=4, Ly=1, Lz=2), walls moving at -1/+1 in x, a liquid
+    stripe |x-2|<1 with a seeded sinusoidal interface perturbation of 0.5 dx,
+    90 deg, ratios 1, Re 10, Ca 0.1, slip 0.05, dt 5e-4, 2000 steps."""
+    L = 2.0
+    rmu = L / 10.0
+    return RunConfig("couette_stripe", 256, 128, 64, 4.0, 2.0, 1.0, dt=5e-4, rhol=1.0, rhog=1.0,
+                     rmul=rmu, rmug=rmu, sigma=rmu * 1.0 / 0.1,
+                     bottom=WallSide(90.0, 0.05), top=WallSide(90.0, 0.05), uwall=1.0, steps=2000,
+                     init="stripe", params=dict(x0=1.0, x1=3.0, amp=0.5, seed=0))
+
+
+def droplet_impact() -> RunConfig:
+    """512x256x128 (Lx=4, Ly=2, Lz=1): droplet of radius 0.2 at (2,1,0.4)
+    moving at w=-1 onto a GNBC wall at 100 deg (slip 0.02), free-slip top
+    (infinite slip, 90 deg), density ratio 1000, viscosity ratio 100, Re 500,
+    We 50, Bond 1.  dt is not given; 1e-4 (CFL 0.013 at the impact speed)."""
+    D = 0.4
+    rmul = D / 500.0
+    sigma = D / 50.0
+    g = 1.0 * sigma / D ** 2
+    return RunConfig("droplet_impact", 512, 128, 256, 4.0, 1.0, 2.0, dt=1e-4, rhol=1.0, rhog=1e-3,
+                     rmul=rmul, rmug=rmul / 100.0, sigma=sigma,
+                     bottom=WallSide(100.0, 0.02), top=WallSide(90.0, 1e30), grvp=-g, steps=1000,
+                     init="sphere", params=dict(r=0.2, cx=2.0, cy=0.4, cz=1.0, vy=-1.0))
+
+
+def patterned_wall() -> RunConfig:
+    """128x128x64 (Lx=Ly=2, Lz=1): a cap of radius 0.4 at (1,1,0) on a
+    bottom wall checkered in 16-cell patches of 60/120 deg (64 patches),
+    ratios 10, Re 50, We 5, dt 2e-4, 400 steps."""
+    D = 0.8
+    rmul = D / 50.0
+    return RunConfig("patterned_wall", 128, 64, 128, 2.0, 1.0, 2.0, dt=2e-4, rhol=1.0, rhog=0.1,
+                     rmul=rmul, rmug=rmul / 10.0, sigma=D / 5.0,
+                     bottom=WallSide(60.0, 0.02, theta_alt=120.0, patch=16),
+                     top=WallSide(90.0, 0.02), steps=400,
+                     init="cap", params=dict(r=0.4, cx=1.0, cz=1.0))
+
+
+CONFIGS = {
+    "sessile_droplet": sessile_droplet,
+    "couette_stripe": couette_stripe,
+    "droplet_impact": droplet_impact,
+    "patterned_wall": patterned_wall,
+}
+
+
+# ---------------------------------------------------------------------------
+# Initial state
+# ---------------------------------------------------------------------------
+
+def _coords(cfg: RunConfig):
+    i = np.arange(-2, cfg.ni + 4)
+    j = np.arange(-2, cfg.nj + 4)
+    k = np.arange(-2, cfg.nk + 4)
+    xc, yc, zc = (i - 0.5) * cfg.dx, (j - 0.5) * cfg.dy, (k - 0.5) * cfg.dz
+    return xc, yc, zc
+
+
+def _tanh_profile(sdf, cfg: RunConfig):
+    """phi = 1/(1+exp(-2 beta sdf/dcell)), sdf > 0 in the ambient phase,
+    set to exactly 0 or 1 beyond |2 beta sdf/dcell| > 30 (the profile of
+    srcs/init_phi.f with its truncation)."""
+    dcell = (cfg.dx * cfg.dy * cfg.dz) ** (1.0 / 3.0)
+    a = 2.0 * cfg.bet * sdf / dcell
+    phi = 1.0 / (1.0 + np.exp(-np.clip(a, -60, 60)))
+    phi[a > 30] = 1.0
+    phi[a < -30] = 0.0
+    return phi
+
+
+def _wall_map(cfg: RunConfig, side: WallSide):
+    """(nk+6, ni+6) static-angle map of one wall: uniform, or a checkerboard
+    of side.patch-cell squares (periodic in the ghosts)."""
+    if side.patch <= 0:
+        return np.full((cfg.nk + 6, cfg.ni + 6), float(side.theta0))
+    ii = ((np.arange(-2, cfg.ni + 4) - 1) % cfg.ni) // side.patch
+    kk = ((np.arange(-2, cfg.nk + 4) - 1) % cfg.nk) // side.patch
+    return np.where(((kk[:, None] + ii[None, :]) % 2) == 0, side.theta0, side.theta_alt)
+
+
+def initial_state(cfg: RunConfig) -> dict[str, np.ndarray]:
+    """Initial fields in the reference layout (nk+6, nj+6, ni+6): u, v, w,
+    p, phi1 and the wall maps theta_0_array, l1_array, l2_array, zeta_array
+    (set on the rows j=1 and j=nj only, as srcs/init_array_x_stripe.f90:12-46
+    does).  phi1 is set on the rows j=1..nj over the whole padded i,k range
+    (srcs/init_phi_legendre.f90:15-28)."""
+    xc, yc, zc = _coords(cfg)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    sh = cfg.shape
+    u, v, w, p = (np.zeros(sh) for _ in range(4))
+    phi = np.zeros(sh)
+    q = cfg.params
+    if cfg.init in ("sphere", "cap"):
+        cy = q.get("cy", 0.0)
+        d = np.sqrt((X - q["cx"]) ** 2 + (Y - cy) ** 2 + (Z - q["cz"]) ** 2) - q["r"]
+        prof = _tanh_profile(d, cfg)
+    elif cfg.init == "stripe":
+        rng = np.random.default_rng(q.get("seed", 0))
+        ph = rng.uniform(0, 2 * np.pi, 2)
+        amp = q["amp"] * cfg.dx
+        pert = amp * np.sin(2 * np.pi * Z / cfg.zl + ph[0]) * np.sin(np.pi * Y / cfg.yl + ph[1])
+        d = np.maximum(q["x0"] - (X + pert), (X + pert) - q["x1"])
+        prof = _tanh_profile(d, cfg)
+    else:
+        raise ValueError(cfg.init)
+    phi[:, 3:cfg.nj + 3, :] = prof[:, 3:cfg.nj + 3, :]
+    if "vy" in q:
+        # v lives on the j+1/2 faces: liquid fraction of the face
+        face = 0.5 * (phi[:, :-1, :] + phi[:, 1:, :])
+        v[:, :-1, :] = np.where(face < 0.5, q["vy"], 0.0)
+        v[:, :3, :] = 0.0
+        v[:, cfg.nj + 2:, :] = 0.0
+    maps = {n: np.zeros(sh) for n in ("theta_0_array", "l1_array", "l2_array", "zeta_array")}
+    for side, jrow in ((cfg.bottom, 1), (cfg.top, cfg.nj)):
+        th = _wall_map(cfg, side)
+        maps["theta_0_array"][:, jrow + 2, :] = th
+        maps["l1_array"][:, jrow + 2, :] = side.slip
+        maps["l2_array"][:, jrow + 2, :] = side.slip
+        maps["zeta_array"][:, jrow + 2, :] = zeta_of(cfg.rmul, cfg.rmug, side.slip, side.slip)
+    out = dict(u=u, v=v, w=w, p=p, phi1=phi)
+    out.update(maps)
+    return out
