@@ -123,19 +123,43 @@ void* scratch(int slot, size_t bytes);
 // Launch helpers --------------------------------------------------------------
 
 // Kernel over an index box, one thread per cell, 64x4 thread blocks.
+//
+// XCD-aware block order: the 8 XCDs of the chip take workgroups round-robin
+// by dispatch id and each has its own L2.  Dispatch id n is mapped to the
+// logical tile L so that XCD x owns one contiguous range of tiles (x fastest
+// within a row of 64, then j, then k): the tiles an XCD runs back to back
+// are neighbours in j and k, and their stencil halos hit its own L2.
+__device__ __forceinline__ int xcd_tile(int n, int ntiles) {
+  if (ntiles < 0) return n;  // plain dispatch order (the launch passes -ntiles)
+  const int x = n & 7, pos = n >> 3;
+  const int q = ntiles >> 3, r = ntiles & 7;
+  return x * q + (x < r ? x : r) + pos;
+}
+
 template <class F>
-__global__ void __launch_bounds__(256) box_kernel(Box b, F f) {
-  const int i = b.i0 + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = b.j0 + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int k = b.k0 + (int)blockIdx.z;
+__global__ void __launch_bounds__(256) box_kernel(Box b, int gx, int gy, int ntiles, F f) {
+  const int L = xcd_tile((int)blockIdx.x, ntiles);  // ntiles < 0: dispatch order
+  const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
+  const int i = b.i0 + bx * 64 + (int)threadIdx.x;
+  const int j = b.j0 + by * 4 + (int)threadIdx.y;
+  const int k = b.k0 + bz;
   if (i <= b.i1 && j <= b.j1) f(i, j, k);
 }
+
+// Tile-count argument for xcd_tile: n with XCD-contiguous order, -n without.
+// Order per kernel class: 0 = stencil kernels (launch_box), 1 = colour-pair
+// Gauss-Seidel, 2 = CSF normals.  Default: dispatch order for all three
+// (measured on MI355X at 512x128x256: contiguous XCD ranges gave no gain --
+// halo re-reads across XCDs are served by the 256 MB last-level cache --
+// and unbalanced the interface-heavy MTHINC kernel); FELGPU_XCD_ORDER
+// (three digits, e.g. "001") overrides.
+int xcd_arg(int kernel_class, int ntiles);
 
 template <class F>
 inline void launch_box(const Box& b, F f) {
   if (b.nx() <= 0 || b.ny() <= 0 || b.nz() <= 0) return;
-  dim3 grid((b.nx() + 63) / 64, (b.ny() + 3) / 4, b.nz());
-  box_kernel<F><<<grid, dim3(64, 4, 1), 0, stream()>>>(b, f);
+  const int gx = (b.nx() + 63) / 64, gy = (b.ny() + 3) / 4, n = gx * gy * b.nz();
+  box_kernel<F><<<dim3(n), dim3(64, 4, 1), 0, stream()>>>(b, gx, gy, xcd_arg(0, n), f);
   FEL_CHECK(hipGetLastError());
 }
 

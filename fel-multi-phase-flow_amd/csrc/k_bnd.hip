@@ -127,11 +127,173 @@ static void bndu_z(const Dims d, double* uk, double* vk, double* wk) {
   });
 }
 
-void bndu(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
-          const double* l2, const double* phi, bool lo, bool hi) {
+void bndu_phased(const Dims d, double* u, double* v, double* w, double uwall, double dy,
+                 const double* l1, const double* l2, const double* phi, bool lo, bool hi) {
   bndu_walls(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi);
   bndu_x(d, u, v, w);
   bndu_z(d, u, v, w);
+}
+
+// ---------------------------------------------------------------------------
+// bndu (srcs/bndu.f) optionally followed by bnd_periodic of one component
+// (srcs/solu_sor4.f:177-179), as a single in-place launch.
+//
+// The reference runs three ordered passes -- wall rows over all (i,k), x
+// ghosts over all (j,k), z ghosts over all (i,j) -- then the periodic pass.
+// Composing them, the final value of a written cell is a pure function of
+// values the launch never writes:
+//   component q after bndu:  Z(i,j,k) = X(i,j,zs(k)),  X(i,j,k) = W(xs(i),j,k)
+//   with W the wall-row formula (reading rows 1, 2, nj-3..nj of the
+//   pre-call field) on the wall rows and the pre-call value elsewhere, and
+//   xs/zs the reference's source maps (shifted by one for u in x and for w
+//   in z, bndu.f:107-109,144-146);
+//   the swept component after bnd_periodic:  W(px(i), j, pz(k)) with the
+//   exact periodic images px, pz.
+// Every source cell lies outside the launch's write set (checked per
+// component: the shifted maps read column ni+1 of u and plane nk+1 of w,
+// which bndu never writes), so one thread per written cell reproduces the
+// reference's values, corner cells included.
+// ---------------------------------------------------------------------------
+struct WallCoef {
+  double c1, c2;
+};
+
+__device__ __forceinline__ WallCoef wall_coef(const Dims& d, int i, int jw, int k, double dy,
+                                              const double* l1, const double* l2,
+                                              const double* phi) {
+  const int ip = (i + 1) % (d.ni + 3);
+  const int cw = d.ix(i, jw, k);
+  const double phi_av = (phi[cw] + phi[d.ix(ip, jw, k)]) / 2.0;
+  const double inv1 = phi_av / l1[cw];
+  const double inv2 = (1.0 - phi_av) / l2[cw];
+  const double ls = 1.0 / (inv1 + inv2);
+  return WallCoef{(2.0 * dy) / (2.0 * ls + dy), (2.0 * ls - dy) / (2.0 * ls + dy)};
+}
+
+// Wall phase value of component q at (i,j,k); q: 0=u, 1=v, 2=w.
+__device__ __forceinline__ double wall_value(const Dims& d, int q, const double* f, int i, int j, int k,
+                                             double uwall, double dy, const double* l1,
+                                             const double* l2, const double* phi, bool lo, bool hi) {
+  const int nj = d.nj;
+  if (q == 1) {
+    if (lo && j <= 0) {
+      if (j == 0) return 0.0;
+      return -f[d.ix(i, j == -1 ? 1 : 2, k)];
+    }
+    if (hi && j >= nj) {
+      if (j == nj) return 0.0;
+      return -f[d.ix(i, 2 * nj - j, k)];
+    }
+    return f[d.ix(i, j, k)];
+  }
+  if (lo && j <= 0) {
+    const WallCoef cf = wall_coef(d, i, 1, k, dy, l1, l2, phi);
+    const double f1 = f[d.ix(i, 1, k)];
+    const double g0 = (q == 0) ? cf.c1 * (-uwall) + cf.c2 * f1 : cf.c2 * f1;
+    if (j == 0) return g0;
+    const double gm = 2.0 * g0 - f1;
+    if (j == -1) return gm;
+    return 2.0 * gm - g0;
+  }
+  if (hi && j >= nj + 1) {
+    const WallCoef cf = wall_coef(d, i, nj, k, dy, l1, l2, phi);
+    const double fn = f[d.ix(i, nj, k)];
+    const double g1 = (q == 0) ? cf.c1 * uwall + cf.c2 * fn : cf.c2 * fn;
+    if (j == nj + 1) return g1;
+    const double g2 = 2.0 * g1 - fn;
+    if (j == nj + 2) return g2;
+    return 2.0 * g2 - g1;
+  }
+  return f[d.ix(i, j, k)];
+}
+
+__device__ __forceinline__ bool wall_row(int q, int j, int nj, bool lo, bool hi) {
+  return (lo && j <= 0) || (hi && (j >= nj + 1 || (q == 1 && j == nj)));
+}
+
+// bndu source maps: -1 when the index is not written by that pass.
+__device__ __forceinline__ int xsrc(int q, int i, int ni) {
+  if (i == 0) return q == 0 ? ni + 1 : ni;
+  if (i == -1) return q == 0 ? ni : ni - 1;
+  if (i == -2) return q == 0 ? ni - 1 : ni - 2;
+  if (i == ni + 1) return q == 0 ? -1 : 1;
+  if (i == ni + 2) return 2;
+  if (i == ni + 3) return 3;
+  return -1;
+}
+
+__device__ __forceinline__ int zsrc(int q, int k, int nk) {
+  if (k == 0) return q == 2 ? nk + 1 : nk;
+  if (k == -1) return q == 2 ? nk : nk - 1;
+  if (k == -2) return q == 2 ? nk - 1 : nk - 2;
+  if (k == nk + 1) return q == 2 ? -1 : 1;
+  if (k == nk + 2) return 2;
+  if (k == nk + 3) return 3;
+  return -1;
+}
+
+__device__ __forceinline__ int pimg(int i, int n) { return i < 1 ? i + n : (i > n ? i - n : i); }
+
+void bc_fused(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
+              const double* l2, const double* phi, bool lo, bool hi, int swept) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, sy = d.sy;
+  // disjoint regions covering every cell any component may write:
+  //  A: the six z-ghost planes (all i, j)
+  //  B: interior k, rows j in {-2,-1,0,nj,nj+1,nj+2,nj+3} (all i)
+  //  C: interior k, rows 1..nj-1, the six x-ghost columns
+  const int na = 6 * sx * sy, nb = 7 * nk * sx, nc = 6 * nk * (nj - 1);
+  launch_flat(na + nb + nc, [=] __device__(int m) {
+    int i, j, k;
+    if (m < na) {
+      i = m % sx - 2;
+      const int r = m / sx;
+      j = r % sy - 2;
+      const int g = r / sy;
+      k = g < 3 ? g - 2 : nk + g - 2;
+    } else if (m < na + nb) {
+      const int t = m - na;
+      i = t % sx - 2;
+      const int r = t / sx;
+      const int g = r % 7;
+      k = r / 7 + 1;
+      j = g < 3 ? g - 2 : nj + g - 3;
+    } else {
+      const int t = m - na - nb;
+      const int g = t % 6, r = t / 6;
+      j = r % (nj - 1) + 1;
+      k = r / (nj - 1) + 1;
+      i = g < 3 ? g - 2 : ni + g - 2;
+    }
+    double* F[3] = {u, v, w};
+    double val[3];
+    bool wr[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const bool gx = i < 1 || i > ni, gz = k < 1 || k > nk;
+      if (q == swept) {
+        // bnd_periodic after bndu: every ghost/wall cell of the swept component
+        wr[q] = gx || gz || wall_row(q, j, nj, lo, hi);
+        if (wr[q]) val[q] = wall_value(d, q, F[q], pimg(i, ni), j, pimg(k, nk), uwall, dy, l1, l2, phi, lo, hi);
+      } else {
+        const int xs = xsrc(q, i, ni), zs = zsrc(q, k, nk);
+        wr[q] = wall_row(q, j, nj, lo, hi) || xs >= 0 || zs >= 0;
+        if (wr[q]) {
+          const int kk = zs >= 0 ? zs : k;
+          const int ii = xs >= 0 ? xs : i;
+          val[q] = wall_value(d, q, F[q], ii, j, kk, uwall, dy, l1, l2, phi, lo, hi);
+        }
+      }
+    }
+    const int c = d.ix(i, j, k);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (wr[q]) F[q][c] = val[q];
+  });
+}
+
+void bndu(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
+          const double* l2, const double* phi, bool lo, bool hi) {
+  bc_fused(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi, -1);
 }
 
 // srcs/bnd_periodic.f:16-43, x then z, in one pass: a ghost cell takes the
@@ -261,6 +423,34 @@ void bndu_(const int* nID, const int* ni, const int* nj, const int* nk, double* 
   double *u = c.io(uk, d.n3), *v = c.io(vk, d.n3), *w = c.io(wk, d.n3);
   const double *a = c.in(l1, d.n3), *b = c.in(l2, d.n3), *p = c.in(phi, d.n3);
   bndu(d, u, v, w, *uwall, *dy, a, b, p, nID[kYMinus] < 0, nID[kYPlus] < 0);
+  c.finish();
+}
+
+// Cross-check entry point: bndu with the reference's three ordered passes
+// (the product path, bndu_, runs them as one composed launch).  Same
+// arguments as bndu_; swept >= 0 also applies bnd_periodic to that component
+// (0 = u, 1 = v, 2 = w) afterwards, as the Gauss-Seidel refresh does.
+void felgpu_bndu_phased_(const int* nID, const int* ni, const int* nj, const int* nk, double* uk,
+                         double* vk, double* wk, const double* uwall, const double* dy,
+                         const double* l1, const double* l2, const double* phi, const int* swept) {
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("felgpu_bndu_phased_");
+  double *u = c.io(uk, d.n3), *v = c.io(vk, d.n3), *w = c.io(wk, d.n3);
+  const double *a = c.in(l1, d.n3), *b = c.in(l2, d.n3), *p = c.in(phi, d.n3);
+  bndu_phased(d, u, v, w, *uwall, *dy, a, b, p, nID[kYMinus] < 0, nID[kYPlus] < 0);
+  if (*swept >= 0) periodic(d, *swept == 0 ? u : (*swept == 1 ? v : w));
+  c.finish();
+}
+
+// The composed single launch of bndu_ + bnd_periodic(component swept).
+void felgpu_bc_fused_(const int* nID, const int* ni, const int* nj, const int* nk, double* uk,
+                      double* vk, double* wk, const double* uwall, const double* dy,
+                      const double* l1, const double* l2, const double* phi, const int* swept) {
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("felgpu_bc_fused_");
+  double *u = c.io(uk, d.n3), *v = c.io(vk, d.n3), *w = c.io(wk, d.n3);
+  const double *a = c.in(l1, d.n3), *b = c.in(l2, d.n3), *p = c.in(phi, d.n3);
+  bc_fused(d, u, v, w, *uwall, *dy, a, b, p, nID[kYMinus] < 0, nID[kYPlus] < 0, *swept);
   c.finish();
 }
 

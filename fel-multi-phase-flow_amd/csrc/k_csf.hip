@@ -22,36 +22,94 @@ void grad_p2uvw(const Dims d, int ID, double dxinv, const double* dyt_dev, doubl
   });
 }
 
-// srcs/calrn.f:64-135: one thread per target cell; targets outside the
-// shifted loop range keep the cleared value 0.
-void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const double* px, const double* py,
-           const double* pz, double* rn) {
-  const int n3 = d.n3;
-  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int ii, int jj, int kk) {
-    const int t = d.ix(ii, jj, kk);
-    if (ii < -2 + ip || jj < -2 + jp || kk < -2 + kp) {
-      rn[t] = 0.0;
-      rn[n3 + t] = 0.0;
-      rn[2 * n3 + t] = 0.0;
-      return;
+// srcs/calrn.f:64-135.  One thread per target column (ii,jj), marching over
+// a chunk of kk planes: the four source values of plane kk are kept for the
+// next plane, so each plane is loaded once per array instead of twice, and
+// the 8-term sums keep the reference's order (plane kk-kp first).  Targets
+// outside the shifted loop range keep the cleared value 0.
+constexpr int kRnChunk = 16;
+
+__global__ void __launch_bounds__(256) calrn_kernel(Dims d, double verysmall, int ip, int jp, int kp,
+                                                    const double* __restrict__ px,
+                                                    const double* __restrict__ py,
+                                                    const double* __restrict__ pz,
+                                                    double* __restrict__ rn, int gx, int gy,
+                                                    int ntiles) {
+  const int L = xcd_tile((int)blockIdx.x, ntiles);
+  const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
+  const int ii = -2 + bx * 64 + (int)threadIdx.x;
+  const int jj = -2 + by * 4 + (int)threadIdx.y;
+  if (ii > d.ni + 3 || jj > d.nj + 3) return;
+  const int n3 = d.n3, Z = d.sx * d.sy;
+  const int k0 = -2 + bz * kRnChunk, k1 = min(d.nk + 3, k0 + kRnChunk - 1);
+  const bool col_ok = ii >= -2 + ip && jj >= -2 + jp;
+  const int X = ip, Y = jp * d.sx;
+  double pa[3][4];  // previous plane: (i,j), (ii,j), (i,jj), (ii,jj)
+  bool have_prev = false;
+  for (int kk = k0; kk <= k1; kk++) {
+    const int tgt = d.ix(ii, jj, kk);
+    if (!col_ok || kk < -2 + kp) {
+      rn[tgt] = 0.0;
+      rn[n3 + tgt] = 0.0;
+      rn[2 * n3 + tgt] = 0.0;
+      have_prev = false;
+      continue;
     }
-    const int X = ip, Y = jp * d.sx, Z = kp * d.sx * d.sy;
-    const int c = t - X - Y - Z;  // (i,j,k) = (ii-ip, jj-jp, kk-kp)
-    double r[3];
-    const double* a[3] = {px, py, pz};
+    const int c = tgt - X - Y;  // (i, j, kk) with i = ii-ip, j = jj-jp
+    double cur[3][4];
+    const double* a3[3] = {px, py, pz};
 #pragma unroll
     for (int m = 0; m < 3; m++) {
-      const double* p = a[m];
-      r[m] = (+p[c] + p[c + X] + p[c + Y] + p[c + X + Y] + p[c + Z] + p[c + X + Z] + p[c + Y + Z] +
-              p[c + X + Y + Z]) *
-             0.125;
+      const double* p = a3[m];
+      cur[m][0] = p[c];
+      cur[m][1] = p[c + X];
+      cur[m][2] = p[c + Y];
+      cur[m][3] = p[c + X + Y];
     }
+    if (kp == 0 || !have_prev) {
+      if (kp == 0) {
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) pa[m][q] = cur[m][q];
+      } else {
+        const int cb = c - Z;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          const double* p = a3[m];
+          pa[m][0] = p[cb];
+          pa[m][1] = p[cb + X];
+          pa[m][2] = p[cb + Y];
+          pa[m][3] = p[cb + X + Y];
+        }
+      }
+    }
+    double r[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+      r[m] = (+pa[m][0] + pa[m][1] + pa[m][2] + pa[m][3] + cur[m][0] + cur[m][1] + cur[m][2] +
+              cur[m][3]) *
+             0.125;
     const double arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
     const double inv = 1.0 / arn;
-    rn[t] = r[0] * inv;
-    rn[n3 + t] = r[1] * inv;
-    rn[2 * n3 + t] = r[2] * inv;
-  });
+    rn[tgt] = r[0] * inv;
+    rn[n3 + tgt] = r[1] * inv;
+    rn[2 * n3 + tgt] = r[2] * inv;
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) pa[m][q] = cur[m][q];
+    have_prev = true;
+  }
+}
+
+void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const double* px, const double* py,
+           const double* pz, double* rn) {
+  const int gx = (d.sx + 63) / 64, gy = (d.sy + 3) / 4, gz = (d.sz + kRnChunk - 1) / kRnChunk;
+  const int n = gx * gy * gz;
+  calrn_kernel<<<dim3(n), dim3(64, 4, 1), 0, stream()>>>(d, verysmall, ip, jp, kp, px, py, pz, rn,
+                                                         gx, gy, xcd_arg(2, n));
+  FEL_CHECK(hipGetLastError());
 }
 
 // srcs/calrkap.f:163-240

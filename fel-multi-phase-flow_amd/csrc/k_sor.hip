@@ -34,14 +34,23 @@ struct SorArgs {
 // complement in every index.
 static const int kIs[4] = {1, 1, 1, 2}, kJs[4] = {1, 1, 2, 1}, kKs[4] = {1, 2, 1, 1};
 
+// k planes per block: one block sums du^2 over kKB planes, which keeps the
+// number of partial sums (and the final reduction) small.
+constexpr int kKB = 1;
+
 template <int COMP, bool ONFLY>
 __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is, int js, int ks,
-                                                       int jend, double* part) {
-  const int k = 1 + (int)blockIdx.z;
-  const int q = (k - ks) & 1;
-  const int i = 1 + ((is + q - 1) & 1) + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = 1 + ((js + q - 1) & 1) + 2 * (int)(blockIdx.y * 4 + threadIdx.y);
+                                                       int jend, int gx, int gy, int ntiles,
+                                                       double* part) {
+  const int L = xcd_tile((int)blockIdx.x, ntiles);
+  const int bx = L % gx, tt = L / gx, by = tt % gy, bz = tt / gy;
   double e = 0.0;
+  for (int kz = 0; kz < kKB; kz++) {
+  const int k = 1 + bz * kKB + kz;
+  if (k > d.nk) break;
+  const int q = (k - ks) & 1;
+  const int i = 1 + ((is + q - 1) & 1) + 2 * (bx * 64 + (int)threadIdx.x);
+  const int j = 1 + ((js + q - 1) & 1) + 2 * (by * 4 + (int)threadIdx.y);
   if (i <= d.ni && j <= jend) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
@@ -65,7 +74,7 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is
                           r00;
       const double du = (visn * 0.5 + a.src[0][c]) / (a.dtinv + cf[6] / r00 * 0.5) - un[c];
       a.un[c] = un[c] + du;
-      e = du * du;
+      e += du * du;
     } else if (COMP == 1) {
       const double r00 = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * 0.25;
       const double visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
@@ -76,7 +85,7 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is
                           r00;
       const double dv = (visn * 0.5 + a.src[1][c]) / (a.dtinv + cf[6] / r00 * 0.5) - vn[c];
       a.vn[c] = vn[c] + dv;
-      e = dv * dv;
+      e += dv * dv;
     } else {
       const double r00 = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * 0.25;
       const double visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
@@ -87,8 +96,9 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is
                           r00;
       const double dw = (visn * 0.5 + a.src[2][c]) / (a.dtinv + cf[6] / r00 * 0.5) - wn[c];
       a.wn[c] = wn[c] + dw;
-      e = dw * dw;
+      e += dw * dw;
     }
+  }
   }
   // block sum of du^2 in a fixed order
   __shared__ double sh[256];
@@ -99,20 +109,37 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is
     if (t < s) sh[t] += sh[t + s];
     __syncthreads();
   }
-  if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = sh[0];
+  if (t == 0) part[L] = sh[0];
 }
 
 struct PartSlots {
   int off[12], n[12];
 };
 
-__global__ void __launch_bounds__(256) sor_err_kernel(const double* part, PartSlots ps, double* out) {
+// Two-stage fixed-order sum of the 12 launches' block partials: stage 1
+// reduces contiguous chunks (one block each), stage 2 the chunk sums.
+constexpr int kErrChunks = 256;
+
+__global__ void __launch_bounds__(256) sor_err_stage1(const double* part, int n, double* chunk) {
   __shared__ double sh[256];
   const int t = (int)threadIdx.x;
+  const int per = (n + kErrChunks - 1) / kErrChunks;
+  const int b0 = (int)blockIdx.x * per, b1 = min(n, b0 + per);
   double acc = 0.0;
-  for (int s = 0; s < 12; s++)
-    for (int b = t; b < ps.n[s]; b += 256) acc += part[ps.off[s] + b];
+  for (int b = b0 + t; b < b1; b += 256) acc += part[b];
   sh[t] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) chunk[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) sor_err_stage2(const double* chunk, double* out) {
+  __shared__ double sh[256];
+  const int t = (int)threadIdx.x;
+  sh[t] = chunk[t];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (t < s) sh[t] += sh[t + s];
@@ -131,15 +158,16 @@ SorResult sor4(const Dims d, SorArgs a, bool onfly, bool lo, bool hi, double uwa
   const int jen = hi ? d.nj - 1 : d.nj;
   const int gx = ((d.ni + 1) / 2 + 63) / 64;  // cells of one parity per row
   const int gy = ((d.nj + 1) / 2 + 3) / 4;    // rows of one parity per plane
-  const int nb = gx * gy * d.nk;
-  double* part = static_cast<double*>(scratch(3, sizeof(double) * (size_t)nb * 12));
+  const int gz = (d.nk + kKB - 1) / kKB;
+  const int nb = gx * gy * gz;
+  double* part = static_cast<double*>(scratch(3, sizeof(double) * ((size_t)nb * 12 + kErrChunks)));
+  double* chunk = part + (size_t)nb * 12;
   PartSlots ps;
   for (int s = 0; s < 12; s++) {
     ps.off[s] = s * nb;
     ps.n[s] = nb;
   }
   double* derr = dev_scalars(1);
-  double* comp_ptr[3] = {a.un, a.vn, a.wn};
   SorResult res{0, 0.0, 1.0};
   double err0 = 1.0, err = 0.0;
   int lp;
@@ -148,27 +176,27 @@ SorResult sor4(const Dims d, SorArgs a, bool onfly, bool lo, bool hi, double uwa
     for (int comp = 0; comp < 3; comp++) {
       const int jend = comp == 1 ? jen : d.nj;
       for (int p = 0; p < 4; p++, slot++) {
-        dim3 grid(gx, gy, d.nk), block(64, 4, 1);
+        dim3 grid(nb), block(64, 4, 1);
         double* pp = part + ps.off[slot];
         prof_begin(3);
         if (onfly) {
-          if (comp == 0) sor_pair_kernel<0, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
-          if (comp == 1) sor_pair_kernel<1, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
-          if (comp == 2) sor_pair_kernel<2, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+          if (comp == 0) sor_pair_kernel<0, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 1) sor_pair_kernel<1, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 2) sor_pair_kernel<2, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
         } else {
-          if (comp == 0) sor_pair_kernel<0, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
-          if (comp == 1) sor_pair_kernel<1, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
-          if (comp == 2) sor_pair_kernel<2, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, pp);
+          if (comp == 0) sor_pair_kernel<0, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 1) sor_pair_kernel<1, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 2) sor_pair_kernel<2, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
         }
         FEL_CHECK(hipGetLastError());
         prof_end(3);
         prof_begin(4);
-        bndu(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi);
-        periodic(d, comp_ptr[comp]);
+        bc_fused(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp);
         prof_end(4);
       }
     }
-    sor_err_kernel<<<1, 256, 0, stream()>>>(part, ps, derr);
+    sor_err_stage1<<<kErrChunks, 256, 0, stream()>>>(part, 12 * nb, chunk);
+    sor_err_stage2<<<1, 256, 0, stream()>>>(chunk, derr);
     FEL_CHECK(hipGetLastError());
     const double sum = read_scalars(derr, 1)[0];
     err = std::sqrt(sum / (double)(d.ni * d.nj * d.nk * ndiv)) + 1.0e-99;

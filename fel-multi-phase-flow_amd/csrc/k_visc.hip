@@ -286,7 +286,9 @@ void cal_arith_coef_vis_(const int* ni, const int* nj, const int* nk, const doub
   // a pending record for other output arrays is materialised before it is replaced
   DeferredCoef& dc = deferred_coef();
   if (dc.active) {
-    bool same = dc.src_host == (const void*)rmuk;
+    // the same output arrays are about to be redefined: drop the pending
+    // record; other outputs still owe their values
+    bool same = true;
     for (int m = 0; m < 45 && same; m++) same = dc.out_host[m] == (const void*)h[m];
     if (!same) deferred_coef_flush();
     dc.active = false;

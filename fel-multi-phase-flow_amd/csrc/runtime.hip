@@ -155,6 +155,18 @@ bool deferred_coef_disabled() {
   return off != 0;
 }
 
+int xcd_arg(int kernel_class, int ntiles) {
+  static char order[4] = {0, 0, 0, 0};
+  static bool init = false;
+  if (!init) {
+    const char* e = std::getenv("FELGPU_XCD_ORDER");
+    const char* def = "000";
+    for (int m = 0; m < 3; m++) order[m] = (e && (int)std::strlen(e) > m) ? e[m] : def[m];
+    init = true;
+  }
+  return order[kernel_class] == '1' ? ntiles : -ntiles;
+}
+
 bool is_registered(const void* host, size_t count) {
   return find_range(R().reg, reinterpret_cast<uintptr_t>(host), count * sizeof(double)) != nullptr;
 }
@@ -171,13 +183,22 @@ void deferred_coef_check(const void* host, bool write) {
   if (!dc.active) return;
   const uintptr_t p = reinterpret_cast<uintptr_t>(host);
   const size_t bytes = sizeof(double) * (size_t)dc.d.n3;
-  bool hit = write && p >= reinterpret_cast<uintptr_t>(dc.src_host) &&
-             p < reinterpret_cast<uintptr_t>(dc.src_host) + bytes;
-  for (int m = 0; m < 45 && !hit; m++) {
+  for (int m = 0; m < 45; m++) {
     const uintptr_t o = reinterpret_cast<uintptr_t>(dc.out_host[m]);
-    if (p >= o && p < o + bytes) hit = true;
+    if (p >= o && p < o + bytes) {
+      deferred_coef_flush();  // a coefficient array is accessed: materialise
+      return;
+    }
   }
-  if (hit) deferred_coef_flush();
+  const uintptr_t s = reinterpret_cast<uintptr_t>(dc.src_host);
+  if (write && dc.src_host && p >= s && p < s + bytes) {
+    // the viscosity is about to change: keep the values the coefficients
+    // are a function of (16 B/cell) instead of materialising 45 arrays
+    double* snap = static_cast<double*>(scratch(7, bytes));
+    FEL_CHECK(hipMemcpyAsync(snap, dc.src_dev, bytes, hipMemcpyDeviceToDevice, stream()));
+    dc.src_dev = snap;
+    dc.src_host = nullptr;
+  }
 }
 
 // --------------------------------------------------------------------------

@@ -349,3 +349,19 @@ def test_solu_sor4(g):
             [D(0.25), D(g.dy), l1.copy(), l1.copy(), phi.copy()]
     a, b = run_both("solu_sor4", mk)
     check(a, b)
+
+
+@pytest.mark.parametrize("swept", [-1, 0, 1, 2])
+def test_bc_fused_equals_phased(g, swept):
+    """The one-launch ghost refresh equals bndu's three ordered passes
+    followed by bnd_periodic of the swept component, bit for bit, including
+    the shifted x source of u and z source of w."""
+    def mk():
+        r = np.random.default_rng(40 + swept)
+        return [NID.copy(), I(g.ni), I(g.nj), I(g.nk)] + [r.standard_normal(g.shape) for _ in range(3)] + \
+            [D(0.25), D(g.dy), r.uniform(0.01, 0.1, g.shape), r.uniform(0.01, 0.1, g.shape),
+             r.uniform(0, 1, g.shape), I(swept)]
+    a, b = mk(), mk()
+    H.call(H.gpu(), "felgpu_bndu_phased_", *a)
+    H.call(H.gpu(), "felgpu_bc_fused_", *b)
+    check(a, b)
