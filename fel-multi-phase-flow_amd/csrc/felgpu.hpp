@@ -2,9 +2,9 @@
 // per-timestep solver of toruneko1129/fel-multi-phase-flow.
 //
 // Arrays keep the reference's image: (-2:ni+3,-2:nj+3,-2:nk+3), x fastest,
-// FP64 (srcs/allocate.h:7-134).  Offsets are 32-bit: the largest grid that
+// FP64 (srcs/allocate.h:7-165).  Offsets are 32-bit: the largest grid that
 // fits (512x256x512 plus ghosts, 6 components) stays below 2^31 elements and
-// felgpu_check_dims() rejects anything larger.
+// make_dims() rejects anything larger.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,6 +13,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+
+// The public C ABI: every extern "C" definition is checked against it.
+#include "felgpu.h"
 
 #define FEL_CHECK(call)                                                                 \
   do {                                                                                  \

@@ -329,7 +329,7 @@ void periodic(const Dims d, double* q) {
   });
 }
 
-// srcs/bnd_periodic.f:60-90 (bnd_neumann)
+// srcs/bnd_neumann.f:19-41
 void neumann(const Dims d, double* q, bool lo, bool hi) {
   const int nj = d.nj;
   launch_box(Box{-2, d.ni + 3, 0, 0, -2, d.nk + 3}, [=] __device__(int i, int, int k) {
@@ -462,7 +462,7 @@ void bnd_periodic_(const int* ni, const int* nj, const int* nk, double* qk) {
   c.finish();
 }
 
-// srcs/bnd_periodic.f:51
+// srcs/bnd_neumann.f:4
 void bnd_neumann_(const int* nID, const int* ni, const int* nj, const int* nk, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
   Call c("bnd_neumann_");

@@ -22,7 +22,7 @@ void grad_p2uvw(const Dims d, int ID, double dxinv, const double* dyt_dev, doubl
   });
 }
 
-// srcs/calrn.f:64-135.  One thread per target column (ii,jj), marching over
+// srcs/calrn.f:23-87.  One thread per target column (ii,jj), marching over
 // a chunk of kk planes: the four source values of plane kk are kept for the
 // next plane, so each plane is loaded once per array instead of twice, and
 // the 8-term sums keep the reference's order (plane kk-kp first).  Targets
@@ -112,7 +112,7 @@ void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const double*
   FEL_CHECK(hipGetLastError());
 }
 
-// srcs/calrkap.f:163-240
+// srcs/calrkap.f:20-97
 void calrkap(const Dims d, double dxinv, double dyinv, double dzinv, const double* rpp,
              const double* r12, const double* r13, const double* r23, double* rk) {
   const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
@@ -135,7 +135,7 @@ void calrkap(const Dims d, double dxinv, double dyinv, double dzinv, const doubl
   });
 }
 
-// srcs/calfst.f:270-290
+// srcs/calfst.f:37-45
 void calfst(const Dims d, double st, const double* px, const double* py, const double* pz,
             const double* rk, double* fu, double* fv, double* fw) {
   const int n3 = d.n3;

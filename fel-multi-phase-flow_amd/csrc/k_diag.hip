@@ -66,7 +66,7 @@ void cal_center_(const int* ipara, const int* ID, const int* svall, const int* n
   if (s1 <= cc && cc <= s2) *area_pre1 = 2;
 }
 
-// srcs/cal_vel.f:121-122 (ipara must be 0)
+// srcs/cal_vel.f:1-2 (ipara must be 0)
 void cal_vel_(const int* ipara, const int* ni, const int* nj, const int* nk, const double* radius,
               const double* rhol, const double* rmul, const double* u, const double* v,
               const double* w, const double* phi, double* vel, double* re) {

@@ -201,7 +201,7 @@ void fft_inverse(const Dims d, const double* pr, const double* pi, double* dp) {
         });
 }
 
-// srcs/solp_fft_tdma2.f:96-137: Thomas solve along j for every (kx,kz) mode
+// srcs/solp_fft_tdma2.f:46-87: Thomas solve along j for every (kx,kz) mode
 // of the centred layout (i fastest, j, k).  c1x/c1z tables come from the
 // host libm; a(j) is kept in device scratch for the back substitution.
 void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, const double* coef,
@@ -253,7 +253,7 @@ void divergence(const Dims d, double dxinv, double dyinv, double dzinv, double d
       [=] __device__(const double* t, double* out) { out[0] = t[0]; }, dsum);
 }
 
-// srcs/bnd_avzero.f:111-124 with srcs/cal_av.f:142-166: the mean stays on
+// srcs/bnd_avzero.f:10-23 with srcs/cal_av.f:20-37: the mean stays on
 // the device.
 void avzero(const Dims d, double* q, int ndiv) {
   double* s = dev_scalars(2);
@@ -265,7 +265,7 @@ void avzero(const Dims d, double* q, int ndiv) {
   launch_flat(d.n3, [=] __device__(int m) { q[m] = q[m] - *av; });
 }
 
-// srcs/corunp_explicit.f:195-243
+// srcs/corunp_explicit.f:26-74
 void correct(const Dims d, int jen, double rho, double dxinv, double dyinv, double dzinv, double dt,
              const double* dp, const double* phat, double* un, double* vn, double* wn, double* pn) {
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
@@ -366,7 +366,7 @@ void trans_w2r_(const int* ipara, const int* ID, const int* ndiv, const int* sva
   trans_r2w_(ipara, ID, ndiv, svall, key, sendbuf, recvbuf, w, r);
 }
 
-// srcs/solp_fft_tdma2.f:54-57.  The 1-D coefficient arrays (as/an/ap_p_fft,
+// srcs/solp_fft_tdma2.f:4-5.  The 1-D coefficient arrays (as/an/ap_p_fft,
 // declared (-2:nj+3)) are host arrays read here; atdma/btdma_r/btdma_i (the
 // reference's per-line work arrays) are not used.
 void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const double* rho,
@@ -404,7 +404,7 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
   FEL_CHECK(hipStreamSynchronize(stream()));  // h is released on return
 }
 
-// srcs/solp_fft_tdma3.f:148-149
+// srcs/solp_fft_tdma3.f:4-5
 void solp_fft_tdma3_(const int* ni, const int* nj, const int* nk, double* fftdata,
                      const double* phir_r, const double* phir_i, double* dp) {
   (void)fftdata;
@@ -496,7 +496,7 @@ void corunp_explicit_(const int* nID, const int* ni, const int* nj, const int* n
   c.finish();
 }
 
-// srcs/cal_coef_prs.f:75-127 (set-up, host arrays declared (-2:nj+3))
+// srcs/cal_coef_prs.f:5-57 (set-up, host arrays declared (-2:nj+3))
 void cal_coef_prs_(const int* nj_, const double* rho_, const double* dxinv_, const double* dyinv_,
                    const double* dzinv_, double* aw, double* ae, double* as, double* an, double* ab,
                    double* at, double* ap) {
@@ -518,7 +518,7 @@ void cal_coef_prs_(const int* nj_, const double* rho_, const double* dxinv_, con
   an[nj] = 0.0;
 }
 
-// srcs/cal_coef_solp_fft.f:132-169 (set-up, host arrays)
+// srcs/cal_coef_solp_fft.f:5-42 (set-up, host arrays)
 void cal_coef_solp_fft_(const int* nj_, const double* as_p, const double* an_p, double* as_fft,
                         double* an_fft, double* ap_fft) {
   const int nj = *nj_;

@@ -56,7 +56,7 @@ void calsij(const Dims d, double dxinv, double dyinv, double dzinv, const double
   });
 }
 
-// srcs/cal_arith_tau.f:164-260
+// srcs/cal_arith_tau.f:21-117
 void arith_tau(const Dims d, const double* s, const double* rmu, double* tau) {
   const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
   launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
@@ -211,7 +211,7 @@ void calsij_(const int* ni, const int* nj, const int* nk, const double* dxinv, c
   c.finish();
 }
 
-// srcs/cal_arith_tau.f:5
+// srcs/cal_arith_tau.f:7
 void cal_arith_tau_(const int* ni, const int* nj, const int* nk, const double* s, const double* rmu,
                     double* tau) {
   const Dims d = make_dims(*ni, *nj, *nk);

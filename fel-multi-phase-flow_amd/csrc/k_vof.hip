@@ -35,7 +35,7 @@ static DyTable dy_table(const double* host_arr, int count, int slot) {
   return t;
 }
 
-// Vertex gradient, srcs/cal_grad_p2a.f:39-80, box [-2,n+2]^3.
+// Vertex gradient, srcs/cal_grad_p2a.f:30-71, box [-2,n+2]^3.
 void grad_p2a(const Dims d, int ID, double dxinv, DyTable dyt, double dzinv, const double* q,
               double* qx, double* qy, double* qz) {
   launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
@@ -221,7 +221,7 @@ void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, doubl
   });
 }
 
-// srcs/solphi_mthinc3.f:84-160: truncation on [0,n+1]^3 (clamped in the
+// srcs/solphi_mthinc3.f:43-159: truncation on [0,n+1]^3 (clamped in the
 // interior in the same pass, the reference's second loop), the three global
 // sums, and the mass correction.  The offset stays on the device.
 void mthinc_mass(const Dims d, MthincConst mc, const double* phix, const double* phiy,
@@ -273,7 +273,7 @@ void mthinc_mass(const Dims d, MthincConst mc, const double* phix, const double*
   });
 }
 
-// srcs/summation.f:14-264 for nbub slices: slice 0 = sum of slices 1..nbub
+// srcs/summation.f:14-50 for nbub slices: slice 0 = sum of slices 1..nbub
 // over the padded image, clamped to [0,1] in the interior.
 void summation(const Dims d, double* qk, int nbub) {
   const int n3 = d.n3;
@@ -292,7 +292,7 @@ using namespace fel;
 
 extern "C" {
 
-// srcs/cal_grad_p2a.f:10-11
+// srcs/cal_grad_p2a.f:1-2
 void cal_grad_p2a_(const int* ID, const int* svall, const int* ni, const int* nj, const int* nk,
                    const double* dxinv, const double* dyinv_array, const double* dzinv,
                    const double* q, double* qx, double* qy, double* qz) {

@@ -1,0 +1,389 @@
+/* felgpu.h -- C ABI of the gfx950 (MI355X) timestep library libfelgpu.so.
+ *
+ * Every solver entry point keeps the argument list of the Fortran subroutine
+ * it replaces and its external name as the Fortran compilers used by the
+ * reference emit it (lower case plus one trailing underscore), so the
+ * reference's implicit-interface `call name(...)` statements link against
+ * this library without source changes.  All arguments are passed by
+ * reference, as Fortran passes them.
+ *
+ * Arrays.  A "padded field" is a REAL(8) array (-2:ni+3,-2:nj+3,-2:nk+3),
+ * i fastest, as allocated in srcs/allocate.h:7-165; "padded x3" is the same
+ * with a trailing dimension 1:3, "padded x6" with 1:6.  j is the wall-normal
+ * direction; i and k are periodic.
+ *
+ * Residency.  A host array registered with felgpu_register_ lives on the
+ * device: entry points resolve any pointer inside a registered range
+ * (interior and slice pointers included) to its device copy and move no
+ * data.  An unregistered array is copied to the device for the call and its
+ * written part copied back, so every entry point also works on plain host
+ * memory (slowly).  Host code that reads or writes a registered array
+ * between calls uses felgpu_download_ / felgpu_upload_.
+ *
+ * Errors.  A HIP failure, an argument the one-GPU library does not support
+ * (ipara != 0, a second MPI rank) or an inconsistent size prints the entry
+ * point's name and the reason to stderr and aborts the process; there is no
+ * host fallback.
+ *
+ * Streams.  All work is queued on one library stream (felgpu_stream_) in
+ * call order; entry points that return host scalars (cal_av_, caldiv_'s
+ * err_div, cal_center_, cal_vel_, felgpu_poisson_residual_) synchronise it.
+ */
+#ifndef FELGPU_H
+#define FELGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Library control (no counterpart in the reference: it runs on host memory).
+ * ---------------------------------------------------------------------- */
+
+/* Selects the HIP device and creates the library stream.  Optional: every
+ * entry point initialises on first use (device 0). */
+void felgpu_init_(const int *device);
+
+/* Makes [host, host + *nbytes) device-resident.  *upload != 0 copies the
+ * host contents; *upload == 0 zero-fills the device copy.  Call once per
+ * allocatable after `allocate`, before the first solver call. */
+void felgpu_register_(const void *host, const int64_t *nbytes, const int *upload);
+
+/* Host -> device copy of the whole registered range containing `host`. */
+void felgpu_upload_(const void *host);
+
+/* Device -> host copy of the whole registered range containing `host`. */
+void felgpu_download_(void *host);
+
+/* Releases the device copy of one registered range / of all of them. */
+void felgpu_unregister_(const void *host);
+void felgpu_unregister_all_(void);
+
+/* Device address backing a registered host address (NULL if unregistered). */
+void *felgpu_device_ptr_(const void *host);
+
+/* Waits for all queued work; returns the library's hipStream_t. */
+void felgpu_sync_(void);
+void *felgpu_stream_(void);
+
+/* Event timers around the hot kernels (ids: 1 MTHINC flux, 2 WENO5,
+ * 3 colour-pair Gauss-Seidel update, 4 Gauss-Seidel ghost refresh,
+ * 5 forward FFT, 6 modal Thomas solve, 7 inverse FFT). */
+void felgpu_prof_enable_(const int *on);
+void felgpu_prof_read_(const int *id, double *ms_total, int64_t *count);
+void felgpu_prof_reset_(void);
+
+/* ------------------------------------------------------------------------
+ * VOF transport (MTHINC).  Call sites: srcs/main.f90:491-509.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/cal_grad_p2a.f:1 -- vertex-centred gradient of q on [-2,n+2]^3.
+ * dyinv_array(-2:svall+3) is a host array; svall = global nj. */
+void cal_grad_p2a_(const int *ID, const int *svall, const int *ni, const int *nj, const int *nk,
+                   const double *dxinv, const double *dyinv_array, const double *dzinv,
+                   const double *q, double *qx, double *qy, double *qz);
+
+/* srcs/solphi_mthinc1.f:1 -- MTHINC face fluxes flphix/y/z (padded). */
+void solphi_mthinc1_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                     const double *dyinv, const double *dzinv, const double *dt,
+                     const double *bet_mthinc, const double *phi, const double *phix,
+                     const double *phiy, const double *phiz, const double *u, const double *v,
+                     const double *w, double *flphix, double *flphiy, double *flphiz);
+
+/* srcs/solphi_mthinc2.f:1 -- conservative update phin = phi - div(flux) + phi div(u) dt. */
+void solphi_mthinc2_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                     const double *dyinv, const double *dzinv, const double *dt, const double *u,
+                     const double *v, const double *w, const double *flphix,
+                     const double *flphiy, const double *flphiz, const double *phi, double *phin);
+
+/* srcs/solphi_mthinc3.f:1 -- truncation of flat cells, clamp to [0,1] and
+ * mass-preserving redistribution.  *ipara must be 0. */
+void solphi_mthinc3_(const int *ipara, const int *ni, const int *nj, const int *nk,
+                     const double *dxinv, const double *dyinv, const double *dzinv,
+                     const double *bet_mthinc, const double *phix, const double *phiy,
+                     const double *phiz, const double *phi, double *phin);
+
+/* srcs/summation.f:1 -- phi(:,:,:,0) = clamp(sum of slices 1..nbub);
+ * qk is the (padded, 0:nbub) array. */
+void summation_(const int *ni, const int *nj, const int *nk, double *qk, const int *nbub);
+
+/* ------------------------------------------------------------------------
+ * Momentum right-hand side.  Call sites: srcs/main.f90:520-579.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/cal_advu_weno5.f:1 -- WENO5 advection terms at u, v, w points. */
+void cal_advu_weno5_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                     const double *dyinv, const double *dzinv, const double *u, const double *v,
+                     const double *w, double *adv_u, double *adv_v, double *adv_w);
+
+/* srcs/cal_arith_mean.f:1 -- qk = phik*qg + (1-phik)*ql on the padded image. */
+void cal_arith_mean_(const int *ni, const int *nj, const int *nk, const double *phik,
+                     const double *ql, const double *qg, double *qk);
+
+/* srcs/calphat.f:1 -- pressure predictor phat = 2p - po. */
+void calphat_(const int *ni, const int *nj, const int *nk, const double *p, const double *po,
+              double *phat);
+
+/* srcs/cal_grad_prs.f:1 -- pressure gradient at u, v, w points. */
+void cal_grad_prs_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                   const double *dyinv, const double *dzinv, const double *q, double *qx_u,
+                   double *qy_v, double *qz_w);
+
+/* srcs/calsij.f:5 -- strain-rate tensor sk (padded x6). */
+void calsij_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+             const double *dyinv, const double *dzinv, const double *uk, const double *vk,
+             const double *wk, double *sk);
+
+/* srcs/cal_arith_tau.f:7 -- stress tau = 2 mu S with face-averaged mu (padded x6). */
+void cal_arith_tau_(const int *ni, const int *nj, const int *nk, const double *s,
+                    const double *rmu, double *tau);
+
+/* srcs/cal_div_tensor.f:1 -- divergence of the stress at u, v, w points. */
+void cal_div_tensor_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                     const double *dyinv, const double *dzinv, const double *tauk, double *uk,
+                     double *vk, double *wk);
+
+/* srcs/init_q.f:1 -- zero three padded fields. */
+void init_q_(const int *ni, const int *nj, const int *nk, double *q1, double *q2, double *q3);
+
+/* srcs/cal_grad_p2uvw.f:1 -- face gradients of q at u, v, w points. */
+void cal_grad_p2uvw_(const int *ID, const int *svall, const int *ni, const int *nj,
+                     const int *nk, const double *dxinv, const double *dyinv_array,
+                     const double *dzinv, const double *q, double *qx_u, double *qy_v,
+                     double *qz_w);
+
+/* srcs/calrn.f:1 -- unit normals averaged onto the (ip,jp,kp)-shifted
+ * staggered position (rn padded x3). */
+void calrn_(const int *ni, const int *nj, const int *nk, const double *dxinv, const double *dyinv,
+            const double *dzinv, const int *ip, const int *jp, const int *kp, const double *phix,
+            const double *phiy, const double *phiz, double *rn);
+
+/* srcs/calrkap.f:1 -- curvature at u, v, w points (rkap padded x3). */
+void calrkap_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+              const double *dyinv, const double *dzinv, const double *rn_pp, const double *rn_12,
+              const double *rn_13, const double *rn_23, double *rkap);
+
+/* srcs/calfst.f:7 -- CSF force -2 sigma/(rho_l+rho_g) kappa grad(phi). */
+void calfst_(const int *ni, const int *nj, const int *nk, const double *rhol, const double *rhog,
+             const double *surface_tension, const double *phix_u, const double *phiy_v,
+             const double *phiz_w, const double *rkap, double *fst_u, double *fst_v,
+             double *fst_w);
+
+/* srcs/sum_fst.f:1 -- sum_* += fst_* over the padded image. */
+void sum_fst_(const int *ni, const int *nj, const int *nk, const double *fst_u,
+              const double *fst_v, const double *fst_w, double *sum_u, double *sum_v,
+              double *sum_w);
+
+/* srcs/cal_av.f:1 -- interior mean of q into the host scalar *av.  *ipara must be 0. */
+void cal_av_(const int *ipara, const int *ndiv, const int *ni, const int *nj, const int *nk,
+             const double *q, double *av);
+
+/* srcs/cal_srcu.f:1 -- explicit right-hand side src_u/v/w of the momentum step. */
+void cal_srcu_(const int *nID, const int *ni, const int *nj, const int *nk, const double *rho_av,
+               const double *dt, const double *grvb, const double *grvp, const double *u,
+               const double *v, const double *w, const double *rho, const double *rhon,
+               const double *adv_u, const double *adv_v, const double *adv_w,
+               const double *adv_uo, const double *adv_vo, const double *adv_wo,
+               const double *prs_u, const double *prs_v, const double *prs_w,
+               const double *vis_u, const double *vis_v, const double *vis_w,
+               const double *fst_u, const double *fst_v, const double *fst_w,
+               const double *fst_un, const double *fst_vn, const double *fst_wn, double *src_u,
+               double *src_v, double *src_w);
+
+/* ------------------------------------------------------------------------
+ * Viscous Crank-Nicolson solve.  Call sites: srcs/main.f90:590, 609.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/cal_arith_coef_vis.f:6 -- the 45 stencil coefficient arrays, in the
+ * reference's argument order (au_w_uk ... av_tn_wk).  When rmuk and all 45
+ * outputs are registered the arrays are not written: the library records
+ * rmuk (and keeps a device snapshot if rmuk is later overwritten) and
+ * solu_sor4_ recomputes the coefficients on the fly; the arrays are
+ * materialised the first time any other entry point or felgpu_download_
+ * touches one of them, so they always read back exactly as the reference
+ * computes them. */
+void cal_arith_coef_vis_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                         const double *dyinv, const double *dzinv, const double *rmuk,
+                         double *au_w_uk, double *au_e_uk, double *au_s_uk, double *au_n_uk,
+                         double *au_b_uk, double *au_t_uk, double *au_p_uk, double *av_ws_uk,
+                         double *av_es_uk, double *av_wn_uk, double *av_en_uk, double *aw_wb_uk,
+                         double *aw_eb_uk, double *aw_wt_uk, double *aw_et_uk, double *av_w_vk,
+                         double *av_e_vk, double *av_s_vk, double *av_n_vk, double *av_b_vk,
+                         double *av_t_vk, double *av_p_vk, double *au_sw_vk, double *au_nw_vk,
+                         double *au_se_vk, double *au_ne_vk, double *aw_sb_vk, double *aw_nb_vk,
+                         double *aw_st_vk, double *aw_nt_vk, double *aw_w_wk, double *aw_e_wk,
+                         double *aw_s_wk, double *aw_n_wk, double *aw_b_wk, double *aw_t_wk,
+                         double *aw_p_wk, double *au_bw_wk, double *au_tw_wk, double *au_be_wk,
+                         double *au_te_wk, double *av_bs_wk, double *av_ts_wk, double *av_bn_wk,
+                         double *av_tn_wk);
+
+/* srcs/solu_sor4.f:1 -- 8-colour Gauss-Seidel for (un,vn,wn), u -> v -> w
+ * per sweep, ghost refresh (bndu + bnd_periodic) after each colour pair,
+ * stop at err/err0 <= 1e-3 or after 21 sweeps.  *ipara must be 0;
+ * sendjb/recvjb are unused on one GPU.  phi is the single phase slice the
+ * wall condition reads. */
+void solu_sor4_(const int *ipara, const int *ID, const int *nID, const int *ndiv, const int *ni,
+                const int *nj, const int *nk, const int *key, double *sendjb, double *recvjb,
+                const int *nstep, const int *imon_t, const double *dt, const double *rho,
+                const double *rhon, double *au_w_u, double *au_e_u, double *au_s_u,
+                double *au_n_u, double *au_b_u, double *au_t_u, double *au_p_u,
+                double *av_ws_u, double *av_es_u, double *av_wn_u, double *av_en_u,
+                double *aw_wb_u, double *aw_eb_u, double *aw_wt_u, double *aw_et_u,
+                double *av_w_v, double *av_e_v, double *av_s_v, double *av_n_v, double *av_b_v,
+                double *av_t_v, double *av_p_v, double *au_sw_v, double *au_nw_v,
+                double *au_se_v, double *au_ne_v, double *aw_sb_v, double *aw_nb_v,
+                double *aw_st_v, double *aw_nt_v, double *aw_w_w, double *aw_e_w, double *aw_s_w,
+                double *aw_n_w, double *aw_b_w, double *aw_t_w, double *aw_p_w,
+                double *au_bw_w, double *au_tw_w, double *au_be_w, double *au_te_w,
+                double *av_bs_w, double *av_ts_w, double *av_bn_w, double *av_tn_w,
+                const double *src_u, const double *src_v, const double *src_w, double *un,
+                double *vn, double *wn, const double *uwall, const double *dy, const double *l1,
+                const double *l2, const double *phi);
+
+/* The sweep index lp (0..20, 21 when the cap is reached) and err/err0 of the
+ * last solu_sor4_ call, the values srcs/solu_sor4.f:298-305 prints.
+ * No counterpart: the reference only prints them. */
+void felgpu_sor_last_(int *lp, double *ratio);
+
+/* ------------------------------------------------------------------------
+ * Boundary conditions.  Call sites: srcs/main.f90:324-422, 497-517, 625-661.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/bndu.f:6 -- Navier-slip / no-penetration ghosts of u, v, w at the
+ * j walls (nID(Y_MINUS/Y_PLUS) < 0) with phase-weighted slip length. */
+void bndu_(const int *nID, const int *ni, const int *nj, const int *nk, double *uk, double *vk,
+           double *wk, const double *uwall, const double *dy, const double *l1, const double *l2,
+           const double *phi);
+
+/* srcs/bnd_periodic.f:4 -- periodic ghosts in i and k. */
+void bnd_periodic_(const int *ni, const int *nj, const int *nk, double *qk);
+
+/* srcs/bnd_neumann.f:4 -- mirrored (zero-gradient) ghosts at the j walls. */
+void bnd_neumann_(const int *nID, const int *ni, const int *nj, const int *nk, double *qk);
+
+/* srcs/bnd_comm.f:1 -- j-halo exchange between MPI ranks: nothing to do on
+ * one GPU; *ipara != 0 is rejected. */
+void bnd_comm_(const int *ipara, const int *nID, const int *ni, const int *nj, const int *nk,
+               const int *key, double *sendjb, double *recvjb, double *qk);
+
+/* srcs/gnbc.f90:1 -- dynamic contact angle theta_array(i,k,side) from the
+ * contact-line slip velocity. */
+void gnbc_(const int *nID, const int *ni, const int *nj, const int *nk, const double *uk,
+           const double *wk, const double *uwall, const double *theta_0_array,
+           const double *surface_tension, const double *zeta_array, double *theta_array);
+
+/* srcs/bnd_contact_angle.f90:3 -- contact-angle ghosts of phi at the j walls. */
+void bnd_contact_angle_(const int *nID, const int *ni, const int *nj, const int *nk, double *qk,
+                        const double *theta_array, const double *dx, const double *dy,
+                        const double *dz);
+
+/* Cross-check entry points (no counterpart): bndu with its three ordered
+ * passes as separate launches, and the single composed launch of bndu plus
+ * bnd_periodic of component *swept (0 u, 1 v, 2 w; -1 none) that the
+ * Gauss-Seidel ghost refresh uses.  Same results as bndu_ followed by
+ * bnd_periodic_. */
+void felgpu_bndu_phased_(const int *nID, const int *ni, const int *nj, const int *nk, double *uk,
+                         double *vk, double *wk, const double *uwall, const double *dy,
+                         const double *l1, const double *l2, const double *phi,
+                         const int *swept);
+void felgpu_bc_fused_(const int *nID, const int *ni, const int *nj, const int *nk, double *uk,
+                      double *vk, double *wk, const double *uwall, const double *dy,
+                      const double *l1, const double *l2, const double *phi, const int *swept);
+
+/* ------------------------------------------------------------------------
+ * Pressure projection (FFT in i,k + Thomas in j).  Call sites:
+ * srcs/main.f90:467-468 (set-up), 633-661.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/caldiv.f:1 -- div = div(u)/dt and *err_div = its max norm.  *ipara must be 0. */
+void caldiv_(const int *ipara, const int *ndiv, const int *ni, const int *nj, const int *nk,
+             const double *dxinv, const double *dyinv, const double *dzinv, const double *dt,
+             const double *uk, const double *vk, const double *wk, double *div, double *err_div);
+
+/* srcs/solp_fft_tdma1.f:4 -- forward 2-D DFT (i,k) of div per j plane into
+ * phir_r/phir_i (centred mode layout).  fftdata (work array) is unused. */
+void solp_fft_tdma1_(const int *ni, const int *nj, const int *nk, double *fftdata,
+                     const double *div, double *phir_r, double *phir_i);
+
+/* srcs/trans_r2w.f:1, srcs/trans_w2r.f:1 -- slab transposes between MPI
+ * ranks; on one GPU the transposed array has the same memory image, so these
+ * copy only when r and w are different arrays. */
+void trans_r2w_(const int *ipara, const int *ID, const int *ndiv, const int *svall,
+                const int *key, double *sendbuf, double *recvbuf, const double *r, double *w);
+void trans_w2r_(const int *ipara, const int *ID, const int *ndiv, const int *svall,
+                const int *key, double *sendbuf, double *recvbuf, const double *w, double *r);
+
+/* srcs/solp_fft_tdma2.f:4 -- one Thomas solve along j per (kx,kz) mode.
+ * as/an/ap_p_fft(-2:nj+3) are host arrays; atdma/btdma_r/btdma_i are unused. */
+void solp_fft_tdma2_(const int *ID, const int *ndiv, const int *svall, const double *rho,
+                     const double *dxinv, const double *dzinv, const double *as_p_fft,
+                     const double *an_p_fft, const double *ap_p_fft, double *atdma,
+                     double *btdma_r, double *btdma_i, double *phiw_r, double *phiw_i);
+
+/* srcs/solp_fft_tdma3.f:4 -- inverse 2-D DFT per j plane into dp (interior).
+ * fftdata is unused. */
+void solp_fft_tdma3_(const int *ni, const int *nj, const int *nk, double *fftdata,
+                     const double *phir_r, const double *phir_i, double *dp);
+
+/* srcs/solp_fft_tdma4.f:4 -- residual monitor: computes and prints the
+ * relative residual every imon_t steps (on ID 0). */
+void solp_fft_tdma4_(const int *ipara, const int *ID, const int *ndiv, const int *ni,
+                     const int *nj, const int *nk, const int *nstep, const int *imon_t,
+                     const double *rho, const double *dxinv, const double *dyinv,
+                     const double *dzinv, const double *div, const double *dp);
+
+/* The monitor's relative residual ||L dp/rho - div|| / ||div|| as a return
+ * value (no counterpart; srcs/solp_fft_tdma4.f:28-72 prints it). */
+double felgpu_poisson_residual_(const int *ni, const int *nj, const int *nk, const double *rho,
+                                const double *dxinv, const double *dyinv, const double *dzinv,
+                                const double *div, const double *dp);
+
+/* srcs/bnd_avzero.f:1 -- subtract the interior mean from q (padded image).
+ * *ipara must be 0. */
+void bnd_avzero_(const int *ipara, const int *ndiv, const int *ni, const int *nj, const int *nk,
+                 double *q);
+
+/* srcs/corunp_explicit.f:1 -- projection un -= dt/rho grad(dp), pn = phat + dp. */
+void corunp_explicit_(const int *nID, const int *ni, const int *nj, const int *nk,
+                      const double *rho, const double *dxinv, const double *dyinv,
+                      const double *dzinv, const double *dt, const double *dp,
+                      const double *phat, double *un, double *vn, double *wn, double *pn);
+
+/* srcs/cal_coef_prs.f:5 -- set-up: 1-D pressure coefficients (-2:nj+3), host arrays. */
+void cal_coef_prs_(const int *nj, const double *rho, const double *dxinv, const double *dyinv,
+                   const double *dzinv, double *aw, double *ae, double *as, double *an,
+                   double *ab, double *at, double *ap);
+
+/* srcs/cal_coef_solp_fft.f:5 -- set-up: Thomas coefficients with the wall
+ * rows folded in (-2:nj+3), host arrays. */
+void cal_coef_solp_fft_(const int *nj, const double *as_p, const double *an_p, double *as_fft,
+                        double *an_fft, double *ap_fft);
+
+/* srcs/cpy.f:1 -- q2 = q1 over the padded image. */
+void cpy_(const int *ni, const int *nj, const int *nk, const double *q1, double *q2);
+
+/* ------------------------------------------------------------------------
+ * Monitors.  Call sites: srcs/main.f90:664, 666.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/cal_center.f:1 -- centroid of the liquid with periodic lap tracking
+ * and its velocity (host scalars in/out).  *ipara must be 0. */
+void cal_center_(const int *ipara, const int *ID, const int *svall, const int *ni, const int *nj,
+                 const int *nk, const double *xl, const double *yl, const double *zl,
+                 const double *dt, const double *rhog, const double *phi, double *center,
+                 double *center_pre1, double *center_pre2, int *area_pre1, int *area_pre2,
+                 int *lap, double *velocity);
+
+/* srcs/cal_vel.f:1 -- liquid-weighted mean velocity vel(3) and Reynolds
+ * numbers re(3).  *ipara must be 0. */
+void cal_vel_(const int *ipara, const int *ni, const int *nj, const int *nk, const double *radius,
+              const double *rhol, const double *rmul, const double *u, const double *v,
+              const double *w, const double *phi, double *vel, double *re);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FELGPU_H */

@@ -29,7 +29,7 @@
 /* VOF: vertex gradients, MTHINC fluxes, update, mass fix                     */
 /* ------------------------------------------------------------------------- */
 
-/* srcs/cal_grad_p2a.f:10-85.  Vertex-centred gradient on [-2,n+2]^3.
+/* srcs/cal_grad_p2a.f:30-71.  Vertex-centred gradient on [-2,n+2]^3.
  * pinned: _ref, plus the affine-field closed form. */
 void orc_cal_grad_p2a(const int *ID, const int *svall, const int *ni_, const int *nj_, const int *nk_,
                       const double *dxinv_, const double *dyinv_array, const double *dzinv_,
@@ -57,7 +57,7 @@ void orc_cal_grad_p2a(const int *ID, const int *svall, const int *ni_, const int
 }
 
 /* Cell normal = mean of the 8 surrounding vertex gradients
- * (srcs/solphi_mthinc1.f:81-107, same sums in srcs/solphi_mthinc3.f:96-122). */
+ * (srcs/solphi_mthinc1.f:81-107, same sums in srcs/solphi_mthinc3.f:47-73). */
 static inline void cell_normal(ptrdiff_t sx, ptrdiff_t sy, const double *phix, const double *phiy,
                                const double *phiz, int i, int j, int k, double *nx, double *ny,
                                double *nz) {
@@ -899,7 +899,7 @@ void orc_bnd_periodic(const int *ni_, const int *nj_, const int *nk_, double *qk
     }
 }
 
-/* srcs/bnd_periodic.f:51-93 (bnd_neumann): mirror ghosts at the j walls.
+/* srcs/bnd_neumann.f:4-46: mirror ghosts at the j walls.
  * pinned: _ref. */
 void orc_bnd_neumann(const int *nID, const int *ni_, const int *nj_, const int *nk_, double *qk) {
   const int ni = *ni_, nj = *nj_, nk = *nk_;
@@ -1268,7 +1268,7 @@ void orc_trans_w2r(const int *ipara, const int *ID, const int *ndiv, const int *
   if (w != r) memcpy(r, w, sizeof(double) * n);
 }
 
-/* srcs/solp_fft_tdma2.f:1-141: one Thomas solve along j per (kx,kz) mode
+/* srcs/solp_fft_tdma2.f:4-91: one Thomas solve along j per (kx,kz) mode
  * with the modal diagonal shift; the singular (0,0) mode is pinned by
  * apinv = -d/(d^2+1e-99).  pinned: _ref (bitwise) and dense solves. */
 void orc_solp_fft_tdma2(const int *ID, const int *ndiv, const int *svall, const double *rho_,
