@@ -1027,10 +1027,11 @@ double orc_sor_last_ratio = 0.0;
  * each over 8 parity colours in the order (111,222,112,221,121,212,211,122),
  * omega = 1, BC refresh (bndu + bnd_periodic of the swept component) after
  * every second colour; stop when err/err0 <= 1e-3 or after 21 sweeps.
- * Unbuildable reference (MPI header).  pinned: converged result against a
- * direct sparse solve of the same operator, and an independent numpy
- * Gauss-Seidel that calls the compiled reference bndu/bnd_periodic between
- * colours (tests/test_oracle_math.py). */
+ * Unbuildable reference (MPI header).  pinned: bit-identical to an
+ * independent numpy colour sweep that calls the compiled reference
+ * bndu/bnd_periodic between colour pairs, and its converged state satisfies
+ * the Crank-Nicolson equation evaluated through the explicit stress chain
+ * (tests/test_oracle_math.py). */
 static void sor4_impl(const int *ipara, const int *ID, const int *nID, const int *ndiv,
                       const int *ni_, const int *nj_, const int *nk_, const int *key,
                       double *sendjb, double *recvjb, const int *nstep, const int *imon_t,

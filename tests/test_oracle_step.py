@@ -1,0 +1,71 @@
+"""Invariants of the oracle's whole timestep (oracle/fel_oracle_step.c, the
+restatement of srcs/main.f90:473-748).  The reference's main program needs
+MPI and cannot run here, so the step order is pinned by what the physics and
+the reference's own guarantees fix:
+
+* volume: the MTHINC mass correction keeps sum(phi) to 1e-12 per step;
+* projection: after the pressure correction the discrete divergence of the
+  velocity is at rounding level;
+* steady state: single-phase Couette flow with Navier slip (linear profile
+  u = a + b y, b = 2 uwall / (Ly + 2 ls)) is an exact discrete solution of
+  every stage of the step and stays unchanged.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import felharness as H
+from felharness import D, I
+from felmpf import config as C
+from oracle_run import OracleRun
+
+
+def _interior(cfg, a):
+    return a[3:cfg.nk + 3, 3:cfg.nj + 3, 3:cfg.ni + 3]
+
+
+def test_step_conserves_volume_and_bounds():
+    cfg = C.droplet_impact().scaled(32, 8, 16)
+    o = OracleRun(cfg)
+    v0 = _interior(cfg, o.view("phi1")).sum()
+    for _ in range(6):
+        o.steps(1)
+        phi = _interior(cfg, o.view("phi1"))
+        assert abs(phi.sum() - v0) <= 1e-12 * v0
+        assert phi.min() >= 0.0 and phi.max() <= 1.0
+    o.close()
+
+
+def test_step_leaves_divergence_free_velocity():
+    cfg = C.sessile_droplet().scaled(16, 16, 16)
+    o = OracleRun(cfg)
+    o.steps(3)
+    u, v, w = o.field("u"), o.field("v"), o.field("w")
+    div, err = np.zeros(cfg.shape), D(0)
+    H.call_oracle("caldiv", I(0), I(1), I(cfg.ni), I(cfg.nj), I(cfg.nk), D(1 / cfg.dx),
+                  D(1 / cfg.dy), D(1 / cfg.dz), D(cfg.dt), u, v, w, div, err)
+    scale = max(np.abs(u).max(), np.abs(v).max(), np.abs(w).max()) / cfg.dx / cfg.dt
+    assert scale > 0
+    assert np.abs(_interior(cfg, div)).max() <= 1e-11 * scale
+    o.close()
+
+
+def test_step_keeps_couette_profile():
+    cfg = C.couette_stripe().scaled(16, 16, 8)
+    st = C.initial_state(cfg)
+    ls = cfg.bottom.slip
+    b = 2 * cfg.uwall / (cfg.yl + 2 * ls)
+    a = -cfg.uwall + ls * b
+    y = (np.arange(-2, cfg.nj + 4) - 0.5) * cfg.dy
+    st["u"] = np.broadcast_to((a + b * y)[None, :, None], cfg.shape).copy()
+    st["v"][...] = 0.0
+    st["w"][...] = 0.0
+    st["phi1"][...] = 0.0  # one phase (the liquid) everywhere
+    o = OracleRun(cfg, st)
+    u0 = _interior(cfg, o.field("u"))
+    o.steps(5)
+    u = _interior(cfg, o.field("u"))
+    assert np.abs(u - u0).max() <= 1e-12 * np.abs(u0).max()
+    assert np.abs(_interior(cfg, o.field("v"))).max() <= 1e-12
+    assert np.abs(_interior(cfg, o.field("w"))).max() <= 1e-12
+    o.close()
