@@ -38,12 +38,34 @@ TIMERS = {1: "mthinc_flux", 2: "weno5", 3: "sor_pair", 4: "sor_bc", 5: "fft_forw
 
 
 def sor_pair_bytes(cfg) -> float:
-    """Algorithmic HBM bytes of one colour-pair launch: it streams the three
-    velocity components (stencil neighbours of the updated cells cover every
-    cache line), the viscosity (coefficients recomputed on the fly), rho and
-    rho^{n+1} and the source of the swept component, and writes the swept
-    component: 7 arrays read + 1 written, 8 B each, per interior cell."""
-    return 8.0 * 8.0 * cfg.cells
+    """Algorithmic HBM bytes of one colour-pair launch, averaged over the three
+    components: every element the launch must read or write, counted once, at
+    N = ni*nj*nk interior cells.  The pair updates N/4 cells of the swept
+    component; their stencils touch every cell of the swept component, of the
+    other two components and of the viscosity (coefficients are recomputed
+    from it), N reads each; the source is read and the swept component
+    written at the N/4 updated cells; rho and rho^{n+1} are read at each
+    updated cell and its partner in the component direction, which covers
+    N/2 cells for u and w and all N cells for v.  Hence 5.5 N (u, w) and
+    6.5 N (v) elements of 8 bytes."""
+    n = float(cfg.cells)
+    return 8.0 * n * (5.5 + 6.5 + 5.5) / 3.0
+
+
+def measured_traffic(cfg):
+    """HBM bytes per colour-pair launch from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, calibrated),
+    or None when no summary exists for the configuration run."""
+    path = os.path.join(REPO, "profiles",
+                        f"hbm_traffic_pmc_{cfg.name}_{cfg.ni}x{cfg.nk}x{cfg.nj}.md")
+    try:
+        for line in open(path):
+            if "traffic_bytes_per_launch:" in line:
+                return float(line.split("traffic_bytes_per_launch:")[1].split("`")[0]), \
+                    os.path.relpath(path, REPO)
+    except OSError:
+        pass
+    return None, None
 
 
 def cpu_baseline(cfg_full, seconds=15.0):
@@ -121,6 +143,7 @@ def main():
     b = sor_pair_bytes(cfg)
     achieved = b / avg_s / 1e9 if n_sor else 0.0
     value = cfg.cells * args.steps / elapsed / 1e6
+    traffic, traffic_src = measured_traffic(cfg)
     phi = run.field("phi1")[3:-3, 3:-3, 3:-3]
     run.close()
 
@@ -143,10 +166,11 @@ def main():
                    "cells": cfg.cells, "precision": "fp64",
                    "sor_iterations_last_step": st["sor_lp"] + 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
                      "kernel": "sor_pair_kernel (colour-pair Gauss-Seidel update)",
                      "bytes_per_launch": b, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "launches": n_sor},
+                     "launches": n_sor, "traffic_source": traffic_src},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
