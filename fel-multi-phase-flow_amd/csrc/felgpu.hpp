@@ -92,6 +92,20 @@ class Call {
   bool finished_ = false;
 };
 
+// Content versions of registered arrays.  Every write through an entry
+// point, a registration or an upload gives the array a fresh number; cpy_
+// hands the source's number to the destination.  Equal numbers therefore
+// mean equal contents, which lets an entry point recognise that it is asked
+// for a result it has already computed.  content_version returns 0 unless
+// `host` is exactly one whole registered array of `count` doubles.
+uint64_t content_version(const void* host, size_t count);
+void copy_content_version(const void* src, const void* dst, size_t count);
+// Replaces the device buffer behind a whole registered array (same size)
+// and returns the previous one; the caller owns the returned buffer.
+double* exchange_device_buffer(const void* host, size_t count, double* buf);
+// Frees the advection result stash (called when all arrays are unregistered).
+void weno_release();
+
 // A deferred producer: the 45 viscous coefficient arrays of
 // cal_arith_coef_vis_ are a pure function of the viscosity.  When all of
 // them are device-resident the entry point only records where they come
@@ -179,6 +193,35 @@ inline void launch_flat(int n, F f) {
   if (blocks > 8192) blocks = 8192;
   flat_kernel<F><<<dim3(blocks), dim3(256), 0, stream()>>>(n, f);
   FEL_CHECK(hipGetLastError());
+}
+
+// Kernel over the ghost shell of the padded image (every cell outside the
+// interior [1,ni]x[1,nj]x[1,nk]): the 6 k-ghost planes, the 6 j-ghost rows
+// of each interior plane, and the 6 i-ghost cells of each interior row.
+template <class F>
+inline void launch_shell(const Dims& d, F f) {
+  const int planes = 6 * d.sx * d.sy, rows = d.nk * 6 * d.sx, cols = d.nk * d.nj * 6;
+  launch_flat(planes + rows + cols, [=] __device__(int m) {
+    int i, j, k;
+    if (m < planes) {
+      const int p = m / (d.sx * d.sy), r = m - p * (d.sx * d.sy);
+      k = p < 3 ? p - 2 : d.nk + p - 2;
+      j = r / d.sx - 2;
+      i = r % d.sx - 2;
+    } else if (m < planes + rows) {
+      const int q = m - planes, kk = q / (6 * d.sx), r = q - kk * (6 * d.sx), jr = r / d.sx;
+      k = kk + 1;
+      j = jr < 3 ? jr - 2 : d.nj + jr - 2;
+      i = r % d.sx - 2;
+    } else {
+      const int q = m - planes - rows, kk = q / (6 * d.nj), r = q - kk * (6 * d.nj), jj = r / 6,
+                ir = r % 6;
+      k = kk + 1;
+      j = jj + 1;
+      i = ir < 3 ? ir - 2 : d.ni + ir - 2;
+    }
+    f(i, j, k);
+  });
 }
 
 // Deterministic FP64 reductions over the interior cells [1,n]^3.  Stage 1:

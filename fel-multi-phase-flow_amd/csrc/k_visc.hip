@@ -322,6 +322,7 @@ void cpy_(const int* ni, const int* nj, const int* nk, const double* q1, double*
   double* b = c.out(q2, d.n3);
   copy(d, a, b);
   c.finish();
+  copy_content_version(q1, q2, d.n3);
 }
 
 // srcs/init_q.f:1

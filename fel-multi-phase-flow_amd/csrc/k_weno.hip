@@ -1,10 +1,12 @@
 // k_weno.hip -- momentum advection, WENO5 in derivative form.
 //
-// Reference: srcs/cal_advu_weno5.f:1-257 with srcs/weno5.h:1-71.  One
-// thread per padded cell: interior cells get the three upwind-weighted
-// derivatives of each velocity component, every other cell is zeroed (the
-// reference's clearing pass, cal_advu_weno5.f:44-52), so the outputs are
-// written exactly once.
+// Reference: srcs/cal_advu_weno5.f:1-257 with srcs/weno5.h:1-71.  Interior
+// cells get the three upwind-weighted derivatives of each velocity
+// component; the ghost shell is zeroed (the reference's clearing pass,
+// cal_advu_weno5.f:44-52), so the outputs are written exactly once.
+#include <cstdlib>
+#include <vector>
+
 #include "felgpu.hpp"
 
 namespace fel {
@@ -69,51 +71,174 @@ __device__ __forceinline__ double weno_dir(const double* __restrict__ f, int c, 
   return weno5(-fm3 + fm2, -fm2 + fm1, -fm1 + f0, -f0 + fp1, -fp1 + fp2, -fp2 + fp3, vel00);
 }
 
-// All three components in one pass over the padded image.
+// The WENO5 derivative from a register queue f(k-3..k+3) of the cell's own
+// column (the z direction), same differences as weno_dir.
+__device__ __forceinline__ double weno_q(const double* q, double vel00) {
+  return weno5(-q[0] + q[1], -q[1] + q[2], -q[2] + q[3], -q[3] + q[4], -q[4] + q[5],
+               -q[5] + q[6], vel00);
+}
+
+// Interior cells, k-marching: one thread per (i,j) column over kWenoK planes.
+// The own column's u, v, w at k-3..k+3 live in register queues, so the
+// z-direction stencils load one new plane per step instead of seven; the
+// in-plane (x, y) stencils and the neighbour-column transport velocities are
+// loaded from the same plane, which the block's other lanes share in cache.
+// Every value entering the arithmetic is the same double as in the
+// one-thread-per-cell form, so the results are identical.
+constexpr int kWenoK = 16;
+
+__global__ void __launch_bounds__(256) weno_march_kernel(Dims d, double dxinv, double dyinv,
+                                                         double dzinv, const double* __restrict__ u,
+                                                         const double* __restrict__ v,
+                                                         const double* __restrict__ w,
+                                                         double* __restrict__ au,
+                                                         double* __restrict__ av,
+                                                         double* __restrict__ aw, int gx, int gy) {
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
+  const int i = 1 + bx * 64 + (int)threadIdx.x, j = 1 + by * 4 + (int)threadIdx.y;
+  if (i > d.ni || j > d.nj) return;
+  const int k0 = 1 + bz * kWenoK, k1 = min(d.nk, k0 + kWenoK - 1);
+  const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+  int c = d.ix(i, j, k0);
+  double qu[7], qv[7], qw[7];
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    qu[m + 1] = u[c + (m - 3) * Z];
+    qv[m + 1] = v[c + (m - 3) * Z];
+    qw[m + 1] = w[c + (m - 3) * Z];
+  }
+  for (int k = k0; k <= k1; k++, c += Z) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      qu[m] = qu[m + 1];
+      qv[m] = qv[m + 1];
+      qw[m] = qw[m + 1];
+    }
+    qu[6] = u[c + 3 * Z];
+    qv[6] = v[c + 3 * Z];
+    qw[6] = w[c + 3 * Z];
+    double a, cm, cp;
+    // u (cal_advu_weno5.f:80-120)
+    cm = (u[c - X] + qu[3]) * 0.5;
+    cp = (qu[3] + u[c + X]) * 0.5;
+    a = weno_dir(u, c, X, (cm + cp) * 0.5) * dxinv;
+    cm = (v[c - Y] + v[c + X - Y]) * 0.5;
+    cp = (qv[3] + v[c + X]) * 0.5;
+    a = a + weno_dir(u, c, Y, (cm + cp) * 0.5) * dyinv;
+    cm = (qw[2] + w[c + X - Z]) * 0.5;
+    cp = (qw[3] + w[c + X]) * 0.5;
+    au[c] = a + weno_q(qu, (cm + cp) * 0.5) * dzinv;
+    // v (cal_advu_weno5.f:147-187)
+    cm = (u[c - X] + u[c - X + Y]) * 0.5;
+    cp = (qu[3] + u[c + Y]) * 0.5;
+    a = weno_dir(v, c, X, (cm + cp) * 0.5) * dxinv;
+    cm = (v[c - Y] + qv[3]) * 0.5;
+    cp = (qv[3] + v[c + Y]) * 0.5;
+    a = a + weno_dir(v, c, Y, (cm + cp) * 0.5) * dyinv;
+    cm = (qw[2] + w[c + Y - Z]) * 0.5;
+    cp = (qw[3] + w[c + Y]) * 0.5;
+    av[c] = a + weno_q(qv, (cm + cp) * 0.5) * dzinv;
+    // w (cal_advu_weno5.f:214-254)
+    cm = (u[c - X] + u[c - X + Z]) * 0.5;
+    cp = (qu[3] + qu[4]) * 0.5;
+    a = weno_dir(w, c, X, (cm + cp) * 0.5) * dxinv;
+    cm = (v[c - Y] + v[c - Y + Z]) * 0.5;
+    cp = (qv[3] + qv[4]) * 0.5;
+    a = a + weno_dir(w, c, Y, (cm + cp) * 0.5) * dyinv;
+    cm = (qw[2] + qw[3]) * 0.5;
+    cp = (qw[3] + qw[4]) * 0.5;
+    aw[c] = a + weno_q(qw, (cm + cp) * 0.5) * dzinv;
+  }
+}
+
+// The reference clears the outputs over the padded image and fills the
+// interior (cal_advu_weno5.f:44-52): interior by the marching kernel, the
+// ghost shell zeroed separately, so every output element is written once.
 void advect_weno5(const Dims d, double dxinv, double dyinv, double dzinv, const double* __restrict__ u,
                   const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ au,
                   double* __restrict__ av, double* __restrict__ aw) {
-  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+  const int gx = (d.ni + 63) / 64, gy = (d.nj + 3) / 4, gz = (d.nk + kWenoK - 1) / kWenoK;
+  weno_march_kernel<<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(d, dxinv, dyinv, dzinv, u, v,
+                                                                         w, au, av, aw, gx, gy);
+  FEL_CHECK(hipGetLastError());
+  launch_shell(d, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    if (i < 1 || i > d.ni || j < 1 || j > d.nj || k < 1 || k > d.nk) {
-      au[c] = 0.0;
-      av[c] = 0.0;
-      aw[c] = 0.0;
-      return;
-    }
-    const int X = 1, Y = d.sx, Z = d.sx * d.sy;
-    double a, cm, cp;
-    // u (cal_advu_weno5.f:80-120)
-    cm = (u[c - X] + u[c]) * 0.5;
-    cp = (u[c] + u[c + X]) * 0.5;
-    a = weno_dir(u, c, X, (cm + cp) * 0.5) * dxinv;
-    cm = (v[c - Y] + v[c + X - Y]) * 0.5;
-    cp = (v[c] + v[c + X]) * 0.5;
-    a = a + weno_dir(u, c, Y, (cm + cp) * 0.5) * dyinv;
-    cm = (w[c - Z] + w[c + X - Z]) * 0.5;
-    cp = (w[c] + w[c + X]) * 0.5;
-    au[c] = a + weno_dir(u, c, Z, (cm + cp) * 0.5) * dzinv;
-    // v (cal_advu_weno5.f:147-187)
-    cm = (u[c - X] + u[c - X + Y]) * 0.5;
-    cp = (u[c] + u[c + Y]) * 0.5;
-    a = weno_dir(v, c, X, (cm + cp) * 0.5) * dxinv;
-    cm = (v[c - Y] + v[c]) * 0.5;
-    cp = (v[c] + v[c + Y]) * 0.5;
-    a = a + weno_dir(v, c, Y, (cm + cp) * 0.5) * dyinv;
-    cm = (w[c - Z] + w[c + Y - Z]) * 0.5;
-    cp = (w[c] + w[c + Y]) * 0.5;
-    av[c] = a + weno_dir(v, c, Z, (cm + cp) * 0.5) * dzinv;
-    // w (cal_advu_weno5.f:214-254)
-    cm = (u[c - X] + u[c - X + Z]) * 0.5;
-    cp = (u[c] + u[c + Z]) * 0.5;
-    a = weno_dir(w, c, X, (cm + cp) * 0.5) * dxinv;
-    cm = (v[c - Y] + v[c - Y + Z]) * 0.5;
-    cp = (v[c] + v[c + Z]) * 0.5;
-    a = a + weno_dir(w, c, Y, (cm + cp) * 0.5) * dyinv;
-    cm = (w[c - Z] + w[c]) * 0.5;
-    cp = (w[c] + w[c + Z]) * 0.5;
-    aw[c] = a + weno_dir(w, c, Z, (cm + cp) * 0.5) * dzinv;
+    au[c] = 0.0;
+    av[c] = 0.0;
+    aw[c] = 0.0;
   });
+}
+
+// Result reuse.  The timestep asks for the advection of (u,v,w) and then of
+// (uo,vo,wo) (srcs/main.f90:532-533); uo..wo are copies of the previous
+// step's u..w (main.f90:686-688), whose advection the previous step already
+// computed into adv_u..adv_w.  Content versions identify that case exactly:
+// before adv_u..adv_w are overwritten, their buffers (holding the advection
+// of the inputs whose versions they were computed from) are parked in a
+// stash, and a later request whose inputs carry those same versions gets the
+// parked buffers swapped in instead of a recomputation -- the same bits, no
+// kernel.  Anything not provably identical is recomputed.
+namespace {
+
+struct WenoKey {
+  uint64_t ver[3];
+  int ni, nj, nk;
+  double dx, dy, dz;
+  bool operator==(const WenoKey& o) const {
+    return ver[0] == o.ver[0] && ver[1] == o.ver[1] && ver[2] == o.ver[2] && ni == o.ni &&
+           nj == o.nj && nk == o.nk && dx == o.dx && dy == o.dy && dz == o.dz;
+  }
+};
+
+struct WenoHeld {  // what one output triple holds, valid while its versions are unchanged
+  const void* out[3];
+  uint64_t out_ver[3];
+  WenoKey key;
+};
+
+struct WenoStash {
+  bool valid = false;
+  WenoKey key{};
+  double* buf[3] = {nullptr, nullptr, nullptr};
+  size_t n = 0;
+};
+
+std::vector<WenoHeld> g_held;
+WenoStash g_stash;
+
+bool reuse_disabled() {
+  static const int off = [] {
+    const char* e = std::getenv("FELGPU_NO_RESULT_REUSE");
+    return (e && std::atoi(e) != 0) ? 1 : 0;
+  }();
+  return off != 0;
+}
+
+WenoHeld* held_for(const void* const* out) {
+  for (auto& h : g_held)
+    if (h.out[0] == out[0] && h.out[1] == out[1] && h.out[2] == out[2]) return &h;
+  return nullptr;
+}
+
+void record_held(const void* const* out, size_t n, const WenoKey& k) {
+  WenoHeld* h = held_for(out);
+  if (!h) {
+    g_held.push_back(WenoHeld{});
+    h = &g_held.back();
+    for (int m = 0; m < 3; m++) h->out[m] = out[m];
+  }
+  for (int m = 0; m < 3; m++) h->out_ver[m] = content_version(out[m], n);
+  h->key = k;
+}
+
+}  // namespace
+
+void weno_release() {
+  for (int m = 0; m < 3; m++)
+    if (g_stash.buf[m]) FEL_CHECK(hipFree(g_stash.buf[m]));
+  g_stash = WenoStash{};
+  g_held.clear();
 }
 
 }  // namespace fel
@@ -127,6 +252,47 @@ void cal_advu_weno5_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* dyinv, const double* dzinv, const double* u, const double* v,
                      const double* w, double* adv_u, double* adv_v, double* adv_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
+  const size_t n = (size_t)d.n3;
+  const void* out[3] = {adv_u, adv_v, adv_w};
+  const void* in[3] = {u, v, w};
+  WenoKey key{{content_version(u, n), content_version(v, n), content_version(w, n)},
+              *ni, *nj, *nk, *dxinv, *dyinv, *dzinv};
+  bool track = !reuse_disabled();
+  for (int m = 0; m < 3 && track; m++) {
+    if (key.ver[m] == 0 || content_version(out[m], n) == 0) track = false;
+    for (int q = 0; q < 3; q++)
+      if (out[m] == in[q] || (q != m && out[m] == out[q])) track = false;
+  }
+  if (track) {
+    WenoHeld* h = held_for(out);
+    bool h_valid = h != nullptr;
+    for (int m = 0; m < 3 && h_valid; m++)
+      if (h->out_ver[m] != content_version(out[m], n)) h_valid = false;
+    if (g_stash.valid && g_stash.key == key) {
+      // the parked buffers hold exactly this result: swap them in
+      for (int m = 0; m < 3; m++) g_stash.buf[m] = exchange_device_buffer(out[m], n, g_stash.buf[m]);
+      g_stash.valid = h_valid;
+      if (h_valid) g_stash.key = h->key;
+      Call c("cal_advu_weno5_");  // the outputs change: fresh versions
+      for (int m = 0; m < 3; m++) c.out(out[m], n);
+      c.finish();
+      record_held(out, n, key);
+      return;
+    }
+    if (h_valid) {
+      // park what the outputs hold before overwriting them
+      if (g_stash.n != n) {
+        for (int m = 0; m < 3; m++) {
+          if (g_stash.buf[m]) FEL_CHECK(hipFree(g_stash.buf[m]));
+          FEL_CHECK(hipMalloc((void**)&g_stash.buf[m], sizeof(double) * n));
+        }
+        g_stash.n = n;
+      }
+      for (int m = 0; m < 3; m++) g_stash.buf[m] = exchange_device_buffer(out[m], n, g_stash.buf[m]);
+      g_stash.valid = true;
+      g_stash.key = h->key;
+    }
+  }
   Call c("cal_advu_weno5_");
   const double* du = c.in(u, d.n3, "u");
   const double* dv = c.in(v, d.n3, "v");
@@ -138,6 +304,7 @@ void cal_advu_weno5_(const int* ni, const int* nj, const int* nk, const double* 
   advect_weno5(d, *dxinv, *dyinv, *dzinv, du, dv, dw, au, av, aw);
   prof_end(2);
   c.finish();
+  if (track) record_held(out, n, key);
 }
 
 }  // extern "C"

@@ -24,7 +24,14 @@ struct Range {
   uintptr_t host;
   size_t bytes;
   double* dev;
+  // content version: a fresh number on every write (entry point output,
+  // registration, upload); a whole-array copy hands the source's number to
+  // the destination, so equal numbers mean equal contents
+  uint64_t ver;
 };
+
+uint64_t g_version = 0;
+uint64_t next_version() { return ++g_version; }
 
 struct Runtime {
   bool init = false;
@@ -55,13 +62,21 @@ Runtime& R() {
   return r;
 }
 
-const Range* find_range(const std::map<uintptr_t, Range>& m, uintptr_t p, size_t bytes) {
+Range* find_range(std::map<uintptr_t, Range>& m, uintptr_t p, size_t bytes) {
   auto it = m.upper_bound(p);
   if (it == m.begin()) return nullptr;
   --it;
-  const Range& r = it->second;
+  Range& r = it->second;
   if (p >= r.host && p + bytes <= r.host + r.bytes) return &r;
   return nullptr;
+}
+
+// The registered range that is exactly [host, host + count doubles).
+Range* whole_range(const void* host, size_t count) {
+  Runtime& r = R();
+  auto it = r.reg.find(reinterpret_cast<uintptr_t>(host));
+  if (it == r.reg.end() || it->second.bytes != count * sizeof(double)) return nullptr;
+  return &it->second;
 }
 
 }  // namespace
@@ -176,6 +191,31 @@ void deferred_coef_flush() {
   if (!dc.active) return;
   dc.active = false;
   coef_vis(dc.d, dc.dxinv, dc.dyinv, dc.dzinv, dc.src_dev, dc.out_dev);
+  for (int m = 0; m < 45; m++)
+    if (Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(dc.out_host[m]), 1))
+      g->ver = next_version();
+}
+
+uint64_t content_version(const void* host, size_t count) {
+  const Range* g = whole_range(host, count);
+  return g ? g->ver : 0;
+}
+
+void copy_content_version(const void* src, const void* dst, size_t count) {
+  const Range* a = whole_range(src, count);
+  Range* b = whole_range(dst, count);
+  if (a && b) b->ver = a->ver;
+}
+
+double* exchange_device_buffer(const void* host, size_t count, double* buf) {
+  Range* g = whole_range(host, count);
+  if (!g) {
+    std::fprintf(stderr, "felgpu: internal: buffer exchange on an unregistered range\n");
+    std::abort();
+  }
+  double* old = g->dev;
+  g->dev = buf;
+  return old;
 }
 
 void deferred_coef_check(const void* host, bool write) {
@@ -220,7 +260,8 @@ double* Call::arr(const void* host, size_t count, Access a, const char* what) {
     std::abort();
   }
   deferred_coef_check(host, (a & kWrite) != 0);
-  if (const Range* g = find_range(r.reg, p, bytes)) {
+  if (Range* g = find_range(r.reg, p, bytes)) {
+    if (a & kWrite) g->ver = next_version();
     return g->dev + (p - g->host) / sizeof(double);
   }
   // per-call mirror
@@ -235,7 +276,7 @@ double* Call::arr(const void* host, size_t count, Access a, const char* what) {
       r.transient.erase(it);
     }
     FEL_CHECK(hipMalloc((void**)&dev, bytes));
-    r.transient[p] = Range{p, bytes, dev};
+    r.transient[p] = Range{p, bytes, dev, 0};
   }
   FEL_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, r.s));
   if (a & kWrite) {
@@ -350,8 +391,9 @@ void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload
     }
     double* dev = nullptr;
     FEL_CHECK(hipMalloc((void**)&dev, bytes));
-    it = r.reg.emplace(p, Range{p, bytes, dev}).first;
+    it = r.reg.emplace(p, Range{p, bytes, dev, 0}).first;
   }
+  it->second.ver = next_version();
   if (*upload)
     FEL_CHECK(hipMemcpyAsync(it->second.dev, host, bytes, hipMemcpyHostToDevice, r.s));
   else
@@ -359,8 +401,8 @@ void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload
   FEL_CHECK(hipStreamSynchronize(r.s));
 }
 
-static const Range& must_find(const void* host, const char* who) {
-  const Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
+static Range& must_find(const void* host, const char* who) {
+  Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
   if (!g) {
     std::fprintf(stderr, "felgpu: %s: %p is not inside a registered range\n", who, host);
     std::abort();
@@ -371,7 +413,8 @@ static const Range& must_find(const void* host, const char* who) {
 // Host -> device copy of a whole registered range (after host-side writes).
 void felgpu_upload_(const void* host) {
   deferred_coef_check(host, true);
-  const Range& g = must_find(host, "felgpu_upload_");
+  Range& g = must_find(host, "felgpu_upload_");
+  g.ver = next_version();
   FEL_CHECK(hipMemcpyAsync(g.dev, (const void*)g.host, g.bytes, hipMemcpyHostToDevice, stream()));
   FEL_CHECK(hipStreamSynchronize(stream()));
 }
@@ -399,6 +442,7 @@ void felgpu_unregister_all_() {
   if (!r.init) return;
   deferred_coef_flush();
   FEL_CHECK(hipStreamSynchronize(r.s));
+  weno_release();
   for (auto& kv : r.reg) FEL_CHECK(hipFree(kv.second.dev));
   r.reg.clear();
   for (auto& kv : r.transient) FEL_CHECK(hipFree(kv.second.dev));
