@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Benchmark: Mcell-steps/s of the FP64 GPU timestep on the droplet-impact
+"""Benchmark: Mcell-steps/s of the GPU timestep on the droplet-impact
 configuration, 512x256x128 cells (reference orientation 512x128x256, wall-
 normal j), one MI355X.
 
@@ -8,7 +8,12 @@ entry points, driven by the Fortran host (host/fel_driver.f90), all fields
 device-resident before the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
-                  [--no-cpu-baseline]
+                  [--no-cpu-baseline] [--precision 64|32]
+
+By default both paths run from the same initial state: `value` is the FP64
+path (results equal to the reference's FP64 arithmetic); `fp32_refine`
+holds the FP32 mode's throughput, its speed-up and the relative L2 errors
+of u, v, w, p against the FP64 run after the same number of steps.
 
 The roofline object describes the dominant kernel (the colour-pair
 Gauss-Seidel update of the viscous solve): algorithmic bytes per launch
@@ -25,6 +30,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "fel-multi-phase-flow_amd"))
@@ -89,6 +96,48 @@ def cpu_baseline(cfg_full, seconds=15.0):
                       f"= {cfg.cells} cells, {n} steps after 2 warm-up steps, {dt:.1f} s, 1 thread"}
 
 
+def measure(cfg, precision, steps, warmup, keep_fields=False):
+    """Runs warmup + steps timesteps at the given precision; returns the
+    timing, the event timers and (optionally) the final fields."""
+    import felmpf
+    from felmpf import _lib
+    run = felmpf.GpuRun(cfg, precision=precision)
+    gpu = _lib.gpu()
+    try:
+        import torch  # device-wide synchronisation around the timed region
+        sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # noqa: BLE001
+        sync_dev = lambda: None  # noqa: E731
+    vol0 = run.field("phi1")[3:-3, 3:-3, 3:-3].sum()
+    run.steps(warmup)
+    run.sync()
+    sync_dev()
+    gpu.felgpu_prof_enable_(ctypes.byref(ctypes.c_int(1)))
+    gpu.felgpu_prof_reset_()
+    run.sync()
+    sync_dev()
+    t0 = time.perf_counter()
+    run.steps(steps)
+    run.sync()
+    sync_dev()
+    t1 = time.perf_counter()
+    gpu.felgpu_prof_enable_(ctypes.byref(ctypes.c_int(0)))
+    st = run.state()
+    timers = {}
+    for tid, name in TIMERS.items():
+        ms, cnt = ctypes.c_double(0), ctypes.c_int64(0)
+        gpu.felgpu_prof_read_(ctypes.byref(ctypes.c_int(tid)), ctypes.byref(ms), ctypes.byref(cnt))
+        timers[name] = (ms.value, cnt.value)
+    phi = run.field("phi1")[3:-3, 3:-3, 3:-3]
+    res = {"elapsed": t1 - t0, "state": st, "timers": timers,
+           "phi_min": float(phi.min()), "phi_max": float(phi.max()),
+           "volume_drift": float(abs(phi.sum() - vol0) / vol0)}
+    if keep_fields:
+        res["fields"] = {f: run.field(f)[3:-3, 3:-3, 3:-3] for f in ("u", "v", "w", "p")}
+    run.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +146,8 @@ def main():
     ap.add_argument("--config", default="droplet_impact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 32, 64),
+                    help="64 or 32: time that path only; default: both (value = FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -104,48 +155,38 @@ def main():
         sys.exit(2)
 
     import felmpf
-    from felmpf import _lib
 
     cfg = felmpf.CONFIGS[args.config]()
-    run = felmpf.GpuRun(cfg)
-    gpu = _lib.gpu()
-    try:
-        import torch  # device-wide synchronisation around the timed region
-        sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # noqa: BLE001
-        sync_dev = lambda: None  # noqa: E731
+    head_prec = 32 if args.precision == 32 else 64
+    both = args.precision == 0
+    main_run = measure(cfg, head_prec, args.steps, args.warmup, keep_fields=both)
+    fp32 = None
+    if both:
+        r32 = measure(cfg, 32, args.steps, args.warmup, keep_fields=True)
+        errs = {}
+        for f in ("u", "v", "w", "p"):
+            a, b = main_run["fields"][f], r32["fields"][f]
+            errs[f] = float(np.linalg.norm(a - b) / np.linalg.norm(a))
+        v32 = cfg.cells * args.steps / r32["elapsed"] / 1e6
+        v64 = cfg.cells * args.steps / main_run["elapsed"] / 1e6
+        fp32 = {"value": round(v32, 3), "unit": "Mcell-steps/s",
+                "ms_per_step": round(r32["elapsed"] / args.steps * 1e3, 3),
+                "speedup_vs_fp64": round(v32 / v64, 3),
+                "rel_l2_error_vs_fp64_after_steps": args.warmup + args.steps,
+                "rel_l2_error_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+                "volume_drift": float(f"{r32['volume_drift']:.3e}"),
+                "sor_iterations_last_step": r32["state"]["sor_lp"] + 1,
+                "dtype": "f32 flow fields and stencils, f64 volume fraction, mass sums, "
+                         "contact-line velocity and refined pressure correction"}
 
-    run.steps(args.warmup)
-    run.sync()
-    sync_dev()
-    on = ctypes.c_int(1)
-    gpu.felgpu_prof_enable_(ctypes.byref(on))
-    gpu.felgpu_prof_reset_()
-    run.sync()
-    sync_dev()
-    t0 = time.perf_counter()
-    run.steps(args.steps)
-    run.sync()
-    sync_dev()
-    t1 = time.perf_counter()
-    off = ctypes.c_int(0)
-    gpu.felgpu_prof_enable_(ctypes.byref(off))
-    elapsed = t1 - t0
-    st = run.state()
-
-    timers = {}
-    for tid, name in TIMERS.items():
-        ms, cnt = ctypes.c_double(0), ctypes.c_int64(0)
-        gpu.felgpu_prof_read_(ctypes.byref(ctypes.c_int(tid)), ctypes.byref(ms), ctypes.byref(cnt))
-        timers[name] = (ms.value, cnt.value)
+    timers = main_run["timers"]
     ms_sor, n_sor = timers["sor_pair"]
     avg_s = ms_sor / 1e3 / max(n_sor, 1)
-    b = sor_pair_bytes(cfg)
+    b = sor_pair_bytes(cfg) * (0.5 if head_prec == 32 else 1.0)
     achieved = b / avg_s / 1e9 if n_sor else 0.0
+    elapsed = main_run["elapsed"]
     value = cfg.cells * args.steps / elapsed / 1e6
-    traffic, traffic_src = measured_traffic(cfg)
-    phi = run.field("phi1")[3:-3, 3:-3, 3:-3]
-    run.close()
+    traffic, traffic_src = measured_traffic(cfg) if head_prec == 64 else (None, None)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(cfg, args.cpu_seconds)
     out = {
@@ -159,25 +200,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head_prec == 64 else "f32+f64-refine",
         "data": "synthetic",
         "config": {"workload": f"{cfg.name} {cfg.ni}x{cfg.nk}x{cfg.nj} (ref i,j,k = {cfg.ni}x{cfg.nj}x{cfg.nk}), "
-                               "FP64 drop-in timestep driven by the Fortran host",
-                   "cells": cfg.cells, "precision": "fp64",
-                   "sor_iterations_last_step": st["sor_lp"] + 1},
+                               "drop-in timestep driven by the Fortran host",
+                   "cells": cfg.cells, "precision": "fp64" if head_prec == 64 else "fp32+refine",
+                   "sor_iterations_last_step": main_run["state"]["sor_lp"] + 1,
+                   "volume_drift": float(f"{main_run['volume_drift']:.3e}")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": "sor_pair_kernel (colour-pair Gauss-Seidel update)",
                      "bytes_per_launch": b, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": n_sor, "traffic_source": traffic_src},
+        "fp32_refine": fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     detail = {k: {"ms_total": round(v[0], 3), "launches": v[1],
                   "ms_per_step": round(v[0] / args.steps, 3)} for k, v in timers.items()}
-    print(json.dumps({"kernel_timers": detail, "phi_min": float(phi.min()),
-                      "phi_max": float(phi.max())}), file=sys.stderr)
+    print(json.dumps({"kernel_timers": detail, "phi_min": main_run["phi_min"],
+                      "phi_max": main_run["phi_max"]}), file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -58,34 +58,66 @@ void ensure_init();
 // How an argument array is used by one drop-in call.
 enum Access { kRead = 1, kWrite = 2, kReadWrite = 3 };
 
+// Precision of the device copies.  In the default FP64 mode every device
+// copy is FP64, like the host arrays.  In the FP32 mode (FELGPU_PRECISION=fp32
+// or felgpu_set_precision_ before the first registration) arrays registered
+// with felgpu_register_ -- the flow fields and their work arrays -- are
+// stored and computed in FP32 on the device, while arrays registered with
+// felgpu_register_fp64_ -- the volume fraction, its fluxes, the contact-line
+// maps, the divergence and the pressure correction -- stay FP64.  Host
+// arrays are FP64 in both modes; uploads and downloads convert.
+bool flow_fp32();
+// Element size (4 or 8) of the device copy of the array containing `host`;
+// 8 for an unregistered array.
+int elem_size(const void* host);
+
 // Resolves the host array arguments of one drop-in call to device memory.
 // Arrays registered with felgpu_register_ are device-resident: the device
 // copy is the current one and no transfer happens.  Any other host pointer is
 // mirrored for the duration of the call: copied to the device before the
 // kernels (every access, so partial writes keep the untouched elements) and
-// back afterwards when written.  finish() completes those copies.
+// back afterwards when written, converted to the element type the kernel
+// reads.  finish() completes those copies.  A registered array whose device
+// precision differs from the element type asked for is an error.
 class Call {
  public:
   explicit Call(const char* name);
   ~Call();
-  double* arr(const void* host, size_t count, Access a, const char* what);
+  template <class T>
+  T* arr(const void* host, size_t count, Access a, const char* what) {
+    return static_cast<T*>(resolve(host, count, a, what, (int)sizeof(T)));
+  }
   double* in(const void* host, size_t count, const char* what = "") {
-    return arr(host, count, kRead, what);
+    return arr<double>(host, count, kRead, what);
   }
   double* out(const void* host, size_t count, const char* what = "") {
-    return arr(host, count, kWrite, what);
+    return arr<double>(host, count, kWrite, what);
   }
   double* io(const void* host, size_t count, const char* what = "") {
-    return arr(host, count, kReadWrite, what);
+    return arr<double>(host, count, kReadWrite, what);
+  }
+  template <class T>
+  T* in_t(const void* host, size_t count, const char* what = "") {
+    return arr<T>(host, count, kRead, what);
+  }
+  template <class T>
+  T* out_t(const void* host, size_t count, const char* what = "") {
+    return arr<T>(host, count, kWrite, what);
+  }
+  template <class T>
+  T* io_t(const void* host, size_t count, const char* what = "") {
+    return arr<T>(host, count, kReadWrite, what);
   }
   void finish();
 
  private:
+  void* resolve(const void* host, size_t count, Access a, const char* what, int esize);
   const char* name_;
   struct Pending {
     void* host;
-    double* dev;
-    size_t bytes;
+    void* dev;
+    size_t count;
+    int esize;
   };
   Pending pend_[64];
   int npend_ = 0;
@@ -100,9 +132,9 @@ class Call {
 // `host` is exactly one whole registered array of `count` doubles.
 uint64_t content_version(const void* host, size_t count);
 void copy_content_version(const void* src, const void* dst, size_t count);
-// Replaces the device buffer behind a whole registered array (same size)
-// and returns the previous one; the caller owns the returned buffer.
-double* exchange_device_buffer(const void* host, size_t count, double* buf);
+// Replaces the device buffer behind a whole registered array (same size and
+// element type) and returns the previous one; the caller owns it.
+void* exchange_device_buffer(const void* host, size_t count, void* buf);
 // Frees the advection result stash (called when all arrays are unregistered).
 void weno_release();
 
@@ -115,10 +147,11 @@ void weno_release();
 // a download), or a write to the viscosity, materialises them first.
 struct DeferredCoef {
   bool active = false;
+  int esize = 8;  // element size of the viscosity and the coefficients
   const void* out_host[45];
-  double* out_dev[45];
+  void* out_dev[45];
   const void* src_host;
-  const double* src_dev;
+  const void* src_dev;
   Dims d;
   double dxinv, dyinv, dzinv;
 };
@@ -127,8 +160,9 @@ bool deferred_coef_disabled();                    // FELGPU_EAGER_COEF=1 turns d
 bool is_registered(const void* host, size_t count);  // whole range device-resident?
 void deferred_coef_flush();                       // materialise now, if pending
 void deferred_coef_check(const void* host, bool write);  // flush if host conflicts
-void coef_vis(const Dims d, double dxinv, double dyinv, double dzinv, const double* rmu,
-              double* const* out45);
+// The 45 coefficient arrays from the viscosity, in the element type esize.
+void coef_vis_any(const Dims d, double dxinv, double dyinv, double dzinv, const void* rmu,
+                  void* const* out45, int esize);
 
 // Pinned host scratch for small device->host read-backs (reductions).
 double* pinned_scalars();  // 64 doubles
@@ -136,6 +170,14 @@ void sync();
 
 // Device scratch reused across calls (grows, never shrinks).
 void* scratch(int slot, size_t bytes);
+
+// A positive guard constant of the reference (1e-99, 1e-80) in element type
+// T: the value itself in FP64; in FP32, where it would flush to zero, 1e-30
+// (still far below every quantity it guards).
+template <class T>
+__host__ __device__ constexpr T guard(double v) {
+  return sizeof(T) == 8 ? T(v) : T(v < 1.0e-30 ? 1.0e-30 : v);
+}
 
 // Launch helpers --------------------------------------------------------------
 

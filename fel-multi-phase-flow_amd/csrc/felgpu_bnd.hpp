@@ -7,18 +7,18 @@
 
 namespace fel {
 
-// srcs/bndu.f in one launch (composed passes, see k_bnd.hip).
-void bndu(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
-          const double* l2, const double* phi, bool lo, bool hi);
-// srcs/bndu.f as the reference's three ordered passes (cross-check of bc_fused).
-void bndu_phased(const Dims d, double* u, double* v, double* w, double uwall, double dy,
-                 const double* l1, const double* l2, const double* phi, bool lo, bool hi);
-// bndu followed by bnd_periodic of component `swept` (0,1,2), one launch;
-// swept = -1 is bndu alone.
-void bc_fused(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
-              const double* l2, const double* phi, bool lo, bool hi, int swept);
+// bndu (srcs/bndu.f) followed by bnd_periodic of component `swept` (0,1,2),
+// one composed launch; swept = -1 is bndu alone.  T: velocity element type.
+// `comps` (bit q for component q) restricts the writes to those components;
+// a component left out must already hold exactly the values the call would
+// write (its interior unchanged since a refresh that wrote them).
+template <class T>
+void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
+              const double* l2, const double* phi, bool lo, bool hi, int swept, int comps);
 // srcs/bnd_periodic.f:4-46 in one in-place pass.
-void periodic(const Dims d, double* q);
-void neumann(const Dims d, double* q, bool lo, bool hi);
+template <class T>
+void periodic(const Dims d, T* q);
+template <class T>
+void neumann(const Dims d, T* q, bool lo, bool hi);
 
 }  // namespace fel

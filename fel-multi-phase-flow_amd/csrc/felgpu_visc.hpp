@@ -12,39 +12,47 @@ namespace fel {
 
 constexpr int kYMinus = 1, kYPlus = 4;  // nID(Y_MINUS), nID(Y_PLUS) as 0-based (srcs/param.h)
 
+// T is the element type of the flow fields: double (FP64 mode) or float
+// (FP32 mode); with T = double every expression is the FP64 one.
+template <class T>
 struct Coef45 {
-  double* p[45];
+  T* p[45];
 };
 
+template <class T>
 struct SrcArgs {
-  const double *u, *v, *w, *rho, *rhon;
-  const double *adv_u, *adv_v, *adv_w, *adv_uo, *adv_vo, *adv_wo;
-  const double *prs_u, *prs_v, *prs_w, *vis_u, *vis_v, *vis_w;
-  const double *fst_u, *fst_v, *fst_w, *fst_un, *fst_vn, *fst_wn;
-  double *src_u, *src_v, *src_w;
+  const T *u, *v, *w, *rho, *rhon;
+  const T *adv_u, *adv_v, *adv_w, *adv_uo, *adv_vo, *adv_wo;
+  const T *prs_u, *prs_v, *prs_w, *vis_u, *vis_v, *vis_w;
+  const T *fst_u, *fst_v, *fst_w, *fst_un, *fst_vn, *fst_wn;
+  T *src_u, *src_v, *src_w;
 };
 
+template <class T>
 struct VisScales {
-  double dx2, dy2, dz2, dxy, dxz, dyz;
+  T dx2, dy2, dz2, dxy, dxz, dyz;
 };
 
-inline VisScales vis_scales(double dxinv, double dyinv, double dzinv) {
-  return VisScales{dxinv * dxinv, dyinv * dyinv, dzinv * dzinv,
-                   dxinv * dyinv, dxinv * dzinv, dyinv * dzinv};
+// The products are formed in FP64 and rounded once to T.
+template <class T>
+inline VisScales<T> vis_scales(double dxinv, double dyinv, double dzinv) {
+  return VisScales<T>{T(dxinv * dxinv), T(dyinv * dyinv), T(dzinv * dzinv),
+                      T(dxinv * dyinv), T(dxinv * dzinv), T(dyinv * dzinv)};
 }
 
 // Coefficients of the u equation in the reference's argument order:
 // w,e,s,n,b,t,p, v_ws,v_es,v_wn,v_en, w_wb,w_eb,w_wt,w_et.
-__device__ __forceinline__ void coef_u(const Dims& d, const VisScales& s, const double* r, int c,
-                                       double* v) {
+template <class T>
+__device__ __forceinline__ void coef_u(const Dims& d, const VisScales<T>& s, const T* r, int c,
+                                       T* v) {
   const int X = 1, Y = d.sx, Z = d.sx * d.sy;
-  const double xm = r[c], xp = r[c + X];
-  const double ym = 0.25 * (+r[c - Y] + r[c + X - Y] + r[c] + r[c + X]);
-  const double yp = 0.25 * (+r[c] + r[c + X] + r[c + Y] + r[c + X + Y]);
-  const double zm = 0.25 * (+r[c - Z] + r[c + X - Z] + r[c] + r[c + X]);
-  const double zp = 0.25 * (+r[c] + r[c + X] + r[c + Z] + r[c + X + Z]);
-  const double b = s.dz2 * zm, ss = s.dy2 * ym, w = s.dx2 * xm * 2.0, e = s.dx2 * xp * 2.0;
-  const double n = s.dy2 * yp, t = s.dz2 * zp;
+  const T xm = r[c], xp = r[c + X];
+  const T ym = T(0.25) * (+r[c - Y] + r[c + X - Y] + r[c] + r[c + X]);
+  const T yp = T(0.25) * (+r[c] + r[c + X] + r[c + Y] + r[c + X + Y]);
+  const T zm = T(0.25) * (+r[c - Z] + r[c + X - Z] + r[c] + r[c + X]);
+  const T zp = T(0.25) * (+r[c] + r[c + X] + r[c + Z] + r[c + X + Z]);
+  const T b = s.dz2 * zm, ss = s.dy2 * ym, w = s.dx2 * xm * T(2.0), e = s.dx2 * xp * T(2.0);
+  const T n = s.dy2 * yp, t = s.dz2 * zp;
   v[0] = w; v[1] = e; v[2] = ss; v[3] = n; v[4] = b; v[5] = t;
   v[6] = +b + ss + w + e + n + t;
   v[7] = s.dxy * ym; v[8] = -(s.dxy * ym); v[9] = -(s.dxy * yp); v[10] = s.dxy * yp;
@@ -52,16 +60,17 @@ __device__ __forceinline__ void coef_u(const Dims& d, const VisScales& s, const 
 }
 
 // v equation: w,e,s,n,b,t,p, u_sw,u_nw,u_se,u_ne, w_sb,w_nb,w_st,w_nt.
-__device__ __forceinline__ void coef_v(const Dims& d, const VisScales& s, const double* r, int c,
-                                       double* v) {
+template <class T>
+__device__ __forceinline__ void coef_v(const Dims& d, const VisScales<T>& s, const T* r, int c,
+                                       T* v) {
   const int X = 1, Y = d.sx, Z = d.sx * d.sy;
-  const double xm = 0.25 * (+r[c - X] + r[c] + r[c - X + Y] + r[c + Y]);
-  const double xp = 0.25 * (+r[c] + r[c + X] + r[c + Y] + r[c + X + Y]);
-  const double ym = r[c], yp = r[c + Y];
-  const double zm = 0.25 * (+r[c - Z] + r[c + Y - Z] + r[c] + r[c + Y]);
-  const double zp = 0.25 * (+r[c] + r[c + Y] + r[c + Z] + r[c + Y + Z]);
-  const double b = s.dz2 * zm, ss = s.dy2 * ym * 2.0, w = s.dx2 * xm, e = s.dx2 * xp;
-  const double n = s.dy2 * yp * 2.0, t = s.dz2 * zp;
+  const T xm = T(0.25) * (+r[c - X] + r[c] + r[c - X + Y] + r[c + Y]);
+  const T xp = T(0.25) * (+r[c] + r[c + X] + r[c + Y] + r[c + X + Y]);
+  const T ym = r[c], yp = r[c + Y];
+  const T zm = T(0.25) * (+r[c - Z] + r[c + Y - Z] + r[c] + r[c + Y]);
+  const T zp = T(0.25) * (+r[c] + r[c + Y] + r[c + Z] + r[c + Y + Z]);
+  const T b = s.dz2 * zm, ss = s.dy2 * ym * T(2.0), w = s.dx2 * xm, e = s.dx2 * xp;
+  const T n = s.dy2 * yp * T(2.0), t = s.dz2 * zp;
   v[0] = w; v[1] = e; v[2] = ss; v[3] = n; v[4] = b; v[5] = t;
   v[6] = +b + ss + w + e + n + t;
   v[7] = s.dxy * xm; v[8] = -(s.dxy * xm); v[9] = -(s.dxy * xp); v[10] = s.dxy * xp;
@@ -69,16 +78,17 @@ __device__ __forceinline__ void coef_v(const Dims& d, const VisScales& s, const 
 }
 
 // w equation: w,e,s,n,b,t,p, u_bw,u_tw,u_be,u_te, v_bs,v_ts,v_bn,v_tn.
-__device__ __forceinline__ void coef_w(const Dims& d, const VisScales& s, const double* r, int c,
-                                       double* v) {
+template <class T>
+__device__ __forceinline__ void coef_w(const Dims& d, const VisScales<T>& s, const T* r, int c,
+                                       T* v) {
   const int X = 1, Y = d.sx, Z = d.sx * d.sy;
-  const double xm = 0.25 * (+r[c - X] + r[c] + r[c - X + Z] + r[c + Z]);
-  const double xp = 0.25 * (+r[c] + r[c + X] + r[c + Z] + r[c + X + Z]);
-  const double ym = 0.25 * (+r[c - Y] + r[c] + r[c - Y + Z] + r[c + Z]);
-  const double yp = 0.25 * (+r[c] + r[c + Y] + r[c + Z] + r[c + Y + Z]);
-  const double zm = r[c], zp = r[c + Z];
-  const double b = s.dz2 * zm * 2.0, ss = s.dy2 * ym, w = s.dx2 * xm, e = s.dx2 * xp;
-  const double n = s.dy2 * yp, t = s.dz2 * zp * 2.0;
+  const T xm = T(0.25) * (+r[c - X] + r[c] + r[c - X + Z] + r[c + Z]);
+  const T xp = T(0.25) * (+r[c] + r[c + X] + r[c + Z] + r[c + X + Z]);
+  const T ym = T(0.25) * (+r[c - Y] + r[c] + r[c - Y + Z] + r[c + Z]);
+  const T yp = T(0.25) * (+r[c] + r[c + Y] + r[c + Z] + r[c + Y + Z]);
+  const T zm = r[c], zp = r[c + Z];
+  const T b = s.dz2 * zm * T(2.0), ss = s.dy2 * ym, w = s.dx2 * xm, e = s.dx2 * xp;
+  const T n = s.dy2 * yp, t = s.dz2 * zp * T(2.0);
   v[0] = w; v[1] = e; v[2] = ss; v[3] = n; v[4] = b; v[5] = t;
   v[6] = +b + ss + w + e + n + t;
   v[7] = s.dxz * xm; v[8] = -(s.dxz * xm); v[9] = -(s.dxz * xp); v[10] = s.dxz * xp;

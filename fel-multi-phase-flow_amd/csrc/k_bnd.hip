@@ -12,8 +12,11 @@
 namespace fel {
 
 // srcs/bndu.f:29-96: wall rows, one thread per (i,k) of the padded plane.
-static void bndu_walls(const Dims d, double* uk, double* vk, double* wk, double uwall, double dy,
-                       const double* l1, const double* l2, const double* phi, bool lo, bool hi) {
+// T: velocity element type; the slip coefficients are FP64 and each ghost
+// value is formed in FP64 and rounded once to T.
+template <class T>
+static void bndu_walls(const Dims d, T* uk, T* vk, T* wk, double uwall, double dy, const double* l1,
+                       const double* l2, const double* phi, bool lo, bool hi) {
   const int ni = d.ni, nj = d.nj;
   launch_box(Box{-2, d.ni + 3, 0, 0, -2, d.nk + 3}, [=] __device__(int i, int, int k) {
     const int ip = (i + 1) % (ni + 3);  // Fortran mod(i+1,ni+3), sign of the dividend
@@ -26,21 +29,21 @@ static void bndu_walls(const Dims d, double* uk, double* vk, double* wk, double 
       const double coef1 = (2.0 * dy) / (2.0 * ls + dy);
       const double coef2 = (2.0 * ls - dy) / (2.0 * ls + dy);
       const int c0 = d.ix(i, 0, k), cm = d.ix(i, -1, k), cmm = d.ix(i, -2, k), c2 = d.ix(i, 2, k);
-      const double u1 = uk[c1];
-      const double u0 = coef1 * (-uwall) + coef2 * u1;
-      const double um = 2.0 * u0 - u1;
+      const T u1 = uk[c1];
+      const T u0 = T(coef1 * (-uwall) + coef2 * (double)u1);
+      const T um = T(2.0) * u0 - u1;
       uk[c0] = u0;
       uk[cm] = um;
-      uk[cmm] = 2.0 * um - u0;
+      uk[cmm] = T(2.0) * um - u0;
       vk[cmm] = -vk[c2];
       vk[cm] = -vk[c1];
-      vk[c0] = 0.0;
-      const double w1 = wk[c1];
-      const double w0 = coef2 * w1;
-      const double wm = 2.0 * w0 - w1;
+      vk[c0] = T(0.0);
+      const T w1 = wk[c1];
+      const T w0 = T(coef2 * (double)w1);
+      const T wm = T(2.0) * w0 - w1;
       wk[c0] = w0;
       wk[cm] = wm;
-      wk[cmm] = 2.0 * wm - w0;
+      wk[cmm] = T(2.0) * wm - w0;
     }
     if (hi) {
       const int cn = d.ix(i, nj, k);
@@ -51,29 +54,30 @@ static void bndu_walls(const Dims d, double* uk, double* vk, double* wk, double 
       const double coef1 = (2.0 * dy) / (2.0 * ls + dy);
       const double coef2 = (2.0 * ls - dy) / (2.0 * ls + dy);
       const int Y = d.sx;
-      const double un = uk[cn];
-      const double u1 = coef1 * uwall + coef2 * un;
-      const double u2 = 2.0 * u1 - un;
+      const T un = uk[cn];
+      const T u1 = T(coef1 * uwall + coef2 * (double)un);
+      const T u2 = T(2.0) * u1 - un;
       uk[cn + Y] = u1;
       uk[cn + 2 * Y] = u2;
-      uk[cn + 3 * Y] = 2.0 * u2 - u1;
-      vk[cn] = 0.0;
+      uk[cn + 3 * Y] = T(2.0) * u2 - u1;
+      vk[cn] = T(0.0);
       vk[cn + Y] = -vk[cn - Y];
       vk[cn + 2 * Y] = -vk[cn - 2 * Y];
       vk[cn + 3 * Y] = -vk[cn - 3 * Y];
-      const double wn = wk[cn];
-      const double w1 = coef2 * wn;
-      const double w2 = 2.0 * w1 - wn;
+      const T wn = wk[cn];
+      const T w1 = T(coef2 * (double)wn);
+      const T w2 = T(2.0) * w1 - wn;
       wk[cn + Y] = w1;
       wk[cn + 2 * Y] = w2;
-      wk[cn + 3 * Y] = 2.0 * w2 - w1;
+      wk[cn + 3 * Y] = T(2.0) * w2 - w1;
     }
   });
 }
 
 // srcs/bndu.f:105-126: x ghosts, including the reference's shifted source
 // for u (u(0)<-u(ni+1), u(-1)<-u(ni), u(-2)<-u(ni-1); u(ni+1) untouched).
-static void bndu_x(const Dims d, double* uk, double* vk, double* wk) {
+template <class T>
+static void bndu_x(const Dims d, T* uk, T* vk, T* wk) {
   const int ni = d.ni;
   launch_box(Box{0, 0, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int, int j, int k) {
     const int r = d.ix(0, j, k);  // offset of i=0 in this row
@@ -101,7 +105,8 @@ static void bndu_x(const Dims d, double* uk, double* vk, double* wk) {
 
 // srcs/bndu.f:136-157: z ghosts, with the shifted source for w
 // (w(0)<-w(nk+1), w(-1)<-w(nk), w(-2)<-w(nk-1); w(nk+1) untouched).
-static void bndu_z(const Dims d, double* uk, double* vk, double* wk) {
+template <class T>
+static void bndu_z(const Dims d, T* uk, T* vk, T* wk) {
   const int nk = d.nk;
   launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, 0, 0}, [=] __device__(int i, int j, int) {
     const int r = d.ix(i, j, 0), Z = d.sx * d.sy;
@@ -127,11 +132,12 @@ static void bndu_z(const Dims d, double* uk, double* vk, double* wk) {
   });
 }
 
-void bndu_phased(const Dims d, double* u, double* v, double* w, double uwall, double dy,
-                 const double* l1, const double* l2, const double* phi, bool lo, bool hi) {
-  bndu_walls(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi);
-  bndu_x(d, u, v, w);
-  bndu_z(d, u, v, w);
+template <class T>
+void bndu_phased(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
+                 const double* l2, const double* phi, bool lo, bool hi) {
+  bndu_walls<T>(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi);
+  bndu_x<T>(d, u, v, w);
+  bndu_z<T>(d, u, v, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,38 +177,39 @@ __device__ __forceinline__ WallCoef wall_coef(const Dims& d, int i, int jw, int 
 }
 
 // Wall phase value of component q at (i,j,k); q: 0=u, 1=v, 2=w.
-__device__ __forceinline__ double wall_value(const Dims& d, int q, const double* f, int i, int j, int k,
-                                             double uwall, double dy, const double* l1,
-                                             const double* l2, const double* phi, bool lo, bool hi) {
+template <class T>
+__device__ __forceinline__ T wall_value(const Dims& d, int q, const T* f, int i, int j, int k,
+                                        double uwall, double dy, const double* l1,
+                                        const double* l2, const double* phi, bool lo, bool hi) {
   const int nj = d.nj;
   if (q == 1) {
     if (lo && j <= 0) {
-      if (j == 0) return 0.0;
+      if (j == 0) return T(0.0);
       return -f[d.ix(i, j == -1 ? 1 : 2, k)];
     }
     if (hi && j >= nj) {
-      if (j == nj) return 0.0;
+      if (j == nj) return T(0.0);
       return -f[d.ix(i, 2 * nj - j, k)];
     }
     return f[d.ix(i, j, k)];
   }
   if (lo && j <= 0) {
     const WallCoef cf = wall_coef(d, i, 1, k, dy, l1, l2, phi);
-    const double f1 = f[d.ix(i, 1, k)];
-    const double g0 = (q == 0) ? cf.c1 * (-uwall) + cf.c2 * f1 : cf.c2 * f1;
+    const T f1 = f[d.ix(i, 1, k)];
+    const T g0 = T((q == 0) ? cf.c1 * (-uwall) + cf.c2 * (double)f1 : cf.c2 * (double)f1);
     if (j == 0) return g0;
-    const double gm = 2.0 * g0 - f1;
+    const T gm = T(2.0) * g0 - f1;
     if (j == -1) return gm;
-    return 2.0 * gm - g0;
+    return T(2.0) * gm - g0;
   }
   if (hi && j >= nj + 1) {
     const WallCoef cf = wall_coef(d, i, nj, k, dy, l1, l2, phi);
-    const double fn = f[d.ix(i, nj, k)];
-    const double g1 = (q == 0) ? cf.c1 * uwall + cf.c2 * fn : cf.c2 * fn;
+    const T fn = f[d.ix(i, nj, k)];
+    const T g1 = T((q == 0) ? cf.c1 * uwall + cf.c2 * (double)fn : cf.c2 * (double)fn);
     if (j == nj + 1) return g1;
-    const double g2 = 2.0 * g1 - fn;
+    const T g2 = T(2.0) * g1 - fn;
     if (j == nj + 2) return g2;
-    return 2.0 * g2 - g1;
+    return T(2.0) * g2 - g1;
   }
   return f[d.ix(i, j, k)];
 }
@@ -234,8 +241,9 @@ __device__ __forceinline__ int zsrc(int q, int k, int nk) {
 
 __device__ __forceinline__ int pimg(int i, int n) { return i < 1 ? i + n : (i > n ? i - n : i); }
 
-void bc_fused(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
-              const double* l2, const double* phi, bool lo, bool hi, int swept) {
+template <class T>
+void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
+              const double* l2, const double* phi, bool lo, bool hi, int swept, int comps) {
   const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, sy = d.sy;
   // disjoint regions covering every cell any component may write:
   //  A: the six z-ghost planes (all i, j)
@@ -264,13 +272,15 @@ void bc_fused(const Dims d, double* u, double* v, double* w, double uwall, doubl
       k = r / (nj - 1) + 1;
       i = g < 3 ? g - 2 : ni + g - 2;
     }
-    double* F[3] = {u, v, w};
-    double val[3];
+    T* F[3] = {u, v, w};
+    T val[3];
     bool wr[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       const bool gx = i < 1 || i > ni, gz = k < 1 || k > nk;
-      if (q == swept) {
+      if (!(comps & (1 << q))) {
+        wr[q] = false;
+      } else if (q == swept) {
         // bnd_periodic after bndu: every ghost/wall cell of the swept component
         wr[q] = gx || gz || wall_row(q, j, nj, lo, hi);
         if (wr[q]) val[q] = wall_value(d, q, F[q], pimg(i, ni), j, pimg(k, nk), uwall, dy, l1, l2, phi, lo, hi);
@@ -291,9 +301,10 @@ void bc_fused(const Dims d, double* u, double* v, double* w, double uwall, doubl
   });
 }
 
-void bndu(const Dims d, double* u, double* v, double* w, double uwall, double dy, const double* l1,
+template <class T>
+void bndu(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
           const double* l2, const double* phi, bool lo, bool hi) {
-  bc_fused(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi, -1);
+  bc_fused<T>(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi, -1, 7);
 }
 
 // srcs/bnd_periodic.f:16-43, x then z, in one pass: a ghost cell takes the
@@ -305,7 +316,8 @@ __device__ __forceinline__ int pmap(int i, int n) {
   return i < 1 ? i + n : (i > n ? i - n : i);
 }
 
-void periodic(const Dims d, double* q) {
+template <class T>
+void periodic(const Dims d, T* q) {
   // ghost cells only: the x-ghost columns of the interior planes, then the
   // six z-ghost planes (all i, all j); the two sets are disjoint
   const int ni = d.ni, nk = d.nk, sx = d.sx, sy = d.sy;
@@ -330,7 +342,8 @@ void periodic(const Dims d, double* q) {
 }
 
 // srcs/bnd_neumann.f:19-41
-void neumann(const Dims d, double* q, bool lo, bool hi) {
+template <class T>
+void neumann(const Dims d, T* q, bool lo, bool hi) {
   const int nj = d.nj;
   launch_box(Box{-2, d.ni + 3, 0, 0, -2, d.nk + 3}, [=] __device__(int i, int, int k) {
     const int Y = d.sx;
@@ -350,18 +363,20 @@ void neumann(const Dims d, double* q, bool lo, bool hi) {
 }
 
 // srcs/gnbc.f90:25-97
-void gnbc(const Dims d, const double* uk, const double* wk, double uwall, const double* th0,
-          double st, const double* zeta, double* theta, bool lo, bool hi, double pi) {
+template <class T>
+void gnbc(const Dims d, const T* uk, const T* wk, double uwall, const double* th0, double st,
+          const double* zeta, double* theta, bool lo, bool hi, double pi) {
   const int ni = d.ni, nj = d.nj;
   launch_box(Box{-2, d.ni + 3, 0, 1, -2, d.nk + 3}, [=] __device__(int i, int side, int k) {
     if ((side == 0 && !lo) || (side == 1 && !hi)) return;
     const int j0 = side == 0 ? 0 : nj, j1 = j0 + 1, jw = side == 0 ? 1 : nj;
     const double uw = side == 0 ? uwall : -uwall;
     const int mi = (i + ni + 2) % (ni + 3);
-    const double ux =
-        (uk[d.ix(mi, j0, k)] + uk[d.ix(mi, j1, k)] + uk[d.ix(i, j0, k)] + uk[d.ix(i, j1, k)]) / 4.0 + uw;
-    const double uz =
-        (wk[d.ix(mi, j0, k)] + wk[d.ix(mi, j1, k)] + wk[d.ix(i, j0, k)] + wk[d.ix(i, j1, k)]) / 4.0;
+    // the contact-line slip velocity is evaluated in FP64
+    const double ux = ((double)uk[d.ix(mi, j0, k)] + (double)uk[d.ix(mi, j1, k)] +
+                       (double)uk[d.ix(i, j0, k)] + (double)uk[d.ix(i, j1, k)]) / 4.0 + uw;
+    const double uz = ((double)wk[d.ix(mi, j0, k)] + (double)wk[d.ix(mi, j1, k)] +
+                       (double)wk[d.ix(i, j0, k)] + (double)wk[d.ix(i, j1, k)]) / 4.0;
     double ucl = copysign(sqrt(ux * ux + uz * uz + 1.0e-12), ux);
     if (i <= ni / 2) ucl = -ucl;
     const int cw = d.ix(i, jw, k);
@@ -408,6 +423,59 @@ void contact_angle(const Dims d, double* q, const double* theta, double ax, doub
   });
 }
 
+template void bc_fused<double>(const Dims, double*, double*, double*, double, double,
+                               const double*, const double*, const double*, bool, bool, int, int);
+template void bc_fused<float>(const Dims, float*, float*, float*, double, double, const double*,
+                              const double*, const double*, bool, bool, int, int);
+template void periodic<double>(const Dims, double*);
+template void periodic<float>(const Dims, float*);
+template void neumann<double>(const Dims, double*, bool, bool);
+template void neumann<float>(const Dims, float*, bool, bool);
+
+// Drop-in bodies for the velocity element type T.
+template <class T>
+void bndu_t(const Dims d, const int* nID, double* uk, double* vk, double* wk, double uwall,
+            double dy, const double* l1, const double* l2, const double* phi, int mode, int swept) {
+  Call c("bndu_");
+  T *u = c.io_t<T>(uk, d.n3, "uk"), *v = c.io_t<T>(vk, d.n3, "vk"), *w = c.io_t<T>(wk, d.n3, "wk");
+  const double *a = c.in(l1, d.n3, "l1"), *b = c.in(l2, d.n3, "l2"), *p = c.in(phi, d.n3, "phi");
+  const bool lo = nID[kYMinus] < 0, hi = nID[kYPlus] < 0;
+  if (mode == 0) {
+    bndu<T>(d, u, v, w, uwall, dy, a, b, p, lo, hi);
+  } else if (mode == 1) {
+    bndu_phased<T>(d, u, v, w, uwall, dy, a, b, p, lo, hi);
+    if (swept >= 0) periodic<T>(d, swept == 0 ? u : (swept == 1 ? v : w));
+  } else {
+    bc_fused<T>(d, u, v, w, uwall, dy, a, b, p, lo, hi, swept, 7);
+  }
+  c.finish();
+}
+
+template <class T>
+void periodic_t(const Dims d, double* qk) {
+  Call c("bnd_periodic_");
+  periodic<T>(d, c.io_t<T>(qk, d.n3, "qk"));
+  c.finish();
+}
+
+template <class T>
+void neumann_t(const Dims d, const int* nID, double* qk) {
+  Call c("bnd_neumann_");
+  neumann<T>(d, c.io_t<T>(qk, d.n3, "qk"), nID[kYMinus] < 0, nID[kYPlus] < 0);
+  c.finish();
+}
+
+template <class T>
+void gnbc_t(const Dims d, const int* nID, const double* uk, const double* wk, double uwall,
+            const double* theta_0_array, double st, const double* zeta_array, double* theta_array) {
+  Call c("gnbc_");
+  const T *u = c.in_t<T>(uk, d.n3, "uk"), *w = c.in_t<T>(wk, d.n3, "wk");
+  const double *t0 = c.in(theta_0_array, d.n3, "theta_0"), *z = c.in(zeta_array, d.n3, "zeta");
+  double* th = c.io(theta_array, d.n3, "theta");
+  gnbc<T>(d, u, w, uwall, t0, st, z, th, nID[kYMinus] < 0, nID[kYPlus] < 0, std::acos(-1.0));
+  c.finish();
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -419,11 +487,8 @@ void bndu_(const int* nID, const int* ni, const int* nj, const int* nk, double* 
            double* wk, const double* uwall, const double* dy, const double* l1, const double* l2,
            const double* phi) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("bndu_");
-  double *u = c.io(uk, d.n3), *v = c.io(vk, d.n3), *w = c.io(wk, d.n3);
-  const double *a = c.in(l1, d.n3), *b = c.in(l2, d.n3), *p = c.in(phi, d.n3);
-  bndu(d, u, v, w, *uwall, *dy, a, b, p, nID[kYMinus] < 0, nID[kYPlus] < 0);
-  c.finish();
+  if (flow_fp32()) bndu_t<float>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 0, -1);
+  else bndu_t<double>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 0, -1);
 }
 
 // Cross-check entry point: bndu with the reference's three ordered passes
@@ -434,12 +499,8 @@ void felgpu_bndu_phased_(const int* nID, const int* ni, const int* nj, const int
                          double* vk, double* wk, const double* uwall, const double* dy,
                          const double* l1, const double* l2, const double* phi, const int* swept) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("felgpu_bndu_phased_");
-  double *u = c.io(uk, d.n3), *v = c.io(vk, d.n3), *w = c.io(wk, d.n3);
-  const double *a = c.in(l1, d.n3), *b = c.in(l2, d.n3), *p = c.in(phi, d.n3);
-  bndu_phased(d, u, v, w, *uwall, *dy, a, b, p, nID[kYMinus] < 0, nID[kYPlus] < 0);
-  if (*swept >= 0) periodic(d, *swept == 0 ? u : (*swept == 1 ? v : w));
-  c.finish();
+  if (flow_fp32()) bndu_t<float>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 1, *swept);
+  else bndu_t<double>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 1, *swept);
 }
 
 // The composed single launch of bndu_ + bnd_periodic(component swept).
@@ -447,27 +508,22 @@ void felgpu_bc_fused_(const int* nID, const int* ni, const int* nj, const int* n
                       double* vk, double* wk, const double* uwall, const double* dy,
                       const double* l1, const double* l2, const double* phi, const int* swept) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("felgpu_bc_fused_");
-  double *u = c.io(uk, d.n3), *v = c.io(vk, d.n3), *w = c.io(wk, d.n3);
-  const double *a = c.in(l1, d.n3), *b = c.in(l2, d.n3), *p = c.in(phi, d.n3);
-  bc_fused(d, u, v, w, *uwall, *dy, a, b, p, nID[kYMinus] < 0, nID[kYPlus] < 0, *swept);
-  c.finish();
+  if (flow_fp32()) bndu_t<float>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 2, *swept);
+  else bndu_t<double>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 2, *swept);
 }
 
 // srcs/bnd_periodic.f:4
 void bnd_periodic_(const int* ni, const int* nj, const int* nk, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("bnd_periodic_");
-  periodic(d, c.io(qk, d.n3));
-  c.finish();
+  if (elem_size(qk) == 4) periodic_t<float>(d, qk);
+  else periodic_t<double>(d, qk);
 }
 
 // srcs/bnd_neumann.f:4
 void bnd_neumann_(const int* nID, const int* ni, const int* nj, const int* nk, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("bnd_neumann_");
-  neumann(d, c.io(qk, d.n3), nID[kYMinus] < 0, nID[kYPlus] < 0);
-  c.finish();
+  if (elem_size(qk) == 4) neumann_t<float>(d, nID, qk);
+  else neumann_t<double>(d, nID, qk);
 }
 
 // srcs/bnd_comm.f:1-18: halo exchange between j-slabs; one GPU holds the
@@ -487,12 +543,10 @@ void gnbc_(const int* nID, const int* ni, const int* nj, const int* nk, const do
            const double* wk, const double* uwall, const double* theta_0_array,
            const double* surface_tension, const double* zeta_array, double* theta_array) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("gnbc_");
-  const double *u = c.in(uk, d.n3), *w = c.in(wk, d.n3);
-  const double *t0 = c.in(theta_0_array, d.n3), *z = c.in(zeta_array, d.n3);
-  double* th = c.io(theta_array, d.n3);
-  gnbc(d, u, w, *uwall, t0, *surface_tension, z, th, nID[kYMinus] < 0, nID[kYPlus] < 0, std::acos(-1.0));
-  c.finish();
+  if (flow_fp32())
+    gnbc_t<float>(d, nID, uk, wk, *uwall, theta_0_array, *surface_tension, zeta_array, theta_array);
+  else
+    gnbc_t<double>(d, nID, uk, wk, *uwall, theta_0_array, *surface_tension, zeta_array, theta_array);
 }
 
 // srcs/bnd_contact_angle.f90:3

@@ -9,16 +9,20 @@
 
 namespace fel {
 
+// T: element type of the gradients, normals, curvature and forces (double,
+// or float in the FP32 mode); the volume fraction is FP64.
+
 // srcs/cal_grad_p2uvw.f:25-43 (per-j dyinv from dyinv_array)
+template <class T>
 void grad_p2uvw(const Dims d, int ID, double dxinv, const double* dyt_dev, double dyu, double dzinv,
-                const double* q, double* qx, double* qy, double* qz) {
+                const double* q, T* qx, T* qy, T* qz) {
   launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
     const double dyinv = dyt_dev ? dyt_dev[j + d.nj * ID + 2] : dyu;
     const double q0 = q[c];
-    qx[c] = (-q0 + q[c + 1]) * dxinv;
-    qy[c] = (-q0 + q[c + d.sx]) * dyinv;
-    qz[c] = (-q0 + q[c + d.sx * d.sy]) * dzinv;
+    qx[c] = T((-q0 + q[c + 1]) * dxinv);
+    qy[c] = T((-q0 + q[c + d.sx]) * dyinv);
+    qz[c] = T((-q0 + q[c + d.sx * d.sy]) * dzinv);
   });
 }
 
@@ -29,12 +33,12 @@ void grad_p2uvw(const Dims d, int ID, double dxinv, const double* dyt_dev, doubl
 // outside the shifted loop range keep the cleared value 0.
 constexpr int kRnChunk = 16;
 
-__global__ void __launch_bounds__(256) calrn_kernel(Dims d, double verysmall, int ip, int jp, int kp,
-                                                    const double* __restrict__ px,
-                                                    const double* __restrict__ py,
-                                                    const double* __restrict__ pz,
-                                                    double* __restrict__ rn, int gx, int gy,
-                                                    int ntiles) {
+template <class T>
+__global__ void __launch_bounds__(256) calrn_kernel(Dims d, T verysmall, int ip, int jp, int kp,
+                                                    const T* __restrict__ px,
+                                                    const T* __restrict__ py,
+                                                    const T* __restrict__ pz, T* __restrict__ rn,
+                                                    int gx, int gy, int ntiles) {
   const int L = xcd_tile((int)blockIdx.x, ntiles);
   const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
   const int ii = -2 + bx * 64 + (int)threadIdx.x;
@@ -44,23 +48,23 @@ __global__ void __launch_bounds__(256) calrn_kernel(Dims d, double verysmall, in
   const int k0 = -2 + bz * kRnChunk, k1 = min(d.nk + 3, k0 + kRnChunk - 1);
   const bool col_ok = ii >= -2 + ip && jj >= -2 + jp;
   const int X = ip, Y = jp * d.sx;
-  double pa[3][4];  // previous plane: (i,j), (ii,j), (i,jj), (ii,jj)
+  T pa[3][4];  // previous plane: (i,j), (ii,j), (i,jj), (ii,jj)
   bool have_prev = false;
   for (int kk = k0; kk <= k1; kk++) {
     const int tgt = d.ix(ii, jj, kk);
     if (!col_ok || kk < -2 + kp) {
-      rn[tgt] = 0.0;
-      rn[n3 + tgt] = 0.0;
-      rn[2 * n3 + tgt] = 0.0;
+      rn[tgt] = T(0.0);
+      rn[n3 + tgt] = T(0.0);
+      rn[2 * n3 + tgt] = T(0.0);
       have_prev = false;
       continue;
     }
     const int c = tgt - X - Y;  // (i, j, kk) with i = ii-ip, j = jj-jp
-    double cur[3][4];
-    const double* a3[3] = {px, py, pz};
+    T cur[3][4];
+    const T* a3[3] = {px, py, pz};
 #pragma unroll
     for (int m = 0; m < 3; m++) {
-      const double* p = a3[m];
+      const T* p = a3[m];
       cur[m][0] = p[c];
       cur[m][1] = p[c + X];
       cur[m][2] = p[c + Y];
@@ -76,7 +80,7 @@ __global__ void __launch_bounds__(256) calrn_kernel(Dims d, double verysmall, in
         const int cb = c - Z;
 #pragma unroll
         for (int m = 0; m < 3; m++) {
-          const double* p = a3[m];
+          const T* p = a3[m];
           pa[m][0] = p[cb];
           pa[m][1] = p[cb + X];
           pa[m][2] = p[cb + Y];
@@ -84,14 +88,14 @@ __global__ void __launch_bounds__(256) calrn_kernel(Dims d, double verysmall, in
         }
       }
     }
-    double r[3];
+    T r[3];
 #pragma unroll
     for (int m = 0; m < 3; m++)
       r[m] = (+pa[m][0] + pa[m][1] + pa[m][2] + pa[m][3] + cur[m][0] + cur[m][1] + cur[m][2] +
               cur[m][3]) *
-             0.125;
-    const double arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
-    const double inv = 1.0 / arn;
+             T(0.125);
+    const T arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
+    const T inv = T(1.0) / arn;
     rn[tgt] = r[0] * inv;
     rn[n3 + tgt] = r[1] * inv;
     rn[2 * n3 + tgt] = r[2] * inv;
@@ -103,23 +107,26 @@ __global__ void __launch_bounds__(256) calrn_kernel(Dims d, double verysmall, in
   }
 }
 
-void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const double* px, const double* py,
-           const double* pz, double* rn) {
+template <class T>
+void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const T* px, const T* py,
+           const T* pz, T* rn) {
   const int gx = (d.sx + 63) / 64, gy = (d.sy + 3) / 4, gz = (d.sz + kRnChunk - 1) / kRnChunk;
   const int n = gx * gy * gz;
-  calrn_kernel<<<dim3(n), dim3(64, 4, 1), 0, stream()>>>(d, verysmall, ip, jp, kp, px, py, pz, rn,
-                                                         gx, gy, xcd_arg(2, n));
+  calrn_kernel<T><<<dim3(n), dim3(64, 4, 1), 0, stream()>>>(d, T(verysmall), ip, jp, kp, px, py, pz,
+                                                            rn, gx, gy, xcd_arg(2, n));
   FEL_CHECK(hipGetLastError());
 }
 
 // srcs/calrkap.f:20-97
-void calrkap(const Dims d, double dxinv, double dyinv, double dzinv, const double* rpp,
-             const double* r12, const double* r13, const double* r23, double* rk) {
+template <class T>
+void calrkap(const Dims d, double dxinv_, double dyinv_, double dzinv_, const T* rpp, const T* r12,
+             const T* r13, const T* r23, T* rk) {
   const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
+  const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_);
   launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int C2 = n3, C3 = 2 * n3;
-    double a = 0.0, b = 0.0, e = 0.0;
+    T a = T(0.0), b = T(0.0), e = T(0.0);
     if (k >= -1 && j >= -1 && i <= ni + 2)
       a = +(-rpp[c] + rpp[c + X]) * dxinv + (-r12[C2 + c - Y] + r12[C2 + c]) * dyinv +
           (-r13[C3 + c - Z] + r13[C3 + c]) * dzinv;
@@ -136,14 +143,77 @@ void calrkap(const Dims d, double dxinv, double dyinv, double dzinv, const doubl
 }
 
 // srcs/calfst.f:37-45
-void calfst(const Dims d, double st, const double* px, const double* py, const double* pz,
-            const double* rk, double* fu, double* fv, double* fw) {
+template <class T>
+void calfst(const Dims d, double st_, const T* px, const T* py, const T* pz, const T* rk, T* fu,
+            T* fv, T* fw) {
   const int n3 = d.n3;
+  const T st = T(st_);
   launch_flat(n3, [=] __device__(int m) {
     fu[m] = -(st * rk[m] * px[m]);
     fv[m] = -(st * rk[n3 + m] * py[m]);
     fw[m] = -(st * rk[2 * n3 + m] * pz[m]);
   });
+}
+
+template <class T>
+void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const double* dyinv_array,
+                      double dzinv, const double* q, double* qx_u, double* qy_v, double* qz_w) {
+  Call c("cal_grad_p2uvw_");
+  bool uni = true;
+  const int cnt = svall + 6;
+  for (int m = 0; m < cnt; m++)
+    if (dyinv_array[m] != dyinv_array[0]) uni = false;
+  const double* table = nullptr;
+  if (!uni) {
+    double* dev = static_cast<double*>(scratch(2, sizeof(double) * cnt));
+    FEL_CHECK(hipMemcpyAsync(dev, dyinv_array, sizeof(double) * cnt, hipMemcpyHostToDevice, stream()));
+    FEL_CHECK(hipStreamSynchronize(stream()));
+    table = dev;
+  }
+  const double* dq = c.in(q, d.n3, "q");
+  T* x = c.io_t<T>(qx_u, d.n3, "qx_u");
+  T* y = c.io_t<T>(qy_v, d.n3, "qy_v");
+  T* z = c.io_t<T>(qz_w, d.n3, "qz_w");
+  grad_p2uvw<T>(d, ID, dxinv, table, dyinv_array[0], dzinv, dq, x, y, z);
+  c.finish();
+}
+
+template <class T>
+void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix, const double* phiy,
+             const double* phiz, double* rn) {
+  Call c("calrn_");
+  const T* x = c.in_t<T>(phix, d.n3, "phix");
+  const T* y = c.in_t<T>(phiy, d.n3, "phiy");
+  const T* z = c.in_t<T>(phiz, d.n3, "phiz");
+  T* r = c.out_t<T>(rn, 3 * (size_t)d.n3, "rn");
+  calrn<T>(d, vs, ip, jp, kp, x, y, z, r);
+  c.finish();
+}
+
+template <class T>
+void calrkap_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* rn_pp,
+               const double* rn_12, const double* rn_13, const double* rn_23, double* rkap) {
+  Call c("calrkap_");
+  const size_t n = 3 * (size_t)d.n3;
+  const T *a = c.in_t<T>(rn_pp, n, "rn_pp"), *b = c.in_t<T>(rn_12, n, "rn_12");
+  const T *e = c.in_t<T>(rn_13, n, "rn_13"), *f = c.in_t<T>(rn_23, n, "rn_23");
+  T* r = c.out_t<T>(rkap, n, "rkap");
+  calrkap<T>(d, dxinv, dyinv, dzinv, a, b, e, f, r);
+  c.finish();
+}
+
+template <class T>
+void calfst_t(const Dims d, double st, const double* phix_u, const double* phiy_v,
+              const double* phiz_w, const double* rkap, double* fst_u, double* fst_v,
+              double* fst_w) {
+  Call c("calfst_");
+  const T *x = c.in_t<T>(phix_u, d.n3, "phix_u"), *y = c.in_t<T>(phiy_v, d.n3, "phiy_v"),
+          *z = c.in_t<T>(phiz_w, d.n3, "phiz_w");
+  const T* rk = c.in_t<T>(rkap, 3 * (size_t)d.n3, "rkap");
+  T *fu = c.out_t<T>(fst_u, d.n3, "fst_u"), *fv = c.out_t<T>(fst_v, d.n3, "fst_v"),
+    *fw = c.out_t<T>(fst_w, d.n3, "fst_w");
+  calfst<T>(d, st, x, y, z, rk, fu, fv, fw);
+  c.finish();
 }
 
 }  // namespace fel
@@ -157,24 +227,10 @@ void cal_grad_p2uvw_(const int* ID, const int* svall, const int* ni, const int* 
                      const double* dxinv, const double* dyinv_array, const double* dzinv,
                      const double* q, double* qx_u, double* qy_v, double* qz_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_grad_p2uvw_");
-  bool uni = true;
-  const int cnt = *svall + 6;
-  for (int m = 0; m < cnt; m++)
-    if (dyinv_array[m] != dyinv_array[0]) uni = false;
-  const double* table = nullptr;
-  if (!uni) {
-    double* dev = static_cast<double*>(scratch(2, sizeof(double) * cnt));
-    FEL_CHECK(hipMemcpyAsync(dev, dyinv_array, sizeof(double) * cnt, hipMemcpyHostToDevice, stream()));
-    FEL_CHECK(hipStreamSynchronize(stream()));
-    table = dev;
-  }
-  const double* dq = c.in(q, d.n3);
-  double* x = c.io(qx_u, d.n3);
-  double* y = c.io(qy_v, d.n3);
-  double* z = c.io(qz_w, d.n3);
-  grad_p2uvw(d, *ID, *dxinv, table, dyinv_array[0], *dzinv, dq, x, y, z);
-  c.finish();
+  if (flow_fp32())
+    cal_grad_p2uvw_t<float>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx_u, qy_v, qz_w);
+  else
+    cal_grad_p2uvw_t<double>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx_u, qy_v, qz_w);
 }
 
 // srcs/calrn.f:1-2
@@ -182,14 +238,9 @@ void calrn_(const int* ni, const int* nj, const int* nk, const double* dxinv, co
             const double* dzinv, const int* ip, const int* jp, const int* kp, const double* phix,
             const double* phiy, const double* phiz, double* rn) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("calrn_");
-  const double* x = c.in(phix, d.n3);
-  const double* y = c.in(phiy, d.n3);
-  const double* z = c.in(phiz, d.n3);
-  double* r = c.out(rn, 3 * (size_t)d.n3);
   const double vs = std::sqrt(*dxinv * *dxinv + *dyinv * *dyinv + *dzinv * *dzinv) * 1.0e-30;
-  calrn(d, vs, *ip, *jp, *kp, x, y, z, r);
-  c.finish();
+  if (flow_fp32()) calrn_t<float>(d, vs, *ip, *jp, *kp, phix, phiy, phiz, rn);
+  else calrn_t<double>(d, vs, *ip, *jp, *kp, phix, phiy, phiz, rn);
 }
 
 // srcs/calrkap.f:1-2
@@ -197,12 +248,8 @@ void calrkap_(const int* ni, const int* nj, const int* nk, const double* dxinv, 
               const double* dzinv, const double* rn_pp, const double* rn_12, const double* rn_13,
               const double* rn_23, double* rkap) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("calrkap_");
-  const size_t n = 3 * (size_t)d.n3;
-  const double *a = c.in(rn_pp, n), *b = c.in(rn_12, n), *e = c.in(rn_13, n), *f = c.in(rn_23, n);
-  double* r = c.out(rkap, n);
-  calrkap(d, *dxinv, *dyinv, *dzinv, a, b, e, f, r);
-  c.finish();
+  if (flow_fp32()) calrkap_t<float>(d, *dxinv, *dyinv, *dzinv, rn_pp, rn_12, rn_13, rn_23, rkap);
+  else calrkap_t<double>(d, *dxinv, *dyinv, *dzinv, rn_pp, rn_12, rn_13, rn_23, rkap);
 }
 
 // srcs/calfst.f:1-3
@@ -210,13 +257,9 @@ void calfst_(const int* ni, const int* nj, const int* nk, const double* rhol, co
              const double* surface_tension, const double* phix_u, const double* phiy_v,
              const double* phiz_w, const double* rkap, double* fst_u, double* fst_v, double* fst_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("calfst_");
-  const double *x = c.in(phix_u, d.n3), *y = c.in(phiy_v, d.n3), *z = c.in(phiz_w, d.n3);
-  const double* rk = c.in(rkap, 3 * (size_t)d.n3);
-  double *fu = c.out(fst_u, d.n3), *fv = c.out(fst_v, d.n3), *fw = c.out(fst_w, d.n3);
   const double st = 2.0 * (*surface_tension) / (*rhol + *rhog);
-  calfst(d, st, x, y, z, rk, fu, fv, fw);
-  c.finish();
+  if (flow_fp32()) calfst_t<float>(d, st, phix_u, phiy_v, phiz_w, rkap, fst_u, fst_v, fst_w);
+  else calfst_t<double>(d, st, phix_u, phiy_v, phiz_w, rkap, fst_u, fst_v, fst_w);
 }
 
 }  // extern "C"

@@ -7,6 +7,38 @@
 
 #include "felgpu.hpp"
 
+namespace fel {
+
+// Phase-weighted velocity sums (FP64) of srcs/cal_vel.f:43-53; T is the
+// velocity element type.  Returns the pinned read-back.
+template <class T>
+const double* vel_sums(const Dims d, const double* phi, const double* u, const double* v,
+                       const double* w) {
+  Call c("cal_vel_");
+  const double* p = c.in(phi, d.n3, "phi");
+  const T *du = c.in_t<T>(u, d.n3, "u"), *dv = c.in_t<T>(v, d.n3, "v"), *dw = c.in_t<T>(w, d.n3, "w");
+  double* s = dev_scalars(7);
+  interior_reduce<4>(
+      d,
+      [=] __device__(int i, int j, int k, double* o) {
+        const int m = d.ix(i, j, k);
+        const double ph = p[m];
+        o[0] = ph;
+        o[1] = ph * (double)du[m];
+        o[2] = ph * (double)dv[m];
+        o[3] = ph * (double)dw[m];
+      },
+      [=] __device__(const double* t, double* out) {
+        for (int m = 0; m < 4; m++) out[m] = t[m];
+      },
+      s);
+  const double* r = read_scalars(s, 4);
+  c.finish();
+  return r;
+}
+
+}  // namespace fel
+
 using namespace fel;
 
 extern "C" {
@@ -72,25 +104,7 @@ void cal_vel_(const int* ipara, const int* ni, const int* nj, const int* nk, con
               const double* w, const double* phi, double* vel, double* re) {
   (void)ipara;
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_vel_");
-  const double *p = c.in(phi, d.n3), *du = c.in(u, d.n3), *dv = c.in(v, d.n3), *dw = c.in(w, d.n3);
-  double* s = dev_scalars(7);
-  interior_reduce<4>(
-      d,
-      [=] __device__(int i, int j, int k, double* o) {
-        const int m = d.ix(i, j, k);
-        const double ph = p[m];
-        o[0] = ph;
-        o[1] = ph * du[m];
-        o[2] = ph * dv[m];
-        o[3] = ph * dw[m];
-      },
-      [=] __device__(const double* t, double* out) {
-        for (int m = 0; m < 4; m++) out[m] = t[m];
-      },
-      s);
-  const double* r = read_scalars(s, 4);
-  c.finish();
+  const double* r = flow_fp32() ? vel_sums<float>(d, phi, u, v, w) : vel_sums<double>(d, phi, u, v, w);
   const double diameter = *radius * 2.0;
   for (int m = 0; m < 3; m++) {
     vel[m] = r[m + 1] / r[0];

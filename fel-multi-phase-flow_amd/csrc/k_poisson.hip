@@ -24,20 +24,37 @@
 
 namespace fel {
 
-// Twiddles exp(+2 pi i t / N), t in [0, N/2), from the host libm.
-static const double2* twiddles(int n) {
-  static std::map<int, double2*> cache;
+// Complex element of the transforms: double2 in FP64, float2 in the FP32 mode.
+template <class T>
+struct Cplx;
+template <>
+struct Cplx<double> {
+  using type = double2;
+  __host__ __device__ static double2 make(double a, double b) { return make_double2(a, b); }
+};
+template <>
+struct Cplx<float> {
+  using type = float2;
+  __host__ __device__ static float2 make(float a, float b) { return make_float2(a, b); }
+};
+
+// Twiddles exp(+2 pi i t / N), t in [0, N/2), from the host libm (FP64,
+// rounded once to T).
+template <class T>
+static const typename Cplx<T>::type* twiddles(int n) {
+  using C = typename Cplx<T>::type;
+  static std::map<int, C*> cache;
   auto it = cache.find(n);
   if (it != cache.end()) return it->second;
-  std::vector<double2> h(n / 2 > 0 ? n / 2 : 1);
+  std::vector<C> h(n / 2 > 0 ? n / 2 : 1);
   const double pi = std::acos(-1.0);
   for (int t = 0; t < n / 2; t++) {
     const double a = 2.0 * pi * (double)t / (double)n;
-    h[t] = make_double2(std::cos(a), std::sin(a));
+    h[t] = Cplx<T>::make(T(std::cos(a)), T(std::sin(a)));
   }
-  double2* d = nullptr;
-  FEL_CHECK(hipMalloc((void**)&d, sizeof(double2) * h.size()));
-  FEL_CHECK(hipMemcpy(d, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+  C* d = nullptr;
+  FEL_CHECK(hipMalloc((void**)&d, sizeof(C) * h.size()));
+  FEL_CHECK(hipMemcpy(d, h.data(), sizeof(C) * h.size(), hipMemcpyHostToDevice));
   cache[n] = d;
   return d;
 }
@@ -45,21 +62,22 @@ static const double2* twiddles(int n) {
 // Stockham radix-2 DIF transform of G-thread groups, one line per group,
 // ping-ponging between x and y (LDS, line stride ls).  sign = -1 forward.
 // Returns the buffer that holds the result.
-__device__ __forceinline__ double2* stockham(double2* x, double2* y, int n, int lane, int g,
-                                              const double2* __restrict__ tw, double sign) {
+template <class T, class C>
+__device__ __forceinline__ C* stockham(C* x, C* y, int n, int lane, int g, const C* __restrict__ tw,
+                                       T sign) {
   for (int s = 1; s < n; s *= 2) {
     const int m = n / (2 * s);
     for (int b = lane; b < n / 2; b += g) {
       const int p = b / s, q = b - p * s;
-      const double2 w0 = tw[p * s];
-      const double wr = w0.x, wi = sign * w0.y;
-      const double2 a = x[q + s * p], c = x[q + s * (p + m)];
-      const double dr = a.x - c.x, di = a.y - c.y;
-      y[q + s * (2 * p)] = make_double2(a.x + c.x, a.y + c.y);
-      y[q + s * (2 * p + 1)] = make_double2(dr * wr - di * wi, dr * wi + di * wr);
+      const C w0 = tw[p * s];
+      const T wr = w0.x, wi = sign * w0.y;
+      const C a = x[q + s * p], c = x[q + s * (p + m)];
+      const T dr = a.x - c.x, di = a.y - c.y;
+      y[q + s * (2 * p)] = Cplx<T>::make(a.x + c.x, a.y + c.y);
+      y[q + s * (2 * p + 1)] = Cplx<T>::make(dr * wr - di * wi, dr * wi + di * wr);
     }
     __syncthreads();
-    double2* t = x;
+    C* t = x;
     x = y;
     y = t;
   }
@@ -67,71 +85,82 @@ __device__ __forceinline__ double2* stockham(double2* x, double2* y, int n, int 
 }
 
 // x lines: N = line length (contiguous), LPB lines per 256-thread block.
-template <class Load, class Store>
-__global__ void __launch_bounds__(256) fft_x_kernel(int n, int nlines, double sign,
-                                                    const double2* __restrict__ tw, Load ld, Store st) {
-  extern __shared__ double2 sm[];
+template <class T, class Load, class Store>
+__global__ void __launch_bounds__(256) fft_x_kernel(int n, int nlines, T sign,
+                                                    const typename Cplx<T>::type* __restrict__ tw,
+                                                    Load ld, Store st) {
+  using C = typename Cplx<T>::type;
+  extern __shared__ unsigned char smem_x[];
+  C* sm = reinterpret_cast<C*>(smem_x);
   const int lpb = n >= 256 ? 1 : 256 / (n / 2 > 0 ? n / 2 : 1);  // lines per block
   const int g = 256 / lpb;                                        // threads per line
   const int li = (int)threadIdx.x / g, lane = (int)threadIdx.x - li * g;
   const int line = (int)blockIdx.x * lpb + li;
-  double2* x = sm + li * 2 * n;
-  double2* y = x + n;
+  C* x = sm + li * 2 * n;
+  C* y = x + n;
   const bool act = line < nlines;
-  for (int e = lane; e < n; e += g) x[e] = act ? ld(line, e) : make_double2(0.0, 0.0);
+  for (int e = lane; e < n; e += g) x[e] = act ? ld(line, e) : Cplx<T>::make(T(0.0), T(0.0));
   __syncthreads();
-  double2* r = stockham(x, y, n, lane, g, tw, sign);
+  C* r = stockham<T, C>(x, y, n, lane, g, tw, sign);
   if (act)
     for (int e = lane; e < n; e += g) st(line, e, r[e]);
 }
 
-// Strided lines: a block owns T lines that are adjacent in memory (tile
-// index t = 0..T-1) and transforms them together; element e of line l is
+// Strided lines: a block owns TL lines that are adjacent in memory (tile
+// index t = 0..TL-1) and transforms them together; element e of line l is
 // loaded/stored by the functors.
-template <class Load, class Store>
-__global__ void __launch_bounds__(256) fft_z_kernel(int n, int tiles, int T, double sign,
-                                                    const double2* __restrict__ tw, Load ld, Store st) {
-  extern __shared__ double2 sm[];
-  const int g = 256 / T;                  // threads per line
+template <class T, class Load, class Store>
+__global__ void __launch_bounds__(256) fft_z_kernel(int n, int tiles, int TL, T sign,
+                                                    const typename Cplx<T>::type* __restrict__ tw,
+                                                    Load ld, Store st) {
+  using C = typename Cplx<T>::type;
+  extern __shared__ unsigned char smem_z[];
+  C* sm = reinterpret_cast<C*>(smem_z);
+  const int g = 256 / TL;                 // threads per line
   const int tile = (int)blockIdx.x;
   // cooperative, row-major loads: consecutive threads take consecutive lines
-  for (int idx = (int)threadIdx.x; idx < T * n; idx += 256) {
-    const int e = idx / T, l = idx - e * T;
+  for (int idx = (int)threadIdx.x; idx < TL * n; idx += 256) {
+    const int e = idx / TL, l = idx - e * TL;
     sm[l * 2 * n + e] = ld(tile, l, e);
   }
   __syncthreads();
   const int l = (int)threadIdx.x / g, lane = (int)threadIdx.x - l * g;
-  double2* x = sm + l * 2 * n;
-  double2* r = stockham(x, x + n, n, lane, g, tw, sign);
+  C* x = sm + l * 2 * n;
+  C* r = stockham<T, C>(x, x + n, n, lane, g, tw, sign);
   const int off = (int)(r - x);  // 0 or n, the same for every line
-  for (int idx = (int)threadIdx.x; idx < T * n; idx += 256) {
-    const int e = idx / T, ll = idx - e * T;
+  for (int idx = (int)threadIdx.x; idx < TL * n; idx += 256) {
+    const int e = idx / TL, ll = idx - e * TL;
     st(tile, ll, e, sm[ll * 2 * n + off + e]);
   }
   (void)tiles;
 }
 
-template <class Load, class Store>
-static void fft_x(int n, int nlines, double sign, Load ld, Store st) {
+template <class T, class Load, class Store>
+static void fft_x(int n, int nlines, T sign, Load ld, Store st) {
+  using C = typename Cplx<T>::type;
   const int lpb = n >= 256 ? 1 : 256 / (n / 2 > 0 ? n / 2 : 1);
-  const size_t shm = sizeof(double2) * 2 * n * lpb;
-  fft_x_kernel<Load, Store><<<(nlines + lpb - 1) / lpb, 256, shm, stream()>>>(n, nlines, sign,
-                                                                           twiddles(n), ld, st);
+  const size_t shm = sizeof(C) * 2 * n * lpb;
+  fft_x_kernel<T, Load, Store><<<(nlines + lpb - 1) / lpb, 256, shm, stream()>>>(
+      n, nlines, sign, twiddles<T>(n), ld, st);
   FEL_CHECK(hipGetLastError());
 }
 
-// Tile width: T lines of n points, two LDS buffers each, <= 64 KiB.
+// Tile width: TL lines of n points, two LDS buffers each, <= 64 KiB.
+template <class T>
 static int z_tile(int n, int nlines_fast) {
-  int T = 16;
-  while (T > 1 && (size_t)T * 2 * n * sizeof(double2) > 65536) T /= 2;
-  while (T > 1 && nlines_fast % T != 0) T /= 2;
-  return T;
+  using C = typename Cplx<T>::type;
+  int TL = 16;
+  while (TL > 1 && (size_t)TL * 2 * n * sizeof(C) > 65536) TL /= 2;
+  while (TL > 1 && nlines_fast % TL != 0) TL /= 2;
+  return TL;
 }
 
-template <class Load, class Store>
-static void fft_z(int n, int tiles, int T, double sign, Load ld, Store st) {
-  const size_t shm = sizeof(double2) * 2 * n * T;
-  fft_z_kernel<Load, Store><<<tiles, 256, shm, stream()>>>(n, tiles, T, sign, twiddles(n), ld, st);
+template <class T, class Load, class Store>
+static void fft_z(int n, int tiles, int TL, T sign, Load ld, Store st) {
+  using C = typename Cplx<T>::type;
+  const size_t shm = sizeof(C) * 2 * n * TL;
+  fft_z_kernel<T, Load, Store><<<tiles, 256, shm, stream()>>>(n, tiles, TL, sign, twiddles<T>(n), ld,
+                                                              st);
   FEL_CHECK(hipGetLastError());
 }
 
@@ -143,89 +172,109 @@ static void check_pow2(int n, const char* who) {
 }
 
 // srcs/solp_fft_tdma1.f:25-47: phir(i,j,k) = DFT(div(1:ni,j,1:nk))/(ni nk)
-// at centred mode indices; T is (nk, nj, ni) complex scratch.
-void fft_forward(const Dims d, const double* div, double* pr, double* pi) {
+// at centred mode indices; the scratch holds (nk, nj, ni) complex values.
+// T: element type of the modes; the divergence is FP64.
+// `rhs(i,j,k)` gives the FP64 right-hand side at interior cell (i,j,k).
+template <class T, class Rhs>
+void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
+  using C = typename Cplx<T>::type;
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   check_pow2(ni, "solp_fft_tdma1_");
   check_pow2(nk, "solp_fft_tdma1_");
-  double2* T = static_cast<double2*>(scratch(4, sizeof(double2) * (size_t)ni * nj * nk));
+  C* TS = static_cast<C*>(scratch(4, sizeof(C) * (size_t)ni * nj * nk));
   // x transforms of the real rows (line = j + nj*k)
-  fft_x(ni, nj * nk, -1.0,
-        [=] __device__(int line, int e) {
-          const int j = line % nj, k = line / nj;
-          return make_double2(div[d.ix(e + 1, j + 1, k + 1)], 0.0);
-        },
-        [=] __device__(int line, int e, double2 v) { T[(size_t)line * ni + e] = v; });
+  fft_x<T>(ni, nj * nk, T(-1.0),
+           [=] __device__(int line, int e) {
+             const int j = line % nj, k = line / nj;
+             return Cplx<T>::make(T(rhs(e + 1, j + 1, k + 1)), T(0.0));
+           },
+           [=] __device__(int line, int e, C v) { TS[(size_t)line * ni + e] = v; });
   // k transforms; tile = (j, kx block), lines adjacent in kx
-  const int Tw = z_tile(nk, ni);
+  const int Tw = z_tile<T>(nk, ni);
   const int tiles_per_j = ni / Tw;
-  const double nrm = (double)(ni * nk);
-  fft_z(nk, nj * tiles_per_j, Tw, -1.0,
-        [=] __device__(int tile, int l, int e) {
-          const int j = tile % nj, kx = (tile / nj) * Tw + l;
-          return T[((size_t)e * nj + j) * ni + kx];
-        },
-        [=] __device__(int tile, int l, int e, double2 v) {
-          const int j = tile % nj, kx = (tile / nj) * Tw + l;
-          const int ic = (kx + ni / 2) % ni, kc = (e + nk / 2) % nk;
-          const size_t o = (size_t)ic + (size_t)ni * ((size_t)j + (size_t)nj * kc);
-          pr[o] = v.x / nrm;
-          pi[o] = v.y / nrm;
-        });
+  const T nrm = T((double)(ni * nk));
+  fft_z<T>(nk, nj * tiles_per_j, Tw, T(-1.0),
+           [=] __device__(int tile, int l, int e) {
+             const int j = tile % nj, kx = (tile / nj) * Tw + l;
+             return TS[((size_t)e * nj + j) * ni + kx];
+           },
+           [=] __device__(int tile, int l, int e, C v) {
+             const int j = tile % nj, kx = (tile / nj) * Tw + l;
+             const int ic = (kx + ni / 2) % ni, kc = (e + nk / 2) % nk;
+             const size_t o = (size_t)ic + (size_t)ni * ((size_t)j + (size_t)nj * kc);
+             pr[o] = v.x / nrm;
+             pi[o] = v.y / nrm;
+           });
 }
 
-// srcs/solp_fft_tdma3.f:25-44: dp(1:ni,j,1:nk) = Re(unnormalised inverse DFT).
-void fft_inverse(const Dims d, const double* pr, const double* pi, double* dp) {
+template <class T>
+void fft_forward(const Dims d, const double* div, T* pr, T* pi) {
+  fft_forward_of<T>(d, [=] __device__(int i, int j, int k) { return div[d.ix(i, j, k)]; }, pr, pi);
+}
+
+// srcs/solp_fft_tdma3.f:25-44: dp(1:ni,j,1:nk) = Re(unnormalised inverse DFT)
+// (added to dp instead when `add`: the refinement corrections).
+template <class T>
+void fft_inverse(const Dims d, const T* pr, const T* pi, double* dp, bool add) {
+  using C = typename Cplx<T>::type;
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   check_pow2(ni, "solp_fft_tdma3_");
   check_pow2(nk, "solp_fft_tdma3_");
-  double2* T = static_cast<double2*>(scratch(4, sizeof(double2) * (size_t)ni * nj * nk));
-  const int Tw = z_tile(nk, ni);
+  C* TS = static_cast<C*>(scratch(4, sizeof(C) * (size_t)ni * nj * nk));
+  const int Tw = z_tile<T>(nk, ni);
   const int tiles_per_j = ni / Tw;
-  fft_z(nk, nj * tiles_per_j, Tw, +1.0,
-        [=] __device__(int tile, int l, int e) {
-          const int j = tile % nj, kx = (tile / nj) * Tw + l;
-          const int ic = (kx + ni / 2) % ni, kc = (e + nk / 2) % nk;
-          const size_t o = (size_t)ic + (size_t)ni * ((size_t)j + (size_t)nj * kc);
-          return make_double2(pr[o], pi[o]);
-        },
-        [=] __device__(int tile, int l, int e, double2 v) {
-          const int j = tile % nj, kx = (tile / nj) * Tw + l;
-          T[((size_t)e * nj + j) * ni + kx] = v;
-        });
-  fft_x(ni, nj * nk, +1.0,
-        [=] __device__(int line, int e) { return T[(size_t)line * ni + e]; },
-        [=] __device__(int line, int e, double2 v) {
-          const int j = line % nj, k = line / nj;
-          dp[d.ix(e + 1, j + 1, k + 1)] = v.x;
-        });
+  fft_z<T>(nk, nj * tiles_per_j, Tw, T(1.0),
+           [=] __device__(int tile, int l, int e) {
+             const int j = tile % nj, kx = (tile / nj) * Tw + l;
+             const int ic = (kx + ni / 2) % ni, kc = (e + nk / 2) % nk;
+             const size_t o = (size_t)ic + (size_t)ni * ((size_t)j + (size_t)nj * kc);
+             return Cplx<T>::make(pr[o], pi[o]);
+           },
+           [=] __device__(int tile, int l, int e, C v) {
+             const int j = tile % nj, kx = (tile / nj) * Tw + l;
+             TS[((size_t)e * nj + j) * ni + kx] = v;
+           });
+  fft_x<T>(ni, nj * nk, T(1.0),
+           [=] __device__(int line, int e) { return TS[(size_t)line * ni + e]; },
+           [=] __device__(int line, int e, C v) {
+             const int j = line % nj, k = line / nj;
+             double* o = dp + d.ix(e + 1, j + 1, k + 1);
+             *o = add ? *o + (double)v.x : (double)v.x;
+           });
 }
 
 // srcs/solp_fft_tdma2.f:46-87: Thomas solve along j for every (kx,kz) mode
 // of the centred layout (i fastest, j, k).  c1x/c1z tables come from the
 // host libm; a(j) is kept in device scratch for the back substitution.
+// In FP32 the last pivot of the singular (0,0) mode, zero in exact
+// arithmetic, is set to pin that mode's x(nj) to 0 exactly, which the FP64
+// expression -d/(d^2+1e-99) does when the pivot rounds to zero.
+template <class T>
 void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, const double* coef,
-                double dx2r, double dz2r, double* pr, double* pi) {
-  double* at = static_cast<double*>(scratch(5, sizeof(double) * (size_t)ni * nj * nk));
+                double dx2r, double dz2r, T* pr, T* pi) {
+  T* at = static_cast<T*>(scratch(5, sizeof(T) * (size_t)ni * nj * nk));
   const double *as = coef, *an = coef + (nj + 6), *ap0 = coef + 2 * (nj + 6);
   launch_flat(ni * nk, [=] __device__(int m) {
     const int i = m % ni, k = m / ni;
     const double c1x = cx[i], c1z = cz[k];
-    double a_prev = 0.0, br = 0.0, bi = 0.0;
+    const bool singular = sizeof(T) == 4 && c1x == 2.0 && c1z == 2.0;
+    T a_prev = T(0.0), br = T(0.0), bi = T(0.0);
     const size_t base = (size_t)i + (size_t)ni * nj * k;
     for (int j = 1; j <= nj; j++) {
       const size_t o = base + (size_t)ni * (j - 1);
-      const double ap = ap0[j + 2] + dx2r * (2.0 - c1x) + dz2r * (2.0 - c1z);
-      const double d00 = -ap + as[j + 2] * a_prev;
-      const double apinv = -d00 / (d00 * d00 + 1.0e-99);
-      a_prev = apinv * an[j + 2];
-      br = apinv * (-pr[o] + as[j + 2] * br);
-      bi = apinv * (-pi[o] + as[j + 2] * bi);
+      const T ap = T(ap0[j + 2] + dx2r * (2.0 - c1x) + dz2r * (2.0 - c1z));
+      const T asj = T(as[j + 2]);
+      const T d00 = -ap + asj * a_prev;
+      T apinv = -d00 / (d00 * d00 + guard<T>(1.0e-99));
+      if (singular && j == nj) apinv = T(0.0);
+      a_prev = apinv * T(an[j + 2]);
+      br = apinv * (-pr[o] + asj * br);
+      bi = apinv * (-pi[o] + asj * bi);
       at[o] = a_prev;
       pr[o] = br;
       pi[o] = bi;
     }
-    double xr = br, xi = bi;
+    T xr = br, xi = bi;
     for (int j = nj - 1; j >= 1; j--) {
       const size_t o = base + (size_t)ni * (j - 1);
       xr = pr[o] + at[o] * xr;
@@ -237,12 +286,16 @@ void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, cons
 }
 
 // srcs/caldiv.f:24-41 on [-1,n+3]^3; the RMS (caldiv.f:43-67) is returned
-// through dsum (device, sum of squares over the interior).
-void divergence(const Dims d, double dxinv, double dyinv, double dzinv, double dtinv,
-                const double* u, const double* v, const double* w, double* div, double* dsum) {
+// through dsum (device, sum of squares over the interior).  The divergence
+// is FP64 (velocity values of either precision are exact in FP64).
+template <class T>
+void divergence(const Dims d, double dxinv, double dyinv, double dzinv, double dtinv, const T* u,
+                const T* v, const T* w, double* div, double* dsum) {
   launch_box(Box{-1, d.ni + 3, -1, d.nj + 3, -1, d.nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    div[c] = ((-u[c - X] + u[c]) * dxinv + (-v[c - Y] + v[c]) * dyinv + (-w[c - Z] + w[c]) * dzinv) * dtinv;
+    div[c] = ((-(double)u[c - X] + (double)u[c]) * dxinv + (-(double)v[c - Y] + (double)v[c]) * dyinv +
+              (-(double)w[c - Z] + (double)w[c]) * dzinv) *
+             dtinv;
   });
   interior_reduce<1>(
       d,
@@ -253,28 +306,30 @@ void divergence(const Dims d, double dxinv, double dyinv, double dzinv, double d
       [=] __device__(const double* t, double* out) { out[0] = t[0]; }, dsum);
 }
 
-// srcs/bnd_avzero.f:10-23 with srcs/cal_av.f:20-37: the mean stays on
-// the device.
-void avzero(const Dims d, double* q, int ndiv) {
+// srcs/bnd_avzero.f:10-23 with srcs/cal_av.f:20-37: the mean (FP64) stays
+// on the device.  Q: element type of the field.
+template <class Q>
+void avzero(const Dims d, Q* q, int ndiv) {
   double* s = dev_scalars(2);
   const double nn = (double)(d.ni * d.nj * d.nk * ndiv);
   interior_reduce<1>(
-      d, [=] __device__(int i, int j, int k, double* o) { o[0] = q[d.ix(i, j, k)]; },
+      d, [=] __device__(int i, int j, int k, double* o) { o[0] = (double)q[d.ix(i, j, k)]; },
       [=] __device__(const double* t, double* out) { out[0] = t[0] / nn; }, s);
   const double* av = s;
-  launch_flat(d.n3, [=] __device__(int m) { q[m] = q[m] - *av; });
+  launch_flat(d.n3, [=] __device__(int m) { q[m] = Q((double)q[m] - *av); });
 }
 
-// srcs/corunp_explicit.f:26-74
+// srcs/corunp_explicit.f:26-74; dp is FP64, the velocities and pressure T.
+template <class T>
 void correct(const Dims d, int jen, double rho, double dxinv, double dyinv, double dzinv, double dt,
-             const double* dp, const double* phat, double* un, double* vn, double* wn, double* pn) {
+             const double* dp, const T* phat, T* un, T* vn, T* wn, T* pn) {
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
     const double p0 = dp[c];
-    un[c] = un[c] + (p0 - dp[c + X]) / rho * dxinv * dt;
-    if (j <= jen) vn[c] = vn[c] + (p0 - dp[c + Y]) / rho * dyinv * dt;
-    wn[c] = wn[c] + (p0 - dp[c + Z]) / rho * dzinv * dt;
-    pn[c] = phat[c] + p0;
+    un[c] = T((double)un[c] + (p0 - dp[c + X]) / rho * dxinv * dt);
+    if (j <= jen) vn[c] = T((double)vn[c] + (p0 - dp[c + Y]) / rho * dyinv * dt);
+    wn[c] = T((double)wn[c] + (p0 - dp[c + Z]) / rho * dzinv * dt);
+    pn[c] = T((double)phat[c] + p0);
   });
 }
 
@@ -302,6 +357,158 @@ void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv,
       dsum);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 mode: iterative refinement of the pressure correction.
+//
+// The modes and the Thomas solves run in FP32; the divergence (right-hand
+// side) and the correction dp are FP64.  After the first solve, each
+// refinement step forms the FP64 residual r = div - L dp of the operator the
+// FFT+TDMA inverts (periodic in i and k, the wall rows' zero fluxes folded
+// into as(1) = an(nj) = 0), solves L c = r in FP32 and adds c to dp.  The
+// residual shrinks by the FP32 solve's relative accuracy per step.
+// solp_fft_tdma1_ records the right-hand side, solp_fft_tdma2_ the modal
+// coefficients; solp_fft_tdma3_ runs the refinement.  FELGPU_REFINE sets
+// the number of steps (default 1: one step takes the FP32 solve's relative
+// error of ~1e-6 to ~1e-12; 0 disables).
+// ---------------------------------------------------------------------------
+struct PoissonRecord {
+  bool rhs = false, coef = false;
+  int ni = 0, nj = 0, nk = 0;
+  const double* div = nullptr;   // device
+  uint64_t div_ver = 0;
+  const void* div_host = nullptr;
+  double dx2r = 0.0, dz2r = 0.0;
+  const double* tab = nullptr;   // device: c1x[ni], c1z[nk], as, an, ap (nj+6 each)
+};
+
+static PoissonRecord& prec() {
+  static PoissonRecord r;
+  return r;
+}
+
+static int refine_steps() {
+  static const int n = [] {
+    const char* e = std::getenv("FELGPU_REFINE");
+    const int v = e ? std::atoi(e) : 1;
+    return v < 0 ? 0 : (v > 8 ? 8 : v);
+  }();
+  return n;
+}
+
+static void poisson_refine(const Dims d, double* dp) {
+  PoissonRecord& pr = prec();
+  const int steps = refine_steps();
+  if (steps == 0 || !pr.rhs || !pr.coef || pr.ni != d.ni || pr.nj != d.nj || pr.nk != d.nk) return;
+  if (content_version(pr.div_host, d.n3) != pr.div_ver) return;  // right-hand side changed
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const size_t nm = (size_t)ni * nj * nk;
+  float* mr = static_cast<float*>(scratch(9, sizeof(float) * 2 * nm));
+  float* mi = mr + nm;
+  const double* div = pr.div;
+  const double* as = pr.tab + ni + nk;
+  const double* an = as + (nj + 6);
+  const double dx2r = pr.dx2r, dz2r = pr.dz2r;
+  for (int it = 0; it < steps; it++) {
+    // the FP64 residual r = div - L dp, formed as the transform loads it
+    fft_forward_of<float>(
+        d,
+        [=] __device__(int i, int j, int k) {
+          const int c = d.ix(i, j, k), Y = d.sx;
+          const int im = i == 1 ? ni : i - 1, ip = i == ni ? 1 : i + 1;
+          const int km = k == 1 ? nk : k - 1, kp = k == nk ? 1 : k + 1;
+          const double p0 = dp[c];
+          double l = dx2r * (dp[d.ix(im, j, k)] - 2.0 * p0 + dp[d.ix(ip, j, k)]) +
+                     dz2r * (dp[d.ix(i, j, km)] - 2.0 * p0 + dp[d.ix(i, j, kp)]);
+          const double a_s = as[j + 2], a_n = an[j + 2];
+          if (a_s != 0.0) l += a_s * (dp[c - Y] - p0);
+          if (a_n != 0.0) l += a_n * (dp[c + Y] - p0);
+          return div[c] - l;
+        },
+        mr, mi);
+    tdma_modes<float>(ni, nj, nk, pr.tab, pr.tab + ni, pr.tab + ni + nk, dx2r, dz2r, mr, mi);
+    fft_inverse<float>(d, mr, mi, dp, true);
+  }
+}
+
+template <class T>
+void caldiv_t(const Dims d, double dxinv, double dyinv, double dzinv, double dt, const double* uk,
+              const double* vk, const double* wk, double* div, double* sum) {
+  Call c("caldiv_");
+  const T *u = c.in_t<T>(uk, d.n3, "uk"), *v = c.in_t<T>(vk, d.n3, "vk"),
+          *w = c.in_t<T>(wk, d.n3, "wk");
+  double* dv = c.io(div, d.n3, "div");
+  double* s = dev_scalars(3);
+  divergence<T>(d, dxinv, dyinv, dzinv, 1.0 / dt, u, v, w, dv, s);
+  *sum = read_scalars(s, 1)[0];
+  c.finish();
+}
+
+template <class T>
+void tdma1_t(const Dims d, const double* div, double* phir_r, double* phir_i) {
+  const size_t nm = (size_t)d.ni * d.nj * d.nk;
+  Call c("solp_fft_tdma1_");
+  const double* dv = c.in(div, d.n3, "div");
+  T* pr = c.out_t<T>(phir_r, nm, "phir_r");
+  T* pi = c.out_t<T>(phir_i, nm, "phir_i");
+  prof_begin(5);
+  fft_forward<T>(d, dv, pr, pi);
+  prof_end(5);
+  c.finish();
+  PoissonRecord& rec = prec();
+  rec.rhs = true;
+  rec.ni = d.ni; rec.nj = d.nj; rec.nk = d.nk;
+  rec.div = dv;
+  rec.div_host = div;
+  rec.div_ver = content_version(div, d.n3);
+  if (rec.div_ver == 0) rec.rhs = false;  // unregistered: no refinement
+}
+
+template <class T>
+void tdma3_t(const Dims d, const double* phir_r, const double* phir_i, double* dp) {
+  const size_t nm = (size_t)d.ni * d.nj * d.nk;
+  Call c("solp_fft_tdma3_");
+  const T* pr = c.in_t<T>(phir_r, nm, "phir_r");
+  const T* pi = c.in_t<T>(phir_i, nm, "phir_i");
+  double* ddp = c.io(dp, d.n3, "dp");
+  prof_begin(7);
+  fft_inverse<T>(d, pr, pi, ddp, false);
+  if (sizeof(T) == 4) poisson_refine(d, ddp);
+  prof_end(7);
+  c.finish();
+}
+
+template <class Q>
+void avzero_t(const Dims d, double* q, int ndiv) {
+  Call c("bnd_avzero_");
+  avzero<Q>(d, c.io_t<Q>(q, d.n3, "q"), ndiv);
+  c.finish();
+}
+
+template <class Q>
+double cal_av_t(const Dims d, const double* q) {
+  Call c("cal_av_");
+  const Q* dq = c.in_t<Q>(q, d.n3, "q");
+  double* s = dev_scalars(5);
+  interior_reduce<1>(
+      d, [=] __device__(int i, int j, int k, double* o) { o[0] = (double)dq[d.ix(i, j, k)]; },
+      [=] __device__(const double* t, double* out) { out[0] = t[0]; }, s);
+  const double sum = read_scalars(s, 1)[0];
+  c.finish();
+  return sum;
+}
+
+template <class T>
+void corunp_t(const Dims d, int jen, double rho, double dxinv, double dyinv, double dzinv, double dt,
+              const double* dp, const double* phat, double* un, double* vn, double* wn, double* pn) {
+  Call c("corunp_explicit_");
+  const double* ddp = c.in(dp, d.n3, "dp");
+  const T* ph = c.in_t<T>(phat, d.n3, "phat");
+  T *u = c.io_t<T>(un, d.n3, "un"), *v = c.io_t<T>(vn, d.n3, "vn"), *w = c.io_t<T>(wn, d.n3, "wn"),
+    *p = c.io_t<T>(pn, d.n3, "pn");
+  correct<T>(d, jen, rho, dxinv, dyinv, dzinv, dt, ddp, ph, u, v, w, p);
+  c.finish();
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -317,13 +524,9 @@ void caldiv_(const int* ipara, const int* ndiv, const int* ni, const int* nj, co
     std::abort();
   }
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("caldiv_");
-  const double *u = c.in(uk, d.n3), *v = c.in(vk, d.n3), *w = c.in(wk, d.n3);
-  double* dv = c.io(div, d.n3);
-  double* s = dev_scalars(3);
-  divergence(d, *dxinv, *dyinv, *dzinv, 1.0 / *dt, u, v, w, dv, s);
-  const double sum = read_scalars(s, 1)[0];
-  c.finish();
+  double sum = 0.0;
+  if (flow_fp32()) caldiv_t<float>(d, *dxinv, *dyinv, *dzinv, *dt, uk, vk, wk, div, &sum);
+  else caldiv_t<double>(d, *dxinv, *dyinv, *dzinv, *dt, uk, vk, wk, div, &sum);
   *err_div = std::sqrt(sum / (double)(*ni * *nj * *nk * *ndiv)) + 1.0e-99;
 }
 
@@ -332,15 +535,8 @@ void solp_fft_tdma1_(const int* ni, const int* nj, const int* nk, double* fftdat
                      double* phir_r, double* phir_i) {
   (void)fftdata;
   const Dims d = make_dims(*ni, *nj, *nk);
-  const size_t nm = (size_t)d.ni * d.nj * d.nk;
-  Call c("solp_fft_tdma1_");
-  const double* dv = c.in(div, d.n3);
-  double* pr = c.out(phir_r, nm);
-  double* pi = c.out(phir_i, nm);
-  prof_begin(5);
-  fft_forward(d, dv, pr, pi);
-  prof_end(5);
-  c.finish();
+  if (flow_fp32()) tdma1_t<float>(d, div, phir_r, phir_i);
+  else tdma1_t<double>(d, div, phir_r, phir_i);
 }
 
 // srcs/trans_r2w.f:1-38 on one rank: the relabelled array has the same
@@ -355,9 +551,15 @@ void trans_r2w_(const int* ipara, const int* ID, const int* ndiv, const int* sva
   if ((const void*)r == (const void*)w) return;
   const size_t n = (size_t)svall[0] * svall[1] * svall[2];
   Call c("trans_r2w_");
-  const double* a = c.in(r, n);
-  double* b = c.out(w, n);
-  FEL_CHECK(hipMemcpyAsync(b, a, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
+  if (elem_size(r) == 4) {
+    const float* a = c.in_t<float>(r, n, "r");
+    float* b = c.out_t<float>(w, n, "w");
+    FEL_CHECK(hipMemcpyAsync(b, a, sizeof(float) * n, hipMemcpyDeviceToDevice, stream()));
+  } else {
+    const double* a = c.in(r, n, "r");
+    double* b = c.out(w, n, "w");
+    FEL_CHECK(hipMemcpyAsync(b, a, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
+  }
   c.finish();
 }
 
@@ -393,15 +595,26 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
   }
   double* tab = static_cast<double*>(scratch(6, sizeof(double) * h.size()));
   FEL_CHECK(hipMemcpyAsync(tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, stream()));
+  const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
   Call c("solp_fft_tdma2_");
-  double* pr = c.io(phiw_r, nm);
-  double* pi = c.io(phiw_i, nm);
   prof_begin(6);
-  tdma_modes(ni, nj, nk, tab, tab + ni, tab + ni + nk, (*dxinv * *dxinv) / *rho,
-             (*dzinv * *dzinv) / *rho, pr, pi);
+  if (flow_fp32()) {
+    float* pr = c.io_t<float>(phiw_r, nm, "phiw_r");
+    float* pi = c.io_t<float>(phiw_i, nm, "phiw_i");
+    tdma_modes<float>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
+  } else {
+    double* pr = c.io(phiw_r, nm, "phiw_r");
+    double* pi = c.io(phiw_i, nm, "phiw_i");
+    tdma_modes<double>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
+  }
   prof_end(6);
   c.finish();
   FEL_CHECK(hipStreamSynchronize(stream()));  // h is released on return
+  PoissonRecord& rec = prec();
+  rec.coef = true;
+  rec.dx2r = dx2r;
+  rec.dz2r = dz2r;
+  rec.tab = tab;
 }
 
 // srcs/solp_fft_tdma3.f:4-5
@@ -409,15 +622,8 @@ void solp_fft_tdma3_(const int* ni, const int* nj, const int* nk, double* fftdat
                      const double* phir_r, const double* phir_i, double* dp) {
   (void)fftdata;
   const Dims d = make_dims(*ni, *nj, *nk);
-  const size_t nm = (size_t)d.ni * d.nj * d.nk;
-  Call c("solp_fft_tdma3_");
-  const double* pr = c.in(phir_r, nm);
-  const double* pi = c.in(phir_i, nm);
-  double* ddp = c.io(dp, d.n3);
-  prof_begin(7);
-  fft_inverse(d, pr, pi, ddp);
-  prof_end(7);
-  c.finish();
+  if (flow_fp32()) tdma3_t<float>(d, phir_r, phir_i, dp);
+  else tdma3_t<double>(d, phir_r, phir_i, dp);
 }
 
 // srcs/solp_fft_tdma4.f:4-5: residual monitor every imon_t steps.
@@ -429,8 +635,8 @@ void solp_fft_tdma4_(const int* ipara, const int* ID, const int* ndiv, const int
   if (!(*imon_t > 0 && (*nstep % *imon_t) == 0)) return;
   const Dims d = make_dims(*ni, *nj, *nk);
   Call c("solp_fft_tdma4_");
-  const double* dv = c.in(div, d.n3);
-  const double* dd = c.in(dp, d.n3);
+  const double* dv = c.in(div, d.n3, "div");
+  const double* dd = c.in(dp, d.n3, "dp");
   double* s = dev_scalars(4);
   poisson_residual_sums(d, *rho, *dxinv, *dyinv, *dzinv, dv, dd, s);
   const double* r = read_scalars(s, 2);
@@ -446,8 +652,8 @@ double felgpu_poisson_residual_(const int* ni, const int* nj, const int* nk, con
                                 const double* div, const double* dp) {
   const Dims d = make_dims(*ni, *nj, *nk);
   Call c("felgpu_poisson_residual_");
-  const double* dv = c.in(div, d.n3);
-  const double* dd = c.in(dp, d.n3);
+  const double* dv = c.in(div, d.n3, "div");
+  const double* dd = c.in(dp, d.n3, "dp");
   double* s = dev_scalars(4);
   poisson_residual_sums(d, *rho, *dxinv, *dyinv, *dzinv, dv, dd, s);
   const double* r = read_scalars(s, 2);
@@ -461,9 +667,8 @@ void bnd_avzero_(const int* ipara, const int* ndiv, const int* ni, const int* nj
                  double* q) {
   (void)ipara;
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("bnd_avzero_");
-  avzero(d, c.io(q, d.n3), *ndiv);
-  c.finish();
+  if (elem_size(q) == 4) avzero_t<float>(d, q, *ndiv);
+  else avzero_t<double>(d, q, *ndiv);
 }
 
 // srcs/cal_av.f:1
@@ -471,14 +676,7 @@ void cal_av_(const int* ipara, const int* ndiv, const int* ni, const int* nj, co
              const double* q, double* av) {
   (void)ipara;
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_av_");
-  const double* dq = c.in(q, d.n3);
-  double* s = dev_scalars(5);
-  interior_reduce<1>(
-      d, [=] __device__(int i, int j, int k, double* o) { o[0] = dq[d.ix(i, j, k)]; },
-      [=] __device__(const double* t, double* out) { out[0] = t[0]; }, s);
-  const double sum = read_scalars(s, 1)[0];
-  c.finish();
+  const double sum = elem_size(q) == 4 ? cal_av_t<float>(d, q) : cal_av_t<double>(d, q);
   *av = sum / (double)(*ni * *nj * *nk * *ndiv);
 }
 
@@ -488,12 +686,9 @@ void corunp_explicit_(const int* nID, const int* ni, const int* nj, const int* n
                       const double* dp, const double* phat, double* un, double* vn, double* wn,
                       double* pn) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("corunp_explicit_");
-  const double *ddp = c.in(dp, d.n3), *ph = c.in(phat, d.n3);
-  double *u = c.io(un, d.n3), *v = c.io(vn, d.n3), *w = c.io(wn, d.n3), *p = c.io(pn, d.n3);
   const int jen = (nID[kYPlus] < 0) ? *nj - 1 : *nj;
-  correct(d, jen, *rho, *dxinv, *dyinv, *dzinv, *dt, ddp, ph, u, v, w, p);
-  c.finish();
+  if (flow_fp32()) corunp_t<float>(d, jen, *rho, *dxinv, *dyinv, *dzinv, *dt, dp, phat, un, vn, wn, pn);
+  else corunp_t<double>(d, jen, *rho, *dxinv, *dyinv, *dzinv, *dt, dp, phat, un, vn, wn, pn);
 }
 
 // srcs/cal_coef_prs.f:5-57 (set-up, host arrays declared (-2:nj+3))

@@ -22,12 +22,15 @@
 
 namespace fel {
 
+// T: element type of the velocities, densities, viscosity and sources
+// (double, or float in the FP32 mode).
+template <class T>
 struct SorArgs {
-  const double *rho, *rhon, *src[3], *rmu;
-  const double* cf[45];
-  double *un, *vn, *wn;
-  double dtinv;
-  VisScales sc;
+  const T *rho, *rhon, *src[3], *rmu;
+  const T* cf[45];
+  T *un, *vn, *wn;
+  T dtinv;
+  VisScales<T> sc;
 };
 
 // Colour-pair base parities (srcs/solu_sor4.f:94-120); the partner is the
@@ -38,8 +41,8 @@ static const int kIs[4] = {1, 1, 1, 2}, kJs[4] = {1, 1, 2, 1}, kKs[4] = {1, 2, 1
 // number of partial sums (and the final reduction) small.
 constexpr int kKB = 1;
 
-template <int COMP, bool ONFLY>
-__global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is, int js, int ks,
+template <int COMP, bool ONFLY, class T>
+__global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
                                                        int jend, int gx, int gy, int ntiles,
                                                        double* part) {
   const int L = xcd_tile((int)blockIdx.x, ntiles);
@@ -54,7 +57,7 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is
   if (i <= d.ni && j <= jend) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
-    double cf[15];
+    T cf[15];
     if (ONFLY) {
       if (COMP == 0) coef_u(d, a.sc, a.rmu, c, cf);
       if (COMP == 1) coef_v(d, a.sc, a.rmu, c, cf);
@@ -63,40 +66,41 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs a, int is
 #pragma unroll
       for (int m = 0; m < 15; m++) cf[m] = a.cf[15 * COMP + m][c];
     }
-    const double *un = a.un, *vn = a.vn, *wn = a.wn;
+    const T *un = a.un, *vn = a.vn, *wn = a.wn;
+    const T q = T(0.25), h = T(0.5);
     if (COMP == 0) {
-      const double r00 = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * 0.25;
-      const double visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
+      const T r00 = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q;
+      const T visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
                            cf[1] * un[c + X] + cf[3] * un[c + Y] + cf[5] * un[c + Z] +
                            cf[7] * vn[c - Y] + cf[8] * vn[c + X - Y] + cf[9] * vn[c] +
                            cf[10] * vn[c + X] + cf[11] * wn[c - Z] + cf[12] * wn[c + X - Z] +
                            cf[13] * wn[c] + cf[14] * wn[c + X]) /
                           r00;
-      const double du = (visn * 0.5 + a.src[0][c]) / (a.dtinv + cf[6] / r00 * 0.5) - un[c];
+      const T du = (visn * h + a.src[0][c]) / (a.dtinv + cf[6] / r00 * h) - un[c];
       a.un[c] = un[c] + du;
-      e += du * du;
+      e += (double)du * (double)du;
     } else if (COMP == 1) {
-      const double r00 = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * 0.25;
-      const double visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
+      const T r00 = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q;
+      const T visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
                            cf[1] * vn[c + X] + cf[3] * vn[c + Y] + cf[5] * vn[c + Z] +
                            cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Y] +
                            cf[10] * un[c + Y] + cf[11] * wn[c - Z] + cf[12] * wn[c + Y - Z] +
                            cf[13] * wn[c] + cf[14] * wn[c + Y]) /
                           r00;
-      const double dv = (visn * 0.5 + a.src[1][c]) / (a.dtinv + cf[6] / r00 * 0.5) - vn[c];
+      const T dv = (visn * h + a.src[1][c]) / (a.dtinv + cf[6] / r00 * h) - vn[c];
       a.vn[c] = vn[c] + dv;
-      e += dv * dv;
+      e += (double)dv * (double)dv;
     } else {
-      const double r00 = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * 0.25;
-      const double visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
+      const T r00 = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q;
+      const T visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
                            cf[1] * wn[c + X] + cf[3] * wn[c + Y] + cf[5] * wn[c + Z] +
                            cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Z] +
                            cf[10] * un[c + Z] + cf[11] * vn[c - Y] + cf[13] * vn[c] +
                            cf[12] * vn[c - Y + Z] + cf[14] * vn[c + Z]) /
                           r00;
-      const double dw = (visn * 0.5 + a.src[2][c]) / (a.dtinv + cf[6] / r00 * 0.5) - wn[c];
+      const T dw = (visn * h + a.src[2][c]) / (a.dtinv + cf[6] / r00 * h) - wn[c];
       a.wn[c] = wn[c] + dw;
-      e += dw * dw;
+      e += (double)dw * (double)dw;
     }
   }
   }
@@ -153,7 +157,8 @@ struct SorResult {
   double err, err0;
 };
 
-SorResult sor4(const Dims d, SorArgs a, bool onfly, bool lo, bool hi, double uwall, double dy,
+template <class T>
+SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double uwall, double dy,
                const double* l1, const double* l2, const double* phi, int ndiv) {
   const int jen = hi ? d.nj - 1 : d.nj;
   const int gx = ((d.ni + 1) / 2 + 63) / 64;  // cells of one parity per row
@@ -180,18 +185,22 @@ SorResult sor4(const Dims d, SorArgs a, bool onfly, bool lo, bool hi, double uwa
         double* pp = part + ps.off[slot];
         prof_begin(3);
         if (onfly) {
-          if (comp == 0) sor_pair_kernel<0, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 1) sor_pair_kernel<1, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 2) sor_pair_kernel<2, true><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 0) sor_pair_kernel<0, true, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 1) sor_pair_kernel<1, true, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 2) sor_pair_kernel<2, true, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
         } else {
-          if (comp == 0) sor_pair_kernel<0, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 1) sor_pair_kernel<1, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 2) sor_pair_kernel<2, false><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 0) sor_pair_kernel<0, false, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 1) sor_pair_kernel<1, false, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
+          if (comp == 2) sor_pair_kernel<2, false, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
         }
         FEL_CHECK(hipGetLastError());
         prof_end(3);
         prof_begin(4);
-        bc_fused(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp);
+        // After the first pair of a component's sweep only that component
+        // changes, and bndu's ghosts of the other two are functions of their
+        // unchanged interiors: rewriting them would store the same values.
+        bc_fused<T>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp,
+                    p == 0 ? 7 : (1 << comp));
         prof_end(4);
       }
     }
@@ -212,6 +221,45 @@ SorResult sor4(const Dims d, SorArgs a, bool onfly, bool lo, bool hi, double uwa
 
 static int g_last_lp = -1;
 static double g_last_ratio = 0.0;
+
+template <class T>
+SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* rhon,
+                      double* const* h, const double* src_u, const double* src_v,
+                      const double* src_w, double* un, double* vn, double* wn, double uwall,
+                      double dy, const double* l1, const double* l2, const double* phi, int jm,
+                      int jp, int ndiv) {
+  // on-the-fly coefficients when they are a pending function of the viscosity
+  DeferredCoef& dc = deferred_coef();
+  bool onfly = dc.active && dc.d.ni == d.ni && dc.d.nj == d.nj && dc.d.nk == d.nk &&
+               dc.esize == (int)sizeof(T);
+  for (int m = 0; m < 45 && onfly; m++) onfly = dc.out_host[m] == (const void*)h[m];
+  Call c("solu_sor4_");
+  SorArgs<T> a;
+  a.rho = c.in_t<T>(rho, d.n3, "rho");
+  a.rhon = c.in_t<T>(rhon, d.n3, "rhon");
+  if (onfly) {
+    a.rmu = static_cast<const T*>(dc.src_dev);
+    a.sc = vis_scales<T>(dc.dxinv, dc.dyinv, dc.dzinv);
+    for (int m = 0; m < 45; m++) a.cf[m] = nullptr;
+  } else {
+    a.rmu = nullptr;
+    a.sc = vis_scales<T>(0.0, 0.0, 0.0);
+    for (int m = 0; m < 45; m++) a.cf[m] = c.in_t<T>(h[m], d.n3, "coefficient");
+  }
+  a.src[0] = c.in_t<T>(src_u, d.n3, "src_u");
+  a.src[1] = c.in_t<T>(src_v, d.n3, "src_v");
+  a.src[2] = c.in_t<T>(src_w, d.n3, "src_w");
+  a.un = c.io_t<T>(un, d.n3, "un");
+  a.vn = c.io_t<T>(vn, d.n3, "vn");
+  a.wn = c.io_t<T>(wn, d.n3, "wn");
+  a.dtinv = T(1.0 / dt);
+  const double* dl1 = c.in(l1, d.n3, "l1");
+  const double* dl2 = c.in(l2, d.n3, "l2");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  const SorResult r = sor4<T>(d, a, onfly, jm < 0, jp < 0, uwall, dy, dl1, dl2, dphi, ndiv);
+  c.finish();
+  return r;
+}
 
 }  // namespace fel
 
@@ -250,35 +298,11 @@ void solu_sor4_(const int* ipara, const int* ID, const int* nID, const int* ndiv
   double* h[45] = {c0,  c1,  c2,  c3,  c4,  c5,  c6,  c7,  c8,  c9,  c10, c11, c12, c13, c14,
                    c15, c16, c17, c18, c19, c20, c21, c22, c23, c24, c25, c26, c27, c28, c29,
                    c30, c31, c32, c33, c34, c35, c36, c37, c38, c39, c40, c41, c42, c43, c44};
-  // on-the-fly coefficients when they are a pending function of the viscosity
-  DeferredCoef& dc = deferred_coef();
-  bool onfly = dc.active && dc.d.ni == d.ni && dc.d.nj == d.nj && dc.d.nk == d.nk;
-  for (int m = 0; m < 45 && onfly; m++) onfly = dc.out_host[m] == (const void*)h[m];
-  Call c("solu_sor4_");
-  SorArgs a;
-  a.rho = c.in(rho, d.n3);
-  a.rhon = c.in(rhon, d.n3);
-  if (onfly) {
-    a.rmu = dc.src_dev;
-    a.sc = vis_scales(dc.dxinv, dc.dyinv, dc.dzinv);
-    for (int m = 0; m < 45; m++) a.cf[m] = nullptr;
-  } else {
-    a.rmu = nullptr;
-    a.sc = vis_scales(0.0, 0.0, 0.0);
-    for (int m = 0; m < 45; m++) a.cf[m] = c.in(h[m], d.n3);
-  }
-  a.src[0] = c.in(src_u, d.n3);
-  a.src[1] = c.in(src_v, d.n3);
-  a.src[2] = c.in(src_w, d.n3);
-  a.un = c.io(un, d.n3);
-  a.vn = c.io(vn, d.n3);
-  a.wn = c.io(wn, d.n3);
-  a.dtinv = 1.0 / *dt;
-  const double* dl1 = c.in(l1, d.n3);
-  const double* dl2 = c.in(l2, d.n3);
-  const double* dphi = c.in(phi, d.n3);
-  const SorResult r = sor4(d, a, onfly, nID[kYMinus] < 0, nID[kYPlus] < 0, *uwall, *dy, dl1, dl2, dphi, *ndiv);
-  c.finish();
+  const int jm = nID[kYMinus], jp = nID[kYPlus];
+  const SorResult r = flow_fp32() ? solu_sor4_t<float>(d, *dt, rho, rhon, h, src_u, src_v, src_w, un,
+                                                      vn, wn, *uwall, *dy, l1, l2, phi, jm, jp, *ndiv)
+                                  : solu_sor4_t<double>(d, *dt, rho, rhon, h, src_u, src_v, src_w, un,
+                                                       vn, wn, *uwall, *dy, l1, l2, phi, jm, jp, *ndiv);
   g_last_lp = r.lp;
   g_last_ratio = r.err / r.err0;
   if (*imon_t > 0 && (*nstep % *imon_t) == 0 && *ID == 0)

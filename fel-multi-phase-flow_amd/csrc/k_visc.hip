@@ -13,25 +13,33 @@ namespace fel {
 
 __device__ __forceinline__ bool inb(int i, int a, int b) { return i >= a && i <= b; }
 
-void arith_mean(const Dims d, const double* phik, double ql, double qg, double* qk) {
+// T is the flow element type (double, or float in the FP32 mode); the volume
+// fraction is always FP64.  With T = double every expression is the
+// reference's FP64 one.
+
+template <class T>
+void arith_mean(const Dims d, const double* phik, double ql, double qg, T* qk) {
   launch_flat(d.n3, [=] __device__(int m) {
     const double pl = 1.0 - phik[m], pg = phik[m];
-    qk[m] = ql * pl + qg * pg;
+    qk[m] = T(ql * pl + qg * pg);
   });
 }
 
-void calphat(const Dims d, const double* p, const double* po, double* phat) {
+template <class T>
+void calphat(const Dims d, const T* p, const T* po, T* phat) {
   launch_box(Box{-1, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    phat[c] = p[c] * 2.0 - po[c];
+    phat[c] = p[c] * T(2.0) - po[c];
   });
 }
 
-void grad_faces(const Dims d, double dxinv, double dyinv, double dzinv, const double* q, double* qx,
-                double* qy, double* qz) {
+template <class T>
+void grad_faces(const Dims d, double dxinv_, double dyinv_, double dzinv_, const T* q, T* qx, T* qy,
+                T* qz) {
+  const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_);
   launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    const double q0 = q[c];
+    const T q0 = q[c];
     qx[c] = (-q0 + q[c + 1]) * dxinv;
     qy[c] = (-q0 + q[c + d.sx]) * dyinv;
     qz[c] = (-q0 + q[c + d.sx * d.sy]) * dzinv;
@@ -39,56 +47,61 @@ void grad_faces(const Dims d, double dxinv, double dyinv, double dzinv, const do
 }
 
 // srcs/calsij.f:28-139
-void calsij(const Dims d, double dxinv, double dyinv, double dzinv, const double* u, const double* v,
-            const double* w, double* s) {
+template <class T>
+void calsij(const Dims d, double dxinv_, double dyinv_, double dzinv_, const T* u, const T* v,
+            const T* w, T* s) {
   const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
+  const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_);
   launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
     if (i >= -1) s[c] = (-u[c - X] + u[c]) * dxinv;
     if (j >= -1) s[n3 + c] = (-v[c - Y] + v[c]) * dyinv;
     if (i >= -1 && j >= -1 && k >= -1) s[2 * n3 + c] = (-w[c - Z] + w[c]) * dzinv;
     if (i <= ni + 2 && j <= nj + 2)
-      s[3 * n3 + c] = (+(-u[c] + u[c + Y]) * dyinv + (-v[c] + v[c + X]) * dxinv) * 0.5;
+      s[3 * n3 + c] = (+(-u[c] + u[c + Y]) * dyinv + (-v[c] + v[c + X]) * dxinv) * T(0.5);
     if (i <= ni + 2 && k <= nk + 2)
-      s[4 * n3 + c] = (+(-u[c] + u[c + Z]) * dzinv + (-w[c] + w[c + X]) * dxinv) * 0.5;
+      s[4 * n3 + c] = (+(-u[c] + u[c + Z]) * dzinv + (-w[c] + w[c + X]) * dxinv) * T(0.5);
     if (j <= nj + 2 && k <= nk + 2)
-      s[5 * n3 + c] = ((-v[c] + v[c + Z]) * dzinv + (-w[c] + w[c + Y]) * dyinv) * 0.5;
+      s[5 * n3 + c] = ((-v[c] + v[c + Z]) * dzinv + (-w[c] + w[c + Y]) * dyinv) * T(0.5);
   });
 }
 
 // srcs/cal_arith_tau.f:21-117
-void arith_tau(const Dims d, const double* s, const double* rmu, double* tau) {
+template <class T>
+void arith_tau(const Dims d, const T* s, const T* rmu, T* tau) {
   const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
   launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    const double r = rmu[c];
-    tau[c] = s[c] * r * 2.0;
-    tau[n3 + c] = s[n3 + c] * r * 2.0;
-    tau[2 * n3 + c] = s[2 * n3 + c] * r * 2.0;
+    const T r = rmu[c], q = T(0.25);
+    tau[c] = s[c] * r * T(2.0);
+    tau[n3 + c] = s[n3 + c] * r * T(2.0);
+    tau[2 * n3 + c] = s[2 * n3 + c] * r * T(2.0);
     if (i <= ni + 2 && j <= nj + 2) {
-      const double m = +0.25 * r + 0.25 * rmu[c + X] + 0.25 * rmu[c + Y] + 0.25 * rmu[c + X + Y];
-      tau[3 * n3 + c] = s[3 * n3 + c] * m * 2.0;
+      const T m = +q * r + q * rmu[c + X] + q * rmu[c + Y] + q * rmu[c + X + Y];
+      tau[3 * n3 + c] = s[3 * n3 + c] * m * T(2.0);
     }
     if (i <= ni + 2 && k <= nk + 2) {
-      const double m = +0.25 * r + 0.25 * rmu[c + X] + 0.25 * rmu[c + Z] + 0.25 * rmu[c + X + Z];
-      tau[4 * n3 + c] = s[4 * n3 + c] * m * 2.0;
+      const T m = +q * r + q * rmu[c + X] + q * rmu[c + Z] + q * rmu[c + X + Z];
+      tau[4 * n3 + c] = s[4 * n3 + c] * m * T(2.0);
     }
     if (j <= nj + 2 && k <= nk + 2) {
-      const double m = +0.25 * r + 0.25 * rmu[c + Y] + 0.25 * rmu[c + Z] + 0.25 * rmu[c + Y + Z];
-      tau[5 * n3 + c] = s[5 * n3 + c] * m * 2.0;
+      const T m = +q * r + q * rmu[c + Y] + q * rmu[c + Z] + q * rmu[c + Y + Z];
+      tau[5 * n3 + c] = s[5 * n3 + c] * m * T(2.0);
     }
   });
 }
 
 // srcs/cal_div_tensor.f:20-86 (clearing folded into the same pass)
-void div_tensor(const Dims d, double dxinv, double dyinv, double dzinv, const double* tau, double* vu,
-                double* vv, double* vw) {
+template <class T>
+void div_tensor(const Dims d, double dxinv_, double dyinv_, double dzinv_, const T* tau, T* vu,
+                T* vv, T* vw) {
   const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
-  const double *t1 = tau, *t2 = tau + n3, *t3 = tau + 2 * n3, *t4 = tau + 3 * n3, *t5 = tau + 4 * n3,
-               *t6 = tau + 5 * n3;
+  const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_);
+  const T *t1 = tau, *t2 = tau + n3, *t3 = tau + 2 * n3, *t4 = tau + 3 * n3, *t5 = tau + 4 * n3,
+          *t6 = tau + 5 * n3;
   launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    double a = 0.0, b = 0.0, e = 0.0;
+    T a = T(0.0), b = T(0.0), e = T(0.0);
     if (inb(k, -1, nk + 3) && inb(j, -1, nj + 3) && inb(i, -2, ni + 2))
       a = +(-t1[c] + t1[c + X]) * dxinv + (-t4[c - Y] + t4[c]) * dyinv + (-t5[c - Z] + t5[c]) * dzinv;
     if (inb(k, -1, nk + 3) && inb(j, -2, nj + 2) && inb(i, -1, ni + 3))
@@ -102,57 +115,223 @@ void div_tensor(const Dims d, double dxinv, double dyinv, double dzinv, const do
 }
 
 // srcs/cal_srcu.f:59-136
-void srcu(const Dims d, int jen, double dtinv, double grvb, double grvp, SrcArgs a) {
+template <class T>
+void srcu(const Dims d, int jen, double dtinv_, double grvb_, double grvp_, SrcArgs<T> a) {
+  const T dtinv = T(dtinv_), grvb = T(grvb_), grvp = T(grvp_);
+  const T q = T(0.25), h = T(0.5), t15 = T(1.5);
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
     {
-      const double r = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * 0.25;
-      a.src_u[c] = +a.u[c] * dtinv - a.adv_u[c] * 1.5 + a.adv_uo[c] * 0.5 - a.prs_u[c] / r +
-                   a.vis_u[c] / r * 0.5 + a.fst_u[c] * 0.5 + a.fst_un[c] * 0.5 + grvb;
+      const T r = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q;
+      a.src_u[c] = +a.u[c] * dtinv - a.adv_u[c] * t15 + a.adv_uo[c] * h - a.prs_u[c] / r +
+                   a.vis_u[c] / r * h + a.fst_u[c] * h + a.fst_un[c] * h + grvb;
     }
     if (j <= jen) {
-      const double r = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * 0.25;
-      a.src_v[c] = +a.v[c] * dtinv - a.adv_v[c] * 1.5 + a.adv_vo[c] * 0.5 - a.prs_v[c] / r +
-                   a.vis_v[c] / r * 0.5 + a.fst_v[c] * 0.5 + a.fst_vn[c] * 0.5 + grvp;
+      const T r = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q;
+      a.src_v[c] = +a.v[c] * dtinv - a.adv_v[c] * t15 + a.adv_vo[c] * h - a.prs_v[c] / r +
+                   a.vis_v[c] / r * h + a.fst_v[c] * h + a.fst_vn[c] * h + grvp;
     }
     {
-      const double r = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * 0.25;
-      a.src_w[c] = +a.w[c] * dtinv - a.adv_w[c] * 1.5 + a.adv_wo[c] * 0.5 - a.prs_w[c] / r +
-                   a.vis_w[c] / r * 0.5 + a.fst_w[c] * 0.5 + a.fst_wn[c] * 0.5;
+      const T r = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q;
+      a.src_w[c] = +a.w[c] * dtinv - a.adv_w[c] * t15 + a.adv_wo[c] * h - a.prs_w[c] / r +
+                   a.vis_w[c] / r * h + a.fst_w[c] * h + a.fst_wn[c] * h;
     }
   });
 }
 
 // srcs/cal_arith_coef_vis.f:99-407: zero on the padded image, the 45
 // coefficients on [-1,n+2]^3.
-void coef_vis(const Dims d, double dxinv, double dyinv, double dzinv, const double* rmu, Coef45 a) {
+template <class T>
+void coef_vis(const Dims d, double dxinv, double dyinv, double dzinv, const T* rmu, Coef45<T> a) {
   const int ni = d.ni, nj = d.nj, nk = d.nk;
-  const VisScales sc = vis_scales(dxinv, dyinv, dzinv);
+  const VisScales<T> sc = vis_scales<T>(dxinv, dyinv, dzinv);
   launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    double v[45];
+    T v[45];
     if (inb(i, -1, ni + 2) && inb(j, -1, nj + 2) && inb(k, -1, nk + 2)) {
       coef_u(d, sc, rmu, c, v);
       coef_v(d, sc, rmu, c, v + 15);
       coef_w(d, sc, rmu, c, v + 30);
     } else {
 #pragma unroll
-      for (int m = 0; m < 45; m++) v[m] = 0.0;
+      for (int m = 0; m < 45; m++) v[m] = T(0.0);
     }
 #pragma unroll
     for (int m = 0; m < 45; m++) a.p[m][c] = v[m];
   });
 }
 
-void coef_vis(const Dims d, double dxinv, double dyinv, double dzinv, const double* rmu,
-              double* const* out45) {
-  Coef45 a;
-  for (int m = 0; m < 45; m++) a.p[m] = out45[m];
-  coef_vis(d, dxinv, dyinv, dzinv, rmu, a);
+void coef_vis_any(const Dims d, double dxinv, double dyinv, double dzinv, const void* rmu,
+                  void* const* out45, int esize) {
+  if (esize == 4) {
+    Coef45<float> a;
+    for (int m = 0; m < 45; m++) a.p[m] = static_cast<float*>(out45[m]);
+    coef_vis<float>(d, dxinv, dyinv, dzinv, static_cast<const float*>(rmu), a);
+  } else {
+    Coef45<double> a;
+    for (int m = 0; m < 45; m++) a.p[m] = static_cast<double*>(out45[m]);
+    coef_vis<double>(d, dxinv, dyinv, dzinv, static_cast<const double*>(rmu), a);
+  }
 }
 
-void copy(const Dims d, const double* q1, double* q2) {
-  FEL_CHECK(hipMemcpyAsync(q2, q1, sizeof(double) * (size_t)d.n3, hipMemcpyDeviceToDevice, stream()));
+// ---------------------------------------------------------------------------
+// Drop-in bodies, instantiated for the flow element type T.
+// ---------------------------------------------------------------------------
+
+template <class T>
+void cal_arith_mean_t(const Dims d, const double* phik, double ql, double qg, double* qk) {
+  Call c("cal_arith_mean_");
+  const double* p = c.in(phik, d.n3, "phik");
+  T* q = c.out_t<T>(qk, d.n3, "qk");
+  arith_mean<T>(d, p, ql, qg, q);
+  c.finish();
+}
+
+template <class T>
+void calphat_t(const Dims d, const double* p, const double* po, double* phat) {
+  Call c("calphat_");
+  const T* dp = c.in_t<T>(p, d.n3, "p");
+  const T* dpo = c.in_t<T>(po, d.n3, "po");
+  T* dph = c.io_t<T>(phat, d.n3, "phat");
+  calphat<T>(d, dp, dpo, dph);
+  c.finish();
+}
+
+template <class T>
+void cal_grad_prs_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* q,
+                    double* qx_u, double* qy_v, double* qz_w) {
+  Call c("cal_grad_prs_");
+  const T* dq = c.in_t<T>(q, d.n3, "q");
+  T* x = c.io_t<T>(qx_u, d.n3, "qx_u");
+  T* y = c.io_t<T>(qy_v, d.n3, "qy_v");
+  T* z = c.io_t<T>(qz_w, d.n3, "qz_w");
+  grad_faces<T>(d, dxinv, dyinv, dzinv, dq, x, y, z);
+  c.finish();
+}
+
+template <class T>
+void calsij_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* uk,
+              const double* vk, const double* wk, double* sk) {
+  Call c("calsij_");
+  const T* u = c.in_t<T>(uk, d.n3, "uk");
+  const T* v = c.in_t<T>(vk, d.n3, "vk");
+  const T* w = c.in_t<T>(wk, d.n3, "wk");
+  T* s = c.io_t<T>(sk, 6 * (size_t)d.n3, "sk");
+  calsij<T>(d, dxinv, dyinv, dzinv, u, v, w, s);
+  c.finish();
+}
+
+template <class T>
+void cal_arith_tau_t(const Dims d, const double* s, const double* rmu, double* tau) {
+  Call c("cal_arith_tau_");
+  const T* ds = c.in_t<T>(s, 6 * (size_t)d.n3, "s");
+  const T* dr = c.in_t<T>(rmu, d.n3, "rmu");
+  T* dt = c.io_t<T>(tau, 6 * (size_t)d.n3, "tau");
+  arith_tau<T>(d, ds, dr, dt);
+  c.finish();
+}
+
+template <class T>
+void cal_div_tensor_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* tauk,
+                      double* uk, double* vk, double* wk) {
+  Call c("cal_div_tensor_");
+  const T* t = c.in_t<T>(tauk, 6 * (size_t)d.n3, "tauk");
+  T* u = c.out_t<T>(uk, d.n3, "uk");
+  T* v = c.out_t<T>(vk, d.n3, "vk");
+  T* w = c.out_t<T>(wk, d.n3, "wk");
+  div_tensor<T>(d, dxinv, dyinv, dzinv, t, u, v, w);
+  c.finish();
+}
+
+template <class T>
+void cal_srcu_t(const Dims d, int jen, double dt, double grvb, double grvp, const double* const* in,
+                double* const* out) {
+  Call c("cal_srcu_");
+  const int n = d.n3;
+  const T* p[23];
+  for (int m = 0; m < 23; m++) p[m] = c.in_t<T>(in[m], n, "cal_srcu_ input");
+  SrcArgs<T> a;
+  a.u = p[0]; a.v = p[1]; a.w = p[2]; a.rho = p[3]; a.rhon = p[4];
+  a.adv_u = p[5]; a.adv_v = p[6]; a.adv_w = p[7]; a.adv_uo = p[8]; a.adv_vo = p[9]; a.adv_wo = p[10];
+  a.prs_u = p[11]; a.prs_v = p[12]; a.prs_w = p[13]; a.vis_u = p[14]; a.vis_v = p[15]; a.vis_w = p[16];
+  a.fst_u = p[17]; a.fst_v = p[18]; a.fst_w = p[19]; a.fst_un = p[20]; a.fst_vn = p[21];
+  a.fst_wn = p[22];
+  a.src_u = c.io_t<T>(out[0], n, "src_u");
+  a.src_v = c.io_t<T>(out[1], n, "src_v");
+  a.src_w = c.io_t<T>(out[2], n, "src_w");
+  srcu<T>(d, jen, 1.0 / dt, grvb, grvp, a);
+  c.finish();
+}
+
+template <class T>
+void cal_arith_coef_vis_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* rmuk,
+                          double* const* h) {
+  Call c("cal_arith_coef_vis_");
+  // a pending record for other output arrays is materialised before it is replaced
+  DeferredCoef& dc = deferred_coef();
+  if (dc.active) {
+    // the same output arrays are about to be redefined: drop the pending
+    // record; other outputs still owe their values
+    bool same = true;
+    for (int m = 0; m < 45 && same; m++) same = dc.out_host[m] == (const void*)h[m];
+    if (!same) deferred_coef_flush();
+    dc.active = false;
+  }
+  bool resident = is_registered(rmuk, d.n3);
+  for (int m = 0; m < 45 && resident; m++) resident = is_registered(h[m], d.n3);
+  const T* r = c.in_t<T>(rmuk, d.n3, "rmuk");
+  if (resident && !deferred_coef_disabled()) {
+    dc.d = d;
+    dc.esize = (int)sizeof(T);
+    dc.dxinv = dxinv; dc.dyinv = dyinv; dc.dzinv = dzinv;
+    dc.src_host = rmuk;
+    dc.src_dev = r;
+    for (int m = 0; m < 45; m++) {
+      dc.out_host[m] = h[m];
+      dc.out_dev[m] = c.out_t<T>(h[m], d.n3, "coefficient");
+    }
+    dc.active = true;
+  } else {
+    Coef45<T> a;
+    for (int m = 0; m < 45; m++) a.p[m] = c.out_t<T>(h[m], d.n3, "coefficient");
+    coef_vis<T>(d, dxinv, dyinv, dzinv, r, a);
+  }
+  c.finish();
+}
+
+template <class T>
+void cpy_t(const Dims d, const double* q1, double* q2) {
+  Call c("cpy_");
+  const T* a = c.in_t<T>(q1, d.n3, "q1");
+  T* b = c.out_t<T>(q2, d.n3, "q2");
+  FEL_CHECK(hipMemcpyAsync(b, a, sizeof(T) * (size_t)d.n3, hipMemcpyDeviceToDevice, stream()));
+  c.finish();
+  copy_content_version(q1, q2, d.n3);
+}
+
+template <class T>
+void init_q_t(const Dims d, double* q1, double* q2, double* q3) {
+  Call c("init_q_");
+  T* p[3] = {c.out_t<T>(q1, d.n3, "q1"), c.out_t<T>(q2, d.n3, "q2"), c.out_t<T>(q3, d.n3, "q3")};
+  for (int m = 0; m < 3; m++)
+    FEL_CHECK(hipMemsetAsync(p[m], 0, sizeof(T) * (size_t)d.n3, stream()));
+  c.finish();
+}
+
+template <class T>
+void sum_fst_t(const Dims d, const double* fst_u, const double* fst_v, const double* fst_w,
+               double* sum_u, double* sum_v, double* sum_w) {
+  Call c("sum_fst_");
+  const T *fu = c.in_t<T>(fst_u, d.n3, "fst_u"), *fv = c.in_t<T>(fst_v, d.n3, "fst_v"),
+          *fw = c.in_t<T>(fst_w, d.n3, "fst_w");
+  T *su = c.io_t<T>(sum_u, d.n3, "sum_u"), *sv = c.io_t<T>(sum_v, d.n3, "sum_v"),
+    *sw = c.io_t<T>(sum_w, d.n3, "sum_w");
+  launch_flat(d.n3, [=] __device__(int m) {
+    su[m] = su[m] + fu[m];
+    sv[m] = sv[m] + fv[m];
+    sw[m] = sw[m] + fw[m];
+  });
+  c.finish();
 }
 
 }  // namespace fel
@@ -165,23 +344,16 @@ extern "C" {
 void cal_arith_mean_(const int* ni, const int* nj, const int* nk, const double* phik,
                      const double* ql, const double* qg, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_arith_mean_");
-  const double* p = c.in(phik, d.n3);
-  double* q = c.out(qk, d.n3);
-  arith_mean(d, p, *ql, *qg, q);
-  c.finish();
+  if (flow_fp32()) cal_arith_mean_t<float>(d, phik, *ql, *qg, qk);
+  else cal_arith_mean_t<double>(d, phik, *ql, *qg, qk);
 }
 
 // srcs/calphat.f:1
 void calphat_(const int* ni, const int* nj, const int* nk, const double* p, const double* po,
               double* phat) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("calphat_");
-  const double* dp = c.in(p, d.n3);
-  const double* dpo = c.in(po, d.n3);
-  double* dph = c.io(phat, d.n3);
-  calphat(d, dp, dpo, dph);
-  c.finish();
+  if (flow_fp32()) calphat_t<float>(d, p, po, phat);
+  else calphat_t<double>(d, p, po, phat);
 }
 
 // srcs/cal_grad_prs.f:1-3
@@ -189,38 +361,24 @@ void cal_grad_prs_(const int* ni, const int* nj, const int* nk, const double* dx
                    const double* dyinv, const double* dzinv, const double* q, double* qx_u,
                    double* qy_v, double* qz_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_grad_prs_");
-  const double* dq = c.in(q, d.n3);
-  double* x = c.io(qx_u, d.n3);
-  double* y = c.io(qy_v, d.n3);
-  double* z = c.io(qz_w, d.n3);
-  grad_faces(d, *dxinv, *dyinv, *dzinv, dq, x, y, z);
-  c.finish();
+  if (flow_fp32()) cal_grad_prs_t<float>(d, *dxinv, *dyinv, *dzinv, q, qx_u, qy_v, qz_w);
+  else cal_grad_prs_t<double>(d, *dxinv, *dyinv, *dzinv, q, qx_u, qy_v, qz_w);
 }
 
 // srcs/calsij.f:5-6
 void calsij_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
              const double* dzinv, const double* uk, const double* vk, const double* wk, double* sk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("calsij_");
-  const double* u = c.in(uk, d.n3);
-  const double* v = c.in(vk, d.n3);
-  const double* w = c.in(wk, d.n3);
-  double* s = c.io(sk, 6 * (size_t)d.n3);
-  calsij(d, *dxinv, *dyinv, *dzinv, u, v, w, s);
-  c.finish();
+  if (flow_fp32()) calsij_t<float>(d, *dxinv, *dyinv, *dzinv, uk, vk, wk, sk);
+  else calsij_t<double>(d, *dxinv, *dyinv, *dzinv, uk, vk, wk, sk);
 }
 
 // srcs/cal_arith_tau.f:7
 void cal_arith_tau_(const int* ni, const int* nj, const int* nk, const double* s, const double* rmu,
                     double* tau) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_arith_tau_");
-  const double* ds = c.in(s, 6 * (size_t)d.n3);
-  const double* dr = c.in(rmu, d.n3);
-  double* dt = c.io(tau, 6 * (size_t)d.n3);
-  arith_tau(d, ds, dr, dt);
-  c.finish();
+  if (flow_fp32()) cal_arith_tau_t<float>(d, s, rmu, tau);
+  else cal_arith_tau_t<double>(d, s, rmu, tau);
 }
 
 // srcs/cal_div_tensor.f:1-2
@@ -228,13 +386,8 @@ void cal_div_tensor_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* dyinv, const double* dzinv, const double* tauk, double* uk,
                      double* vk, double* wk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_div_tensor_");
-  const double* t = c.in(tauk, 6 * (size_t)d.n3);
-  double* u = c.out(uk, d.n3);
-  double* v = c.out(vk, d.n3);
-  double* w = c.out(wk, d.n3);
-  div_tensor(d, *dxinv, *dyinv, *dzinv, t, u, v, w);
-  c.finish();
+  if (flow_fp32()) cal_div_tensor_t<float>(d, *dxinv, *dyinv, *dzinv, tauk, uk, vk, wk);
+  else cal_div_tensor_t<double>(d, *dxinv, *dyinv, *dzinv, tauk, uk, vk, wk);
 }
 
 // srcs/cal_srcu.f:1-10
@@ -249,21 +402,13 @@ void cal_srcu_(const int* nID, const int* ni, const int* nj, const int* nk, cons
                double* src_w) {
   (void)rho_av;
   const Dims d = make_dims(*ni, *nj, *nk);
-  const int n = d.n3;
-  Call c("cal_srcu_");
-  SrcArgs a;
-  a.u = c.in(u, n); a.v = c.in(v, n); a.w = c.in(w, n);
-  a.rho = c.in(rho, n); a.rhon = c.in(rhon, n);
-  a.adv_u = c.in(adv_u, n); a.adv_v = c.in(adv_v, n); a.adv_w = c.in(adv_w, n);
-  a.adv_uo = c.in(adv_uo, n); a.adv_vo = c.in(adv_vo, n); a.adv_wo = c.in(adv_wo, n);
-  a.prs_u = c.in(prs_u, n); a.prs_v = c.in(prs_v, n); a.prs_w = c.in(prs_w, n);
-  a.vis_u = c.in(vis_u, n); a.vis_v = c.in(vis_v, n); a.vis_w = c.in(vis_w, n);
-  a.fst_u = c.in(fst_u, n); a.fst_v = c.in(fst_v, n); a.fst_w = c.in(fst_w, n);
-  a.fst_un = c.in(fst_un, n); a.fst_vn = c.in(fst_vn, n); a.fst_wn = c.in(fst_wn, n);
-  a.src_u = c.io(src_u, n); a.src_v = c.io(src_v, n); a.src_w = c.io(src_w, n);
+  const double* in[23] = {u,      v,      w,      rho,    rhon,   adv_u,  adv_v,  adv_w,
+                          adv_uo, adv_vo, adv_wo, prs_u,  prs_v,  prs_w,  vis_u,  vis_v,
+                          vis_w,  fst_u,  fst_v,  fst_w,  fst_un, fst_vn, fst_wn};
+  double* out[3] = {src_u, src_v, src_w};
   const int jen = (nID[kYPlus] < 0) ? *nj - 1 : *nj;
-  srcu(d, jen, 1.0 / *dt, *grvb, *grvp, a);
-  c.finish();
+  if (flow_fp32()) cal_srcu_t<float>(d, jen, *dt, *grvb, *grvp, in, out);
+  else cal_srcu_t<double>(d, jen, *dt, *grvb, *grvp, in, out);
 }
 
 // srcs/cal_arith_coef_vis.f:6-19
@@ -279,75 +424,33 @@ void cal_arith_coef_vis_(const int* ni, const int* nj, const int* nk, const doub
                          double* a38, double* a39, double* a40, double* a41, double* a42,
                          double* a43, double* a44) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_arith_coef_vis_");
   double* h[45] = {a0,  a1,  a2,  a3,  a4,  a5,  a6,  a7,  a8,  a9,  a10, a11, a12, a13, a14,
                    a15, a16, a17, a18, a19, a20, a21, a22, a23, a24, a25, a26, a27, a28, a29,
                    a30, a31, a32, a33, a34, a35, a36, a37, a38, a39, a40, a41, a42, a43, a44};
-  // a pending record for other output arrays is materialised before it is replaced
-  DeferredCoef& dc = deferred_coef();
-  if (dc.active) {
-    // the same output arrays are about to be redefined: drop the pending
-    // record; other outputs still owe their values
-    bool same = true;
-    for (int m = 0; m < 45 && same; m++) same = dc.out_host[m] == (const void*)h[m];
-    if (!same) deferred_coef_flush();
-    dc.active = false;
-  }
-  bool resident = is_registered(rmuk, d.n3);
-  for (int m = 0; m < 45 && resident; m++) resident = is_registered(h[m], d.n3);
-  const double* r = c.in(rmuk, d.n3);
-  if (resident && !deferred_coef_disabled()) {
-    dc.d = d;
-    dc.dxinv = *dxinv; dc.dyinv = *dyinv; dc.dzinv = *dzinv;
-    dc.src_host = rmuk;
-    dc.src_dev = r;
-    for (int m = 0; m < 45; m++) {
-      dc.out_host[m] = h[m];
-      dc.out_dev[m] = c.out(h[m], d.n3);
-    }
-    dc.active = true;
-  } else {
-    Coef45 a;
-    for (int m = 0; m < 45; m++) a.p[m] = c.out(h[m], d.n3);
-    coef_vis(d, *dxinv, *dyinv, *dzinv, r, a);
-  }
-  c.finish();
+  if (flow_fp32()) cal_arith_coef_vis_t<float>(d, *dxinv, *dyinv, *dzinv, rmuk, h);
+  else cal_arith_coef_vis_t<double>(d, *dxinv, *dyinv, *dzinv, rmuk, h);
 }
 
-// srcs/cpy.f:1
+// srcs/cpy.f:1 (either precision class: copies the device copy as it is held)
 void cpy_(const int* ni, const int* nj, const int* nk, const double* q1, double* q2) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cpy_");
-  const double* a = c.in(q1, d.n3);
-  double* b = c.out(q2, d.n3);
-  copy(d, a, b);
-  c.finish();
-  copy_content_version(q1, q2, d.n3);
+  if (elem_size(q1) == 4) cpy_t<float>(d, q1, q2);
+  else cpy_t<double>(d, q1, q2);
 }
 
 // srcs/init_q.f:1
 void init_q_(const int* ni, const int* nj, const int* nk, double* q1, double* q2, double* q3) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("init_q_");
-  double* p[3] = {c.out(q1, d.n3), c.out(q2, d.n3), c.out(q3, d.n3)};
-  for (int m = 0; m < 3; m++)
-    FEL_CHECK(hipMemsetAsync(p[m], 0, sizeof(double) * (size_t)d.n3, stream()));
-  c.finish();
+  if (elem_size(q1) == 4) init_q_t<float>(d, q1, q2, q3);
+  else init_q_t<double>(d, q1, q2, q3);
 }
 
 // srcs/sum_fst.f:1-3
 void sum_fst_(const int* ni, const int* nj, const int* nk, const double* fst_u, const double* fst_v,
               const double* fst_w, double* sum_u, double* sum_v, double* sum_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("sum_fst_");
-  const double *fu = c.in(fst_u, d.n3), *fv = c.in(fst_v, d.n3), *fw = c.in(fst_w, d.n3);
-  double *su = c.io(sum_u, d.n3), *sv = c.io(sum_v, d.n3), *sw = c.io(sum_w, d.n3);
-  launch_flat(d.n3, [=] __device__(int m) {
-    su[m] = su[m] + fu[m];
-    sv[m] = sv[m] + fv[m];
-    sw[m] = sw[m] + fw[m];
-  });
-  c.finish();
+  if (flow_fp32()) sum_fst_t<float>(d, fst_u, fst_v, fst_w, sum_u, sum_v, sum_w);
+  else sum_fst_t<double>(d, fst_u, fst_v, fst_w, sum_u, sum_v, sum_w);
 }
 
 }  // extern "C"

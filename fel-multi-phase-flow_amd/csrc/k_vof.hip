@@ -35,9 +35,11 @@ static DyTable dy_table(const double* host_arr, int count, int slot) {
   return t;
 }
 
-// Vertex gradient, srcs/cal_grad_p2a.f:30-71, box [-2,n+2]^3.
-void grad_p2a(const Dims d, int ID, double dxinv, DyTable dyt, double dzinv, const double* q,
-              double* qx, double* qy, double* qz) {
+// Vertex gradient, srcs/cal_grad_p2a.f:30-71, box [-2,n+2]^3, formed in FP64
+// from the FP64 volume fraction and stored in the gradient element type T.
+template <class T>
+void grad_p2a(const Dims d, int ID, double dxinv, DyTable dyt, double dzinv, const double* q, T* qx,
+              T* qy, T* qz) {
   launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
@@ -47,27 +49,28 @@ void grad_p2a(const Dims d, int ID, double dxinv, DyTable dyt, double dzinv, con
     const double gy = -q000 - qp00 + q0p0 + qpp0 - q00p - qp0p + q0pp + qppp;
     const double gz = -q000 - qp00 - q0p0 - qpp0 + q00p + qp0p + q0pp + qppp;
     const double dyinv = dyt.at(j + d.nj * ID);
-    qx[c] = gx * 0.25 * dxinv;
-    qy[c] = gy * 0.25 * dyinv;
-    qz[c] = gz * 0.25 * dzinv;
+    qx[c] = T(gx * 0.25 * dxinv);
+    qy[c] = T(gy * 0.25 * dyinv);
+    qz[c] = T(gz * 0.25 * dzinv);
   });
 }
 
 // Cell normal from the 8 surrounding vertex gradients
 // (srcs/solphi_mthinc1.f:81-107).
-__device__ __forceinline__ double vavg8(const double* p, int c, int X, int Y, int Z) {
+template <class T>
+__device__ __forceinline__ T vavg8(const T* p, int c, int X, int Y, int Z) {
   return (+p[c - X - Y - Z] + p[c - Y - Z] + p[c - X - Z] + p[c - Z] + p[c - X - Y] + p[c - Y] +
           p[c - X] + p[c]) *
-         0.125;
+         T(0.125);
 }
 
 // srcs/mthinc_x.h + srcs/mthinc_tanh.h: 8-point Gauss sum of the tanh
 // profile; xs receives the 8 projected Gauss coordinates.
-__device__ __forceinline__ double tanh_sum(double nx, double ny, double nz, double xg1, double xg2,
-                                           double yg1, double yg2, double zg1, double zg2,
-                                           double wght, double bet, double ddd, double xs[8]) {
-  const double x1 = nx * xg1, x2 = nx * xg2, y1 = ny * yg1, y2 = ny * yg2;
-  const double z1 = nz * zg1, z2 = nz * zg2;
+template <class T>
+__device__ __forceinline__ T tanh_sum(T nx, T ny, T nz, T xg1, T xg2, T yg1, T yg2, T zg1, T zg2,
+                                      T wght, T bet, T ddd, T xs[8]) {
+  const T x1 = nx * xg1, x2 = nx * xg2, y1 = ny * yg1, y2 = ny * yg2;
+  const T z1 = nz * zg1, z2 = nz * zg2;
   xs[0] = x1 + y1 + z1;
   xs[1] = x2 + y1 + z1;
   xs[2] = x1 + y2 + z1;
@@ -76,14 +79,23 @@ __device__ __forceinline__ double tanh_sum(double nx, double ny, double nz, doub
   xs[5] = x2 + y1 + z2;
   xs[6] = x1 + y2 + z2;
   xs[7] = x2 + y2 + z2;
-  double s = wght / (1.0 + exp(-(2.0 * bet * (xs[0] + ddd))));
+  T s = wght / (T(1.0) + exp(-(T(2.0) * bet * (xs[0] + ddd))));
 #pragma unroll
-  for (int m = 1; m < 8; m++) s = s + wght / (1.0 + exp(-(2.0 * bet * (xs[m] + ddd))));
+  for (int m = 1; m < 8; m++) s = s + wght / (T(1.0) + exp(-(T(2.0) * bet * (xs[m] + ddd))));
   return s;
 }
 
 struct MthincConst {
   double bet, gp1, gp2, dcellinv, eps_mthinc, eps_trunc;
+};
+
+// The constants in the element type of the MTHINC arithmetic.
+template <class T>
+struct MthincC {
+  T bet, gp1, gp2, dcellinv, eps_mthinc, eps_trunc;
+  explicit MthincC(const MthincConst& m)
+      : bet(T(m.bet)), gp1(T(m.gp1)), gp2(T(m.gp2)), dcellinv(T(m.dcellinv)),
+        eps_mthinc(T(m.eps_mthinc)), eps_trunc(T(m.eps_trunc)) {}
 };
 
 static MthincConst mthinc_const(double dxinv, double dyinv, double dzinv, double bet) {
@@ -109,110 +121,120 @@ static MthincConst mthinc_const(double dxinv, double dyinv, double dzinv, double
 // [0,n+1]^3 -- is zeroed by the cell to its left, which replaces the
 // reference's zero-fill pass (solphi_mthinc1.f:44-52).  Each face therefore
 // has exactly one writer.
-void mthinc_flux(const Dims d, double dxinv, double dyinv, double dzinv, double dt, MthincConst mc,
-                 const double* phi, const double* phix, const double* phiy, const double* phiz,
-                 const double* u, const double* v, const double* w, double* flx, double* fly,
-                 double* flz) {
+// T: element type of the gradients, the velocities and of the MTHINC
+// arithmetic (double, or float in the FP32 mode); the fluxes are stored FP64.
+template <class T>
+void mthinc_flux(const Dims d, double dxinv_, double dyinv_, double dzinv_, double dt_,
+                 MthincConst mc64, const double* phi, const T* phix, const T* phiy, const T* phiz,
+                 const T* u, const T* v, const T* w, double* flx, double* fly, double* flz) {
   const int n1 = d.ni + 1, n2 = d.nj + 1, n3 = d.nk + 1;
+  const MthincC<T> mc(mc64);
+  const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_), dt = T(dt_);
   launch_box(Box{-1, d.ni + 1, -1, d.nj + 1, -1, d.nk + 1}, [=] __device__(int i, int j, int k) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
     const bool in_i = i >= 0, in_j = j >= 0, in_k = k >= 0;      // cell inside [0,n+1]^3 ?
     const bool self = in_i && in_j && in_k;
-    const double up = u[c], vp = v[c], wp = w[c];
+    const T up = u[c], vp = v[c], wp = w[c];
+    const T zero = T(0.0);
     // zero the right faces that have no outflow writer
     {
       const bool rx = (i + 1 <= n1) && in_j && in_k;  // right neighbour in [0,n+1]^3
-      if (up == 0.0 || (up > 0.0 && !self) || (up < 0.0 && !rx) || !in_j || !in_k) flx[c] = 0.0;
+      if (up == zero || (up > zero && !self) || (up < zero && !rx) || !in_j || !in_k) flx[c] = 0.0;
       const bool ry = (j + 1 <= n2) && in_i && in_k;
-      if (vp == 0.0 || (vp > 0.0 && !self) || (vp < 0.0 && !ry) || !in_i || !in_k) fly[c] = 0.0;
+      if (vp == zero || (vp > zero && !self) || (vp < zero && !ry) || !in_i || !in_k) fly[c] = 0.0;
       const bool rz = (k + 1 <= n3) && in_i && in_j;
-      if (wp == 0.0 || (wp > 0.0 && !self) || (wp < 0.0 && !rz) || !in_i || !in_j) flz[c] = 0.0;
+      if (wp == zero || (wp > zero && !self) || (wp < zero && !rz) || !in_i || !in_j) flz[c] = 0.0;
     }
     if (!self) return;
-    const double um = u[c - X], vm = v[c - Y], wm = w[c - Z];
-    double rnx = vavg8(phix, c, X, Y, Z);
-    double rny = vavg8(phiy, c, X, Y, Z);
-    double rnz = vavg8(phiz, c, X, Y, Z);
-    const double q00 = sqrt(rnx * rnx + rny * rny + rnz * rnz);
+    const T um = u[c - X], vm = v[c - Y], wm = w[c - Z];
+    T rnx = vavg8(phix, c, X, Y, Z);
+    T rny = vavg8(phiy, c, X, Y, Z);
+    T rnz = vavg8(phiz, c, X, Y, Z);
+    const T q00 = sqrt(rnx * rnx + rny * rny + rnz * rnz);
     if (q00 <= mc.eps_mthinc * mc.dcellinv) {
       // first-order upwind with the truncated cell value (solphi_mthinc1.f:110-135)
-      double phi00 = phi[c];
-      if (phi00 <= mc.eps_trunc) phi00 = 0.0;
-      if (1.0 - phi00 <= mc.eps_trunc) phi00 = 1.0;
-      if (um < 0.0) flx[c - X] = um * phi00 * dt * dxinv;
-      if (up > 0.0) flx[c] = up * phi00 * dt * dxinv;
-      if (vm < 0.0) fly[c - Y] = vm * phi00 * dt * dyinv;
-      if (vp > 0.0) fly[c] = vp * phi00 * dt * dyinv;
-      if (wm < 0.0) flz[c - Z] = wm * phi00 * dt * dzinv;
-      if (wp > 0.0) flz[c] = wp * phi00 * dt * dzinv;
+      T phi00 = T(phi[c]);
+      if (phi00 <= mc.eps_trunc) phi00 = zero;
+      if (T(1.0) - phi00 <= mc.eps_trunc) phi00 = T(1.0);
+      if (um < zero) flx[c - X] = um * phi00 * dt * dxinv;
+      if (up > zero) flx[c] = up * phi00 * dt * dxinv;
+      if (vm < zero) fly[c - Y] = vm * phi00 * dt * dyinv;
+      if (vp > zero) fly[c] = vp * phi00 * dt * dyinv;
+      if (wm < zero) flz[c - Z] = wm * phi00 * dt * dzinv;
+      if (wp > zero) flz[c] = wp * phi00 * dt * dzinv;
       return;
     }
-    const double coef = 1.0 / q00;
+    const T coef = T(1.0) / q00;
     rnx = rnx * coef;
     rny = rny * coef;
     rnz = rnz * coef;
-    const double bet = mc.bet, gp1 = mc.gp1, gp2 = mc.gp2;
-    const double phic = phi[c];
-    double xs[8];
-    double ddd = 0.0;
+    const T bet = mc.bet, gp1 = mc.gp1, gp2 = mc.gp2;
+    const T phic = T(phi[c]);
+    const T h = T(0.5), e8 = T(0.125);
+    T xs[8];
+    T ddd = zero;
     // Newton iterations for the interface offset (solphi_mthinc1.f:161-176)
     for (int inr = 0; inr < 10; inr++) {
-      const double wt = tanh_sum(rnx, rny, rnz, gp1, gp2, gp1, gp2, gp1, gp2, 0.125, bet, ddd, xs);
-      const double f = wt - phic;
-      double ch = cosh(bet * (xs[0] + ddd));
-      double s = 0.125 / (ch * ch);
+      const T wt = tanh_sum(rnx, rny, rnz, gp1, gp2, gp1, gp2, gp1, gp2, e8, bet, ddd, xs);
+      const T f = wt - phic;
+      T ch = cosh(bet * (xs[0] + ddd));
+      T s = e8 / (ch * ch);
 #pragma unroll
       for (int m = 1; m < 8; m++) {
         ch = cosh(bet * (xs[m] + ddd));
-        s = s + 0.125 / (ch * ch);
+        s = s + e8 / (ch * ch);
       }
-      const double df = 0.5 * bet * s;
-      ddd = ddd - f * df / (df * df + 1.0e-99);
+      const T df = h * bet * s;
+      ddd = ddd - f * df / (df * df + guard<T>(1.0e-99));
     }
     // fluxes through the outflow faces (solphi_mthinc1.f:182-282)
-    if (um < 0.0) {
-      const double vd = um * dt * dxinv, st = -0.5, en = -0.5 - vd;
-      const double g1 = (st + en) * 0.5 + (-st + en) * gp1, g2 = (st + en) * 0.5 + (-st + en) * gp2;
-      flx[c - X] = tanh_sum(rnx, rny, rnz, g1, g2, gp1, gp2, gp1, gp2, 0.125 * vd, bet, ddd, xs);
+    if (um < zero) {
+      const T vd = um * dt * dxinv, st = -h, en = -h - vd;
+      const T g1 = (st + en) * h + (-st + en) * gp1, g2 = (st + en) * h + (-st + en) * gp2;
+      flx[c - X] = tanh_sum(rnx, rny, rnz, g1, g2, gp1, gp2, gp1, gp2, e8 * vd, bet, ddd, xs);
     }
-    if (up > 0.0) {
-      const double vd = up * dt * dxinv, st = 0.5 - vd, en = 0.5;
-      const double g1 = (st + en) * 0.5 + (-st + en) * gp1, g2 = (st + en) * 0.5 + (-st + en) * gp2;
-      flx[c] = tanh_sum(rnx, rny, rnz, g1, g2, gp1, gp2, gp1, gp2, 0.125 * vd, bet, ddd, xs);
+    if (up > zero) {
+      const T vd = up * dt * dxinv, st = h - vd, en = h;
+      const T g1 = (st + en) * h + (-st + en) * gp1, g2 = (st + en) * h + (-st + en) * gp2;
+      flx[c] = tanh_sum(rnx, rny, rnz, g1, g2, gp1, gp2, gp1, gp2, e8 * vd, bet, ddd, xs);
     }
-    if (vm < 0.0) {
-      const double vd = vm * dt * dyinv, st = -0.5, en = -0.5 - vd;
-      const double g1 = (st + en) * 0.5 + (-st + en) * gp1, g2 = (st + en) * 0.5 + (-st + en) * gp2;
-      fly[c - Y] = tanh_sum(rnx, rny, rnz, gp1, gp2, g1, g2, gp1, gp2, 0.125 * vd, bet, ddd, xs);
+    if (vm < zero) {
+      const T vd = vm * dt * dyinv, st = -h, en = -h - vd;
+      const T g1 = (st + en) * h + (-st + en) * gp1, g2 = (st + en) * h + (-st + en) * gp2;
+      fly[c - Y] = tanh_sum(rnx, rny, rnz, gp1, gp2, g1, g2, gp1, gp2, e8 * vd, bet, ddd, xs);
     }
-    if (vp > 0.0) {
-      const double vd = vp * dt * dyinv, st = 0.5 - vd, en = 0.5;
-      const double g1 = (st + en) * 0.5 + (-st + en) * gp1, g2 = (st + en) * 0.5 + (-st + en) * gp2;
-      fly[c] = tanh_sum(rnx, rny, rnz, gp1, gp2, g1, g2, gp1, gp2, 0.125 * vd, bet, ddd, xs);
+    if (vp > zero) {
+      const T vd = vp * dt * dyinv, st = h - vd, en = h;
+      const T g1 = (st + en) * h + (-st + en) * gp1, g2 = (st + en) * h + (-st + en) * gp2;
+      fly[c] = tanh_sum(rnx, rny, rnz, gp1, gp2, g1, g2, gp1, gp2, e8 * vd, bet, ddd, xs);
     }
-    if (wm < 0.0) {
-      const double vd = wm * dt * dzinv, st = -0.5, en = -0.5 - vd;
-      const double g1 = (st + en) * 0.5 + (-st + en) * gp1, g2 = (st + en) * 0.5 + (-st + en) * gp2;
-      flz[c - Z] = tanh_sum(rnx, rny, rnz, gp1, gp2, gp1, gp2, g1, g2, 0.125 * vd, bet, ddd, xs);
+    if (wm < zero) {
+      const T vd = wm * dt * dzinv, st = -h, en = -h - vd;
+      const T g1 = (st + en) * h + (-st + en) * gp1, g2 = (st + en) * h + (-st + en) * gp2;
+      flz[c - Z] = tanh_sum(rnx, rny, rnz, gp1, gp2, gp1, gp2, g1, g2, e8 * vd, bet, ddd, xs);
     }
-    if (wp > 0.0) {
-      const double vd = wp * dt * dzinv, st = 0.5 - vd, en = 0.5;
-      const double g1 = (st + en) * 0.5 + (-st + en) * gp1, g2 = (st + en) * 0.5 + (-st + en) * gp2;
-      flz[c] = tanh_sum(rnx, rny, rnz, gp1, gp2, gp1, gp2, g1, g2, 0.125 * vd, bet, ddd, xs);
+    if (wp > zero) {
+      const T vd = wp * dt * dzinv, st = h - vd, en = h;
+      const T g1 = (st + en) * h + (-st + en) * gp1, g2 = (st + en) * h + (-st + en) * gp2;
+      flz[c] = tanh_sum(rnx, rny, rnz, gp1, gp2, gp1, gp2, g1, g2, e8 * vd, bet, ddd, xs);
     }
   });
 }
 
 // srcs/solphi_mthinc2.f:27-44.
-void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, double dt,
-                   const double* u, const double* v, const double* w, const double* flx,
-                   const double* fly, const double* flz, const double* phi, double* phin) {
+// The update is FP64: velocity differences are formed from the velocities
+// (FP32 values are exact in FP64) and the fluxes are FP64.
+template <class T>
+void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, double dt, const T* u,
+                   const T* v, const T* w, const double* flx, const double* fly, const double* flz,
+                   const double* phi, double* phin) {
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
-    const double div00 = +(-u[c - X] + u[c]) * dxinv + (-v[c - Y] + v[c]) * dyinv +
-                         (-w[c - Z] + w[c]) * dzinv;
+    const double div00 = +(-(double)u[c - X] + (double)u[c]) * dxinv +
+                         (-(double)v[c - Y] + (double)v[c]) * dyinv +
+                         (-(double)w[c - Z] + (double)w[c]) * dzinv;
     double p = phi[c] * (1.0 + dt * div00) + flx[c - X] - flx[c] + fly[c - Y] - fly[c] +
                flz[c - Z] - flz[c];
     p = fmin(p, 1.0);
@@ -224,20 +246,22 @@ void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, doubl
 // srcs/solphi_mthinc3.f:43-159: truncation on [0,n+1]^3 (clamped in the
 // interior in the same pass, the reference's second loop), the three global
 // sums, and the mass correction.  The offset stays on the device.
-void mthinc_mass(const Dims d, MthincConst mc, const double* phix, const double* phiy,
-                 const double* phiz, const double* phi, double* phin) {
+template <class T>
+void mthinc_mass(const Dims d, MthincConst mc64, const T* phix, const T* phiy, const T* phiz,
+                 const double* phi, double* phin) {
+  const MthincC<T> mc(mc64);
   launch_box(Box{0, d.ni + 1, 0, d.nj + 1, 0, d.nk + 1}, [=] __device__(int i, int j, int k) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
-    const double rnx = vavg8(phix, c, X, Y, Z);
-    const double rny = vavg8(phiy, c, X, Y, Z);
-    const double rnz = vavg8(phiz, c, X, Y, Z);
-    const double q00 = sqrt(rnx * rnx + rny * rny + rnz * rnz);
+    const T rnx = vavg8(phix, c, X, Y, Z);
+    const T rny = vavg8(phiy, c, X, Y, Z);
+    const T rnz = vavg8(phiz, c, X, Y, Z);
+    const T q00 = sqrt(rnx * rnx + rny * rny + rnz * rnz);
     double p = phin[c];
     const double p0 = p;
     if (q00 <= mc.eps_mthinc * mc.dcellinv) {
-      if (p <= mc.eps_trunc) p = 0.0;
-      if (1.0 - p <= mc.eps_trunc) p = 1.0;
+      if (p <= mc64.eps_trunc) p = 0.0;
+      if (1.0 - p <= mc64.eps_trunc) p = 1.0;
     }
     if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk) {
       p = fmin(p, 1.0);
@@ -286,6 +310,72 @@ void summation(const Dims d, double* qk, int nbub) {
   });
 }
 
+template <class T>
+void cal_grad_p2a_t(const Dims d, int ID, int svall, double dxinv, const double* dyinv_array,
+                    double dzinv, const double* q, double* qx, double* qy, double* qz) {
+  Call c("cal_grad_p2a_");
+  const double* dq = c.in(q, d.n3, "q");
+  T* dx = c.io_t<T>(qx, d.n3, "qx");
+  T* dy = c.io_t<T>(qy, d.n3, "qy");
+  T* dz = c.io_t<T>(qz, d.n3, "qz");
+  grad_p2a<T>(d, ID, dxinv, dy_table(dyinv_array, svall + 6, 1), dzinv, dq, dx, dy, dz);
+  c.finish();
+}
+
+template <class T>
+void solphi_mthinc1_t(const Dims d, double dxinv, double dyinv, double dzinv, double dt, double bet,
+                      const double* phi, const double* phix, const double* phiy, const double* phiz,
+                      const double* u, const double* v, const double* w, double* flphix,
+                      double* flphiy, double* flphiz) {
+  Call c("solphi_mthinc1_");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  const T* gx = c.in_t<T>(phix, d.n3, "phix");
+  const T* gy = c.in_t<T>(phiy, d.n3, "phiy");
+  const T* gz = c.in_t<T>(phiz, d.n3, "phiz");
+  const T* du = c.in_t<T>(u, d.n3, "u");
+  const T* dv = c.in_t<T>(v, d.n3, "v");
+  const T* dw = c.in_t<T>(w, d.n3, "w");
+  double* fx = c.io(flphix, d.n3, "flphix");
+  double* fy = c.io(flphiy, d.n3, "flphiy");
+  double* fz = c.io(flphiz, d.n3, "flphiz");
+  prof_begin(1);
+  mthinc_flux<T>(d, dxinv, dyinv, dzinv, dt, mthinc_const(dxinv, dyinv, dzinv, bet), dphi, gx, gy,
+                 gz, du, dv, dw, fx, fy, fz);
+  prof_end(1);
+  c.finish();
+}
+
+template <class T>
+void solphi_mthinc2_t(const Dims d, double dxinv, double dyinv, double dzinv, double dt,
+                      const double* u, const double* v, const double* w, const double* flphix,
+                      const double* flphiy, const double* flphiz, const double* phi, double* phin) {
+  Call c("solphi_mthinc2_");
+  const T* du = c.in_t<T>(u, d.n3, "u");
+  const T* dv = c.in_t<T>(v, d.n3, "v");
+  const T* dw = c.in_t<T>(w, d.n3, "w");
+  const double* fx = c.in(flphix, d.n3, "flphix");
+  const double* fy = c.in(flphiy, d.n3, "flphiy");
+  const double* fz = c.in(flphiz, d.n3, "flphiz");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  double* dphin = c.io(phin, d.n3, "phin");
+  mthinc_update<T>(d, dxinv, dyinv, dzinv, dt, du, dv, dw, fx, fy, fz, dphi, dphin);
+  c.finish();
+}
+
+template <class T>
+void solphi_mthinc3_t(const Dims d, double dxinv, double dyinv, double dzinv, double bet,
+                      const double* phix, const double* phiy, const double* phiz, const double* phi,
+                      double* phin) {
+  Call c("solphi_mthinc3_");
+  const T* gx = c.in_t<T>(phix, d.n3, "phix");
+  const T* gy = c.in_t<T>(phiy, d.n3, "phiy");
+  const T* gz = c.in_t<T>(phiz, d.n3, "phiz");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  double* dphin = c.io(phin, d.n3, "phin");
+  mthinc_mass<T>(d, mthinc_const(dxinv, dyinv, dzinv, bet), gx, gy, gz, dphi, dphin);
+  c.finish();
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -297,13 +387,8 @@ void cal_grad_p2a_(const int* ID, const int* svall, const int* ni, const int* nj
                    const double* dxinv, const double* dyinv_array, const double* dzinv,
                    const double* q, double* qx, double* qy, double* qz) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_grad_p2a_");
-  const double* dq = c.in(q, d.n3, "q");
-  double* dx = c.io(qx, d.n3, "qx");
-  double* dy = c.io(qy, d.n3, "qy");
-  double* dz = c.io(qz, d.n3, "qz");
-  grad_p2a(d, *ID, *dxinv, dy_table(dyinv_array, *svall + 6, 1), *dzinv, dq, dx, dy, dz);
-  c.finish();
+  if (flow_fp32()) cal_grad_p2a_t<float>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx, qy, qz);
+  else cal_grad_p2a_t<double>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx, qy, qz);
 }
 
 // srcs/solphi_mthinc1.f:1-6
@@ -313,22 +398,12 @@ void solphi_mthinc1_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* phiy, const double* phiz, const double* u, const double* v,
                      const double* w, double* flphix, double* flphiy, double* flphiz) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("solphi_mthinc1_");
-  const double* dphi = c.in(phi, d.n3, "phi");
-  const double* gx = c.in(phix, d.n3, "phix");
-  const double* gy = c.in(phiy, d.n3, "phiy");
-  const double* gz = c.in(phiz, d.n3, "phiz");
-  const double* du = c.in(u, d.n3, "u");
-  const double* dv = c.in(v, d.n3, "v");
-  const double* dw = c.in(w, d.n3, "w");
-  double* fx = c.io(flphix, d.n3, "flphix");
-  double* fy = c.io(flphiy, d.n3, "flphiy");
-  double* fz = c.io(flphiz, d.n3, "flphiz");
-  prof_begin(1);
-  mthinc_flux(d, *dxinv, *dyinv, *dzinv, *dt, mthinc_const(*dxinv, *dyinv, *dzinv, *bet_mthinc), dphi,
-              gx, gy, gz, du, dv, dw, fx, fy, fz);
-  prof_end(1);
-  c.finish();
+  if (flow_fp32())
+    solphi_mthinc1_t<float>(d, *dxinv, *dyinv, *dzinv, *dt, *bet_mthinc, phi, phix, phiy, phiz, u, v,
+                            w, flphix, flphiy, flphiz);
+  else
+    solphi_mthinc1_t<double>(d, *dxinv, *dyinv, *dzinv, *dt, *bet_mthinc, phi, phix, phiy, phiz, u,
+                             v, w, flphix, flphiy, flphiz);
 }
 
 // srcs/solphi_mthinc2.f:1-2
@@ -337,17 +412,12 @@ void solphi_mthinc2_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* v, const double* w, const double* flphix, const double* flphiy,
                      const double* flphiz, const double* phi, double* phin) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("solphi_mthinc2_");
-  const double* du = c.in(u, d.n3);
-  const double* dv = c.in(v, d.n3);
-  const double* dw = c.in(w, d.n3);
-  const double* fx = c.in(flphix, d.n3);
-  const double* fy = c.in(flphiy, d.n3);
-  const double* fz = c.in(flphiz, d.n3);
-  const double* dphi = c.in(phi, d.n3);
-  double* dphin = c.io(phin, d.n3);
-  mthinc_update(d, *dxinv, *dyinv, *dzinv, *dt, du, dv, dw, fx, fy, fz, dphi, dphin);
-  c.finish();
+  if (flow_fp32())
+    solphi_mthinc2_t<float>(d, *dxinv, *dyinv, *dzinv, *dt, u, v, w, flphix, flphiy, flphiz, phi,
+                            phin);
+  else
+    solphi_mthinc2_t<double>(d, *dxinv, *dyinv, *dzinv, *dt, u, v, w, flphix, flphiy, flphiz, phi,
+                             phin);
 }
 
 // srcs/solphi_mthinc3.f:1-5 (ipara must be 0: one GPU, no MPI)
@@ -360,14 +430,10 @@ void solphi_mthinc3_(const int* ipara, const int* ni, const int* nj, const int* 
     std::abort();
   }
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("solphi_mthinc3_");
-  const double* gx = c.in(phix, d.n3);
-  const double* gy = c.in(phiy, d.n3);
-  const double* gz = c.in(phiz, d.n3);
-  const double* dphi = c.in(phi, d.n3);
-  double* dphin = c.io(phin, d.n3);
-  mthinc_mass(d, mthinc_const(*dxinv, *dyinv, *dzinv, *bet_mthinc), gx, gy, gz, dphi, dphin);
-  c.finish();
+  if (flow_fp32())
+    solphi_mthinc3_t<float>(d, *dxinv, *dyinv, *dzinv, *bet_mthinc, phix, phiy, phiz, phi, phin);
+  else
+    solphi_mthinc3_t<double>(d, *dxinv, *dyinv, *dzinv, *bet_mthinc, phix, phiy, phiz, phi, phin);
 }
 
 // srcs/summation.f:1

@@ -13,67 +13,85 @@ namespace fel {
 
 // srcs/weno5.h:24-71 (nonlinear weights with the 1e-80 floor of
 // cal_advu_weno5.f:31), expressions in the reference's order.
-__device__ __forceinline__ double weno5(double dfm5, double dfm3, double dfm1, double dfp1,
-                                        double dfp3, double dfp5, double vel00) {
-  const double dfm_1 = dfm5 * 2.0 - dfm3 * 7.0 + dfm1 * 11.0;
-  const double dfm_2 = -dfm3 + dfm1 * 5.0 + dfp1 * 2.0;
-  const double dfm_3 = dfm1 * 2.0 + dfp1 * 5.0 - dfp3;
-  const double dfp_1 = dfp5 * 2.0 - dfp3 * 7.0 + dfp1 * 11.0;
-  const double dfp_2 = -dfp3 + dfp1 * 5.0 + dfm1 * 2.0;
-  const double dfp_3 = dfp1 * 2.0 + dfm1 * 5.0 - dfm3;
-  double a, b;
-  a = dfm5 - dfm3 * 2.0 + dfm1;
-  b = dfm5 - dfm3 * 4.0 + dfm1 * 3.0;
-  const double sm_1 = a * a * 13.0 + b * b * 3.0;
-  a = dfm3 - dfm1 * 2.0 + dfp1;
-  b = dfm3 - dfp1;
-  const double sm_2 = a * a * 13.0 + b * b * 3.0;
-  a = dfm1 - dfp1 * 2.0 + dfp3;
-  b = dfm1 * 3.0 - dfp1 * 4.0 + dfp3;
-  const double sm_3 = a * a * 13.0 + b * b * 3.0;
-  a = dfp5 - dfp3 * 2.0 + dfp1;
-  b = dfp5 - dfp3 * 4.0 + dfp1 * 3.0;
-  const double sp_1 = a * a * 13.0 + b * b * 3.0;
-  a = dfp3 - dfp1 * 2.0 + dfm1;
-  b = dfp3 - dfm1;
-  const double sp_2 = a * a * 13.0 + b * b * 3.0;
-  a = dfp1 - dfm1 * 2.0 + dfm3;
-  b = dfp1 * 3.0 - dfm1 * 4.0 + dfm3;
-  const double sp_3 = a * a * 13.0 + b * b * 3.0;
-  double am_1 = sm_2 * sm_3, am_2 = sm_1 * sm_3, am_3 = sm_1 * sm_2;
-  double ap_1 = sp_2 * sp_3, ap_2 = sp_1 * sp_3, ap_3 = sp_1 * sp_2;
+//
+// T = double is the reference's FP64 arithmetic.  T = float (the FP32 mode)
+// evaluates the same weights from differences divided by their largest
+// magnitude: the weights are homogeneous of degree 0 in that scale, so the
+// division changes nothing but keeps the degree-8 products of the
+// smoothness indicators inside the FP32 range; the stencil derivatives use
+// the unscaled differences.
+template <class T>
+__device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp5, T vel00) {
+  const T c2 = T(2.0), c3 = T(3.0), c4 = T(4.0), c5 = T(5.0), c7 = T(7.0), c11 = T(11.0),
+          c13 = T(13.0);
+  const T dfm_1 = dfm5 * c2 - dfm3 * c7 + dfm1 * c11;
+  const T dfm_2 = -dfm3 + dfm1 * c5 + dfp1 * c2;
+  const T dfm_3 = dfm1 * c2 + dfp1 * c5 - dfp3;
+  const T dfp_1 = dfp5 * c2 - dfp3 * c7 + dfp1 * c11;
+  const T dfp_2 = -dfp3 + dfp1 * c5 + dfm1 * c2;
+  const T dfp_3 = dfp1 * c2 + dfm1 * c5 - dfm3;
+  T m5 = dfm5, m3 = dfm3, m1 = dfm1, p1 = dfp1, p3 = dfp3, p5 = dfp5;
+  if constexpr (sizeof(T) == 4) {
+    const T sc = fmax(fmax(fmax(fabs(m5), fabs(m3)), fmax(fabs(m1), fabs(p1))),
+                      fmax(fabs(p3), fabs(p5)));
+    const T inv = sc > T(0.0) ? T(1.0) / sc : T(0.0);
+    m5 = m5 * inv; m3 = m3 * inv; m1 = m1 * inv; p1 = p1 * inv; p3 = p3 * inv; p5 = p5 * inv;
+  }
+  T a, b;
+  a = m5 - m3 * c2 + m1;
+  b = m5 - m3 * c4 + m1 * c3;
+  const T sm_1 = a * a * c13 + b * b * c3;
+  a = m3 - m1 * c2 + p1;
+  b = m3 - p1;
+  const T sm_2 = a * a * c13 + b * b * c3;
+  a = m1 - p1 * c2 + p3;
+  b = m1 * c3 - p1 * c4 + p3;
+  const T sm_3 = a * a * c13 + b * b * c3;
+  a = p5 - p3 * c2 + p1;
+  b = p5 - p3 * c4 + p1 * c3;
+  const T sp_1 = a * a * c13 + b * b * c3;
+  a = p3 - p1 * c2 + m1;
+  b = p3 - m1;
+  const T sp_2 = a * a * c13 + b * b * c3;
+  a = p1 - m1 * c2 + m3;
+  b = p1 * c3 - m1 * c4 + m3;
+  const T sp_3 = a * a * c13 + b * b * c3;
+  T am_1 = sm_2 * sm_3, am_2 = sm_1 * sm_3, am_3 = sm_1 * sm_2;
+  T ap_1 = sp_2 * sp_3, ap_2 = sp_1 * sp_3, ap_3 = sp_1 * sp_2;
   am_1 = am_1 * am_1;
   am_2 = am_2 * am_2;
   am_3 = am_3 * am_3;
   ap_1 = ap_1 * ap_1;
   ap_2 = ap_2 * ap_2;
   ap_3 = ap_3 * ap_3;
-  am_2 = am_2 * 6.0;
-  am_3 = am_3 * 3.0;
-  ap_2 = ap_2 * 6.0;
-  ap_3 = ap_3 * 3.0;
-  const double deninv_am = 1.0 / (am_1 + am_2 + am_3 + 1.0e-80);
-  const double deninv_ap = 1.0 / (ap_1 + ap_2 + ap_3 + 1.0e-80);
-  const double wgm_1 = am_1 * deninv_am, wgm_3 = am_3 * deninv_am;
-  const double wgp_1 = ap_1 * deninv_ap, wgp_3 = ap_3 * deninv_ap;
-  const double wgm_2 = 1.0 - wgm_1 - wgm_3;
-  const double wgp_2 = 1.0 - wgp_1 - wgp_3;
-  const double velab = fabs(vel00);
-  const double velm1 = vel00 + velab, velp1 = vel00 - velab;
+  am_2 = am_2 * T(6.0);
+  am_3 = am_3 * c3;
+  ap_2 = ap_2 * T(6.0);
+  ap_3 = ap_3 * c3;
+  const T deninv_am = T(1.0) / (am_1 + am_2 + am_3 + guard<T>(1.0e-80));
+  const T deninv_ap = T(1.0) / (ap_1 + ap_2 + ap_3 + guard<T>(1.0e-80));
+  const T wgm_1 = am_1 * deninv_am, wgm_3 = am_3 * deninv_am;
+  const T wgp_1 = ap_1 * deninv_ap, wgp_3 = ap_3 * deninv_ap;
+  const T wgm_2 = T(1.0) - wgm_1 - wgm_3;
+  const T wgp_2 = T(1.0) - wgp_1 - wgp_3;
+  const T velab = fabs(vel00);
+  const T velm1 = vel00 + velab, velp1 = vel00 - velab;
   return (velm1 * (wgm_1 * dfm_1 + wgm_2 * dfm_2 + wgm_3 * dfm_3) +
           velp1 * (wgp_1 * dfp_1 + wgp_2 * dfp_2 + wgp_3 * dfp_3)) *
-         (1.0 / 12.0);
+         (T(1.0) / T(12.0));
 }
 
-__device__ __forceinline__ double weno_dir(const double* __restrict__ f, int c, int st, double vel00) {
-  const double fm3 = f[c - 3 * st], fm2 = f[c - 2 * st], fm1 = f[c - st], f0 = f[c];
-  const double fp1 = f[c + st], fp2 = f[c + 2 * st], fp3 = f[c + 3 * st];
+template <class T>
+__device__ __forceinline__ T weno_dir(const T* __restrict__ f, int c, int st, T vel00) {
+  const T fm3 = f[c - 3 * st], fm2 = f[c - 2 * st], fm1 = f[c - st], f0 = f[c];
+  const T fp1 = f[c + st], fp2 = f[c + 2 * st], fp3 = f[c + 3 * st];
   return weno5(-fm3 + fm2, -fm2 + fm1, -fm1 + f0, -f0 + fp1, -fp1 + fp2, -fp2 + fp3, vel00);
 }
 
 // The WENO5 derivative from a register queue f(k-3..k+3) of the cell's own
 // column (the z direction), same differences as weno_dir.
-__device__ __forceinline__ double weno_q(const double* q, double vel00) {
+template <class T>
+__device__ __forceinline__ T weno_q(const T* q, T vel00) {
   return weno5(-q[0] + q[1], -q[1] + q[2], -q[2] + q[3], -q[3] + q[4], -q[4] + q[5],
                -q[5] + q[6], vel00);
 }
@@ -87,13 +105,13 @@ __device__ __forceinline__ double weno_q(const double* q, double vel00) {
 // one-thread-per-cell form, so the results are identical.
 constexpr int kWenoK = 16;
 
-__global__ void __launch_bounds__(256) weno_march_kernel(Dims d, double dxinv, double dyinv,
-                                                         double dzinv, const double* __restrict__ u,
-                                                         const double* __restrict__ v,
-                                                         const double* __restrict__ w,
-                                                         double* __restrict__ au,
-                                                         double* __restrict__ av,
-                                                         double* __restrict__ aw, int gx, int gy) {
+template <class T>
+__global__ void __launch_bounds__(256) weno_march_kernel(Dims d, T dxinv, T dyinv, T dzinv,
+                                                         const T* __restrict__ u,
+                                                         const T* __restrict__ v,
+                                                         const T* __restrict__ w,
+                                                         T* __restrict__ au, T* __restrict__ av,
+                                                         T* __restrict__ aw, int gx, int gy) {
   const int L = (int)blockIdx.x;
   const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
   const int i = 1 + bx * 64 + (int)threadIdx.x, j = 1 + by * 4 + (int)threadIdx.y;
@@ -101,7 +119,8 @@ __global__ void __launch_bounds__(256) weno_march_kernel(Dims d, double dxinv, d
   const int k0 = 1 + bz * kWenoK, k1 = min(d.nk, k0 + kWenoK - 1);
   const int X = 1, Y = d.sx, Z = d.sx * d.sy;
   int c = d.ix(i, j, k0);
-  double qu[7], qv[7], qw[7];
+  T qu[7], qv[7], qw[7];
+  const T h = T(0.5);
 #pragma unroll
   for (int m = 0; m < 6; m++) {
     qu[m + 1] = u[c + (m - 3) * Z];
@@ -118,55 +137,56 @@ __global__ void __launch_bounds__(256) weno_march_kernel(Dims d, double dxinv, d
     qu[6] = u[c + 3 * Z];
     qv[6] = v[c + 3 * Z];
     qw[6] = w[c + 3 * Z];
-    double a, cm, cp;
+    T a, cm, cp;
     // u (cal_advu_weno5.f:80-120)
-    cm = (u[c - X] + qu[3]) * 0.5;
-    cp = (qu[3] + u[c + X]) * 0.5;
-    a = weno_dir(u, c, X, (cm + cp) * 0.5) * dxinv;
-    cm = (v[c - Y] + v[c + X - Y]) * 0.5;
-    cp = (qv[3] + v[c + X]) * 0.5;
-    a = a + weno_dir(u, c, Y, (cm + cp) * 0.5) * dyinv;
-    cm = (qw[2] + w[c + X - Z]) * 0.5;
-    cp = (qw[3] + w[c + X]) * 0.5;
-    au[c] = a + weno_q(qu, (cm + cp) * 0.5) * dzinv;
+    cm = (u[c - X] + qu[3]) * h;
+    cp = (qu[3] + u[c + X]) * h;
+    a = weno_dir(u, c, X, (cm + cp) * h) * dxinv;
+    cm = (v[c - Y] + v[c + X - Y]) * h;
+    cp = (qv[3] + v[c + X]) * h;
+    a = a + weno_dir(u, c, Y, (cm + cp) * h) * dyinv;
+    cm = (qw[2] + w[c + X - Z]) * h;
+    cp = (qw[3] + w[c + X]) * h;
+    au[c] = a + weno_q(qu, (cm + cp) * h) * dzinv;
     // v (cal_advu_weno5.f:147-187)
-    cm = (u[c - X] + u[c - X + Y]) * 0.5;
-    cp = (qu[3] + u[c + Y]) * 0.5;
-    a = weno_dir(v, c, X, (cm + cp) * 0.5) * dxinv;
-    cm = (v[c - Y] + qv[3]) * 0.5;
-    cp = (qv[3] + v[c + Y]) * 0.5;
-    a = a + weno_dir(v, c, Y, (cm + cp) * 0.5) * dyinv;
-    cm = (qw[2] + w[c + Y - Z]) * 0.5;
-    cp = (qw[3] + w[c + Y]) * 0.5;
-    av[c] = a + weno_q(qv, (cm + cp) * 0.5) * dzinv;
+    cm = (u[c - X] + u[c - X + Y]) * h;
+    cp = (qu[3] + u[c + Y]) * h;
+    a = weno_dir(v, c, X, (cm + cp) * h) * dxinv;
+    cm = (v[c - Y] + qv[3]) * h;
+    cp = (qv[3] + v[c + Y]) * h;
+    a = a + weno_dir(v, c, Y, (cm + cp) * h) * dyinv;
+    cm = (qw[2] + w[c + Y - Z]) * h;
+    cp = (qw[3] + w[c + Y]) * h;
+    av[c] = a + weno_q(qv, (cm + cp) * h) * dzinv;
     // w (cal_advu_weno5.f:214-254)
-    cm = (u[c - X] + u[c - X + Z]) * 0.5;
-    cp = (qu[3] + qu[4]) * 0.5;
-    a = weno_dir(w, c, X, (cm + cp) * 0.5) * dxinv;
-    cm = (v[c - Y] + v[c - Y + Z]) * 0.5;
-    cp = (qv[3] + qv[4]) * 0.5;
-    a = a + weno_dir(w, c, Y, (cm + cp) * 0.5) * dyinv;
-    cm = (qw[2] + qw[3]) * 0.5;
-    cp = (qw[3] + qw[4]) * 0.5;
-    aw[c] = a + weno_q(qw, (cm + cp) * 0.5) * dzinv;
+    cm = (u[c - X] + u[c - X + Z]) * h;
+    cp = (qu[3] + qu[4]) * h;
+    a = weno_dir(w, c, X, (cm + cp) * h) * dxinv;
+    cm = (v[c - Y] + v[c - Y + Z]) * h;
+    cp = (qv[3] + qv[4]) * h;
+    a = a + weno_dir(w, c, Y, (cm + cp) * h) * dyinv;
+    cm = (qw[2] + qw[3]) * h;
+    cp = (qw[3] + qw[4]) * h;
+    aw[c] = a + weno_q(qw, (cm + cp) * h) * dzinv;
   }
 }
 
 // The reference clears the outputs over the padded image and fills the
 // interior (cal_advu_weno5.f:44-52): interior by the marching kernel, the
 // ghost shell zeroed separately, so every output element is written once.
-void advect_weno5(const Dims d, double dxinv, double dyinv, double dzinv, const double* __restrict__ u,
-                  const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ au,
-                  double* __restrict__ av, double* __restrict__ aw) {
+template <class T>
+void advect_weno5(const Dims d, double dxinv, double dyinv, double dzinv, const T* __restrict__ u,
+                  const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ au,
+                  T* __restrict__ av, T* __restrict__ aw) {
   const int gx = (d.ni + 63) / 64, gy = (d.nj + 3) / 4, gz = (d.nk + kWenoK - 1) / kWenoK;
-  weno_march_kernel<<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(d, dxinv, dyinv, dzinv, u, v,
-                                                                         w, au, av, aw, gx, gy);
+  weno_march_kernel<T><<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(
+      d, T(dxinv), T(dyinv), T(dzinv), u, v, w, au, av, aw, gx, gy);
   FEL_CHECK(hipGetLastError());
   launch_shell(d, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    au[c] = 0.0;
-    av[c] = 0.0;
-    aw[c] = 0.0;
+    au[c] = T(0.0);
+    av[c] = T(0.0);
+    aw[c] = T(0.0);
   });
 }
 
@@ -200,8 +220,8 @@ struct WenoHeld {  // what one output triple holds, valid while its versions are
 struct WenoStash {
   bool valid = false;
   WenoKey key{};
-  double* buf[3] = {nullptr, nullptr, nullptr};
-  size_t n = 0;
+  void* buf[3] = {nullptr, nullptr, nullptr};
+  size_t bytes = 0;
 };
 
 std::vector<WenoHeld> g_held;
@@ -233,6 +253,22 @@ void record_held(const void* const* out, size_t n, const WenoKey& k) {
 }
 
 }  // namespace
+
+template <class T>
+void weno_compute(const Dims d, double dxinv, double dyinv, double dzinv, const double* u,
+                  const double* v, const double* w, double* adv_u, double* adv_v, double* adv_w) {
+  Call c("cal_advu_weno5_");
+  const T* du = c.in_t<T>(u, d.n3, "u");
+  const T* dv = c.in_t<T>(v, d.n3, "v");
+  const T* dw = c.in_t<T>(w, d.n3, "w");
+  T* au = c.out_t<T>(adv_u, d.n3, "adv_u");
+  T* av = c.out_t<T>(adv_v, d.n3, "adv_v");
+  T* aw = c.out_t<T>(adv_w, d.n3, "adv_w");
+  prof_begin(2);
+  advect_weno5<T>(d, dxinv, dyinv, dzinv, du, dv, dw, au, av, aw);
+  prof_end(2);
+  c.finish();
+}
 
 void weno_release() {
   for (int m = 0; m < 3; m++)
@@ -274,36 +310,31 @@ void cal_advu_weno5_(const int* ni, const int* nj, const int* nk, const double* 
       g_stash.valid = h_valid;
       if (h_valid) g_stash.key = h->key;
       Call c("cal_advu_weno5_");  // the outputs change: fresh versions
-      for (int m = 0; m < 3; m++) c.out(out[m], n);
+      for (int m = 0; m < 3; m++) {
+        if (flow_fp32()) c.out_t<float>(out[m], n, "adv");
+        else c.out(out[m], n, "adv");
+      }
       c.finish();
       record_held(out, n, key);
       return;
     }
     if (h_valid) {
       // park what the outputs hold before overwriting them
-      if (g_stash.n != n) {
+      const size_t bytes = n * (size_t)elem_size(out[0]);
+      if (g_stash.bytes != bytes) {
         for (int m = 0; m < 3; m++) {
           if (g_stash.buf[m]) FEL_CHECK(hipFree(g_stash.buf[m]));
-          FEL_CHECK(hipMalloc((void**)&g_stash.buf[m], sizeof(double) * n));
+          FEL_CHECK(hipMalloc(&g_stash.buf[m], bytes));
         }
-        g_stash.n = n;
+        g_stash.bytes = bytes;
       }
       for (int m = 0; m < 3; m++) g_stash.buf[m] = exchange_device_buffer(out[m], n, g_stash.buf[m]);
       g_stash.valid = true;
       g_stash.key = h->key;
     }
   }
-  Call c("cal_advu_weno5_");
-  const double* du = c.in(u, d.n3, "u");
-  const double* dv = c.in(v, d.n3, "v");
-  const double* dw = c.in(w, d.n3, "w");
-  double* au = c.out(adv_u, d.n3, "adv_u");
-  double* av = c.out(adv_v, d.n3, "adv_v");
-  double* aw = c.out(adv_w, d.n3, "adv_w");
-  prof_begin(2);
-  advect_weno5(d, *dxinv, *dyinv, *dzinv, du, dv, dw, au, av, aw);
-  prof_end(2);
-  c.finish();
+  if (flow_fp32()) weno_compute<float>(d, *dxinv, *dyinv, *dzinv, u, v, w, adv_u, adv_v, adv_w);
+  else weno_compute<double>(d, *dxinv, *dyinv, *dzinv, u, v, w, adv_u, adv_v, adv_w);
   if (track) record_held(out, n, key);
 }
 

@@ -22,8 +22,9 @@ namespace {
 
 struct Range {
   uintptr_t host;
-  size_t bytes;
-  double* dev;
+  size_t bytes;  // host bytes (FP64 elements)
+  void* dev;
+  int esize;     // device element size: 8, or 4 for an FP32 copy
   // content version: a fresh number on every write (entry point output,
   // registration, upload); a whole-array copy hands the source's number to
   // the destination, so equal numbers mean equal contents
@@ -36,6 +37,7 @@ uint64_t next_version() { return ++g_version; }
 struct Runtime {
   bool init = false;
   int device = 0;
+  int flow_esize = 8;  // element size of arrays registered with felgpu_register_
   hipStream_t s = nullptr;
   std::map<uintptr_t, Range> reg;         // registered, keyed by host start
   std::map<uintptr_t, Range> transient;   // per-call mirrors, cached
@@ -91,8 +93,51 @@ void ensure_init() {
   FEL_CHECK(hipHostMalloc((void**)&r.pinned, 64 * sizeof(double), hipHostMallocDefault));
   FEL_CHECK(hipMalloc((void**)&r.dscal, 64 * 16 * sizeof(double)));
   FEL_CHECK(hipMemsetAsync(r.dscal, 0, 64 * 16 * sizeof(double), r.s));
+  const char* pr = std::getenv("FELGPU_PRECISION");
+  if (pr && (std::strcmp(pr, "fp32") == 0 || std::strcmp(pr, "32") == 0)) r.flow_esize = 4;
   r.init = true;
 }
+
+bool flow_fp32() {
+  ensure_init();
+  return R().flow_esize == 4;
+}
+
+int elem_size(const void* host) {
+  const Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
+  return g ? g->esize : 8;
+}
+
+namespace {
+
+// Host FP64 <-> device copies of n elements of size esize.
+void h2d(void* dev, const void* host, size_t n, int esize) {
+  Runtime& r = R();
+  if (esize == 8) {
+    FEL_CHECK(hipMemcpyAsync(dev, host, n * 8, hipMemcpyHostToDevice, r.s));
+    return;
+  }
+  std::vector<float> tmp(n);
+  const double* h = static_cast<const double*>(host);
+  for (size_t m = 0; m < n; m++) tmp[m] = (float)h[m];
+  FEL_CHECK(hipMemcpyAsync(dev, tmp.data(), n * 4, hipMemcpyHostToDevice, r.s));
+  FEL_CHECK(hipStreamSynchronize(r.s));
+}
+
+void d2h(void* host, const void* dev, size_t n, int esize) {
+  Runtime& r = R();
+  if (esize == 8) {
+    FEL_CHECK(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, r.s));
+    return;
+  }
+  std::vector<float> tmp(n);
+  FEL_CHECK(hipMemcpyAsync(tmp.data(), dev, n * 4, hipMemcpyDeviceToHost, r.s));
+  FEL_CHECK(hipStreamSynchronize(r.s));
+  double* h = static_cast<double*>(host);
+  for (size_t m = 0; m < n; m++) h[m] = (double)tmp[m];
+}
+
+}  // namespace
 
 hipStream_t stream() {
   ensure_init();
@@ -190,7 +235,7 @@ void deferred_coef_flush() {
   DeferredCoef& dc = deferred_coef();
   if (!dc.active) return;
   dc.active = false;
-  coef_vis(dc.d, dc.dxinv, dc.dyinv, dc.dzinv, dc.src_dev, dc.out_dev);
+  coef_vis_any(dc.d, dc.dxinv, dc.dyinv, dc.dzinv, dc.src_dev, dc.out_dev, dc.esize);
   for (int m = 0; m < 45; m++)
     if (Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(dc.out_host[m]), 1))
       g->ver = next_version();
@@ -207,13 +252,13 @@ void copy_content_version(const void* src, const void* dst, size_t count) {
   if (a && b) b->ver = a->ver;
 }
 
-double* exchange_device_buffer(const void* host, size_t count, double* buf) {
+void* exchange_device_buffer(const void* host, size_t count, void* buf) {
   Range* g = whole_range(host, count);
   if (!g) {
     std::fprintf(stderr, "felgpu: internal: buffer exchange on an unregistered range\n");
     std::abort();
   }
-  double* old = g->dev;
+  void* old = g->dev;
   g->dev = buf;
   return old;
 }
@@ -234,8 +279,9 @@ void deferred_coef_check(const void* host, bool write) {
   if (write && dc.src_host && p >= s && p < s + bytes) {
     // the viscosity is about to change: keep the values the coefficients
     // are a function of (16 B/cell) instead of materialising 45 arrays
-    double* snap = static_cast<double*>(scratch(7, bytes));
-    FEL_CHECK(hipMemcpyAsync(snap, dc.src_dev, bytes, hipMemcpyDeviceToDevice, stream()));
+    const size_t dbytes = (size_t)dc.esize * (size_t)dc.d.n3;
+    void* snap = scratch(7, dbytes);
+    FEL_CHECK(hipMemcpyAsync(snap, dc.src_dev, dbytes, hipMemcpyDeviceToDevice, stream()));
     dc.src_dev = snap;
     dc.src_host = nullptr;
   }
@@ -251,7 +297,7 @@ Call::~Call() {
   if (!finished_) finish();
 }
 
-double* Call::arr(const void* host, size_t count, Access a, const char* what) {
+void* Call::resolve(const void* host, size_t count, Access a, const char* what, int esize) {
   Runtime& r = R();
   const uintptr_t p = reinterpret_cast<uintptr_t>(host);
   const size_t bytes = count * sizeof(double);
@@ -261,13 +307,21 @@ double* Call::arr(const void* host, size_t count, Access a, const char* what) {
   }
   deferred_coef_check(host, (a & kWrite) != 0);
   if (Range* g = find_range(r.reg, p, bytes)) {
+    if (g->esize != esize) {
+      std::fprintf(stderr,
+                   "felgpu: %s: array %s is held in FP%d on the device, this call needs FP%d "
+                   "(register it with felgpu_register%s_)\n",
+                   name_, what, 8 * g->esize, 8 * esize, esize == 8 ? "_fp64" : "");
+      std::abort();
+    }
     if (a & kWrite) g->ver = next_version();
-    return g->dev + (p - g->host) / sizeof(double);
+    return static_cast<char*>(g->dev) + (p - g->host) / sizeof(double) * (size_t)esize;
   }
-  // per-call mirror
-  double* dev = nullptr;
+  // per-call mirror, in the element type the kernel reads
+  const size_t dbytes = count * (size_t)esize;
+  void* dev = nullptr;
   auto it = r.transient.find(p);
-  if (it != r.transient.end() && it->second.bytes >= bytes) {
+  if (it != r.transient.end() && it->second.bytes >= dbytes) {
     dev = it->second.dev;
   } else {
     if (it != r.transient.end()) {
@@ -275,16 +329,16 @@ double* Call::arr(const void* host, size_t count, Access a, const char* what) {
       FEL_CHECK(hipFree(it->second.dev));
       r.transient.erase(it);
     }
-    FEL_CHECK(hipMalloc((void**)&dev, bytes));
-    r.transient[p] = Range{p, bytes, dev, 0};
+    FEL_CHECK(hipMalloc(&dev, dbytes));
+    r.transient[p] = Range{p, dbytes, dev, esize, 0};
   }
-  FEL_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, r.s));
+  h2d(dev, host, count, esize);
   if (a & kWrite) {
     if (npend_ >= 64) {
       std::fprintf(stderr, "felgpu: %s: too many unregistered output arrays\n", name_);
       std::abort();
     }
-    pend_[npend_++] = Pending{const_cast<void*>(host), dev, bytes};
+    pend_[npend_++] = Pending{const_cast<void*>(host), dev, count, esize};
   }
   return dev;
 }
@@ -293,8 +347,7 @@ void Call::finish() {
   Runtime& r = R();
   finished_ = true;
   if (npend_ == 0) return;
-  for (int m = 0; m < npend_; m++)
-    FEL_CHECK(hipMemcpyAsync(pend_[m].host, pend_[m].dev, pend_[m].bytes, hipMemcpyDeviceToHost, r.s));
+  for (int m = 0; m < npend_; m++) d2h(pend_[m].host, pend_[m].dev, pend_[m].count, pend_[m].esize);
   FEL_CHECK(hipStreamSynchronize(r.s));
   npend_ = 0;
 }
@@ -360,19 +413,22 @@ void felgpu_init_(const int* device) {
   ensure_init();
 }
 
-// Makes [host, host+nbytes) device-resident.  upload != 0 copies the host
-// contents; upload == 0 zero-fills the device copy (the state of a freshly
-// allocated, never-written host array on this platform: large allocations
-// come from zero pages).
-void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload) {
+static void register_range(const void* host, const int64_t* nbytes, const int* upload, int esize) {
   ensure_init();
   Runtime& r = R();
   const uintptr_t p = reinterpret_cast<uintptr_t>(host);
   const size_t bytes = (size_t)*nbytes;
+  if (bytes % sizeof(double) != 0) {
+    std::fprintf(stderr, "felgpu: host range %p: size %zu is not a whole number of doubles\n",
+                 host, bytes);
+    std::abort();
+  }
+  const size_t n = bytes / sizeof(double);
   auto it = r.reg.find(p);
   if (it != r.reg.end()) {
-    if (it->second.bytes != bytes) {
-      std::fprintf(stderr, "felgpu: host range %p registered twice with different sizes\n", host);
+    if (it->second.bytes != bytes || it->second.esize != esize) {
+      std::fprintf(stderr, "felgpu: host range %p registered twice with a different size or "
+                           "precision\n", host);
       std::abort();
     }
   } else {
@@ -389,17 +445,49 @@ void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload
         std::abort();
       }
     }
-    double* dev = nullptr;
-    FEL_CHECK(hipMalloc((void**)&dev, bytes));
-    it = r.reg.emplace(p, Range{p, bytes, dev, 0}).first;
+    void* dev = nullptr;
+    FEL_CHECK(hipMalloc(&dev, n * (size_t)esize));
+    it = r.reg.emplace(p, Range{p, bytes, dev, esize, 0}).first;
   }
   it->second.ver = next_version();
   if (*upload)
-    FEL_CHECK(hipMemcpyAsync(it->second.dev, host, bytes, hipMemcpyHostToDevice, r.s));
+    h2d(it->second.dev, host, n, esize);
   else
-    FEL_CHECK(hipMemsetAsync(it->second.dev, 0, bytes, r.s));
+    FEL_CHECK(hipMemsetAsync(it->second.dev, 0, n * (size_t)esize, r.s));
   FEL_CHECK(hipStreamSynchronize(r.s));
 }
+
+// Makes [host, host+nbytes) device-resident.  upload != 0 copies the host
+// contents; upload == 0 zero-fills the device copy (the state of a freshly
+// allocated, never-written host array on this platform: large allocations
+// come from zero pages).  In the FP32 mode the device copy is FP32.
+void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload) {
+  ensure_init();
+  register_range(host, nbytes, upload, R().flow_esize);
+}
+
+// As felgpu_register_, with an FP64 device copy in either mode.
+void felgpu_register_fp64_(const void* host, const int64_t* nbytes, const int* upload) {
+  register_range(host, nbytes, upload, 8);
+}
+
+// Selects the device precision of arrays registered with felgpu_register_
+// (64 or 32); only while nothing is registered.
+void felgpu_set_precision_(const int* bits) {
+  ensure_init();
+  Runtime& r = R();
+  if (*bits != 32 && *bits != 64) {
+    std::fprintf(stderr, "felgpu: felgpu_set_precision_: %d bits is not 32 or 64\n", *bits);
+    std::abort();
+  }
+  if (!r.reg.empty()) {
+    std::fprintf(stderr, "felgpu: felgpu_set_precision_: arrays are registered\n");
+    std::abort();
+  }
+  r.flow_esize = *bits / 8;
+}
+
+int felgpu_precision_() { return flow_fp32() ? 32 : 64; }
 
 static Range& must_find(const void* host, const char* who) {
   Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
@@ -415,7 +503,7 @@ void felgpu_upload_(const void* host) {
   deferred_coef_check(host, true);
   Range& g = must_find(host, "felgpu_upload_");
   g.ver = next_version();
-  FEL_CHECK(hipMemcpyAsync(g.dev, (const void*)g.host, g.bytes, hipMemcpyHostToDevice, stream()));
+  h2d(g.dev, (const void*)g.host, g.bytes / sizeof(double), g.esize);
   FEL_CHECK(hipStreamSynchronize(stream()));
 }
 
@@ -423,7 +511,7 @@ void felgpu_upload_(const void* host) {
 void felgpu_download_(void* host) {
   deferred_coef_check(host, false);
   const Range& g = must_find(host, "felgpu_download_");
-  FEL_CHECK(hipMemcpyAsync((void*)g.host, g.dev, g.bytes, hipMemcpyDeviceToHost, stream()));
+  d2h((void*)g.host, g.dev, g.bytes / sizeof(double), g.esize);
   FEL_CHECK(hipStreamSynchronize(stream()));
 }
 
@@ -453,7 +541,8 @@ void felgpu_unregister_all_() {
 // own work on the library stream, e.g. the test harness).
 void* felgpu_device_ptr_(const void* host) {
   const Range& g = must_find(host, "felgpu_device_ptr_");
-  return g.dev + (reinterpret_cast<uintptr_t>(host) - g.host) / sizeof(double);
+  return static_cast<char*>(g.dev) +
+         (reinterpret_cast<uintptr_t>(host) - g.host) / sizeof(double) * (size_t)g.esize;
 }
 
 void felgpu_sync_() { sync(); }

@@ -3,6 +3,13 @@
 GpuRun(cfg) allocates the reference-layout host arrays in the Fortran
 driver, lets the caller store the initial state, makes everything
 device-resident and runs timesteps; fields are copied back on request.
+
+precision=64 (default) is the FP64 path, whose every result equals the
+reference's FP64 arithmetic; precision=32 is the library's FP32 mode (flow
+fields, stencils, Gauss-Seidel and FFT+TDMA in FP32; volume-fraction
+fluxes, mass sums, contact-line velocity and the pressure correction in
+FP64 with iterative refinement).  The mode is process-wide while the run
+is open and returns to FP64 when it closes.
 """
 from __future__ import annotations
 
@@ -15,8 +22,12 @@ from .config import RunConfig, initial_state
 
 
 class GpuRun:
-    def __init__(self, cfg: RunConfig, state: dict[str, np.ndarray] | None = None):
+    def __init__(self, cfg: RunConfig, state: dict[str, np.ndarray] | None = None,
+                 precision: int = 64):
+        if precision not in (32, 64):
+            raise ValueError(f"precision must be 32 or 64, not {precision}")
         self.cfg = cfg
+        self.precision = precision
         self.drv = _lib.driver()
         self.gpu = _lib.gpu()
         self.drv.fel_drv_field.restype = ctypes.c_void_p
@@ -27,6 +38,7 @@ class GpuRun:
         st = initial_state(cfg) if state is None else state
         for name, arr in st.items():
             self.view(name)[...] = arr
+        self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(precision)))
         self.drv.fel_drv_start()
 
     def view(self, name: str) -> np.ndarray:
@@ -59,6 +71,8 @@ class GpuRun:
         if self.drv is not None:
             self.drv.fel_drv_destroy()
             self.drv = None
+            if self.precision != 64:
+                self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(64)))
 
     def __del__(self):
         try:

@@ -97,6 +97,20 @@ contains
     call felgpu_register(a, nbytes, up)
   end subroutine reg
 
+  ! As reg, for the arrays that stay FP64 on the device in the library's
+  ! FP32 mode: the volume fraction and its fluxes, the contact-line maps,
+  ! the divergence and the pressure correction.
+  subroutine reg64(a, nelem, upload)
+    real(8), intent(in) :: a(*)
+    integer(c_int64_t), intent(in) :: nelem
+    integer, intent(in) :: upload
+    integer(c_int64_t) :: nbytes
+    integer(c_int) :: up
+    nbytes = nelem * 8_c_int64_t
+    up = upload
+    call felgpu_register_fp64(a, nbytes, up)
+  end subroutine reg64
+
   ! Allocates the run's arrays (srcs/allocate.h:7-165, the subset the step
   ! uses); they are zero on the host until the caller stores initial fields.
   subroutine fel_drv_create(cfg) bind(C, name="fel_drv_create")
@@ -246,7 +260,7 @@ contains
     call reg(uo, n3, 0); call reg(vo, n3, 0); call reg(wo, n3, 0)
     call reg(un, n3, 0); call reg(vn, n3, 0); call reg(wn, n3, 0)
     call reg(p, n3, 1); call reg(po, n3, 0); call reg(pn, n3, 0)
-    call reg(phat, n3, 0); call reg(dp, n3, 0); call reg(div, n3, 0)
+    call reg(phat, n3, 0); call reg64(dp, n3, 0); call reg64(div, n3, 0)
     call reg(rho, n3, 0); call reg(rmu, n3, 0); call reg(rhon, n3, 0); call reg(rmun, n3, 0)
     call reg(adv_u, n3, 0); call reg(adv_v, n3, 0); call reg(adv_w, n3, 0)
     call reg(adv_uo, n3, 0); call reg(adv_vo, n3, 0); call reg(adv_wo, n3, 0)
@@ -258,8 +272,8 @@ contains
     call reg(sum_fst_un, n3, 0); call reg(sum_fst_vn, n3, 0); call reg(sum_fst_wn, n3, 0)
     call reg(src_u, n3, 0); call reg(src_v, n3, 0); call reg(src_w, n3, 0)
     call reg(phix, n3, 0); call reg(phiy, n3, 0); call reg(phiz, n3, 0)
-    call reg(flphix, n3, 0); call reg(flphiy, n3, 0); call reg(flphiz, n3, 0)
-    call reg(phi, 2*n3, 1); call reg(phin, 2*n3, 0)
+    call reg64(flphix, n3, 0); call reg64(flphiy, n3, 0); call reg64(flphiz, n3, 0)
+    call reg64(phi, 2*n3, 1); call reg64(phin, 2*n3, 0)
     call reg(s, 6*n3, 0); call reg(tau, 6*n3, 0)
     call reg(rn_pp, 3*n3, 0); call reg(rn_12, 3*n3, 0); call reg(rn_13, 3*n3, 0); call reg(rn_23, 3*n3, 0)
     call reg(rnn_pp, 3*n3, 0); call reg(rnn_12, 3*n3, 0); call reg(rnn_13, 3*n3, 0); call reg(rnn_23, 3*n3, 0)
@@ -267,8 +281,8 @@ contains
     do m = 1, 45
       call reg(cf(-2,-2,-2,m), n3, 0)
     end do
-    call reg(theta_0_array, n3, 1); call reg(l1_array, n3, 1); call reg(l2_array, n3, 1)
-    call reg(zeta_array, n3, 1); call reg(theta_array, n3, 0)
+    call reg64(theta_0_array, n3, 1); call reg64(l1_array, n3, 1); call reg64(l2_array, n3, 1)
+    call reg64(zeta_array, n3, 1); call reg64(theta_array, n3, 0)
     call reg(phir_r, int(ni,c_int64_t)*nj*nk, 0); call reg(phir_i, int(ni,c_int64_t)*nj*nk, 0)
 
     ! srcs/main.f90:304-366

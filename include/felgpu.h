@@ -51,6 +51,22 @@ void felgpu_init_(const int *device);
  * allocatable after `allocate`, before the first solver call. */
 void felgpu_register_(const void *host, const int64_t *nbytes, const int *upload);
 
+/* As felgpu_register_, with an FP64 device copy in either precision mode.
+ * Used for the arrays that stay FP64 in the FP32 mode: phi, phin, the
+ * MTHINC fluxes, the contact-line maps (theta, theta_0, l1, l2, zeta), the
+ * divergence and the pressure correction dp. */
+void felgpu_register_fp64_(const void *host, const int64_t *nbytes, const int *upload);
+
+/* Device precision of arrays registered with felgpu_register_: *bits = 64
+ * (default; every result equals the reference's FP64 arithmetic) or 32 (the
+ * FP32 mode: flow fields, stencils, Gauss-Seidel and FFT+TDMA in FP32, with
+ * FP64 volume-fraction fluxes and mass sums, FP64 contact-line velocity and
+ * FP64 iterative refinement of the pressure correction).  Only while
+ * nothing is registered; FELGPU_PRECISION=fp32 sets the same at start-up.
+ * felgpu_precision_ returns the current setting. */
+void felgpu_set_precision_(const int *bits);
+int felgpu_precision_(void);
+
 /* Host -> device copy of the whole registered range containing `host`. */
 void felgpu_upload_(const void *host);
 
