@@ -50,13 +50,10 @@ def sor_pair_bytes(cfg) -> float:
     N = ni*nj*nk interior cells.  The pair updates N/4 cells of the swept
     component; their stencils touch every cell of the swept component, of the
     other two components and of the viscosity (coefficients are recomputed
-    from it), N reads each; the source is read and the swept component
-    written at the N/4 updated cells; rho and rho^{n+1} are read at each
-    updated cell and its partner in the component direction, which covers
-    N/2 cells for u and w and all N cells for v.  Hence 5.5 N (u, w) and
-    6.5 N (v) elements of 8 bytes."""
-    n = float(cfg.cells)
-    return 8.0 * n * (5.5 + 6.5 + 5.5) / 3.0
+    from it), N reads each; the source and the face density (computed once
+    per solve) are read and the swept component written at the N/4 updated
+    cells.  Hence 4.75 N elements of 8 bytes."""
+    return 8.0 * float(cfg.cells) * 4.75
 
 
 def measured_traffic(cfg):

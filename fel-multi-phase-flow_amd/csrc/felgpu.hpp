@@ -135,8 +135,9 @@ void copy_content_version(const void* src, const void* dst, size_t count);
 // Replaces the device buffer behind a whole registered array (same size and
 // element type) and returns the previous one; the caller owns it.
 void* exchange_device_buffer(const void* host, size_t count, void* buf);
-// Frees the advection result stash (called when all arrays are unregistered).
+// Free the result stashes (called when all arrays are unregistered).
 void weno_release();
+void rn_release();
 
 // A deferred producer: the 45 viscous coefficient arrays of
 // cal_arith_coef_vis_ are a pure function of the viscosity.  When all of

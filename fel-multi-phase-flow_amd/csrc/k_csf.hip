@@ -4,6 +4,7 @@
 // Reference: srcs/cal_grad_p2uvw.f, srcs/calrn.f, srcs/calrkap.f,
 // srcs/calfst.f (called from srcs/main.f90:545-570).
 #include <cmath>
+#include <cstdlib>
 
 #include "felgpu.hpp"
 
@@ -117,6 +118,118 @@ void calrn(const Dims d, double verysmall, int ip, int jp, int kp, const T* px, 
   FEL_CHECK(hipGetLastError());
 }
 
+// The four staggerings the timestep asks for (srcs/main.f90:549-552 and
+// 555-558) in one pass: (ip,jp,kp) = (1,1,1), (0,0,1), (0,1,0), (1,0,0).
+// Every staggering's 8 source points lie in the 2x2x2 block of cells
+// (ii-1..ii, jj-1..jj, kk-1..kk) of the target (ii,jj,kk); the block's
+// k-1 plane is the previous iteration's k plane.  Each staggering sums its
+// 8 values in calrn_kernel's order, so the four outputs are identical to
+// four separate calls.
+constexpr int kStag[4][3] = {{1, 1, 1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
+
+template <class T>
+__global__ void __launch_bounds__(256) calrn4_kernel(Dims d, T verysmall, const T* __restrict__ px,
+                                                     const T* __restrict__ py,
+                                                     const T* __restrict__ pz, T* __restrict__ r0,
+                                                     T* __restrict__ r1, T* __restrict__ r2,
+                                                     T* __restrict__ r3, int gx, int gy) {
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
+  const int ii = -2 + bx * 64 + (int)threadIdx.x;
+  const int jj = -2 + by * 4 + (int)threadIdx.y;
+  if (ii > d.ni + 3 || jj > d.nj + 3) return;
+  const int n3 = d.n3, X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const int k0 = -2 + bz * kRnChunk, k1 = min(d.nk + 3, k0 + kRnChunk - 1);
+  const bool xm = ii >= -1, ym = jj >= -1;  // the i-1 / j-1 neighbours exist
+  T* out[4] = {r0, r1, r2, r3};
+  const T* a3[3] = {px, py, pz};
+  // V[m][a][b][c]: array m at (ii-1+a, jj-1+b, kk-1+c)
+  T V[3][2][2][2];
+  auto load_plane = [&](int kk, int cpl) {
+    const int c = d.ix(ii, jj, kk);
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const T* p = a3[m];
+      V[m][1][1][cpl] = p[c];
+      V[m][0][1][cpl] = xm ? p[c - X] : T(0.0);
+      V[m][1][0][cpl] = ym ? p[c - Y] : T(0.0);
+      V[m][0][0][cpl] = (xm && ym) ? p[c - X - Y] : T(0.0);
+    }
+  };
+  if (k0 - 1 >= -2) {
+    load_plane(k0 - 1, 1);
+  }
+  for (int kk = k0; kk <= k1; kk++) {
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) V[m][a][b][0] = V[m][a][b][1];
+    load_plane(kk, 1);
+    const int tgt = d.ix(ii, jj, kk);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int ip = kStag[s][0], jp = kStag[s][1], kp = kStag[s][2];
+      T* rn = out[s];
+      if (ii < -2 + ip || jj < -2 + jp || kk < -2 + kp) {
+        rn[tgt] = T(0.0);
+        rn[n3 + tgt] = T(0.0);
+        rn[2 * n3 + tgt] = T(0.0);
+        continue;
+      }
+      const int ai = 1 - ip, bj = 1 - jp, cp = 1 - kp;
+      T r[3];
+#pragma unroll
+      for (int m = 0; m < 3; m++)
+        r[m] = (+V[m][ai][bj][cp] + V[m][1][bj][cp] + V[m][ai][1][cp] + V[m][1][1][cp] +
+                V[m][ai][bj][1] + V[m][1][bj][1] + V[m][ai][1][1] + V[m][1][1][1]) *
+               T(0.125);
+      const T arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
+      const T inv = T(1.0) / arn;
+      rn[tgt] = r[0] * inv;
+      rn[n3 + tgt] = r[1] * inv;
+      rn[2 * n3 + tgt] = r[2] * inv;
+    }
+  }
+}
+
+template <class T>
+void calrn4(const Dims d, double verysmall, const T* px, const T* py, const T* pz, T* const* rn) {
+  const int gx = (d.sx + 63) / 64, gy = (d.sy + 3) / 4, gz = (d.sz + kRnChunk - 1) / kRnChunk;
+  calrn4_kernel<T><<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(
+      d, T(verysmall), px, py, pz, rn[0], rn[1], rn[2], rn[3], gx, gy);
+  FEL_CHECK(hipGetLastError());
+}
+
+// Normals of the other three staggerings, computed by the pass that
+// answered the first request and parked until they are asked for: a request
+// whose inputs carry the same content versions gets the parked buffer
+// swapped in (the same bits, no kernel).
+namespace {
+struct RnStash {
+  bool valid[4] = {false, false, false, false};
+  uint64_t ver[3] = {0, 0, 0};
+  int ni = 0, nj = 0, nk = 0;
+  double vs = 0.0;
+  void* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t bytes = 0;
+};
+RnStash g_rn;
+
+int stagger_index(int ip, int jp, int kp) {
+  for (int s = 0; s < 4; s++)
+    if (kStag[s][0] == ip && kStag[s][1] == jp && kStag[s][2] == kp) return s;
+  return -1;
+}
+}  // namespace
+
+void rn_release() {
+  for (int s = 0; s < 4; s++)
+    if (g_rn.buf[s]) FEL_CHECK(hipFree(g_rn.buf[s]));
+  g_rn = RnStash{};
+}
+
 // srcs/calrkap.f:20-97
 template <class T>
 void calrkap(const Dims d, double dxinv_, double dyinv_, double dzinv_, const T* rpp, const T* r12,
@@ -181,6 +294,46 @@ void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const doubl
 template <class T>
 void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix, const double* phiy,
              const double* phiz, double* rn) {
+  const size_t n = (size_t)d.n3, n3x = 3 * n;
+  const int sreq = stagger_index(ip, jp, kp);
+  const uint64_t ver[3] = {content_version(phix, n), content_version(phiy, n),
+                           content_version(phiz, n)};
+  const bool track = sreq >= 0 && ver[0] && ver[1] && ver[2] && content_version(rn, n3x) != 0 &&
+                     std::getenv("FELGPU_NO_RESULT_REUSE") == nullptr;
+  if (track) {
+    const bool same = g_rn.ver[0] == ver[0] && g_rn.ver[1] == ver[1] && g_rn.ver[2] == ver[2] &&
+                      g_rn.ni == d.ni && g_rn.nj == d.nj && g_rn.nk == d.nk && g_rn.vs == vs;
+    if (same && g_rn.valid[sreq]) {
+      g_rn.buf[sreq] = exchange_device_buffer(rn, n3x, g_rn.buf[sreq]);
+      g_rn.valid[sreq] = false;
+      Call c("calrn_");  // the output changes: a fresh version
+      c.out_t<T>(rn, n3x, "rn");
+      c.finish();
+      return;
+    }
+    const size_t bytes = n3x * sizeof(T);
+    if (g_rn.bytes != bytes) {
+      for (int s = 0; s < 4; s++) {
+        if (g_rn.buf[s]) FEL_CHECK(hipFree(g_rn.buf[s]));
+        FEL_CHECK(hipMalloc(&g_rn.buf[s], bytes));
+      }
+      g_rn.bytes = bytes;
+    }
+    Call c("calrn_");
+    const T* x = c.in_t<T>(phix, n, "phix");
+    const T* y = c.in_t<T>(phiy, n, "phiy");
+    const T* z = c.in_t<T>(phiz, n, "phiz");
+    T* out[4];
+    for (int s = 0; s < 4; s++)
+      out[s] = s == sreq ? c.out_t<T>(rn, n3x, "rn") : static_cast<T*>(g_rn.buf[s]);
+    calrn4<T>(d, vs, x, y, z, out);
+    c.finish();
+    for (int s = 0; s < 4; s++) g_rn.valid[s] = s != sreq;
+    for (int m = 0; m < 3; m++) g_rn.ver[m] = ver[m];
+    g_rn.ni = d.ni; g_rn.nj = d.nj; g_rn.nk = d.nk;
+    g_rn.vs = vs;
+    return;
+  }
   Call c("calrn_");
   const T* x = c.in_t<T>(phix, d.n3, "phix");
   const T* y = c.in_t<T>(phiy, d.n3, "phiy");

@@ -27,6 +27,7 @@ namespace fel {
 template <class T>
 struct SorArgs {
   const T *rho, *rhon, *src[3], *rmu;
+  const T* r00[3];  // face densities (rho+rho'+rhon+rhon')/4 of u, v, w, computed once per solve
   const T* cf[45];
   T *un, *vn, *wn;
   T dtinv;
@@ -67,9 +68,9 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
       for (int m = 0; m < 15; m++) cf[m] = a.cf[15 * COMP + m][c];
     }
     const T *un = a.un, *vn = a.vn, *wn = a.wn;
-    const T q = T(0.25), h = T(0.5);
+    const T h = T(0.5);
+    const T r00 = a.r00[COMP][c];
     if (COMP == 0) {
-      const T r00 = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q;
       const T visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
                            cf[1] * un[c + X] + cf[3] * un[c + Y] + cf[5] * un[c + Z] +
                            cf[7] * vn[c - Y] + cf[8] * vn[c + X - Y] + cf[9] * vn[c] +
@@ -80,7 +81,6 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
       a.un[c] = un[c] + du;
       e += (double)du * (double)du;
     } else if (COMP == 1) {
-      const T r00 = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q;
       const T visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
                            cf[1] * vn[c + X] + cf[3] * vn[c + Y] + cf[5] * vn[c + Z] +
                            cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Y] +
@@ -91,7 +91,6 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
       a.vn[c] = vn[c] + dv;
       e += (double)dv * (double)dv;
     } else {
-      const T r00 = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q;
       const T visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
                            cf[1] * wn[c + X] + cf[3] * wn[c + Y] + cf[5] * wn[c + Z] +
                            cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Z] +
@@ -253,6 +252,23 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
   a.vn = c.io_t<T>(vn, d.n3, "vn");
   a.wn = c.io_t<T>(wn, d.n3, "wn");
   a.dtinv = T(1.0 / dt);
+  // srcs/solu_sor4.f:150-152 (and the v, w forms): the face density of each
+  // update, the same expression as the sweeps evaluate, once for the solve
+  {
+    T* r00[3];
+    for (int m = 0; m < 3; m++) {
+      r00[m] = static_cast<T*>(scratch(10 + m, sizeof(T) * (size_t)d.n3));
+      a.r00[m] = r00[m];
+    }
+    const T *rho_d = a.rho, *rhon_d = a.rhon;
+    launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
+      const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+      const T q = T(0.25);
+      r00[0][c] = (rho_d[c] + rho_d[c + X] + rhon_d[c] + rhon_d[c + X]) * q;
+      r00[1][c] = (rho_d[c] + rho_d[c + Y] + rhon_d[c] + rhon_d[c + Y]) * q;
+      r00[2][c] = (rho_d[c] + rho_d[c + Z] + rhon_d[c] + rhon_d[c + Z]) * q;
+    });
+  }
   const double* dl1 = c.in(l1, d.n3, "l1");
   const double* dl2 = c.in(l2, d.n3, "l2");
   const double* dphi = c.in(phi, d.n3, "phi");

@@ -531,6 +531,7 @@ void felgpu_unregister_all_() {
   deferred_coef_flush();
   FEL_CHECK(hipStreamSynchronize(r.s));
   weno_release();
+  rn_release();
   for (auto& kv : r.reg) FEL_CHECK(hipFree(kv.second.dev));
   r.reg.clear();
   for (auto& kv : r.transient) FEL_CHECK(hipFree(kv.second.dev));

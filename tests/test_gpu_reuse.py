@@ -1,12 +1,14 @@
-"""Result reuse of cal_advu_weno5_ on device-resident arrays.
+"""Result reuse on device-resident arrays.
 
 The timestep advects (u,v,w) and then (uo,vo,wo), where uo..wo are cpy_
-copies of the previous u..w (srcs/main.f90:532-533, 686-688).  The library
-recognises through content versions that the second request is the first
-request of the previous step and swaps in the parked result.  These tests
-drive that sequence through the C ABI on registered host arrays and check
-every result bit for bit against the oracle, including the cases where an
-input changed after the copy and the result must be recomputed.
+copies of the previous u..w (srcs/main.f90:532-533, 686-688), and asks for
+the unit normals of the same gradients at four staggerings in a row
+(srcs/main.f90:549-558).  The library recognises through content versions
+that a request repeats an earlier computation (or one computed alongside
+it) and swaps in the parked result.  These tests drive those sequences
+through the C ABI on registered host arrays and check every result bit for
+bit against the oracle, including the cases where an input changed and the
+result must be recomputed.
 """
 from __future__ import annotations
 
@@ -127,5 +129,48 @@ def test_host_upload_invalidates_parked_result():
         reg.download(wo)
         for got, want in zip(advo, weno_oracle(g, uo, vo, wo)):
             assert np.array_equal(got, want)
+    finally:
+        reg.close()
+
+
+def test_four_staggered_normals_from_one_pass_are_exact():
+    """calrn_ computes the four staggerings the timestep asks for in one pass
+    and parks three of them; the four requests must still return exactly the
+    oracle's values, also when the inputs change between two groups."""
+    g = Grid(32, 16, 16, 0.1, 0.05, 0.1)
+    r = np.random.default_rng(42)
+    reg = Registered()
+    try:
+        px, py, pz = (reg.add(r.standard_normal(g.shape)) for _ in range(3))
+        outs = [reg.add(g.zeros(3)) for _ in range(4)]
+        dims = [I(g.ni), I(g.nj), I(g.nk)]
+        inv = [D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+        stag = [(1, 1, 1), (0, 0, 1), (0, 1, 0), (1, 0, 0)]
+        x0 = [a.copy() for a in (px, py, pz)]
+        for rnd in range(2):
+            for (ip, jp, kp), o in zip(stag, outs):
+                H.call_gpu("calrn", *dims, *inv, I(ip), I(jp), I(kp), px, py, pz, o)
+            for o in outs:
+                reg.download(o)
+            for (ip, jp, kp), o in zip(stag, outs):
+                want = g.zeros(3)
+                H.call_oracle("calrn", *dims, *inv, I(ip), I(jp), I(kp), *[a.copy() for a in x0],
+                              want)
+                assert np.array_equal(o, want), (rnd, ip, jp, kp)
+            # new inputs (a host upload): the next group must be recomputed
+            x0 = [r.standard_normal(g.shape) for _ in range(3)]
+            for a, b in zip((px, py, pz), x0):
+                a[...] = b
+                H.gpu().felgpu_upload_(ctypes.c_void_p(a.ctypes.data))
+        # out-of-order and repeated requests
+        for (ip, jp, kp), o in reversed(list(zip(stag, outs))):
+            H.call_gpu("calrn", *dims, *inv, I(ip), I(jp), I(kp), px, py, pz, o)
+        H.call_gpu("calrn", *dims, *inv, I(0), I(1), I(0), px, py, pz, outs[0])
+        for o in outs:
+            reg.download(o)
+        for (ip, jp, kp), o in zip([(0, 1, 0)] + stag[1:], outs):
+            want = g.zeros(3)
+            H.call_oracle("calrn", *dims, *inv, I(ip), I(jp), I(kp), *[a.copy() for a in x0], want)
+            assert np.array_equal(o, want), ("reordered", ip, jp, kp)
     finally:
         reg.close()

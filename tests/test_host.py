@@ -52,5 +52,5 @@ def test_sor_pair_algorithmic_bytes():
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
     cfg = C.droplet_impact()
-    # (5.5 + 6.5 + 5.5)/3 elements of 8 bytes per interior cell
-    assert abs(bench.sor_pair_bytes(cfg) - 8.0 * cfg.cells * 17.5 / 3.0) < 1.0
+    # 4.75 elements of 8 bytes per interior cell
+    assert abs(bench.sor_pair_bytes(cfg) - 8.0 * cfg.cells * 4.75) < 1.0
