@@ -16,6 +16,7 @@
 
 // The public C ABI: every extern "C" definition is checked against it.
 #include "felgpu.h"
+#include "felgpu_dual.hpp"
 
 #define FEL_CHECK(call)                                                                 \
   do {                                                                                  \
@@ -67,9 +68,29 @@ enum Access { kRead = 1, kWrite = 2, kReadWrite = 3 };
 // maps, the divergence and the pressure correction -- stay FP64.  Host
 // arrays are FP64 in both modes; uploads and downloads convert.
 bool flow_fp32();
-// Element size (4 or 8) of the device copy of the array containing `host`;
-// 8 for an unregistered array.
+// The tangent mode (felgpu_set_tangent_ before the first registration):
+// every device copy holds an FP64 value and its derivative along one input
+// direction (Dual, 16 bytes); the kernels run with T = PT<T> = Dual.
+bool tangent_mode();
+// Element size (4, 8 or 16) of the device copy of the array containing
+// `host`; 8 for an unregistered array.
 int elem_size(const void* host);
+
+// Runs f(T{}) with T the flow element type of the current mode: double,
+// float (FP32 mode) or Dual (tangent mode).
+template <class F>
+inline void with_flow(F&& f) {
+  if (tangent_mode()) f(Dual{});
+  else if (flow_fp32()) f(float{});
+  else f(double{});
+}
+// Runs f(T{}) with T the element type of the given element size.
+template <class F>
+inline void with_elem(int esize, F&& f) {
+  if (esize == 16) f(Dual{});
+  else if (esize == 4) f(float{});
+  else f(double{});
+}
 
 // Resolves the host array arguments of one drop-in call to device memory.
 // Arrays registered with felgpu_register_ are device-resident: the device
@@ -173,11 +194,11 @@ void sync();
 void* scratch(int slot, size_t bytes);
 
 // A positive guard constant of the reference (1e-99, 1e-80) in element type
-// T: the value itself in FP64; in FP32, where it would flush to zero, 1e-30
-// (still far below every quantity it guards).
+// T: the value itself in FP64 (and in the tangent mode); in FP32, where it
+// would flush to zero, 1e-30 (still far below every quantity it guards).
 template <class T>
 __host__ __device__ constexpr T guard(double v) {
-  return sizeof(T) == 8 ? T(v) : T(v < 1.0e-30 ? 1.0e-30 : v);
+  return is_f32<T> ? T(v < 1.0e-30 ? 1.0e-30 : v) : T(v);
 }
 
 // Launch helpers --------------------------------------------------------------
@@ -267,26 +288,28 @@ inline void launch_shell(const Dims& d, F f) {
   });
 }
 
-// Deterministic FP64 reductions over the interior cells [1,n]^3.  Stage 1:
-// fixed grid, each thread sums a fixed strided subset, fixed LDS tree per
-// block.  Stage 2: one block sums the block partials in fixed order and
-// hands the NV totals to g(total, out), which runs on the device, so a
-// following kernel can consume derived scalars without a host round trip.
+// Deterministic reductions over the interior cells [1,n]^3, accumulated in
+// A (double; Dual in the tangent mode, whose value part then adds in the
+// same order as the FP64 sum).  Stage 1: fixed grid, each thread sums a
+// fixed strided subset, fixed LDS tree per block.  Stage 2: one block sums
+// the block partials in fixed order and hands the NV totals to g(total,
+// out), which runs on the device, so a following kernel can consume derived
+// scalars without a host round trip.
 constexpr int kRedBlocks = 1024;
 
-template <int NV, class F>
-__global__ void __launch_bounds__(256) sum_stage1(Dims d, F f, double* part) {
-  __shared__ double sh[NV][256];
-  double acc[NV];
+template <int NV, class A, class F>
+__global__ void __launch_bounds__(256) sum_stage1(Dims d, F f, A* part) {
+  __shared__ A sh[NV][256];
+  A acc[NV];
 #pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = 0.0;
+  for (int c = 0; c < NV; c++) acc[c] = A(0.0);
   const int nxy = d.ni * d.nj, n = nxy * d.nk;
   for (int m = (int)(blockIdx.x * 256 + threadIdx.x); m < n; m += (int)(gridDim.x * 256)) {
     const int k = m / nxy, r = m - k * nxy, j = r / d.ni, i = r - j * d.ni;
-    double v[NV];
+    A v[NV];
     f(i + 1, j + 1, k + 1, v);
 #pragma unroll
-    for (int c = 0; c < NV; c++) acc[c] += v[c];
+    for (int c = 0; c < NV; c++) acc[c] = acc[c] + v[c];
   }
   const int t = (int)threadIdx.x;
 #pragma unroll
@@ -295,7 +318,7 @@ __global__ void __launch_bounds__(256) sum_stage1(Dims d, F f, double* part) {
   for (int s = 128; s > 0; s >>= 1) {
     if (t < s) {
 #pragma unroll
-      for (int c = 0; c < NV; c++) sh[c][t] += sh[c][t + s];
+      for (int c = 0; c < NV; c++) sh[c][t] = sh[c][t] + sh[c][t + s];
     }
     __syncthreads();
   }
@@ -305,16 +328,16 @@ __global__ void __launch_bounds__(256) sum_stage1(Dims d, F f, double* part) {
   }
 }
 
-template <int NV, class G>
-__global__ void __launch_bounds__(256) sum_stage2(int nb, const double* part, double* out, G g) {
-  __shared__ double sh[NV][256];
+template <int NV, class A, class G>
+__global__ void __launch_bounds__(256) sum_stage2(int nb, const A* part, double* out, G g) {
+  __shared__ A sh[NV][256];
   const int t = (int)threadIdx.x;
-  double acc[NV];
+  A acc[NV];
 #pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = 0.0;
+  for (int c = 0; c < NV; c++) acc[c] = A(0.0);
   for (int b = t; b < nb; b += 256) {
 #pragma unroll
-    for (int c = 0; c < NV; c++) acc[c] += part[b * NV + c];
+    for (int c = 0; c < NV; c++) acc[c] = acc[c] + part[b * NV + c];
   }
 #pragma unroll
   for (int c = 0; c < NV; c++) sh[c][t] = acc[c];
@@ -322,25 +345,26 @@ __global__ void __launch_bounds__(256) sum_stage2(int nb, const double* part, do
   for (int s = 128; s > 0; s >>= 1) {
     if (t < s) {
 #pragma unroll
-      for (int c = 0; c < NV; c++) sh[c][t] += sh[c][t + s];
+      for (int c = 0; c < NV; c++) sh[c][t] = sh[c][t] + sh[c][t + s];
     }
     __syncthreads();
   }
   if (t == 0) {
-    double tot[NV];
+    A tot[NV];
 #pragma unroll
     for (int c = 0; c < NV; c++) tot[c] = sh[c][0];
     g(tot, out);
   }
 }
 
-// Runs both stages; `out` is device memory.
-template <int NV, class F, class G>
+// Runs both stages; `out` is device memory.  f(i, j, k, A* v) gives the NV
+// addends of cell (i,j,k); g(const A* totals, double* out) stores results.
+template <int NV, class A = double, class F, class G>
 inline void interior_reduce(const Dims& d, F f, G g, double* out) {
-  double* part = static_cast<double*>(scratch(15, sizeof(double) * NV * kRedBlocks));
-  sum_stage1<NV, F><<<dim3(kRedBlocks), dim3(256), 0, stream()>>>(d, f, part);
+  A* part = static_cast<A*>(scratch(15, sizeof(A) * NV * kRedBlocks));
+  sum_stage1<NV, A, F><<<dim3(kRedBlocks), dim3(256), 0, stream()>>>(d, f, part);
   FEL_CHECK(hipGetLastError());
-  sum_stage2<NV, G><<<dim3(1), dim3(256), 0, stream()>>>(kRedBlocks, (const double*)part, out, g);
+  sum_stage2<NV, A, G><<<dim3(1), dim3(256), 0, stream()>>>(kRedBlocks, (const A*)part, out, g);
   FEL_CHECK(hipGetLastError());
 }
 

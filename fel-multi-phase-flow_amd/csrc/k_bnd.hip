@@ -12,25 +12,26 @@
 namespace fel {
 
 // srcs/bndu.f:29-96: wall rows, one thread per (i,k) of the padded plane.
-// T: velocity element type; the slip coefficients are FP64 and each ghost
-// value is formed in FP64 and rounded once to T.
-template <class T>
-static void bndu_walls(const Dims d, T* uk, T* vk, T* wk, double uwall, double dy, const double* l1,
-                       const double* l2, const double* phi, bool lo, bool hi) {
+// T: velocity element type; the slip coefficients are formed in the phase
+// element type P (FP64, or Dual in the tangent mode) and each ghost value is
+// formed in P and rounded once to T.
+template <class T, class P = PT<T>>
+static void bndu_walls(const Dims d, T* uk, T* vk, T* wk, double uwall, double dy, const P* l1,
+                       const P* l2, const P* phi, bool lo, bool hi) {
   const int ni = d.ni, nj = d.nj;
   launch_box(Box{-2, d.ni + 3, 0, 0, -2, d.nk + 3}, [=] __device__(int i, int, int k) {
     const int ip = (i + 1) % (ni + 3);  // Fortran mod(i+1,ni+3), sign of the dividend
     if (lo) {
       const int c1 = d.ix(i, 1, k);
-      const double phi_av = (phi[c1] + phi[d.ix(ip, 1, k)]) / 2.0;
-      const double inv1 = phi_av / l1[c1];
-      const double inv2 = (1.0 - phi_av) / l2[c1];
-      const double ls = 1.0 / (inv1 + inv2);
-      const double coef1 = (2.0 * dy) / (2.0 * ls + dy);
-      const double coef2 = (2.0 * ls - dy) / (2.0 * ls + dy);
+      const P phi_av = (phi[c1] + phi[d.ix(ip, 1, k)]) / 2.0;
+      const P inv1 = phi_av / l1[c1];
+      const P inv2 = (1.0 - phi_av) / l2[c1];
+      const P ls = 1.0 / (inv1 + inv2);
+      const P coef1 = (2.0 * dy) / (2.0 * ls + dy);
+      const P coef2 = (2.0 * ls - dy) / (2.0 * ls + dy);
       const int c0 = d.ix(i, 0, k), cm = d.ix(i, -1, k), cmm = d.ix(i, -2, k), c2 = d.ix(i, 2, k);
       const T u1 = uk[c1];
-      const T u0 = T(coef1 * (-uwall) + coef2 * (double)u1);
+      const T u0 = T(coef1 * (-uwall) + coef2 * P(u1));
       const T um = T(2.0) * u0 - u1;
       uk[c0] = u0;
       uk[cm] = um;
@@ -39,7 +40,7 @@ static void bndu_walls(const Dims d, T* uk, T* vk, T* wk, double uwall, double d
       vk[cm] = -vk[c1];
       vk[c0] = T(0.0);
       const T w1 = wk[c1];
-      const T w0 = T(coef2 * (double)w1);
+      const T w0 = T(coef2 * P(w1));
       const T wm = T(2.0) * w0 - w1;
       wk[c0] = w0;
       wk[cm] = wm;
@@ -47,15 +48,15 @@ static void bndu_walls(const Dims d, T* uk, T* vk, T* wk, double uwall, double d
     }
     if (hi) {
       const int cn = d.ix(i, nj, k);
-      const double phi_av = (phi[cn] + phi[d.ix(ip, nj, k)]) / 2.0;
-      const double inv1 = phi_av / l1[cn];
-      const double inv2 = (1.0 - phi_av) / l2[cn];
-      const double ls = 1.0 / (inv1 + inv2);
-      const double coef1 = (2.0 * dy) / (2.0 * ls + dy);
-      const double coef2 = (2.0 * ls - dy) / (2.0 * ls + dy);
+      const P phi_av = (phi[cn] + phi[d.ix(ip, nj, k)]) / 2.0;
+      const P inv1 = phi_av / l1[cn];
+      const P inv2 = (1.0 - phi_av) / l2[cn];
+      const P ls = 1.0 / (inv1 + inv2);
+      const P coef1 = (2.0 * dy) / (2.0 * ls + dy);
+      const P coef2 = (2.0 * ls - dy) / (2.0 * ls + dy);
       const int Y = d.sx;
       const T un = uk[cn];
-      const T u1 = T(coef1 * uwall + coef2 * (double)un);
+      const T u1 = T(coef1 * uwall + coef2 * P(un));
       const T u2 = T(2.0) * u1 - un;
       uk[cn + Y] = u1;
       uk[cn + 2 * Y] = u2;
@@ -65,7 +66,7 @@ static void bndu_walls(const Dims d, T* uk, T* vk, T* wk, double uwall, double d
       vk[cn + 2 * Y] = -vk[cn - 2 * Y];
       vk[cn + 3 * Y] = -vk[cn - 3 * Y];
       const T wn = wk[cn];
-      const T w1 = T(coef2 * (double)wn);
+      const T w1 = T(coef2 * P(wn));
       const T w2 = T(2.0) * w1 - wn;
       wk[cn + Y] = w1;
       wk[cn + 2 * Y] = w2;
@@ -132,9 +133,9 @@ static void bndu_z(const Dims d, T* uk, T* vk, T* wk) {
   });
 }
 
-template <class T>
-void bndu_phased(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
-                 const double* l2, const double* phi, bool lo, bool hi) {
+template <class T, class P>
+void bndu_phased(const Dims d, T* u, T* v, T* w, double uwall, double dy, const P* l1,
+                 const P* l2, const P* phi, bool lo, bool hi) {
   bndu_walls<T>(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi);
   bndu_x<T>(d, u, v, w);
   bndu_z<T>(d, u, v, w);
@@ -160,27 +161,28 @@ void bndu_phased(const Dims d, T* u, T* v, T* w, double uwall, double dy, const 
 // which bndu never writes), so one thread per written cell reproduces the
 // reference's values, corner cells included.
 // ---------------------------------------------------------------------------
+template <class P>
 struct WallCoef {
-  double c1, c2;
+  P c1, c2;
 };
 
-__device__ __forceinline__ WallCoef wall_coef(const Dims& d, int i, int jw, int k, double dy,
-                                              const double* l1, const double* l2,
-                                              const double* phi) {
+template <class P>
+__device__ __forceinline__ WallCoef<P> wall_coef(const Dims& d, int i, int jw, int k, double dy,
+                                                 const P* l1, const P* l2, const P* phi) {
   const int ip = (i + 1) % (d.ni + 3);
   const int cw = d.ix(i, jw, k);
-  const double phi_av = (phi[cw] + phi[d.ix(ip, jw, k)]) / 2.0;
-  const double inv1 = phi_av / l1[cw];
-  const double inv2 = (1.0 - phi_av) / l2[cw];
-  const double ls = 1.0 / (inv1 + inv2);
-  return WallCoef{(2.0 * dy) / (2.0 * ls + dy), (2.0 * ls - dy) / (2.0 * ls + dy)};
+  const P phi_av = (phi[cw] + phi[d.ix(ip, jw, k)]) / 2.0;
+  const P inv1 = phi_av / l1[cw];
+  const P inv2 = (1.0 - phi_av) / l2[cw];
+  const P ls = 1.0 / (inv1 + inv2);
+  return WallCoef<P>{(2.0 * dy) / (2.0 * ls + dy), (2.0 * ls - dy) / (2.0 * ls + dy)};
 }
 
 // Wall phase value of component q at (i,j,k); q: 0=u, 1=v, 2=w.
-template <class T>
+template <class T, class P>
 __device__ __forceinline__ T wall_value(const Dims& d, int q, const T* f, int i, int j, int k,
-                                        double uwall, double dy, const double* l1,
-                                        const double* l2, const double* phi, bool lo, bool hi) {
+                                        double uwall, double dy, const P* l1, const P* l2,
+                                        const P* phi, bool lo, bool hi) {
   const int nj = d.nj;
   if (q == 1) {
     if (lo && j <= 0) {
@@ -194,18 +196,18 @@ __device__ __forceinline__ T wall_value(const Dims& d, int q, const T* f, int i,
     return f[d.ix(i, j, k)];
   }
   if (lo && j <= 0) {
-    const WallCoef cf = wall_coef(d, i, 1, k, dy, l1, l2, phi);
+    const WallCoef<P> cf = wall_coef(d, i, 1, k, dy, l1, l2, phi);
     const T f1 = f[d.ix(i, 1, k)];
-    const T g0 = T((q == 0) ? cf.c1 * (-uwall) + cf.c2 * (double)f1 : cf.c2 * (double)f1);
+    const T g0 = T((q == 0) ? cf.c1 * (-uwall) + cf.c2 * P(f1) : cf.c2 * P(f1));
     if (j == 0) return g0;
     const T gm = T(2.0) * g0 - f1;
     if (j == -1) return gm;
     return T(2.0) * gm - g0;
   }
   if (hi && j >= nj + 1) {
-    const WallCoef cf = wall_coef(d, i, nj, k, dy, l1, l2, phi);
+    const WallCoef<P> cf = wall_coef(d, i, nj, k, dy, l1, l2, phi);
     const T fn = f[d.ix(i, nj, k)];
-    const T g1 = T((q == 0) ? cf.c1 * uwall + cf.c2 * (double)fn : cf.c2 * (double)fn);
+    const T g1 = T((q == 0) ? cf.c1 * uwall + cf.c2 * P(fn) : cf.c2 * P(fn));
     if (j == nj + 1) return g1;
     const T g2 = T(2.0) * g1 - fn;
     if (j == nj + 2) return g2;
@@ -241,9 +243,9 @@ __device__ __forceinline__ int zsrc(int q, int k, int nk) {
 
 __device__ __forceinline__ int pimg(int i, int n) { return i < 1 ? i + n : (i > n ? i - n : i); }
 
-template <class T>
-void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
-              const double* l2, const double* phi, bool lo, bool hi, int swept, int comps) {
+template <class T, class P>
+void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const P* l1, const P* l2,
+              const P* phi, bool lo, bool hi, int swept, int comps) {
   const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, sy = d.sy;
   // disjoint regions covering every cell any component may write:
   //  A: the six z-ghost planes (all i, j)
@@ -301,10 +303,10 @@ void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const dou
   });
 }
 
-template <class T>
-void bndu(const Dims d, T* u, T* v, T* w, double uwall, double dy, const double* l1,
-          const double* l2, const double* phi, bool lo, bool hi) {
-  bc_fused<T>(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi, -1, 7);
+template <class T, class P>
+void bndu(const Dims d, T* u, T* v, T* w, double uwall, double dy, const P* l1, const P* l2,
+          const P* phi, bool lo, bool hi) {
+  bc_fused<T, P>(d, u, v, w, uwall, dy, l1, l2, phi, lo, hi, -1, 7);
 }
 
 // srcs/bnd_periodic.f:16-43, x then z, in one pass: a ghost cell takes the
@@ -363,26 +365,26 @@ void neumann(const Dims d, T* q, bool lo, bool hi) {
 }
 
 // srcs/gnbc.f90:25-97
-template <class T>
-void gnbc(const Dims d, const T* uk, const T* wk, double uwall, const double* th0, double st,
-          const double* zeta, double* theta, bool lo, bool hi, double pi) {
+template <class T, class P = PT<T>>
+void gnbc(const Dims d, const T* uk, const T* wk, double uwall, const P* th0, double st,
+          const P* zeta, P* theta, bool lo, bool hi, double pi) {
   const int ni = d.ni, nj = d.nj;
   launch_box(Box{-2, d.ni + 3, 0, 1, -2, d.nk + 3}, [=] __device__(int i, int side, int k) {
     if ((side == 0 && !lo) || (side == 1 && !hi)) return;
     const int j0 = side == 0 ? 0 : nj, j1 = j0 + 1, jw = side == 0 ? 1 : nj;
     const double uw = side == 0 ? uwall : -uwall;
     const int mi = (i + ni + 2) % (ni + 3);
-    // the contact-line slip velocity is evaluated in FP64
-    const double ux = ((double)uk[d.ix(mi, j0, k)] + (double)uk[d.ix(mi, j1, k)] +
-                       (double)uk[d.ix(i, j0, k)] + (double)uk[d.ix(i, j1, k)]) / 4.0 + uw;
-    const double uz = ((double)wk[d.ix(mi, j0, k)] + (double)wk[d.ix(mi, j1, k)] +
-                       (double)wk[d.ix(i, j0, k)] + (double)wk[d.ix(i, j1, k)]) / 4.0;
-    double ucl = copysign(sqrt(ux * ux + uz * uz + 1.0e-12), ux);
+    // the contact-line slip velocity is evaluated in the phase type (FP64)
+    const P ux = (P(uk[d.ix(mi, j0, k)]) + P(uk[d.ix(mi, j1, k)]) + P(uk[d.ix(i, j0, k)]) +
+                  P(uk[d.ix(i, j1, k)])) / 4.0 + uw;
+    const P uz = (P(wk[d.ix(mi, j0, k)]) + P(wk[d.ix(mi, j1, k)]) + P(wk[d.ix(i, j0, k)]) +
+                  P(wk[d.ix(i, j1, k)])) / 4.0;
+    P ucl = copysign(sqrt(ux * ux + uz * uz + 1.0e-12), ux);
     if (i <= ni / 2) ucl = -ucl;
     const int cw = d.ix(i, jw, k);
-    const double th0r = th0[cw] * (pi / 180.0);
-    double ct = cos(th0r) - zeta[cw] * ucl / st;
-    ct = fmin(1.0, fmax(-1.0, ct));
+    const P th0r = th0[cw] * (pi / 180.0);
+    P ct = cos(th0r) - zeta[cw] * ucl / st;
+    ct = fmin(P(1.0), fmax(P(-1.0), ct));
     theta[cw] = acos(ct) * (180.0 / pi);
   });
 }
@@ -392,61 +394,72 @@ void gnbc(const Dims d, const T* uk, const T* wk, double uwall, const double* th
 // reference reads past its row or array) are taken as 0; those ghost values
 // are overwritten by the following periodic fill or never reach the
 // interior.
-void contact_angle(const Dims d, double* q, const double* theta, double ax, double az, bool lo,
-                   bool hi, double pi) {
+template <class P>
+void contact_angle(const Dims d, P* q, const P* theta, double ax, double az, bool lo, bool hi,
+                   double pi) {
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   launch_box(Box{-2, d.ni + 3, 0, 1, -2, d.nk + 3}, [=] __device__(int i, int side, int k) {
     if ((side == 0 && !lo) || (side == 1 && !hi)) return;
     const int jw = side == 0 ? 1 : nj, jg = side == 0 ? 0 : nj + 1, js = side == 0 ? -1 : 1;
     const int cw = d.ix(i, jw, k);
-    const double th = theta[cw] * pi / 180.0;
-    const double ct = cos(th), sn = sin(th);
+    const P th = theta[cw] * pi / 180.0;
+    const P ct = cos(th), sn = sin(th);
     const double sg = copysign(1.0, (double)(i - ni / 2));
-    double nx = sg * sn * ax, nz = sg * sn * az, ny = ct;
-    const double nrm = sqrt(nx * nx + ny * ny + nz * nz + 1.0e-20);
+    P nx = sg * sn * ax, nz = sg * sn * az, ny = ct;
+    const P nrm = sqrt(nx * nx + ny * ny + nz * nz + 1.0e-20);
     nx = nx / nrm;
     ny = ny / nrm;
     nz = nz / nrm;
-    const double q1 = q[cw];
-    const double qe = i + 1 <= ni + 3 ? q[cw + 1] : 0.0;
-    const double qw = i - 1 >= -2 ? q[cw - 1] : 0.0;
-    const double qt = k + 1 <= nk + 3 ? q[cw + d.sx * d.sy] : 0.0;
-    const double qb = k - 1 >= -2 ? q[cw - d.sx * d.sy] : 0.0;
-    const double g = q1 - (ny / (nx + 1.0e-20)) * (qe - qw) * 0.5 * ax -
-                     (ny / (nz + 1.0e-20)) * (qt - qb) * 0.5 * az;
+    const P q1 = q[cw];
+    const P qe = i + 1 <= ni + 3 ? q[cw + 1] : P(0.0);
+    const P qw = i - 1 >= -2 ? q[cw - 1] : P(0.0);
+    const P qt = k + 1 <= nk + 3 ? q[cw + d.sx * d.sy] : P(0.0);
+    const P qb = k - 1 >= -2 ? q[cw - d.sx * d.sy] : P(0.0);
+    // g = q1 - A - B in the reference's order; tangent: an absorbed A or B
+    const P A = (ny / (nx + 1.0e-20)) * (qe - qw) * 0.5 * ax;
+    const P B = (ny / (nz + 1.0e-20)) * (qt - qb) * 0.5 * az;
+    const P g = sub_rounded(sub_rounded(q1, A), B);
     const int Y = d.sx;
     const int c0 = d.ix(i, jg, k);
-    const double g1 = 2.0 * g - q1;
+    const P g1 = 2.0 * g - q1;
     q[c0] = g;
     q[c0 + js * Y] = g1;
     q[c0 + 2 * js * Y] = 2.0 * g1 - g;
   });
 }
 
-template void bc_fused<double>(const Dims, double*, double*, double*, double, double,
-                               const double*, const double*, const double*, bool, bool, int, int);
-template void bc_fused<float>(const Dims, float*, float*, float*, double, double, const double*,
-                              const double*, const double*, bool, bool, int, int);
+template void bc_fused<double, double>(const Dims, double*, double*, double*, double, double,
+                                       const double*, const double*, const double*, bool, bool,
+                                       int, int);
+template void bc_fused<float, double>(const Dims, float*, float*, float*, double, double,
+                                      const double*, const double*, const double*, bool, bool, int,
+                                      int);
+template void bc_fused<Dual, Dual>(const Dims, Dual*, Dual*, Dual*, double, double, const Dual*,
+                                   const Dual*, const Dual*, bool, bool, int, int);
 template void periodic<double>(const Dims, double*);
 template void periodic<float>(const Dims, float*);
+template void periodic<Dual>(const Dims, Dual*);
 template void neumann<double>(const Dims, double*, bool, bool);
 template void neumann<float>(const Dims, float*, bool, bool);
+template void neumann<Dual>(const Dims, Dual*, bool, bool);
 
 // Drop-in bodies for the velocity element type T.
 template <class T>
 void bndu_t(const Dims d, const int* nID, double* uk, double* vk, double* wk, double uwall,
             double dy, const double* l1, const double* l2, const double* phi, int mode, int swept) {
+  using P = PT<T>;
   Call c("bndu_");
   T *u = c.io_t<T>(uk, d.n3, "uk"), *v = c.io_t<T>(vk, d.n3, "vk"), *w = c.io_t<T>(wk, d.n3, "wk");
-  const double *a = c.in(l1, d.n3, "l1"), *b = c.in(l2, d.n3, "l2"), *p = c.in(phi, d.n3, "phi");
+  const P *a = c.in_t<P>(l1, d.n3, "l1"), *b = c.in_t<P>(l2, d.n3, "l2"),
+          *p = c.in_t<P>(phi, d.n3, "phi");
   const bool lo = nID[kYMinus] < 0, hi = nID[kYPlus] < 0;
   if (mode == 0) {
-    bndu<T>(d, u, v, w, uwall, dy, a, b, p, lo, hi);
+    bndu<T, P>(d, u, v, w, uwall, dy, a, b, p, lo, hi);
   } else if (mode == 1) {
-    bndu_phased<T>(d, u, v, w, uwall, dy, a, b, p, lo, hi);
+    bndu_phased<T, P>(d, u, v, w, uwall, dy, a, b, p, lo, hi);
     if (swept >= 0) periodic<T>(d, swept == 0 ? u : (swept == 1 ? v : w));
   } else {
-    bc_fused<T>(d, u, v, w, uwall, dy, a, b, p, lo, hi, swept, 7);
+    bc_fused<T, P>(d, u, v, w, uwall, dy, a, b, p, lo, hi, swept, 7);
   }
   c.finish();
 }
@@ -468,10 +481,11 @@ void neumann_t(const Dims d, const int* nID, double* qk) {
 template <class T>
 void gnbc_t(const Dims d, const int* nID, const double* uk, const double* wk, double uwall,
             const double* theta_0_array, double st, const double* zeta_array, double* theta_array) {
+  using P = PT<T>;
   Call c("gnbc_");
   const T *u = c.in_t<T>(uk, d.n3, "uk"), *w = c.in_t<T>(wk, d.n3, "wk");
-  const double *t0 = c.in(theta_0_array, d.n3, "theta_0"), *z = c.in(zeta_array, d.n3, "zeta");
-  double* th = c.io(theta_array, d.n3, "theta");
+  const P *t0 = c.in_t<P>(theta_0_array, d.n3, "theta_0"), *z = c.in_t<P>(zeta_array, d.n3, "zeta");
+  P* th = c.io_t<P>(theta_array, d.n3, "theta");
   gnbc<T>(d, u, w, uwall, t0, st, z, th, nID[kYMinus] < 0, nID[kYPlus] < 0, std::acos(-1.0));
   c.finish();
 }
@@ -487,8 +501,9 @@ void bndu_(const int* nID, const int* ni, const int* nj, const int* nk, double* 
            double* wk, const double* uwall, const double* dy, const double* l1, const double* l2,
            const double* phi) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) bndu_t<float>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 0, -1);
-  else bndu_t<double>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 0, -1);
+  with_flow([&](auto t) {
+    bndu_t<decltype(t)>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 0, -1);
+  });
 }
 
 // Cross-check entry point: bndu with the reference's three ordered passes
@@ -499,8 +514,9 @@ void felgpu_bndu_phased_(const int* nID, const int* ni, const int* nj, const int
                          double* vk, double* wk, const double* uwall, const double* dy,
                          const double* l1, const double* l2, const double* phi, const int* swept) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) bndu_t<float>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 1, *swept);
-  else bndu_t<double>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 1, *swept);
+  with_flow([&](auto t) {
+    bndu_t<decltype(t)>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 1, *swept);
+  });
 }
 
 // The composed single launch of bndu_ + bnd_periodic(component swept).
@@ -508,22 +524,21 @@ void felgpu_bc_fused_(const int* nID, const int* ni, const int* nj, const int* n
                       double* vk, double* wk, const double* uwall, const double* dy,
                       const double* l1, const double* l2, const double* phi, const int* swept) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) bndu_t<float>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 2, *swept);
-  else bndu_t<double>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 2, *swept);
+  with_flow([&](auto t) {
+    bndu_t<decltype(t)>(d, nID, uk, vk, wk, *uwall, *dy, l1, l2, phi, 2, *swept);
+  });
 }
 
 // srcs/bnd_periodic.f:4
 void bnd_periodic_(const int* ni, const int* nj, const int* nk, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (elem_size(qk) == 4) periodic_t<float>(d, qk);
-  else periodic_t<double>(d, qk);
+  with_elem(elem_size(qk), [&](auto t) { periodic_t<decltype(t)>(d, qk); });
 }
 
 // srcs/bnd_neumann.f:4
 void bnd_neumann_(const int* nID, const int* ni, const int* nj, const int* nk, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (elem_size(qk) == 4) neumann_t<float>(d, nID, qk);
-  else neumann_t<double>(d, nID, qk);
+  with_elem(elem_size(qk), [&](auto t) { neumann_t<decltype(t)>(d, nID, qk); });
 }
 
 // srcs/bnd_comm.f:1-18: halo exchange between j-slabs; one GPU holds the
@@ -543,10 +558,10 @@ void gnbc_(const int* nID, const int* ni, const int* nj, const int* nk, const do
            const double* wk, const double* uwall, const double* theta_0_array,
            const double* surface_tension, const double* zeta_array, double* theta_array) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32())
-    gnbc_t<float>(d, nID, uk, wk, *uwall, theta_0_array, *surface_tension, zeta_array, theta_array);
-  else
-    gnbc_t<double>(d, nID, uk, wk, *uwall, theta_0_array, *surface_tension, zeta_array, theta_array);
+  with_flow([&](auto t) {
+    gnbc_t<decltype(t)>(d, nID, uk, wk, *uwall, theta_0_array, *surface_tension, zeta_array,
+                        theta_array);
+  });
 }
 
 // srcs/bnd_contact_angle.f90:3
@@ -554,11 +569,15 @@ void bnd_contact_angle_(const int* nID, const int* ni, const int* nj, const int*
                         const double* theta_array, const double* dx, const double* dy,
                         const double* dz) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("bnd_contact_angle_");
-  double* q = c.io(qk, d.n3);
-  const double* th = c.in(theta_array, d.n3);
-  contact_angle(d, q, th, *dy / *dx, *dy / *dz, nID[kYMinus] < 0, nID[kYPlus] < 0, std::acos(-1.0));
-  c.finish();
+  with_flow([&](auto t) {
+    using P = PT<decltype(t)>;
+    Call c("bnd_contact_angle_");
+    P* q = c.io_t<P>(qk, d.n3, "qk");
+    const P* th = c.in_t<P>(theta_array, d.n3, "theta");
+    contact_angle<P>(d, q, th, *dy / *dx, *dy / *dz, nID[kYMinus] < 0, nID[kYPlus] < 0,
+                     std::acos(-1.0));
+    c.finish();
+  });
 }
 
 }  // extern "C"

@@ -11,16 +11,17 @@
 namespace fel {
 
 // T: element type of the gradients, normals, curvature and forces (double,
-// or float in the FP32 mode); the volume fraction is FP64.
+// float in the FP32 mode, Dual in the tangent mode); the volume fraction is
+// in the phase type PT<T>.
 
 // srcs/cal_grad_p2uvw.f:25-43 (per-j dyinv from dyinv_array)
-template <class T>
+template <class T, class P = PT<T>>
 void grad_p2uvw(const Dims d, int ID, double dxinv, const double* dyt_dev, double dyu, double dzinv,
-                const double* q, T* qx, T* qy, T* qz) {
+                const P* q, T* qx, T* qy, T* qz) {
   launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
     const double dyinv = dyt_dev ? dyt_dev[j + d.nj * ID + 2] : dyu;
-    const double q0 = q[c];
+    const P q0 = q[c];
     qx[c] = T((-q0 + q[c + 1]) * dxinv);
     qy[c] = T((-q0 + q[c + d.sx]) * dyinv);
     qz[c] = T((-q0 + q[c + d.sx * d.sy]) * dzinv);
@@ -47,6 +48,8 @@ __global__ void __launch_bounds__(256) calrn_kernel(Dims d, T verysmall, int ip,
   if (ii > d.ni + 3 || jj > d.nj + 3) return;
   const int n3 = d.n3, Z = d.sx * d.sy;
   const int k0 = -2 + bz * kRnChunk, k1 = min(d.nk + 3, k0 + kRnChunk - 1);
+  // tangent mode: gradients below 1e-10 |dxinv| count as flat (no interface)
+  const double flat2 = (val(verysmall) * 1.0e20) * (val(verysmall) * 1.0e20);
   const bool col_ok = ii >= -2 + ip && jj >= -2 + jp;
   const int X = ip, Y = jp * d.sx;
   T pa[3][4];  // previous plane: (i,j), (ii,j), (i,jj), (ii,jj)
@@ -95,6 +98,7 @@ __global__ void __launch_bounds__(256) calrn_kernel(Dims d, T verysmall, int ip,
       r[m] = (+pa[m][0] + pa[m][1] + pa[m][2] + pa[m][3] + cur[m][0] + cur[m][1] + cur[m][2] +
               cur[m][3]) *
              T(0.125);
+    flat_zero(r, flat2);
     const T arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
     const T inv = T(1.0) / arn;
     rn[tgt] = r[0] * inv;
@@ -138,8 +142,10 @@ __global__ void __launch_bounds__(256) calrn4_kernel(Dims d, T verysmall, const 
   const int ii = -2 + bx * 64 + (int)threadIdx.x;
   const int jj = -2 + by * 4 + (int)threadIdx.y;
   if (ii > d.ni + 3 || jj > d.nj + 3) return;
-  const int n3 = d.n3, X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const int n3 = d.n3, X = 1, Y = d.sx;
   const int k0 = -2 + bz * kRnChunk, k1 = min(d.nk + 3, k0 + kRnChunk - 1);
+  // tangent mode: gradients below 1e-10 |dxinv| count as flat (no interface)
+  const double flat2 = (val(verysmall) * 1.0e20) * (val(verysmall) * 1.0e20);
   const bool xm = ii >= -1, ym = jj >= -1;  // the i-1 / j-1 neighbours exist
   T* out[4] = {r0, r1, r2, r3};
   const T* a3[3] = {px, py, pz};
@@ -185,6 +191,7 @@ __global__ void __launch_bounds__(256) calrn4_kernel(Dims d, T verysmall, const 
         r[m] = (+V[m][ai][bj][cp] + V[m][1][bj][cp] + V[m][ai][1][cp] + V[m][1][1][cp] +
                 V[m][ai][bj][1] + V[m][1][bj][1] + V[m][ai][1][1] + V[m][1][1][1]) *
                T(0.125);
+      flat_zero(r, flat2);
       const T arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
       const T inv = T(1.0) / arn;
       rn[tgt] = r[0] * inv;
@@ -283,7 +290,7 @@ void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const doubl
     FEL_CHECK(hipStreamSynchronize(stream()));
     table = dev;
   }
-  const double* dq = c.in(q, d.n3, "q");
+  const PT<T>* dq = c.in_t<PT<T>>(q, d.n3, "q");
   T* x = c.io_t<T>(qx_u, d.n3, "qx_u");
   T* y = c.io_t<T>(qy_v, d.n3, "qy_v");
   T* z = c.io_t<T>(qz_w, d.n3, "qz_w");
@@ -380,10 +387,9 @@ void cal_grad_p2uvw_(const int* ID, const int* svall, const int* ni, const int* 
                      const double* dxinv, const double* dyinv_array, const double* dzinv,
                      const double* q, double* qx_u, double* qy_v, double* qz_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32())
-    cal_grad_p2uvw_t<float>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx_u, qy_v, qz_w);
-  else
-    cal_grad_p2uvw_t<double>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx_u, qy_v, qz_w);
+  with_flow([&](auto t) {
+    cal_grad_p2uvw_t<decltype(t)>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx_u, qy_v, qz_w);
+  });
 }
 
 // srcs/calrn.f:1-2
@@ -392,8 +398,7 @@ void calrn_(const int* ni, const int* nj, const int* nk, const double* dxinv, co
             const double* phiy, const double* phiz, double* rn) {
   const Dims d = make_dims(*ni, *nj, *nk);
   const double vs = std::sqrt(*dxinv * *dxinv + *dyinv * *dyinv + *dzinv * *dzinv) * 1.0e-30;
-  if (flow_fp32()) calrn_t<float>(d, vs, *ip, *jp, *kp, phix, phiy, phiz, rn);
-  else calrn_t<double>(d, vs, *ip, *jp, *kp, phix, phiy, phiz, rn);
+  with_flow([&](auto t) { calrn_t<decltype(t)>(d, vs, *ip, *jp, *kp, phix, phiy, phiz, rn); });
 }
 
 // srcs/calrkap.f:1-2
@@ -401,8 +406,9 @@ void calrkap_(const int* ni, const int* nj, const int* nk, const double* dxinv, 
               const double* dzinv, const double* rn_pp, const double* rn_12, const double* rn_13,
               const double* rn_23, double* rkap) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) calrkap_t<float>(d, *dxinv, *dyinv, *dzinv, rn_pp, rn_12, rn_13, rn_23, rkap);
-  else calrkap_t<double>(d, *dxinv, *dyinv, *dzinv, rn_pp, rn_12, rn_13, rn_23, rkap);
+  with_flow([&](auto t) {
+    calrkap_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, rn_pp, rn_12, rn_13, rn_23, rkap);
+  });
 }
 
 // srcs/calfst.f:1-3
@@ -411,8 +417,9 @@ void calfst_(const int* ni, const int* nj, const int* nk, const double* rhol, co
              const double* phiz_w, const double* rkap, double* fst_u, double* fst_v, double* fst_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
   const double st = 2.0 * (*surface_tension) / (*rhol + *rhog);
-  if (flow_fp32()) calfst_t<float>(d, st, phix_u, phiy_v, phiz_w, rkap, fst_u, fst_v, fst_w);
-  else calfst_t<double>(d, st, phix_u, phiy_v, phiz_w, rkap, fst_u, fst_v, fst_w);
+  with_flow([&](auto t) {
+    calfst_t<decltype(t)>(d, st, phix_u, phiy_v, phiz_w, rkap, fst_u, fst_v, fst_w);
+  });
 }
 
 }  // extern "C"

@@ -9,24 +9,25 @@
 
 namespace fel {
 
-// Phase-weighted velocity sums (FP64) of srcs/cal_vel.f:43-53; T is the
-// velocity element type.  Returns the pinned read-back.
+// Phase-weighted velocity sums (FP64, of the values) of
+// srcs/cal_vel.f:43-53; T is the velocity element type.  Returns the pinned
+// read-back.
 template <class T>
 const double* vel_sums(const Dims d, const double* phi, const double* u, const double* v,
                        const double* w) {
   Call c("cal_vel_");
-  const double* p = c.in(phi, d.n3, "phi");
+  const PT<T>* p = c.in_t<PT<T>>(phi, d.n3, "phi");
   const T *du = c.in_t<T>(u, d.n3, "u"), *dv = c.in_t<T>(v, d.n3, "v"), *dw = c.in_t<T>(w, d.n3, "w");
   double* s = dev_scalars(7);
   interior_reduce<4>(
       d,
       [=] __device__(int i, int j, int k, double* o) {
         const int m = d.ix(i, j, k);
-        const double ph = p[m];
+        const double ph = val(p[m]);
         o[0] = ph;
-        o[1] = ph * (double)du[m];
-        o[2] = ph * (double)dv[m];
-        o[3] = ph * (double)dw[m];
+        o[1] = ph * val(du[m]);
+        o[2] = ph * val(dv[m]);
+        o[3] = ph * val(dw[m]);
       },
       [=] __device__(const double* t, double* out) {
         for (int m = 0; m < 4; m++) out[m] = t[m];
@@ -51,35 +52,39 @@ void cal_center_(const int* ipara, const int* ID, const int* svall, const int* n
                  double* velocity) {
   (void)ipara; (void)zl_; (void)rhog;
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("cal_center_");
-  const double* p = c.in(phi, d.n3);
   const double xl = *xl_, yl = *yl_;
   const double delta = yl / (double)(*svall);
   const double s1 = xl * 0.25, s2 = xl * 0.75;
   if (*area_pre1 == 1 && *area_pre2 == 3) *lap = *lap + 1;
   const int ap1 = *area_pre1, lp = *lap, id = *ID, njl = *nj;
   double* s = dev_scalars(6);
-  interior_reduce<4>(
-      d,
-      [=] __device__(int i, int j, int k, double* o) {
-        const double ph = p[d.ix(i, j, k)];
-        double lx = delta * ((double)i - 0.5);
-        const double ly = delta * ((double)(j + id * njl) - 0.5);
-        const double lz = delta * ((double)k - 0.5);
-        if (ap1 == 3 && lx < s1) lx = lx + xl;
-        if (ap1 == 1 && lx > s2) lx = lx - xl;
-        lx = lx + xl * (double)lp;
-        o[0] = ph;
-        o[1] = ph * lx;
-        o[2] = ph * ly;
-        o[3] = ph * lz;
-      },
-      [=] __device__(const double* t, double* out) {
-        for (int m = 0; m < 4; m++) out[m] = t[m];
-      },
-      s);
-  const double* r = read_scalars(s, 4);
-  c.finish();
+  const double* r = nullptr;
+  with_flow([&](auto t) {
+    using P = PT<decltype(t)>;
+    Call c("cal_center_");
+    const P* p = c.in_t<P>(phi, d.n3, "phi");
+    interior_reduce<4>(
+        d,
+        [=] __device__(int i, int j, int k, double* o) {
+          const double ph = val(p[d.ix(i, j, k)]);
+          double lx = delta * ((double)i - 0.5);
+          const double ly = delta * ((double)(j + id * njl) - 0.5);
+          const double lz = delta * ((double)k - 0.5);
+          if (ap1 == 3 && lx < s1) lx = lx + xl;
+          if (ap1 == 1 && lx > s2) lx = lx - xl;
+          lx = lx + xl * (double)lp;
+          o[0] = ph;
+          o[1] = ph * lx;
+          o[2] = ph * ly;
+          o[3] = ph * lz;
+        },
+        [=] __device__(const double* t, double* out) {
+          for (int m = 0; m < 4; m++) out[m] = t[m];
+        },
+        s);
+    r = read_scalars(s, 4);
+    c.finish();
+  });
   center[0] = r[1] / r[0];
   center[1] = r[2] / r[0];
   center[2] = r[3] / r[0];
@@ -104,7 +109,8 @@ void cal_vel_(const int* ipara, const int* ni, const int* nj, const int* nk, con
               const double* w, const double* phi, double* vel, double* re) {
   (void)ipara;
   const Dims d = make_dims(*ni, *nj, *nk);
-  const double* r = flow_fp32() ? vel_sums<float>(d, phi, u, v, w) : vel_sums<double>(d, phi, u, v, w);
+  const double* r = nullptr;
+  with_flow([&](auto t) { r = vel_sums<decltype(t)>(d, phi, u, v, w); });
   const double diameter = *radius * 2.0;
   for (int m = 0; m < 3; m++) {
     vel[m] = r[m + 1] / r[0];

@@ -24,9 +24,20 @@
 
 namespace fel {
 
-// Complex element of the transforms: double2 in FP64, float2 in the FP32 mode.
+// Complex element of the transforms: double2 in FP64, float2 in the FP32
+// mode, a pair of Duals in the tangent mode (the transforms are linear and
+// the twiddles constant, so the derivative part is the transform of the
+// input's derivative part).
+struct Dual2 {
+  Dual x, y;
+};
 template <class T>
 struct Cplx;
+template <>
+struct Cplx<Dual> {
+  using type = Dual2;
+  __host__ __device__ static Dual2 make(Dual a, Dual b) { return Dual2{a, b}; }
+};
 template <>
 struct Cplx<double> {
   using type = double2;
@@ -173,8 +184,8 @@ static void check_pow2(int n, const char* who) {
 
 // srcs/solp_fft_tdma1.f:25-47: phir(i,j,k) = DFT(div(1:ni,j,1:nk))/(ni nk)
 // at centred mode indices; the scratch holds (nk, nj, ni) complex values.
-// T: element type of the modes; the divergence is FP64.
-// `rhs(i,j,k)` gives the FP64 right-hand side at interior cell (i,j,k).
+// T: element type of the modes; the divergence is in the phase type PT<T>.
+// `rhs(i,j,k)` gives the right-hand side at interior cell (i,j,k).
 template <class T, class Rhs>
 void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
   using C = typename Cplx<T>::type;
@@ -208,14 +219,14 @@ void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
 }
 
 template <class T>
-void fft_forward(const Dims d, const double* div, T* pr, T* pi) {
+void fft_forward(const Dims d, const PT<T>* div, T* pr, T* pi) {
   fft_forward_of<T>(d, [=] __device__(int i, int j, int k) { return div[d.ix(i, j, k)]; }, pr, pi);
 }
 
 // srcs/solp_fft_tdma3.f:25-44: dp(1:ni,j,1:nk) = Re(unnormalised inverse DFT)
 // (added to dp instead when `add`: the refinement corrections).
-template <class T>
-void fft_inverse(const Dims d, const T* pr, const T* pi, double* dp, bool add) {
+template <class T, class P = PT<T>>
+void fft_inverse(const Dims d, const T* pr, const T* pi, P* dp, bool add) {
   using C = typename Cplx<T>::type;
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   check_pow2(ni, "solp_fft_tdma3_");
@@ -238,8 +249,8 @@ void fft_inverse(const Dims d, const T* pr, const T* pi, double* dp, bool add) {
            [=] __device__(int line, int e) { return TS[(size_t)line * ni + e]; },
            [=] __device__(int line, int e, C v) {
              const int j = line % nj, k = line / nj;
-             double* o = dp + d.ix(e + 1, j + 1, k + 1);
-             *o = add ? *o + (double)v.x : (double)v.x;
+             P* o = dp + d.ix(e + 1, j + 1, k + 1);
+             *o = add ? *o + P(v.x) : P(v.x);
            });
 }
 
@@ -257,7 +268,7 @@ void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, cons
   launch_flat(ni * nk, [=] __device__(int m) {
     const int i = m % ni, k = m / ni;
     const double c1x = cx[i], c1z = cz[k];
-    const bool singular = sizeof(T) == 4 && c1x == 2.0 && c1z == 2.0;
+    const bool singular = is_f32<T> && c1x == 2.0 && c1z == 2.0;
     T a_prev = T(0.0), br = T(0.0), bi = T(0.0);
     const size_t base = (size_t)i + (size_t)ni * nj * k;
     for (int j = 1; j <= nj; j++) {
@@ -286,56 +297,65 @@ void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, cons
 }
 
 // srcs/caldiv.f:24-41 on [-1,n+3]^3; the RMS (caldiv.f:43-67) is returned
-// through dsum (device, sum of squares over the interior).  The divergence
-// is FP64 (velocity values of either precision are exact in FP64).
-template <class T>
+// through dsum (device, sum of squares of the values over the interior).
+// The divergence is in the phase type P (velocity values of either
+// precision are exact in FP64).
+template <class T, class P = PT<T>>
 void divergence(const Dims d, double dxinv, double dyinv, double dzinv, double dtinv, const T* u,
-                const T* v, const T* w, double* div, double* dsum) {
+                const T* v, const T* w, P* div, double* dsum) {
   launch_box(Box{-1, d.ni + 3, -1, d.nj + 3, -1, d.nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    div[c] = ((-(double)u[c - X] + (double)u[c]) * dxinv + (-(double)v[c - Y] + (double)v[c]) * dyinv +
-              (-(double)w[c - Z] + (double)w[c]) * dzinv) *
+    div[c] = ((-P(u[c - X]) + P(u[c])) * dxinv + (-P(v[c - Y]) + P(v[c])) * dyinv +
+              (-P(w[c - Z]) + P(w[c])) * dzinv) *
              dtinv;
   });
   interior_reduce<1>(
       d,
       [=] __device__(int i, int j, int k, double* o) {
-        const double x = div[d.ix(i, j, k)];
+        const double x = val(div[d.ix(i, j, k)]);
         o[0] = x * x;
       },
       [=] __device__(const double* t, double* out) { out[0] = t[0]; }, dsum);
 }
 
-// srcs/bnd_avzero.f:10-23 with srcs/cal_av.f:20-37: the mean (FP64) stays
-// on the device.  Q: element type of the field.
+// srcs/bnd_avzero.f:10-23 with srcs/cal_av.f:20-37: the mean stays on the
+// device.  Q: element type of the field; the mean is FP64 (Dual in the
+// tangent mode, whose mean removal also removes the derivative's mean).
 template <class Q>
 void avzero(const Dims d, Q* q, int ndiv) {
+  using A = PT<Q>;
   double* s = dev_scalars(2);
   const double nn = (double)(d.ni * d.nj * d.nk * ndiv);
-  interior_reduce<1>(
-      d, [=] __device__(int i, int j, int k, double* o) { o[0] = (double)q[d.ix(i, j, k)]; },
-      [=] __device__(const double* t, double* out) { out[0] = t[0] / nn; }, s);
-  const double* av = s;
-  launch_flat(d.n3, [=] __device__(int m) { q[m] = Q((double)q[m] - *av); });
+  interior_reduce<1, A>(
+      d, [=] __device__(int i, int j, int k, A* o) { o[0] = A(q[d.ix(i, j, k)]); },
+      [=] __device__(const A* t, double* out) { reinterpret_cast<A*>(out)[0] = t[0] / nn; }, s);
+  const A* av = reinterpret_cast<const A*>(s);
+  launch_flat(d.n3, [=] __device__(int m) { q[m] = Q(A(q[m]) - *av); });
 }
 
-// srcs/corunp_explicit.f:26-74; dp is FP64, the velocities and pressure T.
-template <class T>
+// srcs/corunp_explicit.f:26-74; dp is in the phase type P, the velocities
+// and pressure T.
+template <class T, class P = PT<T>>
 void correct(const Dims d, int jen, double rho, double dxinv, double dyinv, double dzinv, double dt,
-             const double* dp, const T* phat, T* un, T* vn, T* wn, T* pn) {
+             const P* dp, const T* phat, T* un, T* vn, T* wn, T* pn) {
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    const double p0 = dp[c];
-    un[c] = T((double)un[c] + (p0 - dp[c + X]) / rho * dxinv * dt);
-    if (j <= jen) vn[c] = T((double)vn[c] + (p0 - dp[c + Y]) / rho * dyinv * dt);
-    wn[c] = T((double)wn[c] + (p0 - dp[c + Z]) / rho * dzinv * dt);
-    pn[c] = T((double)phat[c] + p0);
+    const P p0 = dp[c];
+    un[c] = T(P(un[c]) + (p0 - dp[c + X]) / rho * dxinv * dt);
+    if (j <= jen) vn[c] = T(P(vn[c]) + (p0 - dp[c + Y]) / rho * dyinv * dt);
+    wn[c] = T(P(wn[c]) + (p0 - dp[c + Z]) / rho * dzinv * dt);
+    pn[c] = T(P(phat[c]) + p0);
   });
 }
 
-// Sums for srcs/solp_fft_tdma4.f:28-72: [sum div^2, sum (L dp/rho - div)^2].
+// Sums for srcs/solp_fft_tdma4.f:28-72: [sum div^2, sum (L dp/rho - div)^2],
+// of the values.
+template <class P>
 void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv, double dzinv,
-                           const double* div, const double* dp, double* dsum) {
+                           const P* div_, const P* dp_, double* dsum) {
+  const double* div = reinterpret_cast<const double*>(div_);
+  const double* dp = reinterpret_cast<const double*>(dp_);
+  const int st = (int)(sizeof(P) / sizeof(double));  // value stride: 1, or 2 for Dual
   const double aw = dxinv * dxinv / rho, ae = dxinv * dxinv / rho;
   const double as = dyinv * dyinv / rho, an = dyinv * dyinv / rho;
   const double ab = dzinv * dzinv / rho, at = dzinv * dzinv / rho;
@@ -344,9 +364,10 @@ void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv,
       d,
       [=] __device__(int i, int j, int k, double* o) {
         const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-        const double dv = div[c];
-        const double r = +dp[c - Z] * ab + dp[c - Y] * as + dp[c - X] * aw - dp[c] * ap +
-                         dp[c + X] * ae + dp[c + Y] * an + dp[c + Z] * at - dv;
+        const double dv = div[st * c];
+        const double r = +dp[st * (c - Z)] * ab + dp[st * (c - Y)] * as + dp[st * (c - X)] * aw -
+                         dp[st * c] * ap + dp[st * (c + X)] * ae + dp[st * (c + Y)] * an +
+                         dp[st * (c + Z)] * at - dv;
         o[0] = dv * dv;
         o[1] = r * r;
       },
@@ -374,7 +395,7 @@ void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv,
 struct PoissonRecord {
   bool rhs = false, coef = false;
   int ni = 0, nj = 0, nk = 0;
-  const double* div = nullptr;   // device
+  const double* div = nullptr;   // device (recorded in the FP32 mode only)
   uint64_t div_ver = 0;
   const void* div_host = nullptr;
   double dx2r = 0.0, dz2r = 0.0;
@@ -436,7 +457,7 @@ void caldiv_t(const Dims d, double dxinv, double dyinv, double dzinv, double dt,
   Call c("caldiv_");
   const T *u = c.in_t<T>(uk, d.n3, "uk"), *v = c.in_t<T>(vk, d.n3, "vk"),
           *w = c.in_t<T>(wk, d.n3, "wk");
-  double* dv = c.io(div, d.n3, "div");
+  PT<T>* dv = c.io_t<PT<T>>(div, d.n3, "div");
   double* s = dev_scalars(3);
   divergence<T>(d, dxinv, dyinv, dzinv, 1.0 / dt, u, v, w, dv, s);
   *sum = read_scalars(s, 1)[0];
@@ -447,17 +468,18 @@ template <class T>
 void tdma1_t(const Dims d, const double* div, double* phir_r, double* phir_i) {
   const size_t nm = (size_t)d.ni * d.nj * d.nk;
   Call c("solp_fft_tdma1_");
-  const double* dv = c.in(div, d.n3, "div");
+  const PT<T>* dv = c.in_t<PT<T>>(div, d.n3, "div");
   T* pr = c.out_t<T>(phir_r, nm, "phir_r");
   T* pi = c.out_t<T>(phir_i, nm, "phir_i");
   prof_begin(5);
   fft_forward<T>(d, dv, pr, pi);
   prof_end(5);
   c.finish();
+  if constexpr (!is_f32<T>) return;  // the record serves the FP32 refinement
   PoissonRecord& rec = prec();
   rec.rhs = true;
   rec.ni = d.ni; rec.nj = d.nj; rec.nk = d.nk;
-  rec.div = dv;
+  rec.div = reinterpret_cast<const double*>(dv);
   rec.div_host = div;
   rec.div_ver = content_version(div, d.n3);
   if (rec.div_ver == 0) rec.rhs = false;  // unregistered: no refinement
@@ -469,10 +491,10 @@ void tdma3_t(const Dims d, const double* phir_r, const double* phir_i, double* d
   Call c("solp_fft_tdma3_");
   const T* pr = c.in_t<T>(phir_r, nm, "phir_r");
   const T* pi = c.in_t<T>(phir_i, nm, "phir_i");
-  double* ddp = c.io(dp, d.n3, "dp");
+  PT<T>* ddp = c.io_t<PT<T>>(dp, d.n3, "dp");
   prof_begin(7);
   fft_inverse<T>(d, pr, pi, ddp, false);
-  if (sizeof(T) == 4) poisson_refine(d, ddp);
+  if constexpr (is_f32<T>) poisson_refine(d, ddp);
   prof_end(7);
   c.finish();
 }
@@ -490,7 +512,7 @@ double cal_av_t(const Dims d, const double* q) {
   const Q* dq = c.in_t<Q>(q, d.n3, "q");
   double* s = dev_scalars(5);
   interior_reduce<1>(
-      d, [=] __device__(int i, int j, int k, double* o) { o[0] = (double)dq[d.ix(i, j, k)]; },
+      d, [=] __device__(int i, int j, int k, double* o) { o[0] = val(dq[d.ix(i, j, k)]); },
       [=] __device__(const double* t, double* out) { out[0] = t[0]; }, s);
   const double sum = read_scalars(s, 1)[0];
   c.finish();
@@ -501,7 +523,7 @@ template <class T>
 void corunp_t(const Dims d, int jen, double rho, double dxinv, double dyinv, double dzinv, double dt,
               const double* dp, const double* phat, double* un, double* vn, double* wn, double* pn) {
   Call c("corunp_explicit_");
-  const double* ddp = c.in(dp, d.n3, "dp");
+  const PT<T>* ddp = c.in_t<PT<T>>(dp, d.n3, "dp");
   const T* ph = c.in_t<T>(phat, d.n3, "phat");
   T *u = c.io_t<T>(un, d.n3, "un"), *v = c.io_t<T>(vn, d.n3, "vn"), *w = c.io_t<T>(wn, d.n3, "wn"),
     *p = c.io_t<T>(pn, d.n3, "pn");
@@ -525,8 +547,9 @@ void caldiv_(const int* ipara, const int* ndiv, const int* ni, const int* nj, co
   }
   const Dims d = make_dims(*ni, *nj, *nk);
   double sum = 0.0;
-  if (flow_fp32()) caldiv_t<float>(d, *dxinv, *dyinv, *dzinv, *dt, uk, vk, wk, div, &sum);
-  else caldiv_t<double>(d, *dxinv, *dyinv, *dzinv, *dt, uk, vk, wk, div, &sum);
+  with_flow([&](auto t) {
+    caldiv_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, *dt, uk, vk, wk, div, &sum);
+  });
   *err_div = std::sqrt(sum / (double)(*ni * *nj * *nk * *ndiv)) + 1.0e-99;
 }
 
@@ -535,8 +558,7 @@ void solp_fft_tdma1_(const int* ni, const int* nj, const int* nk, double* fftdat
                      double* phir_r, double* phir_i) {
   (void)fftdata;
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) tdma1_t<float>(d, div, phir_r, phir_i);
-  else tdma1_t<double>(d, div, phir_r, phir_i);
+  with_flow([&](auto t) { tdma1_t<decltype(t)>(d, div, phir_r, phir_i); });
 }
 
 // srcs/trans_r2w.f:1-38 on one rank: the relabelled array has the same
@@ -551,15 +573,12 @@ void trans_r2w_(const int* ipara, const int* ID, const int* ndiv, const int* sva
   if ((const void*)r == (const void*)w) return;
   const size_t n = (size_t)svall[0] * svall[1] * svall[2];
   Call c("trans_r2w_");
-  if (elem_size(r) == 4) {
-    const float* a = c.in_t<float>(r, n, "r");
-    float* b = c.out_t<float>(w, n, "w");
-    FEL_CHECK(hipMemcpyAsync(b, a, sizeof(float) * n, hipMemcpyDeviceToDevice, stream()));
-  } else {
-    const double* a = c.in(r, n, "r");
-    double* b = c.out(w, n, "w");
-    FEL_CHECK(hipMemcpyAsync(b, a, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
-  }
+  with_elem(elem_size(r), [&](auto t) {
+    using T = decltype(t);
+    const T* a = c.in_t<T>(r, n, "r");
+    T* b = c.out_t<T>(w, n, "w");
+    FEL_CHECK(hipMemcpyAsync(b, a, sizeof(T) * n, hipMemcpyDeviceToDevice, stream()));
+  });
   c.finish();
 }
 
@@ -598,15 +617,12 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
   const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
   Call c("solp_fft_tdma2_");
   prof_begin(6);
-  if (flow_fp32()) {
-    float* pr = c.io_t<float>(phiw_r, nm, "phiw_r");
-    float* pi = c.io_t<float>(phiw_i, nm, "phiw_i");
-    tdma_modes<float>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
-  } else {
-    double* pr = c.io(phiw_r, nm, "phiw_r");
-    double* pi = c.io(phiw_i, nm, "phiw_i");
-    tdma_modes<double>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
-  }
+  with_flow([&](auto t) {
+    using T = decltype(t);
+    T* pr = c.io_t<T>(phiw_r, nm, "phiw_r");
+    T* pi = c.io_t<T>(phiw_i, nm, "phiw_i");
+    tdma_modes<T>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
+  });
   prof_end(6);
   c.finish();
   FEL_CHECK(hipStreamSynchronize(stream()));  // h is released on return
@@ -622,8 +638,7 @@ void solp_fft_tdma3_(const int* ni, const int* nj, const int* nk, double* fftdat
                      const double* phir_r, const double* phir_i, double* dp) {
   (void)fftdata;
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) tdma3_t<float>(d, phir_r, phir_i, dp);
-  else tdma3_t<double>(d, phir_r, phir_i, dp);
+  with_flow([&](auto t) { tdma3_t<decltype(t)>(d, phir_r, phir_i, dp); });
 }
 
 // srcs/solp_fft_tdma4.f:4-5: residual monitor every imon_t steps.
@@ -634,13 +649,17 @@ void solp_fft_tdma4_(const int* ipara, const int* ID, const int* ndiv, const int
   (void)ipara;
   if (!(*imon_t > 0 && (*nstep % *imon_t) == 0)) return;
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("solp_fft_tdma4_");
-  const double* dv = c.in(div, d.n3, "div");
-  const double* dd = c.in(dp, d.n3, "dp");
   double* s = dev_scalars(4);
-  poisson_residual_sums(d, *rho, *dxinv, *dyinv, *dzinv, dv, dd, s);
-  const double* r = read_scalars(s, 2);
-  c.finish();
+  const double* r = nullptr;
+  with_flow([&](auto t) {
+    using P = PT<decltype(t)>;
+    Call c("solp_fft_tdma4_");
+    const P* dv = c.in_t<P>(div, d.n3, "div");
+    const P* dd = c.in_t<P>(dp, d.n3, "dp");
+    poisson_residual_sums<P>(d, *rho, *dxinv, *dyinv, *dzinv, dv, dd, s);
+    r = read_scalars(s, 2);
+    c.finish();
+  });
   const double nn = (double)(*ni * *nj * *nk * *ndiv);
   const double e0 = std::sqrt(r[0] / nn) + 1.0e-99, e = std::sqrt(r[1] / nn) + 1.0e-99;
   if (*ID == 0) std::printf("Err_solp %10d%20.10E%20.10E%20.10E\n", *nstep, e / e0, e, e0);
@@ -651,13 +670,17 @@ double felgpu_poisson_residual_(const int* ni, const int* nj, const int* nk, con
                                 const double* dxinv, const double* dyinv, const double* dzinv,
                                 const double* div, const double* dp) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("felgpu_poisson_residual_");
-  const double* dv = c.in(div, d.n3, "div");
-  const double* dd = c.in(dp, d.n3, "dp");
   double* s = dev_scalars(4);
-  poisson_residual_sums(d, *rho, *dxinv, *dyinv, *dzinv, dv, dd, s);
-  const double* r = read_scalars(s, 2);
-  c.finish();
+  const double* r = nullptr;
+  with_flow([&](auto t) {
+    using P = PT<decltype(t)>;
+    Call c("felgpu_poisson_residual_");
+    const P* dv = c.in_t<P>(div, d.n3, "div");
+    const P* dd = c.in_t<P>(dp, d.n3, "dp");
+    poisson_residual_sums<P>(d, *rho, *dxinv, *dyinv, *dzinv, dv, dd, s);
+    r = read_scalars(s, 2);
+    c.finish();
+  });
   const double nn = (double)(*ni * *nj * *nk);
   return (std::sqrt(r[1] / nn) + 1.0e-99) / (std::sqrt(r[0] / nn) + 1.0e-99);
 }
@@ -667,8 +690,7 @@ void bnd_avzero_(const int* ipara, const int* ndiv, const int* ni, const int* nj
                  double* q) {
   (void)ipara;
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (elem_size(q) == 4) avzero_t<float>(d, q, *ndiv);
-  else avzero_t<double>(d, q, *ndiv);
+  with_elem(elem_size(q), [&](auto t) { avzero_t<decltype(t)>(d, q, *ndiv); });
 }
 
 // srcs/cal_av.f:1
@@ -676,7 +698,8 @@ void cal_av_(const int* ipara, const int* ndiv, const int* ni, const int* nj, co
              const double* q, double* av) {
   (void)ipara;
   const Dims d = make_dims(*ni, *nj, *nk);
-  const double sum = elem_size(q) == 4 ? cal_av_t<float>(d, q) : cal_av_t<double>(d, q);
+  double sum = 0.0;
+  with_elem(elem_size(q), [&](auto t) { sum = cal_av_t<decltype(t)>(d, q); });
   *av = sum / (double)(*ni * *nj * *nk * *ndiv);
 }
 
@@ -687,8 +710,9 @@ void corunp_explicit_(const int* nID, const int* ni, const int* nj, const int* n
                       double* pn) {
   const Dims d = make_dims(*ni, *nj, *nk);
   const int jen = (nID[kYPlus] < 0) ? *nj - 1 : *nj;
-  if (flow_fp32()) corunp_t<float>(d, jen, *rho, *dxinv, *dyinv, *dzinv, *dt, dp, phat, un, vn, wn, pn);
-  else corunp_t<double>(d, jen, *rho, *dxinv, *dyinv, *dzinv, *dt, dp, phat, un, vn, wn, pn);
+  with_flow([&](auto t) {
+    corunp_t<decltype(t)>(d, jen, *rho, *dxinv, *dyinv, *dzinv, *dt, dp, phat, un, vn, wn, pn);
+  });
 }
 
 // srcs/cal_coef_prs.f:5-57 (set-up, host arrays declared (-2:nj+3))

@@ -23,7 +23,8 @@
 namespace fel {
 
 // T: element type of the velocities, densities, viscosity and sources
-// (double, or float in the FP32 mode).
+// (double, float in the FP32 mode, Dual in the tangent mode).  The residual
+// of the stop test is summed from the values.
 template <class T>
 struct SorArgs {
   const T *rho, *rhon, *src[3], *rmu;
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
                           r00;
       const T du = (visn * h + a.src[0][c]) / (a.dtinv + cf[6] / r00 * h) - un[c];
       a.un[c] = un[c] + du;
-      e += (double)du * (double)du;
+      e += val(du) * val(du);
     } else if (COMP == 1) {
       const T visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
                            cf[1] * vn[c + X] + cf[3] * vn[c + Y] + cf[5] * vn[c + Z] +
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
                           r00;
       const T dv = (visn * h + a.src[1][c]) / (a.dtinv + cf[6] / r00 * h) - vn[c];
       a.vn[c] = vn[c] + dv;
-      e += (double)dv * (double)dv;
+      e += val(dv) * val(dv);
     } else {
       const T visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
                            cf[1] * wn[c + X] + cf[3] * wn[c + Y] + cf[5] * wn[c + Z] +
@@ -99,7 +100,7 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
                           r00;
       const T dw = (visn * h + a.src[2][c]) / (a.dtinv + cf[6] / r00 * h) - wn[c];
       a.wn[c] = wn[c] + dw;
-      e += (double)dw * (double)dw;
+      e += val(dw) * val(dw);
     }
   }
   }
@@ -156,9 +157,9 @@ struct SorResult {
   double err, err0;
 };
 
-template <class T>
+template <class T, class P = PT<T>>
 SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double uwall, double dy,
-               const double* l1, const double* l2, const double* phi, int ndiv) {
+               const P* l1, const P* l2, const P* phi, int ndiv) {
   const int jen = hi ? d.nj - 1 : d.nj;
   const int gx = ((d.ni + 1) / 2 + 63) / 64;  // cells of one parity per row
   const int gy = ((d.nj + 1) / 2 + 3) / 4;    // rows of one parity per plane
@@ -198,7 +199,7 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
         // After the first pair of a component's sweep only that component
         // changes, and bndu's ghosts of the other two are functions of their
         // unchanged interiors: rewriting them would store the same values.
-        bc_fused<T>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp,
+        bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp,
                     p == 0 ? 7 : (1 << comp));
         prof_end(4);
       }
@@ -269,9 +270,10 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
       r00[2][c] = (rho_d[c] + rho_d[c + Z] + rhon_d[c] + rhon_d[c + Z]) * q;
     });
   }
-  const double* dl1 = c.in(l1, d.n3, "l1");
-  const double* dl2 = c.in(l2, d.n3, "l2");
-  const double* dphi = c.in(phi, d.n3, "phi");
+  using P = PT<T>;
+  const P* dl1 = c.in_t<P>(l1, d.n3, "l1");
+  const P* dl2 = c.in_t<P>(l2, d.n3, "l2");
+  const P* dphi = c.in_t<P>(phi, d.n3, "phi");
   const SorResult r = sor4<T>(d, a, onfly, jm < 0, jp < 0, uwall, dy, dl1, dl2, dphi, ndiv);
   c.finish();
   return r;
@@ -315,10 +317,11 @@ void solu_sor4_(const int* ipara, const int* ID, const int* nID, const int* ndiv
                    c15, c16, c17, c18, c19, c20, c21, c22, c23, c24, c25, c26, c27, c28, c29,
                    c30, c31, c32, c33, c34, c35, c36, c37, c38, c39, c40, c41, c42, c43, c44};
   const int jm = nID[kYMinus], jp = nID[kYPlus];
-  const SorResult r = flow_fp32() ? solu_sor4_t<float>(d, *dt, rho, rhon, h, src_u, src_v, src_w, un,
-                                                      vn, wn, *uwall, *dy, l1, l2, phi, jm, jp, *ndiv)
-                                  : solu_sor4_t<double>(d, *dt, rho, rhon, h, src_u, src_v, src_w, un,
-                                                       vn, wn, *uwall, *dy, l1, l2, phi, jm, jp, *ndiv);
+  SorResult r{0, 0.0, 1.0};
+  with_flow([&](auto t) {
+    r = solu_sor4_t<decltype(t)>(d, *dt, rho, rhon, h, src_u, src_v, src_w, un, vn, wn, *uwall, *dy,
+                                 l1, l2, phi, jm, jp, *ndiv);
+  });
   g_last_lp = r.lp;
   g_last_ratio = r.err / r.err0;
   if (*imon_t > 0 && (*nstep % *imon_t) == 0 && *ID == 0)

@@ -13,14 +13,15 @@ namespace fel {
 
 __device__ __forceinline__ bool inb(int i, int a, int b) { return i >= a && i <= b; }
 
-// T is the flow element type (double, or float in the FP32 mode); the volume
-// fraction is always FP64.  With T = double every expression is the
+// T is the flow element type (double, float in the FP32 mode, Dual in the
+// tangent mode); the volume fraction is in the phase type PT<T> (FP64 unless
+// in the tangent mode).  With T = double every expression is the
 // reference's FP64 one.
 
-template <class T>
-void arith_mean(const Dims d, const double* phik, double ql, double qg, T* qk) {
+template <class T, class P = PT<T>>
+void arith_mean(const Dims d, const P* phik, double ql, double qg, T* qk) {
   launch_flat(d.n3, [=] __device__(int m) {
-    const double pl = 1.0 - phik[m], pg = phik[m];
+    const P pl = 1.0 - phik[m], pg = phik[m];
     qk[m] = T(ql * pl + qg * pg);
   });
 }
@@ -163,15 +164,12 @@ void coef_vis(const Dims d, double dxinv, double dyinv, double dzinv, const T* r
 
 void coef_vis_any(const Dims d, double dxinv, double dyinv, double dzinv, const void* rmu,
                   void* const* out45, int esize) {
-  if (esize == 4) {
-    Coef45<float> a;
-    for (int m = 0; m < 45; m++) a.p[m] = static_cast<float*>(out45[m]);
-    coef_vis<float>(d, dxinv, dyinv, dzinv, static_cast<const float*>(rmu), a);
-  } else {
-    Coef45<double> a;
-    for (int m = 0; m < 45; m++) a.p[m] = static_cast<double*>(out45[m]);
-    coef_vis<double>(d, dxinv, dyinv, dzinv, static_cast<const double*>(rmu), a);
-  }
+  with_elem(esize, [&](auto t) {
+    using T = decltype(t);
+    Coef45<T> a;
+    for (int m = 0; m < 45; m++) a.p[m] = static_cast<T*>(out45[m]);
+    coef_vis<T>(d, dxinv, dyinv, dzinv, static_cast<const T*>(rmu), a);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -181,7 +179,7 @@ void coef_vis_any(const Dims d, double dxinv, double dyinv, double dzinv, const 
 template <class T>
 void cal_arith_mean_t(const Dims d, const double* phik, double ql, double qg, double* qk) {
   Call c("cal_arith_mean_");
-  const double* p = c.in(phik, d.n3, "phik");
+  const PT<T>* p = c.in_t<PT<T>>(phik, d.n3, "phik");
   T* q = c.out_t<T>(qk, d.n3, "qk");
   arith_mean<T>(d, p, ql, qg, q);
   c.finish();
@@ -344,16 +342,14 @@ extern "C" {
 void cal_arith_mean_(const int* ni, const int* nj, const int* nk, const double* phik,
                      const double* ql, const double* qg, double* qk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) cal_arith_mean_t<float>(d, phik, *ql, *qg, qk);
-  else cal_arith_mean_t<double>(d, phik, *ql, *qg, qk);
+  with_flow([&](auto t) { cal_arith_mean_t<decltype(t)>(d, phik, *ql, *qg, qk); });
 }
 
 // srcs/calphat.f:1
 void calphat_(const int* ni, const int* nj, const int* nk, const double* p, const double* po,
               double* phat) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) calphat_t<float>(d, p, po, phat);
-  else calphat_t<double>(d, p, po, phat);
+  with_flow([&](auto t) { calphat_t<decltype(t)>(d, p, po, phat); });
 }
 
 // srcs/cal_grad_prs.f:1-3
@@ -361,24 +357,23 @@ void cal_grad_prs_(const int* ni, const int* nj, const int* nk, const double* dx
                    const double* dyinv, const double* dzinv, const double* q, double* qx_u,
                    double* qy_v, double* qz_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) cal_grad_prs_t<float>(d, *dxinv, *dyinv, *dzinv, q, qx_u, qy_v, qz_w);
-  else cal_grad_prs_t<double>(d, *dxinv, *dyinv, *dzinv, q, qx_u, qy_v, qz_w);
+  with_flow([&](auto t) {
+    cal_grad_prs_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, q, qx_u, qy_v, qz_w);
+  });
 }
 
 // srcs/calsij.f:5-6
 void calsij_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
              const double* dzinv, const double* uk, const double* vk, const double* wk, double* sk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) calsij_t<float>(d, *dxinv, *dyinv, *dzinv, uk, vk, wk, sk);
-  else calsij_t<double>(d, *dxinv, *dyinv, *dzinv, uk, vk, wk, sk);
+  with_flow([&](auto t) { calsij_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, uk, vk, wk, sk); });
 }
 
 // srcs/cal_arith_tau.f:7
 void cal_arith_tau_(const int* ni, const int* nj, const int* nk, const double* s, const double* rmu,
                     double* tau) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) cal_arith_tau_t<float>(d, s, rmu, tau);
-  else cal_arith_tau_t<double>(d, s, rmu, tau);
+  with_flow([&](auto t) { cal_arith_tau_t<decltype(t)>(d, s, rmu, tau); });
 }
 
 // srcs/cal_div_tensor.f:1-2
@@ -386,8 +381,9 @@ void cal_div_tensor_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* dyinv, const double* dzinv, const double* tauk, double* uk,
                      double* vk, double* wk) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) cal_div_tensor_t<float>(d, *dxinv, *dyinv, *dzinv, tauk, uk, vk, wk);
-  else cal_div_tensor_t<double>(d, *dxinv, *dyinv, *dzinv, tauk, uk, vk, wk);
+  with_flow([&](auto t) {
+    cal_div_tensor_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, tauk, uk, vk, wk);
+  });
 }
 
 // srcs/cal_srcu.f:1-10
@@ -407,8 +403,7 @@ void cal_srcu_(const int* nID, const int* ni, const int* nj, const int* nk, cons
                           vis_w,  fst_u,  fst_v,  fst_w,  fst_un, fst_vn, fst_wn};
   double* out[3] = {src_u, src_v, src_w};
   const int jen = (nID[kYPlus] < 0) ? *nj - 1 : *nj;
-  if (flow_fp32()) cal_srcu_t<float>(d, jen, *dt, *grvb, *grvp, in, out);
-  else cal_srcu_t<double>(d, jen, *dt, *grvb, *grvp, in, out);
+  with_flow([&](auto t) { cal_srcu_t<decltype(t)>(d, jen, *dt, *grvb, *grvp, in, out); });
 }
 
 // srcs/cal_arith_coef_vis.f:6-19
@@ -427,30 +422,26 @@ void cal_arith_coef_vis_(const int* ni, const int* nj, const int* nk, const doub
   double* h[45] = {a0,  a1,  a2,  a3,  a4,  a5,  a6,  a7,  a8,  a9,  a10, a11, a12, a13, a14,
                    a15, a16, a17, a18, a19, a20, a21, a22, a23, a24, a25, a26, a27, a28, a29,
                    a30, a31, a32, a33, a34, a35, a36, a37, a38, a39, a40, a41, a42, a43, a44};
-  if (flow_fp32()) cal_arith_coef_vis_t<float>(d, *dxinv, *dyinv, *dzinv, rmuk, h);
-  else cal_arith_coef_vis_t<double>(d, *dxinv, *dyinv, *dzinv, rmuk, h);
+  with_flow([&](auto t) { cal_arith_coef_vis_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, rmuk, h); });
 }
 
 // srcs/cpy.f:1 (either precision class: copies the device copy as it is held)
 void cpy_(const int* ni, const int* nj, const int* nk, const double* q1, double* q2) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (elem_size(q1) == 4) cpy_t<float>(d, q1, q2);
-  else cpy_t<double>(d, q1, q2);
+  with_elem(elem_size(q1), [&](auto t) { cpy_t<decltype(t)>(d, q1, q2); });
 }
 
 // srcs/init_q.f:1
 void init_q_(const int* ni, const int* nj, const int* nk, double* q1, double* q2, double* q3) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (elem_size(q1) == 4) init_q_t<float>(d, q1, q2, q3);
-  else init_q_t<double>(d, q1, q2, q3);
+  with_elem(elem_size(q1), [&](auto t) { init_q_t<decltype(t)>(d, q1, q2, q3); });
 }
 
 // srcs/sum_fst.f:1-3
 void sum_fst_(const int* ni, const int* nj, const int* nk, const double* fst_u, const double* fst_v,
               const double* fst_w, double* sum_u, double* sum_v, double* sum_w) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) sum_fst_t<float>(d, fst_u, fst_v, fst_w, sum_u, sum_v, sum_w);
-  else sum_fst_t<double>(d, fst_u, fst_v, fst_w, sum_u, sum_v, sum_w);
+  with_flow([&](auto t) { sum_fst_t<decltype(t)>(d, fst_u, fst_v, fst_w, sum_u, sum_v, sum_w); });
 }
 
 }  // extern "C"

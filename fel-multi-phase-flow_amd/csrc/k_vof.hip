@@ -35,19 +35,20 @@ static DyTable dy_table(const double* host_arr, int count, int slot) {
   return t;
 }
 
-// Vertex gradient, srcs/cal_grad_p2a.f:30-71, box [-2,n+2]^3, formed in FP64
-// from the FP64 volume fraction and stored in the gradient element type T.
-template <class T>
-void grad_p2a(const Dims d, int ID, double dxinv, DyTable dyt, double dzinv, const double* q, T* qx,
+// Vertex gradient, srcs/cal_grad_p2a.f:30-71, box [-2,n+2]^3, formed in the
+// phase type P (FP64) from the volume fraction and stored in the gradient
+// element type T.
+template <class T, class P = PT<T>>
+void grad_p2a(const Dims d, int ID, double dxinv, DyTable dyt, double dzinv, const P* q, T* qx,
               T* qy, T* qz) {
   launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
-    const double q000 = q[c], qp00 = q[c + X], q0p0 = q[c + Y], qpp0 = q[c + X + Y];
-    const double q00p = q[c + Z], qp0p = q[c + X + Z], q0pp = q[c + Y + Z], qppp = q[c + X + Y + Z];
-    const double gx = -q000 + qp00 - q0p0 + qpp0 - q00p + qp0p - q0pp + qppp;
-    const double gy = -q000 - qp00 + q0p0 + qpp0 - q00p - qp0p + q0pp + qppp;
-    const double gz = -q000 - qp00 - q0p0 - qpp0 + q00p + qp0p + q0pp + qppp;
+    const P q000 = q[c], qp00 = q[c + X], q0p0 = q[c + Y], qpp0 = q[c + X + Y];
+    const P q00p = q[c + Z], qp0p = q[c + X + Z], q0pp = q[c + Y + Z], qppp = q[c + X + Y + Z];
+    const P gx = -q000 + qp00 - q0p0 + qpp0 - q00p + qp0p - q0pp + qppp;
+    const P gy = -q000 - qp00 + q0p0 + qpp0 - q00p - qp0p + q0pp + qppp;
+    const P gz = -q000 - qp00 - q0p0 - qpp0 + q00p + qp0p + q0pp + qppp;
     const double dyinv = dyt.at(j + d.nj * ID);
     qx[c] = T(gx * 0.25 * dxinv);
     qy[c] = T(gy * 0.25 * dyinv);
@@ -122,11 +123,12 @@ static MthincConst mthinc_const(double dxinv, double dyinv, double dzinv, double
 // reference's zero-fill pass (solphi_mthinc1.f:44-52).  Each face therefore
 // has exactly one writer.
 // T: element type of the gradients, the velocities and of the MTHINC
-// arithmetic (double, or float in the FP32 mode); the fluxes are stored FP64.
-template <class T>
+// arithmetic (double, or float in the FP32 mode); the fluxes are stored in
+// the phase type P (FP64).
+template <class T, class P = PT<T>>
 void mthinc_flux(const Dims d, double dxinv_, double dyinv_, double dzinv_, double dt_,
-                 MthincConst mc64, const double* phi, const T* phix, const T* phiy, const T* phiz,
-                 const T* u, const T* v, const T* w, double* flx, double* fly, double* flz) {
+                 MthincConst mc64, const P* phi, const T* phix, const T* phiy, const T* phiz,
+                 const T* u, const T* v, const T* w, P* flx, P* fly, P* flz) {
   const int n1 = d.ni + 1, n2 = d.nj + 1, n3 = d.nk + 1;
   const MthincC<T> mc(mc64);
   const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_), dt = T(dt_);
@@ -140,11 +142,11 @@ void mthinc_flux(const Dims d, double dxinv_, double dyinv_, double dzinv_, doub
     // zero the right faces that have no outflow writer
     {
       const bool rx = (i + 1 <= n1) && in_j && in_k;  // right neighbour in [0,n+1]^3
-      if (up == zero || (up > zero && !self) || (up < zero && !rx) || !in_j || !in_k) flx[c] = 0.0;
+      if (up == zero || (up > zero && !self) || (up < zero && !rx) || !in_j || !in_k) flx[c] = P(0.0);
       const bool ry = (j + 1 <= n2) && in_i && in_k;
-      if (vp == zero || (vp > zero && !self) || (vp < zero && !ry) || !in_i || !in_k) fly[c] = 0.0;
+      if (vp == zero || (vp > zero && !self) || (vp < zero && !ry) || !in_i || !in_k) fly[c] = P(0.0);
       const bool rz = (k + 1 <= n3) && in_i && in_j;
-      if (wp == zero || (wp > zero && !self) || (wp < zero && !rz) || !in_i || !in_j) flz[c] = 0.0;
+      if (wp == zero || (wp > zero && !self) || (wp < zero && !rz) || !in_i || !in_j) flz[c] = P(0.0);
     }
     if (!self) return;
     const T um = u[c - X], vm = v[c - Y], wm = w[c - Z];
@@ -225,20 +227,19 @@ void mthinc_flux(const Dims d, double dxinv_, double dyinv_, double dzinv_, doub
 // srcs/solphi_mthinc2.f:27-44.
 // The update is FP64: velocity differences are formed from the velocities
 // (FP32 values are exact in FP64) and the fluxes are FP64.
-template <class T>
+template <class T, class P = PT<T>>
 void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, double dt, const T* u,
-                   const T* v, const T* w, const double* flx, const double* fly, const double* flz,
-                   const double* phi, double* phin) {
+                   const T* v, const T* w, const P* flx, const P* fly, const P* flz, const P* phi,
+                   P* phin) {
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
-    const double div00 = +(-(double)u[c - X] + (double)u[c]) * dxinv +
-                         (-(double)v[c - Y] + (double)v[c]) * dyinv +
-                         (-(double)w[c - Z] + (double)w[c]) * dzinv;
-    double p = phi[c] * (1.0 + dt * div00) + flx[c - X] - flx[c] + fly[c - Y] - fly[c] +
-               flz[c - Z] - flz[c];
-    p = fmin(p, 1.0);
-    p = fmax(p, 0.0);
+    const P div00 = +(-P(u[c - X]) + P(u[c])) * dxinv + (-P(v[c - Y]) + P(v[c])) * dyinv +
+                    (-P(w[c - Z]) + P(w[c])) * dzinv;
+    P p = phi[c] * (1.0 + dt * div00) + flx[c - X] - flx[c] + fly[c - Y] - fly[c] + flz[c - Z] -
+          flz[c];
+    p = fmin(p, P(1.0));
+    p = fmax(p, P(0.0));
     phin[c] = p;
   });
 }
@@ -246,9 +247,9 @@ void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, doubl
 // srcs/solphi_mthinc3.f:43-159: truncation on [0,n+1]^3 (clamped in the
 // interior in the same pass, the reference's second loop), the three global
 // sums, and the mass correction.  The offset stays on the device.
-template <class T>
+template <class T, class P = PT<T>>
 void mthinc_mass(const Dims d, MthincConst mc64, const T* phix, const T* phiy, const T* phiz,
-                 const double* phi, double* phin) {
+                 const P* phi, P* phin) {
   const MthincC<T> mc(mc64);
   launch_box(Box{0, d.ni + 1, 0, d.nj + 1, 0, d.nk + 1}, [=] __device__(int i, int j, int k) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
@@ -257,55 +258,60 @@ void mthinc_mass(const Dims d, MthincConst mc64, const T* phix, const T* phiy, c
     const T rny = vavg8(phiy, c, X, Y, Z);
     const T rnz = vavg8(phiz, c, X, Y, Z);
     const T q00 = sqrt(rnx * rnx + rny * rny + rnz * rnz);
-    double p = phin[c];
-    const double p0 = p;
+    P p = phin[c];
+    const P p0 = p;
     if (q00 <= mc.eps_mthinc * mc.dcellinv) {
-      if (p <= mc64.eps_trunc) p = 0.0;
-      if (1.0 - p <= mc64.eps_trunc) p = 1.0;
+      if (p <= mc64.eps_trunc) p = P(0.0);
+      if (1.0 - p <= mc64.eps_trunc) p = P(1.0);
     }
     if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk) {
-      p = fmin(p, 1.0);
-      p = fmax(p, 0.0);
+      p = fmin(p, P(1.0));
+      p = fmax(p, P(0.0));
     }
-    if (p != p0) phin[c] = p;
+    // a truncated or clamped cell also loses its derivative
+    if (p != p0 || (is_dual<P> && !(p0 > 0.0 && p0 < 1.0))) phin[c] = p;
   });
+  // the sums and the offset, in P, stay on the device (slot 0 holds 4 P's)
   double* sums = dev_scalars(0);
-  interior_reduce<3>(
+  interior_reduce<3, P>(
       d,
-      [=] __device__(int i, int j, int k, double* v) {
+      [=] __device__(int i, int j, int k, P* v) {
         const int c = d.ix(i, j, k);
-        const double p = phin[c];
+        const P p = phin[c];
         v[0] = phi[c];
         v[1] = p;
         v[2] = p * p * (1.0 - p);
       },
-      [=] __device__(const double* t, double* out) {
-        out[0] = t[0];
-        out[1] = t[1];
-        out[2] = t[2];
-        out[3] = (t[0] - t[1]) / (t[2] + 1.0e-99);
+      [=] __device__(const P* t, double* out) {
+        P* o = reinterpret_cast<P*>(out);
+        o[0] = t[0];
+        o[1] = t[1];
+        o[2] = t[2];
+        o[3] = (t[0] - t[1]) / (t[2] + 1.0e-99);
       },
       sums);
-  const double* offp = sums + 3;
+  const P* offp = reinterpret_cast<const P*>(sums) + 3;
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    const double offset = *offp;
-    double p = phin[c];
+    const P offset = *offp;
+    P p = phin[c];
     p = p + offset * (p * p) * (1.0 - p);
-    p = fmin(p, 1.0);
-    phin[c] = fmax(p, 0.0);
+    p = fmin(p, P(1.0));
+    phin[c] = fmax(p, P(0.0));
   });
 }
 
 // srcs/summation.f:14-50 for nbub slices: slice 0 = sum of slices 1..nbub
 // over the padded image, clamped to [0,1] in the interior.
-void summation(const Dims d, double* qk, int nbub) {
+template <class P>
+void summation(const Dims d, P* qk, int nbub) {
   const int n3 = d.n3;
   launch_flat(n3, [=] __device__(int m) {
-    double s = qk[n3 + m];
+    P s = qk[n3 + m];
     for (int l = 2; l <= nbub; l++) s = s + qk[(size_t)l * n3 + m];
     const int i = m % d.sx - 2, r = m / d.sx, j = r % d.sy - 2, k = r / d.sy - 2;
-    if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk) s = fmin(fmax(s, 0.0), 1.0);
+    if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk)
+      s = fmin(fmax(s, P(0.0)), P(1.0));
     qk[m] = s;
   });
 }
@@ -314,7 +320,7 @@ template <class T>
 void cal_grad_p2a_t(const Dims d, int ID, int svall, double dxinv, const double* dyinv_array,
                     double dzinv, const double* q, double* qx, double* qy, double* qz) {
   Call c("cal_grad_p2a_");
-  const double* dq = c.in(q, d.n3, "q");
+  const PT<T>* dq = c.in_t<PT<T>>(q, d.n3, "q");
   T* dx = c.io_t<T>(qx, d.n3, "qx");
   T* dy = c.io_t<T>(qy, d.n3, "qy");
   T* dz = c.io_t<T>(qz, d.n3, "qz");
@@ -327,17 +333,18 @@ void solphi_mthinc1_t(const Dims d, double dxinv, double dyinv, double dzinv, do
                       const double* phi, const double* phix, const double* phiy, const double* phiz,
                       const double* u, const double* v, const double* w, double* flphix,
                       double* flphiy, double* flphiz) {
+  using P = PT<T>;
   Call c("solphi_mthinc1_");
-  const double* dphi = c.in(phi, d.n3, "phi");
+  const P* dphi = c.in_t<P>(phi, d.n3, "phi");
   const T* gx = c.in_t<T>(phix, d.n3, "phix");
   const T* gy = c.in_t<T>(phiy, d.n3, "phiy");
   const T* gz = c.in_t<T>(phiz, d.n3, "phiz");
   const T* du = c.in_t<T>(u, d.n3, "u");
   const T* dv = c.in_t<T>(v, d.n3, "v");
   const T* dw = c.in_t<T>(w, d.n3, "w");
-  double* fx = c.io(flphix, d.n3, "flphix");
-  double* fy = c.io(flphiy, d.n3, "flphiy");
-  double* fz = c.io(flphiz, d.n3, "flphiz");
+  P* fx = c.io_t<P>(flphix, d.n3, "flphix");
+  P* fy = c.io_t<P>(flphiy, d.n3, "flphiy");
+  P* fz = c.io_t<P>(flphiz, d.n3, "flphiz");
   prof_begin(1);
   mthinc_flux<T>(d, dxinv, dyinv, dzinv, dt, mthinc_const(dxinv, dyinv, dzinv, bet), dphi, gx, gy,
                  gz, du, dv, dw, fx, fy, fz);
@@ -349,15 +356,16 @@ template <class T>
 void solphi_mthinc2_t(const Dims d, double dxinv, double dyinv, double dzinv, double dt,
                       const double* u, const double* v, const double* w, const double* flphix,
                       const double* flphiy, const double* flphiz, const double* phi, double* phin) {
+  using P = PT<T>;
   Call c("solphi_mthinc2_");
   const T* du = c.in_t<T>(u, d.n3, "u");
   const T* dv = c.in_t<T>(v, d.n3, "v");
   const T* dw = c.in_t<T>(w, d.n3, "w");
-  const double* fx = c.in(flphix, d.n3, "flphix");
-  const double* fy = c.in(flphiy, d.n3, "flphiy");
-  const double* fz = c.in(flphiz, d.n3, "flphiz");
-  const double* dphi = c.in(phi, d.n3, "phi");
-  double* dphin = c.io(phin, d.n3, "phin");
+  const P* fx = c.in_t<P>(flphix, d.n3, "flphix");
+  const P* fy = c.in_t<P>(flphiy, d.n3, "flphiy");
+  const P* fz = c.in_t<P>(flphiz, d.n3, "flphiz");
+  const P* dphi = c.in_t<P>(phi, d.n3, "phi");
+  P* dphin = c.io_t<P>(phin, d.n3, "phin");
   mthinc_update<T>(d, dxinv, dyinv, dzinv, dt, du, dv, dw, fx, fy, fz, dphi, dphin);
   c.finish();
 }
@@ -366,12 +374,13 @@ template <class T>
 void solphi_mthinc3_t(const Dims d, double dxinv, double dyinv, double dzinv, double bet,
                       const double* phix, const double* phiy, const double* phiz, const double* phi,
                       double* phin) {
+  using P = PT<T>;
   Call c("solphi_mthinc3_");
   const T* gx = c.in_t<T>(phix, d.n3, "phix");
   const T* gy = c.in_t<T>(phiy, d.n3, "phiy");
   const T* gz = c.in_t<T>(phiz, d.n3, "phiz");
-  const double* dphi = c.in(phi, d.n3, "phi");
-  double* dphin = c.io(phin, d.n3, "phin");
+  const P* dphi = c.in_t<P>(phi, d.n3, "phi");
+  P* dphin = c.io_t<P>(phin, d.n3, "phin");
   mthinc_mass<T>(d, mthinc_const(dxinv, dyinv, dzinv, bet), gx, gy, gz, dphi, dphin);
   c.finish();
 }
@@ -387,8 +396,9 @@ void cal_grad_p2a_(const int* ID, const int* svall, const int* ni, const int* nj
                    const double* dxinv, const double* dyinv_array, const double* dzinv,
                    const double* q, double* qx, double* qy, double* qz) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32()) cal_grad_p2a_t<float>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx, qy, qz);
-  else cal_grad_p2a_t<double>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx, qy, qz);
+  with_flow([&](auto t) {
+    cal_grad_p2a_t<decltype(t)>(d, *ID, *svall, *dxinv, dyinv_array, *dzinv, q, qx, qy, qz);
+  });
 }
 
 // srcs/solphi_mthinc1.f:1-6
@@ -398,12 +408,10 @@ void solphi_mthinc1_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* phiy, const double* phiz, const double* u, const double* v,
                      const double* w, double* flphix, double* flphiy, double* flphiz) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32())
-    solphi_mthinc1_t<float>(d, *dxinv, *dyinv, *dzinv, *dt, *bet_mthinc, phi, phix, phiy, phiz, u, v,
-                            w, flphix, flphiy, flphiz);
-  else
-    solphi_mthinc1_t<double>(d, *dxinv, *dyinv, *dzinv, *dt, *bet_mthinc, phi, phix, phiy, phiz, u,
-                             v, w, flphix, flphiy, flphiz);
+  with_flow([&](auto t) {
+    solphi_mthinc1_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, *dt, *bet_mthinc, phi, phix, phiy,
+                                  phiz, u, v, w, flphix, flphiy, flphiz);
+  });
 }
 
 // srcs/solphi_mthinc2.f:1-2
@@ -412,12 +420,10 @@ void solphi_mthinc2_(const int* ni, const int* nj, const int* nk, const double* 
                      const double* v, const double* w, const double* flphix, const double* flphiy,
                      const double* flphiz, const double* phi, double* phin) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32())
-    solphi_mthinc2_t<float>(d, *dxinv, *dyinv, *dzinv, *dt, u, v, w, flphix, flphiy, flphiz, phi,
-                            phin);
-  else
-    solphi_mthinc2_t<double>(d, *dxinv, *dyinv, *dzinv, *dt, u, v, w, flphix, flphiy, flphiz, phi,
-                             phin);
+  with_flow([&](auto t) {
+    solphi_mthinc2_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, *dt, u, v, w, flphix, flphiy, flphiz,
+                                  phi, phin);
+  });
 }
 
 // srcs/solphi_mthinc3.f:1-5 (ipara must be 0: one GPU, no MPI)
@@ -430,19 +436,22 @@ void solphi_mthinc3_(const int* ipara, const int* ni, const int* nj, const int* 
     std::abort();
   }
   const Dims d = make_dims(*ni, *nj, *nk);
-  if (flow_fp32())
-    solphi_mthinc3_t<float>(d, *dxinv, *dyinv, *dzinv, *bet_mthinc, phix, phiy, phiz, phi, phin);
-  else
-    solphi_mthinc3_t<double>(d, *dxinv, *dyinv, *dzinv, *bet_mthinc, phix, phiy, phiz, phi, phin);
+  with_flow([&](auto t) {
+    solphi_mthinc3_t<decltype(t)>(d, *dxinv, *dyinv, *dzinv, *bet_mthinc, phix, phiy, phiz, phi,
+                                  phin);
+  });
 }
 
 // srcs/summation.f:1
 void summation_(const int* ni, const int* nj, const int* nk, double* qk, const int* nbub) {
   const Dims d = make_dims(*ni, *nj, *nk);
-  Call c("summation_");
-  double* dq = c.io(qk, (size_t)d.n3 * (*nbub + 1));
-  summation(d, dq, *nbub);
-  c.finish();
+  with_flow([&](auto t) {
+    using P = PT<decltype(t)>;
+    Call c("summation_");
+    P* dq = c.io_t<P>(qk, (size_t)d.n3 * (*nbub + 1), "qk");
+    summation<P>(d, dq, *nbub);
+    c.finish();
+  });
 }
 
 }  // extern "C"

@@ -31,7 +31,7 @@ __device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp
   const T dfp_2 = -dfp3 + dfp1 * c5 + dfm1 * c2;
   const T dfp_3 = dfp1 * c2 + dfm1 * c5 - dfm3;
   T m5 = dfm5, m3 = dfm3, m1 = dfm1, p1 = dfp1, p3 = dfp3, p5 = dfp5;
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (is_f32<T>) {
     const T sc = fmax(fmax(fmax(fabs(m5), fabs(m3)), fmax(fabs(m1), fabs(p1))),
                       fmax(fabs(p3), fabs(p5)));
     const T inv = sc > T(0.0) ? T(1.0) / sc : T(0.0);
@@ -310,10 +310,9 @@ void cal_advu_weno5_(const int* ni, const int* nj, const int* nk, const double* 
       g_stash.valid = h_valid;
       if (h_valid) g_stash.key = h->key;
       Call c("cal_advu_weno5_");  // the outputs change: fresh versions
-      for (int m = 0; m < 3; m++) {
-        if (flow_fp32()) c.out_t<float>(out[m], n, "adv");
-        else c.out(out[m], n, "adv");
-      }
+      with_flow([&](auto t) {
+        for (int m = 0; m < 3; m++) c.out_t<decltype(t)>(out[m], n, "adv");
+      });
       c.finish();
       record_held(out, n, key);
       return;
@@ -333,8 +332,9 @@ void cal_advu_weno5_(const int* ni, const int* nj, const int* nk, const double* 
       g_stash.key = h->key;
     }
   }
-  if (flow_fp32()) weno_compute<float>(d, *dxinv, *dyinv, *dzinv, u, v, w, adv_u, adv_v, adv_w);
-  else weno_compute<double>(d, *dxinv, *dyinv, *dzinv, u, v, w, adv_u, adv_v, adv_w);
+  with_flow([&](auto t) {
+    weno_compute<decltype(t)>(d, *dxinv, *dyinv, *dzinv, u, v, w, adv_u, adv_v, adv_w);
+  });
   if (track) record_held(out, n, key);
 }
 

@@ -24,7 +24,7 @@ struct Range {
   uintptr_t host;
   size_t bytes;  // host bytes (FP64 elements)
   void* dev;
-  int esize;     // device element size: 8, or 4 for an FP32 copy
+  int esize;     // device element size: 8, 4 for an FP32 copy, 16 for a Dual copy
   // content version: a fresh number on every write (entry point output,
   // registration, upload); a whole-array copy hands the source's number to
   // the destination, so equal numbers mean equal contents
@@ -38,6 +38,7 @@ struct Runtime {
   bool init = false;
   int device = 0;
   int flow_esize = 8;  // element size of arrays registered with felgpu_register_
+  bool tangent = false;  // every device copy is a Dual (value, derivative)
   hipStream_t s = nullptr;
   std::map<uintptr_t, Range> reg;         // registered, keyed by host start
   std::map<uintptr_t, Range> transient;   // per-call mirrors, cached
@@ -103,6 +104,11 @@ bool flow_fp32() {
   return R().flow_esize == 4;
 }
 
+bool tangent_mode() {
+  ensure_init();
+  return R().tangent;
+}
+
 int elem_size(const void* host) {
   const Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
   return g ? g->esize : 8;
@@ -110,11 +116,20 @@ int elem_size(const void* host) {
 
 namespace {
 
-// Host FP64 <-> device copies of n elements of size esize.
+// Host FP64 <-> device copies of n elements of size esize.  A Dual copy
+// takes the host values with a zero derivative, and gives back its values.
 void h2d(void* dev, const void* host, size_t n, int esize) {
   Runtime& r = R();
   if (esize == 8) {
     FEL_CHECK(hipMemcpyAsync(dev, host, n * 8, hipMemcpyHostToDevice, r.s));
+    return;
+  }
+  if (esize == 16) {
+    std::vector<Dual> tmp(n);
+    const double* h = static_cast<const double*>(host);
+    for (size_t m = 0; m < n; m++) tmp[m] = Dual(h[m], 0.0);
+    FEL_CHECK(hipMemcpyAsync(dev, tmp.data(), n * 16, hipMemcpyHostToDevice, r.s));
+    FEL_CHECK(hipStreamSynchronize(r.s));
     return;
   }
   std::vector<float> tmp(n);
@@ -128,6 +143,14 @@ void d2h(void* host, const void* dev, size_t n, int esize) {
   Runtime& r = R();
   if (esize == 8) {
     FEL_CHECK(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, r.s));
+    return;
+  }
+  if (esize == 16) {
+    std::vector<Dual> tmp(n);
+    FEL_CHECK(hipMemcpyAsync(tmp.data(), dev, n * 16, hipMemcpyDeviceToHost, r.s));
+    FEL_CHECK(hipStreamSynchronize(r.s));
+    double* h = static_cast<double*>(host);
+    for (size_t m = 0; m < n; m++) h[m] = tmp[m].v;
     return;
   }
   std::vector<float> tmp(n);
@@ -309,9 +332,9 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
   if (Range* g = find_range(r.reg, p, bytes)) {
     if (g->esize != esize) {
       std::fprintf(stderr,
-                   "felgpu: %s: array %s is held in FP%d on the device, this call needs FP%d "
-                   "(register it with felgpu_register%s_)\n",
-                   name_, what, 8 * g->esize, 8 * esize, esize == 8 ? "_fp64" : "");
+                   "felgpu: %s: array %s is held with %d-byte elements on the device, this call "
+                   "needs %d-byte elements (register it with felgpu_register%s_)\n",
+                   name_, what, g->esize, esize, esize == 8 ? "_fp64" : "");
       std::abort();
     }
     if (a & kWrite) g->ver = next_version();
@@ -463,12 +486,23 @@ static void register_range(const void* host, const int64_t* nbytes, const int* u
 // come from zero pages).  In the FP32 mode the device copy is FP32.
 void felgpu_register_(const void* host, const int64_t* nbytes, const int* upload) {
   ensure_init();
-  register_range(host, nbytes, upload, R().flow_esize);
+  register_range(host, nbytes, upload, R().tangent ? 16 : R().flow_esize);
 }
 
-// As felgpu_register_, with an FP64 device copy in either mode.
+// As felgpu_register_, with an FP64 device copy in either precision (a Dual
+// copy in the tangent mode).
 void felgpu_register_fp64_(const void* host, const int64_t* nbytes, const int* upload) {
-  register_range(host, nbytes, upload, 8);
+  ensure_init();
+  register_range(host, nbytes, upload, R().tangent ? 16 : 8);
+}
+
+static Range& must_find(const void* host, const char* who) {
+  Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
+  if (!g) {
+    std::fprintf(stderr, "felgpu: %s: %p is not inside a registered range\n", who, host);
+    std::abort();
+  }
+  return *g;
 }
 
 // Selects the device precision of arrays registered with felgpu_register_
@@ -484,18 +518,65 @@ void felgpu_set_precision_(const int* bits) {
     std::fprintf(stderr, "felgpu: felgpu_set_precision_: arrays are registered\n");
     std::abort();
   }
+  if (*bits == 32 && r.tangent) {
+    std::fprintf(stderr, "felgpu: felgpu_set_precision_: the tangent mode is FP64\n");
+    std::abort();
+  }
   r.flow_esize = *bits / 8;
 }
 
 int felgpu_precision_() { return flow_fp32() ? 32 : 64; }
 
-static Range& must_find(const void* host, const char* who) {
-  Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
-  if (!g) {
-    std::fprintf(stderr, "felgpu: %s: %p is not inside a registered range\n", who, host);
+// Switches the tangent mode on (1) or off (0); only while nothing is
+// registered, and only in FP64.
+void felgpu_set_tangent_(const int* on) {
+  ensure_init();
+  Runtime& r = R();
+  if (!r.reg.empty()) {
+    std::fprintf(stderr, "felgpu: felgpu_set_tangent_: arrays are registered\n");
     std::abort();
   }
-  return *g;
+  if (*on && r.flow_esize != 8) {
+    std::fprintf(stderr, "felgpu: felgpu_set_tangent_: the tangent mode needs FP64 precision\n");
+    std::abort();
+  }
+  r.tangent = *on != 0;
+}
+
+int felgpu_tangent_() { return tangent_mode() ? 1 : 0; }
+
+// Sets the derivative part of a whole registered array from the host array
+// `seed` (same number of elements); the values are kept.
+void felgpu_tangent_seed_(const void* host, const double* seed) {
+  deferred_coef_check(host, true);
+  Range& g = must_find(host, "felgpu_tangent_seed_");
+  if (g.esize != 16) {
+    std::fprintf(stderr, "felgpu: felgpu_tangent_seed_: not in the tangent mode\n");
+    std::abort();
+  }
+  const size_t n = g.bytes / sizeof(double);
+  std::vector<Dual> tmp(n);
+  FEL_CHECK(hipMemcpyAsync(tmp.data(), g.dev, n * 16, hipMemcpyDeviceToHost, stream()));
+  FEL_CHECK(hipStreamSynchronize(stream()));
+  for (size_t m = 0; m < n; m++) tmp[m].d = seed[m];
+  FEL_CHECK(hipMemcpyAsync(g.dev, tmp.data(), n * 16, hipMemcpyHostToDevice, stream()));
+  FEL_CHECK(hipStreamSynchronize(stream()));
+  g.ver = next_version();
+}
+
+// Copies the derivative part of a whole registered array into `out`.
+void felgpu_tangent_fetch_(const void* host, double* out) {
+  deferred_coef_check(host, false);
+  const Range& g = must_find(host, "felgpu_tangent_fetch_");
+  if (g.esize != 16) {
+    std::fprintf(stderr, "felgpu: felgpu_tangent_fetch_: not in the tangent mode\n");
+    std::abort();
+  }
+  const size_t n = g.bytes / sizeof(double);
+  std::vector<Dual> tmp(n);
+  FEL_CHECK(hipMemcpyAsync(tmp.data(), g.dev, n * 16, hipMemcpyDeviceToHost, stream()));
+  FEL_CHECK(hipStreamSynchronize(stream()));
+  for (size_t m = 0; m < n; m++) out[m] = tmp[m].d;
 }
 
 // Host -> device copy of a whole registered range (after host-side writes).

@@ -10,6 +10,13 @@ fields, stencils, Gauss-Seidel and FFT+TDMA in FP32; volume-fraction
 fluxes, mass sums, contact-line velocity and the pressure correction in
 FP64 with iterative refinement).  The mode is process-wide while the run
 is open and returns to FP64 when it closes.
+
+tangent=True is the library's tangent mode (FP64 values and their
+derivative along one direction of the inputs): after construction, seed()
+sets the direction on input arrays (the wall maps theta_0_array, l1_array,
+l2_array, zeta_array, or any field), steps() advances values and
+derivatives together, and tangent() returns a field's derivative.  Values
+are bitwise those of the FP64 run.
 """
 from __future__ import annotations
 
@@ -22,12 +29,19 @@ from .config import RunConfig, initial_state
 
 
 class GpuRun:
+    # registered arrays that hold several slices: name -> (base name, slice)
+    _SLICED = {"phi0": ("phi0", 0), "phi1": ("phi0", 1), "phin0": ("phin0", 0),
+               "phin1": ("phin0", 1)}
+
     def __init__(self, cfg: RunConfig, state: dict[str, np.ndarray] | None = None,
-                 precision: int = 64):
+                 precision: int = 64, tangent: bool = False):
         if precision not in (32, 64):
             raise ValueError(f"precision must be 32 or 64, not {precision}")
+        if tangent and precision != 64:
+            raise ValueError("the tangent mode is FP64")
         self.cfg = cfg
         self.precision = precision
+        self.tangent_mode = bool(tangent)
         self.drv = _lib.driver()
         self.gpu = _lib.gpu()
         self.drv.fel_drv_field.restype = ctypes.c_void_p
@@ -39,6 +53,7 @@ class GpuRun:
         for name, arr in st.items():
             self.view(name)[...] = arr
         self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(precision)))
+        self.gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(1 if tangent else 0)))
         self.drv.fel_drv_start()
 
     def view(self, name: str) -> np.ndarray:
@@ -54,6 +69,49 @@ class GpuRun:
         """Device copy of a field, as a new host array."""
         self.drv.fel_drv_download(name.encode())
         return self.view(name).copy()
+
+    def _range(self, name: str):
+        """(host pointer, element count, slice) of the registered array
+        that holds `name`."""
+        base, sl = self._SLICED.get(name, (name, None))
+        v = self.view(base)
+        n = v.size * (2 if sl is not None else 1)
+        return v.ctypes.data, n, sl
+
+    def upload(self, name: str):
+        """Host -> device copy after the caller changed view(name)."""
+        self.gpu.felgpu_upload_(ctypes.c_void_p(self._range(name)[0]))
+
+    def seed(self, name: str, direction: np.ndarray):
+        """Tangent mode: sets the derivative part of field `name` to
+        `direction` (same shape as the field); the values are kept."""
+        if not self.tangent_mode:
+            raise RuntimeError("seed() needs tangent=True")
+        ptr, n, sl = self._range(name)
+        full = np.zeros(n)
+        if sl is None:
+            full[:] = np.asarray(direction, dtype=np.float64).ravel()
+        else:
+            full = self._tangent_full(ptr, n)
+            m = n // 2
+            full[sl * m:(sl + 1) * m] = np.asarray(direction, dtype=np.float64).ravel()
+        self.gpu.felgpu_tangent_seed_(ctypes.c_void_p(ptr), full.ctypes.data_as(ctypes.c_void_p))
+
+    def _tangent_full(self, ptr, n):
+        out = np.empty(n)
+        self.gpu.felgpu_tangent_fetch_(ctypes.c_void_p(ptr), out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def tangent(self, name: str) -> np.ndarray:
+        """Tangent mode: the derivative part of field `name`."""
+        if not self.tangent_mode:
+            raise RuntimeError("tangent() needs tangent=True")
+        ptr, n, sl = self._range(name)
+        full = self._tangent_full(ptr, n)
+        if sl is not None:
+            m = n // 2
+            full = full[sl * m:(sl + 1) * m]
+        return full.reshape(self.cfg.shape)
 
     def steps(self, n: int):
         self.drv.fel_drv_steps(int(n))
@@ -71,6 +129,8 @@ class GpuRun:
         if self.drv is not None:
             self.drv.fel_drv_destroy()
             self.drv = None
+            if self.tangent_mode:
+                self.gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(0)))
             if self.precision != 64:
                 self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(64)))
 

@@ -67,6 +67,26 @@ void felgpu_register_fp64_(const void *host, const int64_t *nbytes, const int *u
 void felgpu_set_precision_(const int *bits);
 int felgpu_precision_(void);
 
+/* Tangent mode (no counterpart in the reference, which has no derivatives):
+ * *on = 1 makes every registered device copy a pair (FP64 value, derivative
+ * along one direction of the inputs) and runs every entry point on it, so a
+ * run of the unchanged timestep computes the values (bitwise equal to the
+ * FP64 mode) and their directional derivative (forward-mode
+ * differentiation of the discrete step).  Only while nothing is registered,
+ * in FP64.  felgpu_tangent_ returns 1 when on.
+ *   felgpu_tangent_seed_  sets the derivative part of the whole registered
+ *                         array containing `host` from seed[] (one double
+ *                         per element; e.g. 1 on the wall cells of a
+ *                         contact-angle stripe in theta_0, or 1 everywhere
+ *                         in l1 and l2 for the slip length);
+ *   felgpu_tangent_fetch_ copies the derivative part into out[].
+ * Uploads and registrations set the derivative to zero; downloads return
+ * the values. */
+void felgpu_set_tangent_(const int *on);
+int felgpu_tangent_(void);
+void felgpu_tangent_seed_(const void *host, const double *seed);
+void felgpu_tangent_fetch_(const void *host, double *out);
+
 /* Host -> device copy of the whole registered range containing `host`. */
 void felgpu_upload_(const void *host);
 
