@@ -10,10 +10,14 @@ device-resident before the timed region.  Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
                   [--no-cpu-baseline] [--precision 64|32]
 
-By default both paths run from the same initial state: `value` is the FP64
-path (results equal to the reference's FP64 arithmetic); `fp32_refine`
+By default three runs start from the same initial state: `value` is the
+FP64 path (results equal to the reference's FP64 arithmetic); `fp32_refine`
 holds the FP32 mode's throughput, its speed-up and the relative L2 errors
-of u, v, w, p against the FP64 run after the same number of steps.
+of u, v, w, p against the FP64 run after the same number of steps;
+`tangent` holds the time of the tangent mode (the same step computing the
+FP64 values and their derivative along the bottom wall's contact angle:
+forward-mode differentiation, one parameter direction per run) relative to
+the FP64 step, and whether its values equal the FP64 run's bit for bit.
 
 The roofline object describes the dominant kernel (the colour-pair
 Gauss-Seidel update of the viscous solve): algorithmic bytes per launch
@@ -93,13 +97,18 @@ def cpu_baseline(cfg_full, seconds=15.0):
                       f"= {cfg.cells} cells, {n} steps after 2 warm-up steps, {dt:.1f} s, 1 thread"}
 
 
-def measure(cfg, precision, steps, warmup, keep_fields=False):
-    """Runs warmup + steps timesteps at the given precision; returns the
+def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False):
+    """Runs warmup + steps timesteps at the given precision (or in the
+    tangent mode, seeded along the bottom wall's contact angle); returns the
     timing, the event timers and (optionally) the final fields."""
     import felmpf
     from felmpf import _lib
-    run = felmpf.GpuRun(cfg, precision=precision)
+    run = felmpf.GpuRun(cfg, precision=precision, tangent=tangent)
     gpu = _lib.gpu()
+    if tangent:
+        seed = np.zeros(cfg.shape)
+        seed[:, 3, :] = 1.0  # d/d(theta_0) of the whole bottom wall row
+        run.seed("theta_0_array", seed)
     try:
         import torch  # device-wide synchronisation around the timed region
         sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -144,7 +153,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--precision", type=int, default=0, choices=(0, 32, 64),
-                    help="64 or 32: time that path only; default: both (value = FP64)")
+                    help="64 or 32: time that path only; default: FP64, FP32 and tangent "
+                         "(value = FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -175,6 +185,16 @@ def main():
                 "sor_iterations_last_step": r32["state"]["sor_lp"] + 1,
                 "dtype": "f32 flow fields and stencils, f64 volume fraction, mass sums, "
                          "contact-line velocity and refined pressure correction"}
+    tangent = None
+    if both:
+        rt = measure(cfg, 64, args.steps, args.warmup, keep_fields=True, tangent=True)
+        same = all(np.array_equal(main_run["fields"][f], rt["fields"][f]) for f in ("u", "v", "w", "p"))
+        tangent = {"ms_per_step": round(rt["elapsed"] / args.steps * 1e3, 3),
+                   "time_vs_fp64_forward": round(rt["elapsed"] / main_run["elapsed"], 3),
+                   "directions": 1,
+                   "values_equal_fp64": bool(same),
+                   "kind": "forward-mode derivative of the step (FP64 value + derivative per "
+                           "element); a reverse-mode adjoint is not built"}
 
     timers = main_run["timers"]
     ms_sor, n_sor = timers["sor_pair"]
@@ -211,6 +231,7 @@ def main():
                      "bytes_per_launch": b, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": n_sor, "traffic_source": traffic_src},
         "fp32_refine": fp32,
+        "tangent": tangent,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
