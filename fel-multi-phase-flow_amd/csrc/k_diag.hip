@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "felgpu.hpp"
+#include "felgpu_visc.hpp"
 
 namespace fel {
 
@@ -34,6 +35,64 @@ const double* vel_sums(const Dims d, const double* phi, const double* u, const d
       },
       s);
   const double* r = read_scalars(s, 4);
+  c.finish();
+  return r;
+}
+
+// Largest advective rate of srcs/caldt.f:33-84 over the interior:
+// max(|u| dxinv, |v| dyinv on j <= jen, |w| dzinv, 1e-20).  A maximum does
+// not depend on the order of evaluation, so the two-stage reduction gives
+// the reference's value exactly.  T: velocity element type (values only).
+template <class T>
+__global__ void __launch_bounds__(256) advmax_stage1(Dims d, int jen, double dxinv, double dyinv,
+                                                     double dzinv, const T* u, const T* v,
+                                                     const T* w, double* part) {
+  __shared__ double sh[256];
+  double m = 1.0e-20;
+  const int nxy = d.ni * d.nj, n = nxy * d.nk;
+  for (int q = (int)(blockIdx.x * 256 + threadIdx.x); q < n; q += (int)(gridDim.x * 256)) {
+    const int k = q / nxy, r = q - k * nxy, j = r / d.ni, i = r - j * d.ni;
+    const int c = d.ix(i + 1, j + 1, k + 1);
+    m = fmax(fabs(val(u[c]) * dxinv), m);
+    if (j + 1 <= jen) m = fmax(fabs(val(v[c]) * dyinv), m);
+    m = fmax(fabs(val(w[c]) * dzinv), m);
+  }
+  const int t = (int)threadIdx.x;
+  sh[t] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) advmax_stage2(int nb, const double* part, double* out) {
+  __shared__ double sh[256];
+  const int t = (int)threadIdx.x;
+  double m = 1.0e-20;
+  for (int b = t; b < nb; b += 256) m = fmax(m, part[b]);
+  sh[t] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  if (t == 0) out[0] = sh[0];
+}
+
+template <class T>
+double advmax_t(const Dims d, int jen, double dxinv, double dyinv, double dzinv, const double* u,
+                const double* v, const double* w) {
+  Call c("caldt_");
+  const T *du = c.in_t<T>(u, d.n3, "u"), *dv = c.in_t<T>(v, d.n3, "v"), *dw = c.in_t<T>(w, d.n3, "w");
+  double* part = static_cast<double*>(scratch(15, sizeof(double) * kRedBlocks));
+  double* out = dev_scalars(8);
+  advmax_stage1<T><<<kRedBlocks, 256, 0, stream()>>>(d, jen, dxinv, dyinv, dzinv, du, dv, dw, part);
+  FEL_CHECK(hipGetLastError());
+  advmax_stage2<<<1, 256, 0, stream()>>>(kRedBlocks, part, out);
+  FEL_CHECK(hipGetLastError());
+  const double r = read_scalars(out, 1)[0];
   c.finish();
   return r;
 }
@@ -116,6 +175,41 @@ void cal_vel_(const int* ipara, const int* ni, const int* nj, const int* nk, con
     vel[m] = r[m + 1] / r[0];
     re[m] = *rhol * vel[m] * diameter / *rmul;
   }
+}
+
+// srcs/caldt.f:1-116 (ipara must be 0): the CFL time step.  The largest
+// advective rate is a device reduction; the three limits and the update of
+// time are evaluated on the host with the reference's expressions.
+void caldt_(const int* ipara, const int* nID, const int* ID, const int* ndiv, const int* ni,
+            const int* nj, const int* nk, const int* nstep, const int* imon_t, const double* dxinv,
+            const double* dyinv, const double* dzinv, const double* cfl, const double* rhol,
+            const double* rhog, const double* rmul, const double* rmug,
+            const double* surface_tension, const double* u, const double* v, const double* w,
+            double* dt, double* time) {
+  (void)ndiv;
+  if (*ipara != 0) {
+    std::fprintf(stderr, "felgpu: caldt_: ipara=%d, only the single-domain case (0) is supported\n", *ipara);
+    std::abort();
+  }
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int jen = nID[kYPlus] < 0 ? *nj - 1 : *nj;
+  double advmax = 0.0;
+  with_flow([&](auto t) {
+    advmax = advmax_t<decltype(t)>(d, jen, *dxinv, *dyinv, *dzinv, u, v, w);
+  });
+  const double dcell = std::pow(*dxinv * *dyinv * *dzinv, -1.0 / 3.0);
+  const double rnu = std::fmax(*rmul / *rhol, *rmug / *rhog);
+  const double st_rhominv = 2.0 * *surface_tension / (*rhol + *rhog);
+  const double dt1 = *cfl / advmax;
+  const double dt2 = *cfl * 0.5 * (dcell * dcell) / rnu * 1.0e2;
+  const double dt3 = *cfl * std::sqrt((dcell * dcell * dcell) / st_rhominv);
+  double step = std::fmin(dt1, dt2);
+  step = std::fmin(step, dt3);
+  *dt = step;
+  *time = *time + step;
+  if (*imon_t > 0 && *nstep % *imon_t == 0 && *ID == 0)
+    std::printf("time=%17.10E dt=%17.10E dt_adv=%17.10E dt_vis=%17.10E dt_srf=%17.10E\n", *time, step,
+                dt1, dt2, dt3);
 }
 
 }  // extern "C"

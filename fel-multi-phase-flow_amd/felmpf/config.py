@@ -30,7 +30,7 @@ class FelCfg(ctypes.Structure):
         ("rmul", ctypes.c_double), ("rmug", ctypes.c_double), ("surface_tension", ctypes.c_double),
         ("uwall", ctypes.c_double), ("grvb", ctypes.c_double), ("grvp", ctypes.c_double),
         ("xl", ctypes.c_double), ("yl", ctypes.c_double), ("zl", ctypes.c_double),
-        ("particle_radius", ctypes.c_double),
+        ("particle_radius", ctypes.c_double), ("cfl", ctypes.c_double),
     ]
 
 
@@ -67,6 +67,7 @@ class RunConfig:
     init: str = "droplet"
     params: dict = field(default_factory=dict)
     radius: float = 5e-4  # cal_vel's particle radius (diagnostic only)
+    cfl: float = 0.0      # > 0: adaptive dt by caldt each step (srcs/caldt.f); 0: fixed dt
 
     @property
     def dx(self):
@@ -91,7 +92,7 @@ class RunConfig:
     def cstruct(self) -> FelCfg:
         return FelCfg(self.ni, self.nj, self.nk, self.imon_t, self.dx, self.dy, self.dz, self.dt,
                       self.bet, self.rhol, self.rhog, self.rmul, self.rmug, self.sigma, self.uwall,
-                      0.0, self.grvp, self.xl, self.yl, self.zl, self.radius)
+                      0.0, self.grvp, self.xl, self.yl, self.zl, self.radius, self.cfl)
 
     def scaled(self, ni, nj, nk, **kw) -> "RunConfig":
         """The same physics on another grid (same domain lengths)."""

@@ -113,6 +113,28 @@ class GpuRun:
             full = full[sl * m:(sl + 1) * m]
         return full.reshape(self.cfg.shape)
 
+    def dataou(self, path: str):
+        """Writes a checkpoint in the reference's restart format
+        (srcs/dataou.f: u, v, w, p, uo, vo, wo, po, phi, time, nstep)."""
+        b = path.encode()
+        self.drv.fel_drv_dataou(ctypes.c_char_p(b), ctypes.c_int(len(b)))
+
+    def datain(self, path: str):
+        """Restarts this (started) run from a checkpoint file, with the
+        reference's restart ghost fills (srcs/main.f90:375-404)."""
+        b = path.encode()
+        self.drv.fel_drv_datain(ctypes.c_char_p(b), ctypes.c_int(len(b)))
+
+    def output(self, what: str):
+        """One output cadence of srcs/main.f90:663-743 for the current step,
+        written to the current directory: "vtk" (phi_<n>.vtk, p_<n>.vtk and
+        the interface-position monitor lines), "uwphi" (the j-slice file
+        uwphi<n>) or "center_vel" (a line of center_01.dat and
+        velocity_01.dat)."""
+        fn = {"vtk": self.drv.fel_drv_vtk, "uwphi": self.drv.fel_drv_uwphi,
+              "center_vel": self.drv.fel_drv_center_vel}[what]
+        fn()
+
     def steps(self, n: int):
         self.drv.fel_drv_steps(int(n))
 
@@ -120,10 +142,11 @@ class GpuRun:
         self.gpu.felgpu_sync_()
 
     def state(self) -> dict:
-        out = (ctypes.c_double * 12)()
+        out = (ctypes.c_double * 13)()
         self.drv.fel_drv_state(out)
         return dict(nstep=int(out[0]), time=out[1], sor_lp=int(out[2]), sor_ratio=out[3],
-                    err_div0=out[4], center=tuple(out[5:8]), velocity=out[8], vel=tuple(out[9:12]))
+                    err_div0=out[4], center=tuple(out[5:8]), velocity=out[8], vel=tuple(out[9:12]),
+                    dt=out[12])
 
     def close(self):
         if self.drv is not None:

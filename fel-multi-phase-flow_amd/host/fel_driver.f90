@@ -29,13 +29,14 @@ module fel_driver
     real(c_double) :: dx, dy, dz, dt
     real(c_double) :: bet_mthinc, rhol, rhog, rmul, rmug, surface_tension, uwall, grvb, grvp
     real(c_double) :: xl, yl, zl, particle_radius
+    real(c_double) :: cfl   ! > 0: adaptive dt by caldt each step; 0: fixed dt
   end type fel_cfg
 
   type(fel_cfg), save :: c
   logical, save :: created = .false.
   integer, save :: ni, nj, nk, nstep, ipara, ID, ndiv
   integer, save :: nID(6), svall(3), key(8)
-  real(8), save :: dxinv, dyinv, dzinv, time, err_div0, err_div
+  real(8), save :: dxinv, dyinv, dzinv, time, err_div0, err_div, dt
   real(8), save :: center(3), center_pre1, center_pre2, velocity, vel(3), re(3)
   integer, save :: area_pre1, area_pre2, lap
 
@@ -317,13 +318,20 @@ contains
     integer :: l
     real(8) :: rho_av, rhon_av, avphin
     nstep = nstep + 1
-    time = time + c%dt
+    if (c%cfl > 0.0d0) then
+      ! srcs/main.f90:479 (commented out in the reference, which runs a fixed dt)
+      call caldt(ipara,nID,ID,ndiv,ni,nj,nk,nstep,c%imon_t,dxinv,dyinv,dzinv,c%cfl,c%rhol,c%rhog, &
+                 c%rmul,c%rmug,c%surface_tension,u,v,w,dt,time)
+    else
+      dt = c%dt
+      time = time + dt
+    end if
     l = 1
     ! VOF (main.f90:489-504)
     call cal_grad_p2a(ID,svall(2),ni,nj,nk,dxinv,dyinv_array,dzinv,phi(-2,-2,-2,l),phix,phiy,phiz)
-    call solphi_mthinc1(ni,nj,nk,dxinv,dyinv,dzinv,c%dt,c%bet_mthinc,phi(-2,-2,-2,l),phix,phiy,phiz, &
+    call solphi_mthinc1(ni,nj,nk,dxinv,dyinv,dzinv,dt,c%bet_mthinc,phi(-2,-2,-2,l),phix,phiy,phiz, &
                         u,v,w,flphix,flphiy,flphiz)
-    call solphi_mthinc2(ni,nj,nk,dxinv,dyinv,dzinv,c%dt,u,v,w,flphix,flphiy,flphiz, &
+    call solphi_mthinc2(ni,nj,nk,dxinv,dyinv,dzinv,dt,u,v,w,flphix,flphiy,flphiz, &
                         phi(-2,-2,-2,l),phin(-2,-2,-2,l))
     call cal_grad_p2a(ID,svall(2),ni,nj,nk,dxinv,dyinv_array,dzinv,phin(-2,-2,-2,l),phix,phiy,phiz)
     call solphi_mthinc3(ipara,ni,nj,nk,dxinv,dyinv,dzinv,c%bet_mthinc,phix,phiy,phiz, &
@@ -383,7 +391,7 @@ contains
     call cal_av(ipara,ndiv,ni,nj,nk,rhon,rhon_av)
     rho_av = (rho_av+rhon_av)*0.5d0
     ! source (main.f90:579-588)
-    call cal_srcu(nID,ni,nj,nk,rho_av,c%dt,c%grvb,c%grvp,u,v,w,rho,rhon, &
+    call cal_srcu(nID,ni,nj,nk,rho_av,dt,c%grvb,c%grvp,u,v,w,rho,rhon, &
                   adv_u,adv_v,adv_w,adv_uo,adv_vo,adv_wo,prs_u,prs_v,prs_w,vis_u,vis_v,vis_w, &
                   sum_fst_u,sum_fst_v,sum_fst_w,sum_fst_un,sum_fst_vn,sum_fst_wn,src_u,src_v,src_w)
     ! semi-implicit viscous step (main.f90:590-631)
@@ -400,7 +408,7 @@ contains
     call cpy(ni,nj,nk,u,un)
     call cpy(ni,nj,nk,v,vn)
     call cpy(ni,nj,nk,w,wn)
-    call solu_sor4(ipara,ID,nID,ndiv,ni,nj,nk,key,sendjb,recvjb,nstep,c%imon_t,c%dt,rho,rhon, &
+    call solu_sor4(ipara,ID,nID,ndiv,ni,nj,nk,key,sendjb,recvjb,nstep,c%imon_t,dt,rho,rhon, &
       cf(-2,-2,-2, 1),cf(-2,-2,-2, 2),cf(-2,-2,-2, 3),cf(-2,-2,-2, 4),cf(-2,-2,-2, 5), &
       cf(-2,-2,-2, 6),cf(-2,-2,-2, 7),cf(-2,-2,-2, 8),cf(-2,-2,-2, 9),cf(-2,-2,-2,10), &
       cf(-2,-2,-2,11),cf(-2,-2,-2,12),cf(-2,-2,-2,13),cf(-2,-2,-2,14),cf(-2,-2,-2,15), &
@@ -419,7 +427,7 @@ contains
     call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,vn)
     call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,wn)
     ! projection (main.f90:633-661)
-    call caldiv(ipara,ndiv,ni,nj,nk,dxinv,dyinv,dzinv,c%dt,un,vn,wn,div,err_div0)
+    call caldiv(ipara,ndiv,ni,nj,nk,dxinv,dyinv,dzinv,dt,un,vn,wn,div,err_div0)
     call solp_fft_tdma1(ni,nj,nk,fftdata,div,phir_r,phir_i)
     call trans_r2w(ipara,ID,ndiv,svall,key,sendbuf,recvbuf,phir_r,phir_r)
     call trans_r2w(ipara,ID,ndiv,svall,key,sendbuf,recvbuf,phir_i,phir_i)
@@ -433,7 +441,7 @@ contains
     call bnd_periodic(ni,nj,nk,dp)
     call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,dp)
     call solp_fft_tdma4(ipara,ID,ndiv,ni,nj,nk,nstep,c%imon_t,c%rhog,dxinv,dyinv,dzinv,div,dp)
-    call corunp_explicit(nID,ni,nj,nk,c%rhog,dxinv,dyinv,dzinv,c%dt,dp,phat,un,vn,wn,pn)
+    call corunp_explicit(nID,ni,nj,nk,c%rhog,dxinv,dyinv,dzinv,dt,dp,phat,un,vn,wn,pn)
     call bndu(nID,ni,nj,nk,un,vn,wn,c%uwall,c%dy,l1_array,l2_array,phi(-2,-2,-2,1))
     call bnd_periodic(ni,nj,nk,un)
     call bnd_periodic(ni,nj,nk,vn)
@@ -443,12 +451,12 @@ contains
     call bnd_periodic(ni,nj,nk,pn)
     call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,pn)
     ! diagnostics (main.f90:663-684)
-    call cal_center(ipara,ID,svall(2),ni,nj,nk,c%xl,c%yl,c%zl,c%dt,c%rhog,phi(-2,-2,-2,1), &
+    call cal_center(ipara,ID,svall(2),ni,nj,nk,c%xl,c%yl,c%zl,dt,c%rhog,phi(-2,-2,-2,1), &
                     center,center_pre1,center_pre2,area_pre1,area_pre2,lap,velocity)
     call cal_vel(ipara,ni,nj,nk,c%particle_radius,c%rhol,c%rmul,u,v,w,phi(-2,-2,-2,1),vel,re)
     if (c%imon_t > 0) then
       if (mod(nstep,c%imon_t) == 0) then
-        call caldiv(ipara,ndiv,ni,nj,nk,dxinv,dyinv,dzinv,c%dt,un,vn,wn,div,err_div)
+        call caldiv(ipara,ndiv,ni,nj,nk,dxinv,dyinv,dzinv,dt,un,vn,wn,div,err_div)
         write(*,'("Err_div  ",1i10,20e20.10)') nstep, err_div/err_div0, err_div, err_div0
       end if
     end if
@@ -494,15 +502,258 @@ contains
     call felgpu_download_c(ptr)
   end subroutine felgpu_download_ptr
 
+  ! ---------------------------------------------------------------------
+  ! Checkpoint/restart in the reference's file format.  srcs/dataou.f:24-38
+  ! and srcs/datain.f:24-38 write/read ONE unformatted sequential record:
+  ! the padded images of u, v, w, p, uo, vo, wo, po (column-major, ghosts
+  ! included), phi(...,0:nbub), time (real*8) and nstep (default integer).
+  ! A whole-array I/O list transfers the same elements in the same order as
+  ! the reference's implied-do loops, so the files are interchangeable with
+  ! the reference's (checked against its own dataou/datain).
+  ! ---------------------------------------------------------------------
+
+  ! Writes one checkpoint record from host arrays (no device involved).
+  subroutine fel_ckpt_write(path, plen, ci, cj, ck, cb, step, t, a1, a2, a3, a4, a5, a6, a7, a8, &
+                            ph) bind(C, name="fel_ckpt_write")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), value :: plen, ci, cj, ck, cb
+    integer(c_int), intent(in) :: step
+    real(c_double), intent(in) :: t
+    real(c_double), intent(in), dimension(-2:ci+3,-2:cj+3,-2:ck+3) :: a1, a2, a3, a4, a5, a6, a7, a8
+    real(c_double), intent(in) :: ph(-2:ci+3,-2:cj+3,-2:ck+3,0:cb)
+    integer :: lun, ios
+    open(newunit=lun, file=c_path(path, plen), form='unformatted', access='sequential', &
+         status='replace', action='write', iostat=ios)
+    if (ios /= 0) error stop 'fel_ckpt_write: cannot open the checkpoint file'
+    write(lun) a1, a2, a3, a4, a5, a6, a7, a8, ph, t, step
+    close(lun)
+  end subroutine fel_ckpt_write
+
+  ! Reads one checkpoint record into host arrays (no device involved).
+  subroutine fel_ckpt_read(path, plen, ci, cj, ck, cb, step, t, a1, a2, a3, a4, a5, a6, a7, a8, &
+                           ph) bind(C, name="fel_ckpt_read")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), value :: plen, ci, cj, ck, cb
+    integer(c_int), intent(out) :: step
+    real(c_double), intent(out) :: t
+    real(c_double), intent(out), dimension(-2:ci+3,-2:cj+3,-2:ck+3) :: a1, a2, a3, a4, a5, a6, a7, a8
+    real(c_double), intent(out) :: ph(-2:ci+3,-2:cj+3,-2:ck+3,0:cb)
+    integer :: lun, ios
+    open(newunit=lun, file=c_path(path, plen), form='unformatted', access='sequential', &
+         status='old', action='read', iostat=ios)
+    if (ios /= 0) error stop 'fel_ckpt_read: cannot open the checkpoint file'
+    read(lun, iostat=ios) a1, a2, a3, a4, a5, a6, a7, a8, ph, t, step
+    if (ios /= 0) error stop 'fel_ckpt_read: the record does not match this grid'
+    close(lun)
+  end subroutine fel_ckpt_read
+
+  function c_path(path, plen) result(f)
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), intent(in) :: plen
+    character(len=plen) :: f
+    integer :: m
+    do m = 1, plen
+      f(m:m) = path(m)
+    end do
+  end function c_path
+
+  ! srcs/main.f90:702-704: the run's state to a checkpoint file (device ->
+  ! host copies of the nine arrays first).
+  subroutine fel_drv_dataou(path, plen) bind(C, name="fel_drv_dataou")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), value :: plen
+    call felgpu_download_ptr(c_loc(u)); call felgpu_download_ptr(c_loc(v))
+    call felgpu_download_ptr(c_loc(w)); call felgpu_download_ptr(c_loc(p))
+    call felgpu_download_ptr(c_loc(uo)); call felgpu_download_ptr(c_loc(vo))
+    call felgpu_download_ptr(c_loc(wo)); call felgpu_download_ptr(c_loc(po))
+    call felgpu_download_ptr(c_loc(phi))
+    call fel_ckpt_write(path, plen, ni, nj, nk, nbub, nstep, time, u, v, w, p, uo, vo, wo, po, phi)
+  end subroutine fel_drv_dataou
+
+  ! srcs/main.f90:375-404: restart of a started run from a checkpoint file:
+  ! the nine arrays, time and nstep are replaced (and uploaded), then the
+  ! reference's restart fills refresh the ghosts -- bndu of (u,v,w) and
+  ! (uo,vo,wo), bnd_periodic of the eight fields, gnbc and the contact-angle
+  ! ghosts of phi(...,0).
+  subroutine fel_drv_datain(path, plen) bind(C, name="fel_drv_datain")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), value :: plen
+    call fel_ckpt_read(path, plen, ni, nj, nk, nbub, nstep, time, u, v, w, p, uo, vo, wo, po, phi)
+    call felgpu_upload(u); call felgpu_upload(v); call felgpu_upload(w)
+    call felgpu_upload(p); call felgpu_upload(uo); call felgpu_upload(vo)
+    call felgpu_upload(wo); call felgpu_upload(po); call felgpu_upload(phi)
+    call bndu(nID,ni,nj,nk,u ,v ,w ,c%uwall,c%dy,l1_array,l2_array,phi(-2,-2,-2,1))
+    call bndu(nID,ni,nj,nk,uo,vo,wo,c%uwall,c%dy,l1_array,l2_array,phi(-2,-2,-2,1))
+    call bnd_periodic(ni,nj,nk,u ); call bnd_periodic(ni,nj,nk,v ); call bnd_periodic(ni,nj,nk,w )
+    call bnd_periodic(ni,nj,nk,p ); call bnd_periodic(ni,nj,nk,uo); call bnd_periodic(ni,nj,nk,vo)
+    call bnd_periodic(ni,nj,nk,wo); call bnd_periodic(ni,nj,nk,po)
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,u )
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,v )
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,w )
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,p )
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,uo)
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,vo)
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,wo)
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,po)
+    call gnbc(nID, ni, nj, nk, u, w, c%uwall, theta_0_array, c%surface_tension, zeta_array, theta_array)
+    call bnd_contact_angle(nID,ni,nj,nk,phi(-2,-2,-2,0),theta_array,c%dx,c%dy,c%dz)
+    call bnd_periodic(ni,nj,nk,phi(-2,-2,-2,0))
+    call bnd_comm(ipara,nID,ni,nj,nk,key,sendjb,recvjb,phi(-2,-2,-2,0))
+  end subroutine fel_drv_datain
+
+  ! ---------------------------------------------------------------------
+  ! Output cadences of srcs/main.f90:706-743 and 663-672.  The device
+  ! reduces the fields (felgpu_vtk_points_, mk_slice_j_,
+  ! felgpu_interface_crossings_); these routines format the reference's
+  ! files and monitor lines with the same Fortran edit descriptors.
+  ! ---------------------------------------------------------------------
+
+  ! The VTK file of srcs/mkvtk_phi.f:55-83 (and mkvtk_p.f) from the vertex
+  ! values vals((mi+1)(mj+1)(mk+1), i fastest): header, then one value per
+  ! line in E10.3.
+  subroutine fel_vtk_write(path, plen, mi, mj, mk, step, ddx, ddy, ddz, sname, slen, vals) &
+      bind(C, name="fel_vtk_write")
+    character(kind=c_char), intent(in) :: path(*), sname(*)
+    integer(c_int), value :: plen, mi, mj, mk, step, slen
+    real(c_double), value :: ddx, ddy, ddz
+    real(c_double), intent(in) :: vals(*)
+    integer :: lun, m, n
+    n = (mi+1)*(mj+1)*(mk+1)
+    open(newunit=lun, file=c_path(path, plen), status='replace')
+    write(lun, "('# vtk DataFile Version 2.0')")
+    write(lun, "('t=',I0)") step
+    write(lun, "('ASCII')")
+    write(lun, "('DATASET STRUCTURED_POINTS')")
+    write(lun, "('DIMENSIONS ',I0,' ',I0,' ',I0)") mi+1, mj+1, mk+1
+    write(lun, "('ORIGIN ',e11.4,' ',e11.4,' ',e11.4)") 0.0, 0.0, 0.0
+    write(lun, "('ASPECT_RATIO ',e11.4,' ',e11.4,' ',e11.4)") ddx, ddy, ddz
+    write(lun, "('')")
+    write(lun, "('POINT_DATA ',I0)") n
+    write(lun, "('SCALARS ',A,' float')") c_path(sname, slen)
+    write(lun, "('LOOKUP_TABLE default')")
+    do m = 1, n
+      write(lun, "(e10.3)") vals(m)
+    end do
+    close(lun)
+  end subroutine fel_vtk_write
+
+  ! The monitor lines of srcs/find_interface_positions_upper.f90:21 (every
+  ! crossing x with its z, kind 0) and _lower.f90:24-33 (the mean over k of
+  ! the first crossing, 0 where a row has none, printed negated; kind 1),
+  ! from the crossings x(ni, nk) / cnt(nk) of felgpu_interface_crossings_.
+  subroutine fel_interface_print(kind, ci, ck, ddz, x, cnt) bind(C, name="fel_interface_print")
+    integer(c_int), value :: kind, ci, ck
+    real(c_double), value :: ddz
+    real(c_double), intent(in) :: x(ci, ck)
+    integer(c_int), intent(in) :: cnt(ck)
+    integer :: k, m
+    real(8) :: z, sum_x, x_k
+    if (kind == 0) then
+      do k = 1, ck
+        z = (k - 0.5d0) * ddz
+        do m = 1, cnt(k)
+          print *, x(m, k), z
+        end do
+      end do
+    else
+      sum_x = 0.0d0
+      do k = 1, ck
+        x_k = 0.0d0
+        if (cnt(k) > 0) x_k = x(1, k)
+        sum_x = sum_x + x_k
+      end do
+      print *, 'Lower Interface Position (Average over k):'
+      print *, -(sum_x / ck)
+    end if
+  end subroutine fel_interface_print
+
+  ! Flushes the Fortran runtime's standard output (monitor lines).
+  subroutine fel_drv_flush() bind(C, name="fel_drv_flush")
+    flush(6)
+  end subroutine fel_drv_flush
+
+  ! srcs/main.f90:716-743 without the vorticity fields: phi_<nstep>.vtk and
+  ! p_<nstep>.vtk in the current directory, then the interface positions on
+  ! the top and bottom wall rows.  phi(...,0) is the field the reference
+  ! gathers (phi_all) for these.
+  subroutine fel_drv_vtk() bind(C, name="fel_drv_vtk")
+    real(8), allocatable :: vals(:), xc(:)
+    integer(c_int), allocatable :: cnt(:)
+    character(len=32) :: fname
+    integer(c_int) :: kind, jrow, first
+    allocate(vals((ni+1)*(nj+1)*(nk+1)), xc(ni*nk), cnt(nk))
+    kind = 0
+    call felgpu_vtk_points(ni, nj, nk, phi, kind, vals)
+    write(fname, '("phi_",i7.7,".vtk")') nstep
+    call fel_vtk_write(fname, len_trim(fname), ni, nj, nk, nstep, c%dx, c%dy, c%dz, 'phi', 3, vals)
+    kind = 1
+    call felgpu_vtk_points(ni, nj, nk, p, kind, vals)
+    write(fname, '("p_",i7.7,".vtk")') nstep
+    call fel_vtk_write(fname, len_trim(fname), ni, nj, nk, nstep, c%dx, c%dy, c%dz, 'p', 1, vals)
+    jrow = nj; first = 0
+    call felgpu_interface_crossings(ni, nj, nk, phi, jrow, c%dx, c%xl, first, xc, cnt)
+    call fel_interface_print(0, ni, nk, c%dz, xc, cnt)
+    jrow = 1; first = 1
+    call felgpu_interface_crossings(ni, nj, nk, phi, jrow, c%dx, c%xl, first, xc, cnt)
+    call fel_interface_print(1, ni, nk, c%dz, xc, cnt)
+    deallocate(vals, xc, cnt)
+  end subroutine fel_drv_vtk
+
+  ! srcs/main.f90:706-713 with srcs/mkuwp.h: the slices of u, w and phi at
+  ! j = particle_radius/dy and the file uwphi<nstep> (x, z, u, w, phi per
+  ! line in 20E20.10, values below 1e-40 in magnitude written as 0, a blank
+  ! line after each k).
+  subroutine fel_drv_uwphi() bind(C, name="fel_drv_uwphi")
+    real(8), allocatable :: us(:,:), ws(:,:), ps(:,:), wrk(:,:)
+    character(len=32) :: fname
+    integer :: j, i, k, lun
+    real(8) :: u00, w00, q00
+    allocate(us(0:ni,0:nk), ws(0:ni,0:nk), ps(0:ni,0:nk), wrk(0:ni,0:nk))
+    us = 0.0d0; ws = 0.0d0; ps = 0.0d0
+    j = int(c%particle_radius / c%dy)
+    call mk_slice_j(ipara,ID,nID,ndiv,ni,nj,nk,j,0,1,1,wrk,u,us)
+    call mk_slice_j(ipara,ID,nID,ndiv,ni,nj,nk,j,1,1,0,wrk,w,ws)
+    call mk_slice_j(ipara,ID,nID,ndiv,ni,nj,nk,j,1,1,1,wrk,phi,ps)
+    write(fname, '("uwphi",i7.7)') nstep
+    open(newunit=lun, file=trim(fname))
+    do k = 0, nk
+      do i = 0, ni
+        u00 = us(i,k); w00 = ws(i,k); q00 = ps(i,k)
+        if (abs(u00) < 1.0d-40) u00 = 0.0d0
+        if (abs(w00) < 1.0d-40) w00 = 0.0d0
+        if (abs(q00) < 1.0d-40) q00 = 0.0d0
+        write(lun, '(20e20.10)') dble(i)*c%dx, dble(k)*c%dz, u00, w00, q00
+      end do
+      write(lun, '()')
+    end do
+    close(lun)
+    deallocate(us, ws, ps, wrk)
+  end subroutine fel_drv_uwphi
+
+  ! srcs/mk_center.f and srcs/mk_vel.f for the last step's cal_center and
+  ! cal_vel results: one line appended to center_01.dat and velocity_01.dat.
+  subroutine fel_drv_center_vel() bind(C, name="fel_drv_center_vel")
+    integer :: lun
+    open(newunit=lun, file='center_01.dat', status='unknown', position='append')
+    write(lun, '(20e20.10)') time, center(1), center(2), center(3), velocity, &
+      sqrt((center(1)-0.5d0)**2 + (center(3)-0.5d0)**2)
+    close(lun)
+    open(newunit=lun, file='velocity_01.dat', status='unknown', position='append')
+    write(lun, '(20e20.10)') time, vel(1), vel(2), vel(3), re(1), re(2), velocity
+    close(lun)
+  end subroutine fel_drv_center_vel
+
   ! Step counter, time, last SOR iteration count and residual ratio, the
-  ! pre-projection divergence RMS, and the diagnostics of cal_center/cal_vel.
+  ! pre-projection divergence RMS, the diagnostics of cal_center/cal_vel and
+  ! the last step's dt.
   subroutine fel_drv_state(out) bind(C, name="fel_drv_state")
-    real(c_double), intent(out) :: out(12)
+    real(c_double), intent(out) :: out(13)
     integer(c_int) :: lp
     real(c_double) :: ratio
     call felgpu_sor_last(lp, ratio)
     out(1) = nstep; out(2) = time; out(3) = lp; out(4) = ratio; out(5) = err_div0
     out(6:8) = center; out(9) = velocity; out(10:12) = vel
+    out(13) = dt
   end subroutine fel_drv_state
 
   subroutine fel_drv_destroy() bind(C, name="fel_drv_destroy")

@@ -418,6 +418,46 @@ void cal_vel_(const int *ipara, const int *ni, const int *nj, const int *nk, con
               const double *rhol, const double *rmul, const double *u, const double *v,
               const double *w, const double *phi, double *vel, double *re);
 
+/* srcs/caldt.f:1 -- CFL time step: dt = min(cfl/max(|u|dxinv, |v|dyinv
+ * (j <= jen), |w|dzinv), cfl*0.5*dcell^2/nu*100, cfl*sqrt(dcell^3 (rho_l +
+ * rho_g)/(2 sigma))) and time += dt; prints the reference's monitor line
+ * every imon_t steps on ID 0.  The reference's time loop has the call
+ * commented out (srcs/main.f90:479); the Fortran host calls it when
+ * fel_cfg.cfl > 0.  ipara must be 0. */
+void caldt_(const int *ipara, const int *nID, const int *ID, const int *ndiv, const int *ni,
+            const int *nj, const int *nk, const int *nstep, const int *imon_t, const double *dxinv,
+            const double *dyinv, const double *dzinv, const double *cfl, const double *rhol,
+            const double *rhog, const double *rmul, const double *rmug,
+            const double *surface_tension, const double *u, const double *v, const double *w,
+            double *dt, double *time);
+
+/* ------------------------------------------------------------------------
+ * Output cadences (srcs/main.f90:706-743): the device computes the reduced
+ * data, the host formats the files (host/fel_driver.f90).
+ * ---------------------------------------------------------------------- */
+
+/* srcs/mk_slice_j.f:1 -- 8-point average of qk at plane j with offsets
+ * (ip,jp,kp) into the (0:ni,0:nk) host slices.  ipara must be 0. */
+void mk_slice_j_(const int *ipara, const int *ID, const int *nID, const int *ndiv, const int *ni,
+                 const int *nj, const int *nk, const int *j, const int *ip, const int *jp,
+                 const int *kp, double *wrk_slice, const double *qk, double *qk_slice);
+
+/* Vertex values written by srcs/mkvtk_phi.f (*kind = 0: ghost fills x
+ * periodic, y and z Neumann) and srcs/mkvtk_p.f (*kind = 1: x and z
+ * periodic, y Neumann): out[(ni+1)(nj+1)(nk+1)], i fastest, the 8-point
+ * average at each vertex (0..ni, 0..nj, 0..nk).  No counterpart as a
+ * separate routine in the reference (its writers compute them inline). */
+void felgpu_vtk_points_(const int *ni, const int *nj, const int *nk, const double *q,
+                        const int *kind, double *out);
+
+/* Interface crossings phi(i) > 0.5 > phi(i+1) along row *jrow of each
+ * k-plane, at x = (i-1/2)dx + (1/2-phi(i))dx/(phi(i+1)-phi(i)) - 3/4 xl, as
+ * srcs/find_interface_positions_upper.f90 (every crossing) and _lower.f90
+ * (*first_only != 0: the first) compute them: x[(k-1)*ni + m], m < cnt[k-1]. */
+void felgpu_interface_crossings_(const int *ni, const int *nj, const int *nk, const double *phi,
+                                 const int *jrow, const double *dx, const double *xl,
+                                 const int *first_only, double *x, int *cnt);
+
 #ifdef __cplusplus
 }
 #endif

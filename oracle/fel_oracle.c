@@ -1602,3 +1602,53 @@ void orc_cal_vel(const int *ipara, const int *ni_, const int *nj_, const int *nk
   vel[2] = pw / sp;
   for (int c = 0; c < 3; c++) re[c] = *rhol * vel[c] * diameter / *rmul;
 }
+
+/* srcs/caldt.f:1-116: CFL time step from the largest advective rate
+ * |u|/dx, |v|/dy (v on j <= jen: nj-1 below a wall at Y_PLUS), |w|/dz over
+ * the interior (floor 1e-20), the viscous limit cfl*0.5*dcell^2/nu*100 and
+ * the capillary limit cfl*sqrt(dcell^3/(2 sigma/(rho_l+rho_g))), with
+ * dcell = (dxinv*dyinv*dzinv)^(-1/3) and nu the larger kinematic
+ * viscosity; dt = min of the three, time += dt.  The reference prints a
+ * monitor line every imon_t steps on ID 0.  Unbuildable reference (MPI
+ * header); pinned by closed forms (each limit selected in turn) and a
+ * numpy brute force in tests/test_oracle_math.py.  Returns dt1..dt3 in
+ * lim[0..2] when lim is not NULL. */
+void orc_caldt(const int *ipara, const int *nID, const int *ID, const int *ndiv, const int *ni_,
+               const int *nj_, const int *nk_, const int *nstep, const int *imon_t,
+               const double *dxinv, const double *dyinv, const double *dzinv, const double *cfl,
+               const double *rhol, const double *rhog, const double *rmul, const double *rmug,
+               const double *surface_tension, const double *u, const double *v, const double *w,
+               double *dt, double *time, double *lim) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  (void)ipara; (void)ndiv;
+  const double dcell = pow(*dxinv * *dyinv * *dzinv, -1.0 / 3.0);
+  const double rnu = fmax(*rmul / *rhol, *rmug / *rhog);
+  const double st_rhominv = 2.0 * *surface_tension / (*rhol + *rhog);
+  const int jen = nID[4] < 0 ? nj - 1 : nj;
+  double advmax = 1.0e-20;
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) advmax = fmax(fabs(u[OIX(i, j, k)] * *dxinv), advmax);
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= jen; j++)
+      for (int i = 1; i <= ni; i++) advmax = fmax(fabs(v[OIX(i, j, k)] * *dyinv), advmax);
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) advmax = fmax(fabs(w[OIX(i, j, k)] * *dzinv), advmax);
+  const double dt1 = *cfl / advmax;
+  const double dt2 = *cfl * 0.5 * (dcell * dcell) / rnu * 1.0e2;
+  const double dt3 = *cfl * sqrt((dcell * dcell * dcell) / st_rhominv);
+  double d = fmin(dt1, dt2);
+  d = fmin(d, dt3);
+  *dt = d;
+  *time = *time + d;
+  if (lim) {
+    lim[0] = dt1;
+    lim[1] = dt2;
+    lim[2] = dt3;
+  }
+  if (*imon_t > 0 && *nstep % *imon_t == 0 && *ID == 0)
+    printf("time=%17.10E dt=%17.10E dt_adv=%17.10E dt_vis=%17.10E dt_srf=%17.10E\n", *time, d,
+           dt1, dt2, dt3);
+}

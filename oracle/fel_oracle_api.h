@@ -127,4 +127,9 @@ void orc_cal_center(const int *ipara, const int *ID, const int *svall, const int
 void orc_cal_vel(const int *ipara, const int *ni_, const int *nj_, const int *nk_, const double
     *radius, const double *rhol, const double *rmul, const double *u, const double *v, const double
     *w, const double *phi, double *vel, double *re);
+void orc_caldt(const int *ipara, const int *nID, const int *ID, const int *ndiv, const int *ni_,
+    const int *nj_, const int *nk_, const int *nstep, const int *imon_t, const double *dxinv,
+    const double *dyinv, const double *dzinv, const double *cfl, const double *rhol, const double
+    *rhog, const double *rmul, const double *rmug, const double *surface_tension, const double *u,
+    const double *v, const double *w, double *dt, double *time, double *lim);
 #endif

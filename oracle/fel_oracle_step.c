@@ -34,6 +34,7 @@ typedef struct {
   double dx, dy, dz, dt;
   double bet_mthinc, rhol, rhog, rmul, rmug, surface_tension, uwall, grvb, grvp;
   double xl, yl, zl, particle_radius;
+  double cfl; /* > 0: adaptive dt by caldt each step (srcs/main.f90:479); 0: fixed dt */
 } orc_cfg;
 
 enum {
@@ -71,6 +72,7 @@ typedef struct {
   int area_pre1, area_pre2, lap;
   int last_sor_lp;
   double last_sor_ratio;
+  double dt_cur; /* this step's dt */
 } orc_state;
 
 static double *zalloc(size_t n) {
@@ -150,6 +152,8 @@ void orc_state_diag(const orc_state *S, double out[8]) {
   out[4] = S->vel[0]; out[5] = S->vel[1]; out[6] = S->vel[2]; out[7] = S->time;
 }
 
+double orc_state_dt(const orc_state *S) { return S->dt_cur; }
+
 #define FLD(x) (S->f3[x])
 
 /* Start-up boundary fills, srcs/main.f90:322-366, then the pressure
@@ -195,11 +199,19 @@ static void orc_state_step1(orc_state *S) {
   const int *nID = S->nID;
   double *phi1 = S->phi + S->n3, *phi0 = S->phi;
   double *phin1 = S->phin + S->n3, *phin0 = S->phin;
-  const double *dxinv = &S->dxinv, *dyinv = &S->dyinv, *dzinv = &S->dzinv, *dt = &c->dt;
+  const double *dxinv = &S->dxinv, *dyinv = &S->dyinv, *dzinv = &S->dzinv, *dt = &S->dt_cur;
   int key[8] = {0};
   double sendjb[1], recvjb[1];
   S->nstep++;
-  S->time = S->time + c->dt;
+  if (c->cfl > 0.0) {
+    /* srcs/main.f90:479 (commented out in the reference, which runs a fixed dt) */
+    orc_caldt(&S->ipara, nID, &S->ID, &S->ndiv, ni, nj, nk, &S->nstep, &c->imon_t, dxinv, dyinv,
+              dzinv, &c->cfl, &c->rhol, &c->rhog, &c->rmul, &c->rmug, &c->surface_tension,
+              FLD(F_U), FLD(F_V), FLD(F_W), &S->dt_cur, &S->time, NULL);
+  } else {
+    S->dt_cur = c->dt;
+    S->time = S->time + c->dt;
+  }
   /* VOF (main.f90:489-504) */
   orc_cal_grad_p2a(&S->ID, &S->svall[1], ni, nj, nk, dxinv, S->dyinv_array, dzinv, phi1,
                    FLD(F_PHIX), FLD(F_PHIY), FLD(F_PHIZ));
@@ -358,6 +370,44 @@ static void orc_state_step1(orc_state *S) {
   orc_cpy(ni, nj, nk, FLD(F_PN), FLD(F_P));
   orc_cpy(ni, nj, nk, phin0, phi0);
   orc_cpy(ni, nj, nk, phin1, phi1);
+}
+
+/* Restart from a checkpoint in the reference's format, srcs/main.f90:375-404:
+ * the record of srcs/datain.f:24-38 (one unformatted sequential record with
+ * 4-byte length markers: u, v, w, p, uo, vo, wo, po, phi(...,0:nbub), time,
+ * nstep) replaces those arrays, time and nstep of a started state; then the
+ * restart fills: bndu of (u,v,w) and (uo,vo,wo), bnd_periodic of the eight
+ * fields, gnbc and the contact-angle ghosts of phi(...,0) with its periodic
+ * fill.  Returns 0, or -1 when the file does not hold this grid's record. */
+int orc_state_datain(orc_state *S, const char *path) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  const size_t n3 = (size_t)S->n3;
+  const size_t body = (8 + 2) * n3 * sizeof(double) + sizeof(double) + sizeof(int);
+  int head = 0, tail = 0, step = 0;
+  double t = 0.0;
+  int ok = fread(&head, sizeof head, 1, f) == 1 && (size_t)head == body;
+  const int order[8] = {F_U, F_V, F_W, F_P, F_UO, F_VO, F_WO, F_PO};
+  for (int m = 0; m < 8 && ok; m++) ok = fread(S->f3[order[m]], sizeof(double), n3, f) == n3;
+  if (ok) ok = fread(S->phi, sizeof(double), 2 * n3, f) == 2 * n3;
+  if (ok) ok = fread(&t, sizeof t, 1, f) == 1 && fread(&step, sizeof step, 1, f) == 1;
+  if (ok) ok = fread(&tail, sizeof tail, 1, f) == 1 && tail == head;
+  fclose(f);
+  if (!ok) return -1;
+  S->time = t;
+  S->nstep = step;
+  const int *ni = &S->c.ni, *nj = &S->c.nj, *nk = &S->c.nk;
+  double *phi0 = S->phi, *phi1 = S->phi + S->n3;
+  orc_bndu(S->nID, ni, nj, nk, FLD(F_U), FLD(F_V), FLD(F_W), &S->c.uwall, &S->c.dy, FLD(F_L1),
+           FLD(F_L2), phi1);
+  orc_bndu(S->nID, ni, nj, nk, FLD(F_UO), FLD(F_VO), FLD(F_WO), &S->c.uwall, &S->c.dy, FLD(F_L1),
+           FLD(F_L2), phi1);
+  for (int m = 0; m < 8; m++) orc_bnd_periodic(ni, nj, nk, FLD(order[m]));
+  orc_gnbc(S->nID, ni, nj, nk, FLD(F_U), FLD(F_W), &S->c.uwall, FLD(F_TH0), &S->c.surface_tension,
+           FLD(F_ZETA), FLD(F_THETA));
+  orc_bnd_contact_angle(S->nID, ni, nj, nk, phi0, FLD(F_THETA), &S->c.dx, &S->c.dy, &S->c.dz);
+  orc_bnd_periodic(ni, nj, nk, phi0);
+  return 0;
 }
 
 void orc_state_step(orc_state *S, int nsteps) {

@@ -24,6 +24,8 @@ class OracleRun:
         self.lib.orc_state_last_sor_ratio.argtypes = [ctypes.c_void_p]
         self.lib.orc_state_last_sor_ratio.restype = ctypes.c_double
         self.lib.orc_state_diag.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        self.lib.orc_state_dt.argtypes = [ctypes.c_void_p]
+        self.lib.orc_state_dt.restype = ctypes.c_double
         self._c = cfg.cstruct()
         self.h = self.lib.orc_state_create(ctypes.byref(self._c))
         st = initial_state(cfg) if state is None else state
@@ -45,12 +47,19 @@ class OracleRun:
     def steps(self, n):
         self.lib.orc_state_step(self.h, int(n))
 
+    def datain(self, path: str):
+        """Restart from a checkpoint file (srcs/main.f90:375-404)."""
+        self.lib.orc_state_datain.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        if self.lib.orc_state_datain(self.h, path.encode()) != 0:
+            raise ValueError(f"{path}: not a checkpoint of this grid")
+
     def state(self):
         d = (ctypes.c_double * 8)()
         self.lib.orc_state_diag(self.h, d)
         return dict(sor_lp=self.lib.orc_state_last_sor_lp(self.h),
                     sor_ratio=self.lib.orc_state_last_sor_ratio(self.h),
-                    center=tuple(d[0:3]), velocity=d[3], vel=tuple(d[4:7]), time=d[7])
+                    center=tuple(d[0:3]), velocity=d[3], vel=tuple(d[4:7]), time=d[7],
+                    dt=self.lib.orc_state_dt(self.h))
 
     def close(self):
         if self.h:

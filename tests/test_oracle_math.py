@@ -680,3 +680,58 @@ def test_cal_vel_and_center_closed_forms():
     assert c1.value == center[0]
     # x-centroid 1.125 < xl/4 = 1.0? no: 1.125 lies in [1, 3] -> area 2
     assert a1.value == 2 and a2.value == 2 and lap.value == 0
+
+
+# --------------------------------------------------------------------------
+# caldt (srcs/caldt.f): closed forms for each limit and a brute force
+# --------------------------------------------------------------------------
+
+def _caldt(g, u, v, w, cfl, rhol, rhog, rmul, rmug, sigma, walls=True, time=0.0):
+    nid = np.full(6, -1 if walls else 0, dtype=np.int32)
+    dt, t = D(0.0), D(time)
+    lim = np.zeros(3)
+    H.call_oracle("caldt", I(0), nid, I(0), I(1), I(g.ni), I(g.nj), I(g.nk), I(1), I(0),
+                  D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), D(cfl), D(rhol), D(rhog), D(rmul), D(rmug),
+                  D(sigma), u, v, w, dt, t, lim)
+    return dt.value, t.value, lim
+
+
+def test_caldt_limits_closed_form():
+    g = Grid(8, 6, 4, 0.1, 0.05, 0.2)
+    dcell = (g.dx * g.dy * g.dz) ** (1.0 / 3.0)
+    u, v, w = g.zeros(), g.zeros(), g.zeros()
+    # advection-limited: the largest |u_c|/d_c is |w|/dz = 3/0.2 < |v|/dy = 2/0.05
+    u[...] = 1.0
+    v[...] = -2.0
+    w[...] = 3.0
+    dt, t, lim = _caldt(g, u, v, w, 0.1, 1.0, 0.1, 1e-3, 1e-5, 1e-3, time=1.0)
+    assert lim[0] == pytest.approx(0.1 / (2.0 / 0.05), rel=1e-15)
+    nu = max(1e-3 / 1.0, 1e-5 / 0.1)
+    assert lim[1] == pytest.approx(0.1 * 0.5 * dcell ** 2 / nu * 100.0, rel=1e-14)
+    assert lim[2] == pytest.approx(0.1 * np.sqrt(dcell ** 3 / (2e-3 / 1.1)), rel=1e-14)
+    assert dt == min(lim) and t == 1.0 + dt
+    assert dt == lim[0]
+    # at rest the floor 1e-20 leaves the viscous or capillary limit
+    z = g.zeros()
+    dt, _, lim = _caldt(g, z, z, z, 0.1, 1.0, 0.1, 10.0, 1.0, 1e-6)
+    assert lim[0] == 0.1 / 1e-20 and dt == lim[1] < lim[2]
+    dt, _, lim = _caldt(g, z, z, z, 0.1, 1.0, 0.1, 1e-6, 1e-7, 10.0)
+    assert dt == lim[2] < lim[1]
+
+
+def test_caldt_max_against_numpy_and_wall_row():
+    g = Grid(10, 7, 6, 0.1, 0.1, 0.1)
+    rng = np.random.default_rng(77)
+    u, v, w = (rng.standard_normal(g.shape) for _ in range(3))
+    # ghosts and the v row j = nj under the Y_PLUS wall do not count
+    v[:, g.nj + 2, :] = 1e6
+    u[:, :, 0] = 1e6
+    inner = (slice(3, g.nk + 3), slice(3, g.nj + 3), slice(3, g.ni + 3))
+    vin = (slice(3, g.nk + 3), slice(3, g.nj + 2), slice(3, g.ni + 3))
+    want = max(np.abs(u[inner] / g.dx).max(), np.abs(v[vin] / g.dy).max(),
+               np.abs(w[inner] / g.dz).max())
+    dt, _, lim = _caldt(g, u, v, w, 0.2, 1.0, 1.0, 1e-6, 1e-6, 1e-6)
+    assert lim[0] == pytest.approx(0.2 / want, rel=1e-15)
+    # without walls the row j = nj counts
+    _, _, lim2 = _caldt(g, u, v, w, 0.2, 1.0, 1.0, 1e-6, 1e-6, 1e-6, walls=False)
+    assert lim2[0] == pytest.approx(0.2 / (1e6 / g.dy), rel=1e-15)
