@@ -153,6 +153,11 @@ class Call {
 // `host` is exactly one whole registered array of `count` doubles.
 uint64_t content_version(const void* host, size_t count);
 void copy_content_version(const void* src, const void* dst, size_t count);
+// cpy_ between two whole registered arrays of one element type: moves the
+// source's device buffer to the destination and defers the source's copy
+// until the source is accessed or the destination changes.  Returns false
+// (nothing done) for any other pair of arrays.
+bool deferred_copy(const void* src, const void* dst, size_t count);
 // Replaces the device buffer behind a whole registered array (same size and
 // element type) and returns the previous one; the caller owns it.
 void* exchange_device_buffer(const void* host, size_t count, void* buf);

@@ -161,23 +161,6 @@ void bndu_phased(const Dims d, T* u, T* v, T* w, double uwall, double dy, const 
 // which bndu never writes), so one thread per written cell reproduces the
 // reference's values, corner cells included.
 // ---------------------------------------------------------------------------
-template <class P>
-struct WallCoef {
-  P c1, c2;
-};
-
-template <class P>
-__device__ __forceinline__ WallCoef<P> wall_coef(const Dims& d, int i, int jw, int k, double dy,
-                                                 const P* l1, const P* l2, const P* phi) {
-  const int ip = (i + 1) % (d.ni + 3);
-  const int cw = d.ix(i, jw, k);
-  const P phi_av = (phi[cw] + phi[d.ix(ip, jw, k)]) / 2.0;
-  const P inv1 = phi_av / l1[cw];
-  const P inv2 = (1.0 - phi_av) / l2[cw];
-  const P ls = 1.0 / (inv1 + inv2);
-  return WallCoef<P>{(2.0 * dy) / (2.0 * ls + dy), (2.0 * ls - dy) / (2.0 * ls + dy)};
-}
-
 // Wall phase value of component q at (i,j,k); q: 0=u, 1=v, 2=w.
 template <class T, class P>
 __device__ __forceinline__ T wall_value(const Dims& d, int q, const T* f, int i, int j, int k,

@@ -49,7 +49,7 @@ struct Cplx<float> {
   __host__ __device__ static float2 make(float a, float b) { return make_float2(a, b); }
 };
 
-// Twiddles exp(+2 pi i t / N), t in [0, N/2), from the host libm (FP64,
+// Twiddles exp(+2 pi i t / N), t in [0, N), from the host libm (FP64,
 // rounded once to T).
 template <class T>
 static const typename Cplx<T>::type* twiddles(int n) {
@@ -57,9 +57,9 @@ static const typename Cplx<T>::type* twiddles(int n) {
   static std::map<int, C*> cache;
   auto it = cache.find(n);
   if (it != cache.end()) return it->second;
-  std::vector<C> h(n / 2 > 0 ? n / 2 : 1);
+  std::vector<C> h(n > 0 ? n : 1);
   const double pi = std::acos(-1.0);
-  for (int t = 0; t < n / 2; t++) {
+  for (int t = 0; t < n; t++) {
     const double a = 2.0 * pi * (double)t / (double)n;
     h[t] = Cplx<T>::make(T(std::cos(a)), T(std::sin(a)));
   }
@@ -70,22 +70,52 @@ static const typename Cplx<T>::type* twiddles(int n) {
   return d;
 }
 
-// Stockham radix-2 DIF transform of G-thread groups, one line per group,
-// ping-ponging between x and y (LDS, line stride ls).  sign = -1 forward.
-// Returns the buffer that holds the result.
+// (a) * conj-or-not twiddle w: w = (wr, sign * wi) of exp(+2 pi i t / N).
+template <class T, class C>
+__device__ __forceinline__ C twiddle_mul(C a, C w0, T sign) {
+  const T wr = w0.x, wi = sign * w0.y;
+  return Cplx<T>::make(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+}
+
+// Mixed-radix Stockham autosort DIF transform (radix-4 stages, a final
+// radix-2 stage when log2 N is odd) of G-thread groups, one line per group,
+// ping-ponging between x and y in LDS.  At stride s with radix r and
+// m = N/(r s), butterfly b = (p, q) = (b / s, b % s) reads
+// x[q + s (p + j m)], j < r, and writes y[q + s (r p + k)] = (sum_j a_j
+// e^{sign 2 pi i j k / r}) e^{sign 2 pi i k p s / N}.  Half the passes (and
+// barriers) of radix 2.  sign = -1 forward.  Returns the buffer that holds
+// the result.
 template <class T, class C>
 __device__ __forceinline__ C* stockham(C* x, C* y, int n, int lane, int g, const C* __restrict__ tw,
                                        T sign) {
-  for (int s = 1; s < n; s *= 2) {
-    const int m = n / (2 * s);
-    for (int b = lane; b < n / 2; b += g) {
+  int s = 1;
+  for (; n / s >= 4; s *= 4) {
+    const int m = n / (4 * s);
+    for (int b = lane; b < n / 4; b += g) {
       const int p = b / s, q = b - p * s;
-      const C w0 = tw[p * s];
-      const T wr = w0.x, wi = sign * w0.y;
-      const C a = x[q + s * p], c = x[q + s * (p + m)];
-      const T dr = a.x - c.x, di = a.y - c.y;
-      y[q + s * (2 * p)] = Cplx<T>::make(a.x + c.x, a.y + c.y);
-      y[q + s * (2 * p + 1)] = Cplx<T>::make(dr * wr - di * wi, dr * wi + di * wr);
+      const C a0 = x[q + s * p], a1 = x[q + s * (p + m)], a2 = x[q + s * (p + 2 * m)],
+              a3 = x[q + s * (p + 3 * m)];
+      const C t0 = Cplx<T>::make(a0.x + a2.x, a0.y + a2.y);
+      const C t1 = Cplx<T>::make(a0.x - a2.x, a0.y - a2.y);
+      const C t2 = Cplx<T>::make(a1.x + a3.x, a1.y + a3.y);
+      const T dx = a1.x - a3.x, dy = a1.y - a3.y;
+      const C t3 = Cplx<T>::make(-(sign * dy), sign * dx);  // (a1 - a3) * (sign i)
+      const int o = q + s * (4 * p), ps = p * s;
+      y[o] = Cplx<T>::make(t0.x + t2.x, t0.y + t2.y);
+      y[o + s] = twiddle_mul<T, C>(Cplx<T>::make(t1.x + t3.x, t1.y + t3.y), tw[ps], sign);
+      y[o + 2 * s] = twiddle_mul<T, C>(Cplx<T>::make(t0.x - t2.x, t0.y - t2.y), tw[2 * ps], sign);
+      y[o + 3 * s] = twiddle_mul<T, C>(Cplx<T>::make(t1.x - t3.x, t1.y - t3.y), tw[3 * ps], sign);
+    }
+    __syncthreads();
+    C* t = x;
+    x = y;
+    y = t;
+  }
+  if (s < n) {  // n / s == 2: the last radix-2 stage (m = 1, p = 0: no twiddles)
+    for (int q = lane; q < s; q += g) {
+      const C a = x[q], c = x[q + s];
+      y[q] = Cplx<T>::make(a.x + c.x, a.y + c.y);
+      y[q + s] = Cplx<T>::make(a.x - c.x, a.y - c.y);
     }
     __syncthreads();
     C* t = x;
@@ -395,7 +425,6 @@ void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv,
 struct PoissonRecord {
   bool rhs = false, coef = false;
   int ni = 0, nj = 0, nk = 0;
-  const double* div = nullptr;   // device (recorded in the FP32 mode only)
   uint64_t div_ver = 0;
   const void* div_host = nullptr;
   double dx2r = 0.0, dz2r = 0.0;
@@ -425,7 +454,9 @@ static void poisson_refine(const Dims d, double* dp) {
   const size_t nm = (size_t)ni * nj * nk;
   float* mr = static_cast<float*>(scratch(9, sizeof(float) * 2 * nm));
   float* mi = mr + nm;
-  const double* div = pr.div;
+  // the right-hand side's current device copy (its buffer may have moved)
+  Call c("solp_fft_tdma3_");
+  const double* div = c.in_t<double>(pr.div_host, d.n3, "div");
   const double* as = pr.tab + ni + nk;
   const double* an = as + (nj + 6);
   const double dx2r = pr.dx2r, dz2r = pr.dz2r;
@@ -479,7 +510,6 @@ void tdma1_t(const Dims d, const double* div, double* phir_r, double* phir_i) {
   PoissonRecord& rec = prec();
   rec.rhs = true;
   rec.ni = d.ni; rec.nj = d.nj; rec.nk = d.nk;
-  rec.div = reinterpret_cast<const double*>(dv);
   rec.div_host = div;
   rec.div_ver = content_version(div, d.n3);
   if (rec.div_ver == 0) rec.rhs = false;  // unregistered: no refinement

@@ -33,7 +33,71 @@ struct SorArgs {
   T *un, *vn, *wn;
   T dtinv;
   VisScales<T> sc;
+  // ghost refresh of the swept component inside the pair launch (see
+  // write_images): slip-wall data and which walls exist
+  const PT<T> *l1, *l2, *phi;
+  double uwall, dy;
+  bool lo, hi;
 };
+
+// The refresh after a colour pair (bndu + bnd_periodic of the swept
+// component, srcs/solu_sor4.f:176-180) rewrites the ghost cells whose
+// source -- the periodic image, or the first interior row of a slip wall --
+// was updated by the pair; every other ghost keeps its value.  The updating
+// thread can write those ghosts itself: a ghost of an updated cell is never
+// read by another cell of the same pair (the face neighbour of the ghost
+// (0,j,k) is (1,j,k), whose parity differs from the source (ni,j,k) in i
+// only when ni is even, so the two are never in one colour pair; likewise
+// in k; a wall ghost's only interior neighbour is its source), so the pair
+// launch with these writes leaves exactly the state of pair + refresh.
+// Images of source (i,j,k) of component q with new value f: x ghosts i-ni
+// (i >= ni-2) and i+ni (i <= 3), z likewise, and the wall rows: for u, w the
+// three ghost rows of bndu's slip formula from row 1 (row nj); for v the
+// mirrored rows -1, -2 from rows 1, 2 (nj+1.. from nj-1..).
+template <int COMP, class T>
+__device__ __forceinline__ void write_images(const Dims& d, const SorArgs<T>& a, T* F, int i, int j,
+                                             int k, T f) {
+  using P = PT<T>;
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  int gi[3], gk[3], nx = 1, nz = 1;
+  gi[0] = i;
+  gk[0] = k;
+  if (i >= ni - 2) gi[nx++] = i - ni;
+  if (i <= 3) gi[nx++] = i + ni;
+  if (k >= nk - 2) gk[nz++] = k - nk;
+  if (k <= 3) gk[nz++] = k + nk;
+  int gj[4];
+  T gv[4];
+  int ny = 1;
+  gj[0] = j;
+  gv[0] = f;
+  if (COMP == 1) {
+    if (a.lo && (j == 1 || j == 2)) { gj[ny] = -j; gv[ny++] = -f; }
+    if (a.hi && j >= nj - 3) { gj[ny] = 2 * nj - j; gv[ny++] = -f; }
+  } else {
+    if (a.lo && j == 1) {
+      const WallCoef<P> cf = wall_coef(d, i, 1, k, a.dy, a.l1, a.l2, a.phi);
+      const T g0 = T(COMP == 0 ? cf.c1 * (-a.uwall) + cf.c2 * P(f) : cf.c2 * P(f));
+      const T gm = T(2.0) * g0 - f;
+      gj[ny] = 0; gv[ny++] = g0;
+      gj[ny] = -1; gv[ny++] = gm;
+      gj[ny] = -2; gv[ny++] = T(2.0) * gm - g0;
+    }
+    if (a.hi && j == nj) {
+      const WallCoef<P> cf = wall_coef(d, i, nj, k, a.dy, a.l1, a.l2, a.phi);
+      const T g1 = T(COMP == 0 ? cf.c1 * a.uwall + cf.c2 * P(f) : cf.c2 * P(f));
+      const T g2 = T(2.0) * g1 - f;
+      gj[ny] = nj + 1; gv[ny++] = g1;
+      gj[ny] = nj + 2; gv[ny++] = g2;
+      gj[ny] = nj + 3; gv[ny++] = T(2.0) * g2 - g1;
+    }
+  }
+  if (nx == 1 && ny == 1 && nz == 1) return;
+  for (int c = 0; c < nz; c++)
+    for (int b = 0; b < ny; b++)
+      for (int x = 0; x < nx; x++)
+        if (x | b | c) F[d.ix(gi[x], gj[b], gk[c])] = gv[b];
+}
 
 // Colour-pair base parities (srcs/solu_sor4.f:94-120); the partner is the
 // complement in every index.
@@ -43,7 +107,7 @@ static const int kIs[4] = {1, 1, 1, 2}, kJs[4] = {1, 1, 2, 1}, kKs[4] = {1, 2, 1
 // number of partial sums (and the final reduction) small.
 constexpr int kKB = 1;
 
-template <int COMP, bool ONFLY, class T>
+template <int COMP, bool ONFLY, bool FUSE, class T>
 __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
                                                        int jend, int gx, int gy, int ntiles,
                                                        double* part) {
@@ -79,7 +143,9 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
                            cf[13] * wn[c] + cf[14] * wn[c + X]) /
                           r00;
       const T du = (visn * h + a.src[0][c]) / (a.dtinv + cf[6] / r00 * h) - un[c];
-      a.un[c] = un[c] + du;
+      const T nv = un[c] + du;
+      a.un[c] = nv;
+      if (FUSE) write_images<0>(d, a, a.un, i, j, k, nv);
       e += val(du) * val(du);
     } else if (COMP == 1) {
       const T visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
@@ -89,7 +155,9 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
                            cf[13] * wn[c] + cf[14] * wn[c + Y]) /
                           r00;
       const T dv = (visn * h + a.src[1][c]) / (a.dtinv + cf[6] / r00 * h) - vn[c];
-      a.vn[c] = vn[c] + dv;
+      const T nv = vn[c] + dv;
+      a.vn[c] = nv;
+      if (FUSE) write_images<1>(d, a, a.vn, i, j, k, nv);
       e += val(dv) * val(dv);
     } else {
       const T visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
@@ -99,7 +167,9 @@ __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int
                            cf[12] * vn[c - Y + Z] + cf[14] * vn[c + Z]) /
                           r00;
       const T dw = (visn * h + a.src[2][c]) / (a.dtinv + cf[6] / r00 * h) - wn[c];
-      a.wn[c] = wn[c] + dw;
+      const T nv = wn[c] + dw;
+      a.wn[c] = nv;
+      if (FUSE) write_images<2>(d, a, a.wn, i, j, k, nv);
       e += val(dw) * val(dw);
     }
   }
@@ -157,6 +227,20 @@ struct SorResult {
   double err, err0;
 };
 
+template <int COMP, class T>
+void launch_pair(bool onfly, bool fuse, dim3 grid, dim3 block, const Dims& d, const SorArgs<T>& a,
+                 int p, int jend, int gx, int gy, int nt, double* pp) {
+  hipStream_t s = stream();
+  if (onfly && fuse)
+    sor_pair_kernel<COMP, true, true, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+  else if (onfly)
+    sor_pair_kernel<COMP, true, false, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+  else if (fuse)
+    sor_pair_kernel<COMP, false, true, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+  else
+    sor_pair_kernel<COMP, false, false, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+}
+
 template <class T, class P = PT<T>>
 SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double uwall, double dy,
                const P* l1, const P* l2, const P* phi, int ndiv) {
@@ -173,6 +257,15 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
     ps.n[s] = nb;
   }
   double* derr = dev_scalars(1);
+  // in-launch ghost refresh needs even ni, nk (colour parity of periodic images)
+  const bool fuse_ok = d.ni % 2 == 0 && d.nk % 2 == 0;
+  a.l1 = l1;
+  a.l2 = l2;
+  a.phi = phi;
+  a.uwall = uwall;
+  a.dy = dy;
+  a.lo = lo;
+  a.hi = hi;
   SorResult res{0, 0.0, 1.0};
   double err0 = 1.0, err = 0.0;
   int lp;
@@ -183,25 +276,26 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
       for (int p = 0; p < 4; p++, slot++) {
         dim3 grid(nb), block(64, 4, 1);
         double* pp = part + ps.off[slot];
+        // Pairs after the first write the swept component's ghost images
+        // themselves (write_images); the refresh after the first pair also
+        // rewrites the other components' ghosts, so it stays a launch.
+        const bool fuse = fuse_ok && p > 0;
         prof_begin(3);
-        if (onfly) {
-          if (comp == 0) sor_pair_kernel<0, true, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 1) sor_pair_kernel<1, true, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 2) sor_pair_kernel<2, true, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-        } else {
-          if (comp == 0) sor_pair_kernel<0, false, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 1) sor_pair_kernel<1, false, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-          if (comp == 2) sor_pair_kernel<2, false, T><<<grid, block, 0, stream()>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, xcd_arg(1, nb), pp);
-        }
+        const int nt = xcd_arg(1, nb);
+        if (comp == 0) launch_pair<0, T>(onfly, fuse, grid, block, d, a, p, jend, gx, gy, nt, pp);
+        if (comp == 1) launch_pair<1, T>(onfly, fuse, grid, block, d, a, p, jend, gx, gy, nt, pp);
+        if (comp == 2) launch_pair<2, T>(onfly, fuse, grid, block, d, a, p, jend, gx, gy, nt, pp);
         FEL_CHECK(hipGetLastError());
         prof_end(3);
-        prof_begin(4);
-        // After the first pair of a component's sweep only that component
-        // changes, and bndu's ghosts of the other two are functions of their
-        // unchanged interiors: rewriting them would store the same values.
-        bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp,
-                    p == 0 ? 7 : (1 << comp));
-        prof_end(4);
+        if (!fuse) {
+          prof_begin(4);
+          // After the first pair of a component's sweep only that component
+          // changes, and bndu's ghosts of the other two are functions of their
+          // unchanged interiors: rewriting them would store the same values.
+          bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp,
+                         p == 0 ? 7 : (1 << comp));
+          prof_end(4);
+        }
       }
     }
     sor_err_stage1<<<kErrChunks, 256, 0, stream()>>>(part, 12 * nb, chunk);

@@ -299,6 +299,11 @@ void cal_arith_coef_vis_t(const Dims d, double dxinv, double dyinv, double dzinv
 
 template <class T>
 void cpy_t(const Dims d, const double* q1, double* q2) {
+  if (q1 == q2) return;
+  if (deferred_copy(q1, q2, d.n3)) {
+    copy_content_version(q1, q2, d.n3);
+    return;
+  }
   Call c("cpy_");
   const T* a = c.in_t<T>(q1, d.n3, "q1");
   T* b = c.out_t<T>(q2, d.n3, "q2");

@@ -29,6 +29,10 @@ struct Range {
   // registration, upload); a whole-array copy hands the source's number to
   // the destination, so equal numbers mean equal contents
   uint64_t ver;
+  // deferred copy: this range's contents equal those of `lazy` (which holds
+  // them in its buffer); ndeps counts the ranges deferred on this one
+  struct Range* lazy;
+  int ndeps;
 };
 
 uint64_t g_version = 0;
@@ -80,6 +84,44 @@ Range* whole_range(const void* host, size_t count) {
   auto it = r.reg.find(reinterpret_cast<uintptr_t>(host));
   if (it == r.reg.end() || it->second.bytes != count * sizeof(double)) return nullptr;
   return &it->second;
+}
+
+// ---------------------------------------------------------------------------
+// Deferred whole-array copies.  cpy_ between two whole registered arrays
+// moves the source's buffer to the destination and leaves the source
+// "deferred": its contents are defined as the destination's.  The copy is
+// made only when the deferred array is accessed, or before the array it
+// defers to changes.  The timestep's rotation (uo <- u, u <- un, and
+// un <- u before the viscous solve) then costs one copy per component
+// instead of three.
+// ---------------------------------------------------------------------------
+void stream_copy(void* dst, const void* src, size_t bytes) {
+  FEL_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, R().s));
+}
+
+void materialize(Range& g) {
+  if (!g.lazy) return;
+  stream_copy(g.dev, g.lazy->dev, g.bytes / sizeof(double) * (size_t)g.esize);
+  g.lazy->ndeps--;
+  g.lazy = nullptr;
+}
+
+void materialize_dependents(Range& g) {
+  if (g.ndeps == 0) return;
+  for (auto& kv : R().reg)
+    if (kv.second.lazy == &g) materialize(kv.second);
+}
+
+void drop_lazy(Range& g) {
+  if (!g.lazy) return;
+  g.lazy->ndeps--;
+  g.lazy = nullptr;
+}
+
+// Everything that reads or writes the device copy of g directly.
+void settle(Range& g, bool write) {
+  materialize(g);
+  if (write) materialize_dependents(g);
 }
 
 }  // namespace
@@ -254,6 +296,29 @@ bool is_registered(const void* host, size_t count) {
   return find_range(R().reg, reinterpret_cast<uintptr_t>(host), count * sizeof(double)) != nullptr;
 }
 
+bool deferred_copy(const void* src_host, const void* dst_host, size_t count) {
+  Range* src = whole_range(src_host, count);
+  Range* dst = whole_range(dst_host, count);
+  if (!src || !dst || src == dst || src->esize != dst->esize) return false;
+  deferred_coef_check(src_host, true);
+  deferred_coef_check(dst_host, true);
+  if (src->lazy == dst) return true;  // the contents are already equal
+  materialize_dependents(*dst);       // the destination's contents change
+  drop_lazy(*dst);
+  Range* holder = src->lazy ? src->lazy : src;
+  std::swap(holder->dev, dst->dev);   // dst now holds the contents
+  // everything deferred on the old holder, and the holder itself, now defers to dst
+  for (auto& kv : R().reg)
+    if (kv.second.lazy == holder) {
+      kv.second.lazy = dst;
+      dst->ndeps++;
+    }
+  holder->ndeps = 0;
+  holder->lazy = dst;
+  dst->ndeps++;
+  return true;
+}
+
 void deferred_coef_flush() {
   DeferredCoef& dc = deferred_coef();
   if (!dc.active) return;
@@ -281,6 +346,7 @@ void* exchange_device_buffer(const void* host, size_t count, void* buf) {
     std::fprintf(stderr, "felgpu: internal: buffer exchange on an unregistered range\n");
     std::abort();
   }
+  settle(*g, true);
   void* old = g->dev;
   g->dev = buf;
   return old;
@@ -330,6 +396,7 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
   }
   deferred_coef_check(host, (a & kWrite) != 0);
   if (Range* g = find_range(r.reg, p, bytes)) {
+    settle(*g, (a & kWrite) != 0);
     if (g->esize != esize) {
       std::fprintf(stderr,
                    "felgpu: %s: array %s is held with %d-byte elements on the device, this call "
@@ -353,7 +420,7 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
       r.transient.erase(it);
     }
     FEL_CHECK(hipMalloc(&dev, dbytes));
-    r.transient[p] = Range{p, dbytes, dev, esize, 0};
+    r.transient[p] = Range{p, dbytes, dev, esize, 0, nullptr, 0};
   }
   h2d(dev, host, count, esize);
   if (a & kWrite) {
@@ -470,7 +537,7 @@ static void register_range(const void* host, const int64_t* nbytes, const int* u
     }
     void* dev = nullptr;
     FEL_CHECK(hipMalloc(&dev, n * (size_t)esize));
-    it = r.reg.emplace(p, Range{p, bytes, dev, esize, 0}).first;
+    it = r.reg.emplace(p, Range{p, bytes, dev, esize, 0, nullptr, 0}).first;
   }
   it->second.ver = next_version();
   if (*upload)
@@ -550,6 +617,7 @@ int felgpu_tangent_() { return tangent_mode() ? 1 : 0; }
 void felgpu_tangent_seed_(const void* host, const double* seed) {
   deferred_coef_check(host, true);
   Range& g = must_find(host, "felgpu_tangent_seed_");
+  settle(g, true);
   if (g.esize != 16) {
     std::fprintf(stderr, "felgpu: felgpu_tangent_seed_: not in the tangent mode\n");
     std::abort();
@@ -567,7 +635,8 @@ void felgpu_tangent_seed_(const void* host, const double* seed) {
 // Copies the derivative part of a whole registered array into `out`.
 void felgpu_tangent_fetch_(const void* host, double* out) {
   deferred_coef_check(host, false);
-  const Range& g = must_find(host, "felgpu_tangent_fetch_");
+  Range& g = must_find(host, "felgpu_tangent_fetch_");
+  settle(g, false);
   if (g.esize != 16) {
     std::fprintf(stderr, "felgpu: felgpu_tangent_fetch_: not in the tangent mode\n");
     std::abort();
@@ -583,6 +652,8 @@ void felgpu_tangent_fetch_(const void* host, double* out) {
 void felgpu_upload_(const void* host) {
   deferred_coef_check(host, true);
   Range& g = must_find(host, "felgpu_upload_");
+  materialize_dependents(g);
+  drop_lazy(g);  // the whole contents are replaced
   g.ver = next_version();
   h2d(g.dev, (const void*)g.host, g.bytes / sizeof(double), g.esize);
   FEL_CHECK(hipStreamSynchronize(stream()));
@@ -591,7 +662,8 @@ void felgpu_upload_(const void* host) {
 // Device -> host copy of a whole registered range (before host-side reads).
 void felgpu_download_(void* host) {
   deferred_coef_check(host, false);
-  const Range& g = must_find(host, "felgpu_download_");
+  Range& g = must_find(host, "felgpu_download_");
+  settle(g, false);
   d2h((void*)g.host, g.dev, g.bytes / sizeof(double), g.esize);
   FEL_CHECK(hipStreamSynchronize(stream()));
 }
@@ -601,6 +673,8 @@ void felgpu_unregister_(const void* host) {
   Runtime& r = R();
   auto it = r.reg.find(reinterpret_cast<uintptr_t>(host));
   if (it == r.reg.end()) return;
+  materialize_dependents(it->second);
+  drop_lazy(it->second);
   FEL_CHECK(hipStreamSynchronize(r.s));
   FEL_CHECK(hipFree(it->second.dev));
   r.reg.erase(it);
@@ -622,7 +696,8 @@ void felgpu_unregister_all_() {
 // Device address of a registered host address (for callers that launch their
 // own work on the library stream, e.g. the test harness).
 void* felgpu_device_ptr_(const void* host) {
-  const Range& g = must_find(host, "felgpu_device_ptr_");
+  Range& g = must_find(host, "felgpu_device_ptr_");
+  settle(g, true);  // the caller may read or write it directly
   return static_cast<char*>(g.dev) +
          (reinterpret_cast<uintptr_t>(host) - g.host) / sizeof(double) * (size_t)g.esize;
 }
