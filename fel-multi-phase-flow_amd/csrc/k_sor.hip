@@ -38,6 +38,10 @@ struct SorArgs {
   const PT<T> *l1, *l2, *phi;
   double uwall, dy;
   bool lo, hi;
+  // slip-wall coefficients c1, c2 of wall_coef at every (i,k) of the
+  // bottom (j = 1) and top (j = nj) walls: [c1 lo | c2 lo | c1 hi | c2 hi]
+  const PT<T>* wcoef;
+  int wplane;  // (ni+6)(nk+6)
 };
 
 // The refresh after a colour pair (bndu + bnd_periodic of the swept
@@ -48,55 +52,55 @@ struct SorArgs {
 // read by another cell of the same pair (the face neighbour of the ghost
 // (0,j,k) is (1,j,k), whose parity differs from the source (ni,j,k) in i
 // only when ni is even, so the two are never in one colour pair; likewise
-// in k; a wall ghost's only interior neighbour is its source), so the pair
-// launch with these writes leaves exactly the state of pair + refresh.
-// Images of source (i,j,k) of component q with new value f: x ghosts i-ni
-// (i >= ni-2) and i+ni (i <= 3), z likewise, and the wall rows: for u, w the
-// three ghost rows of bndu's slip formula from row 1 (row nj); for v the
-// mirrored rows -1, -2 from rows 1, 2 (nj+1.. from nj-1..).
+// in k; a wall ghost's only interior neighbour is its source).
+// Images of source (i,j,k) of component q with new value f written here: x
+// ghosts i-ni (i >= ni-2) and i+ni (i <= 3), z likewise, and for u, w the
+// first wall ghost row (j = 0 from row 1, j = nj+1 from row nj) of bndu's
+// slip formula.  The deeper wall rows (u, w: j = -1, -2 and nj+2, nj+3; v:
+// the mirrored rows) are read by no stencil of the solve; they are written
+// once after the last pair, with the refreshes that leave every ghost as the
+// per-pair refreshes do (each is a function of the final interiors).
+
+// Stores v at the cell c0 (when `self`) and at its periodic images: x
+// (c0 - ni when xa, c0 + ni when xb), z (c0 -+ nk*sx*sy when za / zb) and
+// the xz corners.  Straight-line conditional stores keep the hot kernel's
+// register count unchanged.
+template <class T>
+__device__ __forceinline__ void put_row(T* F, int c0, int ni, int nkz, bool xa, bool xb, bool za,
+                                        bool zb, bool self, T v) {
+  if (self) F[c0] = v;
+  if (xa) F[c0 - ni] = v;
+  if (xb) F[c0 + ni] = v;
+  if (za) {
+    F[c0 - nkz] = v;
+    if (xa) F[c0 - nkz - ni] = v;
+    if (xb) F[c0 - nkz + ni] = v;
+  }
+  if (zb) {
+    F[c0 + nkz] = v;
+    if (xa) F[c0 + nkz - ni] = v;
+    if (xb) F[c0 + nkz + ni] = v;
+  }
+}
+
 template <int COMP, class T>
 __device__ __forceinline__ void write_images(const Dims& d, const SorArgs<T>& a, T* F, int i, int j,
                                              int k, T f) {
   using P = PT<T>;
-  const int ni = d.ni, nj = d.nj, nk = d.nk;
-  int gi[3], gk[3], nx = 1, nz = 1;
-  gi[0] = i;
-  gk[0] = k;
-  if (i >= ni - 2) gi[nx++] = i - ni;
-  if (i <= 3) gi[nx++] = i + ni;
-  if (k >= nk - 2) gk[nz++] = k - nk;
-  if (k <= 3) gk[nz++] = k + nk;
-  int gj[4];
-  T gv[4];
-  int ny = 1;
-  gj[0] = j;
-  gv[0] = f;
-  if (COMP == 1) {
-    if (a.lo && (j == 1 || j == 2)) { gj[ny] = -j; gv[ny++] = -f; }
-    if (a.hi && j >= nj - 3) { gj[ny] = 2 * nj - j; gv[ny++] = -f; }
-  } else {
-    if (a.lo && j == 1) {
-      const WallCoef<P> cf = wall_coef(d, i, 1, k, a.dy, a.l1, a.l2, a.phi);
-      const T g0 = T(COMP == 0 ? cf.c1 * (-a.uwall) + cf.c2 * P(f) : cf.c2 * P(f));
-      const T gm = T(2.0) * g0 - f;
-      gj[ny] = 0; gv[ny++] = g0;
-      gj[ny] = -1; gv[ny++] = gm;
-      gj[ny] = -2; gv[ny++] = T(2.0) * gm - g0;
-    }
-    if (a.hi && j == nj) {
-      const WallCoef<P> cf = wall_coef(d, i, nj, k, a.dy, a.l1, a.l2, a.phi);
-      const T g1 = T(COMP == 0 ? cf.c1 * a.uwall + cf.c2 * P(f) : cf.c2 * P(f));
-      const T g2 = T(2.0) * g1 - f;
-      gj[ny] = nj + 1; gv[ny++] = g1;
-      gj[ny] = nj + 2; gv[ny++] = g2;
-      gj[ny] = nj + 3; gv[ny++] = T(2.0) * g2 - g1;
-    }
-  }
-  if (nx == 1 && ny == 1 && nz == 1) return;
-  for (int c = 0; c < nz; c++)
-    for (int b = 0; b < ny; b++)
-      for (int x = 0; x < nx; x++)
-        if (x | b | c) F[d.ix(gi[x], gj[b], gk[c])] = gv[b];
+  const int ni = d.ni, nj = d.nj, nk = d.nk, Y = d.sx, nkz = nk * d.sx * d.sy;
+  const bool xa = i >= ni - 2, xb = i <= 3, za = k >= nk - 2, zb = k <= 3;
+  const bool wall_src = COMP != 1 && ((a.lo && j == 1) || (a.hi && j == nj));
+  if (!(xa || xb || za || zb || wall_src)) return;
+  const int c0 = d.ix(i, j, k);
+  put_row(F, c0, ni, nkz, xa, xb, za, zb, false, f);
+  if (COMP == 1 || !wall_src) return;
+  // g0 = c1 (-+uwall) + c2 f with the wall coefficients precomputed for the
+  // solve (wall_coef of this cell)
+  const bool low = j == 1;
+  const int wc = (i + 2) + d.sx * (k + 2) + (low ? 0 : 2 * a.wplane);
+  const P c1 = a.wcoef[wc], c2 = a.wcoef[wc + a.wplane];
+  const T g0 = T(COMP == 0 ? c1 * (low ? -a.uwall : a.uwall) + c2 * P(f) : c2 * P(f));
+  put_row(F, c0 + (low ? -Y : Y), ni, nkz, xa, xb, za, zb, true, g0);
 }
 
 // Colour-pair base parities (srcs/solu_sor4.f:94-120); the partner is the
@@ -108,9 +112,21 @@ static const int kIs[4] = {1, 1, 1, 2}, kJs[4] = {1, 1, 2, 1}, kKs[4] = {1, 2, 1
 constexpr int kKB = 1;
 
 template <int COMP, bool ONFLY, bool FUSE, class T>
+__device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a, int is, int js,
+                                              int ks, int jend, int gx, int gy, int ntiles,
+                                              double* part);
+
+template <int COMP, bool ONFLY, bool FUSE, class T>
 __global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
                                                        int jend, int gx, int gy, int ntiles,
                                                        double* part) {
+  sor_pair_body<COMP, ONFLY, FUSE, T>(d, a, is, js, ks, jend, gx, gy, ntiles, part);
+}
+
+template <int COMP, bool ONFLY, bool FUSE, class T>
+__device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a, int is, int js,
+                                              int ks, int jend, int gx, int gy, int ntiles,
+                                              double* part) {
   const int L = xcd_tile((int)blockIdx.x, ntiles);
   const int bx = L % gx, tt = L / gx, by = tt % gy, bz = tt / gy;
   double e = 0.0;
@@ -266,6 +282,25 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
   a.dy = dy;
   a.lo = lo;
   a.hi = hi;
+  a.wplane = d.sx * d.sz;
+  {
+    P* wc = static_cast<P*>(scratch(13, sizeof(P) * 4 * (size_t)a.wplane));
+    const int wp = a.wplane, nj = d.nj;
+    launch_flat(wp, [=] __device__(int m) {
+      const int i = m % d.sx - 2, k = m / d.sx - 2;
+      if (lo) {
+        const WallCoef<P> cf = wall_coef(d, i, 1, k, dy, l1, l2, phi);
+        wc[m] = cf.c1;
+        wc[m + wp] = cf.c2;
+      }
+      if (hi) {
+        const WallCoef<P> cf = wall_coef(d, i, nj, k, dy, l1, l2, phi);
+        wc[m + 2 * wp] = cf.c1;
+        wc[m + 3 * wp] = cf.c2;
+      }
+    });
+    a.wcoef = wc;
+  }
   SorResult res{0, 0.0, 1.0};
   double err0 = 1.0, err = 0.0;
   int lp;
@@ -305,6 +340,15 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
     err = std::sqrt(sum / (double)(d.ni * d.nj * d.nk * ndiv)) + 1.0e-99;
     if (lp == 0) err0 = err;
     if (err / err0 <= 1.0e-3) break;
+  }
+  if (fuse_ok) {
+    // The deeper wall rows the fused pairs leave behind (write_images), in
+    // the state the per-pair refreshes leave: each component's ghosts in the
+    // swept form of its last pair, then bndu's rows of u and v as rewritten
+    // by the refreshes of the later components' pairs.
+    bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, 0, 1);
+    bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, 1, 2);
+    bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, 2, 7);
   }
   if (lp > 20) lp = 20;
   res.lp = lp;
