@@ -13,6 +13,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <initializer_list>
+#include <vector>
 
 // The public C ABI: every extern "C" definition is checked against it.
 #include "felgpu.h"
@@ -57,7 +60,9 @@ hipStream_t stream();
 void ensure_init();
 
 // How an argument array is used by one drop-in call.
-enum Access { kRead = 1, kWrite = 2, kReadWrite = 3 };
+// kOverwrite: every element of the argument is written (no element of the
+// previous contents survives the call).
+enum Access { kRead = 1, kWrite = 2, kReadWrite = 3, kOverwrite = 6 };
 
 // Precision of the device copies.  In the default FP64 mode every device
 // copy is FP64, like the host arrays.  In the FP32 mode (FELGPU_PRECISION=fp32
@@ -68,6 +73,9 @@ enum Access { kRead = 1, kWrite = 2, kReadWrite = 3 };
 // maps, the divergence and the pressure correction -- stay FP64.  Host
 // arrays are FP64 in both modes; uploads and downloads convert.
 bool flow_fp32();
+// Reuse of results already computed (parked results, the result memo): on
+// unless FELGPU_NO_RESULT_REUSE=1 or felgpu_set_result_reuse_(0).
+bool result_reuse();
 // The tangent mode (felgpu_set_tangent_ before the first registration):
 // every device copy holds an FP64 value and its derivative along one input
 // direction (Dual, 16 bytes); the kernels run with T = PT<T> = Dual.
@@ -129,6 +137,11 @@ class Call {
   T* io_t(const void* host, size_t count, const char* what = "") {
     return arr<T>(host, count, kReadWrite, what);
   }
+  // an output whose every element the call writes
+  template <class T>
+  T* all_t(const void* host, size_t count, const char* what = "") {
+    return arr<T>(host, count, kOverwrite, what);
+  }
   void finish();
 
  private:
@@ -152,7 +165,81 @@ class Call {
 // for a result it has already computed.  content_version returns 0 unless
 // `host` is exactly one whole registered array of `count` doubles.
 uint64_t content_version(const void* host, size_t count);
+// As content_version, also for a sub-range of a registered array (a slice
+// such as phi(:,:,:,l)): a number that stays the same until a write
+// overlaps the sub-range.  0 for memory that is not registered.
+uint64_t content_key(const void* host, size_t count);
+// After a copy of count doubles from src to dst (whole arrays or slices):
+// dst's contents are numbered as src's.
 void copy_content_version(const void* src, const void* dst, size_t count);
+// Content number of the elements of a whole padded array outside the box
+// (-2..n+2)^3, and its restoration by an entry point that writes exactly
+// that box (the number survives such a write).
+uint64_t shell_key(const void* host, size_t count);
+void keep_shell(const void* host, size_t count, uint64_t key);
+
+// Result memo.  An entry point whose outputs are a pure function of its
+// arguments describes the call by a key: an operation number, the scalar
+// arguments (bit patterns) and the content numbers of the input arrays (and
+// of whatever part of an output the call does not overwrite).  memo_record
+// notes which arrays hold the outputs of a key; a later call with the same
+// key finds them with memo_reuse while they still hold those contents and
+// gets them without a kernel: in place when the output array is the holder,
+// else (allow_copy) by a deferred copy from the holder.  A key computed
+// again gives its outputs the same content numbers as before, so a chain of
+// calls repeated on equal inputs is recognised link by link.
+// FELGPU_NO_RESULT_REUSE=1 turns reuse off.
+struct MemoKey {
+  std::vector<uint64_t> w;
+  bool ok = true;  // false when an input is not a registered array
+  explicit MemoKey(int op) { w.push_back((uint64_t)op); }
+  MemoKey& num(int64_t x) {
+    w.push_back((uint64_t)x);
+    return *this;
+  }
+  MemoKey& num(double x) {
+    uint64_t b;
+    std::memcpy(&b, &x, sizeof b);
+    w.push_back(b);
+    return *this;
+  }
+  MemoKey& dims(const Dims& d) { return num((int64_t)d.ni).num((int64_t)d.nj).num((int64_t)d.nk); }
+  MemoKey& content(const void* host, size_t count);
+  MemoKey& shell(const void* host, size_t count);
+};
+struct MemoOut {
+  const void* host;
+  size_t count;
+};
+bool memo_reuse(const MemoKey& k, std::initializer_list<MemoOut> outs, bool allow_copy);
+void memo_record(const MemoKey& k, std::initializer_list<MemoOut> outs);
+void memo_release();
+// Key of a face-gradient call (cal_grad_p2a_, cal_grad_p2uvw_): the
+// arguments, the input's contents and the shells of the three outputs,
+// which the call leaves as they are; sh receives those shell numbers.
+inline MemoKey gradient_key(int op, int esize, const Dims& d, int ID, int svall, double dxinv,
+                            const double* dyinv_array, double dzinv, const void* q,
+                            const void* const* out, uint64_t* sh) {
+  MemoKey k(op);
+  k.num((int64_t)esize).dims(d).num((int64_t)ID).num((int64_t)svall).num(dxinv).num(dzinv);
+  for (int m = 0; m < svall + 6; m++) k.num(dyinv_array[m]);
+  k.content(q, (size_t)d.n3);
+  for (int m = 0; m < 3; m++) {
+    k.shell(out[m], (size_t)d.n3);
+    sh[m] = k.w.back();
+  }
+  return k;
+}
+// Operation numbers of the memoised entry points.
+enum MemoOp {
+  kOpGradP2a = 1,
+  kOpGradP2uvw,
+  kOpCalrn,
+  kOpCalrkap,
+  kOpCalfst,
+  kOpSumFst,
+  kOpZero
+};
 // cpy_ between two whole registered arrays of one element type: moves the
 // source's device buffer to the destination and defers the source's copy
 // until the source is accessed or the destination changes.  Returns false

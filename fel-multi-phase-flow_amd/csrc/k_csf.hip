@@ -278,6 +278,12 @@ void calfst(const Dims d, double st_, const T* px, const T* py, const T* pz, con
 template <class T>
 void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const double* dyinv_array,
                       double dzinv, const double* q, double* qx_u, double* qy_v, double* qz_w) {
+  const size_t n = (size_t)d.n3;
+  const void* outs[3] = {qx_u, qy_v, qz_w};
+  uint64_t sh[3];
+  const MemoKey k = gradient_key(kOpGradP2uvw, (int)sizeof(T), d, ID, svall, dxinv, dyinv_array,
+                                 dzinv, q, outs, sh);
+  if (memo_reuse(k, {{qx_u, n}, {qy_v, n}, {qz_w, n}}, false)) return;
   Call c("cal_grad_p2uvw_");
   bool uni = true;
   const int cnt = svall + 6;
@@ -296,17 +302,23 @@ void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const doubl
   T* z = c.io_t<T>(qz_w, d.n3, "qz_w");
   grad_p2uvw<T>(d, ID, dxinv, table, dyinv_array[0], dzinv, dq, x, y, z);
   c.finish();
+  memo_record(k, {{qx_u, n}, {qy_v, n}, {qz_w, n}});
+  for (int m = 0; m < 3; m++) keep_shell(outs[m], n, sh[m]);  // written: the box only
 }
 
 template <class T>
 void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix, const double* phiy,
              const double* phiz, double* rn) {
   const size_t n = (size_t)d.n3, n3x = 3 * n;
+  MemoKey k(kOpCalrn);
+  k.num((int64_t)sizeof(T)).dims(d).num(vs).num((int64_t)ip).num((int64_t)jp).num((int64_t)kp);
+  k.content(phix, n).content(phiy, n).content(phiz, n);
+  if (memo_reuse(k, {{rn, n3x}}, true)) return;
   const int sreq = stagger_index(ip, jp, kp);
   const uint64_t ver[3] = {content_version(phix, n), content_version(phiy, n),
                            content_version(phiz, n)};
   const bool track = sreq >= 0 && ver[0] && ver[1] && ver[2] && content_version(rn, n3x) != 0 &&
-                     std::getenv("FELGPU_NO_RESULT_REUSE") == nullptr;
+                     result_reuse();
   if (track) {
     const bool same = g_rn.ver[0] == ver[0] && g_rn.ver[1] == ver[1] && g_rn.ver[2] == ver[2] &&
                       g_rn.ni == d.ni && g_rn.nj == d.nj && g_rn.nk == d.nk && g_rn.vs == vs;
@@ -314,8 +326,9 @@ void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix
       g_rn.buf[sreq] = exchange_device_buffer(rn, n3x, g_rn.buf[sreq]);
       g_rn.valid[sreq] = false;
       Call c("calrn_");  // the output changes: a fresh version
-      c.out_t<T>(rn, n3x, "rn");
+      c.all_t<T>(rn, n3x, "rn");
       c.finish();
+      memo_record(k, {{rn, n3x}});
       return;
     }
     const size_t bytes = n3x * sizeof(T);
@@ -332,48 +345,61 @@ void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix
     const T* z = c.in_t<T>(phiz, n, "phiz");
     T* out[4];
     for (int s = 0; s < 4; s++)
-      out[s] = s == sreq ? c.out_t<T>(rn, n3x, "rn") : static_cast<T*>(g_rn.buf[s]);
+      out[s] = s == sreq ? c.all_t<T>(rn, n3x, "rn") : static_cast<T*>(g_rn.buf[s]);
     calrn4<T>(d, vs, x, y, z, out);
     c.finish();
     for (int s = 0; s < 4; s++) g_rn.valid[s] = s != sreq;
     for (int m = 0; m < 3; m++) g_rn.ver[m] = ver[m];
     g_rn.ni = d.ni; g_rn.nj = d.nj; g_rn.nk = d.nk;
     g_rn.vs = vs;
+    memo_record(k, {{rn, n3x}});
     return;
   }
   Call c("calrn_");
   const T* x = c.in_t<T>(phix, d.n3, "phix");
   const T* y = c.in_t<T>(phiy, d.n3, "phiy");
   const T* z = c.in_t<T>(phiz, d.n3, "phiz");
-  T* r = c.out_t<T>(rn, 3 * (size_t)d.n3, "rn");
+  T* r = c.all_t<T>(rn, 3 * (size_t)d.n3, "rn");
   calrn<T>(d, vs, ip, jp, kp, x, y, z, r);
   c.finish();
+  memo_record(k, {{rn, n3x}});
 }
 
 template <class T>
 void calrkap_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* rn_pp,
                const double* rn_12, const double* rn_13, const double* rn_23, double* rkap) {
-  Call c("calrkap_");
   const size_t n = 3 * (size_t)d.n3;
+  MemoKey k(kOpCalrkap);
+  k.num((int64_t)sizeof(T)).dims(d).num(dxinv).num(dyinv).num(dzinv);
+  k.content(rn_pp, n).content(rn_12, n).content(rn_13, n).content(rn_23, n);
+  if (memo_reuse(k, {{rkap, n}}, true)) return;
+  Call c("calrkap_");
   const T *a = c.in_t<T>(rn_pp, n, "rn_pp"), *b = c.in_t<T>(rn_12, n, "rn_12");
   const T *e = c.in_t<T>(rn_13, n, "rn_13"), *f = c.in_t<T>(rn_23, n, "rn_23");
-  T* r = c.out_t<T>(rkap, n, "rkap");
+  T* r = c.all_t<T>(rkap, n, "rkap");
   calrkap<T>(d, dxinv, dyinv, dzinv, a, b, e, f, r);
   c.finish();
+  memo_record(k, {{rkap, n}});
 }
 
 template <class T>
 void calfst_t(const Dims d, double st, const double* phix_u, const double* phiy_v,
               const double* phiz_w, const double* rkap, double* fst_u, double* fst_v,
               double* fst_w) {
+  const size_t n = (size_t)d.n3;
+  MemoKey k(kOpCalfst);
+  k.num((int64_t)sizeof(T)).dims(d).num(st);
+  k.content(phix_u, n).content(phiy_v, n).content(phiz_w, n).content(rkap, 3 * n);
+  if (memo_reuse(k, {{fst_u, n}, {fst_v, n}, {fst_w, n}}, true)) return;
   Call c("calfst_");
   const T *x = c.in_t<T>(phix_u, d.n3, "phix_u"), *y = c.in_t<T>(phiy_v, d.n3, "phiy_v"),
           *z = c.in_t<T>(phiz_w, d.n3, "phiz_w");
   const T* rk = c.in_t<T>(rkap, 3 * (size_t)d.n3, "rkap");
-  T *fu = c.out_t<T>(fst_u, d.n3, "fst_u"), *fv = c.out_t<T>(fst_v, d.n3, "fst_v"),
-    *fw = c.out_t<T>(fst_w, d.n3, "fst_w");
+  T *fu = c.all_t<T>(fst_u, d.n3, "fst_u"), *fv = c.all_t<T>(fst_v, d.n3, "fst_v"),
+    *fw = c.all_t<T>(fst_w, d.n3, "fst_w");
   calfst<T>(d, st, x, y, z, rk, fu, fv, fw);
   c.finish();
+  memo_record(k, {{fst_u, n}, {fst_v, n}, {fst_w, n}});
 }
 
 }  // namespace fel

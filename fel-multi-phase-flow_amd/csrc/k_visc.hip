@@ -314,16 +314,29 @@ void cpy_t(const Dims d, const double* q1, double* q2) {
 
 template <class T>
 void init_q_t(const Dims d, double* q1, double* q2, double* q3) {
-  Call c("init_q_");
-  T* p[3] = {c.out_t<T>(q1, d.n3, "q1"), c.out_t<T>(q2, d.n3, "q2"), c.out_t<T>(q3, d.n3, "q3")};
-  for (int m = 0; m < 3; m++)
-    FEL_CHECK(hipMemsetAsync(p[m], 0, sizeof(T) * (size_t)d.n3, stream()));
-  c.finish();
+  const size_t n = (size_t)d.n3;
+  double* q[3] = {q1, q2, q3};
+  MemoKey k(kOpZero);
+  k.num((int64_t)sizeof(T)).num((int64_t)n);
+  for (int m = 0; m < 3; m++) {
+    if (memo_reuse(k, {{q[m], n}}, false)) continue;  // already zero
+    Call c("init_q_");
+    T* p = c.all_t<T>(q[m], n, "q");
+    FEL_CHECK(hipMemsetAsync(p, 0, sizeof(T) * n, stream()));
+    c.finish();
+    memo_record(k, {{q[m], n}});
+  }
 }
 
 template <class T>
 void sum_fst_t(const Dims d, const double* fst_u, const double* fst_v, const double* fst_w,
                double* sum_u, double* sum_v, double* sum_w) {
+  const size_t n = (size_t)d.n3;
+  MemoKey k(kOpSumFst);
+  k.num((int64_t)sizeof(T)).dims(d);
+  k.content(fst_u, n).content(fst_v, n).content(fst_w, n);
+  k.content(sum_u, n).content(sum_v, n).content(sum_w, n);
+  if (memo_reuse(k, {{sum_u, n}, {sum_v, n}, {sum_w, n}}, true)) return;
   Call c("sum_fst_");
   const T *fu = c.in_t<T>(fst_u, d.n3, "fst_u"), *fv = c.in_t<T>(fst_v, d.n3, "fst_v"),
           *fw = c.in_t<T>(fst_w, d.n3, "fst_w");
@@ -335,6 +348,7 @@ void sum_fst_t(const Dims d, const double* fst_u, const double* fst_v, const dou
     sw[m] = sw[m] + fw[m];
   });
   c.finish();
+  memo_record(k, {{sum_u, n}, {sum_v, n}, {sum_w, n}});
 }
 
 }  // namespace fel

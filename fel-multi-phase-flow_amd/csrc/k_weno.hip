@@ -227,13 +227,7 @@ struct WenoStash {
 std::vector<WenoHeld> g_held;
 WenoStash g_stash;
 
-bool reuse_disabled() {
-  static const int off = [] {
-    const char* e = std::getenv("FELGPU_NO_RESULT_REUSE");
-    return (e && std::atoi(e) != 0) ? 1 : 0;
-  }();
-  return off != 0;
-}
+bool reuse_disabled() { return !result_reuse(); }
 
 WenoHeld* held_for(const void* const* out) {
   for (auto& h : g_held)

@@ -33,10 +33,38 @@ struct Range {
   // them in its buffer); ndeps counts the ranges deferred on this one
   struct Range* lazy;
   int ndeps;
+  // content number of the elements outside the box (-2..n+2)^3 of a padded
+  // array (0: none assigned yet); kept by the entry points that write
+  // exactly that box, renewed by every other write
+  uint64_t shell = 0;
+  // content numbers of sub-ranges (e.g. one slice phi(:,:,:,l)), assigned
+  // on request and dropped by any overlapping write
+  struct Sub {
+    size_t off, count;  // in elements
+    uint64_t key;
+  };
+  std::vector<Sub> subs;
 };
 
 uint64_t g_version = 0;
 uint64_t next_version() { return ++g_version; }
+
+// A write to [p, p+bytes) of g: fresh content number for the whole range,
+// unknown shell, and the sub-range numbers the write overlaps dropped.
+void note_write(Range& g, uintptr_t p, size_t bytes) {
+  g.ver = next_version();
+  g.shell = 0;
+  if (p == g.host && bytes == g.bytes) {
+    g.subs.clear();
+    return;
+  }
+  const size_t a = (p - g.host) / sizeof(double), b = a + bytes / sizeof(double);
+  std::vector<Range::Sub> keep;
+  for (const Range::Sub& x : g.subs)
+    if (x.off + x.count <= a || x.off >= b) keep.push_back(x);
+  g.subs.swap(keep);
+}
+void note_write(Range& g) { note_write(g, g.host, g.bytes); }
 
 struct Runtime {
   bool init = false;
@@ -144,6 +172,18 @@ void ensure_init() {
 bool flow_fp32() {
   ensure_init();
   return R().flow_esize == 4;
+}
+
+namespace {
+int g_reuse = -1;  // -1: not yet read from the environment
+}
+
+bool result_reuse() {
+  if (g_reuse < 0) {
+    const char* e = std::getenv("FELGPU_NO_RESULT_REUSE");
+    g_reuse = (e && std::atoi(e) != 0) ? 0 : 1;
+  }
+  return g_reuse != 0;
 }
 
 bool tangent_mode() {
@@ -326,7 +366,7 @@ void deferred_coef_flush() {
   coef_vis_any(dc.d, dc.dxinv, dc.dyinv, dc.dzinv, dc.src_dev, dc.out_dev, dc.esize);
   for (int m = 0; m < 45; m++)
     if (Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(dc.out_host[m]), 1))
-      g->ver = next_version();
+      note_write(*g);
 }
 
 uint64_t content_version(const void* host, size_t count) {
@@ -334,11 +374,162 @@ uint64_t content_version(const void* host, size_t count) {
   return g ? g->ver : 0;
 }
 
-void copy_content_version(const void* src, const void* dst, size_t count) {
-  const Range* a = whole_range(src, count);
-  Range* b = whole_range(dst, count);
-  if (a && b) b->ver = a->ver;
+uint64_t content_key(const void* host, size_t count) {
+  const uintptr_t p = reinterpret_cast<uintptr_t>(host);
+  Range* g = find_range(R().reg, p, count * sizeof(double));
+  if (!g) return 0;
+  if (p == g->host && count * sizeof(double) == g->bytes) return g->ver;
+  const size_t off = (p - g->host) / sizeof(double);
+  for (const Range::Sub& x : g->subs)
+    if (x.off == off && x.count == count) return x.key;
+  g->subs.push_back(Range::Sub{off, count, next_version()});
+  return g->subs.back().key;
 }
+
+void copy_content_version(const void* src, const void* dst, size_t count) {
+  const uintptr_t q = reinterpret_cast<uintptr_t>(dst);
+  Range* b = find_range(R().reg, q, count * sizeof(double));
+  const uint64_t key = content_key(src, count);
+  if (!b || key == 0) return;
+  const Range* a = whole_range(src, count);
+  if (q == b->host && count * sizeof(double) == b->bytes) {
+    b->ver = key;
+    b->subs = a ? a->subs : std::vector<Range::Sub>{};
+    b->shell = a ? a->shell : 0;
+    return;
+  }
+  // a slice: the copy's write has dropped what it overlapped
+  b->subs.push_back(Range::Sub{(q - b->host) / sizeof(double), count, key});
+}
+
+uint64_t shell_key(const void* host, size_t count) {
+  Range* g = whole_range(host, count);
+  if (!g) return 0;
+  if (g->shell == 0) g->shell = next_version();
+  return g->shell;
+}
+
+void keep_shell(const void* host, size_t count, uint64_t key) {
+  if (Range* g = whole_range(host, count)) g->shell = key;
+}
+
+// --------------------------------------------------------------------------
+// Result memo
+// --------------------------------------------------------------------------
+
+namespace {
+struct MemoEntry {
+  std::vector<uint64_t> key;
+  int nout = 0;
+  uintptr_t host[4];
+  size_t count[4];
+  uint64_t ver[4];
+  uint64_t used = 0;
+};
+std::vector<MemoEntry> g_memo;
+uint64_t g_memo_tick = 0;
+int64_t g_memo_hits = 0;
+constexpr size_t kMemoEntries = 64;
+
+bool memo_enabled() { return result_reuse(); }
+
+MemoEntry* memo_find(const MemoKey& k) {
+  for (MemoEntry& e : g_memo)
+    if (e.key == k.w) return &e;
+  return nullptr;
+}
+
+bool outs_registered(std::initializer_list<MemoOut> outs) {
+  if (outs.size() > 4) return false;
+  for (const MemoOut& o : outs)
+    if (!whole_range(o.host, o.count)) return false;
+  return true;
+}
+}  // namespace
+
+MemoKey& MemoKey::content(const void* host, size_t count) {
+  const uint64_t v = content_key(host, count);
+  if (v == 0) ok = false;
+  w.push_back(v);
+  return *this;
+}
+
+MemoKey& MemoKey::shell(const void* host, size_t count) {
+  const uint64_t v = shell_key(host, count);
+  if (v == 0) ok = false;
+  w.push_back(v);
+  return *this;
+}
+
+bool memo_reuse(const MemoKey& k, std::initializer_list<MemoOut> outs, bool allow_copy) {
+  if (!k.ok || !memo_enabled() || !outs_registered(outs)) return false;
+  MemoEntry* e = memo_find(k);
+  if (!e || e->nout != (int)outs.size()) return false;
+  int m = 0;
+  for (const MemoOut& o : outs) {
+    Range* h = whole_range(reinterpret_cast<const void*>(e->host[m]), e->count[m]);
+    Range* d = whole_range(o.host, o.count);
+    if (!h || h->ver != e->ver[m] || e->count[m] != o.count || h->esize != d->esize) return false;
+    if (h != d && !allow_copy) return false;
+    m++;
+  }
+  m = 0;
+  for (const MemoOut& o : outs) {
+    const void* src = reinterpret_cast<const void*>(e->host[m]);
+    if (src != o.host) {
+      if (!deferred_copy(src, o.host, o.count)) {
+        std::fprintf(stderr, "felgpu: internal: memo copy refused\n");
+        std::abort();
+      }
+      copy_content_version(src, o.host, o.count);
+      e->host[m] = reinterpret_cast<uintptr_t>(o.host);
+    }
+    m++;
+  }
+  e->used = ++g_memo_tick;
+  g_memo_hits++;
+  return true;
+}
+
+void memo_record(const MemoKey& k, std::initializer_list<MemoOut> outs) {
+  if (!k.ok || !memo_enabled() || !outs_registered(outs)) return;
+  MemoEntry* e = memo_find(k);
+  if (e && e->nout == (int)outs.size()) {
+    // the same function of the same inputs: the outputs carry the contents
+    // numbers given to this key before
+    int m = 0;
+    for (const MemoOut& o : outs) {
+      whole_range(o.host, o.count)->ver = e->ver[m];
+      e->host[m] = reinterpret_cast<uintptr_t>(o.host);
+      e->count[m] = o.count;
+      m++;
+    }
+    e->used = ++g_memo_tick;
+    return;
+  }
+  if (!e) {
+    if (g_memo.size() < kMemoEntries) {
+      g_memo.emplace_back();
+      e = &g_memo.back();
+    } else {
+      e = &g_memo[0];
+      for (MemoEntry& x : g_memo)
+        if (x.used < e->used) e = &x;
+    }
+  }
+  e->key = k.w;
+  e->nout = (int)outs.size();
+  int m = 0;
+  for (const MemoOut& o : outs) {
+    e->host[m] = reinterpret_cast<uintptr_t>(o.host);
+    e->count[m] = o.count;
+    e->ver[m] = whole_range(o.host, o.count)->ver;
+    m++;
+  }
+  e->used = ++g_memo_tick;
+}
+
+void memo_release() { g_memo.clear(); }
 
 void* exchange_device_buffer(const void* host, size_t count, void* buf) {
   Range* g = whole_range(host, count);
@@ -346,7 +537,9 @@ void* exchange_device_buffer(const void* host, size_t count, void* buf) {
     std::fprintf(stderr, "felgpu: internal: buffer exchange on an unregistered range\n");
     std::abort();
   }
-  settle(*g, true);
+  // the whole contents are replaced
+  materialize_dependents(*g);
+  drop_lazy(*g);
   void* old = g->dev;
   g->dev = buf;
   return old;
@@ -396,7 +589,14 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
   }
   deferred_coef_check(host, (a & kWrite) != 0);
   if (Range* g = find_range(r.reg, p, bytes)) {
-    settle(*g, (a & kWrite) != 0);
+    if ((a & kOverwrite) == kOverwrite && p == g->host && bytes == g->bytes) {
+      // every element is replaced: a deferred copy of the old contents is
+      // not needed, only the ranges deferred on this one
+      materialize_dependents(*g);
+      drop_lazy(*g);
+    } else {
+      settle(*g, (a & kWrite) != 0);
+    }
     if (g->esize != esize) {
       std::fprintf(stderr,
                    "felgpu: %s: array %s is held with %d-byte elements on the device, this call "
@@ -404,7 +604,7 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
                    name_, what, g->esize, esize, esize == 8 ? "_fp64" : "");
       std::abort();
     }
-    if (a & kWrite) g->ver = next_version();
+    if (a & kWrite) note_write(*g, p, bytes);
     return static_cast<char*>(g->dev) + (p - g->host) / sizeof(double) * (size_t)esize;
   }
   // per-call mirror, in the element type the kernel reads
@@ -539,7 +739,7 @@ static void register_range(const void* host, const int64_t* nbytes, const int* u
     FEL_CHECK(hipMalloc(&dev, n * (size_t)esize));
     it = r.reg.emplace(p, Range{p, bytes, dev, esize, 0, nullptr, 0}).first;
   }
-  it->second.ver = next_version();
+  note_write(it->second);
   if (*upload)
     h2d(it->second.dev, host, n, esize);
   else
@@ -612,6 +812,17 @@ void felgpu_set_tangent_(const int* on) {
 
 int felgpu_tangent_() { return tangent_mode() ? 1 : 0; }
 
+// Turns the reuse of results already computed (parked results, the result
+// memo) on or off; the default is on unless FELGPU_NO_RESULT_REUSE=1.
+// Results are the same either way.
+void felgpu_set_result_reuse_(const int* on) {
+  result_reuse();
+  g_reuse = *on != 0 ? 1 : 0;
+}
+
+// Number of entry-point calls answered from the result memo so far.
+int64_t felgpu_result_reuse_hits_() { return g_memo_hits; }
+
 // Sets the derivative part of a whole registered array from the host array
 // `seed` (same number of elements); the values are kept.
 void felgpu_tangent_seed_(const void* host, const double* seed) {
@@ -629,7 +840,7 @@ void felgpu_tangent_seed_(const void* host, const double* seed) {
   for (size_t m = 0; m < n; m++) tmp[m].d = seed[m];
   FEL_CHECK(hipMemcpyAsync(g.dev, tmp.data(), n * 16, hipMemcpyHostToDevice, stream()));
   FEL_CHECK(hipStreamSynchronize(stream()));
-  g.ver = next_version();
+  note_write(g);
 }
 
 // Copies the derivative part of a whole registered array into `out`.
@@ -654,7 +865,7 @@ void felgpu_upload_(const void* host) {
   Range& g = must_find(host, "felgpu_upload_");
   materialize_dependents(g);
   drop_lazy(g);  // the whole contents are replaced
-  g.ver = next_version();
+  note_write(g);
   h2d(g.dev, (const void*)g.host, g.bytes / sizeof(double), g.esize);
   FEL_CHECK(hipStreamSynchronize(stream()));
 }
@@ -687,6 +898,7 @@ void felgpu_unregister_all_() {
   FEL_CHECK(hipStreamSynchronize(r.s));
   weno_release();
   rn_release();
+  memo_release();
   for (auto& kv : r.reg) FEL_CHECK(hipFree(kv.second.dev));
   r.reg.clear();
   for (auto& kv : r.transient) FEL_CHECK(hipFree(kv.second.dev));
@@ -698,6 +910,7 @@ void felgpu_unregister_all_() {
 void* felgpu_device_ptr_(const void* host) {
   Range& g = must_find(host, "felgpu_device_ptr_");
   settle(g, true);  // the caller may read or write it directly
+  note_write(g);    // so its contents count as changed
   return static_cast<char*>(g.dev) +
          (reinterpret_cast<uintptr_t>(host) - g.host) / sizeof(double) * (size_t)g.esize;
 }

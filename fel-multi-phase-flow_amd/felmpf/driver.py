@@ -34,7 +34,7 @@ class GpuRun:
                "phin1": ("phin0", 1)}
 
     def __init__(self, cfg: RunConfig, state: dict[str, np.ndarray] | None = None,
-                 precision: int = 64, tangent: bool = False):
+                 precision: int = 64, tangent: bool = False, reuse: bool = True):
         if precision not in (32, 64):
             raise ValueError(f"precision must be 32 or 64, not {precision}")
         if tangent and precision != 64:
@@ -45,6 +45,7 @@ class GpuRun:
         self.drv = _lib.driver()
         self.gpu = _lib.gpu()
         self.drv.fel_drv_field.restype = ctypes.c_void_p
+        self.gpu.felgpu_result_reuse_hits_.restype = ctypes.c_int64
         self.drv.fel_drv_field.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         self.drv.fel_drv_steps.argtypes = [ctypes.c_int]
         self._c = cfg.cstruct()
@@ -54,6 +55,9 @@ class GpuRun:
             self.view(name)[...] = arr
         self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(precision)))
         self.gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(1 if tangent else 0)))
+        # results already computed are taken over instead of recomputed
+        # (bitwise the same either way)
+        self.gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(1 if reuse else 0)))
         self.drv.fel_drv_start()
 
     def view(self, name: str) -> np.ndarray:
@@ -63,7 +67,11 @@ class GpuRun:
         if not ptr:
             raise KeyError(name)
         buf = (ctypes.c_double * n.value).from_address(ptr)
-        return np.ctypeslib.as_array(buf).reshape(self.cfg.shape)
+        a = np.ctypeslib.as_array(buf)
+        per = int(np.prod(self.cfg.shape))
+        if n.value == per:
+            return a.reshape(self.cfg.shape)
+        return a.reshape((n.value // per,) + tuple(self.cfg.shape))  # vector fields (rn, rkap)
 
     def field(self, name: str) -> np.ndarray:
         """Device copy of a field, as a new host array."""

@@ -244,6 +244,16 @@ contains
     case ('phix'); ptr = c_loc(phix)
     case ('phiy'); ptr = c_loc(phiy)
     case ('phiz'); ptr = c_loc(phiz)
+    case ('rn_pp'); ptr = c_loc(rn_pp); n = 3 * n
+    case ('rn_12'); ptr = c_loc(rn_12); n = 3 * n
+    case ('rn_13'); ptr = c_loc(rn_13); n = 3 * n
+    case ('rn_23'); ptr = c_loc(rn_23); n = 3 * n
+    case ('rnn_pp'); ptr = c_loc(rnn_pp); n = 3 * n
+    case ('rnn_12'); ptr = c_loc(rnn_12); n = 3 * n
+    case ('rnn_13'); ptr = c_loc(rnn_13); n = 3 * n
+    case ('rnn_23'); ptr = c_loc(rnn_23); n = 3 * n
+    case ('rkap'); ptr = c_loc(rkap); n = 3 * n
+    case ('rkapn'); ptr = c_loc(rkapn); n = 3 * n
     case default; n = 0
     end select
   end function fel_drv_field

@@ -87,6 +87,17 @@ int felgpu_tangent_(void);
 void felgpu_tangent_seed_(const void *host, const double *seed);
 void felgpu_tangent_fetch_(const void *host, double *out);
 
+/* Reuse of results already computed (no counterpart in the reference):
+ * when an entry point is asked for a result whose inputs have exactly the
+ * contents of an earlier call's (tracked by content numbers; e.g. the
+ * surface-tension chain of phi at step n+1 repeats the chain of phin at
+ * step n), the result is taken from the arrays that still hold it, by a
+ * buffer exchange, instead of being recomputed.  *on = 0 turns this off
+ * (as FELGPU_NO_RESULT_REUSE=1 does at start-up); results are bitwise the
+ * same either way.  felgpu_result_reuse_hits_ counts the calls answered so. */
+void felgpu_set_result_reuse_(const int *on);
+int64_t felgpu_result_reuse_hits_(void);
+
 /* Host -> device copy of the whole registered range containing `host`. */
 void felgpu_upload_(const void *host);
 
@@ -97,7 +108,9 @@ void felgpu_download_(void *host);
 void felgpu_unregister_(const void *host);
 void felgpu_unregister_all_(void);
 
-/* Device address backing a registered host address (NULL if unregistered). */
+/* Device address backing a registered host address (an unregistered one is
+ * an error).  The caller may write through it, so the range's contents
+ * count as changed. */
 void *felgpu_device_ptr_(const void *host);
 
 /* Waits for all queued work; returns the library's hipStream_t. */

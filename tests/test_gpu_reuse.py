@@ -174,3 +174,119 @@ def test_four_staggered_normals_from_one_pass_are_exact():
             assert np.array_equal(o, want), ("reordered", ip, jp, kp)
     finally:
         reg.close()
+
+
+STAG = [(1, 1, 1), (0, 0, 1), (0, 1, 0), (1, 0, 0)]
+
+
+def csf_chain(call, g, dyinv, q, qn, a):
+    """The surface-tension sequence of one bubble (srcs/main.f90:543-571):
+    normals, curvature and force of q and of qn, summed into sum / sumn."""
+    dims = [I(g.ni), I(g.nj), I(g.nk)]
+    inv = [D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+    grad = [I(0), I(g.nj)] + dims + [D(1 / g.dx), dyinv, D(1 / g.dz)]
+    call("init_q", *dims, *a["sum"])
+    call("init_q", *dims, *a["sumn"])
+    call("cal_grad_p2a", *grad, q, *a["px"])
+    for (ip, jp, kp), o in zip(STAG, a["rn"]):
+        call("calrn", *dims, *inv, I(ip), I(jp), I(kp), *a["px"], o)
+    call("cal_grad_p2a", *grad, qn, *a["px"])
+    for (ip, jp, kp), o in zip(STAG, a["rnn"]):
+        call("calrn", *dims, *inv, I(ip), I(jp), I(kp), *a["px"], o)
+    call("calrkap", *dims, *inv, *a["rn"], a["rkap"])
+    call("calrkap", *dims, *inv, *a["rnn"], a["rkapn"])
+    st = [D(1.0), D(0.001), D(0.07)]
+    call("cal_grad_p2uvw", *grad, q, *a["px"])
+    call("calfst", *dims, *st, *a["px"], a["rkap"], *a["fst"])
+    call("cal_grad_p2uvw", *grad, qn, *a["px"])
+    call("calfst", *dims, *st, *a["px"], a["rkapn"], *a["fstn"])
+    call("sum_fst", *dims, *a["fst"], *a["sum"])
+    call("sum_fst", *dims, *a["fstn"], *a["sumn"])
+
+
+def chain_arrays(g, make):
+    return {"px": [make(None) for _ in range(3)], "rn": [make(3) for _ in range(4)],
+            "rnn": [make(3) for _ in range(4)], "rkap": make(3), "rkapn": make(3),
+            "fst": [make(None) for _ in range(3)], "fstn": [make(None) for _ in range(3)],
+            "sum": [make(None) for _ in range(3)], "sumn": [make(None) for _ in range(3)]}
+
+
+def flat(a):
+    out = []
+    for k in sorted(a):
+        out += a[k] if isinstance(a[k], list) else [a[k]]
+    return out
+
+
+def test_surface_tension_chain_of_the_copied_phase_is_reused_exactly():
+    """Step n+1 repeats step n's chain of phin on phi = copy of phin (cpy_ of
+    slice 1): the normals, the curvature and the force are taken over from
+    the arrays of step n (6 calls answered without a kernel), and every
+    array equals the oracle's chain bit for bit."""
+    g = Grid(24, 12, 16, 0.1, 0.05, 0.1)
+    r = np.random.default_rng(43)
+    reg = Registered()
+    try:
+        phi = reg.add(np.stack([H.smooth_phi(g, r), H.smooth_phi(g, r)]))
+        phin = reg.add(np.stack([H.smooth_phi(g, r), H.smooth_phi(g, r)]))
+        nxt = reg.add(H.smooth_phi(g, r))
+        a = chain_arrays(g, lambda m: reg.add(g.zeros(m)))
+        dyinv = np.full(g.nj + 6, 1 / g.dy)
+        dims = [I(g.ni), I(g.nj), I(g.nk)]
+        host_phi, host_phin, host_nxt = phi[1].copy(), phin[1].copy(), nxt.copy()
+        csf_chain(H.call_gpu, g, dyinv, phi[1], phin[1], a)
+        # the step's end: phi(:,:,:,1) <- phin(:,:,:,1); a new phin
+        H.call_gpu("cpy", *dims, phin[1], phi[1])
+        H.call_gpu("cpy", *dims, nxt, phin[1])
+        H.gpu().felgpu_result_reuse_hits_.restype = ctypes.c_int64
+        hits0 = H.gpu().felgpu_result_reuse_hits_()
+        csf_chain(H.call_gpu, g, dyinv, phi[1], phin[1], a)
+        hits = H.gpu().felgpu_result_reuse_hits_() - hits0
+        for x in flat(a):
+            reg.download(x)
+        want = chain_arrays(g, lambda m: g.zeros(m))
+        csf_chain(H.call_oracle, g, dyinv, host_phi, host_phin, want)
+        csf_chain(H.call_oracle, g, dyinv, host_phin, host_nxt, want)
+        for k in sorted(a):
+            for m, (x, y) in enumerate(zip(a[k] if isinstance(a[k], list) else [a[k]],
+                                           want[k] if isinstance(want[k], list) else [want[k]])):
+                assert np.array_equal(x, y), (k, m)
+        assert hits == 6, hits
+    finally:
+        reg.close()
+
+
+RUN_FIELDS = ["u", "v", "w", "p", "uo", "vo", "wo", "po", "phi0", "phi1", "phin1", "rho", "rmu",
+              "rhon", "rmun", "src_u", "src_v", "src_w", "adv_u", "adv_uo", "vis_u", "fst_u",
+              "fst_un", "fst_w", "sum_fst_u", "sum_fst_vn", "sum_fst_w", "phix", "phiz", "rn_pp",
+              "rn_13", "rnn_23", "rkap", "rkapn", "dp", "div"]
+
+
+@pytest.mark.parametrize("precision,tangent", [(64, False), (32, False), (64, True)],
+                         ids=["fp64", "fp32", "tangent"])
+def test_runs_with_and_without_reuse_are_bitwise_equal(precision, tangent):
+    """Every array of the run after several steps is the same with result
+    reuse on (parked results, the memo of the surface-tension chain) as with
+    every result recomputed, and reuse is actually exercised."""
+    from felmpf import config as C
+    from felmpf import sensitivity as S
+    from felmpf.driver import GpuRun
+    cfg = C.droplet_impact().scaled(32, 16, 32)
+    got = {}
+    for reuse in (False, True):
+        run = GpuRun(cfg, precision=precision, tangent=tangent, reuse=reuse)
+        try:
+            if tangent:  # derivative along the bottom wall's contact angle
+                for name, dirn in S.parameter_seeds(cfg, slip=False)[0].items():
+                    run.seed(name, dirn)
+            h0 = run.gpu.felgpu_result_reuse_hits_()
+            run.steps(4)
+            hits = run.gpu.felgpu_result_reuse_hits_() - h0
+            got[reuse] = {f: run.field(f) for f in RUN_FIELDS}
+            if tangent:
+                got[reuse].update({"d_" + f: run.tangent(f) for f in ("u", "w", "phi1")})
+        finally:
+            run.close()
+        assert (hits > 0) == reuse, (reuse, hits)
+    for f in got[False]:
+        assert np.array_equal(got[False][f], got[True][f]), f
