@@ -245,6 +245,21 @@ enum MemoOp {
 // until the source is accessed or the destination changes.  Returns false
 // (nothing done) for any other pair of arrays.
 bool deferred_copy(const void* src, const void* dst, size_t count);
+// init_q_ on a whole registered array: its contents become zeros without a
+// fill; the fill is made when the array is accessed (a whole overwrite needs
+// none).  False (nothing done) for any other array.
+bool defer_zero(const void* host, size_t count);
+// sum_fst_ of src[m] onto dst[m] (m < narr) whose contents are deferred
+// zero fills: each dst's contents become 0.0 + src element by element, made
+// when dst is accessed or before src changes.  All or nothing: false
+// (nothing done) unless every pair is two distinct whole registered arrays
+// of one element type, dst a deferred zero, and no dst is another pair's
+// array.
+bool defer_add_zero(const void* const* dst, const void* const* src, int narr, size_t count);
+// The device buffer a deferred 0.0 + src array is made from (src's
+// contents), or null when `host` is not such an array; a consumer may read
+// it and add 0.0 itself.
+const void* deferred_add_zero_source(const void* host, size_t count);
 // Replaces the device buffer behind a whole registered array (same size and
 // element type) and returns the previous one; the caller owns it.
 void* exchange_device_buffer(const void* host, size_t count, void* buf);

@@ -26,6 +26,9 @@ struct SrcArgs {
   const T *prs_u, *prs_v, *prs_w, *vis_u, *vis_v, *vis_w;
   const T *fst_u, *fst_v, *fst_w, *fst_un, *fst_vn, *fst_wn;
   T *src_u, *src_v, *src_w;
+  // bit m set: the m-th force sum (fst_u, fst_v, fst_w, fst_un, fst_vn,
+  // fst_wn) points at the force it is 0.0 + of, element by element
+  int addz;
 };
 
 template <class T>

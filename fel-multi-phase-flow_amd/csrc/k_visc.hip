@@ -122,20 +122,21 @@ void srcu(const Dims d, int jen, double dtinv_, double grvb_, double grvp_, SrcA
   const T q = T(0.25), h = T(0.5), t15 = T(1.5);
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    auto sum = [&](const T* f, int bit) { return (a.addz & bit) ? T(0.0) + f[c] : f[c]; };
     {
       const T r = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q;
       a.src_u[c] = +a.u[c] * dtinv - a.adv_u[c] * t15 + a.adv_uo[c] * h - a.prs_u[c] / r +
-                   a.vis_u[c] / r * h + a.fst_u[c] * h + a.fst_un[c] * h + grvb;
+                   a.vis_u[c] / r * h + sum(a.fst_u, 1) * h + sum(a.fst_un, 8) * h + grvb;
     }
     if (j <= jen) {
       const T r = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q;
       a.src_v[c] = +a.v[c] * dtinv - a.adv_v[c] * t15 + a.adv_vo[c] * h - a.prs_v[c] / r +
-                   a.vis_v[c] / r * h + a.fst_v[c] * h + a.fst_vn[c] * h + grvp;
+                   a.vis_v[c] / r * h + sum(a.fst_v, 2) * h + sum(a.fst_vn, 16) * h + grvp;
     }
     {
       const T r = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q;
       a.src_w[c] = +a.w[c] * dtinv - a.adv_w[c] * t15 + a.adv_wo[c] * h - a.prs_w[c] / r +
-                   a.vis_w[c] / r * h + a.fst_w[c] * h + a.fst_wn[c] * h;
+                   a.vis_w[c] / r * h + sum(a.fst_w, 4) * h + sum(a.fst_wn, 32) * h;
     }
   });
 }
@@ -247,8 +248,19 @@ void cal_srcu_t(const Dims d, int jen, double dt, double grvb, double grvp, cons
   Call c("cal_srcu_");
   const int n = d.n3;
   const T* p[23];
-  for (int m = 0; m < 23; m++) p[m] = c.in_t<T>(in[m], n, "cal_srcu_ input");
+  int addz = 0;
+  for (int m = 0; m < 23; m++) {
+    // a force sum that is a deferred 0.0 + force: read the force instead
+    const void* f = m >= 17 ? deferred_add_zero_source(in[m], n) : nullptr;
+    if (f) {
+      p[m] = static_cast<const T*>(f);
+      addz |= 1 << (m - 17);
+    } else {
+      p[m] = c.in_t<T>(in[m], n, "cal_srcu_ input");
+    }
+  }
   SrcArgs<T> a;
+  a.addz = addz;
   a.u = p[0]; a.v = p[1]; a.w = p[2]; a.rho = p[3]; a.rhon = p[4];
   a.adv_u = p[5]; a.adv_v = p[6]; a.adv_w = p[7]; a.adv_uo = p[8]; a.adv_vo = p[9]; a.adv_wo = p[10];
   a.prs_u = p[11]; a.prs_v = p[12]; a.prs_w = p[13]; a.vis_u = p[14]; a.vis_v = p[15]; a.vis_w = p[16];
@@ -316,15 +328,14 @@ template <class T>
 void init_q_t(const Dims d, double* q1, double* q2, double* q3) {
   const size_t n = (size_t)d.n3;
   double* q[3] = {q1, q2, q3};
-  MemoKey k(kOpZero);
-  k.num((int64_t)sizeof(T)).num((int64_t)n);
   for (int m = 0; m < 3; m++) {
-    if (memo_reuse(k, {{q[m], n}}, false)) continue;  // already zero
+    // device-resident: a deferred fill (the accumulation of sum_fst_ that
+    // follows in the timestep usually makes it unnecessary)
+    if (defer_zero(q[m], n)) continue;
     Call c("init_q_");
     T* p = c.all_t<T>(q[m], n, "q");
     FEL_CHECK(hipMemsetAsync(p, 0, sizeof(T) * n, stream()));
     c.finish();
-    memo_record(k, {{q[m], n}});
   }
 }
 
@@ -332,11 +343,11 @@ template <class T>
 void sum_fst_t(const Dims d, const double* fst_u, const double* fst_v, const double* fst_w,
                double* sum_u, double* sum_v, double* sum_w) {
   const size_t n = (size_t)d.n3;
-  MemoKey k(kOpSumFst);
-  k.num((int64_t)sizeof(T)).dims(d);
-  k.content(fst_u, n).content(fst_v, n).content(fst_w, n);
-  k.content(sum_u, n).content(sum_v, n).content(sum_w, n);
-  if (memo_reuse(k, {{sum_u, n}, {sum_v, n}, {sum_w, n}}, true)) return;
+  // sums still holding a deferred zero fill become deferred 0.0 + fst
+  // (srcs/sum_fst.f:28-30 on zeros), read as such by cal_srcu_
+  const void* f[3] = {fst_u, fst_v, fst_w};
+  const void* s[3] = {sum_u, sum_v, sum_w};
+  if (defer_add_zero(s, f, 3, n)) return;
   Call c("sum_fst_");
   const T *fu = c.in_t<T>(fst_u, d.n3, "fst_u"), *fv = c.in_t<T>(fst_v, d.n3, "fst_v"),
           *fw = c.in_t<T>(fst_w, d.n3, "fst_w");
@@ -348,7 +359,6 @@ void sum_fst_t(const Dims d, const double* fst_u, const double* fst_v, const dou
     sw[m] = sw[m] + fw[m];
   });
   c.finish();
-  memo_record(k, {{sum_u, n}, {sum_v, n}, {sum_w, n}});
 }
 
 }  // namespace fel

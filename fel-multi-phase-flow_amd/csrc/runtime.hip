@@ -33,6 +33,11 @@ struct Range {
   // them in its buffer); ndeps counts the ranges deferred on this one
   struct Range* lazy;
   int ndeps;
+  // what the deferred contents are: kCopy (lazy's contents) or kAddZero
+  // (0.0 + lazy's contents, element by element: sum_fst_ onto zeros)
+  int lkind = 0;
+  // deferred zero fill (init_q_): the contents are zeros, the buffer is not
+  bool zero = false;
   // content number of the elements outside the box (-2..n+2)^3 of a padded
   // array (0: none assigned yet); kept by the entry points that write
   // exactly that box, renewed by every other write
@@ -127,11 +132,40 @@ void stream_copy(void* dst, const void* src, size_t bytes) {
   FEL_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, R().s));
 }
 
+constexpr int kCopy = 0, kAddZero = 1;
+
+template <class T>
+__global__ void __launch_bounds__(256) add_zero_kernel(T* dst, const T* src, size_t n) {
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) dst[m] = T(0.0) + src[m];
+}
+
+void add_zero(void* dst, const void* src, size_t n, int esize) {
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipStream_t s = R().s;
+  if (esize == 16)
+    add_zero_kernel<Dual><<<nb, 256, 0, s>>>(static_cast<Dual*>(dst), static_cast<const Dual*>(src), n);
+  else if (esize == 4)
+    add_zero_kernel<float><<<nb, 256, 0, s>>>(static_cast<float*>(dst), static_cast<const float*>(src), n);
+  else
+    add_zero_kernel<double><<<nb, 256, 0, s>>>(static_cast<double*>(dst), static_cast<const double*>(src), n);
+  FEL_CHECK(hipGetLastError());
+}
+
 void materialize(Range& g) {
+  if (g.zero) {
+    FEL_CHECK(hipMemsetAsync(g.dev, 0, g.bytes / sizeof(double) * (size_t)g.esize, R().s));
+    g.zero = false;
+  }
   if (!g.lazy) return;
-  stream_copy(g.dev, g.lazy->dev, g.bytes / sizeof(double) * (size_t)g.esize);
+  const size_t n = g.bytes / sizeof(double);
+  if (g.lkind == kAddZero)
+    add_zero(g.dev, g.lazy->dev, n, g.esize);
+  else
+    stream_copy(g.dev, g.lazy->dev, n * (size_t)g.esize);
   g.lazy->ndeps--;
   g.lazy = nullptr;
+  g.lkind = kCopy;
 }
 
 void materialize_dependents(Range& g) {
@@ -140,10 +174,14 @@ void materialize_dependents(Range& g) {
     if (kv.second.lazy == &g) materialize(kv.second);
 }
 
+// The contents are about to be replaced as a whole: nothing deferred needs
+// to be made.
 void drop_lazy(Range& g) {
+  g.zero = false;
   if (!g.lazy) return;
   g.lazy->ndeps--;
   g.lazy = nullptr;
+  g.lkind = kCopy;
 }
 
 // Everything that reads or writes the device copy of g directly.
@@ -342,7 +380,9 @@ bool deferred_copy(const void* src_host, const void* dst_host, size_t count) {
   if (!src || !dst || src == dst || src->esize != dst->esize) return false;
   deferred_coef_check(src_host, true);
   deferred_coef_check(dst_host, true);
-  if (src->lazy == dst) return true;  // the contents are already equal
+  if (src->lazy == dst && src->lkind == kCopy) return true;  // the contents are already equal
+  // a source whose contents are not its holder's buffer as it is: make them
+  if (src->zero || (src->lazy && src->lkind != kCopy)) materialize(*src);
   materialize_dependents(*dst);       // the destination's contents change
   drop_lazy(*dst);
   Range* holder = src->lazy ? src->lazy : src;
@@ -355,14 +395,62 @@ bool deferred_copy(const void* src_host, const void* dst_host, size_t count) {
     }
   holder->ndeps = 0;
   holder->lazy = dst;
+  holder->lkind = kCopy;
   dst->ndeps++;
   return true;
+}
+
+bool defer_zero(const void* host, size_t count) {
+  Range* g = whole_range(host, count);
+  if (!g) return false;
+  deferred_coef_check(host, true);
+  materialize_dependents(*g);
+  drop_lazy(*g);
+  g->zero = true;
+  note_write(*g);
+  return true;
+}
+
+bool defer_add_zero(const void* const* dst_host, const void* const* src_host, int narr,
+                    size_t count) {
+  for (int m = 0; m < narr; m++) {
+    Range* dst = whole_range(dst_host[m], count);
+    Range* src = whole_range(src_host[m], count);
+    if (!dst || !src || dst == src || !dst->zero || dst->ndeps != 0 || dst->esize != src->esize)
+      return false;
+    for (int q = 0; q < narr; q++)
+      if (q != m && (dst_host[q] == dst_host[m] || src_host[q] == dst_host[m])) return false;
+  }
+  for (int m = 0; m < narr; m++) {
+    Range* dst = whole_range(dst_host[m], count);
+    Range* src = whole_range(src_host[m], count);
+    deferred_coef_check(src_host[m], false);
+    if (src->zero || (src->lazy && src->lkind != kCopy)) materialize(*src);
+    Range* holder = src->lazy ? src->lazy : src;
+    dst->zero = false;
+    dst->lazy = holder;
+    dst->lkind = kAddZero;
+    holder->ndeps++;
+    note_write(*dst);
+  }
+  return true;
+}
+
+const void* deferred_add_zero_source(const void* host, size_t count) {
+  Range* g = whole_range(host, count);
+  if (!g || !g->lazy || g->lkind != kAddZero) return nullptr;
+  return g->lazy->dev;
 }
 
 void deferred_coef_flush() {
   DeferredCoef& dc = deferred_coef();
   if (!dc.active) return;
   dc.active = false;
+  for (int m = 0; m < 45; m++)  // every element is rewritten
+    if (Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(dc.out_host[m]), 1)) {
+      materialize_dependents(*g);
+      drop_lazy(*g);
+    }
   coef_vis_any(dc.d, dc.dxinv, dc.dyinv, dc.dzinv, dc.src_dev, dc.out_dev, dc.esize);
   for (int m = 0; m < 45; m++)
     if (Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(dc.out_host[m]), 1))

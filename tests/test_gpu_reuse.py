@@ -22,6 +22,13 @@ from felharness import D, I, Grid
 
 pytestmark = pytest.mark.gpu
 
+NID = np.full(6, -1, dtype=np.int32)
+
+
+def same_bits(a, b):
+    """Bitwise equality (tells -0.0 from +0.0)."""
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
 
 class Registered:
     """Registers numpy arrays with the library for the duration of a test."""
@@ -250,7 +257,7 @@ def test_surface_tension_chain_of_the_copied_phase_is_reused_exactly():
         for k in sorted(a):
             for m, (x, y) in enumerate(zip(a[k] if isinstance(a[k], list) else [a[k]],
                                            want[k] if isinstance(want[k], list) else [want[k]])):
-                assert np.array_equal(x, y), (k, m)
+                assert same_bits(x, y), (k, m)
         assert hits == 6, hits
     finally:
         reg.close()
@@ -289,4 +296,56 @@ def test_runs_with_and_without_reuse_are_bitwise_equal(precision, tangent):
             run.close()
         assert (hits > 0) == reuse, (reuse, hits)
     for f in got[False]:
-        assert np.array_equal(got[False][f], got[True][f]), f
+        assert same_bits(got[False][f], got[True][f]), f
+
+
+def test_deferred_zero_and_force_sums_are_exact():
+    """init_q_ on device-resident arrays defers the fill, and sum_fst_ onto
+    such zeros defers 0.0 + force (srcs/sum_fst.f:28-30); cal_srcu_ reads
+    the forces in their place.  Every observable result equals the oracle's
+    sequence bit for bit, including -0.0 forces (0.0 + -0.0 = +0.0), a
+    force overwritten after the deferral, and an accumulation onto a
+    deferred sum."""
+    g = Grid(16, 12, 10, 0.1, 0.05, 0.1)
+    r = np.random.default_rng(44)
+    reg = Registered()
+    try:
+        base = [r.standard_normal(g.shape) for _ in range(3)] + \
+            [r.uniform(0.5, 2.0, g.shape) for _ in range(2)] + \
+            [r.standard_normal(g.shape) for _ in range(12)]
+        fst = [r.standard_normal(g.shape) for _ in range(6)]
+        for f in fst:
+            f[r.random(g.shape) < 0.05] = -0.0
+        host_fst = [f.copy() for f in fst]
+        dev_base = [reg.add(a.copy()) for a in base]
+        dev_fst = [reg.add(f) for f in fst]
+        dev_sum = [reg.add(r.standard_normal(g.shape)) for _ in range(6)]
+        dev_src = [reg.add(g.zeros()) for _ in range(3)]
+        dims = [I(g.ni), I(g.nj), I(g.nk)]
+        head = [NID.copy(), *dims, D(0.9), D(0.003), D(0.1), D(-0.2)]
+        H.call_gpu("init_q", *dims, *dev_sum[:3])
+        H.call_gpu("init_q", *dims, *dev_sum[3:])
+        H.call_gpu("sum_fst", *dims, *dev_fst[:3], *dev_sum[:3])
+        H.call_gpu("sum_fst", *dims, *dev_fst[3:], *dev_sum[3:])
+        H.call_gpu("cal_srcu", *head, *dev_base, *dev_sum, *dev_src)
+        want_sum = [g.zeros() for _ in range(6)]
+        H.call_oracle("sum_fst", *dims, *[f.copy() for f in host_fst[:3]], *want_sum[:3])
+        H.call_oracle("sum_fst", *dims, *[f.copy() for f in host_fst[3:]], *want_sum[3:])
+        want_src = [g.zeros() for _ in range(3)]
+        H.call_oracle("cal_srcu", *head, *[a.copy() for a in base], *want_sum, *want_src)
+        for x, y in zip(dev_src, want_src):
+            assert same_bits(reg.download(x), y)
+        assert not np.any(np.signbit(want_sum[0]) & (want_sum[0] == 0.0))  # -0.0 became +0.0
+        # a force rewritten while its sum is deferred: the sum keeps 0.0 + the old force
+        H.call_gpu("cpy", *dims, dev_base[5], dev_fst[0])
+        assert same_bits(reg.download(dev_sum[0]), want_sum[0])
+        # a second accumulation onto a deferred sum (two bubbles)
+        H.call_gpu("sum_fst", *dims, *dev_fst[3:], *dev_sum[3:])
+        H.call_oracle("sum_fst", *dims, *[f.copy() for f in host_fst[3:]], *want_sum[3:])
+        for x, y in zip(dev_sum[1:], want_sum[1:]):
+            assert same_bits(reg.download(x), y)
+        # a deferred zero that is read
+        H.call_gpu("init_q", *dims, *dev_sum[:3])
+        assert not np.any(reg.download(dev_sum[1]))
+    finally:
+        reg.close()
