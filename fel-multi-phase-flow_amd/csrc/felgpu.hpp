@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <vector>
 
@@ -292,6 +293,26 @@ void deferred_coef_check(const void* host, bool write);  // flush if host confli
 // The 45 coefficient arrays from the viscosity, in the element type esize.
 void coef_vis_any(const Dims d, double dxinv, double dyinv, double dzinv, const void* rmu,
                   void* const* out45, int esize);
+
+// A deferred chain of entry points: consecutive calls of the reference's
+// step whose intermediate arrays the next call consumes (calsij_ ->
+// cal_arith_tau_ -> cal_div_tensor_) are recorded instead of run, and the
+// last one runs the whole chain in one kernel that writes every output of
+// every call.  `outs` are the arrays whose contents are pending, `srcs` the
+// arrays the pending contents are computed from; any access to an output,
+// or a write to a source, through any entry point runs `flush` first (the
+// chain's own kernels, writing what the calls so far would have written).
+// One chain is pending at a time.  FELGPU_EAGER_CHAINS=1 turns this off.
+struct DeferredChain {
+  bool active = false;
+  int kind = 0;
+  std::vector<const void*> outs, srcs;  // host base addresses of whole registered arrays
+  std::function<void()> flush;
+};
+DeferredChain& deferred_chain();
+bool chain_deferral_enabled();
+void deferred_chain_flush();
+void deferred_chain_check(const void* host, bool write);
 
 // Pinned host scratch for small device->host read-backs (reductions).
 double* pinned_scalars();  // 64 doubles

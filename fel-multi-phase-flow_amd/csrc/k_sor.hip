@@ -107,6 +107,10 @@ __device__ __forceinline__ void write_images(const Dims& d, const SorArgs<T>& a,
 // complement in every index.
 static const int kIs[4] = {1, 1, 1, 2}, kJs[4] = {1, 1, 2, 1}, kKs[4] = {1, 2, 1, 1};
 
+// Rows of one parity per block (8 measured no faster: the neighbouring
+// rows a block re-reads at its edges come from L2).
+static int sor_rows() { return 4; }
+
 // k planes per block: one block sums du^2 over kKB planes, which keeps the
 // number of partial sums (and the final reduction) small.
 constexpr int kKB = 1;
@@ -117,7 +121,7 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
                                               double* part);
 
 template <int COMP, bool ONFLY, bool FUSE, class T>
-__global__ void __launch_bounds__(256) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
+__global__ void __launch_bounds__(512) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
                                                        int jend, int gx, int gy, int ntiles,
                                                        double* part) {
   sor_pair_body<COMP, ONFLY, FUSE, T>(d, a, is, js, ks, jend, gx, gy, ntiles, part);
@@ -135,7 +139,7 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
   if (k > d.nk) break;
   const int q = (k - ks) & 1;
   const int i = 1 + ((is + q - 1) & 1) + 2 * (bx * 64 + (int)threadIdx.x);
-  const int j = 1 + ((js + q - 1) & 1) + 2 * (by * 4 + (int)threadIdx.y);
+  const int j = 1 + ((js + q - 1) & 1) + 2 * (by * (int)blockDim.y + (int)threadIdx.y);
   if (i <= d.ni && j <= jend) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
     const int c = d.ix(i, j, k);
@@ -191,11 +195,11 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
   }
   }
   // block sum of du^2 in a fixed order
-  __shared__ double sh[256];
+  __shared__ double sh[512];
   const int t = (int)(threadIdx.y * 64 + threadIdx.x);
   sh[t] = e;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = (int)(blockDim.y * 32); s > 0; s >>= 1) {
     if (t < s) sh[t] += sh[t + s];
     __syncthreads();
   }
@@ -262,7 +266,8 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
                const P* l1, const P* l2, const P* phi, int ndiv) {
   const int jen = hi ? d.nj - 1 : d.nj;
   const int gx = ((d.ni + 1) / 2 + 63) / 64;  // cells of one parity per row
-  const int gy = ((d.nj + 1) / 2 + 3) / 4;    // rows of one parity per plane
+  const int rows = sor_rows();                // rows of one parity per block
+  const int gy = ((d.nj + 1) / 2 + rows - 1) / rows;
   const int gz = (d.nk + kKB - 1) / kKB;
   const int nb = gx * gy * gz;
   double* part = static_cast<double*>(scratch(3, sizeof(double) * ((size_t)nb * 12 + kErrChunks)));
@@ -309,7 +314,7 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
     for (int comp = 0; comp < 3; comp++) {
       const int jend = comp == 1 ? jen : d.nj;
       for (int p = 0; p < 4; p++, slot++) {
-        dim3 grid(nb), block(64, 4, 1);
+        dim3 grid(nb), block(64, rows, 1);
         double* pp = part + ps.off[slot];
         // Pairs after the first write the swept component's ghost images
         // themselves (write_images); the refresh after the first pair also

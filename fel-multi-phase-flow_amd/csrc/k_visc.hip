@@ -115,6 +115,79 @@ void div_tensor(const Dims d, double dxinv_, double dyinv_, double dzinv_, const
   });
 }
 
+// calsij followed by arith_tau in one pass (the chain calsij_ ->
+// cal_arith_tau_ recorded by the runtime): every thread computes s at its
+// cell, writes it where calsij writes, and writes tau where arith_tau
+// writes, from that s and the viscosity -- the same operations in the same
+// order, so the values are bitwise those of the two kernels, without the
+// round trip of s through memory.  Where calsij leaves s untouched, tau is
+// computed from the untouched contents (a.s[c] is never written there, so
+// the read does not race).
+template <class T>
+struct SijTau {
+  const T *u, *v, *w, *rmu;
+  T *s, *tau;
+  T dxinv, dyinv, dzinv;
+};
+
+template <class T>
+__device__ __forceinline__ T sij_at(const Dims& d, const SijTau<T>& a, int m, int i, int j, int k,
+                                    int c, bool& written) {
+  const int n3 = d.n3, X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const T *u = a.u, *v = a.v, *w = a.w;
+  switch (m) {
+    case 0:
+      written = i >= -1;
+      return written ? (-u[c - X] + u[c]) * a.dxinv : a.s[c];
+    case 1:
+      written = j >= -1;
+      return written ? (-v[c - Y] + v[c]) * a.dyinv : a.s[n3 + c];
+    case 2:
+      written = i >= -1 && j >= -1 && k >= -1;
+      return written ? (-w[c - Z] + w[c]) * a.dzinv : a.s[2 * n3 + c];
+    case 3:
+      written = i <= d.ni + 2 && j <= d.nj + 2;
+      return written ? (+(-u[c] + u[c + Y]) * a.dyinv + (-v[c] + v[c + X]) * a.dxinv) * T(0.5)
+                     : a.s[3 * n3 + c];
+    case 4:
+      written = i <= d.ni + 2 && k <= d.nk + 2;
+      return written ? (+(-u[c] + u[c + Z]) * a.dzinv + (-w[c] + w[c + X]) * a.dxinv) * T(0.5)
+                     : a.s[4 * n3 + c];
+    default:
+      written = j <= d.nj + 2 && k <= d.nk + 2;
+      return written ? ((-v[c] + v[c + Z]) * a.dzinv + (-w[c] + w[c + Y]) * a.dyinv) * T(0.5)
+                     : a.s[5 * n3 + c];
+  }
+}
+
+template <class T>
+void sij_tau(const Dims d, SijTau<T> a) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int n3 = d.n3, c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    const T* rmu = a.rmu;
+    const T r = rmu[c], q = T(0.25);
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      bool sw;
+      const T sv = sij_at<T>(d, a, m, i, j, k, c, sw);
+      if (sw) a.s[(size_t)m * n3 + c] = sv;
+      if (m < 3) {
+        a.tau[(size_t)m * n3 + c] = sv * r * T(2.0);
+      } else {
+        const bool tw = m == 3 ? (i <= ni + 2 && j <= nj + 2)
+                        : m == 4 ? (i <= ni + 2 && k <= nk + 2)
+                                 : (j <= nj + 2 && k <= nk + 2);
+        if (tw) {
+          const int o1 = m == 5 ? Y : X, o2 = m == 3 ? Y : Z;
+          const T mu = +q * r + q * rmu[c + o1] + q * rmu[c + o2] + q * rmu[c + o1 + o2];
+          a.tau[(size_t)m * n3 + c] = sv * mu * T(2.0);
+        }
+      }
+    }
+  });
+}
+
 // srcs/cal_srcu.f:59-136
 template <class T>
 void srcu(const Dims d, int jen, double dtinv_, double grvb_, double grvp_, SrcArgs<T> a) {
@@ -208,24 +281,100 @@ void cal_grad_prs_t(const Dims d, double dxinv, double dyinv, double dzinv, cons
   c.finish();
 }
 
+// The viscous-stress chain of the step (srcs/main.f90:537-540): calsij_
+// records s, cal_arith_tau_ records tau, cal_div_tensor_ runs sij_tau and
+// the divergence.  Any other access to s or tau, or a write to u, v, w or the
+// viscosity, in between runs what has been recorded (calsij, then
+// arith_tau) at that point.
+constexpr int kChainVisc = 1;
+
+struct ViscRecord {
+  Dims d;
+  int esize, stage;  // stage 1: s recorded, 2: s and tau recorded
+  double dxinv, dyinv, dzinv;
+  const void *u, *v, *w, *rmu;  // device
+  void *s, *tau;                // device
+  const void* s_host;
+  const void* tau_host;
+};
+static ViscRecord g_visc;
+
+static bool whole_registered(const void* host, size_t count, int esize) {
+  return is_registered(host, count) && content_version(host, count) != 0 && elem_size(host) == esize;
+}
+
+template <class T>
+static void visc_record_flush() {
+  const ViscRecord r = g_visc;
+  if (r.stage == 2) {
+    sij_tau<T>(r.d, SijTau<T>{static_cast<const T*>(r.u), static_cast<const T*>(r.v),
+                              static_cast<const T*>(r.w), static_cast<const T*>(r.rmu),
+                              static_cast<T*>(r.s), static_cast<T*>(r.tau), T(r.dxinv),
+                              T(r.dyinv), T(r.dzinv)});
+    return;
+  }
+  calsij<T>(r.d, r.dxinv, r.dyinv, r.dzinv, static_cast<const T*>(r.u), static_cast<const T*>(r.v),
+            static_cast<const T*>(r.w), static_cast<T*>(r.s));
+}
+
 template <class T>
 void calsij_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* uk,
               const double* vk, const double* wk, double* sk) {
+  const size_t n = (size_t)d.n3, n6 = 6 * n;
+  const int es = (int)sizeof(T);
+  const bool defer = chain_deferral_enabled() && whole_registered(uk, n, es) &&
+                     whole_registered(vk, n, es) && whole_registered(wk, n, es) &&
+                     whole_registered(sk, n6, es);
   Call c("calsij_");
-  const T* u = c.in_t<T>(uk, d.n3, "uk");
-  const T* v = c.in_t<T>(vk, d.n3, "vk");
-  const T* w = c.in_t<T>(wk, d.n3, "wk");
-  T* s = c.io_t<T>(sk, 6 * (size_t)d.n3, "sk");
+  const T* u = c.in_t<T>(uk, n, "uk");
+  const T* v = c.in_t<T>(vk, n, "vk");
+  const T* w = c.in_t<T>(wk, n, "wk");
+  T* s = c.io_t<T>(sk, n6, "sk");
+  if (defer && sk != uk && sk != vk && sk != wk) {
+    deferred_chain_flush();
+    g_visc = ViscRecord{d, es, 1, dxinv, dyinv, dzinv, u, v, w, nullptr, s, nullptr, sk, nullptr};
+    DeferredChain& ch = deferred_chain();
+    ch.active = true;
+    ch.kind = kChainVisc;
+    ch.outs = {sk};
+    ch.srcs = {uk, vk, wk};
+    ch.flush = [] { visc_record_flush<T>(); };
+    c.finish();
+    return;
+  }
   calsij<T>(d, dxinv, dyinv, dzinv, u, v, w, s);
   c.finish();
 }
 
 template <class T>
 void cal_arith_tau_t(const Dims d, const double* s, const double* rmu, double* tau) {
+  const size_t n = (size_t)d.n3, n6 = 6 * n;
+  DeferredChain& ch = deferred_chain();
+  const bool chained = ch.active && ch.kind == kChainVisc && g_visc.stage == 1 &&
+                       g_visc.s_host == s && g_visc.esize == (int)sizeof(T) &&
+                       g_visc.d.ni == d.ni && g_visc.d.nj == d.nj && g_visc.d.nk == d.nk &&
+                       whole_registered(rmu, n, (int)sizeof(T)) &&
+                       whole_registered(tau, n6, (int)sizeof(T));
+  if (chained) {
+    // an argument that is one of the recorded arrays runs the chain here
+    Call c("cal_arith_tau_");
+    const T* dr = c.in_t<T>(rmu, n, "rmu");
+    T* dt = c.io_t<T>(tau, n6, "tau");
+    c.finish();
+    if (ch.active && ch.kind == kChainVisc && g_visc.stage == 1) {
+      g_visc.stage = 2;
+      g_visc.rmu = dr;
+      g_visc.tau = dt;
+      g_visc.tau_host = tau;
+      ch.outs = {s, tau};
+      ch.srcs.push_back(rmu);
+      return;
+    }
+  }
   Call c("cal_arith_tau_");
-  const T* ds = c.in_t<T>(s, 6 * (size_t)d.n3, "s");
-  const T* dr = c.in_t<T>(rmu, d.n3, "rmu");
-  T* dt = c.io_t<T>(tau, 6 * (size_t)d.n3, "tau");
+  const T* ds = c.in_t<T>(s, n6, "s");
+  const T* dr = c.in_t<T>(rmu, n, "rmu");
+  T* dt = c.io_t<T>(tau, n6, "tau");
   arith_tau<T>(d, ds, dr, dt);
   c.finish();
 }
@@ -233,6 +382,30 @@ void cal_arith_tau_t(const Dims d, const double* s, const double* rmu, double* t
 template <class T>
 void cal_div_tensor_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* tauk,
                       double* uk, double* vk, double* wk) {
+  DeferredChain& ch = deferred_chain();
+  if (ch.active && ch.kind == kChainVisc && g_visc.stage == 2 && g_visc.tau_host == tauk &&
+      g_visc.esize == (int)sizeof(T) && g_visc.dxinv == dxinv && g_visc.dyinv == dyinv &&
+      g_visc.dzinv == dzinv && g_visc.d.ni == d.ni && g_visc.d.nj == d.nj && g_visc.d.nk == d.nk) {
+    Call c("cal_div_tensor_");
+    T* u = c.out_t<T>(uk, d.n3, "uk");  // a recorded array among these runs the chain first
+    T* v = c.out_t<T>(vk, d.n3, "vk");
+    T* w = c.out_t<T>(wk, d.n3, "wk");
+    if (ch.active) {
+      ch.active = false;
+      ch.flush = nullptr;
+      ch.outs.clear();
+      ch.srcs.clear();
+      const ViscRecord& r = g_visc;
+      sij_tau<T>(d, SijTau<T>{static_cast<const T*>(r.u), static_cast<const T*>(r.v),
+                              static_cast<const T*>(r.w), static_cast<const T*>(r.rmu),
+                              static_cast<T*>(r.s), static_cast<T*>(r.tau), T(r.dxinv),
+                              T(r.dyinv), T(r.dzinv)});
+      div_tensor<T>(d, dxinv, dyinv, dzinv, static_cast<const T*>(r.tau), u, v, w);
+      c.finish();
+      return;
+    }
+    c.finish();
+  }
   Call c("cal_div_tensor_");
   const T* t = c.in_t<T>(tauk, 6 * (size_t)d.n3, "tauk");
   T* u = c.out_t<T>(uk, d.n3, "uk");

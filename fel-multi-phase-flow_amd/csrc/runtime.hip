@@ -425,6 +425,7 @@ bool defer_add_zero(const void* const* dst_host, const void* const* src_host, in
     Range* dst = whole_range(dst_host[m], count);
     Range* src = whole_range(src_host[m], count);
     deferred_coef_check(src_host[m], false);
+    deferred_coef_check(dst_host[m], true);
     if (src->zero || (src->lazy && src->lkind != kCopy)) materialize(*src);
     Range* holder = src->lazy ? src->lazy : src;
     dst->zero = false;
@@ -439,6 +440,7 @@ bool defer_add_zero(const void* const* dst_host, const void* const* src_host, in
 const void* deferred_add_zero_source(const void* host, size_t count) {
   Range* g = whole_range(host, count);
   if (!g || !g->lazy || g->lkind != kAddZero) return nullptr;
+  deferred_coef_check(reinterpret_cast<const void*>(g->lazy->host), false);
   return g->lazy->dev;
 }
 
@@ -625,6 +627,7 @@ void* exchange_device_buffer(const void* host, size_t count, void* buf) {
     std::fprintf(stderr, "felgpu: internal: buffer exchange on an unregistered range\n");
     std::abort();
   }
+  deferred_coef_check(host, true);
   // the whole contents are replaced
   materialize_dependents(*g);
   drop_lazy(*g);
@@ -633,7 +636,50 @@ void* exchange_device_buffer(const void* host, size_t count, void* buf) {
   return old;
 }
 
+// --------------------------------------------------------------------------
+// Deferred chain of entry points
+// --------------------------------------------------------------------------
+
+DeferredChain& deferred_chain() {
+  static DeferredChain w;
+  return w;
+}
+
+bool chain_deferral_enabled() {
+  static const int on = [] {
+    const char* e = std::getenv("FELGPU_EAGER_CHAINS");
+    return (e && std::atoi(e) != 0) ? 0 : 1;
+  }();
+  return on != 0;
+}
+
+static bool chain_holds(const std::vector<const void*>& v, const Range* g) {
+  for (const void* h : v)
+    if (find_range(R().reg, reinterpret_cast<uintptr_t>(h), 1) == g) return true;
+  return false;
+}
+
+void deferred_chain_flush() {
+  DeferredChain& w = deferred_chain();
+  if (!w.active) return;
+  w.active = false;
+  std::function<void()> f;
+  f.swap(w.flush);
+  w.outs.clear();
+  w.srcs.clear();
+  if (f) f();
+}
+
+void deferred_chain_check(const void* host, bool write) {
+  DeferredChain& w = deferred_chain();
+  if (!w.active) return;
+  const Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(host), 1);
+  if (!g) return;
+  if (chain_holds(w.outs, g) || (write && chain_holds(w.srcs, g))) deferred_chain_flush();
+}
+
 void deferred_coef_check(const void* host, bool write) {
+  deferred_chain_check(host, write);
   DeferredCoef& dc = deferred_coef();
   if (!dc.active) return;
   const uintptr_t p = reinterpret_cast<uintptr_t>(host);
@@ -982,6 +1028,7 @@ void felgpu_unregister_(const void* host) {
 void felgpu_unregister_all_() {
   Runtime& r = R();
   if (!r.init) return;
+  deferred_chain_flush();
   deferred_coef_flush();
   FEL_CHECK(hipStreamSynchronize(r.s));
   weno_release();
@@ -996,6 +1043,7 @@ void felgpu_unregister_all_() {
 // Device address of a registered host address (for callers that launch their
 // own work on the library stream, e.g. the test harness).
 void* felgpu_device_ptr_(const void* host) {
+  deferred_coef_check(host, true);
   Range& g = must_find(host, "felgpu_device_ptr_");
   settle(g, true);  // the caller may read or write it directly
   note_write(g);    // so its contents count as changed
