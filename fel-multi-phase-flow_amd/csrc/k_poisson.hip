@@ -7,8 +7,9 @@
 // srcs/solp_fft_tdma4.f, srcs/bnd_avzero.f, srcs/cal_av.f,
 // srcs/corunp_explicit.f, srcs/cal_coef_prs.f, srcs/cal_coef_solp_fft.f.
 //
-// The transforms are this library's own batched radix-2 Stockham FFTs run in
-// LDS (the reference's srcs/fourn.f is not used): the x lines are
+// The transforms are this library's own batched mixed-radix (radix-4 with a
+// final radix-2 stage) Stockham FFTs run in LDS (the reference's
+// srcs/fourn.f is not used): the x lines are
 // contiguous and one workgroup transforms whole lines; the k lines are
 // strided, so one workgroup transforms a tile of lines that are adjacent in
 // memory, making every global access a contiguous row of the tile.  They
