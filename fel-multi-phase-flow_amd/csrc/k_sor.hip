@@ -28,7 +28,11 @@ namespace fel {
 template <class T>
 struct SorArgs {
   const T *rho, *rhon, *src[3], *rmu;
-  const T* r00[3];  // face densities (rho+rho'+rhon+rhon')/4 of u, v, w, computed once per solve
+  // face densities (rho+rho'+rhon+rhon')/4 of u, v, w, computed once per
+  // solve and stored pair by pair in the launch's thread order (r00_index),
+  // so a launch reads its N/4 values densely; the launch's pointer is
+  // offset to its pair
+  const T* r00[3];
   const T* cf[45];
   T *un, *vn, *wn;
   T dtinv;
@@ -120,6 +124,28 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
                                               int ks, int jend, int gx, int gy, int ntiles,
                                               double* part);
 
+// Dense index of a pair's cell: ii-th cell of its parity in row jj of its
+// parity, plane k (every cell (i,j,k) of a pair has ii = (i-1)/2,
+// jj = (j-1)/2).  One pair's slab holds nk * ceil(nj/2) * ceil(ni/2) values.
+__host__ __device__ __forceinline__ int r00_index(const Dims& d, int ii, int jj, int k) {
+  const int hx = (d.ni + 1) / 2, hy = (d.nj + 1) / 2;
+  return ((k - 1) * hy + jj) * hx + ii;
+}
+__host__ __device__ __forceinline__ size_t r00_slab(const Dims& d) {
+  return (size_t)d.nk * ((d.nj + 1) / 2) * ((d.ni + 1) / 2);
+}
+
+// The colour pair (0..3) that cell (i,j,k) belongs to.
+__host__ __device__ __forceinline__ int pair_of(int i, int j, int k) {
+  const int ci = 2 - (i & 1), cj = 2 - (j & 1), ck = 2 - (k & 1);  // 1 odd, 2 even
+  // pairs (111,222), (112,221), (121,212), (211,122): one parity differs
+  // from the other two, or none does
+  if (ci == cj && cj == ck) return 0;
+  if (ci == cj) return 1;
+  if (ci == ck) return 2;
+  return 3;
+}
+
 template <int COMP, bool ONFLY, bool FUSE, class T>
 __global__ void __launch_bounds__(512) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
                                                        int jend, int gx, int gy, int ntiles,
@@ -154,7 +180,8 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
     }
     const T *un = a.un, *vn = a.vn, *wn = a.wn;
     const T h = T(0.5);
-    const T r00 = a.r00[COMP][c];
+    const T r00 = a.r00[COMP][r00_index(d, bx * 64 + (int)threadIdx.x,
+                                        by * (int)blockDim.y + (int)threadIdx.y, k)];
     if (COMP == 0) {
       const T visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
                            cf[1] * un[c + X] + cf[3] * un[c + Y] + cf[5] * un[c + Z] +
@@ -251,14 +278,17 @@ template <int COMP, class T>
 void launch_pair(bool onfly, bool fuse, dim3 grid, dim3 block, const Dims& d, const SorArgs<T>& a,
                  int p, int jend, int gx, int gy, int nt, double* pp) {
   hipStream_t s = stream();
+  SorArgs<T> a_ = a;  // the face densities of this pair
+  a_.r00[COMP] = a.r00[COMP] + (size_t)p * r00_slab(d);
+  const SorArgs<T>& ap = a_;
   if (onfly && fuse)
-    sor_pair_kernel<COMP, true, true, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+    sor_pair_kernel<COMP, true, true, T><<<grid, block, 0, s>>>(d, ap, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
   else if (onfly)
-    sor_pair_kernel<COMP, true, false, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+    sor_pair_kernel<COMP, true, false, T><<<grid, block, 0, s>>>(d, ap, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
   else if (fuse)
-    sor_pair_kernel<COMP, false, true, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+    sor_pair_kernel<COMP, false, true, T><<<grid, block, 0, s>>>(d, ap, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
   else
-    sor_pair_kernel<COMP, false, false, T><<<grid, block, 0, s>>>(d, a, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
+    sor_pair_kernel<COMP, false, false, T><<<grid, block, 0, s>>>(d, ap, kIs[p], kJs[p], kKs[p], jend, gx, gy, nt, pp);
 }
 
 template <class T, class P = PT<T>>
@@ -400,17 +430,19 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
   // update, the same expression as the sweeps evaluate, once for the solve
   {
     T* r00[3];
+    const size_t slab = r00_slab(d);
     for (int m = 0; m < 3; m++) {
-      r00[m] = static_cast<T*>(scratch(10 + m, sizeof(T) * (size_t)d.n3));
+      r00[m] = static_cast<T*>(scratch(10 + m, sizeof(T) * 4 * slab));
       a.r00[m] = r00[m];
     }
     const T *rho_d = a.rho, *rhon_d = a.rhon;
     launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
       const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+      const size_t o = pair_of(i, j, k) * slab + r00_index(d, (i - 1) >> 1, (j - 1) >> 1, k);
       const T q = T(0.25);
-      r00[0][c] = (rho_d[c] + rho_d[c + X] + rhon_d[c] + rhon_d[c + X]) * q;
-      r00[1][c] = (rho_d[c] + rho_d[c + Y] + rhon_d[c] + rhon_d[c + Y]) * q;
-      r00[2][c] = (rho_d[c] + rho_d[c + Z] + rhon_d[c] + rhon_d[c + Z]) * q;
+      r00[0][o] = (rho_d[c] + rho_d[c + X] + rhon_d[c] + rhon_d[c + X]) * q;
+      r00[1][o] = (rho_d[c] + rho_d[c + Y] + rhon_d[c] + rhon_d[c + Y]) * q;
+      r00[2][o] = (rho_d[c] + rho_d[c + Z] + rhon_d[c] + rhon_d[c + Z]) * q;
     });
   }
   using P = PT<T>;
