@@ -214,6 +214,14 @@ struct MemoOut {
 };
 bool memo_reuse(const MemoKey& k, std::initializer_list<MemoOut> outs, bool allow_copy);
 void memo_record(const MemoKey& k, std::initializer_list<MemoOut> outs);
+// Before a call overwrites outputs that still hold a memoised result: the
+// result is parked -- the buffer that holds it is kept by the memo and the
+// array gets another (with the contents outside the box (-2..n+2)^3 copied
+// over when `box`, the call writing the box only) -- so a later call with
+// that result's key gets the buffer back instead of recomputing it.  At
+// most a fixed number of bytes is parked; the least recently used results
+// are dropped first.
+void memo_park(const Dims& d, std::initializer_list<MemoOut> outs, bool box);
 void memo_release();
 // Key of a face-gradient call (cal_grad_p2a_, cal_grad_p2uvw_): the
 // arguments, the input's contents and the shells of the three outputs,

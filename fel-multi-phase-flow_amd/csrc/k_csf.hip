@@ -283,7 +283,11 @@ void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const doubl
   uint64_t sh[3];
   const MemoKey k = gradient_key(kOpGradP2uvw, (int)sizeof(T), d, ID, svall, dxinv, dyinv_array,
                                  dzinv, q, outs, sh);
-  if (memo_reuse(k, {{qx_u, n}, {qy_v, n}, {qz_w, n}}, false)) return;
+  if (memo_reuse(k, {{qx_u, n}, {qy_v, n}, {qz_w, n}}, false)) {
+    for (int m = 0; m < 3; m++) keep_shell(outs[m], n, sh[m]);
+    return;
+  }
+  memo_park(d, {{qx_u, n}, {qy_v, n}, {qz_w, n}}, true);
   Call c("cal_grad_p2uvw_");
   bool uni = true;
   const int cnt = svall + 6;

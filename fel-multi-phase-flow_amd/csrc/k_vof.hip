@@ -324,8 +324,12 @@ void cal_grad_p2a_t(const Dims d, int ID, int svall, double dxinv, const double*
   uint64_t sh[3];
   const MemoKey k = gradient_key(kOpGradP2a, (int)sizeof(T), d, ID, svall, dxinv, dyinv_array,
                                  dzinv, q, outs, sh);
-  // the arrays still hold this very result: nothing to do
-  if (memo_reuse(k, {{qx, n}, {qy, n}, {qz, n}}, false)) return;
+  // the arrays, or parked buffers, still hold this very result
+  if (memo_reuse(k, {{qx, n}, {qy, n}, {qz, n}}, false)) {
+    for (int m = 0; m < 3; m++) keep_shell(outs[m], n, sh[m]);
+    return;
+  }
+  memo_park(d, {{qx, n}, {qy, n}, {qz, n}}, true);
   Call c("cal_grad_p2a_");
   const PT<T>* dq = c.in_t<PT<T>>(q, d.n3, "q");
   T* dx = c.io_t<T>(qx, d.n3, "qx");

@@ -514,12 +514,119 @@ struct MemoEntry {
   uintptr_t host[4];
   size_t count[4];
   uint64_t ver[4];
+  // a result whose array was about to be overwritten, kept in the buffer
+  // that held it (owned by the memo; null: none)
+  void* parked[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t pbytes[4] = {0, 0, 0, 0};
   uint64_t used = 0;
 };
 std::vector<MemoEntry> g_memo;
 uint64_t g_memo_tick = 0;
 int64_t g_memo_hits = 0;
 constexpr size_t kMemoEntries = 64;
+constexpr size_t kParkedMax = (size_t)4 << 30;  // bytes of parked results at most
+size_t g_parked_total = 0;
+
+// Device buffers of whole-array size, recycled between the ranges and the
+// parked results (all work is on one stream, so a recycled buffer is only
+// written after every earlier use of it).
+std::multimap<size_t, void*> g_pool;
+
+void* pool_get(size_t bytes) {
+  auto it = g_pool.find(bytes);
+  if (it != g_pool.end()) {
+    void* p = it->second;
+    g_pool.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  FEL_CHECK(hipMalloc(&p, bytes));
+  return p;
+}
+
+void pool_put(void* p, size_t bytes) { g_pool.emplace(bytes, p); }
+
+void unpark(MemoEntry& e, int m) {
+  if (!e.parked[m]) return;
+  pool_put(e.parked[m], e.pbytes[m]);
+  g_parked_total -= e.pbytes[m];
+  e.parked[m] = nullptr;
+  e.pbytes[m] = 0;
+}
+
+void unpark_all(MemoEntry& e) {
+  for (int m = 0; m < 4; m++) unpark(e, m);
+}
+
+// Parks g's buffer with the memo entry whose result it holds (not `keep`'s
+// entries' room), evicting the least recently used parked results to stay
+// within the limit; true when parked (g->dev is then the memo's).
+bool park_buffer(Range& g, const MemoEntry* keep) {
+  if (g.lazy || g.zero) return false;
+  MemoEntry* e = nullptr;
+  int m = -1;
+  for (MemoEntry& x : g_memo)
+    for (int q = 0; q < x.nout; q++)
+      if (x.host[q] == g.host && x.ver[q] == g.ver && !x.parked[q]) {
+        e = &x;
+        m = q;
+      }
+  if (!e) return false;
+  const size_t bytes = g.bytes / sizeof(double) * (size_t)g.esize;
+  while (g_parked_total + bytes > kParkedMax) {  // the least recently used go first
+    MemoEntry* v = nullptr;
+    for (MemoEntry& x : g_memo)
+      for (int q = 0; q < 4; q++)
+        if (x.parked[q] && &x != e && &x != keep && (!v || x.used < v->used)) v = &x;
+    if (!v) break;
+    unpark_all(*v);
+  }
+  if (g_parked_total + bytes > kParkedMax) return false;
+  e->parked[m] = g.dev;
+  e->pbytes[m] = bytes;
+  g_parked_total += bytes;
+  return true;
+}
+
+// Cells of a padded array outside the box (-2..n+2)^3: the faces i = ni+3,
+// j = nj+3 (i <= ni+2) and k = nk+3 (i <= ni+2, j <= nj+2), each cell once.
+template <int B>
+__global__ void __launch_bounds__(256) copy_shell_kernel(Dims d, char* dst, const char* src) {
+  const int fi = d.sy * d.sz, fj = (d.sx - 1) * d.sz, fk = (d.sx - 1) * (d.sy - 1);
+  const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int i, j, k;
+  if (m < fi) {
+    i = d.ni + 3;
+    j = m % d.sy - 2;
+    k = m / d.sy - 2;
+  } else if (m < fi + fj) {
+    const int t = m - fi;
+    j = d.nj + 3;
+    i = t % (d.sx - 1) - 2;
+    k = t / (d.sx - 1) - 2;
+  } else if (m < fi + fj + fk) {
+    const int t = m - fi - fj;
+    k = d.nk + 3;
+    i = t % (d.sx - 1) - 2;
+    j = t / (d.sx - 1) - 2;
+  } else {
+    return;
+  }
+  const size_t o = (size_t)d.ix(i, j, k) * B;
+  for (int b = 0; b < B; b += 4)
+    *reinterpret_cast<uint32_t*>(dst + o + b) = *reinterpret_cast<const uint32_t*>(src + o + b);
+}
+
+void copy_shell(const Dims& d, void* dst, const void* src, int esize) {
+  const int n = d.sy * d.sz + (d.sx - 1) * d.sz + (d.sx - 1) * (d.sy - 1);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  char* o = static_cast<char*>(dst);
+  const char* i = static_cast<const char*>(src);
+  if (esize == 16) copy_shell_kernel<16><<<nb, 256, 0, R().s>>>(d, o, i);
+  else if (esize == 4) copy_shell_kernel<4><<<nb, 256, 0, R().s>>>(d, o, i);
+  else copy_shell_kernel<8><<<nb, 256, 0, R().s>>>(d, o, i);
+  FEL_CHECK(hipGetLastError());
+}
 
 bool memo_enabled() { return result_reuse(); }
 
@@ -555,18 +662,40 @@ bool memo_reuse(const MemoKey& k, std::initializer_list<MemoOut> outs, bool allo
   if (!k.ok || !memo_enabled() || !outs_registered(outs)) return false;
   MemoEntry* e = memo_find(k);
   if (!e || e->nout != (int)outs.size()) return false;
+  bool held[4];
   int m = 0;
   for (const MemoOut& o : outs) {
     Range* h = whole_range(reinterpret_cast<const void*>(e->host[m]), e->count[m]);
     Range* d = whole_range(o.host, o.count);
-    if (!h || h->ver != e->ver[m] || e->count[m] != o.count || h->esize != d->esize) return false;
-    if (h != d && !allow_copy) return false;
+    held[m] = h && h->ver == e->ver[m] && e->count[m] == o.count && h->esize == d->esize &&
+              (h == d || allow_copy);
+    const bool parked = e->parked[m] && e->count[m] == o.count &&
+                        e->pbytes[m] == d->bytes / sizeof(double) * (size_t)d->esize;
+    if (!held[m] && !parked) return false;
     m++;
   }
   m = 0;
   for (const MemoOut& o : outs) {
     const void* src = reinterpret_cast<const void*>(e->host[m]);
-    if (src != o.host) {
+    if (!held[m]) {
+      // the parked buffer becomes the output's; the output's own contents
+      // are parked in turn when they are a memoised result
+      Range* d = whole_range(o.host, o.count);
+      deferred_coef_check(o.host, true);
+      materialize_dependents(*d);
+      drop_lazy(*d);
+      void* mine = e->parked[m];
+      const size_t pb = e->pbytes[m];
+      e->parked[m] = nullptr;
+      e->pbytes[m] = 0;
+      g_parked_total -= pb;
+      if (!park_buffer(*d, e)) pool_put(d->dev, pb);
+      d->dev = mine;
+      d->ver = e->ver[m];
+      d->subs.clear();
+      d->shell = 0;
+      e->host[m] = reinterpret_cast<uintptr_t>(o.host);
+    } else if (src != o.host) {
       if (!deferred_copy(src, o.host, o.count)) {
         std::fprintf(stderr, "felgpu: internal: memo copy refused\n");
         std::abort();
@@ -587,6 +716,7 @@ void memo_record(const MemoKey& k, std::initializer_list<MemoOut> outs) {
   if (e && e->nout == (int)outs.size()) {
     // the same function of the same inputs: the outputs carry the contents
     // numbers given to this key before
+    unpark_all(*e);
     int m = 0;
     for (const MemoOut& o : outs) {
       whole_range(o.host, o.count)->ver = e->ver[m];
@@ -607,6 +737,7 @@ void memo_record(const MemoKey& k, std::initializer_list<MemoOut> outs) {
         if (x.used < e->used) e = &x;
     }
   }
+  unpark_all(*e);
   e->key = k.w;
   e->nout = (int)outs.size();
   int m = 0;
@@ -619,7 +750,28 @@ void memo_record(const MemoKey& k, std::initializer_list<MemoOut> outs) {
   e->used = ++g_memo_tick;
 }
 
-void memo_release() { g_memo.clear(); }
+void memo_park(const Dims& d, std::initializer_list<MemoOut> outs, bool box) {
+  if (!memo_enabled()) return;
+  for (const MemoOut& o : outs) {
+    Range* g = whole_range(o.host, o.count);
+    if (!g || g->lazy || g->zero) continue;
+    deferred_coef_check(o.host, true);  // a recorded chain reading this buffer runs now
+    materialize_dependents(*g);         // copies deferred on this buffer are made now
+    void* old = g->dev;
+    if (!park_buffer(*g, nullptr)) continue;
+    const size_t bytes = g->bytes / sizeof(double) * (size_t)g->esize;
+    g->dev = pool_get(bytes);
+    if (box) copy_shell(d, g->dev, old, g->esize);  // the call writes the box only
+  }
+}
+
+void memo_release() {
+  for (MemoEntry& e : g_memo) unpark_all(e);
+  g_memo.clear();
+  for (auto& kv : g_pool) FEL_CHECK(hipFree(kv.second));
+  g_pool.clear();
+  g_parked_total = 0;
+}
 
 void* exchange_device_buffer(const void* host, size_t count, void* buf) {
   Range* g = whole_range(host, count);
