@@ -63,7 +63,9 @@ void ensure_init();
 // How an argument array is used by one drop-in call.
 // kOverwrite: every element of the argument is written (no element of the
 // previous contents survives the call).
-enum Access { kRead = 1, kWrite = 2, kReadWrite = 3, kOverwrite = 6 };
+// kInterior: the call writes exactly the interior cells [1,n]^3 of a
+// padded array and reads none of it.
+enum Access { kRead = 1, kWrite = 2, kReadWrite = 3, kOverwrite = 6, kInterior = 18 };
 
 // Precision of the device copies.  In the default FP64 mode every device
 // copy is FP64, like the host arrays.  In the FP32 mode (FELGPU_PRECISION=fp32
@@ -143,11 +145,19 @@ class Call {
   T* all_t(const void* host, size_t count, const char* what = "") {
     return arr<T>(host, count, kOverwrite, what);
   }
+  // an output of which the call writes exactly the interior cells of the
+  // padded grid d (deferred contents are then made for the ghost cells only)
+  template <class T>
+  T* interior_t(const void* host, const Dims& d, const char* what = "") {
+    box_ = d;
+    return arr<T>(host, (size_t)d.n3, kInterior, what);
+  }
   void finish();
 
  private:
   void* resolve(const void* host, size_t count, Access a, const char* what, int esize);
   const char* name_;
+  Dims box_{};
   struct Pending {
     void* host;
     void* dev;
@@ -292,12 +302,22 @@ struct DeferredCoef {
   const void* src_dev;
   Dims d;
   double dxinv, dyinv, dzinv;
+  // the viscosity's former buffer, taken over when the viscosity array was
+  // overwritten as a whole (returned when the record ends)
+  void* owned = nullptr;
+  size_t owned_bytes = 0;
 };
 DeferredCoef& deferred_coef();
 bool deferred_coef_disabled();                    // FELGPU_EAGER_COEF=1 turns deferral off
 bool is_registered(const void* host, size_t count);  // whole range device-resident?
 void deferred_coef_flush();                       // materialise now, if pending
-void deferred_coef_check(const void* host, bool write);  // flush if host conflicts
+// Flush if host conflicts.  `whole`: the access overwrites the whole array
+// containing host, so a source about to change can hand over its buffer
+// instead of being copied.
+void deferred_coef_check(const void* host, bool write, bool whole = false);
+// Ends the pending record without materialising it (the caller redefines
+// every one of its outputs).
+void deferred_coef_drop();
 // The 45 coefficient arrays from the viscosity, in the element type esize.
 void coef_vis_any(const Dims d, double dxinv, double dyinv, double dzinv, const void* rmu,
                   void* const* out45, int esize);

@@ -557,7 +557,7 @@ void corunp_t(const Dims d, int jen, double rho, double dxinv, double dyinv, dou
   const PT<T>* ddp = c.in_t<PT<T>>(dp, d.n3, "dp");
   const T* ph = c.in_t<T>(phat, d.n3, "phat");
   T *u = c.io_t<T>(un, d.n3, "un"), *v = c.io_t<T>(vn, d.n3, "vn"), *w = c.io_t<T>(wn, d.n3, "wn"),
-    *p = c.io_t<T>(pn, d.n3, "pn");
+    *p = c.interior_t<T>(pn, d, "pn");  // written on the interior only
   correct<T>(d, jen, rho, dxinv, dyinv, dzinv, dt, ddp, ph, u, v, w, p);
   c.finish();
 }

@@ -254,7 +254,7 @@ template <class T>
 void cal_arith_mean_t(const Dims d, const double* phik, double ql, double qg, double* qk) {
   Call c("cal_arith_mean_");
   const PT<T>* p = c.in_t<PT<T>>(phik, d.n3, "phik");
-  T* q = c.out_t<T>(qk, d.n3, "qk");
+  T* q = c.all_t<T>(qk, d.n3, "qk");  // every element is written
   arith_mean<T>(d, p, ql, qg, q);
   c.finish();
 }
@@ -458,7 +458,7 @@ void cal_arith_coef_vis_t(const Dims d, double dxinv, double dyinv, double dzinv
     bool same = true;
     for (int m = 0; m < 45 && same; m++) same = dc.out_host[m] == (const void*)h[m];
     if (!same) deferred_coef_flush();
-    dc.active = false;
+    deferred_coef_drop();
   }
   bool resident = is_registered(rmuk, d.n3);
   for (int m = 0; m < 45 && resident; m++) resident = is_registered(h[m], d.n3);

@@ -152,6 +152,28 @@ void add_zero(void* dst, const void* src, size_t n, int esize) {
   FEL_CHECK(hipGetLastError());
 }
 
+// Copies the cells of a padded array outside the interior [1,n]^3.
+template <int B>
+__global__ void __launch_bounds__(256) copy_outside_kernel(Dims d, char* dst, const char* src) {
+  const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (m >= d.n3) return;
+  const int i = m % d.sx - 2, r = m / d.sx, j = r % d.sy - 2, k = r / d.sy - 2;
+  if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk) return;
+  const size_t o = (size_t)m * B;
+  for (int b = 0; b < B; b += 4)
+    *reinterpret_cast<uint32_t*>(dst + o + b) = *reinterpret_cast<const uint32_t*>(src + o + b);
+}
+
+void copy_outside_interior(const Dims& d, void* dst, const void* src, int esize) {
+  const unsigned nb = (unsigned)((d.n3 + 255) / 256);
+  char* o = static_cast<char*>(dst);
+  const char* i = static_cast<const char*>(src);
+  if (esize == 16) copy_outside_kernel<16><<<nb, 256, 0, R().s>>>(d, o, i);
+  else if (esize == 4) copy_outside_kernel<4><<<nb, 256, 0, R().s>>>(d, o, i);
+  else copy_outside_kernel<8><<<nb, 256, 0, R().s>>>(d, o, i);
+  FEL_CHECK(hipGetLastError());
+}
+
 void materialize(Range& g) {
   if (g.zero) {
     FEL_CHECK(hipMemsetAsync(g.dev, 0, g.bytes / sizeof(double) * (size_t)g.esize, R().s));
@@ -444,6 +466,16 @@ const void* deferred_add_zero_source(const void* host, size_t count) {
   return g->lazy->dev;
 }
 
+void pool_put_buffer(void* p, size_t bytes);
+
+void deferred_coef_drop() {
+  DeferredCoef& dc = deferred_coef();
+  dc.active = false;
+  if (dc.owned) pool_put_buffer(dc.owned, dc.owned_bytes);
+  dc.owned = nullptr;
+  dc.owned_bytes = 0;
+}
+
 void deferred_coef_flush() {
   DeferredCoef& dc = deferred_coef();
   if (!dc.active) return;
@@ -457,6 +489,7 @@ void deferred_coef_flush() {
   for (int m = 0; m < 45; m++)
     if (Range* g = find_range(R().reg, reinterpret_cast<uintptr_t>(dc.out_host[m]), 1))
       note_write(*g);
+  deferred_coef_drop();  // the kernel is queued before the buffer can be reused
 }
 
 uint64_t content_version(const void* host, size_t count) {
@@ -545,6 +578,12 @@ void* pool_get(size_t bytes) {
 }
 
 void pool_put(void* p, size_t bytes) { g_pool.emplace(bytes, p); }
+}  // namespace
+
+void* pool_get_buffer(size_t bytes) { return pool_get(bytes); }
+void pool_put_buffer(void* p, size_t bytes) { pool_put(p, bytes); }
+
+namespace {
 
 void unpark(MemoEntry& e, int m) {
   if (!e.parked[m]) return;
@@ -830,7 +869,7 @@ void deferred_chain_check(const void* host, bool write) {
   if (chain_holds(w.outs, g) || (write && chain_holds(w.srcs, g))) deferred_chain_flush();
 }
 
-void deferred_coef_check(const void* host, bool write) {
+void deferred_coef_check(const void* host, bool write, bool whole) {
   deferred_chain_check(host, write);
   DeferredCoef& dc = deferred_coef();
   if (!dc.active) return;
@@ -844,6 +883,19 @@ void deferred_coef_check(const void* host, bool write) {
     }
   }
   const uintptr_t s = reinterpret_cast<uintptr_t>(dc.src_host);
+  if (write && whole && dc.src_host && p == s) {
+    // the viscosity is overwritten as a whole: keep its buffer, give the
+    // array another
+    Range* g = whole_range(dc.src_host, (size_t)dc.d.n3);
+    if (g && !g->lazy && !g->zero && g->dev == dc.src_dev) {
+      const size_t b = (size_t)g->esize * (size_t)dc.d.n3;
+      dc.owned = g->dev;
+      dc.owned_bytes = b;
+      g->dev = pool_get_buffer(b);
+      dc.src_host = nullptr;
+      return;
+    }
+  }
   if (write && dc.src_host && p >= s && p < s + bytes) {
     // the viscosity is about to change: keep the values the coefficients
     // are a function of (16 B/cell) instead of materialising 45 arrays
@@ -873,9 +925,19 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
     std::fprintf(stderr, "felgpu: %s: null array argument %s\n", name_, what);
     std::abort();
   }
-  deferred_coef_check(host, (a & kWrite) != 0);
+  {
+    const Range* w = find_range(r.reg, p, bytes);
+    const bool whole = (a & kOverwrite) == kOverwrite && w && p == w->host && bytes == w->bytes;
+    deferred_coef_check(host, (a & kWrite) != 0, whole);
+  }
   if (Range* g = find_range(r.reg, p, bytes)) {
-    if ((a & kOverwrite) == kOverwrite && p == g->host && bytes == g->bytes) {
+    if (a == kInterior && p == g->host && bytes == g->bytes && g->lazy && g->lkind == kCopy) {
+      // the interior is about to be rewritten: only the ghost cells of the
+      // deferred copy are made
+      copy_outside_interior(box_, g->dev, g->lazy->dev, g->esize);
+      drop_lazy(*g);
+      materialize_dependents(*g);
+    } else if ((a & kOverwrite) == kOverwrite && p == g->host && bytes == g->bytes) {
       // every element is replaced: a deferred copy of the old contents is
       // not needed, only the ranges deferred on this one
       materialize_dependents(*g);

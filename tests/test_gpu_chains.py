@@ -103,3 +103,58 @@ def test_viscous_chain_sources_rewritten_after_the_chain():
             assert same_bits(reg.download(x), y)
     finally:
         reg.close()
+
+
+NID = np.full(6, -1, dtype=np.int32)
+
+
+def test_interior_write_of_a_deferred_copy():
+    """pn deferred on p (cpy_ pn -> p) and then written by corunp_explicit_
+    on the interior only: its ghost cells keep pn's contents, made from the
+    deferred copy without copying the interior."""
+    g = Grid(16, 10, 12, 0.1, 0.05, 0.08)
+    r = np.random.default_rng(52)
+    host = [r.standard_normal(g.shape) for _ in range(7)]  # dp, phat, un, vn, wn, pn, p
+    want = [a.copy() for a in host]
+    args = [NID.copy(), I(g.ni), I(g.nj), I(g.nk), D(0.81), D(1 / g.dx), D(1 / g.dy),
+            D(1 / g.dz), D(0.003)]
+    H.call_oracle("corunp_explicit", *args, *want[:6])
+    want[6] = host[5].copy()
+    reg = Registered()
+    try:
+        dev = [reg.add(a.copy()) for a in host]
+        H.call_gpu("cpy", I(g.ni), I(g.nj), I(g.nk), dev[5], dev[6])  # p <- pn (pn deferred)
+        H.call_gpu("corunp_explicit", *args, *dev[:6])
+        for name, x, y in zip(["dp", "phat", "un", "vn", "wn", "pn", "p"], dev, want):
+            assert same_bits(reg.download(x), y), name
+    finally:
+        reg.close()
+
+
+def test_coefficients_after_their_viscosity_is_overwritten():
+    """cal_arith_coef_vis_ records its 45 outputs as a function of the
+    viscosity; cal_arith_mean_ then overwrites the viscosity as a whole (the
+    record takes over the old buffer); a later read of a coefficient array
+    makes them from the old viscosity."""
+    g = Grid(12, 8, 10, 0.1, 0.05, 0.08)
+    r = np.random.default_rng(53)
+    rmu = r.uniform(0.01, 0.2, g.shape)
+    phi = r.random(g.shape)
+    dims = [I(g.ni), I(g.nj), I(g.nk)]
+    inv = [D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+    want = [g.zeros() for _ in range(45)]
+    H.call_oracle("cal_arith_coef_vis", *dims, *inv, rmu.copy(), *want)
+    want_mu = g.zeros()
+    H.call_oracle("cal_arith_mean", *dims, phi.copy(), D(1e-3), D(1e-5), want_mu)
+    reg = Registered()
+    try:
+        dmu = reg.add(rmu.copy())
+        dphi = reg.add(phi.copy())
+        coef = [reg.add(r.standard_normal(g.shape)) for _ in range(45)]
+        H.call_gpu("cal_arith_coef_vis", *dims, *inv, dmu, *coef)
+        H.call_gpu("cal_arith_mean", *dims, dphi, D(1e-3), D(1e-5), dmu)
+        assert same_bits(reg.download(dmu), want_mu)
+        for m in (0, 7, 22, 44):
+            assert same_bits(reg.download(coef[m]), want[m]), m
+    finally:
+        reg.close()
