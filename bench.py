@@ -10,10 +10,12 @@ device-resident before the timed region.  Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
                   [--no-cpu-baseline] [--precision 64|32]
 
-By default three runs start from the same initial state: `value` is the
+By default four runs start from the same initial state: `value` is the
 FP64 path (results equal to the reference's FP64 arithmetic); `fp32_refine`
-holds the FP32 mode's throughput, its speed-up and the relative L2 errors
-of u, v, w, p against the FP64 run after the same number of steps;
+holds the FP32 mode's throughput (FP32 pressure-solve modes with one FP64
+refinement step), its speed-up and the relative L2 errors of u, v, w, p
+against the FP64 run after the same number of steps; `fp32_fp64_poisson`
+the same for the FP32 mode with FP64 pressure-solve modes;
 `tangent` holds the time of the tangent mode (the same step computing the
 FP64 values and their derivative along the bottom wall's contact angle:
 forward-mode differentiation, one parameter direction per run) relative to
@@ -97,13 +99,15 @@ def cpu_baseline(cfg_full, seconds=15.0):
                       f"= {cfg.cells} cells, {n} steps after 2 warm-up steps, {dt:.1f} s, 1 thread"}
 
 
-def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False):
+def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp32_modes=True):
     """Runs warmup + steps timesteps at the given precision (or in the
     tangent mode, seeded along the bottom wall's contact angle); returns the
-    timing, the event timers and (optionally) the final fields."""
+    timing, the event timers and (optionally) the final fields.  In the
+    FP32 mode, fp32_modes selects FP32 pressure-solve modes with FP64
+    refinement (True) or FP64 modes (False)."""
     import felmpf
     from felmpf import _lib
-    run = felmpf.GpuRun(cfg, precision=precision, tangent=tangent)
+    run = felmpf.GpuRun(cfg, precision=precision, tangent=tangent, fp32_modes=fp32_modes)
     gpu = _lib.gpu()
     if tangent:
         seed = np.zeros(cfg.shape)
@@ -167,24 +171,33 @@ def main():
     head_prec = 32 if args.precision == 32 else 64
     both = args.precision == 0
     main_run = measure(cfg, head_prec, args.steps, args.warmup, keep_fields=both)
-    fp32 = None
-    if both:
-        r32 = measure(cfg, 32, args.steps, args.warmup, keep_fields=True)
+    fp32 = fp32_64 = None
+
+    def fp32_summary(r32, dtype):
         errs = {}
         for f in ("u", "v", "w", "p"):
             a, b = main_run["fields"][f], r32["fields"][f]
             errs[f] = float(np.linalg.norm(a - b) / np.linalg.norm(a))
         v32 = cfg.cells * args.steps / r32["elapsed"] / 1e6
         v64 = cfg.cells * args.steps / main_run["elapsed"] / 1e6
-        fp32 = {"value": round(v32, 3), "unit": "Mcell-steps/s",
+        return {"value": round(v32, 3), "unit": "Mcell-steps/s",
                 "ms_per_step": round(r32["elapsed"] / args.steps * 1e3, 3),
                 "speedup_vs_fp64": round(v32 / v64, 3),
                 "rel_l2_error_vs_fp64_after_steps": args.warmup + args.steps,
                 "rel_l2_error_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
                 "volume_drift": float(f"{r32['volume_drift']:.3e}"),
                 "sor_iterations_last_step": r32["state"]["sor_lp"] + 1,
-                "dtype": "f32 flow fields and stencils, f64 volume fraction, mass sums, "
-                         "contact-line velocity and refined pressure correction"}
+                "dtype": dtype}
+
+    if both:
+        fp32 = fp32_summary(
+            measure(cfg, 32, args.steps, args.warmup, keep_fields=True, fp32_modes=True),
+            "f32 flow fields, stencils and pressure-solve modes; f64 volume fraction, mass "
+            "sums, contact-line velocity and pressure correction (one FP64 refinement step)")
+        fp32_64 = fp32_summary(
+            measure(cfg, 32, args.steps, args.warmup, keep_fields=True, fp32_modes=False),
+            "f32 flow fields and stencils; f64 volume fraction, mass sums, contact-line "
+            "velocity, pressure-solve modes and pressure correction (no refinement needed)")
     tangent = None
     if both:
         rt = measure(cfg, 64, args.steps, args.warmup, keep_fields=True, tangent=True)
@@ -231,6 +244,7 @@ def main():
                      "bytes_per_launch": b, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": n_sor, "traffic_source": traffic_src},
         "fp32_refine": fp32,
+        "fp32_fp64_poisson": fp32_64,
         "tangent": tangent,
         "cpu_baseline": cpu,
     }

@@ -589,7 +589,9 @@ void solp_fft_tdma1_(const int* ni, const int* nj, const int* nk, double* fftdat
                      double* phir_r, double* phir_i) {
   (void)fftdata;
   const Dims d = make_dims(*ni, *nj, *nk);
-  with_flow([&](auto t) { tdma1_t<decltype(t)>(d, div, phir_r, phir_i); });
+  // the modes' precision is the modal arrays' (FP32 in the FP32 mode when
+  // they are registered with the flow class: FP32 solve + FP64 refinement)
+  with_elem(elem_size(phir_r), [&](auto t) { tdma1_t<decltype(t)>(d, div, phir_r, phir_i); });
 }
 
 // srcs/trans_r2w.f:1-38 on one rank: the relabelled array has the same
@@ -648,7 +650,7 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
   const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
   Call c("solp_fft_tdma2_");
   prof_begin(6);
-  with_flow([&](auto t) {
+  with_elem(elem_size(phiw_r), [&](auto t) {
     using T = decltype(t);
     T* pr = c.io_t<T>(phiw_r, nm, "phiw_r");
     T* pi = c.io_t<T>(phiw_i, nm, "phiw_i");
@@ -669,7 +671,7 @@ void solp_fft_tdma3_(const int* ni, const int* nj, const int* nk, double* fftdat
                      const double* phir_r, const double* phir_i, double* dp) {
   (void)fftdata;
   const Dims d = make_dims(*ni, *nj, *nk);
-  with_flow([&](auto t) { tdma3_t<decltype(t)>(d, phir_r, phir_i, dp); });
+  with_elem(elem_size(phir_r), [&](auto t) { tdma3_t<decltype(t)>(d, phir_r, phir_i, dp); });
 }
 
 // srcs/solp_fft_tdma4.f:4-5: residual monitor every imon_t steps.

@@ -34,7 +34,8 @@ class GpuRun:
                "phin1": ("phin0", 1)}
 
     def __init__(self, cfg: RunConfig, state: dict[str, np.ndarray] | None = None,
-                 precision: int = 64, tangent: bool = False, reuse: bool = True):
+                 precision: int = 64, tangent: bool = False, reuse: bool = True,
+                 fp32_modes: bool = False):
         if precision not in (32, 64):
             raise ValueError(f"precision must be 32 or 64, not {precision}")
         if tangent and precision != 64:
@@ -58,6 +59,9 @@ class GpuRun:
         # results already computed are taken over instead of recomputed
         # (bitwise the same either way)
         self.gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(1 if reuse else 0)))
+        # pressure-solve modes: FP64 (default), or the flow class (FP32 solve
+        # with FP64 refinement in the FP32 mode)
+        self.drv.fel_drv_set_modes_flow_class(ctypes.c_int(1 if fp32_modes else 0))
         self.drv.fel_drv_start()
 
     def view(self, name: str) -> np.ndarray:

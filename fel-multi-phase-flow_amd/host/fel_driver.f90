@@ -34,6 +34,9 @@ module fel_driver
 
   type(fel_cfg), save :: c
   logical, save :: created = .false.
+  ! the modal arrays of the pressure solve in the flow class (FP32 in the
+  ! library's FP32 mode: FP32 solve + FP64 refinement) instead of FP64
+  logical, save :: modes_flow_class = .false.
   integer, save :: ni, nj, nk, nstep, ipara, ID, ndiv
   integer, save :: nID(6), svall(3), key(8)
   real(8), save :: dxinv, dyinv, dzinv, time, err_div0, err_div, dt
@@ -261,6 +264,14 @@ contains
   ! Makes every array device-resident (uploading the fields the caller
   ! initialised) and runs the start-up boundary fills of
   ! srcs/main.f90:303-366 and the pressure coefficients of main.f90:467-468.
+  ! Before fel_drv_start: flag /= 0 registers the pressure solve's modal
+  ! arrays in the flow class (in the library's FP32 mode the solve then runs
+  ! in FP32 with FP64 refinement); 0 (default) keeps them FP64.
+  subroutine fel_drv_set_modes_flow_class(flag) bind(C, name="fel_drv_set_modes_flow_class")
+    integer(c_int), value :: flag
+    modes_flow_class = flag /= 0
+  end subroutine fel_drv_set_modes_flow_class
+
   subroutine fel_drv_start() bind(C, name="fel_drv_start")
     integer(c_int64_t) :: n3
     integer :: m
@@ -294,7 +305,11 @@ contains
     end do
     call reg64(theta_0_array, n3, 1); call reg64(l1_array, n3, 1); call reg64(l2_array, n3, 1)
     call reg64(zeta_array, n3, 1); call reg64(theta_array, n3, 0)
-    call reg(phir_r, int(ni,c_int64_t)*nj*nk, 0); call reg(phir_i, int(ni,c_int64_t)*nj*nk, 0)
+    if (modes_flow_class) then
+      call reg(phir_r, int(ni,c_int64_t)*nj*nk, 0); call reg(phir_i, int(ni,c_int64_t)*nj*nk, 0)
+    else
+      call reg64(phir_r, int(ni,c_int64_t)*nj*nk, 0); call reg64(phir_i, int(ni,c_int64_t)*nj*nk, 0)
+    end if
 
     ! srcs/main.f90:304-366
     call cpy(ni,nj,nk,u,un)
