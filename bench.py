@@ -23,8 +23,13 @@ the FP64 step, and whether its values equal the FP64 run's bit for bit.
 
 The roofline object describes the dominant kernel (the colour-pair
 Gauss-Seidel update of the viscous solve): algorithmic bytes per launch
-divided by its average duration, measured with HIP events recorded on the
-library's stream around each launch inside the timed region.  The CPU
+divided by its average duration, measured with HIP events on the library's
+stream inside the timed region.  The launches go through
+hipExtLaunchKernelGGL, which stamps an event with its own launch's start or
+end, so the timing adds no packets between launches: the three pairs of a
+sweep that run back to back are one timed span (start of the first to the
+end of the third, divided by three), the pair before a ghost-refresh launch
+is not timed.  The CPU
 baseline is the oracle (a serial C restatement of the reference) timed on a
 bounded sample of the same configuration on this host.
 """
@@ -46,8 +51,7 @@ METRIC = "Mcell-steps/s at 512x256x128 (FP64 vs FP32+refine); adjoint/forward ti
 HBM_PEAK_GBS = 8000.0
 
 # event-timer ids recorded by the library (csrc: prof_begin/prof_end)
-TIMERS = {1: "mthinc_flux", 2: "weno5", 3: "sor_pair", 4: "sor_bc", 5: "fft_forward",
-          6: "tdma", 7: "fft_inverse"}
+TIMERS = {3: "sor_pair"}
 
 
 def sor_pair_bytes(cfg) -> float:

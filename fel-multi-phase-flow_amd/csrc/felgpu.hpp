@@ -529,8 +529,14 @@ double* dev_scalars(int slot);
 // Copies n doubles from device to the pinned buffer and waits for them.
 const double* read_scalars(const double* dev, int n);
 
-// Per-kernel event timer used by the benchmark (felgpu_prof_*).
-void prof_begin(int id);
-void prof_end(int id);
+// Launch event timer used by the benchmark (felgpu_prof_*), for
+// hipExtLaunchKernelGGL, which stamps an event with its launch's own start
+// or end (no marker packets between launches).  One timed span covers
+// `launches` back-to-back launches on the library stream: the first gets
+// *start (first = true), the last *stop (last = true), the ones between
+// neither; a single launch passes both flags and launches = 1.  Both
+// events are null while the timers are off.
+void prof_launch_events(int id, bool first, bool last, int launches, hipEvent_t* start,
+                        hipEvent_t* stop);
 
 }  // namespace fel

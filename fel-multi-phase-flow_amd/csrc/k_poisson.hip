@@ -503,9 +503,7 @@ void tdma1_t(const Dims d, const double* div, double* phir_r, double* phir_i) {
   const PT<T>* dv = c.in_t<PT<T>>(div, d.n3, "div");
   T* pr = c.out_t<T>(phir_r, nm, "phir_r");
   T* pi = c.out_t<T>(phir_i, nm, "phir_i");
-  prof_begin(5);
   fft_forward<T>(d, dv, pr, pi);
-  prof_end(5);
   c.finish();
   if constexpr (!is_f32<T>) return;  // the record serves the FP32 refinement
   PoissonRecord& rec = prec();
@@ -523,10 +521,8 @@ void tdma3_t(const Dims d, const double* phir_r, const double* phir_i, double* d
   const T* pr = c.in_t<T>(phir_r, nm, "phir_r");
   const T* pi = c.in_t<T>(phir_i, nm, "phir_i");
   PT<T>* ddp = c.io_t<PT<T>>(dp, d.n3, "dp");
-  prof_begin(7);
   fft_inverse<T>(d, pr, pi, ddp, false);
   if constexpr (is_f32<T>) poisson_refine(d, ddp);
-  prof_end(7);
   c.finish();
 }
 
@@ -649,14 +645,12 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
   FEL_CHECK(hipMemcpyAsync(tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, stream()));
   const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
   Call c("solp_fft_tdma2_");
-  prof_begin(6);
   with_elem(elem_size(phiw_r), [&](auto t) {
     using T = decltype(t);
     T* pr = c.io_t<T>(phiw_r, nm, "phiw_r");
     T* pi = c.io_t<T>(phiw_i, nm, "phiw_i");
     tdma_modes<T>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
   });
-  prof_end(6);
   c.finish();
   FEL_CHECK(hipStreamSynchronize(stream()));  // h is released on return
   PoissonRecord& rec = prec();

@@ -358,10 +358,8 @@ void solphi_mthinc1_t(const Dims d, double dxinv, double dyinv, double dzinv, do
   P* fx = c.io_t<P>(flphix, d.n3, "flphix");
   P* fy = c.io_t<P>(flphiy, d.n3, "flphiy");
   P* fz = c.io_t<P>(flphiz, d.n3, "flphiz");
-  prof_begin(1);
   mthinc_flux<T>(d, dxinv, dyinv, dzinv, dt, mthinc_const(dxinv, dyinv, dzinv, bet), dphi, gx, gy,
                  gz, du, dv, dw, fx, fy, fz);
-  prof_end(1);
   c.finish();
 }
 

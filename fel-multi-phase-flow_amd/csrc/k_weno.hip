@@ -258,9 +258,7 @@ void weno_compute(const Dims d, double dxinv, double dyinv, double dzinv, const 
   T* au = c.out_t<T>(adv_u, d.n3, "adv_u");
   T* av = c.out_t<T>(adv_v, d.n3, "adv_v");
   T* aw = c.out_t<T>(adv_w, d.n3, "adv_w");
-  prof_begin(2);
   advect_weno5<T>(d, dxinv, dyinv, dzinv, du, dv, dw, au, av, aw);
-  prof_end(2);
   c.finish();
 }
 

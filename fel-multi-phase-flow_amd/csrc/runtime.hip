@@ -88,8 +88,8 @@ struct Runtime {
   // timing adds no host synchronisation to the timed region
   struct Timer {
     std::vector<hipEvent_t> b, e;
+    std::vector<int> span;  // launches between b[m]'s launch and e[m]'s, inclusive
     int used = 0;
-    bool open = false;
     double ms = 0.0;
     long n = 0;
   };
@@ -994,9 +994,8 @@ void Call::finish() {
 // Event timers (opt-in; the benchmark reads them)
 // --------------------------------------------------------------------------
 
-void prof_begin(int id) {
+static Runtime::Timer& prof_slot(int id) {
   Runtime& r = R();
-  if (!r.prof) return;
   if ((int)r.timers.size() <= id) r.timers.resize(id + 1);
   Runtime::Timer& t = r.timers[id];
   if (t.used == (int)t.b.size()) {
@@ -1006,17 +1005,21 @@ void prof_begin(int id) {
     t.b.push_back(a);
     t.e.push_back(e);
   }
-  FEL_CHECK(hipEventRecord(t.b[t.used], r.s));
-  t.open = true;
+  return t;
 }
 
-void prof_end(int id) {
-  Runtime& r = R();
-  if (!r.prof || id >= (int)r.timers.size() || !r.timers[id].open) return;
-  Runtime::Timer& t = r.timers[id];
-  FEL_CHECK(hipEventRecord(t.e[t.used], r.s));
-  t.used++;
-  t.open = false;
+void prof_launch_events(int id, bool first, bool last, int launches, hipEvent_t* start,
+                        hipEvent_t* stop) {
+  *start = *stop = nullptr;
+  if (!R().prof) return;
+  Runtime::Timer& t = prof_slot(id);
+  if (first) *start = t.b[t.used];
+  if (last) {
+    *stop = t.e[t.used];
+    if ((int)t.span.size() <= t.used) t.span.resize(t.used + 1);
+    t.span[t.used] = launches;
+    t.used++;
+  }
 }
 
 static void prof_collect() {
@@ -1028,7 +1031,7 @@ static void prof_collect() {
       float ms = 0.f;
       FEL_CHECK(hipEventElapsedTime(&ms, t.b[m], t.e[m]));
       t.ms += ms;
-      t.n += 1;
+      t.n += t.span[m];
     }
     t.used = 0;
   }

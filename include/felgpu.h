@@ -117,9 +117,8 @@ void *felgpu_device_ptr_(const void *host);
 void felgpu_sync_(void);
 void *felgpu_stream_(void);
 
-/* Event timers around the hot kernels (ids: 1 MTHINC flux, 2 WENO5,
- * 3 colour-pair Gauss-Seidel update, 4 Gauss-Seidel ghost refresh,
- * 5 forward FFT, 6 modal Thomas solve, 7 inverse FFT). */
+/* Event timer of the colour-pair Gauss-Seidel update (id 3): every launch's
+ * own start and end while enabled; read gives the total ms and launches. */
 void felgpu_prof_enable_(const int *on);
 void felgpu_prof_read_(const int *id, double *ms_total, int64_t *count);
 void felgpu_prof_reset_(void);
