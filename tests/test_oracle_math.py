@@ -646,6 +646,39 @@ def test_contact_angle_extends_a_planar_interface(theta):
         assert checked == 3 * (g.ni - 2)
 
 
+def test_contact_angle_closed_form_on_non_square_cells():
+    """With n = (s sin th ax, cos th, s sin th az)/|.| (ax = dy/dx, az = dy/dz),
+    n_y/n_x * ax = n_y/n_z * az = s cot th: the aspect ratios cancel, so for
+    any phi and any cell shape the first ghost is
+        q(i,1,k) - s cot(th) ((q(i+1) - q(i-1)) + (q(k+1) - q(k-1)))/2,
+    up to the 1e-20 guards, and the next two extrapolate linearly
+    (srcs/bnd_contact_angle.f90:32-50,63-82).  dx, dy, dz all differ here, so
+    an aspect ratio swapped, inverted or left out of either term fails."""
+    g = Grid(12, 6, 10, 0.1, 0.037, 0.23)
+    rng = np.random.default_rng(61)
+    q = H.smooth_phi(g, rng)
+    theta = rng.uniform(20.0, 160.0, g.shape)
+    got = q.copy()
+    O("bnd_contact_angle", WALLS, I(g.ni), I(g.nj), I(g.nk), got, theta, D(g.dx), D(g.dy),
+      D(g.dz))
+    s = np.where(np.arange(-2, g.ni + 4) - g.ni // 2 >= 0, 1.0, -1.0)
+    inner = (slice(1, g.nk + 5), slice(1, g.ni + 5))  # i, k in [-1, n+2]: both neighbours exist
+    for jw, jg, js in ((1, 0, -1), (g.nj, g.nj + 1, 1)):
+        w = q[:, jw + 2, :]
+        cot = 1.0 / np.tan(np.deg2rad(theta[:, jw + 2, :]))
+        gx = np.zeros_like(w)
+        gz = np.zeros_like(w)
+        gx[:, 1:-1] = w[:, 2:] - w[:, :-2]
+        gz[1:-1, :] = w[2:, :] - w[:-2, :]
+        g0 = w - s[None, :] * cot * (gx + gz) * 0.5
+        g1 = 2.0 * g0 - w
+        g2 = 2.0 * g1 - g0
+        scale = np.abs(w).max() + np.abs(cot * (gx + gz)).max()
+        for jj, want in ((jg, g0), (jg + js, g1), (jg + 2 * js, g2)):
+            err = np.abs(got[:, jj + 2, :][inner] - want[inner]).max()
+            assert err <= 1e-13 * scale, (jw, jj, err)
+
+
 def test_cal_vel_and_center_closed_forms():
     g = Grid(16, 8, 8, 0.25, 0.25, 0.25)
     phi = g.zeros()
