@@ -529,6 +529,10 @@ double* dev_scalars(int slot);
 // Copies n doubles from device to the pinned buffer and waits for them.
 const double* read_scalars(const double* dev, int n);
 
+// With FELGPU_TRACE_COPIES=1, notes a copy or transfer on stderr with the
+// entry point and argument it serves (a diagnostic).
+void trace_event(const char* kind, size_t bytes);
+
 // Launch event timer used by the benchmark (felgpu_prof_*), for
 // hipExtLaunchKernelGGL, which stamps an event with its launch's own start
 // or end (no marker packets between launches).  One timed span covers

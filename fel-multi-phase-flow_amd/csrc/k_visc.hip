@@ -492,6 +492,7 @@ void cpy_t(const Dims d, const double* q1, double* q2) {
   Call c("cpy_");
   const T* a = c.in_t<T>(q1, d.n3, "q1");
   T* b = c.out_t<T>(q2, d.n3, "q2");
+  trace_event("array copy", sizeof(T) * (size_t)d.n3);
   FEL_CHECK(hipMemcpyAsync(b, a, sizeof(T) * (size_t)d.n3, hipMemcpyDeviceToDevice, stream()));
   c.finish();
   copy_content_version(q1, q2, d.n3);

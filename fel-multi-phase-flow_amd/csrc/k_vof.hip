@@ -302,17 +302,17 @@ void mthinc_mass(const Dims d, MthincConst mc64, const T* phix, const T* phiy, c
 }
 
 // srcs/summation.f:14-50 for nbub slices: slice 0 = sum of slices 1..nbub
-// over the padded image, clamped to [0,1] in the interior.
+// over the padded image, clamped to [0,1] in the interior.  One launch per
+// added slice (first: sum = slice 1; last: clamp), which keeps the
+// reference's summation order; the slices need not be adjacent on the device.
 template <class P>
-void summation(const Dims d, P* qk, int nbub) {
-  const int n3 = d.n3;
-  launch_flat(n3, [=] __device__(int m) {
-    P s = qk[n3 + m];
-    for (int l = 2; l <= nbub; l++) s = s + qk[(size_t)l * n3 + m];
+void summation(const Dims d, P* dst, const P* src, bool first, bool last) {
+  launch_flat(d.n3, [=] __device__(int m) {
+    P s = first ? src[m] : dst[m] + src[m];
     const int i = m % d.sx - 2, r = m / d.sx, j = r % d.sy - 2, k = r / d.sy - 2;
-    if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk)
+    if (last && i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk)
       s = fmin(fmax(s, P(0.0)), P(1.0));
-    qk[m] = s;
+    dst[m] = s;
   });
 }
 
@@ -376,7 +376,7 @@ void solphi_mthinc2_t(const Dims d, double dxinv, double dyinv, double dzinv, do
   const P* fy = c.in_t<P>(flphiy, d.n3, "flphiy");
   const P* fz = c.in_t<P>(flphiz, d.n3, "flphiz");
   const P* dphi = c.in_t<P>(phi, d.n3, "phi");
-  P* dphin = c.io_t<P>(phin, d.n3, "phin");
+  P* dphin = c.interior_t<P>(phin, d, "phin");  // written on the interior only
   mthinc_update<T>(d, dxinv, dyinv, dzinv, dt, du, dv, dw, fx, fy, fz, dphi, dphin);
   c.finish();
 }
@@ -459,8 +459,17 @@ void summation_(const int* ni, const int* nj, const int* nk, double* qk, const i
   with_flow([&](auto t) {
     using P = PT<decltype(t)>;
     Call c("summation_");
-    P* dq = c.io_t<P>(qk, (size_t)d.n3 * (*nbub + 1), "qk");
-    summation<P>(d, dq, *nbub);
+    // slice by slice: a caller may register each slice as a range of its own
+    const size_t n = (size_t)d.n3;
+    const int nb = *nbub;
+    if (nb < 1) {
+      std::fprintf(stderr, "felgpu: summation_: nbub = %d, at least one slice is summed\n", nb);
+      std::abort();
+    }
+    std::vector<const P*> src(nb);
+    for (int l = 1; l <= nb; l++) src[l - 1] = c.in_t<P>(qk + (size_t)l * n, n, "qk(l)");
+    P* dst = c.all_t<P>(qk, n, "qk(0)");
+    for (int l = 1; l <= nb; l++) summation<P>(d, dst, src[l - 1], l == 1, l == nb);
     c.finish();
   });
 }

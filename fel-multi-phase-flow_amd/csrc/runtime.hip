@@ -174,13 +174,40 @@ void copy_outside_interior(const Dims& d, void* dst, const void* src, int esize)
   FEL_CHECK(hipGetLastError());
 }
 
+// FELGPU_TRACE_COPIES=1: every deferred copy, sum or fill made, with the
+// entry point and argument that needed it, on stderr (a diagnostic).
+const char* g_trace_call = "";
+const char* g_trace_what = "";
+bool trace_copies() {
+  static const int on = [] {
+    const char* e = std::getenv("FELGPU_TRACE_COPIES");
+    return e && *e == '1' ? 1 : 0;
+  }();
+  return on != 0;
+}
+}  // namespace
+
+void trace_event(const char* kind, size_t bytes) {
+  if (trace_copies())
+    std::fprintf(stderr, "felgpu: %s %zu B, in %s (%s)\n", kind, bytes, g_trace_call, g_trace_what);
+}
+
+namespace {
+void trace_copy(const char* kind, const Range& g, size_t bytes) {
+  if (trace_copies())
+    std::fprintf(stderr, "felgpu: %s %zu B into host range %p, in %s (%s)\n", kind, bytes,
+                 reinterpret_cast<void*>(g.host), g_trace_call, g_trace_what);
+}
+
 void materialize(Range& g) {
   if (g.zero) {
+    trace_copy("zero fill", g, g.bytes / sizeof(double) * (size_t)g.esize);
     FEL_CHECK(hipMemsetAsync(g.dev, 0, g.bytes / sizeof(double) * (size_t)g.esize, R().s));
     g.zero = false;
   }
   if (!g.lazy) return;
   const size_t n = g.bytes / sizeof(double);
+  trace_copy(g.lkind == kAddZero ? "deferred sum" : "deferred copy", g, n * (size_t)g.esize);
   if (g.lkind == kAddZero)
     add_zero(g.dev, g.lazy->dev, n, g.esize);
   else
@@ -262,6 +289,7 @@ namespace {
 // takes the host values with a zero derivative, and gives back its values.
 void h2d(void* dev, const void* host, size_t n, int esize) {
   Runtime& r = R();
+  trace_event("upload", n * (size_t)esize);
   if (esize == 8) {
     FEL_CHECK(hipMemcpyAsync(dev, host, n * 8, hipMemcpyHostToDevice, r.s));
     return;
@@ -283,6 +311,7 @@ void h2d(void* dev, const void* host, size_t n, int esize) {
 
 void d2h(void* host, const void* dev, size_t n, int esize) {
   Runtime& r = R();
+  trace_event("download", n * (size_t)esize);
   if (esize == 8) {
     FEL_CHECK(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, r.s));
     return;
@@ -901,6 +930,9 @@ void deferred_coef_check(const void* host, bool write, bool whole) {
     // are a function of (16 B/cell) instead of materialising 45 arrays
     const size_t dbytes = (size_t)dc.esize * (size_t)dc.d.n3;
     void* snap = scratch(7, dbytes);
+    if (trace_copies())
+      std::fprintf(stderr, "felgpu: viscosity snapshot %zu B, in %s (%s)\n", dbytes, g_trace_call,
+                   g_trace_what);
     FEL_CHECK(hipMemcpyAsync(snap, dc.src_dev, dbytes, hipMemcpyDeviceToDevice, stream()));
     dc.src_dev = snap;
     dc.src_host = nullptr;
@@ -911,7 +943,11 @@ void deferred_coef_check(const void* host, bool write, bool whole) {
 // Call: argument resolution
 // --------------------------------------------------------------------------
 
-Call::Call(const char* name) : name_(name) { ensure_init(); }
+Call::Call(const char* name) : name_(name) {
+  ensure_init();
+  g_trace_call = name;
+  g_trace_what = "";
+}
 
 Call::~Call() {
   if (!finished_) finish();
@@ -919,6 +955,7 @@ Call::~Call() {
 
 void* Call::resolve(const void* host, size_t count, Access a, const char* what, int esize) {
   Runtime& r = R();
+  g_trace_what = what;
   const uintptr_t p = reinterpret_cast<uintptr_t>(host);
   const size_t bytes = count * sizeof(double);
   if (host == nullptr) {
@@ -934,6 +971,7 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
     if (a == kInterior && p == g->host && bytes == g->bytes && g->lazy && g->lkind == kCopy) {
       // the interior is about to be rewritten: only the ghost cells of the
       // deferred copy are made
+      trace_copy("ghost copy", *g, 0);
       copy_outside_interior(box_, g->dev, g->lazy->dev, g->esize);
       drop_lazy(*g);
       materialize_dependents(*g);

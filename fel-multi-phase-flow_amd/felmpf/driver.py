@@ -30,9 +30,6 @@ from .config import RunConfig, initial_state
 
 class GpuRun:
     # registered arrays that hold several slices: name -> (base name, slice)
-    _SLICED = {"phi0": ("phi0", 0), "phi1": ("phi0", 1), "phin0": ("phin0", 0),
-               "phin1": ("phin0", 1)}
-
     def __init__(self, cfg: RunConfig, state: dict[str, np.ndarray] | None = None,
                  precision: int = 64, tangent: bool = False, reuse: bool = True,
                  fp32_modes: bool = False):
@@ -83,12 +80,10 @@ class GpuRun:
         return self.view(name).copy()
 
     def _range(self, name: str):
-        """(host pointer, element count, slice) of the registered array
-        that holds `name`."""
-        base, sl = self._SLICED.get(name, (name, None))
-        v = self.view(base)
-        n = v.size * (2 if sl is not None else 1)
-        return v.ctypes.data, n, sl
+        """(host pointer, element count) of the registered range `name`
+        (each slice of phi and phin is a range of its own)."""
+        v = self.view(name)
+        return v.ctypes.data, v.size
 
     def upload(self, name: str):
         """Host -> device copy after the caller changed view(name)."""
@@ -99,14 +94,9 @@ class GpuRun:
         `direction` (same shape as the field); the values are kept."""
         if not self.tangent_mode:
             raise RuntimeError("seed() needs tangent=True")
-        ptr, n, sl = self._range(name)
-        full = np.zeros(n)
-        if sl is None:
-            full[:] = np.asarray(direction, dtype=np.float64).ravel()
-        else:
-            full = self._tangent_full(ptr, n)
-            m = n // 2
-            full[sl * m:(sl + 1) * m] = np.asarray(direction, dtype=np.float64).ravel()
+        ptr, n = self._range(name)
+        full = np.ascontiguousarray(np.asarray(direction, dtype=np.float64).ravel())
+        assert full.size == n, (name, full.size, n)
         self.gpu.felgpu_tangent_seed_(ctypes.c_void_p(ptr), full.ctypes.data_as(ctypes.c_void_p))
 
     def _tangent_full(self, ptr, n):
@@ -118,12 +108,8 @@ class GpuRun:
         """Tangent mode: the derivative part of field `name`."""
         if not self.tangent_mode:
             raise RuntimeError("tangent() needs tangent=True")
-        ptr, n, sl = self._range(name)
-        full = self._tangent_full(ptr, n)
-        if sl is not None:
-            m = n // 2
-            full = full[sl * m:(sl + 1) * m]
-        return full.reshape(self.cfg.shape)
+        ptr, n = self._range(name)
+        return self._tangent_full(ptr, n).reshape(self.cfg.shape)
 
     def dataou(self, path: str):
         """Writes a checkpoint in the reference's restart format

@@ -295,7 +295,11 @@ contains
     call reg(src_u, n3, 0); call reg(src_v, n3, 0); call reg(src_w, n3, 0)
     call reg(phix, n3, 0); call reg(phiy, n3, 0); call reg(phiz, n3, 0)
     call reg64(flphix, n3, 0); call reg64(flphiy, n3, 0); call reg64(flphiz, n3, 0)
-    call reg64(phi, 2*n3, 1); call reg64(phin, 2*n3, 0)
+    ! each slice of phi and phin its own range, so that the step's slice
+    ! copies (main.f90:695-697) are deferred like the other rotations
+    do m = 0, nbub
+      call reg64(phi(-2,-2,-2,m), n3, 1); call reg64(phin(-2,-2,-2,m), n3, 0)
+    end do
     call reg(s, 6*n3, 0); call reg(tau, 6*n3, 0)
     call reg(rn_pp, 3*n3, 0); call reg(rn_12, 3*n3, 0); call reg(rn_13, 3*n3, 0); call reg(rn_23, 3*n3, 0)
     call reg(rnn_pp, 3*n3, 0); call reg(rnn_12, 3*n3, 0); call reg(rnn_13, 3*n3, 0); call reg(rnn_23, 3*n3, 0)
@@ -587,11 +591,14 @@ contains
   subroutine fel_drv_dataou(path, plen) bind(C, name="fel_drv_dataou")
     character(kind=c_char), intent(in) :: path(*)
     integer(c_int), value :: plen
+    integer :: l
     call felgpu_download_ptr(c_loc(u)); call felgpu_download_ptr(c_loc(v))
     call felgpu_download_ptr(c_loc(w)); call felgpu_download_ptr(c_loc(p))
     call felgpu_download_ptr(c_loc(uo)); call felgpu_download_ptr(c_loc(vo))
     call felgpu_download_ptr(c_loc(wo)); call felgpu_download_ptr(c_loc(po))
-    call felgpu_download_ptr(c_loc(phi))
+    do l = 0, nbub
+      call felgpu_download_ptr(c_loc(phi(-2,-2,-2,l)))
+    end do
     call fel_ckpt_write(path, plen, ni, nj, nk, nbub, nstep, time, u, v, w, p, uo, vo, wo, po, phi)
   end subroutine fel_drv_dataou
 
@@ -603,10 +610,14 @@ contains
   subroutine fel_drv_datain(path, plen) bind(C, name="fel_drv_datain")
     character(kind=c_char), intent(in) :: path(*)
     integer(c_int), value :: plen
+    integer :: l
     call fel_ckpt_read(path, plen, ni, nj, nk, nbub, nstep, time, u, v, w, p, uo, vo, wo, po, phi)
     call felgpu_upload(u); call felgpu_upload(v); call felgpu_upload(w)
     call felgpu_upload(p); call felgpu_upload(uo); call felgpu_upload(vo)
-    call felgpu_upload(wo); call felgpu_upload(po); call felgpu_upload(phi)
+    call felgpu_upload(wo); call felgpu_upload(po)
+    do l = 0, nbub
+      call felgpu_upload(phi(-2,-2,-2,l))
+    end do
     call bndu(nID,ni,nj,nk,u ,v ,w ,c%uwall,c%dy,l1_array,l2_array,phi(-2,-2,-2,1))
     call bndu(nID,ni,nj,nk,uo,vo,wo,c%uwall,c%dy,l1_array,l2_array,phi(-2,-2,-2,1))
     call bnd_periodic(ni,nj,nk,u ); call bnd_periodic(ni,nj,nk,v ); call bnd_periodic(ni,nj,nk,w )
