@@ -993,6 +993,19 @@ void* Call::resolve(const void* host, size_t count, Access a, const char* what, 
     if (a & kWrite) note_write(*g, p, bytes);
     return static_cast<char*>(g->dev) + (p - g->host) / sizeof(double) * (size_t)esize;
   }
+  // an argument that overlaps registered memory without lying inside one
+  // range (e.g. a 4-D array whose slices are registered one by one): a
+  // mirror of the host copy would read stale values
+  {
+    auto it = r.reg.upper_bound(p + bytes - 1);
+    if (it != r.reg.begin() && (--it)->second.host + it->second.bytes > p) {
+      std::fprintf(stderr,
+                   "felgpu: %s: array %s (%zu bytes at %p) overlaps the registered range at %p "
+                   "without lying inside it; pass each registered range on its own\n",
+                   name_, what, bytes, host, reinterpret_cast<void*>(it->second.host));
+      std::abort();
+    }
+  }
   // per-call mirror, in the element type the kernel reads
   const size_t dbytes = count * (size_t)esize;
   void* dev = nullptr;

@@ -350,3 +350,33 @@ def test_deferred_zero_and_force_sums_are_exact():
         assert not np.any(reg.download(dev_sum[1]))
     finally:
         reg.close()
+
+
+@pytest.mark.parametrize("per_slice", [False, True])
+def test_summation_over_registered_slices(per_slice):
+    """summation_ (srcs/summation.f:14-50, nbub = 2) on a 3-slice array
+    registered as one range or slice by slice, after a whole-slice cpy_ into
+    slice 2 (a deferred buffer hand-over when slices are ranges of their
+    own): bitwise the oracle's result on the same values."""
+    g = Grid(12, 6, 10, 0.1, 0.1, 0.1)
+    rng = np.random.default_rng(71)
+    q = rng.uniform(-0.2, 0.7, (3,) + g.shape)
+    extra = rng.uniform(-0.2, 0.7, g.shape)
+    want = q.copy()
+    want[2] = extra
+    H.call_oracle("summation", I(g.ni), I(g.nj), I(g.nk), want, I(2))
+    reg = Registered()
+    try:
+        if per_slice:
+            for m in range(3):
+                reg.add(q[m])
+        else:
+            reg.add(q)
+        reg.add(extra)
+        H.call_gpu("cpy", I(g.ni), I(g.nj), I(g.nk), extra, q[2])
+        H.call_gpu("summation", I(g.ni), I(g.nj), I(g.nk), q, I(2))
+        for m in range(3) if per_slice else [None]:
+            reg.download(q[m] if per_slice else q)
+        assert same_bits(q, want)
+    finally:
+        reg.close()
