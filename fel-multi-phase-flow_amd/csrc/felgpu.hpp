@@ -460,12 +460,31 @@ __global__ void __launch_bounds__(256) sum_stage1(Dims d, F f, A* part) {
 #pragma unroll
   for (int c = 0; c < NV; c++) acc[c] = A(0.0);
   const int nxy = d.ni * d.nj, n = nxy * d.nk;
-  for (int m = (int)(blockIdx.x * 256 + threadIdx.x); m < n; m += (int)(gridDim.x * 256)) {
-    const int k = m / nxy, r = m - k * nxy, j = r / d.ni, i = r - j * d.ni;
+  // thread (b, t) adds the cells m = b*256 + t + q*G, q = 0, 1, ..., in
+  // order; (i, j, k) of m advance by the decomposition of G with carries
+  // instead of two integer divisions per cell
+  const int G = (int)(gridDim.x * 256);
+  const int gk = G / nxy, gr = G - gk * nxy, gj = gr / d.ni, gi = gr - gj * d.ni;
+  int m = (int)(blockIdx.x * 256 + threadIdx.x);
+  int k = m / nxy, i = m - k * nxy, j = i / d.ni;
+  i -= j * d.ni;
+#pragma unroll 4
+  for (; m < n; m += G) {
     A v[NV];
     f(i + 1, j + 1, k + 1, v);
 #pragma unroll
     for (int c = 0; c < NV; c++) acc[c] = acc[c] + v[c];
+    i += gi;
+    j += gj;
+    k += gk;
+    if (i >= d.ni) {
+      i -= d.ni;
+      j++;
+    }
+    if (j >= d.nj) {
+      j -= d.nj;
+      k++;
+    }
   }
   const int t = (int)threadIdx.x;
 #pragma unroll

@@ -152,26 +152,23 @@ void add_zero(void* dst, const void* src, size_t n, int esize) {
   FEL_CHECK(hipGetLastError());
 }
 
-// Copies the cells of a padded array outside the interior [1,n]^3.
+// Copies the cells of a padded array outside the interior [1,n]^3: one
+// thread per ghost cell (launch_shell), B bytes each.
 template <int B>
-__global__ void __launch_bounds__(256) copy_outside_kernel(Dims d, char* dst, const char* src) {
-  const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (m >= d.n3) return;
-  const int i = m % d.sx - 2, r = m / d.sx, j = r % d.sy - 2, k = r / d.sy - 2;
-  if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk) return;
-  const size_t o = (size_t)m * B;
-  for (int b = 0; b < B; b += 4)
-    *reinterpret_cast<uint32_t*>(dst + o + b) = *reinterpret_cast<const uint32_t*>(src + o + b);
+void copy_outside_t(const Dims& d, char* dst, const char* src) {
+  launch_shell(d, [=] __device__(int i, int j, int k) {
+    const size_t o = (size_t)d.ix(i, j, k) * B;
+    for (int b = 0; b < B; b += 4)
+      *reinterpret_cast<uint32_t*>(dst + o + b) = *reinterpret_cast<const uint32_t*>(src + o + b);
+  });
 }
 
 void copy_outside_interior(const Dims& d, void* dst, const void* src, int esize) {
-  const unsigned nb = (unsigned)((d.n3 + 255) / 256);
   char* o = static_cast<char*>(dst);
   const char* i = static_cast<const char*>(src);
-  if (esize == 16) copy_outside_kernel<16><<<nb, 256, 0, R().s>>>(d, o, i);
-  else if (esize == 4) copy_outside_kernel<4><<<nb, 256, 0, R().s>>>(d, o, i);
-  else copy_outside_kernel<8><<<nb, 256, 0, R().s>>>(d, o, i);
-  FEL_CHECK(hipGetLastError());
+  if (esize == 16) copy_outside_t<16>(d, o, i);
+  else if (esize == 4) copy_outside_t<4>(d, o, i);
+  else copy_outside_t<8>(d, o, i);
 }
 
 // FELGPU_TRACE_COPIES=1: every deferred copy, sum or fill made, with the
