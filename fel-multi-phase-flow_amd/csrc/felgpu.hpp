@@ -285,6 +285,7 @@ void* exchange_device_buffer(const void* host, size_t count, void* buf);
 // Free the result stashes (called when all arrays are unregistered).
 void weno_release();
 void rn_release();
+void dense_src_release();
 
 // A deferred producer: the 45 viscous coefficient arrays of
 // cal_arith_coef_vis_ are a pure function of the viscosity.  When all of

@@ -29,7 +29,47 @@ struct SrcArgs {
   // bit m set: the m-th force sum (fst_u, fst_v, fst_w, fst_un, fst_vn,
   // fst_wn) points at the force it is 0.0 + of, element by element
   int addz;
+  // when not null: the sources also stored pair by pair for the
+  // Gauss-Seidel colour pairs (component q, pair p at
+  // dense + (4 q + p) r00_slab + r00_index)
+  T* dense;
 };
+
+// Layout of values stored pair by pair for the Gauss-Seidel colour pairs
+// (srcs/solu_sor4.f:94-120).  Dense index of a pair's cell: ii-th cell of its
+// parity in row jj of its parity, plane k (every cell (i,j,k) of a pair has
+// ii = (i-1)/2, jj = (j-1)/2).  One pair's slab holds
+// nk * ceil(nj/2) * ceil(ni/2) values.
+__host__ __device__ __forceinline__ int r00_index(const Dims& d, int ii, int jj, int k) {
+  const int hx = (d.ni + 1) / 2, hy = (d.nj + 1) / 2;
+  return ((k - 1) * hy + jj) * hx + ii;
+}
+__host__ __device__ __forceinline__ size_t r00_slab(const Dims& d) {
+  return (size_t)d.nk * ((d.nj + 1) / 2) * ((d.ni + 1) / 2);
+}
+
+// The colour pair (0..3) that cell (i,j,k) belongs to.
+__host__ __device__ __forceinline__ int pair_of(int i, int j, int k) {
+  const int ci = 2 - (i & 1), cj = 2 - (j & 1), ck = 2 - (k & 1);  // 1 odd, 2 even
+  // pairs (111,222), (112,221), (121,212), (211,122): one parity differs
+  // from the other two, or none does
+  if (ci == cj && cj == ck) return 0;
+  if (ci == cj) return 1;
+  if (ci == ck) return 2;
+  return 3;
+}
+
+// The pair-by-pair copy of the last cal_srcu_ sources: valid for a
+// solu_sor4_ whose source arrays still carry the content numbers recorded
+// here (the arrays are registered and unchanged since).
+struct DenseSrc {
+  bool valid = false;
+  const void* host[3] = {nullptr, nullptr, nullptr};
+  uint64_t key[3] = {0, 0, 0};
+  int ni = 0, nj = 0, nk = 0, esize = 0;
+  void* buf = nullptr;
+};
+DenseSrc& dense_src();
 
 template <class T>
 struct VisScales {

@@ -35,6 +35,9 @@ struct SorArgs {
   // so a launch reads its N/4 values densely; the launch's pointer is
   // offset to its pair
   const T* r00[3];
+  // the sources stored the same way when cal_srcu_ made them (else null;
+  // the launch's pointer is offset to its pair)
+  const T* srcd[3];
   const T* cf[45];
   T *un, *vn, *wn;
   T dtinv;
@@ -126,27 +129,6 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
                                               int ks, int jend, int gx, int gy, int ntiles,
                                               double* part);
 
-// Dense index of a pair's cell: ii-th cell of its parity in row jj of its
-// parity, plane k (every cell (i,j,k) of a pair has ii = (i-1)/2,
-// jj = (j-1)/2).  One pair's slab holds nk * ceil(nj/2) * ceil(ni/2) values.
-__host__ __device__ __forceinline__ int r00_index(const Dims& d, int ii, int jj, int k) {
-  const int hx = (d.ni + 1) / 2, hy = (d.nj + 1) / 2;
-  return ((k - 1) * hy + jj) * hx + ii;
-}
-__host__ __device__ __forceinline__ size_t r00_slab(const Dims& d) {
-  return (size_t)d.nk * ((d.nj + 1) / 2) * ((d.ni + 1) / 2);
-}
-
-// The colour pair (0..3) that cell (i,j,k) belongs to.
-__host__ __device__ __forceinline__ int pair_of(int i, int j, int k) {
-  const int ci = 2 - (i & 1), cj = 2 - (j & 1), ck = 2 - (k & 1);  // 1 odd, 2 even
-  // pairs (111,222), (112,221), (121,212), (211,122): one parity differs
-  // from the other two, or none does
-  if (ci == cj && cj == ck) return 0;
-  if (ci == cj) return 1;
-  if (ci == ck) return 2;
-  return 3;
-}
 
 template <int COMP, bool ONFLY, bool FUSE, class T>
 __global__ void __launch_bounds__(512) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
@@ -182,8 +164,10 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
     }
     const T *un = a.un, *vn = a.vn, *wn = a.wn;
     const T h = T(0.5);
-    const T r00 = a.r00[COMP][r00_index(d, bx * 64 + (int)threadIdx.x,
-                                        by * (int)blockDim.y + (int)threadIdx.y, k)];
+    const int ridx = r00_index(d, bx * 64 + (int)threadIdx.x,
+                               by * (int)blockDim.y + (int)threadIdx.y, k);
+    const T r00 = a.r00[COMP][ridx];
+    const T srcv = a.srcd[COMP] ? a.srcd[COMP][ridx] : a.src[COMP][c];
     if (COMP == 0) {
       const T visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
                            cf[1] * un[c + X] + cf[3] * un[c + Y] + cf[5] * un[c + Z] +
@@ -191,7 +175,7 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
                            cf[10] * vn[c + X] + cf[11] * wn[c - Z] + cf[12] * wn[c + X - Z] +
                            cf[13] * wn[c] + cf[14] * wn[c + X]) /
                           r00;
-      const T du = (visn * h + a.src[0][c]) / (a.dtinv + cf[6] / r00 * h) - un[c];
+      const T du = (visn * h + srcv) / (a.dtinv + cf[6] / r00 * h) - un[c];
       const T nv = un[c] + du;
       a.un[c] = nv;
       if (FUSE) write_images<0>(d, a, a.un, i, j, k, nv);
@@ -203,7 +187,7 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
                            cf[10] * un[c + Y] + cf[11] * wn[c - Z] + cf[12] * wn[c + Y - Z] +
                            cf[13] * wn[c] + cf[14] * wn[c + Y]) /
                           r00;
-      const T dv = (visn * h + a.src[1][c]) / (a.dtinv + cf[6] / r00 * h) - vn[c];
+      const T dv = (visn * h + srcv) / (a.dtinv + cf[6] / r00 * h) - vn[c];
       const T nv = vn[c] + dv;
       a.vn[c] = nv;
       if (FUSE) write_images<1>(d, a, a.vn, i, j, k, nv);
@@ -215,7 +199,7 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
                            cf[10] * un[c + Z] + cf[11] * vn[c - Y] + cf[13] * vn[c] +
                            cf[12] * vn[c - Y + Z] + cf[14] * vn[c + Z]) /
                           r00;
-      const T dw = (visn * h + a.src[2][c]) / (a.dtinv + cf[6] / r00 * h) - wn[c];
+      const T dw = (visn * h + srcv) / (a.dtinv + cf[6] / r00 * h) - wn[c];
       const T nv = wn[c] + dw;
       a.wn[c] = nv;
       if (FUSE) write_images<2>(d, a, a.wn, i, j, k, nv);
@@ -283,6 +267,7 @@ void launch_pair(bool onfly, bool fuse, dim3 grid, dim3 block, const Dims& d, co
   hipStream_t s = stream();
   SorArgs<T> a_ = a;  // the face densities of this pair
   a_.r00[COMP] = a.r00[COMP] + (size_t)p * r00_slab(d);
+  if (a.srcd[COMP]) a_.srcd[COMP] = a.srcd[COMP] + (size_t)p * r00_slab(d);
   const SorArgs<T>& ap = a_;
   auto go = [&](auto kernel) {
     hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, d, ap, kIs[p], kJs[p], kKs[p],
@@ -432,6 +417,17 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
   a.src[0] = c.in_t<T>(src_u, d.n3, "src_u");
   a.src[1] = c.in_t<T>(src_v, d.n3, "src_v");
   a.src[2] = c.in_t<T>(src_w, d.n3, "src_w");
+  {
+    // the pair-by-pair sources of cal_srcu_, if they are still these arrays'
+    const DenseSrc& ds = dense_src();
+    const void* sh[3] = {src_u, src_v, src_w};
+    bool ok = ds.valid && ds.ni == d.ni && ds.nj == d.nj && ds.nk == d.nk &&
+              ds.esize == (int)sizeof(T);
+    for (int m = 0; m < 3 && ok; m++)
+      ok = ds.host[m] == sh[m] && ds.key[m] == content_key(sh[m], d.n3);
+    for (int m = 0; m < 3; m++)
+      a.srcd[m] = ok ? static_cast<const T*>(ds.buf) + (size_t)4 * m * r00_slab(d) : nullptr;
+  }
   a.un = c.io_t<T>(un, d.n3, "un");
   a.vn = c.io_t<T>(vn, d.n3, "vn");
   a.wn = c.io_t<T>(wn, d.n3, "wn");

@@ -196,20 +196,28 @@ void srcu(const Dims d, int jen, double dtinv_, double grvb_, double grvp_, SrcA
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
     auto sum = [&](const T* f, int bit) { return (a.addz & bit) ? T(0.0) + f[c] : f[c]; };
+    const size_t slab4 = 4 * r00_slab(d);
+    const size_t o = pair_of(i, j, k) * r00_slab(d) + r00_index(d, (i - 1) >> 1, (j - 1) >> 1, k);
     {
       const T r = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q;
-      a.src_u[c] = +a.u[c] * dtinv - a.adv_u[c] * t15 + a.adv_uo[c] * h - a.prs_u[c] / r +
-                   a.vis_u[c] / r * h + sum(a.fst_u, 1) * h + sum(a.fst_un, 8) * h + grvb;
+      const T s = +a.u[c] * dtinv - a.adv_u[c] * t15 + a.adv_uo[c] * h - a.prs_u[c] / r +
+                  a.vis_u[c] / r * h + sum(a.fst_u, 1) * h + sum(a.fst_un, 8) * h + grvb;
+      a.src_u[c] = s;
+      if (a.dense) a.dense[o] = s;
     }
     if (j <= jen) {
       const T r = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q;
-      a.src_v[c] = +a.v[c] * dtinv - a.adv_v[c] * t15 + a.adv_vo[c] * h - a.prs_v[c] / r +
-                   a.vis_v[c] / r * h + sum(a.fst_v, 2) * h + sum(a.fst_vn, 16) * h + grvp;
+      const T s = +a.v[c] * dtinv - a.adv_v[c] * t15 + a.adv_vo[c] * h - a.prs_v[c] / r +
+                  a.vis_v[c] / r * h + sum(a.fst_v, 2) * h + sum(a.fst_vn, 16) * h + grvp;
+      a.src_v[c] = s;
+      if (a.dense) a.dense[slab4 + o] = s;
     }
     {
       const T r = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q;
-      a.src_w[c] = +a.w[c] * dtinv - a.adv_w[c] * t15 + a.adv_wo[c] * h - a.prs_w[c] / r +
-                   a.vis_w[c] / r * h + sum(a.fst_w, 4) * h + sum(a.fst_wn, 32) * h;
+      const T s = +a.w[c] * dtinv - a.adv_w[c] * t15 + a.adv_wo[c] * h - a.prs_w[c] / r +
+                  a.vis_w[c] / r * h + sum(a.fst_w, 4) * h + sum(a.fst_wn, 32) * h;
+      a.src_w[c] = s;
+      if (a.dense) a.dense[2 * slab4 + o] = s;
     }
   });
 }
@@ -442,9 +450,34 @@ void cal_srcu_t(const Dims d, int jen, double dt, double grvb, double grvp, cons
   a.src_u = c.io_t<T>(out[0], n, "src_u");
   a.src_v = c.io_t<T>(out[1], n, "src_v");
   a.src_w = c.io_t<T>(out[2], n, "src_w");
+  // registered sources are also stored pair by pair for solu_sor4_
+  DenseSrc& ds = dense_src();
+  ds.valid = false;
+  bool reg = true;
+  for (int m = 0; m < 3; m++) reg = reg && content_key(out[m], n) != 0;
+  a.dense = reg ? static_cast<T*>(scratch(17, sizeof(T) * 12 * r00_slab(d))) : nullptr;
   srcu<T>(d, jen, 1.0 / dt, grvb, grvp, a);
   c.finish();
+  if (reg) {
+    ds.valid = true;
+    for (int m = 0; m < 3; m++) {
+      ds.host[m] = out[m];
+      ds.key[m] = content_key(out[m], n);
+    }
+    ds.ni = d.ni;
+    ds.nj = d.nj;
+    ds.nk = d.nk;
+    ds.esize = (int)sizeof(T);
+    ds.buf = a.dense;
+  }
 }
+
+DenseSrc& dense_src() {
+  static DenseSrc s;
+  return s;
+}
+
+void dense_src_release() { dense_src() = DenseSrc{}; }
 
 template <class T>
 void cal_arith_coef_vis_t(const Dims d, double dxinv, double dyinv, double dzinv, const double* rmuk,

@@ -1298,6 +1298,7 @@ void felgpu_unregister_all_() {
   FEL_CHECK(hipStreamSynchronize(r.s));
   weno_release();
   rn_release();
+  dense_src_release();
   memo_release();
   for (auto& kv : r.reg) FEL_CHECK(hipFree(kv.second.dev));
   r.reg.clear();

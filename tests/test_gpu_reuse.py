@@ -380,3 +380,53 @@ def test_summation_over_registered_slices(per_slice):
         assert same_bits(q, want)
     finally:
         reg.close()
+
+
+@pytest.mark.parametrize("rewrite", [False, True])
+def test_pair_dense_sources_equal_the_arrays(rewrite):
+    """cal_srcu_ into registered source arrays also stores the sources pair by
+    pair, and solu_sor4_ reads that copy while the arrays keep its content
+    numbers: the solve equals the one on unregistered arrays bit for bit.
+    With rewrite, the host changes src_u after cal_srcu_ (felgpu_upload_):
+    the copy is then stale and the solve must read the new values."""
+    g = Grid(16, 8, 12, 1 / 16, 1 / 8, 1 / 12)
+    r = np.random.default_rng(81)
+    ins = [r.standard_normal(g.shape) for _ in range(3)] + \
+        [r.uniform(0.5, 2.0, g.shape) for _ in range(2)] + \
+        [0.1 * r.standard_normal(g.shape) for _ in range(18)]
+    head = [NID.copy(), I(g.ni), I(g.nj), I(g.nk), D(0.9), D(0.003), D(0.1), D(-0.2)]
+    rmu = r.uniform(0.01, 0.2, g.shape)
+    coef = [g.zeros() for _ in range(45)]
+    H.call_oracle("cal_arith_coef_vis", I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), D(1 / g.dy),
+                  D(1 / g.dz), rmu, *coef)
+    uvw0 = [H.smooth_vel(g, r) for _ in range(3)]
+    phi = r.uniform(0, 1, g.shape)
+    l1 = np.full(g.shape, 0.05)
+
+    def solve(src):
+        uvw = [x.copy() for x in uvw0]
+        H.call_gpu("solu_sor4", I(0), I(0), NID.copy(), I(1), I(g.ni), I(g.nj), I(g.nk),
+                   np.zeros(8, np.int32), np.zeros(8), np.zeros(8), I(1), I(0), D(0.003),
+                   ins[3].copy(), ins[4].copy(), *[c.copy() for c in coef], *src, *uvw,
+                   D(0.25), D(g.dy), l1.copy(), l1.copy(), phi.copy())
+        return uvw
+
+    # unregistered sources: the solve reads the arrays themselves
+    plain = [g.zeros() for _ in range(3)]
+    H.call_gpu("cal_srcu", *head, *[x.copy() for x in ins], *plain)
+    if rewrite:
+        plain[0] *= 0.5
+    want = solve(plain)
+    reg = Registered()
+    try:
+        src = [reg.add(g.zeros()) for _ in range(3)]
+        H.call_gpu("cal_srcu", *head, *[x.copy() for x in ins], *src)
+        if rewrite:
+            reg.download(src[0])
+            src[0] *= 0.5
+            H.gpu().felgpu_upload_(ctypes.c_void_p(src[0].ctypes.data))
+        got = solve(src)
+        for a, b in zip(got, want):
+            assert same_bits(a, b)
+    finally:
+        reg.close()
