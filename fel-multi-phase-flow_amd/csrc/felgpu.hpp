@@ -286,6 +286,8 @@ void* exchange_device_buffer(const void* host, size_t count, void* buf);
 void weno_release();
 void rn_release();
 void dense_src_release();
+// Counts a call answered from a pass of an earlier call (felgpu_stash_hits_).
+void count_stash_hit();
 
 // A deferred producer: the 45 viscous coefficient arrays of
 // cal_arith_coef_vis_ are a pure function of the viscosity.  When all of

@@ -209,6 +209,149 @@ void calrn4(const Dims d, double verysmall, const T* px, const T* py, const T* p
   FEL_CHECK(hipGetLastError());
 }
 
+// The four staggered normals of calrn4_kernel and, from them, the
+// curvature of srcs/calrkap.f:42-98 in one pass.  A 64 x 8 block covers
+// 62 x 6 target columns plus a one-column / one-row halo on each side: the
+// halo threads compute the normals their neighbours' curvature reads, and
+// the six x / y neighbour values go through LDS.  Marching in k, the
+// curvature of plane kk-1 is formed once the normals of plane kk (its +z
+// neighbour) exist.  The normals are calrn4_kernel's values and the
+// curvature is calrkap's expression on them, so the outputs equal those of
+// the two separate passes bit for bit.
+constexpr int kKapTx = 62, kKapTy = 6;
+
+template <class T>
+__global__ void __launch_bounds__(512) calrn4_kap_kernel(
+    Dims d, T verysmall, T dxinv, T dyinv, T dzinv, const T* __restrict__ px,
+    const T* __restrict__ py, const T* __restrict__ pz, T* __restrict__ r0, T* __restrict__ r1,
+    T* __restrict__ r2, T* __restrict__ r3, T* __restrict__ rk, int gx, int gy) {
+  // [plane parity][rpp x, r12 x, r13 x, rpp y, r12 y, r23 y][row][lane]
+  __shared__ T xs[2][6][kKapTy + 2][64];
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
+  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const int ii = -3 + bx * kKapTx + tx;
+  const int jj = -3 + by * kKapTy + ty;
+  const bool vt = ii >= -2 && ii <= ni + 3 && jj >= -2 && jj <= nj + 3;
+  const bool own = vt && tx >= 1 && tx <= kKapTx && ty >= 1 && ty <= kKapTy;
+  const int n3 = d.n3, X = 1, Y = d.sx;
+  const int k0 = -2 + bz * kRnChunk, k1 = min(nk + 3, k0 + kRnChunk - 1);
+  const double flat2 = (val(verysmall) * 1.0e20) * (val(verysmall) * 1.0e20);
+  const bool xm = ii >= -1, ym = jj >= -1;
+  T* out[4] = {r0, r1, r2, r3};
+  const T* a3[3] = {px, py, pz};
+  T V[3][2][2][2];
+#pragma unroll
+  for (int m = 0; m < 3; m++)
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) V[m][a][b][0] = V[m][a][b][1] = T(0.0);
+  auto load_plane = [&](int kk) {
+    const int c = d.ix(ii, jj, kk);
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const T* p = a3[m];
+      V[m][1][1][1] = p[c];
+      V[m][0][1][1] = xm ? p[c - X] : T(0.0);
+      V[m][1][0][1] = ym ? p[c - Y] : T(0.0);
+      V[m][0][0][1] = (xm && ym) ? p[c - X - Y] : T(0.0);
+    }
+  };
+  // the gradient plane below the first normal plane (k0 - 1)
+  if (vt && k0 - 2 >= -2) load_plane(k0 - 2);
+  // own normals of the previous plane (kk-1) that its curvature reads, and
+  // the z components of rn_13, rn_23 two planes back (kk-2)
+  T o_pp0 = T(0.0), o_pp1 = T(0.0), o_pp2 = T(0.0), o_120 = T(0.0), o_121 = T(0.0);
+  T o_132 = T(0.0), o_130 = T(0.0), o_231 = T(0.0), o_232 = T(0.0);
+  T z_132 = T(0.0), z_232 = T(0.0);
+  for (int kk = k0 - 1; kk <= k1 + 1; kk++) {
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) V[m][a][b][0] = V[m][a][b][1];
+    const bool vp = vt && kk >= -2 && kk <= nk + 3;
+    if (vp) load_plane(kk);
+    T nr[4][3];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int ip = kStag[s][0], jp = kStag[s][1], kp = kStag[s][2];
+      nr[s][0] = nr[s][1] = nr[s][2] = T(0.0);
+      if (!vp || ii < -2 + ip || jj < -2 + jp || kk < -2 + kp) continue;
+      const int ai = 1 - ip, bj = 1 - jp, cp = 1 - kp;
+      T r[3];
+#pragma unroll
+      for (int m = 0; m < 3; m++)
+        r[m] = (+V[m][ai][bj][cp] + V[m][1][bj][cp] + V[m][ai][1][cp] + V[m][1][1][cp] +
+                V[m][ai][bj][1] + V[m][1][bj][1] + V[m][ai][1][1] + V[m][1][1][1]) *
+               T(0.125);
+      flat_zero(r, flat2);
+      const T arn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + verysmall;
+      const T inv = T(1.0) / arn;
+      nr[s][0] = r[0] * inv;
+      nr[s][1] = r[1] * inv;
+      nr[s][2] = r[2] * inv;
+    }
+    if (own && kk >= k0 && kk <= k1) {
+      const int tgt = d.ix(ii, jj, kk);
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        out[s][tgt] = nr[s][0];
+        out[s][n3 + tgt] = nr[s][1];
+        out[s][2 * n3 + tgt] = nr[s][2];
+      }
+    }
+    const int w = kk & 1;
+    xs[w][0][ty][tx] = nr[0][0];
+    xs[w][1][ty][tx] = nr[1][0];
+    xs[w][2][ty][tx] = nr[2][0];
+    xs[w][3][ty][tx] = nr[0][1];
+    xs[w][4][ty][tx] = nr[1][1];
+    xs[w][5][ty][tx] = nr[3][1];
+    __syncthreads();
+    // curvature of plane kc = kk - 1 (srcs/calrkap.f:42-98)
+    const int kc = kk - 1;
+    if (own && kc >= k0 && kc <= k1) {
+      const int q = kc & 1;
+      T a = T(0.0), b = T(0.0), e = T(0.0);
+      if (kc >= -1 && jj >= -1 && ii <= ni + 2)
+        a = +(-o_pp0 + xs[q][0][ty][tx + 1]) * dxinv + (-xs[q][4][ty - 1][tx] + o_121) * dyinv +
+            (-z_132 + o_132) * dzinv;
+      if (kc >= -1 && jj <= nj + 2 && ii >= -1)
+        b = +(-xs[q][1][ty][tx - 1] + o_120) * dxinv + (-o_pp1 + xs[q][3][ty + 1][tx]) * dyinv +
+            (-z_232 + o_232) * dzinv;
+      if (kc <= nk + 2 && jj >= -1 && ii >= -1)
+        e = +(-xs[q][2][ty][tx - 1] + o_130) * dxinv + (-xs[q][5][ty - 1][tx] + o_231) * dyinv +
+            (-o_pp2 + nr[0][2]) * dzinv;
+      const int c = d.ix(ii, jj, kc);
+      rk[c] = a;
+      rk[n3 + c] = b;
+      rk[2 * n3 + c] = e;
+    }
+    z_132 = o_132;
+    z_232 = o_232;
+    o_pp0 = nr[0][0]; o_pp1 = nr[0][1]; o_pp2 = nr[0][2];
+    o_120 = nr[1][0]; o_121 = nr[1][1];
+    o_130 = nr[2][0]; o_132 = nr[2][2];
+    o_231 = nr[3][1]; o_232 = nr[3][2];
+    __syncthreads();  // the buffer of plane kk-1 is rewritten for plane kk+1
+  }
+}
+
+template <class T>
+void calrn4_kap(const Dims d, double verysmall, double dxinv, double dyinv, double dzinv,
+                const T* px, const T* py, const T* pz, T* const* rn, T* rk) {
+  const int gx = (d.sx + kKapTx - 1) / kKapTx, gy = (d.sy + kKapTy - 1) / kKapTy;
+  const int gz = (d.sz + kRnChunk - 1) / kRnChunk;
+  calrn4_kap_kernel<T><<<dim3(gx * gy * gz), dim3(64, kKapTy + 2, 1), 0, stream()>>>(
+      d, T(verysmall), T(dxinv), T(dyinv), T(dzinv), px, py, pz, rn[0], rn[1], rn[2], rn[3], rk,
+      gx, gy);
+  FEL_CHECK(hipGetLastError());
+}
+
 // Normals of the other three staggerings, computed by the pass that
 // answered the first request and parked until they are asked for: a request
 // whose inputs carry the same content versions gets the parked buffer
@@ -221,6 +364,14 @@ struct RnStash {
   double vs = 0.0;
   void* buf[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t bytes = 0;
+  // the curvature of the four normals, formed by the same pass
+  // (calrn4_kap): valid for a calrkap_ request with these scales whose four
+  // inputs still hold the normals as delivered (host array, content number)
+  bool kap_valid = false;
+  double kdx = 0.0, kdy = 0.0, kdz = 0.0;
+  void* kap = nullptr;
+  const void* dhost[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t dver[4] = {0, 0, 0, 0};
 };
 RnStash g_rn;
 
@@ -234,6 +385,7 @@ int stagger_index(int ip, int jp, int kp) {
 void rn_release() {
   for (int s = 0; s < 4; s++)
     if (g_rn.buf[s]) FEL_CHECK(hipFree(g_rn.buf[s]));
+  if (g_rn.kap) FEL_CHECK(hipFree(g_rn.kap));
   g_rn = RnStash{};
 }
 
@@ -311,8 +463,8 @@ void cal_grad_p2uvw_t(const Dims d, int ID, int svall, double dxinv, const doubl
 }
 
 template <class T>
-void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix, const double* phiy,
-             const double* phiz, double* rn) {
+void calrn_t(const Dims d, double vs, double dxinv, double dyinv, double dzinv, int ip, int jp,
+             int kp, const double* phix, const double* phiy, const double* phiz, double* rn) {
   const size_t n = (size_t)d.n3, n3x = 3 * n;
   MemoKey k(kOpCalrn);
   k.num((int64_t)sizeof(T)).dims(d).num(vs).num((int64_t)ip).num((int64_t)jp).num((int64_t)kp);
@@ -333,6 +485,9 @@ void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix
       c.all_t<T>(rn, n3x, "rn");
       c.finish();
       memo_record(k, {{rn, n3x}});
+      g_rn.dhost[sreq] = rn;
+      g_rn.dver[sreq] = content_version(rn, n3x);
+      count_stash_hit();
       return;
     }
     const size_t bytes = n3x * sizeof(T);
@@ -341,6 +496,8 @@ void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix
         if (g_rn.buf[s]) FEL_CHECK(hipFree(g_rn.buf[s]));
         FEL_CHECK(hipMalloc(&g_rn.buf[s], bytes));
       }
+      if (g_rn.kap) FEL_CHECK(hipFree(g_rn.kap));
+      g_rn.kap = nullptr;
       g_rn.bytes = bytes;
     }
     Call c("calrn_");
@@ -350,13 +507,28 @@ void calrn_t(const Dims d, double vs, int ip, int jp, int kp, const double* phix
     T* out[4];
     for (int s = 0; s < 4; s++)
       out[s] = s == sreq ? c.all_t<T>(rn, n3x, "rn") : static_cast<T*>(g_rn.buf[s]);
-    calrn4<T>(d, vs, x, y, z, out);
+    g_rn.kap_valid = false;
+    if constexpr (!is_dual<T>) {
+      // the curvature calrkap_ will ask of these normals, in the same pass
+      if (!g_rn.kap) FEL_CHECK(hipMalloc(&g_rn.kap, bytes));
+      calrn4_kap<T>(d, vs, dxinv, dyinv, dzinv, x, y, z, out, static_cast<T*>(g_rn.kap));
+      g_rn.kap_valid = true;
+      g_rn.kdx = dxinv; g_rn.kdy = dyinv; g_rn.kdz = dzinv;
+    } else {
+      calrn4<T>(d, vs, x, y, z, out);
+    }
     c.finish();
     for (int s = 0; s < 4; s++) g_rn.valid[s] = s != sreq;
     for (int m = 0; m < 3; m++) g_rn.ver[m] = ver[m];
     g_rn.ni = d.ni; g_rn.nj = d.nj; g_rn.nk = d.nk;
     g_rn.vs = vs;
     memo_record(k, {{rn, n3x}});
+    for (int s = 0; s < 4; s++) {
+      g_rn.dhost[s] = nullptr;
+      g_rn.dver[s] = 0;
+    }
+    g_rn.dhost[sreq] = rn;
+    g_rn.dver[sreq] = content_version(rn, n3x);
     return;
   }
   Call c("calrn_");
@@ -377,6 +549,26 @@ void calrkap_t(const Dims d, double dxinv, double dyinv, double dzinv, const dou
   k.num((int64_t)sizeof(T)).dims(d).num(dxinv).num(dyinv).num(dzinv);
   k.content(rn_pp, n).content(rn_12, n).content(rn_13, n).content(rn_23, n);
   if (memo_reuse(k, {{rkap, n}}, true)) return;
+  {
+    // the curvature formed with the normals by calrn4_kap, while the four
+    // inputs hold those normals
+    const void* in4[4] = {rn_pp, rn_12, rn_13, rn_23};
+    bool ok = g_rn.kap_valid && result_reuse() && g_rn.ni == d.ni && g_rn.nj == d.nj &&
+              g_rn.nk == d.nk && g_rn.kdx == dxinv && g_rn.kdy == dyinv && g_rn.kdz == dzinv &&
+              g_rn.bytes == n * sizeof(T) && content_version(rkap, n) != 0;
+    for (int s = 0; s < 4 && ok; s++)
+      ok = g_rn.dhost[s] == in4[s] && g_rn.dver[s] != 0 && content_version(in4[s], n) == g_rn.dver[s];
+    if (ok) {
+      g_rn.kap = exchange_device_buffer(rkap, n, g_rn.kap);
+      g_rn.kap_valid = false;
+      Call c("calrkap_");  // the output changes: a fresh version
+      c.all_t<T>(rkap, n, "rkap");
+      c.finish();
+      memo_record(k, {{rkap, n}});
+      count_stash_hit();
+      return;
+    }
+  }
   Call c("calrkap_");
   const T *a = c.in_t<T>(rn_pp, n, "rn_pp"), *b = c.in_t<T>(rn_12, n, "rn_12");
   const T *e = c.in_t<T>(rn_13, n, "rn_13"), *f = c.in_t<T>(rn_23, n, "rn_23");
@@ -428,7 +620,9 @@ void calrn_(const int* ni, const int* nj, const int* nk, const double* dxinv, co
             const double* phiy, const double* phiz, double* rn) {
   const Dims d = make_dims(*ni, *nj, *nk);
   const double vs = std::sqrt(*dxinv * *dxinv + *dyinv * *dyinv + *dzinv * *dzinv) * 1.0e-30;
-  with_flow([&](auto t) { calrn_t<decltype(t)>(d, vs, *ip, *jp, *kp, phix, phiy, phiz, rn); });
+  with_flow([&](auto t) {
+    calrn_t<decltype(t)>(d, vs, *dxinv, *dyinv, *dzinv, *ip, *jp, *kp, phix, phiy, phiz, rn);
+  });
 }
 
 // srcs/calrkap.f:1-2

@@ -582,6 +582,7 @@ struct MemoEntry {
 std::vector<MemoEntry> g_memo;
 uint64_t g_memo_tick = 0;
 int64_t g_memo_hits = 0;
+int64_t g_stash_hits = 0;
 constexpr size_t kMemoEntries = 64;
 constexpr size_t kParkedMax = (size_t)4 << 30;  // bytes of parked results at most
 size_t g_parked_total = 0;
@@ -1085,6 +1086,8 @@ static void prof_collect() {
   }
 }
 
+void count_stash_hit() { g_stash_hits++; }
+
 }  // namespace fel
 
 using namespace fel;
@@ -1221,6 +1224,9 @@ void felgpu_set_result_reuse_(const int* on) {
 
 // Number of entry-point calls answered from the result memo so far.
 int64_t felgpu_result_reuse_hits_() { return g_memo_hits; }
+
+// Number of entry-point calls answered from a pass of an earlier call.
+int64_t felgpu_stash_hits_() { return g_stash_hits; }
 
 // Sets the derivative part of a whole registered array from the host array
 // `seed` (same number of elements); the values are kept.

@@ -97,6 +97,10 @@ void felgpu_tangent_fetch_(const void *host, double *out);
  * same either way.  felgpu_result_reuse_hits_ counts the calls answered so. */
 void felgpu_set_result_reuse_(const int *on);
 int64_t felgpu_result_reuse_hits_(void);
+/* Calls answered from a result a previous call's pass formed alongside its
+ * own (the other staggered normals of calrn_, the curvature of calrkap_);
+ * no counterpart in the reference, for tests and diagnostics. */
+int64_t felgpu_stash_hits_(void);
 
 /* Host -> device copy of the whole registered range containing `host`. */
 void felgpu_upload_(const void *host);
