@@ -20,6 +20,18 @@ namespace fel {
 // division changes nothing but keeps the degree-8 products of the
 // smoothness indicators inside the FP32 range; the stencil derivatives use
 // the unscaled differences.
+// 1/x for the weights: IEEE division in FP64 (and the tangent mode); in the
+// FP32 mode the hardware reciprocal (v_rcp_f32, 1 ulp), whose rounding is
+// far below the FP32 mode's accuracy bar and which replaces a ten-instruction
+// division sequence in a compute-bound kernel.  The weight sums are floored
+// at 1e-30; the scale is the largest difference magnitude, and the two forms
+// differ only for a scale below the smallest normal float.
+template <class T>
+__device__ __forceinline__ T weight_rcp(T x) {
+  if constexpr (is_f32<T>) return __builtin_amdgcn_rcpf(x);
+  else return T(1.0) / x;
+}
+
 template <class T>
 __device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp5, T vel00) {
   const T c2 = T(2.0), c3 = T(3.0), c4 = T(4.0), c5 = T(5.0), c7 = T(7.0), c11 = T(11.0),
@@ -34,7 +46,7 @@ __device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp
   if constexpr (is_f32<T>) {
     const T sc = fmax(fmax(fmax(fabs(m5), fabs(m3)), fmax(fabs(m1), fabs(p1))),
                       fmax(fabs(p3), fabs(p5)));
-    const T inv = sc > T(0.0) ? T(1.0) / sc : T(0.0);
+    const T inv = sc > T(0.0) ? weight_rcp(sc) : T(0.0);
     m5 = m5 * inv; m3 = m3 * inv; m1 = m1 * inv; p1 = p1 * inv; p3 = p3 * inv; p5 = p5 * inv;
   }
   T a, b;
@@ -68,8 +80,8 @@ __device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp
   am_3 = am_3 * c3;
   ap_2 = ap_2 * T(6.0);
   ap_3 = ap_3 * c3;
-  const T deninv_am = T(1.0) / (am_1 + am_2 + am_3 + guard<T>(1.0e-80));
-  const T deninv_ap = T(1.0) / (ap_1 + ap_2 + ap_3 + guard<T>(1.0e-80));
+  const T deninv_am = weight_rcp(am_1 + am_2 + am_3 + guard<T>(1.0e-80));
+  const T deninv_ap = weight_rcp(ap_1 + ap_2 + ap_3 + guard<T>(1.0e-80));
   const T wgm_1 = am_1 * deninv_am, wgm_3 = am_3 * deninv_am;
   const T wgp_1 = ap_1 * deninv_ap, wgp_3 = ap_3 * deninv_ap;
   const T wgm_2 = T(1.0) - wgm_1 - wgm_3;
