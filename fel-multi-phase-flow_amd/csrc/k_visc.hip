@@ -130,62 +130,79 @@ struct SijTau {
   T dxinv, dyinv, dzinv;
 };
 
+// One thread per (i,j) column of the padded image, marching over a chunk of
+// kSijChunk planes: the column's u, v (at k+1), w (at k) and viscosity (at
+// k+1, i+1 / j+1 beside it) loaded for one plane are the values the next
+// plane reads at k (and w at k-1), so each plane of the inputs is fetched
+// once per column instead of once per plane touching it.  Per cell the
+// operations are calsij's (srcs/calsij.f:28-139, where it writes) and
+// arith_tau's (srcs/cal_arith_tau.f:21-118), in the same order.
+constexpr int kSijChunk = 16;
+
 template <class T>
-__device__ __forceinline__ T sij_at(const Dims& d, const SijTau<T>& a, int m, int i, int j, int k,
-                                    int c, bool& written) {
+__global__ void __launch_bounds__(256) sij_tau_kernel(Dims d, SijTau<T> a, int gx, int gy) {
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, t = L / gx, by = t % gy, bz = t / gy;
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const int i = -2 + bx * 64 + (int)threadIdx.x;
+  const int j = -2 + by * 4 + (int)threadIdx.y;
+  if (i > ni + 3 || j > nj + 3) return;
   const int n3 = d.n3, X = 1, Y = d.sx, Z = d.sx * d.sy;
-  const T *u = a.u, *v = a.v, *w = a.w;
-  switch (m) {
-    case 0:
-      written = i >= -1;
-      return written ? (-u[c - X] + u[c]) * a.dxinv : a.s[c];
-    case 1:
-      written = j >= -1;
-      return written ? (-v[c - Y] + v[c]) * a.dyinv : a.s[n3 + c];
-    case 2:
-      written = i >= -1 && j >= -1 && k >= -1;
-      return written ? (-w[c - Z] + w[c]) * a.dzinv : a.s[2 * n3 + c];
-    case 3:
-      written = i <= d.ni + 2 && j <= d.nj + 2;
-      return written ? (+(-u[c] + u[c + Y]) * a.dyinv + (-v[c] + v[c + X]) * a.dxinv) * T(0.5)
-                     : a.s[3 * n3 + c];
-    case 4:
-      written = i <= d.ni + 2 && k <= d.nk + 2;
-      return written ? (+(-u[c] + u[c + Z]) * a.dzinv + (-w[c] + w[c + X]) * a.dxinv) * T(0.5)
-                     : a.s[4 * n3 + c];
-    default:
-      written = j <= d.nj + 2 && k <= d.nk + 2;
-      return written ? ((-v[c] + v[c + Z]) * a.dzinv + (-w[c] + w[c + Y]) * a.dyinv) * T(0.5)
-                     : a.s[5 * n3 + c];
+  const int k0 = -2 + bz * kSijChunk, k1 = min(nk + 3, k0 + kSijChunk - 1);
+  const T *u = a.u, *v = a.v, *w = a.w, *rmu = a.rmu;
+  const bool xl = i >= -1, xr = i <= ni + 2, yl = j >= -1, yr = j <= nj + 2;
+  const T q = T(0.25), h = T(0.5), two = T(2.0);
+  // the column at plane k0 (and w at k0-1)
+  int c = d.ix(i, j, k0);
+  T u_c = u[c], v_c = v[c], r_c = rmu[c];
+  T r_x = xr ? rmu[c + X] : T(0.0), r_y = yr ? rmu[c + Y] : T(0.0);
+  T w_m = k0 >= -1 ? w[c - Z] : T(0.0);
+  for (int k = k0; k <= k1; k++, c += Z) {
+    const bool kz = k <= nk + 2;
+    const T u_xm = xl ? u[c - X] : T(0.0), u_yp = yr ? u[c + Y] : T(0.0),
+            u_zp = kz ? u[c + Z] : T(0.0);
+    const T v_ym = yl ? v[c - Y] : T(0.0), v_xp = xr ? v[c + X] : T(0.0),
+            v_zp = kz ? v[c + Z] : T(0.0);
+    const T w_c = w[c], w_xp = xr ? w[c + X] : T(0.0), w_yp = yr ? w[c + Y] : T(0.0);
+    const T r_xy = (xr && yr) ? rmu[c + X + Y] : T(0.0), r_z = kz ? rmu[c + Z] : T(0.0);
+    const T r_xz = (xr && kz) ? rmu[c + X + Z] : T(0.0), r_yz = (yr && kz) ? rmu[c + Y + Z] : T(0.0);
+    T sv[6];
+    bool sw[6];
+    sw[0] = xl;
+    sv[0] = sw[0] ? (-u_xm + u_c) * a.dxinv : a.s[c];
+    sw[1] = yl;
+    sv[1] = sw[1] ? (-v_ym + v_c) * a.dyinv : a.s[n3 + c];
+    sw[2] = xl && yl && k >= -1;
+    sv[2] = sw[2] ? (-w_m + w_c) * a.dzinv : a.s[2 * n3 + c];
+    sw[3] = xr && yr;
+    sv[3] = sw[3] ? (+(-u_c + u_yp) * a.dyinv + (-v_c + v_xp) * a.dxinv) * h : a.s[3 * n3 + c];
+    sw[4] = xr && kz;
+    sv[4] = sw[4] ? (+(-u_c + u_zp) * a.dzinv + (-w_c + w_xp) * a.dxinv) * h : a.s[4 * n3 + c];
+    sw[5] = yr && kz;
+    sv[5] = sw[5] ? ((-v_c + v_zp) * a.dzinv + (-w_c + w_yp) * a.dyinv) * h : a.s[5 * n3 + c];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+      if (sw[m]) a.s[(size_t)m * n3 + c] = sv[m];
+#pragma unroll
+    for (int m = 0; m < 3; m++) a.tau[(size_t)m * n3 + c] = sv[m] * r_c * two;
+    // tau where arith_tau writes it: the 4-point viscosity of the edge
+    if (sw[3]) a.tau[3 * (size_t)n3 + c] = sv[3] * (+q * r_c + q * r_x + q * r_y + q * r_xy) * two;
+    if (sw[4]) a.tau[4 * (size_t)n3 + c] = sv[4] * (+q * r_c + q * r_x + q * r_z + q * r_xz) * two;
+    if (sw[5]) a.tau[5 * (size_t)n3 + c] = sv[5] * (+q * r_c + q * r_y + q * r_z + q * r_yz) * two;
+    u_c = u_zp;
+    v_c = v_zp;
+    w_m = w_c;
+    r_c = r_z;
+    r_x = r_xz;
+    r_y = r_yz;
   }
 }
 
 template <class T>
 void sij_tau(const Dims d, SijTau<T> a) {
-  const int ni = d.ni, nj = d.nj, nk = d.nk;
-  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
-    const int n3 = d.n3, c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    const T* rmu = a.rmu;
-    const T r = rmu[c], q = T(0.25);
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-      bool sw;
-      const T sv = sij_at<T>(d, a, m, i, j, k, c, sw);
-      if (sw) a.s[(size_t)m * n3 + c] = sv;
-      if (m < 3) {
-        a.tau[(size_t)m * n3 + c] = sv * r * T(2.0);
-      } else {
-        const bool tw = m == 3 ? (i <= ni + 2 && j <= nj + 2)
-                        : m == 4 ? (i <= ni + 2 && k <= nk + 2)
-                                 : (j <= nj + 2 && k <= nk + 2);
-        if (tw) {
-          const int o1 = m == 5 ? Y : X, o2 = m == 3 ? Y : Z;
-          const T mu = +q * r + q * rmu[c + o1] + q * rmu[c + o2] + q * rmu[c + o1 + o2];
-          a.tau[(size_t)m * n3 + c] = sv * mu * T(2.0);
-        }
-      }
-    }
-  });
+  const int gx = (d.sx + 63) / 64, gy = (d.sy + 3) / 4, gz = (d.sz + kSijChunk - 1) / kSijChunk;
+  sij_tau_kernel<T><<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(d, a, gx, gy);
+  FEL_CHECK(hipGetLastError());
 }
 
 // srcs/cal_srcu.f:59-136
