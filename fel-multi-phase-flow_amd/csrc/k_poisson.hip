@@ -641,8 +641,26 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
     h[ni + nk + (nj + 6) + j] = an_p_fft[j];
     h[ni + nk + 2 * (nj + 6) + j] = ap_p_fft[j];
   }
-  double* tab = static_cast<double*>(scratch(6, sizeof(double) * h.size()));
-  FEL_CHECK(hipMemcpyAsync(tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, stream()));
+  // The device table is uploaded only when it changes (the coefficients are
+  // fixed for a run), so the step has no host synchronisation here.
+  static std::vector<double> up;  // what the device table holds
+  static double* tab = nullptr;
+  static size_t cap = 0;
+  if (h.size() > cap) {
+    if (tab) {
+      FEL_CHECK(hipStreamSynchronize(stream()));
+      FEL_CHECK(hipFree(tab));
+    }
+    FEL_CHECK(hipMalloc(&tab, sizeof(double) * h.size()));
+    cap = h.size();
+    up.clear();
+  }
+  if (h != up) {
+    up = h;
+    FEL_CHECK(hipMemcpyAsync(tab, up.data(), sizeof(double) * up.size(), hipMemcpyHostToDevice,
+                             stream()));
+    FEL_CHECK(hipStreamSynchronize(stream()));
+  }
   const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
   Call c("solp_fft_tdma2_");
   with_elem(elem_size(phiw_r), [&](auto t) {
@@ -652,7 +670,6 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
     tdma_modes<T>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
   });
   c.finish();
-  FEL_CHECK(hipStreamSynchronize(stream()));  // h is released on return
   PoissonRecord& rec = prec();
   rec.coef = true;
   rec.dx2r = dx2r;
