@@ -50,7 +50,7 @@ sys.path.insert(0, os.path.join(REPO, "fel-multi-phase-flow_amd"))
 METRIC = "Mcell-steps/s at 512x256x128 (FP64 vs FP32+refine); adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
 
-# event-timer ids recorded by the library (csrc: prof_begin/prof_end)
+# event-timer ids recorded by the library (csrc: prof_launch_events)
 TIMERS = {3: "sor_pair"}
 
 
