@@ -150,18 +150,23 @@ __global__ void __launch_bounds__(256) fft_x_kernel(int n, int nlines, T sign,
 
 // Strided lines: a block owns TL lines that are adjacent in memory (tile
 // index t = 0..TL-1) and transforms them together; element e of line l is
-// loaded/stored by the functors.
+// loaded/stored by the functors.  512 threads: a tile's two LDS buffers fill
+// 64 KiB, so two blocks share a CU, and 512-thread blocks keep 16 waves per
+// CU in flight for the strided loads (measured: 256-thread blocks reached
+// 2.0-2.6 TB/s).
+constexpr int kZThreads = 512;
+
 template <class T, class Load, class Store>
-__global__ void __launch_bounds__(256) fft_z_kernel(int n, int tiles, int TL, T sign,
+__global__ void __launch_bounds__(kZThreads) fft_z_kernel(int n, int tiles, int TL, T sign,
                                                     const typename Cplx<T>::type* __restrict__ tw,
                                                     Load ld, Store st) {
   using C = typename Cplx<T>::type;
   extern __shared__ unsigned char smem_z[];
   C* sm = reinterpret_cast<C*>(smem_z);
-  const int g = 256 / TL;                 // threads per line
+  const int g = kZThreads / TL;           // threads per line
   const int tile = (int)blockIdx.x;
   // cooperative, row-major loads: consecutive threads take consecutive lines
-  for (int idx = (int)threadIdx.x; idx < TL * n; idx += 256) {
+  for (int idx = (int)threadIdx.x; idx < TL * n; idx += kZThreads) {
     const int e = idx / TL, l = idx - e * TL;
     sm[l * 2 * n + e] = ld(tile, l, e);
   }
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(256) fft_z_kernel(int n, int tiles, int TL, T 
   C* x = sm + l * 2 * n;
   C* r = stockham<T, C>(x, x + n, n, lane, g, tw, sign);
   const int off = (int)(r - x);  // 0 or n, the same for every line
-  for (int idx = (int)threadIdx.x; idx < TL * n; idx += 256) {
+  for (int idx = (int)threadIdx.x; idx < TL * n; idx += kZThreads) {
     const int e = idx / TL, ll = idx - e * TL;
     st(tile, ll, e, sm[ll * 2 * n + off + e]);
   }
@@ -201,8 +206,8 @@ template <class T, class Load, class Store>
 static void fft_z(int n, int tiles, int TL, T sign, Load ld, Store st) {
   using C = typename Cplx<T>::type;
   const size_t shm = sizeof(C) * 2 * n * TL;
-  fft_z_kernel<T, Load, Store><<<tiles, 256, shm, stream()>>>(n, tiles, TL, sign, twiddles<T>(n), ld,
-                                                              st);
+  fft_z_kernel<T, Load, Store><<<tiles, kZThreads, shm, stream()>>>(n, tiles, TL, sign,
+                                                                    twiddles<T>(n), ld, st);
   FEL_CHECK(hipGetLastError());
 }
 
