@@ -32,10 +32,12 @@ __device__ __forceinline__ WallCoef<P> wall_coef(const Dims& d, int i, int jw, i
 // P: element type of the slip lengths and the volume fraction.
 // `comps` (bit q for component q) restricts the writes to those components;
 // a component left out must already hold exactly the values the call would
-// write (its interior unchanged since a refresh that wrote them).
+// write (its interior unchanged since a refresh that wrote them).  `lean`
+// leaves out the deep wall rows (j = -2, -1, nj+2, nj+3), which no stencil
+// of the viscous solve reads; a caller using it rewrites them afterwards.
 template <class T, class P>
 void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const P* l1, const P* l2,
-              const P* phi, bool lo, bool hi, int swept, int comps);
+              const P* phi, bool lo, bool hi, int swept, int comps, bool lean = false);
 // srcs/bnd_periodic.f:4-46 in one in-place pass.
 template <class T>
 void periodic(const Dims d, T* q);

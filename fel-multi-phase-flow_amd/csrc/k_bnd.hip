@@ -228,13 +228,15 @@ __device__ __forceinline__ int pimg(int i, int n) { return i < 1 ? i + n : (i > 
 
 template <class T, class P>
 void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const P* l1, const P* l2,
-              const P* phi, bool lo, bool hi, int swept, int comps) {
+              const P* phi, bool lo, bool hi, int swept, int comps, bool lean) {
   const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, sy = d.sy;
   // disjoint regions covering every cell any component may write:
-  //  A: the six z-ghost planes (all i, j)
-  //  B: interior k, rows j in {-2,-1,0,nj,nj+1,nj+2,nj+3} (all i)
+  //  A: the six z-ghost planes (all i, j; lean: j in 0..nj+1)
+  //  B: interior k, rows j in {-2,-1,0,nj,nj+1,nj+2,nj+3} (lean: {0,nj,nj+1})
+  //     (all i)
   //  C: interior k, rows 1..nj-1, the six x-ghost columns
-  const int na = 6 * sx * sy, nb = 7 * nk * sx, nc = 6 * nk * (nj - 1);
+  const int nrow = lean ? 3 : 7;
+  const int na = 6 * sx * sy, nb = nrow * nk * sx, nc = 6 * nk * (nj - 1);
   launch_flat(na + nb + nc, [=] __device__(int m) {
     int i, j, k;
     if (m < na) {
@@ -243,13 +245,15 @@ void bc_fused(const Dims d, T* u, T* v, T* w, double uwall, double dy, const P* 
       j = r % sy - 2;
       const int g = r / sy;
       k = g < 3 ? g - 2 : nk + g - 2;
+      if (lean && (j < 0 || j > nj + 1)) return;
     } else if (m < na + nb) {
       const int t = m - na;
       i = t % sx - 2;
       const int r = t / sx;
-      const int g = r % 7;
-      k = r / 7 + 1;
-      j = g < 3 ? g - 2 : nj + g - 3;
+      const int g = r % nrow;
+      k = r / nrow + 1;
+      if (lean) j = g == 0 ? 0 : nj + g - 1;
+      else j = g < 3 ? g - 2 : nj + g - 3;
     } else {
       const int t = m - na - nb;
       const int g = t % 6, r = t / 6;
@@ -413,12 +417,12 @@ void contact_angle(const Dims d, P* q, const P* theta, double ax, double az, boo
 
 template void bc_fused<double, double>(const Dims, double*, double*, double*, double, double,
                                        const double*, const double*, const double*, bool, bool,
-                                       int, int);
+                                       int, int, bool);
 template void bc_fused<float, double>(const Dims, float*, float*, float*, double, double,
                                       const double*, const double*, const double*, bool, bool, int,
-                                      int);
+                                      int, bool);
 template void bc_fused<Dual, Dual>(const Dims, Dual*, Dual*, Dual*, double, double, const Dual*,
-                                   const Dual*, const Dual*, bool, bool, int, int);
+                                   const Dual*, const Dual*, bool, bool, int, int, bool);
 template void periodic<double>(const Dims, double*);
 template void periodic<float>(const Dims, float*);
 template void periodic<Dual>(const Dims, Dual*);

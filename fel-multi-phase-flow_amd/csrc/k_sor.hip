@@ -358,8 +358,10 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
           // After the first pair of a component's sweep only that component
           // changes, and bndu's ghosts of the other two are functions of their
           // unchanged interiors: rewriting them would store the same values.
+          // With fused images the deep wall rows, read by no stencil of the
+          // solve, are left to the refreshes after the last sweep.
           bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, comp,
-                         p == 0 ? 7 : (1 << comp));
+                         p == 0 ? 7 : (1 << comp), fuse_ok);
         }
       }
     }
