@@ -1,0 +1,54 @@
+"""The timestep at the benchmark's full size: droplet impact, 512x256x128
+(reference i,j,k = 512x128x256, 16.8 M cells), through properties that hold
+at any size -- phi stays in [0,1], the liquid volume is conserved by the
+mass correction (srcs/solphi_mthinc3.f:101-160) to 1e-12, the projected
+velocity is divergence-free to the Poisson solver's precision
+(srcs/corunp_explicit.f after srcs/solp_fft_tdma1-3.f), every field stays
+finite, and runs with and without result reuse are bitwise equal.  The
+oracle is too slow at this size; tests/test_gpu_step.py holds the GPU path
+to it on smaller grids.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "w", "p", "phi1")
+
+
+def _run(steps, reuse=True):
+    from felmpf import config as C
+    from felmpf.driver import GpuRun
+    from felmpf.longrun import diagnostics
+    cfg = C.droplet_impact()
+    run = GpuRun(cfg, reuse=reuse)
+    try:
+        d0 = diagnostics(cfg, run)
+        run.steps(steps)
+        d = diagnostics(cfg, run)
+        fields = {f: run.field(f) for f in FIELDS}
+        return cfg, d0, d, fields
+    finally:
+        run.close()
+
+
+def test_full_size_step_invariants():
+    cfg, d0, d, fields = _run(5)
+    assert (cfg.ni, cfg.nj, cfg.nk) == (512, 128, 256)
+    phi = fields["phi1"][3:-3, 3:-3, 3:-3]
+    assert phi.min() >= 0.0 and phi.max() <= 1.0
+    assert abs(d["volume"] - d0["volume"]) <= 1e-12 * d0["volume"], (d["volume"], d0["volume"])
+    # max |div u| after the projection; the velocity is of order 1 and
+    # dx = 1/128, so 1e-11 leaves a wide margin over FP64 rounding (~1e-14)
+    assert d["max_div"] <= 1e-11, d["max_div"]
+    for f, a in fields.items():
+        assert np.all(np.isfinite(a)), f
+
+
+def test_full_size_reuse_is_bitwise_neutral():
+    _, _, _, on = _run(2, reuse=True)
+    _, _, _, off = _run(2, reuse=False)
+    for f in FIELDS:
+        assert np.array_equal(on[f].view(np.int64), off[f].view(np.int64)), f
