@@ -21,6 +21,7 @@ are bitwise those of the FP64 run.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -40,6 +41,7 @@ class GpuRun:
         self.cfg = cfg
         self.precision = precision
         self.tangent_mode = bool(tangent)
+        self.reuse = bool(reuse)
         self.drv = _lib.driver()
         self.gpu = _lib.gpu()
         self.drv.fel_drv_field.restype = ctypes.c_void_p
@@ -154,6 +156,9 @@ class GpuRun:
                 self.gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(0)))
             if self.precision != 64:
                 self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(64)))
+            if not self.reuse and os.environ.get("FELGPU_NO_RESULT_REUSE") != "1":
+                # back to the library's default for the calls that follow
+                self.gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(1)))
 
     def __del__(self):
         try:
