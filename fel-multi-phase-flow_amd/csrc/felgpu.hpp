@@ -551,6 +551,11 @@ double* dev_scalars(int slot);
 // Copies n doubles from device to the pinned buffer and waits for them.
 const double* read_scalars(const double* dev, int n);
 
+// Fortran list of edit descriptor Ew.d: v right-justified in w characters
+// as 0.<d digits>E+xx (E dropped for 3-digit exponents), the form the
+// reference's monitor lines print (e.g. srcs/solu_sor4.f:299).  Returns out.
+char* fortran_e(char* out, double v, int w, int d);
+
 // With FELGPU_TRACE_COPIES=1, notes a copy or transfer on stderr with the
 // entry point and argument it serves (a diagnostic).
 void trace_event(const char* kind, size_t bytes);

@@ -207,9 +207,12 @@ void caldt_(const int* ipara, const int* nID, const int* ID, const int* ndiv, co
   step = std::fmin(step, dt3);
   *dt = step;
   *time = *time + step;
-  if (*imon_t > 0 && *nstep % *imon_t == 0 && *ID == 0)
-    std::printf("time=%17.10E dt=%17.10E dt_adv=%17.10E dt_vis=%17.10E dt_srf=%17.10E\n", *time, step,
-                dt1, dt2, dt3);
+  if (*imon_t > 0 && *nstep % *imon_t == 0 && *ID == 0) {
+    char a[5][32];
+    std::printf("time=%s dt=%s dt_adv=%s dt_vis=%s dt_srf=%s\n", fortran_e(a[0], *time, 17, 10),
+                fortran_e(a[1], step, 17, 10), fortran_e(a[2], dt1, 17, 10),
+                fortran_e(a[3], dt2, 17, 10), fortran_e(a[4], dt3, 17, 10));
+  }
 }
 
 }  // extern "C"

@@ -711,7 +711,11 @@ void solp_fft_tdma4_(const int* ipara, const int* ID, const int* ndiv, const int
   });
   const double nn = (double)(*ni * *nj * *nk * *ndiv);
   const double e0 = std::sqrt(r[0] / nn) + 1.0e-99, e = std::sqrt(r[1] / nn) + 1.0e-99;
-  if (*ID == 0) std::printf("Err_solp %10d%20.10E%20.10E%20.10E\n", *nstep, e / e0, e, e0);
+  if (*ID == 0) {
+    char a[3][32];
+    std::printf("Err_solp %10d%s%s%s\n", *nstep, fortran_e(a[0], e / e0, 20, 10),
+                fortran_e(a[1], e, 20, 10), fortran_e(a[2], e0, 20, 10));
+  }
 }
 
 // Relative Poisson residual (the monitor's value), for tests and drivers.

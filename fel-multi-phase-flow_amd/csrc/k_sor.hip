@@ -382,7 +382,8 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
     bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, 1, 2);
     bc_fused<T, P>(d, a.un, a.vn, a.wn, uwall, dy, l1, l2, phi, lo, hi, 2, 7);
   }
-  if (lp > 20) lp = 20;
+  // lp is the loop variable of srcs/solu_sor4.f:131 on exit: the sweep
+  // index that met the tolerance, or 21 after 21 sweeps without it
   res.lp = lp;
   res.err = err;
   res.err0 = err0;
@@ -508,7 +509,11 @@ void solu_sor4_(const int* ipara, const int* ID, const int* nID, const int* ndiv
   g_last_lp = r.lp;
   g_last_ratio = r.err / r.err0;
   if (*imon_t > 0 && (*nstep % *imon_t) == 0 && *ID == 0)
-    std::printf("Err_solu %10d%10d%20.10E%20.10E%20.10E\n", *nstep, r.lp, r.err / r.err0, r.err, r.err0);
+  {
+    char a[3][32];
+    std::printf("Err_solu %10d%10d%s%s%s\n", *nstep, r.lp, fortran_e(a[0], r.err / r.err0, 20, 10),
+                fortran_e(a[1], r.err, 20, 10), fortran_e(a[2], r.err0, 20, 10));
+  }
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 // the same offset of its device copy.  Unregistered pointers are mirrored per
 // call (copy in, run, copy out), so every entry point also works, slowly, on
 // plain host memory.
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <string>
@@ -380,6 +381,34 @@ double* pinned_scalars() {
 double* dev_scalars(int slot) {
   ensure_init();
   return R().dscal + 16 * slot;
+}
+
+char* fortran_e(char* out, double v, int w, int d) {
+  char body[64];
+  if (v == 0.0 || !std::isfinite(v)) {
+    if (v == 0.0)
+      std::snprintf(body, sizeof body, "%s0.%0*dE+00", std::signbit(v) ? "-" : "", d, 0);
+    else
+      std::snprintf(body, sizeof body, "%s", std::isnan(v) ? "NaN" : (v > 0 ? "Infinity" : "-Infinity"));
+  } else {
+    // d significant digits, correctly rounded, then shifted behind "0."
+    char m[64];
+    std::snprintf(m, sizeof m, "%.*E", d - 1, std::fabs(v));
+    const char* e = std::strchr(m, 'E');
+    const int ex = std::atoi(e + 1) + 1;
+    char digits[64];
+    int nd = 0;
+    for (const char* c = m; c < e; c++)
+      if (*c >= '0' && *c <= '9') digits[nd++] = *c;
+    digits[nd] = 0;
+    const char* sg = v < 0 ? "-" : "";
+    if (std::abs(ex) < 100)
+      std::snprintf(body, sizeof body, "%s0.%sE%c%02d", sg, digits, ex < 0 ? '-' : '+', std::abs(ex));
+    else
+      std::snprintf(body, sizeof body, "%s0.%s%c%03d", sg, digits, ex < 0 ? '-' : '+', std::abs(ex));
+  }
+  std::snprintf(out, (size_t)w + 1, "%*s", w, body);
+  return out;
 }
 
 const double* read_scalars(const double* dev, int n) {

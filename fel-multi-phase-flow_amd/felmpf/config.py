@@ -15,6 +15,7 @@ mu_l = L/Re, sigma = L/We (or mu_l U / Ca), g = Bo sigma / L^2 acting along
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass, field, replace
 
 import numpy as np
@@ -160,11 +161,47 @@ def patterned_wall() -> RunConfig:
                      init="cap", params=dict(r=0.4, cx=1.0, cz=1.0))
 
 
+def reference_program(name: str = "reference_program", nsv: int = 8, svall=(32, 8, 32),
+                      xl=17.0, yl=4.25, zl=17.0, rhol=8.1e-1, rhog=8.1e-1, rmul=1.95,
+                      rmug=1.95, surface_tension=5.5, uwall=0.25, l1_a=2.165, theta_0_a=90.0,
+                      l1_b=2.379, theta_0_b=64.0, period=1, ratio_a=1, grv=0.0, bet=2.0,
+                      steps=100, imon_t=0) -> RunConfig:
+    """The reference program's own configuration (srcs/main.f90:91-249):
+    Couette flow between walls moving at -/+uwall past a stripe of the "g"
+    phase (phi = 1 for nsv < i <= 3 nsv, srcs/init_phi_legendre.f90:15-28),
+    a uniform bottom wall (theta_0_a, l1_a = l2_a) and a top wall of
+    x-stripes alternating the "b" and "c" properties
+    (srcs/init_array_x_stripe.f90:12-46).  Derived values are formed as
+    main.f90 forms them: l2_b = l1_b*3.67/1.625, the "c" side swaps the
+    slip lengths and mirrors the angle, zeta = 0.21*6*(mul/l1 + mug/l2)
+    (:169-182), grvp = -grv cos(0) (:190-194) and dt = 0.32/nsv (:481).
+    The defaults are the reference's physics on a 32x8x32 grid of cubic
+    cells."""
+    l2_a = l1_a
+    zeta_a = 0.21 * 6.0 * (rmul / l1_a + rmug / l2_a)
+    l2_b = l1_b * 3.67 / 1.625
+    zeta_b = 0.21 * 6.0 * (rmul / l1_b + rmug / l2_b)
+    l1_c, l2_c = l2_b, l1_b
+    theta_0_c = 180.0 - theta_0_b
+    zeta_c = zeta_b
+    ni, nj, nk = svall
+    stripes = dict(theta=(theta_0_a, theta_0_b, theta_0_c), l1=(l1_a, l1_b, l1_c),
+                   l2=(l2_a, l2_b, l2_c), zeta=(zeta_a, zeta_b, zeta_c), period=period,
+                   ratio_a=ratio_a)
+    dt = 32.0e-2 / nsv * 1
+    return RunConfig(name, ni, nj, nk, xl, yl, zl, dt=dt, rhol=rhol, rhog=rhog, rmul=rmul,
+                     rmug=rmug, sigma=surface_tension, bottom=WallSide(theta_0_a, l1_a),
+                     top=WallSide(theta_0_b, l1_b), uwall=uwall, grvp=-grv * math.cos(0.0),
+                     bet=bet, imon_t=imon_t, steps=steps, init="reference_stripe",
+                     params=dict(nsv=nsv, x_stripe=stripes))
+
+
 CONFIGS = {
     "sessile_droplet": sessile_droplet,
     "couette_stripe": couette_stripe,
     "droplet_impact": droplet_impact,
     "patterned_wall": patterned_wall,
+    "reference_program": reference_program,
 }
 
 
@@ -202,6 +239,27 @@ def _wall_map(cfg: RunConfig, side: WallSide):
     return np.where(((kk[:, None] + ii[None, :]) % 2) == 0, side.theta0, side.theta_alt)
 
 
+def x_stripe_map(cfg: RunConfig, qa: float, qb: float, qc: float, period: int,
+                 ratio_a: int) -> np.ndarray:
+    """srcs/init_array_x_stripe.f90:12-46 on the padded (nk+6, nj+6, ni+6)
+    image: zero, q_a on the row j = 1, and on the row j = nj q_b where the
+    parity is 0 and q_c elsewhere, the parity being mod(i+ni-1, 2) for
+    period 1, else that of (ratio_a*kk + i+ni-1)/period with
+    kk = (k+nk-1)/period (integer division; both numerators are positive)."""
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    i = np.arange(-2, ni + 4)[None, :]
+    k = np.arange(-2, nk + 4)[:, None]
+    if period == 1:
+        parity = np.broadcast_to((i + ni - 1) % 2, (nk + 6, ni + 6))
+    else:
+        kk = (k + nk - 1) // period
+        parity = ((ratio_a * kk + i + ni - 1) // period) % 2
+    q = np.zeros(cfg.shape)
+    q[:, 3, :] = qa
+    q[:, nj + 2, :] = np.where(parity == 0, qb, qc)
+    return q
+
+
 def initial_state(cfg: RunConfig) -> dict[str, np.ndarray]:
     """Initial fields in the reference layout (nk+6, nj+6, ni+6): u, v, w,
     p, phi1 and the wall maps theta_0_array, l1_array, l2_array, zeta_array
@@ -225,6 +283,12 @@ def initial_state(cfg: RunConfig) -> dict[str, np.ndarray]:
         pert = amp * np.sin(2 * np.pi * Z / cfg.zl + ph[0]) * np.sin(np.pi * Y / cfg.yl + ph[1])
         d = np.maximum(q["x0"] - (X + pert), (X + pert) - q["x1"])
         prof = _tanh_profile(d, cfg)
+    elif cfg.init == "reference_stripe":
+        # srcs/init_phi_legendre.f90:15-28: phi = 1 for nsv < i <= 3 nsv on
+        # the rows j = 1..nj over the padded i and k ranges
+        nsv = q["nsv"]
+        i = np.arange(-2, cfg.ni + 4)
+        prof = np.broadcast_to(np.where((i > nsv) & (i <= 3 * nsv), 1.0, 0.0), sh)
     else:
         raise ValueError(cfg.init)
     phi[:, 3:cfg.nj + 3, :] = prof[:, 3:cfg.nj + 3, :]
@@ -235,6 +299,14 @@ def initial_state(cfg: RunConfig) -> dict[str, np.ndarray]:
         v[:, :3, :] = 0.0
         v[:, cfg.nj + 2:, :] = 0.0
     maps = {n: np.zeros(sh) for n in ("theta_0_array", "l1_array", "l2_array", "zeta_array")}
+    if "x_stripe" in q:
+        xs = q["x_stripe"]
+        for n, key in (("theta_0_array", "theta"), ("l1_array", "l1"), ("l2_array", "l2"),
+                       ("zeta_array", "zeta")):
+            maps[n] = x_stripe_map(cfg, *xs[key], xs["period"], xs["ratio_a"])
+        out = dict(u=u, v=v, w=w, p=p, phi1=phi)
+        out.update(maps)
+        return out
     for side, jrow in ((cfg.bottom, 1), (cfg.top, cfg.nj)):
         th = _wall_map(cfg, side)
         maps["theta_0_array"][:, jrow + 2, :] = th

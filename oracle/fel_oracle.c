@@ -1130,7 +1130,7 @@ static void sor4_impl(const int *ipara, const int *ID, const int *nID, const int
     if (lp == 0) err0 = err;
     if (err / err0 <= 1.0e-3) break;
   }
-  if (lp > 20) lp = 20;
+  /* the loop variable on exit: 21 after 21 sweeps without convergence */
   orc_sor_last_lp = lp;
   orc_sor_last_ratio = err / err0;
   if (*imon_t > 0 && (*nstep % *imon_t) == 0 && *ID == 0)

@@ -11,6 +11,8 @@ call changed in <case>.npz:
          every value is a float32 number, which the seeded inputs are)
   x<k>   the k-th array argument after the call (only where it changed)
 
+  y<k>   the k-th scalar argument after the call (only where it changed)
+
 tests/test_golden.py replays every case through the oracle (and, on a GPU,
 through the GPU library), so both stay pinned to the reference where the
 reference tree is absent.  Regenerate with
@@ -128,6 +130,36 @@ def cases():
     yield "solp_fft_tdma3", [*dims, np.zeros(2 * nm), r.standard_normal(nm),
                              r.standard_normal(nm), g.zeros()]
     yield "corunp_explicit", [NID.copy(), *dims, D(0.81), *inv, D(0.003)] + [n() for _ in range(6)]
+    # routines that include mpif.h (no MPI call at one rank)
+    phin = np.clip(phi + 0.02 * n(), -0.05, 1.05)
+    hx, hy, hz = g.zeros(), g.zeros(), g.zeros()
+    H.call_ref("cal_grad_p2a", I(0), I(g.nj), *dims, inv[0], dyinv, inv[2], phin, hx, hy, hz)
+    yield "solphi_mthinc3", [I(0), *dims, *inv, D(2.0), hx, hy, hz, phi.copy(), phin]
+    rmu = r.uniform(0.05, 1.0, g.shape)
+    cf = [g.zeros() for _ in range(45)]
+    H.call_ref("cal_arith_coef_vis", *dims, *inv, rmu, *cf)
+    for tag, dt in (("", 1e-3), ("_capped", 0.05)):
+        yield "solu_sor4" + tag, [I(0), I(0), NID.copy(), I(1), *dims, np.zeros(2, dtype=np.int32),
+                                  np.zeros(g.n3), np.zeros(g.n3), I(1), I(1), D(dt),
+                                  r.uniform(0.5, 2.0, g.shape), r.uniform(0.5, 2.0, g.shape)] + \
+            [c.copy() for c in cf] + [n() for _ in range(6)] + \
+            [D(0.25), D(g.dy), r.uniform(0.01, 0.1, g.shape), r.uniform(0.01, 0.1, g.shape),
+             r.uniform(0.0, 1.0, g.shape)]
+    yield "caldiv", [I(0), I(1), *dims, *inv, D(0.003), n(), n(), n(), g.zeros(), D(0.0)]
+    yield "cal_av", [I(0), I(1), *dims, n(), D(0.0)]
+    yield "bnd_avzero", [I(0), I(1), *dims, n()]
+    for name in ("trans_r2w", "trans_w2r"):
+        yield name, [I(0), I(0), I(1), svall.copy(), np.zeros(2, dtype=np.int32), np.zeros(nm),
+                     np.zeros(nm), r.standard_normal(nm), np.zeros(nm)]
+    yield "cal_center", [I(0), I(0), I(g.nj), *dims, D(g.ni * g.dx), D(g.nj * g.dy),
+                         D(g.nk * g.dz), D(0.003), D(8.1e-4), phi.copy(), np.zeros(3), D(0.4),
+                         D(0.3), I(3), I(1), I(1), D(0.0)]
+    yield "cal_vel", [I(0), *dims, D(5e-4), D(0.81), D(1.95), n(), n(), n(), phi.copy(),
+                      np.zeros(3), np.zeros(3)]
+    yield "caldt", [I(0), NID.copy(), I(0), I(1), *dims, I(2), I(1000), *inv, D(0.05), D(0.81),
+                    D(8.1e-4), D(1.95), D(0.195), D(5.5), n(), n(), n(), D(0.0), D(1.5)]
+    th = r.uniform(20.0, 160.0, g.shape)
+    yield "bnd_contact_angle", [NID.copy(), *dims, phi.copy(), th, D(g.dx), D(g.dy), D(g.dz)]
 
 
 # routine name of a case (cases may run one routine on several inputs)
@@ -136,7 +168,25 @@ def routine(case: str) -> str:
         return "calrn"
     if case == "cal_advu_weno5_tiny":
         return "cal_advu_weno5"
+    if case == "solu_sor4_capped":
+        return "solu_sor4"
     return case
+
+
+def call_ref_padded(name, args, k):
+    """Runs the reference with array argument k inside a zero-padded
+    buffer (one plane before and after): bnd_contact_angle reads one cell
+    outside its row at the i and k edges, before the first and after the
+    last plane (srcs/bnd_contact_angle.f90:42-44,75-77)."""
+    a = args[k]
+    plane = a.shape[-1] * a.shape[-2]
+    big = np.zeros(a.size + 2 * plane)
+    view = big[plane:plane + a.size].reshape(a.shape)
+    view[...] = a
+    args = list(args)
+    args[k] = view
+    H.call_ref(name, *args)
+    a[...] = view
 
 
 # the work arrays of the FFT stages are scratch whose contents are not part
@@ -146,12 +196,18 @@ SCRATCH = {"solp_fft_tdma1": {3}, "solp_fft_tdma3": {3}, "solp_fft_tdma2": {9, 1
 
 def save(case, args):
     spec, arrs = [], {}
-    before = [a.copy() if isinstance(a, np.ndarray) else None for a in args]
-    H.call_ref(routine(case), *args)
+    before = [a.copy() if isinstance(a, np.ndarray) else a.value for a in args]
+    if case == "bnd_contact_angle":
+        call_ref_padded(case, args, 4)
+    else:
+        H.call_ref(routine(case), *args)
     for k, a in enumerate(args):
         if isinstance(a, H.Ref):
             kind = "i" if isinstance(a.obj, type(I(0).obj)) else "d"
-            spec.append([kind, a.value])
+            spec.append([kind, before[k]])
+            if a.value != before[k]:
+                # a scalar output (INTENT(OUT) or updated argument)
+                arrs[f"y{k}"] = np.array(a.value)
         else:
             spec.append(["ia" if a.dtype == np.int32 else "a", None])
             arrs[f"a{k}"] = _compact(before[k])

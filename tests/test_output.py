@@ -107,11 +107,13 @@ def test_interface_monitor_lines_equal_the_reference(capfd):
     xl = 2.0
     drv = _drv()
     drv.fel_drv_flush()  # earlier monitor lines still in the Fortran runtime's buffer
+    H.ref().felref_flush()
     for jrow, first, ref in ((g.nj, 0, "find_interface_positions_upper"),
                              (1, 1, "find_interface_positions_lower")):
         capfd.readouterr()
         H.call_ref(ref, I(g.ni), I(g.nj), I(g.nk), q, D(g.dx), D(g.dy), D(g.dz), D(xl))
-        drv.fel_drv_flush()  # the Fortran runtime buffers unit 6
+        # the compiled reference binds its own Fortran runtime, which buffers unit 6
+        H.ref().felref_flush()
         theirs = capfd.readouterr().out
         x, cnt = crossings_np(g, q, jrow, g.dx, xl, first)
         xt = np.ascontiguousarray(x)  # (nk, ni) C order = x(ni, nk) column-major
