@@ -30,8 +30,10 @@ end, so the timing adds no packets between launches: the three pairs of a
 sweep that run back to back are one timed span (start of the first to the
 end of the third, divided by three), the pair before a ghost-refresh launch
 is not timed.  The CPU
-baseline is the oracle (a serial C restatement of the reference) timed on a
-bounded sample of the same configuration on this host.
+baseline is the reference's own program (srcs/main.f90 and its makefile
+sources, built with OpenMP by oracle/ref_main.py) timed on this host's cores
+on a bounded sample of the same configuration: half the resolution in every
+direction, three steps after the start-up step.
 """
 from __future__ import annotations
 
@@ -82,10 +84,78 @@ def measured_traffic(cfg):
     return None, None
 
 
-def cpu_baseline(cfg_full, seconds=15.0):
-    """Times the oracle's timestep (serial C restatement, 1 thread) on the same
-    configuration with the grid coarsened 4x in every direction (1/64 of the
-    cells, same domain and physics), for about `seconds`."""
+def host_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS where it is set (the
+    machine's share of cores), else the cores this process may run on."""
+    try:
+        return max(1, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        return len(os.sched_getaffinity(0))
+
+
+# Sample of the reference program's timing: the configuration at half the
+# resolution in every direction (1/8 of the cells, same domain and physics),
+# one start-up step and REF_TIMED_STEPS timed steps.
+REF_FACTOR = 2
+REF_TIMED_STEPS = 3
+
+
+def reference_case(cfg_full):
+    """(case name, run configuration, parameter assignments) of the reference
+    program's timing sample (oracle/ref_main.py builds it)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import ref_main as R  # the checker's recipe for the reference's own program
+    f = REF_FACTOR
+    cfg = cfg_full.scaled(cfg_full.ni // f, cfg_full.nj // f, cfg_full.nk // f)
+    case = f"bench_{cfg.name}_{cfg.ni}x{cfg.nj}x{cfg.nk}"
+    return case, cfg, R.restart_params(cfg, steps=1 + REF_TIMED_STEPS)
+
+
+def cpu_baseline_reference(cfg_full):
+    """Times the reference's own program (srcs/main.f90 with its makefile
+    sources, amdflang -O2 -fopenmp, built by oracle/ref_main.py) on this
+    host's cores: the configuration at half resolution, started from a
+    restart record of the same initial state; steps 2..4 are timed from the
+    arrival of step 2's first monitor line to that of step 4's last (the
+    program flushes its output at both points, srcs/main.f90:487,700).
+    Returns None when the program is not built."""
+    import tempfile
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import ref_main as R
+    from felmpf.config import initial_state
+    case, cfg, _ = reference_case(cfg_full)
+    if not os.path.exists(R.exe_path(case, omp=True)):
+        return None
+    threads = host_threads()
+    wd = tempfile.mkdtemp(prefix="felref_bench_")
+    try:
+        R.write_restart(os.path.join(wd, "x.inp"), initial_state(cfg))
+        out = R.run(case, wd, threads=threads, omp=True, timeout=600)
+    finally:
+        import shutil
+        shutil.rmtree(wd, ignore_errors=True)
+    t_start = [t for t, line in out if line.startswith("nstep=") and int(line.split()[1]) == 2]
+    t_end = [t for t, line in out if line.startswith("Err_div") and
+             int(line.split()[1]) == 1 + REF_TIMED_STEPS]
+    if not t_start or not t_end:
+        raise RuntimeError("reference program ended early:\n" +
+                           "\n".join(line for _, line in out[-20:]))
+    dt = t_end[0] - t_start[0]
+    sweeps = [int(line.split()[2]) for _, line in out if line.startswith("Err_solu")]
+    return {"value": round(cfg.cells * REF_TIMED_STEPS / dt / 1e6, 4), "unit": "Mcell-steps/s",
+            "cores": threads, "kind": "reference",
+            "sample": f"reference program srcs/main.f90 (makefile sources, amdflang -O2 -fopenmp, "
+                      f"1 MPI rank, {threads} OpenMP threads) on {cfg.name} at "
+                      f"{cfg.ni}x{cfg.nj}x{cfg.nk} (ref i,j,k) = {cfg.cells} cells (1/8 of the "
+                      f"metric's), from a restart record of the same initial state; steps 2-"
+                      f"{1 + REF_TIMED_STEPS} timed ({dt:.1f} s), step 1 is the start-up transient; "
+                      f"monitors every step; Gauss-Seidel sweeps per step {sweeps}"}
+
+
+def cpu_baseline_port(cfg_full, seconds=15.0):
+    """Fallback where the reference program is not built: the oracle's
+    timestep (serial C restatement, 1 thread) on the configuration with the
+    grid coarsened 4x in every direction, for about `seconds`."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from oracle_run import OracleRun  # test infrastructure: the checker, timed as baseline
     cfg = cfg_full.scaled(cfg_full.ni // 4, cfg_full.nj // 4, cfg_full.nk // 4)
@@ -128,6 +198,8 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
     sync_dev()
     gpu.felgpu_prof_enable_(ctypes.byref(ctypes.c_int(1)))
     gpu.felgpu_prof_reset_()
+    calls, sweeps = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    gpu.felgpu_sor_stats_(ctypes.byref(ctypes.c_int(1)), ctypes.byref(calls), ctypes.byref(sweeps))
     run.sync()
     sync_dev()
     t0 = time.perf_counter()
@@ -136,6 +208,7 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
     sync_dev()
     t1 = time.perf_counter()
     gpu.felgpu_prof_enable_(ctypes.byref(ctypes.c_int(0)))
+    gpu.felgpu_sor_stats_(ctypes.byref(ctypes.c_int(0)), ctypes.byref(calls), ctypes.byref(sweeps))
     st = run.state()
     timers = {}
     for tid, name in TIMERS.items():
@@ -144,6 +217,8 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
         timers[name] = (ms.value, cnt.value)
     phi = run.field("phi1")[3:-3, 3:-3, 3:-3]
     res = {"elapsed": t1 - t0, "state": st, "timers": timers,
+           "sweeps_mean": sweeps.value / max(calls.value, 1),
+           "sweeps_last": min(st["sor_lp"] + 1, 21),
            "phi_min": float(phi.min()), "phi_max": float(phi.max()),
            "volume_drift": float(abs(phi.sum() - vol0) / vol0)}
     if keep_fields:
@@ -172,6 +247,11 @@ def main():
     import felmpf
 
     cfg = felmpf.CONFIGS[args.config]()
+    # the CPU baseline runs first: it starts the reference's program as a
+    # child process, before this process touches the GPU
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_reference(cfg) or cpu_baseline_port(cfg, args.cpu_seconds)
     head_prec = 32 if args.precision == 32 else 64
     both = args.precision == 0
     main_run = measure(cfg, head_prec, args.steps, args.warmup, keep_fields=both)
@@ -190,7 +270,8 @@ def main():
                 "rel_l2_error_vs_fp64_after_steps": args.warmup + args.steps,
                 "rel_l2_error_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
                 "volume_drift": float(f"{r32['volume_drift']:.3e}"),
-                "sor_iterations_last_step": r32["state"]["sor_lp"] + 1,
+                "sor_sweeps_last_step": r32["sweeps_last"],
+                "sor_sweeps_per_step_mean": round(r32["sweeps_mean"], 3),
                 "dtype": dtype}
 
     if both:
@@ -222,7 +303,6 @@ def main():
     value = cfg.cells * args.steps / elapsed / 1e6
     traffic, traffic_src = measured_traffic(cfg) if head_prec == 64 else (None, None)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(cfg, args.cpu_seconds)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -239,7 +319,8 @@ def main():
         "config": {"workload": f"{cfg.name} {cfg.ni}x{cfg.nk}x{cfg.nj} (ref i,j,k = {cfg.ni}x{cfg.nj}x{cfg.nk}), "
                                "drop-in timestep driven by the Fortran host",
                    "cells": cfg.cells, "precision": "fp64" if head_prec == 64 else "fp32+refine",
-                   "sor_iterations_last_step": main_run["state"]["sor_lp"] + 1,
+                   "sor_sweeps_last_step": main_run["sweeps_last"],
+                   "sor_sweeps_per_step_mean": round(main_run["sweeps_mean"], 3),
                    "volume_drift": float(f"{main_run['volume_drift']:.3e}")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

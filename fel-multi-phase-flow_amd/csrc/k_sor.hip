@@ -392,6 +392,8 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
 
 static int g_last_lp = -1;
 static double g_last_ratio = 0.0;
+// solu_sor4_ calls and colour sweeps since the last felgpu_sor_stats_ reset
+static long long g_sor_calls = 0, g_sor_sweeps = 0;
 
 template <class T>
 SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* rhon,
@@ -476,6 +478,14 @@ void felgpu_sor_last_(int* lp, double* ratio) {
   *ratio = g_last_ratio;
 }
 
+// Number of solu_sor4_ calls and of colour sweeps they made since the last
+// reset (*reset != 0 clears the counts after reading them).
+void felgpu_sor_stats_(const int* reset, long long* calls, long long* sweeps) {
+  *calls = g_sor_calls;
+  *sweeps = g_sor_sweeps;
+  if (*reset) g_sor_calls = g_sor_sweeps = 0;
+}
+
 // srcs/solu_sor4.f:1-17 (ipara must be 0)
 void solu_sor4_(const int* ipara, const int* ID, const int* nID, const int* ndiv, const int* ni,
                 const int* nj, const int* nk, const int* key, double* sendjb, double* recvjb,
@@ -508,6 +518,8 @@ void solu_sor4_(const int* ipara, const int* ID, const int* nID, const int* ndiv
   });
   g_last_lp = r.lp;
   g_last_ratio = r.err / r.err0;
+  g_sor_calls += 1;
+  g_sor_sweeps += r.lp < 21 ? r.lp + 1 : 21;
   if (*imon_t > 0 && (*nstep % *imon_t) == 0 && *ID == 0)
   {
     char a[3][32];

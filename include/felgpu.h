@@ -299,6 +299,11 @@ void solu_sor4_(const int *ipara, const int *ID, const int *nID, const int *ndiv
  * No counterpart: the reference only prints them. */
 void felgpu_sor_last_(int *lp, double *ratio);
 
+/* Number of solu_sor4_ calls and of colour sweeps (lp+1, or 21 at the cap)
+ * they made since the last reset; *reset != 0 clears the counts after
+ * reading them.  No counterpart: a benchmark statistic. */
+void felgpu_sor_stats_(const int *reset, long long *calls, long long *sweeps);
+
 /* ------------------------------------------------------------------------
  * Boundary conditions.  Call sites: srcs/main.f90:324-422, 497-517, 625-661.
  * ---------------------------------------------------------------------- */

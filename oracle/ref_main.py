@@ -73,6 +73,60 @@ CASES = {
 }
 
 
+def fortran_real(v: float) -> str:
+    """A REAL(8) literal that reads back as exactly v."""
+    return repr(float(v)).replace("e", "d") + ("" if "e" in repr(float(v)) else "d0")
+
+
+def restart_params(cfg, steps: int) -> dict[str, str]:
+    """Parameter assignments that run the reference's program on a run
+    configuration of felmpf.config from a restart record (irestart = 1,
+    srcs/main.f90:375-404): grid, lengths, phases, gravity, the bottom wall
+    as the uniform "a" side and the top wall as the "b" side of
+    init_array_x_stripe (its "c" side mirrors the angle and swaps the slip
+    lengths, srcs/main.f90:179-182), the step dt = 0.32/nsv nearest to the
+    configuration's, `steps` steps with the monitors on every step and no
+    file output beyond the start-up's."""
+    nsv = max(1, round(0.32 / cfg.dt))
+    return {
+        "nsv": str(nsv), "svall(1)": str(cfg.ni), "svall(2)": str(cfg.nj),
+        "svall(3)": str(cfg.nk), "irestart": "1",
+        "xl": fortran_real(cfg.xl), "yl": fortran_real(cfg.yl), "zl": fortran_real(cfg.zl),
+        "rhol": fortran_real(cfg.rhol), "rhog": fortran_real(cfg.rhog),
+        "rmul": fortran_real(cfg.rmul), "rmug": fortran_real(cfg.rmug),
+        "surface_tension": fortran_real(cfg.sigma), "uwall": fortran_real(cfg.uwall),
+        "l1_a": fortran_real(cfg.bottom.slip), "theta_0_a": fortran_real(cfg.bottom.theta0),
+        "l1_b": fortran_real(cfg.top.slip), "theta_0_b": fortran_real(cfg.top.theta0),
+        "grv": fortran_real(-cfg.grvp), "bet_mthinc": fortran_real(cfg.bet),
+        "nmax": str(steps), "idout": "1000000", "imkuvp": "1000000", "imkvtk": "1000000",
+        "imon_t": "1",
+    }
+
+
+def write_restart(path: str, fields: dict, time: float = 0.0, nstep: int = 0):
+    """The restart record srcs/datain.f:20-39 reads: one unformatted
+    sequential record (4-byte length markers) of u, v, w, p, uo, vo, wo, po
+    and phi(...,0:1) in the padded column-major image, time and nstep.
+    `fields` holds u, v, w, p and phi1 as (nk+6, nj+6, ni+6) arrays; uo..po
+    are zero (srcs/init.f:24-41) and phi(...,0) = phi(...,1) (one phase)."""
+    import numpy as np
+    u = fields["u"]
+    zero = np.zeros_like(u)
+    order = [u, fields["v"], fields["w"], fields["p"], zero, zero, zero, zero,
+             fields["phi1"], fields["phi1"]]
+    n = sum(a.nbytes for a in order) + 12
+    if n >= 2 ** 31:
+        raise ValueError("record of 2 GiB or more (needs subrecords)")
+    marker = np.array([n], dtype=np.int32).tobytes()
+    with open(path, "wb") as f:
+        f.write(marker)
+        for a in order:
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        f.write(np.array([time], dtype=np.float64).tobytes())
+        f.write(np.array([nstep], dtype=np.int32).tobytes())
+        f.write(marker)
+
+
 def configured_main(params: dict[str, str]) -> str:
     """The text of main.f90 with the case's assignments and the slice fix."""
     with open(MAIN_SRC) as f:
