@@ -15,17 +15,16 @@ namespace fel {
 // cal_advu_weno5.f:31), expressions in the reference's order.
 //
 // T = double is the reference's FP64 arithmetic.  T = float (the FP32 mode)
-// evaluates the same weights from differences divided by their largest
-// magnitude: the weights are homogeneous of degree 0 in that scale, so the
-// division changes nothing but keeps the degree-8 products of the
-// smoothness indicators inside the FP32 range; the stencil derivatives use
-// the unscaled differences.
+// evaluates the same weights from differences scaled by the power of two
+// that brings their largest magnitude to [1, 2): the weights are homogeneous
+// of degree 0 in that scale, so the scaling changes nothing but keeps the
+// degree-8 products of the smoothness indicators inside the FP32 range; the
+// stencil derivatives use the unscaled differences.
 // 1/x for the weights: IEEE division in FP64 (and the tangent mode); in the
 // FP32 mode the hardware reciprocal (v_rcp_f32, 1 ulp), whose rounding is
 // far below the FP32 mode's accuracy bar and which replaces a ten-instruction
 // division sequence in a compute-bound kernel.  The weight sums are floored
-// at 1e-30; the scale is the largest difference magnitude, and the two forms
-// differ only for a scale below the smallest normal float.
+// at 1e-30 and, after the scaling, lie far above the smallest normal float.
 template <class T>
 __device__ __forceinline__ T weight_rcp(T x) {
   if constexpr (is_f32<T>) return __builtin_amdgcn_rcpf(x);
@@ -46,8 +45,12 @@ __device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp
   if constexpr (is_f32<T>) {
     const T sc = fmax(fmax(fmax(fabs(m5), fabs(m3)), fmax(fabs(m1), fabs(p1))),
                       fmax(fabs(p3), fabs(p5)));
-    const T inv = sc > T(0.0) ? weight_rcp(sc) : T(0.0);
-    m5 = m5 * inv; m3 = m3 * inv; m1 = m1 * inv; p1 = p1 * inv; p3 = p3 * inv; p5 = p5 * inv;
+    // scale by the exact power of two that brings the largest difference to
+    // [1, 2): the weights are invariant to it, and unlike 1/sc (which
+    // overflows for sc below ~2.9e-39) it stays finite for subnormal sc
+    const int e = sc > T(0.0) ? ilogbf(sc) : 0;
+    m5 = ldexpf(m5, -e); m3 = ldexpf(m3, -e); m1 = ldexpf(m1, -e);
+    p1 = ldexpf(p1, -e); p3 = ldexpf(p3, -e); p5 = ldexpf(p5, -e);
   }
   T a, b;
   a = m5 - m3 * c2 + m1;

@@ -463,6 +463,12 @@ bool deferred_copy(const void* src_host, const void* dst_host, size_t count) {
   materialize_dependents(*dst);       // the destination's contents change
   drop_lazy(*dst);
   Range* holder = src->lazy ? src->lazy : src;
+  if (holder != src) {
+    // the holder's buffer changes hands: a record that reads it (the
+    // viscosity of deferred coefficients, a source of a deferred chain)
+    // settles first, as for a write to the holder
+    deferred_coef_check(reinterpret_cast<const void*>(holder->host), true);
+  }
   std::swap(holder->dev, dst->dev);   // dst now holds the contents
   // everything deferred on the old holder, and the holder itself, now defers to dst
   for (auto& kv : R().reg)
@@ -1152,6 +1158,12 @@ static void register_range(const void* host, const int64_t* nbytes, const int* u
                            "precision\n", host);
       std::abort();
     }
+    // registered again: the contents are replaced, as by felgpu_upload_ --
+    // records reading the range settle, ranges deferring to it take their
+    // contents, and its own deferred state is dropped
+    deferred_coef_check(host, true);
+    materialize_dependents(it->second);
+    drop_lazy(it->second);
   } else {
     // overlap check
     auto nx = r.reg.lower_bound(p);
@@ -1250,6 +1262,9 @@ void felgpu_set_result_reuse_(const int* on) {
   result_reuse();
   g_reuse = *on != 0 ? 1 : 0;
 }
+
+// The current reuse setting (1 on, 0 off).
+int felgpu_result_reuse_() { return result_reuse() ? 1 : 0; }
 
 // Number of entry-point calls answered from the result memo so far.
 int64_t felgpu_result_reuse_hits_() { return g_memo_hits; }

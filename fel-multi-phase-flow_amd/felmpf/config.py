@@ -39,8 +39,9 @@ class FelCfg(ctypes.Structure):
 class WallSide:
     theta0: float          # static angle [deg]
     slip: float            # slip length l1 = l2
-    theta_alt: float = 0.0  # second angle of a checkerboard pattern
+    theta_alt: float = 0.0  # second angle of a checkerboard or stripe pattern
     patch: int = 0          # checkerboard patch size in cells (0: uniform wall)
+    stripe: int = 0         # period in cells of x-stripes theta0 / theta_alt (0: none)
 
 
 @dataclass
@@ -161,6 +162,34 @@ def patterned_wall() -> RunConfig:
                      init="cap", params=dict(r=0.4, cx=1.0, cz=1.0))
 
 
+def reference_walls(rmul: float, rmug: float, l1_a: float, theta_0_a: float, l1_b: float,
+                    theta_0_b: float, period: int = 1, ratio_a: int = 1) -> dict:
+    """The wall maps srcs/main.f90 builds with init_array_x_stripe
+    (:169-186, :207-210): a uniform bottom wall (theta_0_a, l1_a = l2_a)
+    and a top wall of x-stripes alternating the "b" side (theta_0_b, l1_b,
+    l2_b = l1_b*3.67/1.625) and the "c" side (the mirrored angle and the
+    swapped slip lengths), each with zeta = 0.21*6*(mul/l1 + mug/l2)."""
+    l2_a = l1_a
+    zeta_a = 0.21 * 6.0 * (rmul / l1_a + rmug / l2_a)
+    l2_b = l1_b * 3.67 / 1.625
+    zeta_b = 0.21 * 6.0 * (rmul / l1_b + rmug / l2_b)
+    l1_c, l2_c = l2_b, l1_b
+    theta_0_c = 180.0 - theta_0_b
+    zeta_c = zeta_b
+    return dict(theta=(theta_0_a, theta_0_b, theta_0_c), l1=(l1_a, l1_b, l1_c),
+                l2=(l2_a, l2_b, l2_c), zeta=(zeta_a, zeta_b, zeta_c), period=period,
+                ratio_a=ratio_a)
+
+
+def with_reference_walls(cfg: RunConfig) -> RunConfig:
+    """cfg with the wall maps the reference's program builds from its bottom
+    and top wall parameters (reference_walls): what it runs when it restarts
+    from a record of cfg's initial state."""
+    walls = reference_walls(cfg.rmul, cfg.rmug, cfg.bottom.slip, cfg.bottom.theta0,
+                            cfg.top.slip, cfg.top.theta0)
+    return replace(cfg, params=dict(cfg.params, x_stripe=walls))
+
+
 def reference_program(name: str = "reference_program", nsv: int = 8, svall=(32, 8, 32),
                       xl=17.0, yl=4.25, zl=17.0, rhol=8.1e-1, rhog=8.1e-1, rmul=1.95,
                       rmug=1.95, surface_tension=5.5, uwall=0.25, l1_a=2.165, theta_0_a=90.0,
@@ -177,17 +206,8 @@ def reference_program(name: str = "reference_program", nsv: int = 8, svall=(32, 
     (:169-182), grvp = -grv cos(0) (:190-194) and dt = 0.32/nsv (:481).
     The defaults are the reference's physics on a 32x8x32 grid of cubic
     cells."""
-    l2_a = l1_a
-    zeta_a = 0.21 * 6.0 * (rmul / l1_a + rmug / l2_a)
-    l2_b = l1_b * 3.67 / 1.625
-    zeta_b = 0.21 * 6.0 * (rmul / l1_b + rmug / l2_b)
-    l1_c, l2_c = l2_b, l1_b
-    theta_0_c = 180.0 - theta_0_b
-    zeta_c = zeta_b
     ni, nj, nk = svall
-    stripes = dict(theta=(theta_0_a, theta_0_b, theta_0_c), l1=(l1_a, l1_b, l1_c),
-                   l2=(l2_a, l2_b, l2_c), zeta=(zeta_a, zeta_b, zeta_c), period=period,
-                   ratio_a=ratio_a)
+    stripes = reference_walls(rmul, rmug, l1_a, theta_0_a, l1_b, theta_0_b, period, ratio_a)
     dt = 32.0e-2 / nsv * 1
     return RunConfig(name, ni, nj, nk, xl, yl, zl, dt=dt, rhol=rhol, rhog=rhog, rmul=rmul,
                      rmug=rmug, sigma=surface_tension, bottom=WallSide(theta_0_a, l1_a),
@@ -196,7 +216,60 @@ def reference_program(name: str = "reference_program", nsv: int = 8, svall=(32, 
                      params=dict(nsv=nsv, x_stripe=stripes))
 
 
+def seeded_droplets(n: int, seed: int, lx: float, lz: float, ly: float, rmin: float,
+                    rmax: float, vstd: float, gap: float, max_tries: int = 100000) -> list:
+    """n non-overlapping droplets in the lower half of the wall-normal
+    extent (ref y in [0, ly/2]): radii uniform in [rmin, rmax], centres
+    uniform with the sphere inside that half and `gap` clear of the wall,
+    accepted when clear of every droplet placed before by `gap` (periodic
+    distance in x and z); velocities normal with std vstd per component.
+    Returns (x, y, z, r, u, v, w) tuples in reference axes."""
+    rng = np.random.default_rng(seed)
+    drops = []
+    tries = 0
+    while len(drops) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"placed {len(drops)} of {n} droplets in {max_tries} tries")
+        r = rng.uniform(rmin, rmax)
+        x, z = rng.uniform(0.0, lx), rng.uniform(0.0, lz)
+        y = rng.uniform(r + gap, ly / 2 - r)
+        ok = True
+        for (x2, y2, z2, r2, *_) in drops:
+            dx = abs(x - x2)
+            dx = min(dx, lx - dx)
+            dz = abs(z - z2)
+            dz = min(dz, lz - dz)
+            if dx * dx + (y - y2) ** 2 + dz * dz < (r + r2 + gap) ** 2:
+                ok = False
+                break
+        if ok:
+            u, v, w = rng.normal(0.0, vstd, 3)
+            drops.append((x, y, z, r, u, v, w))
+    return drops
+
+
+def scale_run() -> RunConfig:
+    """512x512x256 (Lx=Ly=4, Lz=2): 32 non-overlapping droplets (seed 0),
+    radii uniform in [0.03, 0.08] Lx, centres uniform in the lower half with
+    a rejection test, velocities normal with std 0.2 per component; bottom
+    wall of x-stripes 60/100 deg with a 32-cell period (slip 0.02), free-slip
+    top (infinite slip, 90 deg); density ratio 100, viscosity ratio 50,
+    Re 1000, We 20 with L = the mean droplet diameter 0.44 and U = 1, no
+    gravity; dt by the CFL condition of srcs/caldt.f (cfl 0.05), 500 steps."""
+    lx, ly, lz = 4.0, 2.0, 4.0  # reference axes: x, wall-normal y, z
+    D = 2.0 * 0.055 * lx
+    rmul = D / 1000.0
+    drops = seeded_droplets(32, 0, lx, lz, ly, 0.03 * lx, 0.08 * lx, 0.2, gap=4 * lx / 512)
+    return RunConfig("scale_run", 512, 256, 512, lx, ly, lz, dt=0.0, rhol=1.0, rhog=0.01,
+                     rmul=rmul, rmug=rmul / 50.0, sigma=D / 20.0,
+                     bottom=WallSide(60.0, 0.02, theta_alt=100.0, stripe=32),
+                     top=WallSide(90.0, 1e30), steps=500, init="droplets",
+                     params=dict(droplets=drops), cfl=0.05)
+
+
 CONFIGS = {
+    "scale_run": scale_run,
     "sessile_droplet": sessile_droplet,
     "couette_stripe": couette_stripe,
     "droplet_impact": droplet_impact,
@@ -232,6 +305,12 @@ def _tanh_profile(sdf, cfg: RunConfig):
 def _wall_map(cfg: RunConfig, side: WallSide):
     """(nk+6, ni+6) static-angle map of one wall: uniform, or a checkerboard
     of side.patch-cell squares (periodic in the ghosts)."""
+    if side.stripe > 0:
+        # x-stripes: theta0 on the first half of each period, theta_alt on
+        # the second (periodic in the ghosts)
+        ii = ((np.arange(-2, cfg.ni + 4) - 1) % cfg.ni) % side.stripe < side.stripe // 2
+        return np.broadcast_to(np.where(ii, side.theta0, side.theta_alt)[None, :],
+                               (cfg.nk + 6, cfg.ni + 6)).copy()
     if side.patch <= 0:
         return np.full((cfg.nk + 6, cfg.ni + 6), float(side.theta0))
     ii = ((np.arange(-2, cfg.ni + 4) - 1) % cfg.ni) // side.patch
@@ -260,6 +339,68 @@ def x_stripe_map(cfg: RunConfig, qa: float, qb: float, qc: float, period: int,
     return q
 
 
+def _droplets_state(cfg: RunConfig) -> dict[str, np.ndarray]:
+    """Several droplets (phi = 0 inside, the tanh profile of _tanh_profile)
+    with their own velocities on the faces whose liquid fraction exceeds one
+    half; each droplet's profile is evaluated in its own padded box only
+    (beyond 8 cells from its surface the profile is exactly 1)."""
+    sh = cfg.shape
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    dcell = (cfg.dx * cfg.dy * cfg.dz) ** (1.0 / 3.0)
+    phi = np.ones(sh)
+    owner = np.full(sh, -1, dtype=np.int32)  # droplet of the cell (liquid fraction > 0)
+    xi = (np.arange(-2, ni + 4) - 0.5) * cfg.dx
+    yj = (np.arange(-2, nj + 4) - 0.5) * cfg.dy
+    zk = (np.arange(-2, nk + 4) - 0.5) * cfg.dz
+    drops = cfg.params["droplets"]
+    for m, (x0, y0, z0, r, *_v) in enumerate(drops):
+        reach = r + 8 * dcell
+        # periodic images in x and z: distances measured to the nearest image
+        def axis(c, lo_hi, n, L, d):
+            idx = np.arange(n + 6)
+            dist = np.abs(lo_hi - c)
+            dist = np.minimum(dist, L - dist) if L else dist
+            return idx[dist <= reach], dist[dist <= reach]
+        ii, dxs = axis(x0, xi, ni, cfg.xl, cfg.dx)
+        jj, dys = axis(y0, yj, nj, 0.0, cfg.dy)
+        kk, dzs = axis(z0, zk, nk, cfg.zl, cfg.dz)
+        sdf = np.sqrt(dzs[:, None, None] ** 2 + dys[None, :, None] ** 2 +
+                      dxs[None, None, :] ** 2) - r
+        a = 2.0 * cfg.bet * sdf / dcell
+        prof = 1.0 / (1.0 + np.exp(-np.clip(a, -60, 60)))
+        prof[a > 30] = 1.0
+        prof[a < -30] = 0.0
+        sub = np.ix_(kk, jj, ii)
+        cur = phi[sub]
+        mine = prof < cur
+        phi[sub] = np.where(mine, prof, cur)
+        o = owner[sub]
+        owner[sub] = np.where(mine & (prof < 1.0), m, o)
+    out_phi = np.zeros(sh)
+    out_phi[:, 3:nj + 3, :] = phi[:, 3:nj + 3, :]
+    u, v, w, p = (np.zeros(sh) for _ in range(4))
+    vel = np.array([d[4:7] for d in drops] + [(0.0, 0.0, 0.0)])
+    for comp, (arr, ax) in enumerate(((u, 2), (v, 1), (w, 0))):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        face = 0.5 * (out_phi[tuple(lo)] + out_phi[tuple(hi)])
+        own = np.maximum(owner[tuple(lo)], owner[tuple(hi)])
+        arr[tuple(lo)] = np.where(face < 0.5, vel[own, comp], 0.0)
+    v[:, :3, :] = 0.0
+    v[:, nj + 2:, :] = 0.0
+    maps = {n: np.zeros(sh) for n in ("theta_0_array", "l1_array", "l2_array", "zeta_array")}
+    for side, jrow in ((cfg.bottom, 1), (cfg.top, nj)):
+        maps["theta_0_array"][:, jrow + 2, :] = _wall_map(cfg, side)
+        maps["l1_array"][:, jrow + 2, :] = side.slip
+        maps["l2_array"][:, jrow + 2, :] = side.slip
+        maps["zeta_array"][:, jrow + 2, :] = zeta_of(cfg.rmul, cfg.rmug, side.slip, side.slip)
+    out = dict(u=u, v=v, w=w, p=p, phi1=out_phi)
+    out.update(maps)
+    return out
+
+
 def initial_state(cfg: RunConfig) -> dict[str, np.ndarray]:
     """Initial fields in the reference layout (nk+6, nj+6, ni+6): u, v, w,
     p, phi1 and the wall maps theta_0_array, l1_array, l2_array, zeta_array
@@ -283,6 +424,8 @@ def initial_state(cfg: RunConfig) -> dict[str, np.ndarray]:
         pert = amp * np.sin(2 * np.pi * Z / cfg.zl + ph[0]) * np.sin(np.pi * Y / cfg.yl + ph[1])
         d = np.maximum(q["x0"] - (X + pert), (X + pert) - q["x1"])
         prof = _tanh_profile(d, cfg)
+    elif cfg.init == "droplets":
+        return _droplets_state(cfg)
     elif cfg.init == "reference_stripe":
         # srcs/init_phi_legendre.f90:15-28: phi = 1 for nsv < i <= 3 nsv on
         # the rows j = 1..nj over the padded i and k ranges

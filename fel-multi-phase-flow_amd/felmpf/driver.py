@@ -21,7 +21,6 @@ are bitwise those of the FP64 run.
 from __future__ import annotations
 
 import ctypes
-import os
 
 import numpy as np
 
@@ -56,7 +55,10 @@ class GpuRun:
         self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(precision)))
         self.gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(1 if tangent else 0)))
         # results already computed are taken over instead of recomputed
-        # (bitwise the same either way)
+        # (bitwise the same either way); the library's setting before this
+        # run is restored when it closes
+        self.gpu.felgpu_result_reuse_.restype = ctypes.c_int
+        self._reuse_before = int(self.gpu.felgpu_result_reuse_())
         self.gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(1 if reuse else 0)))
         # pressure-solve modes: FP64 (default), or the flow class (FP32 solve
         # with FP64 refinement in the FP32 mode)
@@ -156,9 +158,8 @@ class GpuRun:
                 self.gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(0)))
             if self.precision != 64:
                 self.gpu.felgpu_set_precision_(ctypes.byref(ctypes.c_int(64)))
-            if not self.reuse and os.environ.get("FELGPU_NO_RESULT_REUSE") != "1":
-                # back to the library's default for the calls that follow
-                self.gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(1)))
+            # back to the library's setting before this run
+            self.gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(self._reuse_before)))
 
     def __del__(self):
         try:

@@ -96,6 +96,8 @@ void felgpu_tangent_fetch_(const void *host, double *out);
  * (as FELGPU_NO_RESULT_REUSE=1 does at start-up); results are bitwise the
  * same either way.  felgpu_result_reuse_hits_ counts the calls answered so. */
 void felgpu_set_result_reuse_(const int *on);
+/* The current setting: 1 on, 0 off. */
+int felgpu_result_reuse_(void);
 int64_t felgpu_result_reuse_hits_(void);
 /* Calls answered from a result a previous call's pass formed alongside its
  * own (the other staggered normals of calrn_, the curvature of calrkap_);

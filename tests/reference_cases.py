@@ -23,6 +23,35 @@ CASES = {
 
 FIELDS = ("u", "v", "w", "p", "phi0", "phi1")
 
+# Droplet configurations the reference's program ran from a restart record
+# of their initial state (tests/golden/make_trajectory.py, RESTART): case ->
+# the step at which the reference blows up (err0 of its Gauss-Seidel monitor
+# >= 1e3 or NaN), or None.
+RESTART_CASES = {"sessile_32cube_restart": None, "sessile_64cube_restart": 25,
+                 "wall120_64x32x64_restart": 16}
+RESTART_FIELDS = ("u", "v", "w", "p", "phi1")
+
+
+def restart_config(fx) -> C.RunConfig:
+    """The run configuration a restart fixture was written from, with the
+    wall maps the reference's program builds from its wall parameters
+    (config.with_reference_walls: x-stripes of the top wall)."""
+    from dataclasses import replace
+    spec = json.loads(str(fx["spec"]))
+    cfg = C.CONFIGS[spec["config"]]().scaled(*spec["grid"])
+    if "bottom_theta" in spec:
+        cfg = replace(cfg, bottom=C.WallSide(spec["bottom_theta"], cfg.bottom.slip))
+    return C.with_reference_walls(cfg)
+
+
+def write_restart_record(path: str, cfg: C.RunConfig):
+    """The restart record the reference's program started from (its format,
+    srcs/datain.f:20-39, from oracle/ref_main.py)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(GOLDEN), "..", "oracle"))
+    import ref_main
+    ref_main.write_restart(path, C.initial_state(cfg))
+
 
 def fixture(case: str):
     return np.load(os.path.join(GOLDEN, f"traj_{case}.npz"))

@@ -34,14 +34,17 @@ def _interior(cfg, a):
     return a[3:cfg.nk + 3, 3:cfg.nj + 3, 3:cfg.ni + 3]
 
 
+@pytest.mark.parametrize("fp32_modes", [True, False], ids=["fp32_refine", "fp64_modes"])
 @pytest.mark.parametrize("make", [C.droplet_impact, C.sessile_droplet],
                          ids=["droplet_impact", "sessile_droplet"])
-def test_fp32_mode_tracks_fp64(make):
+def test_fp32_mode_tracks_fp64(make, fp32_modes):
+    """Both FP32 configurations: FP32 pressure-solve modes with FP64
+    refinement (the benchmark's FP32+refine) and FP64 modes."""
     base = make()
     cfg = base.scaled(64, 16, 32) if base.name == "droplet_impact" else base.scaled(32, 32, 32)
     out = {}
     for prec in (64, 32):
-        g = GpuRun(cfg, precision=prec)
+        g = GpuRun(cfg, precision=prec, fp32_modes=fp32_modes and prec == 32)
         assert H.gpu().felgpu_precision_() == prec
         v0 = _interior(cfg, g.field("phi1")).sum()
         g.steps(20)
@@ -105,3 +108,57 @@ def test_fp32_precision_is_per_array_class():
         for a in arrays:
             lib.felgpu_unregister_(ctypes.c_void_p(a.ctypes.data))
         lib.felgpu_set_precision_(ctypes.byref(ctypes.c_int(64)))
+
+
+def test_fp32_weno_with_subnormal_differences():
+    """FP32 WENO5 on a u field whose differences are subnormal floats (the
+    largest below 2.9e-39, where 1/x overflows), advected by v = w = 1:
+    every output is finite and adv_u is not all zero."""
+    g = H.Grid(16, 8, 16, 1.0, 1.0, 1.0)
+    lib = H.gpu()
+    lib.felgpu_set_precision_(ctypes.byref(ctypes.c_int(32)))
+    arrays = []
+    try:
+        rng = np.random.default_rng(53)
+        u = (np.float32(1e-39) * rng.uniform(-1, 1, g.shape).astype(np.float32)).astype(np.float64)
+        assert 0 < np.abs(np.diff(u, axis=2)).max() < 2.9e-39
+        vel = [u, np.ones(g.shape), np.ones(g.shape)]
+        adv = [g.zeros() for _ in range(3)]
+        for a in vel + adv:
+            n = ctypes.c_int64(a.nbytes)
+            lib.felgpu_register_(ctypes.c_void_p(a.ctypes.data), ctypes.byref(n),
+                                 ctypes.byref(ctypes.c_int(1)))
+            arrays.append(a)
+        H.call_gpu("cal_advu_weno5", I(g.ni), I(g.nj), I(g.nk), D(1.0), D(1.0), D(1.0), *vel, *adv)
+        for a in adv:
+            lib.felgpu_download_(ctypes.c_void_p(a.ctypes.data))
+            assert np.isfinite(a).all()
+        assert np.abs(g.interior(adv[0])).max() > 0
+    finally:
+        for a in arrays:
+            lib.felgpu_unregister_(ctypes.c_void_p(a.ctypes.data))
+        lib.felgpu_set_precision_(ctypes.byref(ctypes.c_int(64)))
+
+
+def test_poisson_residual_equals_the_oracle():
+    """felgpu_poisson_residual_ (the FP64 residual operator of the
+    refinement, srcs/solp_fft_tdma4.f:55-72) against orc_poisson_residual on
+    the same seeded div and dp: relative residual and its absolute form."""
+    g = H.Grid(32, 16, 32, 1 / 32, 1 / 16, 1 / 32)
+    r = np.random.default_rng(54)
+    div, dp = r.standard_normal(g.shape), r.standard_normal(g.shape)
+    lib, orc = H.gpu(), H.oracle()
+    fn = lib.felgpu_poisson_residual_
+    fn.restype = ctypes.c_double
+    got = fn(ctypes.byref(ctypes.c_int(g.ni)), ctypes.byref(ctypes.c_int(g.nj)),
+             ctypes.byref(ctypes.c_int(g.nk)), ctypes.byref(ctypes.c_double(0.81)),
+             ctypes.byref(ctypes.c_double(1 / g.dx)), ctypes.byref(ctypes.c_double(1 / g.dy)),
+             ctypes.byref(ctypes.c_double(1 / g.dz)), div.ctypes.data_as(ctypes.c_void_p),
+             dp.ctypes.data_as(ctypes.c_void_p))
+    orc.orc_poisson_residual.restype = ctypes.c_double
+    orc.orc_poisson_residual.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + \
+        [ctypes.c_void_p] * 3
+    eabs = ctypes.c_double(0.0)
+    want = orc.orc_poisson_residual(g.ni, g.nj, g.nk, 0.81, 1 / g.dx, 1 / g.dy, 1 / g.dz,
+                                    div.ctypes.data, dp.ctypes.data, ctypes.addressof(eabs))
+    assert abs(got / want - 1.0) <= 1e-13, (got, want)

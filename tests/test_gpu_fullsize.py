@@ -1,11 +1,12 @@
-"""The timestep at the benchmark's full size: droplet impact, 512x256x128
+"""The timestep at full size: droplet impact, 512x256x128
 (reference i,j,k = 512x128x256, 16.8 M cells), through properties that hold
 at any size -- phi stays in [0,1], the liquid volume is conserved by the
 mass correction (srcs/solphi_mthinc3.f:101-160) to 1e-12, the projected
 velocity is divergence-free to the Poisson solver's precision
 (srcs/corunp_explicit.f after srcs/solp_fft_tdma1-3.f), every field stays
 finite, and runs with and without result reuse are bitwise equal; the FP32
-mode is held to its own bars.  The oracle is too slow at this size;
+mode is held to its own bars, also on the scale run (512x512x256, 67 M
+cells).  The oracle is too slow at these sizes;
 tests/test_gpu_step.py holds the GPU path to it on smaller grids.
 """
 from __future__ import annotations
@@ -18,11 +19,11 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "phi1")
 
 
-def _run(steps, reuse=True, precision=64, fp32_modes=False):
+def _run(steps, reuse=True, precision=64, fp32_modes=False, config="droplet_impact"):
     from felmpf import config as C
     from felmpf.driver import GpuRun
     from felmpf.longrun import diagnostics
-    cfg = C.droplet_impact()
+    cfg = C.CONFIGS[config]()
     run = GpuRun(cfg, reuse=reuse, precision=precision, fp32_modes=fp32_modes)
     try:
         d0 = diagnostics(cfg, run)
@@ -60,6 +61,22 @@ def test_full_size_fp32_mode_invariants():
     stay FP64), max |div u| at FP32 rounding of order-1 velocities (1.4e-5
     measured over 1,000 steps), every field finite."""
     _, d0, d, fields = _run(5, precision=32, fp32_modes=True)
+    phi = fields["phi1"][3:-3, 3:-3, 3:-3]
+    assert phi.min() >= 0.0 and phi.max() <= 1.0
+    assert abs(d["volume"] - d0["volume"]) <= 1e-10 * d0["volume"], (d["volume"], d0["volume"])
+    assert d["max_div"] <= 1e-4, d["max_div"]
+    for f, a in fields.items():
+        assert np.all(np.isfinite(a)), f
+
+
+def test_scale_run_fp32_refine_full_size():
+    """The scale run at its stated 512x512x256 (reference 512x256x512, 67 M
+    cells: 32 seeded droplets, bottom x-stripes of 60/100 deg, CFL step) in
+    the FP32+refine mode for 3 steps: phi in [0,1], volume within 1e-10,
+    max |div u| at FP32 rounding, every field finite."""
+    print("scale_run 512x256x512 fp32+refine, 3 steps", flush=True)
+    cfg, d0, d, fields = _run(3, precision=32, fp32_modes=True, config="scale_run")
+    assert (cfg.ni, cfg.nj, cfg.nk) == (512, 256, 512)
     phi = fields["phi1"][3:-3, 3:-3, 3:-3]
     assert phi.min() >= 0.0 and phi.max() <= 1.0
     assert abs(d["volume"] - d0["volume"]) <= 1e-10 * d0["volume"], (d["volume"], d0["volume"])

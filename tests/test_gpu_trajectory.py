@@ -79,3 +79,55 @@ def test_gpu_follows_the_reference_program(case, reuse, capfd):
     for k, k0 in (("solp_err", "solp_err0"), ("div_err", "div_err0")):
         assert np.all(mon[k] <= 1e-10 * mon[k0]), k
         assert np.all(fx[f"mon_{k}"] <= 1e-10 * fx[f"mon_{k0}"]), k
+
+
+@pytest.mark.parametrize("case", list(RC.RESTART_CASES))
+def test_gpu_follows_the_reference_program_from_a_restart(case, tmp_path, capfd):
+    """Droplet configurations started, like the reference's program, from a
+    restart record (fel_drv_datain, srcs/main.f90:375-404): the fields of the
+    stored steps within 1e-10 (step 1) and 1e-8, every monitor line before
+    the reference blows up within 1e-8 (sweep counts exact; the converged
+    last sweep's err and err/err0, rounding-level here, within 1e-6), and
+    the blow-up
+    at the reference's step (err0 of the Gauss-Seidel monitor >= 1e3 or NaN
+    in both)."""
+    fx = RC.fixture(case)
+    cfg = RC.restart_config(fx)
+    cfg.imon_t = 1
+    blow = RC.RESTART_CASES[case]
+    nsteps = blow if blow else len(fx["mon_sor_lp"])
+    stored = [int(s) for s in fx["steps"]]
+    path = str(tmp_path / "x.inp")
+    RC.write_restart_record(path, cfg)
+    print(f"case {case} grid {cfg.ni}x{cfg.nj}x{cfg.nk} steps {nsteps}", flush=True)
+    g = felmpf.GpuRun(cfg)
+    try:
+        g.datain(path)
+        flush_all(g)
+        capfd.readouterr()
+        done = 0
+        for n in stored + [nsteps]:
+            if n <= done:
+                continue
+            g.steps(n - done)
+            done = n
+            if n in stored:
+                tol = 1e-10 if n == 1 else 1e-8
+                for f in RC.RESTART_FIELDS:
+                    e = RC.max_rel(RC.interior(cfg, g.field(f)), fx[f"s{n}_{f}"])
+                    assert e <= tol, (case, n, f, e)
+        flush_all(g)
+        text = capfd.readouterr().out
+    finally:
+        g.close()
+    mon = RC.parse_monitor_lines(text, nsteps)
+    good = slice(0, nsteps - 1) if blow else slice(0, nsteps)
+    assert np.array_equal(mon["sor_lp"][good], fx["mon_sor_lp"][good])
+    for k in ("av_phi", "sor_ratio", "sor_err", "sor_err0", "solp_err0", "div_err0"):
+        e = np.max(np.abs(mon[k][good] / fx[f"mon_{k}"][good] - 1.0))
+        # err of the converged last sweep is ~1e-9 of the velocities here:
+        # rounding-level, so it and err/err0 agree to ~1e-7
+        assert e <= (1e-6 if k in ("sor_ratio", "sor_err") else 1e-8), (k, e)
+    if blow:
+        assert not (mon["sor_err0"][blow - 1] < 1e3), mon["sor_err0"][blow - 1]
+        assert not (fx["mon_sor_err0"][blow - 1] < 1e3)

@@ -93,3 +93,45 @@ def test_oracle_follows_the_reference_program(case):
                     assert e <= tol, (case, n, f, e)
     finally:
         o.close()
+
+
+@pytest.mark.parametrize("case", list(RC.RESTART_CASES))
+def test_oracle_follows_the_reference_program_from_a_restart(case, tmp_path):
+    """Droplet configurations, started like the reference's program from a
+    restart record (srcs/main.f90:375-404): the same sweep count and err/err0
+    (to 1e-6: the last sweep's change is rounding-level here) every step
+    before the reference blows up, the fields of the stored
+    steps within 1e-12 (step 1) and 1e-9, and the blow-up at the
+    reference's step, where the velocity jumps by six orders of magnitude
+    or becomes NaN."""
+    fx = RC.fixture(case)
+    cfg = RC.restart_config(fx)
+    blow = RC.RESTART_CASES[case]
+    path = str(tmp_path / "x.inp")
+    RC.write_restart_record(path, cfg)
+    o = OracleRun(cfg)
+    o.datain(path)
+    stored = [int(s) for s in fx["steps"]]
+    last = blow if blow else len(fx["mon_sor_lp"])
+    try:
+        umax_prev = None
+        for n in range(1, last + 1):
+            o.steps(1)
+            st = o.state()
+            u = RC.interior(cfg, o.field("u"))
+            umax = float(np.max(np.abs(u)))
+            if n == blow:
+                assert not (umax < 1e6 * umax_prev), (n, umax, umax_prev)
+                break
+            assert st["sor_lp"] == fx["mon_sor_lp"][n - 1], (n, st)
+            # err of the converged last sweep (~1e-9 of the velocities) is
+            # rounding-level: its ratio to err0 agrees to ~1e-7
+            assert abs(st["sor_ratio"] / fx["mon_sor_ratio"][n - 1] - 1) <= 1e-6, (n, st)
+            umax_prev = umax
+            if n in stored:
+                tol = 1e-12 if n == 1 else 1e-9
+                for f in RC.RESTART_FIELDS:
+                    e = RC.max_rel(RC.interior(cfg, o.field(f)), fx[f"s{n}_{f}"])
+                    assert e <= tol, (case, n, f, e)
+    finally:
+        o.close()
