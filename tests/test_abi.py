@@ -82,3 +82,22 @@ def test_driver_library_exports_host_api():
     for n in ("fel_drv_create", "fel_drv_field", "fel_drv_start", "fel_drv_steps",
               "fel_drv_download", "fel_drv_state", "fel_drv_destroy"):
         assert n in syms, f"libfeldrv.so does not export {n}"
+
+
+def test_monitor_lines_use_the_reference_edit_descriptor():
+    """The drop-ins print their monitor lines (Err_solu, Err_solp, caldt's
+    step line) with Fortran's Ew.d edit descriptor, as the reference's
+    write statements do (srcs/solu_sor4.f:299, srcs/solp_fft_tdma4.f:88,
+    srcs/caldt.f:111).  Expected strings are what amdflang prints for
+    write(*,'(e20.10)') and '(e17.10)' of the same values."""
+    lib = ctypes.CDLL(GPU_SO)
+    fn = lib._ZN3fel9fortran_eEPcdii  # fel::fortran_e(char*, double, int, int), a host function
+    fn.restype = ctypes.c_char_p
+    fn.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    buf = ctypes.create_string_buffer(64)
+    cases = [(0.0, 20, "    0.0000000000E+00"), (-0.0, 20, "   -0.0000000000E+00"),
+             (3.535572739e-3, 20, "    0.3535572739E-02"),
+             (-1.0e-101, 20, "   -0.1000000000-100"), (1.0e100, 20, "    0.1000000000+101"),
+             (123456.789, 17, " 0.1234567890E+06"), (0.99999999999, 20, "    0.1000000000E+01")]
+    for v, w, want in cases:
+        assert fn(buf, v, w, 10).decode() == want, (v, want)
