@@ -551,6 +551,10 @@ double* dev_scalars(int slot);
 // Copies n doubles from device to the pinned buffer and waits for them.
 const double* read_scalars(const double* dev, int n);
 
+// Finds this device's copy of the allocation-free solve's static work array
+// (k_sor.hip); once per device, with the device current.
+void sor_noalloc_setup();
+
 // Fortran list of edit descriptor Ew.d: v right-justified in w characters
 // as 0.<d digits>E+xx (E dropped for 3-digit exponents), the form the
 // reference's monitor lines print (e.g. srcs/solu_sor4.f:299).  Returns out.

@@ -17,6 +17,7 @@
 #include <hip/hip_ext.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "felgpu.hpp"
 #include "felgpu_bnd.hpp"
@@ -124,20 +125,23 @@ static int sor_rows() { return 4; }
 // number of partial sums (and the final reduction) small.
 constexpr int kKB = 1;
 
-template <int COMP, bool ONFLY, bool FUSE, class T>
+// R00I: the face density of each update is formed in the launch from rho and
+// rhon with the expression of the per-solve table (the same bits), for the
+// allocation-free entry point, which keeps no table.
+template <int COMP, bool ONFLY, bool FUSE, class T, bool R00I = false>
 __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a, int is, int js,
                                               int ks, int jend, int gx, int gy, int ntiles,
                                               double* part);
 
 
-template <int COMP, bool ONFLY, bool FUSE, class T>
+template <int COMP, bool ONFLY, bool FUSE, class T, bool R00I = false>
 __global__ void __launch_bounds__(512) sor_pair_kernel(Dims d, SorArgs<T> a, int is, int js, int ks,
                                                        int jend, int gx, int gy, int ntiles,
                                                        double* part) {
-  sor_pair_body<COMP, ONFLY, FUSE, T>(d, a, is, js, ks, jend, gx, gy, ntiles, part);
+  sor_pair_body<COMP, ONFLY, FUSE, T, R00I>(d, a, is, js, ks, jend, gx, gy, ntiles, part);
 }
 
-template <int COMP, bool ONFLY, bool FUSE, class T>
+template <int COMP, bool ONFLY, bool FUSE, class T, bool R00I>
 __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a, int is, int js,
                                               int ks, int jend, int gx, int gy, int ntiles,
                                               double* part) {
@@ -166,7 +170,14 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
     const T h = T(0.5);
     const int ridx = r00_index(d, bx * 64 + (int)threadIdx.x,
                                by * (int)blockDim.y + (int)threadIdx.y, k);
-    const T r00 = a.r00[COMP][ridx];
+    T r00;
+    if (R00I) {
+      const int F = COMP == 0 ? X : (COMP == 1 ? Y : Z);
+      const T q = T(0.25);
+      r00 = (a.rho[c] + a.rho[c + F] + a.rhon[c] + a.rhon[c + F]) * q;
+    } else {
+      r00 = a.r00[COMP][ridx];
+    }
     const T srcv = a.srcd[COMP] ? a.srcd[COMP][ridx] : a.src[COMP][c];
     if (COMP == 0) {
       const T visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
@@ -265,37 +276,81 @@ void launch_pair(bool onfly, bool fuse, dim3 grid, dim3 block, const Dims& d, co
                  int p, int jend, int gx, int gy, int nt, double* pp, hipEvent_t e0,
                  hipEvent_t e1) {
   hipStream_t s = stream();
+  const bool r00i = a.r00[COMP] == nullptr;  // no per-solve table: formed in the launch
   SorArgs<T> a_ = a;  // the face densities of this pair
-  a_.r00[COMP] = a.r00[COMP] + (size_t)p * r00_slab(d);
+  if (!r00i) a_.r00[COMP] = a.r00[COMP] + (size_t)p * r00_slab(d);
   if (a.srcd[COMP]) a_.srcd[COMP] = a.srcd[COMP] + (size_t)p * r00_slab(d);
   const SorArgs<T>& ap = a_;
   auto go = [&](auto kernel) {
     hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, d, ap, kIs[p], kJs[p], kKs[p],
                           jend, gx, gy, nt, pp);
   };
+  if (r00i) {
+    // the allocation-free form: coefficients from the viscosity, FP64 only
+    if constexpr (std::is_same_v<T, double>) {
+      if (fuse) go(sor_pair_kernel<COMP, true, true, T, true>);
+      else go(sor_pair_kernel<COMP, true, false, T, true>);
+    }
+    return;
+  }
   if (onfly && fuse) go(sor_pair_kernel<COMP, true, true, T>);
   else if (onfly) go(sor_pair_kernel<COMP, true, false, T>);
   else if (fuse) go(sor_pair_kernel<COMP, false, true, T>);
   else go(sor_pair_kernel<COMP, false, false, T>);
 }
 
+// Launch blocks of one colour-pair launch.
+inline int sor_blocks(const Dims& d) {
+  const int gx = ((d.ni + 1) / 2 + 63) / 64;
+  const int gy = ((d.nj + 1) / 2 + sor_rows() - 1) / sor_rows();
+  const int gz = (d.nk + kKB - 1) / kKB;
+  return gx * gy * gz;
+}
+
+// Device work memory of one solve: the block partials of the 12 launches of
+// a sweep and their chunk sums (12 nb + kErrChunks doubles), the residual
+// sum (1 double) and the slip-wall coefficients (4 (ni+6)(nk+6) elements).
+template <class P>
+struct SorWork {
+  double* part;
+  double* derr;
+  P* wcoef;
+};
+
+template <class P>
+size_t sor_work_bytes(const Dims& d) {
+  const size_t a = sizeof(double) * ((size_t)sor_blocks(d) * 12 + kErrChunks + 1);
+  const size_t wb = sizeof(P) * 4 * (size_t)d.sx * d.sz;
+  return ((a + 255) / 256) * 256 + wb;
+}
+
+// Lays the work memory out in one buffer of sor_work_bytes<P>(d) bytes.
+template <class P>
+SorWork<P> sor_work_in(const Dims& d, void* buf) {
+  const size_t a = sizeof(double) * ((size_t)sor_blocks(d) * 12 + kErrChunks + 1);
+  SorWork<P> w;
+  w.part = static_cast<double*>(buf);
+  w.derr = w.part + (size_t)sor_blocks(d) * 12 + kErrChunks;
+  w.wcoef = reinterpret_cast<P*>(static_cast<char*>(buf) + ((a + 255) / 256) * 256);
+  return w;
+}
+
 template <class T, class P = PT<T>>
 SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double uwall, double dy,
-               const P* l1, const P* l2, const P* phi, int ndiv) {
+               const P* l1, const P* l2, const P* phi, int ndiv, const SorWork<P>& work) {
   const int jen = hi ? d.nj - 1 : d.nj;
   const int gx = ((d.ni + 1) / 2 + 63) / 64;  // cells of one parity per row
   const int rows = sor_rows();                // rows of one parity per block
   const int gy = ((d.nj + 1) / 2 + rows - 1) / rows;
-  const int gz = (d.nk + kKB - 1) / kKB;
-  const int nb = gx * gy * gz;
-  double* part = static_cast<double*>(scratch(3, sizeof(double) * ((size_t)nb * 12 + kErrChunks)));
+  const int nb = sor_blocks(d);
+  double* part = work.part;
   double* chunk = part + (size_t)nb * 12;
   PartSlots ps;
   for (int s = 0; s < 12; s++) {
     ps.off[s] = s * nb;
     ps.n[s] = nb;
   }
-  double* derr = dev_scalars(1);
+  double* derr = work.derr;
   // in-launch ghost refresh needs even ni, nk (colour parity of periodic images)
   const bool fuse_ok = d.ni % 2 == 0 && d.nk % 2 == 0;
   a.l1 = l1;
@@ -307,7 +362,7 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
   a.hi = hi;
   a.wplane = d.sx * d.sz;
   {
-    P* wc = static_cast<P*>(scratch(13, sizeof(P) * 4 * (size_t)a.wplane));
+    P* wc = work.wcoef;
     const int wp = a.wplane, nj = d.nj;
     launch_flat(wp, [=] __device__(int m) {
       const int i = m % d.sx - 2, k = m / d.sx - 2;
@@ -460,9 +515,39 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
   const P* dl1 = c.in_t<P>(l1, d.n3, "l1");
   const P* dl2 = c.in_t<P>(l2, d.n3, "l2");
   const P* dphi = c.in_t<P>(phi, d.n3, "phi");
-  const SorResult r = sor4<T>(d, a, onfly, jm < 0, jp < 0, uwall, dy, dl1, dl2, dphi, ndiv);
+  const SorWork<P> work = sor_work_in<P>(d, scratch(3, sor_work_bytes<P>(d)));
+  const SorResult r = sor4<T>(d, a, onfly, jm < 0, jp < 0, uwall, dy, dl1, dl2, dphi, ndiv, work);
   c.finish();
   return r;
+}
+
+}  // namespace fel
+
+// ---------------------------------------------------------------------------
+// Allocation-free solve (felgpu_solu_sor4_noalloc_): its work memory -- the
+// residual partials, the residual sum and the slip-wall coefficients -- lives
+// in this static device array instead of library-allocated buffers.  Each
+// device holds its own copy at its own address, found once per device with
+// hipGetSymbolAddress (fel::sor_noalloc_setup, called from the library's
+// initialisation) and used through that address only: the array's name in
+// host code is not a device address.  Every call on a device uses the whole
+// array from its start, so calls that overlap on one device race; nothing in
+// it needs to start at zero (every element is written before it is read).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = size_t(16) << 20;  // 16 MiB per device
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+static void* g_noalloc_ptr[kNoAllocDevices];
+
+namespace fel {
+
+void sor_noalloc_setup() {
+  int dev = 0;
+  FEL_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices || g_noalloc_ptr[dev]) return;
+  void* ptr = nullptr;
+  FEL_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch)));
+  g_noalloc_ptr[dev] = ptr;
 }
 
 }  // namespace fel
@@ -470,6 +555,70 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
 using namespace fel;
 
 extern "C" {
+
+// Allocation-free form of solu_sor4_ (srcs/solu_sor4.f:1-308): the same
+// 8-colour Gauss-Seidel, refreshes and stop rule on device arrays, FP64,
+// coefficients formed from the viscosity as cal_arith_coef_vis forms them
+// and face densities from rho and rhon as solu_sor4 forms them.  Returns 0,
+// 1 when the grid needs more work memory than the static array holds, 2 for
+// invalid arguments (both before any launch).
+int felgpu_solu_sor4_noalloc_(const int* nID, const int* ni, const int* nj, const int* nk,
+                              const double* dt, const double* dxinv, const double* dyinv,
+                              const double* dzinv, const double* rho, const double* rhon,
+                              const double* rmun, const double* src_u, const double* src_v,
+                              const double* src_w, double* un, double* vn, double* wn,
+                              const double* uwall, const double* dy, const double* l1,
+                              const double* l2, const double* phi, int* lp, double* ratio) {
+  if (*ni < 4 || *nj < 4 || *nk < 4 ||
+      (long long)(*ni + 6) * (*nj + 6) * (*nk + 6) * 6 >= (1LL << 31)) {
+    std::fprintf(stderr, "felgpu: felgpu_solu_sor4_noalloc_: invalid grid %dx%dx%d\n", *ni, *nj, *nk);
+    return 2;
+  }
+  // the size checks are host arithmetic only: a rejected call touches no device
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const size_t need = sor_work_bytes<double>(d);
+  if (need > kNoAllocBytes) {
+    std::fprintf(stderr,
+                 "felgpu: felgpu_solu_sor4_noalloc_: grid %dx%dx%d needs %zu B of work memory, "
+                 "more than the %zu B static array\n", *ni, *nj, *nk, need, kNoAllocBytes);
+    return 1;
+  }
+  if (flow_fp32() || tangent_mode()) {
+    std::fprintf(stderr, "felgpu: felgpu_solu_sor4_noalloc_: FP64 mode only\n");
+    return 2;
+  }
+  sor_noalloc_setup();  // no-op after the library's initialisation on this device
+  int dev = 0;
+  FEL_CHECK(hipGetDevice(&dev));
+  void* buf = dev >= 0 && dev < kNoAllocDevices ? g_noalloc_ptr[dev] : nullptr;
+  if (!buf) {
+    std::fprintf(stderr, "felgpu: felgpu_solu_sor4_noalloc_: device %d has no work array\n", dev);
+    return 2;
+  }
+  SorArgs<double> a;
+  a.rho = rho;
+  a.rhon = rhon;
+  a.rmu = rmun;
+  a.sc = vis_scales<double>(*dxinv, *dyinv, *dzinv);
+  for (int m = 0; m < 45; m++) a.cf[m] = nullptr;
+  a.src[0] = src_u;
+  a.src[1] = src_v;
+  a.src[2] = src_w;
+  for (int m = 0; m < 3; m++) {
+    a.r00[m] = nullptr;
+    a.srcd[m] = nullptr;
+  }
+  a.un = un;
+  a.vn = vn;
+  a.wn = wn;
+  a.dtinv = 1.0 / *dt;
+  const int jm = nID[kYMinus], jp = nID[kYPlus];
+  const SorResult r = sor4<double>(d, a, true, jm < 0, jp < 0, *uwall, *dy, l1, l2, phi, 1,
+                                   sor_work_in<double>(d, buf));
+  *lp = r.lp;
+  *ratio = r.err / r.err0;
+  return 0;
+}
 
 // Iteration count and err/err0 of the last solu_sor4_ call (the reference
 // prints them every imon_t steps, srcs/solu_sor4.f:298-305).

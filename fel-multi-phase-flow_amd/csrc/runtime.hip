@@ -245,6 +245,7 @@ void ensure_init() {
   const char* dev = std::getenv("FELGPU_DEVICE");
   r.device = dev ? std::atoi(dev) : 0;
   FEL_CHECK(hipSetDevice(r.device));
+  sor_noalloc_setup();
   FEL_CHECK(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
   FEL_CHECK(hipHostMalloc((void**)&r.pinned, 64 * sizeof(double), hipHostMallocDefault));
   FEL_CHECK(hipMalloc((void**)&r.dscal, 64 * 16 * sizeof(double)));

@@ -301,6 +301,32 @@ void solu_sor4_(const int *ipara, const int *ID, const int *nID, const int *ndiv
  * No counterpart: the reference only prints them. */
 void felgpu_sor_last_(int *lp, double *ratio);
 
+/* Allocation-free form of solu_sor4_.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * 8-colour Gauss-Seidel, ghost refreshes and stop rule as solu_sor4_
+ * (srcs/solu_sor4.f:1-308), FP64 only, bit-identical to solu_sor4_ with the
+ * coefficients of cal_arith_coef_vis_.  Every array argument is a DEVICE
+ * pointer to a padded (-2:n+3) column-major image: rho, rhon, rmun (the
+ * viscosity the 45 coefficients are formed from), src_u..w, l1, l2, phi in;
+ * un, vn, wn in/out.  Scalars are host pointers as elsewhere.  The work
+ * memory (residual partials, slip-wall coefficients; the face densities are
+ * formed in the sweep kernels) lives in a static 16 MiB device array per
+ * GPU, located once per device at the library's initialisation: the call
+ * allocates nothing.  Limit: grids whose work memory exceeds 16 MiB
+ * (8 (12 nb + 257) + 32 (ni+6)(nk+6) bytes, nb the blocks of one launch;
+ * 5.9 MB at 512x128x256, 14.9 MB at 512x256x512) are refused.  Calls that
+ * overlap on one GPU race on that array.  *lp, *ratio: as felgpu_sor_last_.
+ * Returns 0; 1 when the grid needs more work memory than the array holds;
+ * 2 for an invalid grid or a non-FP64 mode (both before any launch). */
+int felgpu_solu_sor4_noalloc_(const int *nID, const int *ni, const int *nj, const int *nk,
+                              const double *dt, const double *dxinv, const double *dyinv,
+                              const double *dzinv, const double *rho, const double *rhon,
+                              const double *rmun, const double *src_u, const double *src_v,
+                              const double *src_w, double *un, double *vn, double *wn,
+                              const double *uwall, const double *dy, const double *l1,
+                              const double *l2, const double *phi, int *lp, double *ratio);
+
 /* Number of solu_sor4_ calls and of colour sweeps (lp+1, or 21 at the cap)
  * they made since the last reset; *reset != 0 clears the counts after
  * reading them.  No counterpart: a benchmark statistic. */
