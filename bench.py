@@ -173,16 +173,20 @@ def cpu_baseline_port(cfg_full, seconds=15.0):
                       f"= {cfg.cells} cells, {n} steps after 2 warm-up steps, {dt:.1f} s, 1 thread"}
 
 
-def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp32_modes=True):
+def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp32_modes=True,
+            refine=(3, 1.0e-7)):
     """Runs warmup + steps timesteps at the given precision (or in the
     tangent mode, seeded along the bottom wall's contact angle); returns the
     timing, the event timers and (optionally) the final fields.  In the
     FP32 mode, fp32_modes selects FP32 pressure-solve modes with FP64
-    refinement (True) or FP64 modes (False)."""
+    refinement (True) or FP64 modes (False); refine = (at most so many
+    refinement steps, relative residual tolerance)."""
     import felmpf
     from felmpf import _lib
-    run = felmpf.GpuRun(cfg, precision=precision, tangent=tangent, fp32_modes=fp32_modes)
     gpu = _lib.gpu()
+    gpu.felgpu_set_refinement_(ctypes.byref(ctypes.c_int(refine[0])),
+                               ctypes.byref(ctypes.c_double(refine[1])))
+    run = felmpf.GpuRun(cfg, precision=precision, tangent=tangent, fp32_modes=fp32_modes)
     if tangent:
         seed = np.zeros(cfg.shape)
         seed[:, 3, :] = 1.0  # d/d(theta_0) of the whole bottom wall row
@@ -200,6 +204,11 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
     gpu.felgpu_prof_reset_()
     calls, sweeps = ctypes.c_longlong(0), ctypes.c_longlong(0)
     gpu.felgpu_sor_stats_(ctypes.byref(ctypes.c_int(1)), ctypes.byref(calls), ctypes.byref(sweeps))
+    psolves, pwarm, prefine = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+    pfirst, pfinal = ctypes.c_double(0), ctypes.c_double(0)
+    pstats = (ctypes.byref(psolves), ctypes.byref(pwarm), ctypes.byref(prefine),
+              ctypes.byref(pfirst), ctypes.byref(pfinal))
+    gpu.felgpu_poisson_stats_(ctypes.byref(ctypes.c_int(1)), *pstats)
     run.sync()
     sync_dev()
     t0 = time.perf_counter()
@@ -209,6 +218,7 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
     t1 = time.perf_counter()
     gpu.felgpu_prof_enable_(ctypes.byref(ctypes.c_int(0)))
     gpu.felgpu_sor_stats_(ctypes.byref(ctypes.c_int(0)), ctypes.byref(calls), ctypes.byref(sweeps))
+    gpu.felgpu_poisson_stats_(ctypes.byref(ctypes.c_int(0)), *pstats)
     st = run.state()
     timers = {}
     for tid, name in TIMERS.items():
@@ -221,6 +231,12 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
            "sweeps_last": min(st["sor_lp"] + 1, 21),
            "phi_min": float(phi.min()), "phi_max": float(phi.max()),
            "volume_drift": float(abs(phi.sum() - vol0) / vol0)}
+    if psolves.value:
+        res["poisson_refine"] = {
+            "solves": psolves.value, "warm_started": pwarm.value,
+            "refine_steps_per_solve": round(prefine.value / psolves.value, 3),
+            "rel_residual_after_first_solve_last_step": float(f"{pfirst.value:.3e}"),
+            "rel_residual_last_step": float(f"{pfinal.value:.3e}")}
     if keep_fields:
         res["fields"] = {f: run.field(f)[3:-3, 3:-3, 3:-3] for f in ("u", "v", "w", "p")}
     run.close()
@@ -272,13 +288,23 @@ def main():
                 "volume_drift": float(f"{r32['volume_drift']:.3e}"),
                 "sor_sweeps_last_step": r32["sweeps_last"],
                 "sor_sweeps_per_step_mean": round(r32["sweeps_mean"], 3),
+                **({"poisson_refine": r32["poisson_refine"]} if "poisson_refine" in r32 else {}),
                 "dtype": dtype}
 
     if both:
         fp32 = fp32_summary(
             measure(cfg, 32, args.steps, args.warmup, keep_fields=True, fp32_modes=True),
             "f32 flow fields, stencils and pressure-solve modes; f64 volume fraction, mass "
-            "sums, contact-line velocity and pressure correction (one FP64 refinement step)")
+            "sums, contact-line velocity and pressure correction (warm-started solve, FP64 "
+            "residual refined to 1e-7)")
+        # the same mode with the residual refined to FP64 level, and unrefined
+        fp32["variants"] = {}
+        for key, lim in (("refined_to_1e-10", (3, 1.0e-10)), ("unrefined", (0, 1.0e-7))):
+            v = fp32_summary(measure(cfg, 32, args.steps, args.warmup, keep_fields=True,
+                                     fp32_modes=True, refine=lim), "")
+            fp32["variants"][key] = {k: v[k] for k in (
+                "ms_per_step", "speedup_vs_fp64", "rel_l2_error_vs_fp64", "poisson_refine")
+                if k in v}
         fp32_64 = fp32_summary(
             measure(cfg, 32, args.steps, args.warmup, keep_fields=True, fp32_modes=False),
             "f32 flow fields and stencils; f64 volume fraction, mass sums, contact-line "
@@ -321,6 +347,8 @@ def main():
                    "cells": cfg.cells, "precision": "fp64" if head_prec == 64 else "fp32+refine",
                    "sor_sweeps_last_step": main_run["sweeps_last"],
                    "sor_sweeps_per_step_mean": round(main_run["sweeps_mean"], 3),
+                   **({"poisson_refine": main_run["poisson_refine"]}
+                      if "poisson_refine" in main_run else {}),
                    "volume_drift": float(f"{main_run['volume_drift']:.3e}")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

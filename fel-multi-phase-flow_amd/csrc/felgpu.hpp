@@ -16,6 +16,7 @@
 #include <cstring>
 #include <functional>
 #include <initializer_list>
+#include <type_traits>
 #include <vector>
 
 // The public C ABI: every extern "C" definition is checked against it.
@@ -545,6 +546,79 @@ inline void interior_reduce(const Dims& d, F f, G g, double* out) {
   sum_stage2<NV, A, G><<<dim3(1), dim3(256), 0, stream()>>>(kRedBlocks, (const A*)part, out, g);
   FEL_CHECK(hipGetLastError());
 }
+
+// Reduction over the interior in tiles of launch_box's shape (64 x 4 cells,
+// kBoxRedPlanes consecutive planes per 256-thread block), for functors that
+// read neighbour cells: neighbouring rows are read by the same block and a
+// thread's next plane is its last plane's upper neighbour.  Deterministic
+// like interior_reduce: each thread adds its cells plane by plane, a fixed
+// LDS tree per block, and sum_stage2 adds the block partials in order.
+constexpr int kBoxRedPlanes = 8;
+
+template <int NV, class A, class F>
+__global__ void __launch_bounds__(256) box_sum_stage1(Dims d, int gx, int gy, F f, A* part) {
+  __shared__ A sh[NV][256];
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, tt = L / gx, by = tt % gy, bz = tt / gy;
+  const int i = 1 + bx * 64 + (int)threadIdx.x, j = 1 + by * 4 + (int)threadIdx.y;
+  A acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = A(0.0);
+  if (i <= d.ni && j <= d.nj) {
+    const int k0 = 1 + bz * kBoxRedPlanes;
+    const int k1 = k0 + kBoxRedPlanes - 1 < d.nk ? k0 + kBoxRedPlanes - 1 : d.nk;
+    for (int k = k0; k <= k1; k++) {
+      A v[NV];
+      f(i, j, k, v);
+#pragma unroll
+      for (int c = 0; c < NV; c++) acc[c] = acc[c] + v[c];
+    }
+  }
+  const int t = (int)threadIdx.x + 64 * (int)threadIdx.y;
+#pragma unroll
+  for (int c = 0; c < NV; c++) sh[c][t] = acc[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int c = 0; c < NV; c++) sh[c][t] = sh[c][t] + sh[c][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) part[(size_t)L * NV + c] = sh[c][0];
+  }
+}
+
+// interior_reduce for stencil functors (box_sum_stage1, then sum_stage2).
+template <int NV, class A = double, class F, class G>
+inline void interior_reduce_stencil(const Dims& d, F f, G g, double* out) {
+  const int gx = (d.ni + 63) / 64, gy = (d.nj + 3) / 4;
+  const int gz = (d.nk + kBoxRedPlanes - 1) / kBoxRedPlanes;
+  const int nb = gx * gy * gz;
+  A* part = static_cast<A*>(scratch(18, sizeof(A) * NV * (size_t)nb));
+  box_sum_stage1<NV, A, F><<<dim3(nb), dim3(64, 4, 1), 0, stream()>>>(d, gx, gy, f, part);
+  FEL_CHECK(hipGetLastError());
+  sum_stage2<NV, A, G><<<dim3(1), dim3(256), 0, stream()>>>(nb, (const A*)part, out, g);
+  FEL_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// FP32 mode: updates of the prognostic state.
+//
+// In FP32 arithmetic an update u + du loses the part of du below half an ulp
+// of u at every operation, and it loses it in the same direction step after
+// step wherever the flow changes slowly, so the difference from FP64 grows
+// with the step count.  The velocity and pressure updates of the FP32 mode
+// are therefore formed in FP64 from the FP32 inputs and rounded to FP32 once,
+// at the store (acc_t is their arithmetic type).  FP64 and the tangent mode
+// evaluate in their element type as the reference does.
+// ---------------------------------------------------------------------------
+
+// Arithmetic type of those updates: FP64 for FP32 storage, else the element.
+template <class T>
+using acc_t = std::conditional_t<is_f32<T>, double, T>;
 
 // Small device buffer for reduction results (slot-indexed, 16 doubles each).
 double* dev_scalars(int slot);

@@ -374,6 +374,7 @@ void avzero(const Dims d, Q* q, int ndiv) {
 template <class T, class P = PT<T>>
 void correct(const Dims d, int jen, double rho, double dxinv, double dyinv, double dzinv, double dt,
              const P* dp, const T* phat, T* un, T* vn, T* wn, T* pn) {
+  // the sums are formed in P (FP64 for FP32 storage) and rounded once
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
     const P p0 = dp[c];
@@ -396,7 +397,7 @@ void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv,
   const double as = dyinv * dyinv / rho, an = dyinv * dyinv / rho;
   const double ab = dzinv * dzinv / rho, at = dzinv * dzinv / rho;
   const double ap = aw + ae + as + an + ab + at;
-  interior_reduce<2>(
+  interior_reduce_stencil<2>(
       d,
       [=] __device__(int i, int j, int k, double* o) {
         const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
@@ -418,15 +419,18 @@ void poisson_residual_sums(const Dims d, double rho, double dxinv, double dyinv,
 // FP32 mode: iterative refinement of the pressure correction.
 //
 // The modes and the Thomas solves run in FP32; the divergence (right-hand
-// side) and the correction dp are FP64.  After the first solve, each
-// refinement step forms the FP64 residual r = div - L dp of the operator the
-// FFT+TDMA inverts (periodic in i and k, the wall rows' zero fluxes folded
-// into as(1) = an(nj) = 0), solves L c = r in FP32 and adds c to dp.  The
-// residual shrinks by the FP32 solve's relative accuracy per step.
-// solp_fft_tdma1_ records the right-hand side, solp_fft_tdma2_ the modal
-// coefficients; solp_fft_tdma3_ runs the refinement.  FELGPU_REFINE sets
-// the number of steps (default 1: one step takes the FP32 solve's relative
-// error of ~1e-6 to ~1e-12; 0 disables).
+// side) and the correction dp are FP64.  Each refinement step forms the FP64
+// residual r = div - L dp of the operator the FFT+TDMA inverts (periodic in
+// i and k, the wall rows' zero fluxes folded into as(1) = an(nj) = 0),
+// solves L c = r in FP32 and adds c to dp; the residual shrinks by the FP32
+// solve's relative accuracy (~1e-6) per step.  solp_fft_tdma1_ records the
+// right-hand side, solp_fft_tdma2_ the modal coefficients; solp_fft_tdma3_
+// refines until the relative residual |r|/|div| (L2 over the interior) is
+// at most FELGPU_REFINE_TOL (default 1e-7: the relative rounding of the FP32
+// velocities the correction updates is 6e-8), with at most FELGPU_REFINE
+// steps (default 3; 0 turns refinement and the warm start off); the
+// felgpu_set_refinement_ entry point overrides both.  Each check is one
+// FP64 reduction read back on the host.
 // ---------------------------------------------------------------------------
 struct PoissonRecord {
   bool rhs = false, coef = false;
@@ -435,6 +439,18 @@ struct PoissonRecord {
   const void* div_host = nullptr;
   double dx2r = 0.0, dz2r = 0.0;
   const double* tab = nullptr;   // device: c1x[ni], c1z[nk], as, an, ap (nj+6 each)
+  uint64_t tab_gen = 0;          // counts uploads of a changed table
+  // warm start: the correction array of the last solve, and whether (and
+  // from which contents of it and which table) solp_fft_tdma1_ transformed
+  // the residual div - L dp instead of div
+  const void* dp_host = nullptr;
+  bool warm = false;
+  uint64_t dp0_ver = 0, warm_gen = 0;
+  // statistics (felgpu_poisson_stats_): solves, warm starts, refinement
+  // steps, and the relative residuals after the first FP32 solve and at the
+  // end of the last solve
+  uint64_t solves = 0, warm_solves = 0, refine_total = 0;
+  double first_rel = 0.0, final_rel = 0.0;
 };
 
 static PoissonRecord& prec() {
@@ -442,50 +458,174 @@ static PoissonRecord& prec() {
   return r;
 }
 
-static int refine_steps() {
-  static const int n = [] {
+// Refinement limits: at most `steps` steps, stopping at relative residual
+// `tol`; from FELGPU_REFINE / FELGPU_REFINE_TOL, or felgpu_set_refinement_.
+struct RefineLimits {
+  int steps;
+  double tol;
+};
+
+static RefineLimits& refine_limits() {
+  static RefineLimits r = [] {
     const char* e = std::getenv("FELGPU_REFINE");
-    const int v = e ? std::atoi(e) : 1;
-    return v < 0 ? 0 : (v > 8 ? 8 : v);
+    const int v = e ? std::atoi(e) : 3;
+    const char* et = std::getenv("FELGPU_REFINE_TOL");
+    const double t = et ? std::atof(et) : 1.0e-7;
+    return RefineLimits{v < 0 ? 0 : (v > 8 ? 8 : v), t > 0.0 ? t : 1.0e-7};
   }();
-  return n;
+  return r;
 }
 
-static void poisson_refine(const Dims d, double* dp) {
+static int refine_steps() { return refine_limits().steps; }
+static double refine_tol() { return refine_limits().tol; }
+
+// The FP64 residual r = div - L dp of the operator the FFT+TDMA inverts
+// (periodic in i and k; the wall rows' zero fluxes are as(1) = an(nj) = 0).
+struct PoissonResidual {
+  Dims d;
+  const double *div, *dp, *as, *an;
+  double dx2r, dz2r;
+};
+
+static PoissonResidual poisson_residual_args(const Dims d, const double* div, const double* dp) {
+  const PoissonRecord& pr = prec();
+  const double* as = pr.tab + d.ni + d.nk;
+  return PoissonResidual{d, div, dp, as, as + (d.nj + 6), pr.dx2r, pr.dz2r};
+}
+
+// One pass of the residual: r = div - L dp in FP64 at every interior cell,
+// stored rounded to float into rc (compact, i fastest, then j, then k: the
+// real input of the x transforms), and per block the sums of div^2 and r^2.
+// A block covers 64 x 4 cells of kResPlanes planes; each thread marches in
+// k keeping dp(k-1), dp(k), dp(k+1) of its column in registers.
+constexpr int kResPlanes = 8;
+
+__global__ void __launch_bounds__(256) poisson_residual_kernel(PoissonResidual a, int gx, int gy,
+                                                               float* rc, double* part) {
+  __shared__ double sh[2][256];
+  const Dims& d = a.d;
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, tt = L / gx, by = tt % gy, bz = tt / gy;
+  const int i = 1 + bx * 64 + (int)threadIdx.x, j = 1 + by * 4 + (int)threadIdx.y;
+  double s0 = 0.0, s1 = 0.0;
+  if (i <= ni && j <= nj) {
+    const int Y = d.sx, Z = d.sx * d.sy;
+    const int dim = (i == 1 ? ni : i - 1) - i, dip = (i == ni ? 1 : i + 1) - i;
+    const double a_s = a.as[j + 2], a_n = a.an[j + 2];
+    const int k0 = 1 + bz * kResPlanes;
+    const int k1 = k0 + kResPlanes - 1 < nk ? k0 + kResPlanes - 1 : nk;
+    int c = d.ix(i, j, k0);
+    double pm = a.dp[k0 == 1 ? d.ix(i, j, nk) : c - Z];
+    double p0 = a.dp[c];
+    size_t o = (size_t)(i - 1) + (size_t)ni * ((size_t)(j - 1) + (size_t)nj * (size_t)(k0 - 1));
+    const size_t plane = (size_t)ni * nj;
+    for (int k = k0; k <= k1; k++) {
+      const double pp = a.dp[k == nk ? d.ix(i, j, 1) : c + Z];
+      double l = a.dx2r * (a.dp[c + dim] - 2.0 * p0 + a.dp[c + dip]) +
+                 a.dz2r * (pm - 2.0 * p0 + pp);
+      if (a_s != 0.0) l += a_s * (a.dp[c - Y] - p0);
+      if (a_n != 0.0) l += a_n * (a.dp[c + Y] - p0);
+      const double dv = a.div[c];
+      const double r = dv - l;
+      rc[o] = (float)r;
+      s0 += dv * dv;
+      s1 += r * r;
+      pm = p0;
+      p0 = pp;
+      c += Z;
+      o += plane;
+    }
+  }
+  const int t = (int)threadIdx.x + 64 * (int)threadIdx.y;
+  sh[0][t] = s0;
+  sh[1][t] = s1;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) {
+      sh[0][t] += sh[0][t + st];
+      sh[1][t] += sh[1][t + st];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * (size_t)L] = sh[0][0];
+    part[2 * (size_t)L + 1] = sh[1][0];
+  }
+}
+
+// The residual of dp into the compact float array rc; the sums of div^2 and
+// r^2 go to the device scalars `sums` (two doubles).
+static void poisson_residual_pass(const Dims d, const double* div, const double* dp, float* rc,
+                                  double* sums) {
+  const int gx = (d.ni + 63) / 64, gy = (d.nj + 3) / 4, gz = (d.nk + kResPlanes - 1) / kResPlanes;
+  const int nb = gx * gy * gz;
+  double* part = static_cast<double*>(scratch(18, sizeof(double) * 2 * (size_t)nb));
+  poisson_residual_kernel<<<dim3(nb), dim3(64, 4, 1), 0, stream()>>>(
+      poisson_residual_args(d, div, dp), gx, gy, rc, part);
+  FEL_CHECK(hipGetLastError());
+  auto g = [=] __device__(const double* t, double* out) {
+    out[0] = t[0];
+    out[1] = t[1];
+  };
+  sum_stage2<2, double, decltype(g)><<<dim3(1), dim3(256), 0, stream()>>>(nb, part, sums, g);
+  FEL_CHECK(hipGetLastError());
+}
+
+// |r| / |div| from the sums (0 when both vanish); waits for them.
+static double residual_ratio(const double* sums) {
+  const double* r = read_scalars(sums, 2);
+  if (r[0] > 0.0) return std::sqrt(r[1] / r[0]);
+  return r[1] > 0.0 ? HUGE_VAL : 0.0;
+}
+
+// The x transforms' real input from the compact residual.
+static auto compact_rhs(const Dims d, const float* rc) {
+  const size_t ni = (size_t)d.ni, nj = (size_t)d.nj;
+  return [=] __device__(int i, int j, int k) {
+    return rc[(size_t)(i - 1) + ni * ((size_t)(j - 1) + nj * (size_t)(k - 1))];
+  };
+}
+
+static float* residual_buffer(const Dims d) {
+  return static_cast<float*>(scratch(19, sizeof(float) * (size_t)d.ni * d.nj * d.nk));
+}
+
+// Refines dp (each step one FP32 solve of the FP64 residual, added in FP64)
+// until the relative residual is at most refine_tol(), with at most
+// refine_steps() steps; `warm` counts the solve as a warm start.  Each
+// residual pass checks the last iterate and is the next step's right-hand
+// side.
+static void poisson_refine(const Dims d, double* dp, bool warm) {
   PoissonRecord& pr = prec();
   const int steps = refine_steps();
-  if (steps == 0 || !pr.rhs || !pr.coef || pr.ni != d.ni || pr.nj != d.nj || pr.nk != d.nk) return;
+  if (steps <= 0 || !pr.rhs || !pr.coef || pr.ni != d.ni || pr.nj != d.nj || pr.nk != d.nk) return;
   if (content_version(pr.div_host, d.n3) != pr.div_ver) return;  // right-hand side changed
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   const size_t nm = (size_t)ni * nj * nk;
   float* mr = static_cast<float*>(scratch(9, sizeof(float) * 2 * nm));
   float* mi = mr + nm;
+  float* rc = residual_buffer(d);
+  double* sums = dev_scalars(9);
   // the right-hand side's current device copy (its buffer may have moved)
   Call c("solp_fft_tdma3_");
   const double* div = c.in_t<double>(pr.div_host, d.n3, "div");
-  const double* as = pr.tab + ni + nk;
-  const double* an = as + (nj + 6);
-  const double dx2r = pr.dx2r, dz2r = pr.dz2r;
-  for (int it = 0; it < steps; it++) {
-    // the FP64 residual r = div - L dp, formed as the transform loads it
-    fft_forward_of<float>(
-        d,
-        [=] __device__(int i, int j, int k) {
-          const int c = d.ix(i, j, k), Y = d.sx;
-          const int im = i == 1 ? ni : i - 1, ip = i == ni ? 1 : i + 1;
-          const int km = k == 1 ? nk : k - 1, kp = k == nk ? 1 : k + 1;
-          const double p0 = dp[c];
-          double l = dx2r * (dp[d.ix(im, j, k)] - 2.0 * p0 + dp[d.ix(ip, j, k)]) +
-                     dz2r * (dp[d.ix(i, j, km)] - 2.0 * p0 + dp[d.ix(i, j, kp)]);
-          const double a_s = as[j + 2], a_n = an[j + 2];
-          if (a_s != 0.0) l += a_s * (dp[c - Y] - p0);
-          if (a_n != 0.0) l += a_n * (dp[c + Y] - p0);
-          return div[c] - l;
-        },
-        mr, mi);
-    tdma_modes<float>(ni, nj, nk, pr.tab, pr.tab + ni, pr.tab + ni + nk, dx2r, dz2r, mr, mi);
+  const double tol = refine_tol();
+  poisson_residual_pass(d, div, dp, rc, sums);
+  double rel = residual_ratio(sums);
+  pr.first_rel = rel;
+  int it = 0;
+  for (; it < steps && rel > tol; it++) {
+    fft_forward_of<float>(d, compact_rhs(d, rc), mr, mi);
+    tdma_modes<float>(ni, nj, nk, pr.tab, pr.tab + ni, pr.tab + ni + nk, pr.dx2r, pr.dz2r, mr, mi);
     fft_inverse<float>(d, mr, mi, dp, true);
+    poisson_residual_pass(d, div, dp, rc, sums);
+    rel = residual_ratio(sums);
   }
+  pr.solves++;
+  pr.warm_solves += warm ? 1 : 0;
+  pr.refine_total += (uint64_t)it;
+  pr.final_rel = rel;
 }
 
 template <class T>
@@ -501,33 +641,77 @@ void caldiv_t(const Dims d, double dxinv, double dyinv, double dzinv, double dt,
   c.finish();
 }
 
+// Warm start of the FP32 solve (FP32 modes, FELGPU_REFINE >= 1): from the
+// second solve on, solp_fft_tdma1_ transforms the FP64 residual
+// r = div - L dp0 of the correction dp0 the last solve left in its array
+// instead of div, and solp_fft_tdma3_ adds the solved correction to dp0:
+// one refinement step from dp0, after which the residual check of
+// poisson_refine decides on further steps.  The mode arrays then hold the
+// residual's transform, not div's.  If dp0 or the modal table changed
+// between the two calls, tdma3 refines from dp = 0 instead.
 template <class T>
 void tdma1_t(const Dims d, const double* div, double* phir_r, double* phir_i) {
   const size_t nm = (size_t)d.ni * d.nj * d.nk;
+  PoissonRecord& rec = prec();
+  bool warm = false;
+  if constexpr (is_f32<T>)
+    warm = refine_steps() > 0 && rec.coef && rec.dp_host && rec.ni == d.ni && rec.nj == d.nj &&
+           rec.nk == d.nk && content_version(div, d.n3) != 0 &&
+           content_version(rec.dp_host, d.n3) != 0;
   Call c("solp_fft_tdma1_");
   const PT<T>* dv = c.in_t<PT<T>>(div, d.n3, "div");
   T* pr = c.out_t<T>(phir_r, nm, "phir_r");
   T* pi = c.out_t<T>(phir_i, nm, "phir_i");
-  fft_forward<T>(d, dv, pr, pi);
+  if constexpr (is_f32<T>) {
+    if (warm) {
+      const double* dp0 = c.in_t<double>(rec.dp_host, d.n3, "dp");
+      float* rc = residual_buffer(d);
+      poisson_residual_pass(d, dv, dp0, rc, dev_scalars(9));
+      fft_forward_of<T>(d, compact_rhs(d, rc), pr, pi);
+    }
+  }
+  if (!warm) fft_forward<T>(d, dv, pr, pi);
   c.finish();
   if constexpr (!is_f32<T>) return;  // the record serves the FP32 refinement
-  PoissonRecord& rec = prec();
   rec.rhs = true;
   rec.ni = d.ni; rec.nj = d.nj; rec.nk = d.nk;
   rec.div_host = div;
   rec.div_ver = content_version(div, d.n3);
   if (rec.div_ver == 0) rec.rhs = false;  // unregistered: no refinement
+  rec.warm = warm;
+  if (warm) {
+    rec.dp0_ver = content_version(rec.dp_host, d.n3);
+    rec.warm_gen = rec.tab_gen;
+  }
 }
 
 template <class T>
 void tdma3_t(const Dims d, const double* phir_r, const double* phir_i, double* dp) {
   const size_t nm = (size_t)d.ni * d.nj * d.nk;
+  PoissonRecord& rec = prec();
+  bool warm = false;
+  if constexpr (is_f32<T>)
+    warm = rec.warm && rec.dp_host == (const void*)dp && rec.rhs &&
+           content_version(dp, d.n3) == rec.dp0_ver && rec.warm_gen == rec.tab_gen;
+  const bool stale = is_f32<T> && rec.warm && !warm;  // modes of a residual no longer valid
   Call c("solp_fft_tdma3_");
   const T* pr = c.in_t<T>(phir_r, nm, "phir_r");
   const T* pi = c.in_t<T>(phir_i, nm, "phir_i");
   PT<T>* ddp = c.io_t<PT<T>>(dp, d.n3, "dp");
-  fft_inverse<T>(d, pr, pi, ddp, false);
-  if constexpr (is_f32<T>) poisson_refine(d, ddp);
+  if constexpr (is_f32<T>) {
+    if (stale) {
+      // refine from dp = 0 (its residual is div)
+      launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk},
+                 [=] __device__(int i, int j, int k) { ddp[d.ix(i, j, k)] = 0.0; });
+    } else {
+      fft_inverse<T>(d, pr, pi, ddp, warm);
+    }
+    poisson_refine(d, ddp, warm);
+    rec.warm = false;
+    rec.dp_host = dp;
+  } else {
+    fft_inverse<T>(d, pr, pi, ddp, false);
+  }
   c.finish();
 }
 
@@ -660,13 +844,18 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
     cap = h.size();
     up.clear();
   }
-  if (h != up) {
+  const bool changed = h != up;
+  if (changed) {
     up = h;
     FEL_CHECK(hipMemcpyAsync(tab, up.data(), sizeof(double) * up.size(), hipMemcpyHostToDevice,
                              stream()));
     FEL_CHECK(hipStreamSynchronize(stream()));
   }
   const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
+  {
+    PoissonRecord& rec = prec();
+    if (changed || rec.dx2r != dx2r || rec.dz2r != dz2r) rec.tab_gen++;  // a warm start's operator
+  }
   Call c("solp_fft_tdma2_");
   with_elem(elem_size(phiw_r), [&](auto t) {
     using T = decltype(t);
@@ -736,6 +925,27 @@ double felgpu_poisson_residual_(const int* ni, const int* nj, const int* nk, con
   });
   const double nn = (double)(*ni * *nj * *nk);
   return (std::sqrt(r[1] / nn) + 1.0e-99) / (std::sqrt(r[0] / nn) + 1.0e-99);
+}
+
+// Refinement limits of the FP32 pressure solve: *max_steps < 0 or
+// *tol <= 0 keeps the current value.
+void felgpu_set_refinement_(const int* max_steps, const double* tol) {
+  RefineLimits& r = refine_limits();
+  if (*max_steps >= 0) r.steps = *max_steps > 8 ? 8 : *max_steps;
+  if (*tol > 0.0) r.tol = *tol;
+}
+
+// Statistics of the FP32 mode's refined pressure solves (see
+// poisson_refine); *reset != 0 clears the counts after reading them.
+void felgpu_poisson_stats_(const int* reset, long long* solves, long long* warm_solves,
+                           long long* refine_steps, double* first_rel, double* final_rel) {
+  PoissonRecord& pr = prec();
+  *solves = (long long)pr.solves;
+  *warm_solves = (long long)pr.warm_solves;
+  *refine_steps = (long long)pr.refine_total;
+  *first_rel = pr.first_rel;
+  *final_rel = pr.final_rel;
+  if (*reset) pr.solves = pr.warm_solves = pr.refine_total = 0;
 }
 
 // srcs/bnd_avzero.f:1 (ipara must be 0)

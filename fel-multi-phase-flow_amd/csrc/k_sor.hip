@@ -152,7 +152,8 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
   const int k = 1 + bz * kKB + kz;
   if (k > d.nk) break;
   const int q = (k - ks) & 1;
-  const int i = 1 + ((is + q - 1) & 1) + 2 * (bx * 64 + (int)threadIdx.x);
+  const int ii = bx * 64 + (int)threadIdx.x;
+  const int i = 1 + ((is + q - 1) & 1) + 2 * ii;
   const int j = 1 + ((js + q - 1) & 1) + 2 * (by * (int)blockDim.y + (int)threadIdx.y);
   if (i <= d.ni && j <= jend) {
     const int X = 1, Y = d.sx, Z = d.sx * d.sy;
@@ -168,8 +169,8 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
     }
     const T *un = a.un, *vn = a.vn, *wn = a.wn;
     const T h = T(0.5);
-    const int ridx = r00_index(d, bx * 64 + (int)threadIdx.x,
-                               by * (int)blockDim.y + (int)threadIdx.y, k);
+    using A = acc_t<T>;  // FP64 for FP32 storage (rounded once, at the store), else T
+    const int ridx = r00_index(d, ii, by * (int)blockDim.y + (int)threadIdx.y, k);
     T r00;
     if (R00I) {
       const int F = COMP == 0 ? X : (COMP == 1 ? Y : Z);
@@ -180,38 +181,59 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
     }
     const T srcv = a.srcd[COMP] ? a.srcd[COMP][ridx] : a.src[COMP][c];
     if (COMP == 0) {
-      const T visn = (+cf[4] * un[c - Z] + cf[2] * un[c - Y] + cf[0] * un[c - X] +
-                           cf[1] * un[c + X] + cf[3] * un[c + Y] + cf[5] * un[c + Z] +
-                           cf[7] * vn[c - Y] + cf[8] * vn[c + X - Y] + cf[9] * vn[c] +
-                           cf[10] * vn[c + X] + cf[11] * wn[c - Z] + cf[12] * wn[c + X - Z] +
-                           cf[13] * wn[c] + cf[14] * wn[c + X]) /
-                          r00;
-      const T du = (visn * h + srcv) / (a.dtinv + cf[6] / r00 * h) - un[c];
-      const T nv = un[c] + du;
+      const A visn = (+A(cf[4]) * A(un[c - Z]) + A(cf[2]) * A(un[c - Y]) + A(cf[0]) * A(un[c - X]) +
+                           A(cf[1]) * A(un[c + X]) + A(cf[3]) * A(un[c + Y]) + A(cf[5]) * A(un[c + Z]) +
+                           A(cf[7]) * A(vn[c - Y]) + A(cf[8]) * A(vn[c + X - Y]) + A(cf[9]) * A(vn[c]) +
+                           A(cf[10]) * A(vn[c + X]) + A(cf[11]) * A(wn[c - Z]) + A(cf[12]) * A(wn[c + X - Z]) +
+                           A(cf[13]) * A(wn[c]) + A(cf[14]) * A(wn[c + X])) /
+                          A(r00);
+      const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
+      T du, nv;
+      if constexpr (is_f32<T>) {
+        nv = T(f);
+        du = nv - un[c];
+      } else {
+        du = T(f) - un[c];
+        nv = un[c] + du;
+      }
       a.un[c] = nv;
       if (FUSE) write_images<0>(d, a, a.un, i, j, k, nv);
       e += val(du) * val(du);
     } else if (COMP == 1) {
-      const T visn = (+cf[4] * vn[c - Z] + cf[2] * vn[c - Y] + cf[0] * vn[c - X] +
-                           cf[1] * vn[c + X] + cf[3] * vn[c + Y] + cf[5] * vn[c + Z] +
-                           cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Y] +
-                           cf[10] * un[c + Y] + cf[11] * wn[c - Z] + cf[12] * wn[c + Y - Z] +
-                           cf[13] * wn[c] + cf[14] * wn[c + Y]) /
-                          r00;
-      const T dv = (visn * h + srcv) / (a.dtinv + cf[6] / r00 * h) - vn[c];
-      const T nv = vn[c] + dv;
+      const A visn = (+A(cf[4]) * A(vn[c - Z]) + A(cf[2]) * A(vn[c - Y]) + A(cf[0]) * A(vn[c - X]) +
+                           A(cf[1]) * A(vn[c + X]) + A(cf[3]) * A(vn[c + Y]) + A(cf[5]) * A(vn[c + Z]) +
+                           A(cf[7]) * A(un[c - X]) + A(cf[9]) * A(un[c]) + A(cf[8]) * A(un[c - X + Y]) +
+                           A(cf[10]) * A(un[c + Y]) + A(cf[11]) * A(wn[c - Z]) + A(cf[12]) * A(wn[c + Y - Z]) +
+                           A(cf[13]) * A(wn[c]) + A(cf[14]) * A(wn[c + Y])) /
+                          A(r00);
+      const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
+      T dv, nv;
+      if constexpr (is_f32<T>) {
+        nv = T(f);
+        dv = nv - vn[c];
+      } else {
+        dv = T(f) - vn[c];
+        nv = vn[c] + dv;
+      }
       a.vn[c] = nv;
       if (FUSE) write_images<1>(d, a, a.vn, i, j, k, nv);
       e += val(dv) * val(dv);
     } else {
-      const T visn = (+cf[4] * wn[c - Z] + cf[2] * wn[c - Y] + cf[0] * wn[c - X] +
-                           cf[1] * wn[c + X] + cf[3] * wn[c + Y] + cf[5] * wn[c + Z] +
-                           cf[7] * un[c - X] + cf[9] * un[c] + cf[8] * un[c - X + Z] +
-                           cf[10] * un[c + Z] + cf[11] * vn[c - Y] + cf[13] * vn[c] +
-                           cf[12] * vn[c - Y + Z] + cf[14] * vn[c + Z]) /
-                          r00;
-      const T dw = (visn * h + srcv) / (a.dtinv + cf[6] / r00 * h) - wn[c];
-      const T nv = wn[c] + dw;
+      const A visn = (+A(cf[4]) * A(wn[c - Z]) + A(cf[2]) * A(wn[c - Y]) + A(cf[0]) * A(wn[c - X]) +
+                           A(cf[1]) * A(wn[c + X]) + A(cf[3]) * A(wn[c + Y]) + A(cf[5]) * A(wn[c + Z]) +
+                           A(cf[7]) * A(un[c - X]) + A(cf[9]) * A(un[c]) + A(cf[8]) * A(un[c - X + Z]) +
+                           A(cf[10]) * A(un[c + Z]) + A(cf[11]) * A(vn[c - Y]) + A(cf[13]) * A(vn[c]) +
+                           A(cf[12]) * A(vn[c - Y + Z]) + A(cf[14]) * A(vn[c + Z])) /
+                          A(r00);
+      const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
+      T dw, nv;
+      if constexpr (is_f32<T>) {
+        nv = T(f);
+        dw = nv - wn[c];
+      } else {
+        dw = T(f) - wn[c];
+        nv = wn[c] + dw;
+      }
       a.wn[c] = nv;
       if (FUSE) write_images<2>(d, a, a.wn, i, j, k, nv);
       e += val(dw) * val(dw);

@@ -28,9 +28,10 @@ void arith_mean(const Dims d, const P* phik, double ql, double qg, T* qk) {
 
 template <class T>
 void calphat(const Dims d, const T* p, const T* po, T* phat) {
+  using A = acc_t<T>;  // FP64 for FP32 storage, rounded once at the store
   launch_box(Box{-1, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k);
-    phat[c] = p[c] * T(2.0) - po[c];
+    phat[c] = T(A(p[c]) * A(2.0) - A(po[c]));
   });
 }
 
@@ -208,31 +209,41 @@ void sij_tau(const Dims d, SijTau<T> a) {
 // srcs/cal_srcu.f:59-136
 template <class T>
 void srcu(const Dims d, int jen, double dtinv_, double grvb_, double grvp_, SrcArgs<T> a) {
-  const T dtinv = T(dtinv_), grvb = T(grvb_), grvp = T(grvp_);
-  const T q = T(0.25), h = T(0.5), t15 = T(1.5);
+  // A: FP64 for FP32 storage, whose source (u/dt plus increments) is
+  // rounded once at the store; FP64 and the tangent mode evaluate in T
+  using A = acc_t<T>;
+  const A dtinv = A(T(dtinv_)), grvb = A(T(grvb_)), grvp = A(T(grvp_));
+  const A q = A(0.25), h = A(0.5), t15 = A(1.5);
   launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
     const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    auto sum = [&](const T* f, int bit) { return (a.addz & bit) ? T(0.0) + f[c] : f[c]; };
+    auto sum = [&](const T* f, int bit) { return (a.addz & bit) ? A(T(0.0) + f[c]) : A(f[c]); };
+    auto ld = [&](const T* f, int o_) { return A(f[o_]); };
     const size_t slab4 = 4 * r00_slab(d);
     const size_t o = pair_of(i, j, k) * r00_slab(d) + r00_index(d, (i - 1) >> 1, (j - 1) >> 1, k);
     {
-      const T r = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q;
-      const T s = +a.u[c] * dtinv - a.adv_u[c] * t15 + a.adv_uo[c] * h - a.prs_u[c] / r +
-                  a.vis_u[c] / r * h + sum(a.fst_u, 1) * h + sum(a.fst_un, 8) * h + grvb;
+      const A r = (ld(a.rho, c) + ld(a.rho, c + X) + ld(a.rhon, c) + ld(a.rhon, c + X)) * q;
+      const A sv = +ld(a.u, c) * dtinv - ld(a.adv_u, c) * t15 + ld(a.adv_uo, c) * h -
+                   ld(a.prs_u, c) / r + ld(a.vis_u, c) / r * h + sum(a.fst_u, 1) * h +
+                   sum(a.fst_un, 8) * h + grvb;
+      const T s = T(sv);
       a.src_u[c] = s;
       if (a.dense) a.dense[o] = s;
     }
     if (j <= jen) {
-      const T r = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q;
-      const T s = +a.v[c] * dtinv - a.adv_v[c] * t15 + a.adv_vo[c] * h - a.prs_v[c] / r +
-                  a.vis_v[c] / r * h + sum(a.fst_v, 2) * h + sum(a.fst_vn, 16) * h + grvp;
+      const A r = (ld(a.rho, c) + ld(a.rho, c + Y) + ld(a.rhon, c) + ld(a.rhon, c + Y)) * q;
+      const A sv = +ld(a.v, c) * dtinv - ld(a.adv_v, c) * t15 + ld(a.adv_vo, c) * h -
+                   ld(a.prs_v, c) / r + ld(a.vis_v, c) / r * h + sum(a.fst_v, 2) * h +
+                   sum(a.fst_vn, 16) * h + grvp;
+      const T s = T(sv);
       a.src_v[c] = s;
       if (a.dense) a.dense[slab4 + o] = s;
     }
     {
-      const T r = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q;
-      const T s = +a.w[c] * dtinv - a.adv_w[c] * t15 + a.adv_wo[c] * h - a.prs_w[c] / r +
-                  a.vis_w[c] / r * h + sum(a.fst_w, 4) * h + sum(a.fst_wn, 32) * h;
+      const A r = (ld(a.rho, c) + ld(a.rho, c + Z) + ld(a.rhon, c) + ld(a.rhon, c + Z)) * q;
+      const A sv = +ld(a.w, c) * dtinv - ld(a.adv_w, c) * t15 + ld(a.adv_wo, c) * h -
+                   ld(a.prs_w, c) / r + ld(a.vis_w, c) / r * h + sum(a.fst_w, 4) * h +
+                   sum(a.fst_wn, 32) * h;
+      const T s = T(sv);
       a.src_w[c] = s;
       if (a.dense) a.dense[2 * slab4 + o] = s;
     }

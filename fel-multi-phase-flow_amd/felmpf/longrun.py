@@ -75,8 +75,14 @@ def _run(cfg, checkpoints, **kw):
 
 
 def mixed_precision(name: str = "droplet_impact", steps: int = 1000, every: int = 250,
-                    fp32_modes: bool = True) -> dict:
+                    fp32_modes: bool = True, cfl: float = 0.0) -> dict:
+    """FP64 and FP32 runs of the same configuration from the same initial
+    state; cfl > 0 takes each step's dt from caldt (srcs/caldt.f) instead of
+    the configuration's fixed dt."""
+    from dataclasses import replace
     cfg = CONFIGS[name]()
+    if cfl > 0:
+        cfg = replace(cfg, cfl=cfl)
     checkpoints = list(range(every, steps + 1, every))
     if not checkpoints or checkpoints[-1] != steps:
         checkpoints.append(steps)
@@ -98,6 +104,7 @@ def mixed_precision(name: str = "droplet_impact", steps: int = 1000, every: int 
                              "fp32": abs(m["volume"] - b0["volume"]) / b0["volume"]},
         })
     return {"config": f"{cfg.name} {cfg.ni}x{cfg.nk}x{cfg.nj} (ref i,j,k = {cfg.ni}x{cfg.nj}x{cfg.nk})",
+            "dt": f"caldt, cfl {cfg.cfl}" if cfg.cfl > 0 else cfg.dt,
             "steps": steps, "fp32_variant": "FP32+refine" if fp32_modes else "FP32, FP64 solve modes",
             "ms_per_step": {"fp64": 1e3 * t64 / steps, "fp32": 1e3 * t32 / steps},
             "checkpoints": rows}
@@ -187,6 +194,8 @@ def main():
     ap.add_argument("--directions", type=int, default=0,
                     help="> 0: the directional-derivative check instead (patterned wall)")
     ap.add_argument("--h", type=float, default=1e-6)
+    ap.add_argument("--cfl", type=float, default=0.0,
+                    help="> 0: dt from caldt each step (CFL number) instead of the fixed dt")
     ap.add_argument("--scale", default="", help="ni,nj,nk (reference i,j,k) of a coarser grid")
     args = ap.parse_args()
     if args.directions > 0:
@@ -195,7 +204,7 @@ def main():
         rep = directional_check(name, args.steps, args.directions, args.h, scale=scale)
         print(json.dumps(rep), flush=True)
         return
-    rep = mixed_precision(args.config, args.steps, args.every, not args.fp64_modes)
+    rep = mixed_precision(args.config, args.steps, args.every, not args.fp64_modes, args.cfl)
     print(json.dumps(rep), flush=True)
 
 

@@ -425,6 +425,22 @@ double felgpu_poisson_residual_(const int *ni, const int *nj, const int *nk, con
                                 const double *dxinv, const double *dyinv, const double *dzinv,
                                 const double *div, const double *dp);
 
+/* Limits of the FP32 mode's iterative refinement of the pressure
+ * correction: at most *max_steps refinement steps (0..8; 0 also turns the
+ * warm start off), stopping once |div - L dp| / |div| <= *tol.  A negative
+ * *max_steps or a non-positive *tol keeps that limit.  Defaults: 3 and 1e-7
+ * (FELGPU_REFINE, FELGPU_REFINE_TOL).  No counterpart: the reference solves
+ * in FP64 without refinement. */
+void felgpu_set_refinement_(const int *max_steps, const double *tol);
+
+/* The FP32 mode's refined pressure solves since the last reset: solves,
+ * warm-started solves, refinement steps, and the relative residual
+ * |div - L dp| / |div| after the last solve's first FP32 solve and at its
+ * end; *reset != 0 clears the counts after reading them.  No counterpart:
+ * the reference solves in FP64 without refinement. */
+void felgpu_poisson_stats_(const int *reset, long long *solves, long long *warm_solves,
+                           long long *refine_steps, double *first_rel, double *final_rel);
+
 /* srcs/bnd_avzero.f:1 -- subtract the interior mean from q (padded image).
  * *ipara must be 0. */
 void bnd_avzero_(const int *ipara, const int *ndiv, const int *ni, const int *nj, const int *nk,
