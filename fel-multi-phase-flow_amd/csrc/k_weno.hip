@@ -31,16 +31,50 @@ __device__ __forceinline__ T weight_rcp(T x) {
   else return T(1.0) / x;
 }
 
+// One upwind-biased side of srcs/weno5.h: w1*df1 + w2*df2 + w3*df3 of the
+// side whose differences are (d5, d3, d1, e1, e3) -- the m side (dfm5,
+// dfm3, dfm1, dfp1, dfp3), the p side mirrored (dfp5, dfp3, dfp1, dfm1,
+// dfm3) -- with the candidate derivatives from those differences and the
+// smoothness indicators from their scaled copies (s5, s3, s1, t1, t3);
+// expressions in the reference's order.
 template <class T>
-__device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp5, T vel00) {
+__device__ __forceinline__ T weno_side(T d5, T d3, T d1, T e1, T e3, T s5, T s3, T s1, T t1,
+                                       T t3) {
   const T c2 = T(2.0), c3 = T(3.0), c4 = T(4.0), c5 = T(5.0), c7 = T(7.0), c11 = T(11.0),
           c13 = T(13.0);
-  const T dfm_1 = dfm5 * c2 - dfm3 * c7 + dfm1 * c11;
-  const T dfm_2 = -dfm3 + dfm1 * c5 + dfp1 * c2;
-  const T dfm_3 = dfm1 * c2 + dfp1 * c5 - dfp3;
-  const T dfp_1 = dfp5 * c2 - dfp3 * c7 + dfp1 * c11;
-  const T dfp_2 = -dfp3 + dfp1 * c5 + dfm1 * c2;
-  const T dfp_3 = dfp1 * c2 + dfm1 * c5 - dfm3;
+  const T df_1 = d5 * c2 - d3 * c7 + d1 * c11;
+  const T df_2 = -d3 + d1 * c5 + e1 * c2;
+  const T df_3 = d1 * c2 + e1 * c5 - e3;
+  T a, b;
+  a = s5 - s3 * c2 + s1;
+  b = s5 - s3 * c4 + s1 * c3;
+  const T sm_1 = a * a * c13 + b * b * c3;
+  a = s3 - s1 * c2 + t1;
+  b = s3 - t1;
+  const T sm_2 = a * a * c13 + b * b * c3;
+  a = s1 - t1 * c2 + t3;
+  b = s1 * c3 - t1 * c4 + t3;
+  const T sm_3 = a * a * c13 + b * b * c3;
+  T a_1 = sm_2 * sm_3, a_2 = sm_1 * sm_3, a_3 = sm_1 * sm_2;
+  a_1 = a_1 * a_1;
+  a_2 = a_2 * a_2;
+  a_3 = a_3 * a_3;
+  a_2 = a_2 * T(6.0);
+  a_3 = a_3 * c3;
+  const T deninv = weight_rcp(a_1 + a_2 + a_3 + guard<T>(1.0e-80));
+  const T w_1 = a_1 * deninv, w_3 = a_3 * deninv;
+  const T w_2 = T(1.0) - w_1 - w_3;
+  return w_1 * df_1 + w_2 * df_2 + w_3 * df_3;
+}
+
+// srcs/weno5.h:24-71: (velm1 * m side + velp1 * p side) / 12 with velm1 =
+// vel + |vel| and velp1 = vel - |vel|.  For vel != 0 one of the two factors
+// is an exact zero and its side is not evaluated: the sum is the other
+// term's value, the same bits (up to the sign of a zero result), and in the
+// tangent mode the zero factor's derivative is zero too.  vel = 0 (or NaN)
+// evaluates both sides as written.
+template <class T>
+__device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp5, T vel00) {
   T m5 = dfm5, m3 = dfm3, m1 = dfm1, p1 = dfp1, p3 = dfp3, p5 = dfp5;
   if constexpr (is_f32<T>) {
     const T sc = fmax(fmax(fmax(fabs(m5), fabs(m3)), fmax(fabs(m1), fabs(p1))),
@@ -52,48 +86,16 @@ __device__ __forceinline__ T weno5(T dfm5, T dfm3, T dfm1, T dfp1, T dfp3, T dfp
     m5 = ldexpf(m5, -e); m3 = ldexpf(m3, -e); m1 = ldexpf(m1, -e);
     p1 = ldexpf(p1, -e); p3 = ldexpf(p3, -e); p5 = ldexpf(p5, -e);
   }
-  T a, b;
-  a = m5 - m3 * c2 + m1;
-  b = m5 - m3 * c4 + m1 * c3;
-  const T sm_1 = a * a * c13 + b * b * c3;
-  a = m3 - m1 * c2 + p1;
-  b = m3 - p1;
-  const T sm_2 = a * a * c13 + b * b * c3;
-  a = m1 - p1 * c2 + p3;
-  b = m1 * c3 - p1 * c4 + p3;
-  const T sm_3 = a * a * c13 + b * b * c3;
-  a = p5 - p3 * c2 + p1;
-  b = p5 - p3 * c4 + p1 * c3;
-  const T sp_1 = a * a * c13 + b * b * c3;
-  a = p3 - p1 * c2 + m1;
-  b = p3 - m1;
-  const T sp_2 = a * a * c13 + b * b * c3;
-  a = p1 - m1 * c2 + m3;
-  b = p1 * c3 - m1 * c4 + m3;
-  const T sp_3 = a * a * c13 + b * b * c3;
-  T am_1 = sm_2 * sm_3, am_2 = sm_1 * sm_3, am_3 = sm_1 * sm_2;
-  T ap_1 = sp_2 * sp_3, ap_2 = sp_1 * sp_3, ap_3 = sp_1 * sp_2;
-  am_1 = am_1 * am_1;
-  am_2 = am_2 * am_2;
-  am_3 = am_3 * am_3;
-  ap_1 = ap_1 * ap_1;
-  ap_2 = ap_2 * ap_2;
-  ap_3 = ap_3 * ap_3;
-  am_2 = am_2 * T(6.0);
-  am_3 = am_3 * c3;
-  ap_2 = ap_2 * T(6.0);
-  ap_3 = ap_3 * c3;
-  const T deninv_am = weight_rcp(am_1 + am_2 + am_3 + guard<T>(1.0e-80));
-  const T deninv_ap = weight_rcp(ap_1 + ap_2 + ap_3 + guard<T>(1.0e-80));
-  const T wgm_1 = am_1 * deninv_am, wgm_3 = am_3 * deninv_am;
-  const T wgp_1 = ap_1 * deninv_ap, wgp_3 = ap_3 * deninv_ap;
-  const T wgm_2 = T(1.0) - wgm_1 - wgm_3;
-  const T wgp_2 = T(1.0) - wgp_1 - wgp_3;
   const T velab = fabs(vel00);
   const T velm1 = vel00 + velab, velp1 = vel00 - velab;
-  return (velm1 * (wgm_1 * dfm_1 + wgm_2 * dfm_2 + wgm_3 * dfm_3) +
-          velp1 * (wgp_1 * dfp_1 + wgp_2 * dfp_2 + wgp_3 * dfp_3)) *
-         (T(1.0) / T(12.0));
+  const T r12 = T(1.0) / T(12.0);
+  if (vel00 > T(0.0))
+    return (velm1 * weno_side(dfm5, dfm3, dfm1, dfp1, dfp3, m5, m3, m1, p1, p3)) * r12;
+  if (vel00 < T(0.0))
+    return (velp1 * weno_side(dfp5, dfp3, dfp1, dfm1, dfm3, p5, p3, p1, m1, m3)) * r12;
+  return (velm1 * weno_side(dfm5, dfm3, dfm1, dfp1, dfp3, m5, m3, m1, p1, p3) +
+          velp1 * weno_side(dfp5, dfp3, dfp1, dfm1, dfm3, p5, p3, p1, m1, m3)) *
+         r12;
 }
 
 template <class T>
