@@ -6,6 +6,8 @@
 // srcs/cal_srcu.f, srcs/cal_arith_coef_vis.f, srcs/cpy.f, srcs/init_q.f,
 // srcs/sum_fst.f.  Each kernel writes exactly the index ranges the reference
 // loops write and leaves every other element untouched.
+#include <cstdlib>
+
 #include "felgpu.hpp"
 #include "felgpu_visc.hpp"
 
@@ -93,27 +95,62 @@ void arith_tau(const Dims d, const T* s, const T* rmu, T* tau) {
   });
 }
 
-// srcs/cal_div_tensor.f:20-86 (clearing folded into the same pass)
+// srcs/cal_div_tensor.f:20-86 (clearing folded into the same pass).  One
+// thread per (i,j) column of the padded image and kDivChunk planes: tau13
+// and tau23 at k-1 and tau33 at k+1 are the column's own values of the
+// neighbouring plane, kept in registers instead of read again.  Each output
+// is the expression of the one-thread-per-cell form on the same operands.
+// Measured at 512x256x128 (FP64): one cell per thread 345 us, chunks of 2
+// planes 328 us, 4: 334, 8: 344, 16: 361 (longer chunks leave too few
+// threads to cover the load latency).
+constexpr int kDivChunk = 2;
+
 template <class T>
-void div_tensor(const Dims d, double dxinv_, double dyinv_, double dzinv_, const T* tau, T* vu,
-                T* vv, T* vw) {
-  const int n3 = d.n3, ni = d.ni, nj = d.nj, nk = d.nk;
-  const T dxinv = T(dxinv_), dyinv = T(dyinv_), dzinv = T(dzinv_);
+__global__ void __launch_bounds__(256) div_tensor_kernel(Dims d, T dxinv, T dyinv, T dzinv,
+                                                         const T* __restrict__ tau,
+                                                         T* __restrict__ vu, T* __restrict__ vv,
+                                                         T* __restrict__ vw, int gx, int gy) {
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, tt = L / gx, by = tt % gy, bz = tt / gy;
+  const int ni = d.ni, nj = d.nj, nk = d.nk, n3 = d.n3;
+  const int i = -2 + bx * 64 + (int)threadIdx.x, j = -2 + by * 4 + (int)threadIdx.y;
+  if (i > ni + 3 || j > nj + 3) return;
+  const int k0 = -2 + bz * kDivChunk, k1 = min(nk + 3, k0 + kDivChunk - 1);
   const T *t1 = tau, *t2 = tau + n3, *t3 = tau + 2 * n3, *t4 = tau + 3 * n3, *t5 = tau + 4 * n3,
           *t6 = tau + 5 * n3;
-  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
-    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const bool ia = inb(i, -2, ni + 2), ib = inb(i, -1, ni + 3);
+  const bool ja = inb(j, -1, nj + 3), jb = inb(j, -2, nj + 2);
+  int c = d.ix(i, j, k0);
+  T t5m = k0 - 1 >= -2 ? t5[c - Z] : T(0.0);  // tau13, tau23 at k-1
+  T t6m = k0 - 1 >= -2 ? t6[c - Z] : T(0.0);
+  T t3c = t3[c];                              // tau33 at k
+  for (int k = k0; k <= k1; k++, c += Z) {
+    const T t5c = t5[c], t6c = t6[c];
+    const T t3p = k + 1 <= nk + 3 ? t3[c + Z] : T(0.0);
     T a = T(0.0), b = T(0.0), e = T(0.0);
-    if (inb(k, -1, nk + 3) && inb(j, -1, nj + 3) && inb(i, -2, ni + 2))
-      a = +(-t1[c] + t1[c + X]) * dxinv + (-t4[c - Y] + t4[c]) * dyinv + (-t5[c - Z] + t5[c]) * dzinv;
-    if (inb(k, -1, nk + 3) && inb(j, -2, nj + 2) && inb(i, -1, ni + 3))
-      b = +(-t4[c - X] + t4[c]) * dxinv + (-t2[c] + t2[c + Y]) * dyinv + (-t6[c - Z] + t6[c]) * dzinv;
-    if (inb(k, -2, nk + 2) && inb(j, -1, nj + 3) && inb(i, -1, ni + 3))
-      e = +(-t5[c - X] + t5[c]) * dxinv + (-t6[c - Y] + t6[c]) * dyinv + (-t3[c] + t3[c + Z]) * dzinv;
+    if (inb(k, -1, nk + 3) && ja && ia)
+      a = +(-t1[c] + t1[c + X]) * dxinv + (-t4[c - Y] + t4[c]) * dyinv + (-t5m + t5c) * dzinv;
+    if (inb(k, -1, nk + 3) && jb && ib)
+      b = +(-t4[c - X] + t4[c]) * dxinv + (-t2[c] + t2[c + Y]) * dyinv + (-t6m + t6c) * dzinv;
+    if (inb(k, -2, nk + 2) && ja && ib)
+      e = +(-t5[c - X] + t5c) * dxinv + (-t6[c - Y] + t6c) * dyinv + (-t3c + t3p) * dzinv;
     vu[c] = a;
     vv[c] = b;
     vw[c] = e;
-  });
+    t5m = t5c;
+    t6m = t6c;
+    t3c = t3p;
+  }
+}
+
+template <class T>
+void div_tensor(const Dims d, double dxinv, double dyinv, double dzinv, const T* tau, T* vu,
+                T* vv, T* vw) {
+  const int gx = (d.sx + 63) / 64, gy = (d.sy + 3) / 4, gz = (d.sz + kDivChunk - 1) / kDivChunk;
+  div_tensor_kernel<T><<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(
+      d, T(dxinv), T(dyinv), T(dzinv), tau, vu, vv, vw, gx, gy);
+  FEL_CHECK(hipGetLastError());
 }
 
 // calsij followed by arith_tau in one pass (the chain calsij_ ->
@@ -138,7 +175,9 @@ struct SijTau {
 // once per column instead of once per plane touching it.  Per cell the
 // operations are calsij's (srcs/calsij.f:28-139, where it writes) and
 // arith_tau's (srcs/cal_arith_tau.f:21-118), in the same order.
-constexpr int kSijChunk = 16;
+// (measured at 512x256x128 in FP64: 674 us with 2 planes, 691 with 4, 723
+// with 8, 752 with 16: short chunks leave more threads to cover latency)
+constexpr int kSijChunk = 2;
 
 template <class T>
 __global__ void __launch_bounds__(256) sij_tau_kernel(Dims d, SijTau<T> a, int gx, int gy) {

@@ -7,6 +7,7 @@
 // order (the library is compiled without contraction).
 #include <cmath>
 
+#include <cstdlib>
 #include "felgpu.hpp"
 
 namespace fel {
@@ -247,30 +248,78 @@ void mthinc_update(const Dims d, double dxinv, double dyinv, double dzinv, doubl
 // srcs/solphi_mthinc3.f:43-159: truncation on [0,n+1]^3 (clamped in the
 // interior in the same pass, the reference's second loop), the three global
 // sums, and the mass correction.  The offset stays on the device.
-template <class T, class P = PT<T>>
-void mthinc_mass(const Dims d, MthincConst mc64, const T* phix, const T* phiy, const T* phiz,
-                 const P* phi, P* phin) {
-  const MthincC<T> mc(mc64);
-  launch_box(Box{0, d.ni + 1, 0, d.nj + 1, 0, d.nk + 1}, [=] __device__(int i, int j, int k) {
-    const int X = 1, Y = d.sx, Z = d.sx * d.sy;
-    const int c = d.ix(i, j, k);
-    const T rnx = vavg8(phix, c, X, Y, Z);
-    const T rny = vavg8(phiy, c, X, Y, Z);
-    const T rnz = vavg8(phiz, c, X, Y, Z);
-    const T q00 = sqrt(rnx * rnx + rny * rny + rnz * rnz);
+// The truncation pass of srcs/solphi_mthinc3.f:43-83 on [0,n+1]^3.  One
+// thread per (i,j) column, marching over kMassChunk planes: the 8-point
+// averages of the gradients take their k-1 plane from the values the column
+// loaded for the previous plane, so each plane of phix, phiy, phiz is
+// fetched once per column.  The sums are vavg8's, in its order.
+// (measured at 512x256x128 in FP64: 177 us with 8 planes, 183 with 2, 176
+// with 4, 180 with 16)
+constexpr int kMassChunk = 8;
+
+template <class T, class P>
+__global__ void __launch_bounds__(256) mthinc_trunc_kernel(Dims d, MthincC<T> mc, double eps_trunc,
+                                                           const T* __restrict__ phix,
+                                                           const T* __restrict__ phiy,
+                                                           const T* __restrict__ phiz,
+                                                           P* __restrict__ phin, int gx, int gy) {
+  const int L = (int)blockIdx.x;
+  const int bx = L % gx, tt = L / gx, by = tt % gy, bz = tt / gy;
+  const int i = bx * 64 + (int)threadIdx.x, j = by * 4 + (int)threadIdx.y;
+  if (i > d.ni + 1 || j > d.nj + 1) return;
+  const int k0 = bz * kMassChunk, k1 = min(d.nk + 1, k0 + kMassChunk - 1);
+  const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const T* g[3] = {phix, phiy, phiz};
+  // q[m][0..3]: array m at (i-1,j-1), (i,j-1), (i-1,j), (i,j) of plane k-1
+  T q[3][4];
+  int c = d.ix(i, j, k0);
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    const int b = c - Z;
+    q[m][0] = g[m][b - X - Y];
+    q[m][1] = g[m][b - Y];
+    q[m][2] = g[m][b - X];
+    q[m][3] = g[m][b];
+  }
+  const bool interior = i >= 1 && i <= d.ni && j >= 1 && j <= d.nj;
+  for (int k = k0; k <= k1; k++, c += Z) {
+    T r[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const T a0 = g[m][c - X - Y], a1 = g[m][c - Y], a2 = g[m][c - X], a3 = g[m][c];
+      r[m] = (+q[m][0] + q[m][1] + q[m][2] + q[m][3] + a0 + a1 + a2 + a3) * T(0.125);
+      q[m][0] = a0;
+      q[m][1] = a1;
+      q[m][2] = a2;
+      q[m][3] = a3;
+    }
+    const T q00 = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     P p = phin[c];
     const P p0 = p;
     if (q00 <= mc.eps_mthinc * mc.dcellinv) {
-      if (p <= mc64.eps_trunc) p = P(0.0);
-      if (1.0 - p <= mc64.eps_trunc) p = P(1.0);
+      if (p <= eps_trunc) p = P(0.0);
+      if (1.0 - p <= eps_trunc) p = P(1.0);
     }
-    if (i >= 1 && i <= d.ni && j >= 1 && j <= d.nj && k >= 1 && k <= d.nk) {
+    if (interior && k >= 1 && k <= d.nk) {
       p = fmin(p, P(1.0));
       p = fmax(p, P(0.0));
     }
     // a truncated or clamped cell also loses its derivative
     if (p != p0 || (is_dual<P> && !(p0 > 0.0 && p0 < 1.0))) phin[c] = p;
-  });
+  }
+}
+
+template <class T, class P = PT<T>>
+void mthinc_mass(const Dims d, MthincConst mc64, const T* phix, const T* phiy, const T* phiz,
+                 const P* phi, P* phin) {
+  const MthincC<T> mc(mc64);
+  {
+    const int gx = (d.ni + 2 + 63) / 64, gy = (d.nj + 2 + 3) / 4;
+    const int gz = (d.nk + 2 + kMassChunk - 1) / kMassChunk;
+    mthinc_trunc_kernel<T, P><<<dim3(gx * gy * gz), dim3(64, 4, 1), 0, stream()>>>(
+        d, mc, mc64.eps_trunc, phix, phiy, phiz, phin, gx, gy);
+    FEL_CHECK(hipGetLastError());
+  }
   // the sums and the offset, in P, stay on the device (slot 0 holds 4 P's)
   double* sums = dev_scalars(0);
   interior_reduce<3, P>(
