@@ -53,7 +53,7 @@ METRIC = "Mcell-steps/s at 512x256x128 (FP64 vs FP32+refine); adjoint/forward ti
 HBM_PEAK_GBS = 8000.0
 
 # event-timer ids recorded by the library (csrc: prof_launch_events)
-TIMERS = {3: "sor_pair"}
+TIMERS = {3: "sor_pair", 4: "sor_pair12"}
 
 
 def sor_pair_bytes(cfg) -> float:

@@ -139,6 +139,71 @@ constexpr int kKB = 1;
 // R00I: the face density of each update is formed in the launch from rho and
 // rhon with the expression of the per-solve table (the same bits), for the
 // allocation-free entry point, which keeps no table.
+// One cell's update of component COMP (srcs/solu_sor4.f:131-175) at c, with
+// the swept component's x neighbours xm = F[c-X], xp = F[c+X] supplied by
+// the caller; returns the new value (not stored) and adds du^2 to e.
+template <int COMP, bool ONFLY, class T, bool R00I>
+__device__ __forceinline__ T sor_cell(const Dims& d, const SorArgs<T>& a, int c, int ridx, T xm,
+                                      T xp, double& e) {
+  const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+  T cf[15];
+  if (ONFLY) {
+    if (COMP == 0) coef_u(d, a.sc, a.rmu, c, cf);
+    if (COMP == 1) coef_v(d, a.sc, a.rmu, c, cf);
+    if (COMP == 2) coef_w(d, a.sc, a.rmu, c, cf);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 15; m++) cf[m] = a.cf[15 * COMP + m][c];
+  }
+  const T *un = a.un, *vn = a.vn, *wn = a.wn;
+  const T h = T(0.5);
+  using A = acc_t<T>;  // FP64 for FP32 storage (rounded once, at the store), else T
+  T r00;
+  if (R00I) {
+    const int F = COMP == 0 ? X : (COMP == 1 ? Y : Z);
+    const T q = T(0.25);
+    r00 = (a.rho[c] + a.rho[c + F] + a.rhon[c] + a.rhon[c + F]) * q;
+  } else {
+    r00 = a.r00[COMP][ridx];
+  }
+  const T srcv = a.srcd[COMP] ? a.srcd[COMP][ridx] : a.src[COMP][c];
+  const T* F = COMP == 0 ? un : (COMP == 1 ? vn : wn);
+  A visn;
+  if (COMP == 0) {
+    visn = (+A(cf[4]) * A(un[c - Z]) + A(cf[2]) * A(un[c - Y]) + A(cf[0]) * A(xm) +
+                 A(cf[1]) * A(xp) + A(cf[3]) * A(un[c + Y]) + A(cf[5]) * A(un[c + Z]) +
+                 A(cf[7]) * A(vn[c - Y]) + A(cf[8]) * A(vn[c + X - Y]) + A(cf[9]) * A(vn[c]) +
+                 A(cf[10]) * A(vn[c + X]) + A(cf[11]) * A(wn[c - Z]) + A(cf[12]) * A(wn[c + X - Z]) +
+                 A(cf[13]) * A(wn[c]) + A(cf[14]) * A(wn[c + X])) /
+                A(r00);
+  } else if (COMP == 1) {
+    visn = (+A(cf[4]) * A(vn[c - Z]) + A(cf[2]) * A(vn[c - Y]) + A(cf[0]) * A(xm) +
+                 A(cf[1]) * A(xp) + A(cf[3]) * A(vn[c + Y]) + A(cf[5]) * A(vn[c + Z]) +
+                 A(cf[7]) * A(un[c - X]) + A(cf[9]) * A(un[c]) + A(cf[8]) * A(un[c - X + Y]) +
+                 A(cf[10]) * A(un[c + Y]) + A(cf[11]) * A(wn[c - Z]) + A(cf[12]) * A(wn[c + Y - Z]) +
+                 A(cf[13]) * A(wn[c]) + A(cf[14]) * A(wn[c + Y])) /
+                A(r00);
+  } else {
+    visn = (+A(cf[4]) * A(wn[c - Z]) + A(cf[2]) * A(wn[c - Y]) + A(cf[0]) * A(xm) +
+                 A(cf[1]) * A(xp) + A(cf[3]) * A(wn[c + Y]) + A(cf[5]) * A(wn[c + Z]) +
+                 A(cf[7]) * A(un[c - X]) + A(cf[9]) * A(un[c]) + A(cf[8]) * A(un[c - X + Z]) +
+                 A(cf[10]) * A(un[c + Z]) + A(cf[11]) * A(vn[c - Y]) + A(cf[13]) * A(vn[c]) +
+                 A(cf[12]) * A(vn[c - Y + Z]) + A(cf[14]) * A(vn[c + Z])) /
+                A(r00);
+  }
+  const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
+  T du, nv;
+  if constexpr (is_f32<T>) {
+    nv = T(f);
+    du = nv - F[c];
+  } else {
+    du = T(f) - F[c];
+    nv = F[c] + du;
+  }
+  e += val(du) * val(du);
+  return nv;
+}
+
 template <int COMP, bool ONFLY, bool FUSE, class T, bool R00I = false>
 __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a, int is, int js,
                                               int ks, int jend, int gx, int gy, int ntiles,
@@ -167,88 +232,13 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
   const int i = 1 + ((is + q - 1) & 1) + 2 * ii;
   const int j = 1 + ((js + q - 1) & 1) + 2 * (by * (int)blockDim.y + (int)threadIdx.y);
   if (i <= d.ni && j <= jend) {
-    const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+    const int X = 1;
     const int c = d.ix(i, j, k);
-    T cf[15];
-    if (ONFLY) {
-      if (COMP == 0) coef_u(d, a.sc, a.rmu, c, cf);
-      if (COMP == 1) coef_v(d, a.sc, a.rmu, c, cf);
-      if (COMP == 2) coef_w(d, a.sc, a.rmu, c, cf);
-    } else {
-#pragma unroll
-      for (int m = 0; m < 15; m++) cf[m] = a.cf[15 * COMP + m][c];
-    }
-    const T *un = a.un, *vn = a.vn, *wn = a.wn;
-    const T h = T(0.5);
-    using A = acc_t<T>;  // FP64 for FP32 storage (rounded once, at the store), else T
     const int ridx = r00_index(d, ii, by * (int)blockDim.y + (int)threadIdx.y, k);
-    T r00;
-    if (R00I) {
-      const int F = COMP == 0 ? X : (COMP == 1 ? Y : Z);
-      const T q = T(0.25);
-      r00 = (a.rho[c] + a.rho[c + F] + a.rhon[c] + a.rhon[c + F]) * q;
-    } else {
-      r00 = a.r00[COMP][ridx];
-    }
-    const T srcv = a.srcd[COMP] ? a.srcd[COMP][ridx] : a.src[COMP][c];
-    if (COMP == 0) {
-      const A visn = (+A(cf[4]) * A(un[c - Z]) + A(cf[2]) * A(un[c - Y]) + A(cf[0]) * A(un[c - X]) +
-                           A(cf[1]) * A(un[c + X]) + A(cf[3]) * A(un[c + Y]) + A(cf[5]) * A(un[c + Z]) +
-                           A(cf[7]) * A(vn[c - Y]) + A(cf[8]) * A(vn[c + X - Y]) + A(cf[9]) * A(vn[c]) +
-                           A(cf[10]) * A(vn[c + X]) + A(cf[11]) * A(wn[c - Z]) + A(cf[12]) * A(wn[c + X - Z]) +
-                           A(cf[13]) * A(wn[c]) + A(cf[14]) * A(wn[c + X])) /
-                          A(r00);
-      const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
-      T du, nv;
-      if constexpr (is_f32<T>) {
-        nv = T(f);
-        du = nv - un[c];
-      } else {
-        du = T(f) - un[c];
-        nv = un[c] + du;
-      }
-      put_update(a.un, c, nv);
-      if (FUSE) write_images<0>(d, a, a.un, i, j, k, nv);
-      e += val(du) * val(du);
-    } else if (COMP == 1) {
-      const A visn = (+A(cf[4]) * A(vn[c - Z]) + A(cf[2]) * A(vn[c - Y]) + A(cf[0]) * A(vn[c - X]) +
-                           A(cf[1]) * A(vn[c + X]) + A(cf[3]) * A(vn[c + Y]) + A(cf[5]) * A(vn[c + Z]) +
-                           A(cf[7]) * A(un[c - X]) + A(cf[9]) * A(un[c]) + A(cf[8]) * A(un[c - X + Y]) +
-                           A(cf[10]) * A(un[c + Y]) + A(cf[11]) * A(wn[c - Z]) + A(cf[12]) * A(wn[c + Y - Z]) +
-                           A(cf[13]) * A(wn[c]) + A(cf[14]) * A(wn[c + Y])) /
-                          A(r00);
-      const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
-      T dv, nv;
-      if constexpr (is_f32<T>) {
-        nv = T(f);
-        dv = nv - vn[c];
-      } else {
-        dv = T(f) - vn[c];
-        nv = vn[c] + dv;
-      }
-      put_update(a.vn, c, nv);
-      if (FUSE) write_images<1>(d, a, a.vn, i, j, k, nv);
-      e += val(dv) * val(dv);
-    } else {
-      const A visn = (+A(cf[4]) * A(wn[c - Z]) + A(cf[2]) * A(wn[c - Y]) + A(cf[0]) * A(wn[c - X]) +
-                           A(cf[1]) * A(wn[c + X]) + A(cf[3]) * A(wn[c + Y]) + A(cf[5]) * A(wn[c + Z]) +
-                           A(cf[7]) * A(un[c - X]) + A(cf[9]) * A(un[c]) + A(cf[8]) * A(un[c - X + Z]) +
-                           A(cf[10]) * A(un[c + Z]) + A(cf[11]) * A(vn[c - Y]) + A(cf[13]) * A(vn[c]) +
-                           A(cf[12]) * A(vn[c - Y + Z]) + A(cf[14]) * A(vn[c + Z])) /
-                          A(r00);
-      const A f = (visn * A(h) + A(srcv)) / (A(a.dtinv) + A(cf[6]) / A(r00) * A(h));
-      T dw, nv;
-      if constexpr (is_f32<T>) {
-        nv = T(f);
-        dw = nv - wn[c];
-      } else {
-        dw = T(f) - wn[c];
-        nv = wn[c] + dw;
-      }
-      put_update(a.wn, c, nv);
-      if (FUSE) write_images<2>(d, a, a.wn, i, j, k, nv);
-      e += val(dw) * val(dw);
-    }
+    T* F = COMP == 0 ? a.un : (COMP == 1 ? a.vn : a.wn);
+    const T nv = sor_cell<COMP, ONFLY, T, R00I>(d, a, c, ridx, F[c - X], F[c + X], e);
+    put_update(F, c, nv);
+    if (FUSE) write_images<COMP>(d, a, F, i, j, k, nv);
   }
   }
   // block sum of du^2 in a fixed order
@@ -261,6 +251,80 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
     __syncthreads();
   }
   if (t == 0) part[L] = sh[0];
+}
+
+// Colour pairs 1 and 2 of a component's sweep in one launch.  Pair 1 =
+// (112,221) and pair 2 = (121,212): on plane k both lie on the rows of one j
+// parity (k even: odd rows, pair 1 at odd i and pair 2 at even i; k odd:
+// even rows, pair 1 at even i and pair 2 at odd i).  A pair-2 cell's face
+// neighbours are pair-1 cells only along x (its y and z neighbours belong
+// to pairs 0 and 3, which neither pair updates), and a pair-1 cell's x
+// neighbours are pair-2 cells.  So a block that owns whole rows (one row,
+// thread t: the pair-1 and pair-2 cells of the t-th x parity pair) updates
+// its pair-1 cells from the old values, stores them, passes them through
+// LDS, and then updates its pair-2 cells with the new pair-1 values as x
+// neighbours -- the periodic x ghost of a row end is the other end's cell,
+// which the same block owns.  Every operand is the one the two separate
+// launches (with the in-launch ghost images between them) would read, so
+// the values are identical.  Needs even ni and nk (the images' parity,
+// as for the fused images) and ni / 2 <= 512 threads.
+template <int COMP, bool ONFLY, class T>
+__global__ void __launch_bounds__(512) sor_pair12_kernel(Dims d, SorArgs<T> a1, SorArgs<T> a2,
+                                                         int jend, double* part1,
+                                                         double* part2) {
+  __shared__ T s1[512];
+  __shared__ double sh[2][512];
+  const int hx = (int)blockDim.x, hy = (d.nj + 1) / 2;
+  const int t = (int)threadIdx.x, L = (int)blockIdx.x;
+  const int jj = L % hy, k = 1 + L / hy;
+  const int q1 = (k - 2) & 1, q2 = (k - 1) & 1;  // pair 1 (1,1,2), pair 2 (1,2,1)
+  const int j = 1 + (q1 & 1) + 2 * jj;           // the row of both pairs
+  const int i1 = 1 + (q1 & 1) + 2 * t, i2 = 1 + (q2 & 1) + 2 * t;
+  const bool row = j <= jend;
+  const int X = 1;
+  T* F = COMP == 0 ? a1.un : (COMP == 1 ? a1.vn : a1.wn);
+  const int ridx = r00_index(d, t, jj, k);
+  double e1 = 0.0, e2 = 0.0;
+  T nv1 = T(0.0);
+  if (row) {
+    const int c = d.ix(i1, j, k);
+    nv1 = sor_cell<COMP, ONFLY, T, false>(d, a1, c, ridx, F[c - X], F[c + X], e1);
+    put_update(F, c, nv1);
+    write_images<COMP>(d, a1, F, i1, j, k, nv1);
+  }
+  s1[t] = nv1;
+  __syncthreads();
+  if (row) {
+    const int c = d.ix(i2, j, k);
+    T xm, xp;
+    if (i2 > i1) {  // pair-1 neighbours: own (left) and the next thread's (right)
+      xm = s1[t];
+      xp = s1[t + 1 < hx ? t + 1 : 0];
+    } else {        // the previous thread's (left) and own (right)
+      xm = s1[t > 0 ? t - 1 : hx - 1];
+      xp = s1[t];
+    }
+    const T nv2 = sor_cell<COMP, ONFLY, T, false>(d, a2, c, ridx, xm, xp, e2);
+    put_update(F, c, nv2);
+    write_images<COMP>(d, a2, F, i2, j, k, nv2);
+  }
+  // block sums of du^2 of each pair in a fixed order
+  sh[0][t] = e1;
+  sh[1][t] = e2;
+  __syncthreads();
+  int p2 = 1;
+  while (p2 < hx) p2 <<= 1;
+  for (int st = p2 >> 1; st > 0; st >>= 1) {
+    if (t < st && t + st < hx) {
+      sh[0][t] += sh[0][t + st];
+      sh[1][t] += sh[1][t + st];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part1[L] = sh[0][0];
+    part2[L] = sh[1][0];
+  }
 }
 
 struct PartSlots {
@@ -332,12 +396,46 @@ void launch_pair(bool onfly, bool fuse, dim3 grid, dim3 block, const Dims& d, co
   else go(sor_pair_kernel<COMP, false, false, T>);
 }
 
+// Whether pairs 1 and 2 run as one launch (sor_pair12_kernel): even ni and
+// nk, and a row of ni/2 threads in one block.
+inline bool sor_pair12_ok(const Dims& d) {
+  return d.ni % 2 == 0 && d.nk % 2 == 0 && d.ni / 2 <= 512 && d.ni >= 2;
+}
+
+template <int COMP, class T>
+void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, double* part1,
+                   double* part2, hipEvent_t e0, hipEvent_t e1) {
+  SorArgs<T> a1 = a, a2 = a;  // the face densities and sources of pairs 1 and 2
+  a1.r00[COMP] = a.r00[COMP] + r00_slab(d);
+  a2.r00[COMP] = a.r00[COMP] + 2 * r00_slab(d);
+  if (a.srcd[COMP]) {
+    a1.srcd[COMP] = a.srcd[COMP] + r00_slab(d);
+    a2.srcd[COMP] = a.srcd[COMP] + 2 * r00_slab(d);
+  }
+  const dim3 grid(((d.nj + 1) / 2) * d.nk), block(d.ni / 2, 1, 1);
+  if (onfly)
+    hipExtLaunchKernelGGL(sor_pair12_kernel<COMP, true, T>, grid, block, 0, stream(), e0, e1, 0, d,
+                          a1, a2, jend, part1, part2);
+  else
+    hipExtLaunchKernelGGL(sor_pair12_kernel<COMP, false, T>, grid, block, 0, stream(), e0, e1, 0,
+                          d, a1, a2, jend, part1, part2);
+}
+
 // Launch blocks of one colour-pair launch.
 inline int sor_blocks(const Dims& d) {
   const int gx = ((d.ni + 1) / 2 + 63) / 64;
   const int gy = ((d.nj + 1) / 2 + sor_rows() - 1) / sor_rows();
   const int gz = (d.nk + kKB - 1) / kKB;
   return gx * gy * gz;
+}
+
+// Entries per launch slot in the block-partial array: the larger of the
+// pair launch's and the pair-1/2 launch's block counts (the unused tail of a
+// slot stays zero).
+inline int sor_slot(const Dims& d) {
+  const int n12 = sor_pair12_ok(d) ? ((d.nj + 1) / 2) * d.nk : 0;
+  const int nb = sor_blocks(d);
+  return n12 > nb ? n12 : nb;
 }
 
 // Device work memory of one solve: the block partials of the 12 launches of
@@ -352,7 +450,7 @@ struct SorWork {
 
 template <class P>
 size_t sor_work_bytes(const Dims& d) {
-  const size_t a = sizeof(double) * ((size_t)sor_blocks(d) * 12 + kErrChunks + 1);
+  const size_t a = sizeof(double) * ((size_t)sor_slot(d) * 12 + kErrChunks + 1);
   const size_t wb = sizeof(P) * 4 * (size_t)d.sx * d.sz;
   return ((a + 255) / 256) * 256 + wb;
 }
@@ -360,10 +458,10 @@ size_t sor_work_bytes(const Dims& d) {
 // Lays the work memory out in one buffer of sor_work_bytes<P>(d) bytes.
 template <class P>
 SorWork<P> sor_work_in(const Dims& d, void* buf) {
-  const size_t a = sizeof(double) * ((size_t)sor_blocks(d) * 12 + kErrChunks + 1);
+  const size_t a = sizeof(double) * ((size_t)sor_slot(d) * 12 + kErrChunks + 1);
   SorWork<P> w;
   w.part = static_cast<double*>(buf);
-  w.derr = w.part + (size_t)sor_blocks(d) * 12 + kErrChunks;
+  w.derr = w.part + (size_t)sor_slot(d) * 12 + kErrChunks;
   w.wcoef = reinterpret_cast<P*>(static_cast<char*>(buf) + ((a + 255) / 256) * 256);
   return w;
 }
@@ -376,16 +474,24 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
   const int rows = sor_rows();                // rows of one parity per block
   const int gy = ((d.nj + 1) / 2 + rows - 1) / rows;
   const int nb = sor_blocks(d);
+  const int slotn = sor_slot(d);
   double* part = work.part;
-  double* chunk = part + (size_t)nb * 12;
+  double* chunk = part + (size_t)slotn * 12;
   PartSlots ps;
   for (int s = 0; s < 12; s++) {
-    ps.off[s] = s * nb;
+    ps.off[s] = s * slotn;
     ps.n[s] = nb;
+  }
+  // the slots' unused tails add zero to the stop test's sum
+  if (slotn > nb || sor_pair12_ok(d)) {
+    const int n = 12 * slotn;
+    launch_flat(n, [=] __device__(int m) { part[m] = 0.0; });
   }
   double* derr = work.derr;
   // in-launch ghost refresh needs even ni, nk (colour parity of periodic images)
   const bool fuse_ok = d.ni % 2 == 0 && d.nk % 2 == 0;
+  // pairs 1 and 2 in one launch (with the per-solve face-density table)
+  const bool p12 = fuse_ok && sor_pair12_ok(d) && a.r00[0] != nullptr;
   a.l1 = l1;
   a.l2 = l2;
   a.phi = phi;
@@ -422,6 +528,16 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
       for (int p = 0; p < 4; p++, slot++) {
         dim3 grid(nb), block(64, rows, 1);
         double* pp = part + ps.off[slot];
+        if (p12 && p == 2) continue;  // ran with pair 1
+        if (p12 && p == 1) {
+          hipEvent_t e0, e1;
+          prof_launch_events(4, true, true, 1, &e0, &e1);
+          if (comp == 0) launch_pair12<0, T>(onfly, d, a, jend, pp, part + ps.off[slot + 1], e0, e1);
+          if (comp == 1) launch_pair12<1, T>(onfly, d, a, jend, pp, part + ps.off[slot + 1], e0, e1);
+          if (comp == 2) launch_pair12<2, T>(onfly, d, a, jend, pp, part + ps.off[slot + 1], e0, e1);
+          FEL_CHECK(hipGetLastError());
+          continue;
+        }
         // Pairs after the first write the swept component's ghost images
         // themselves (write_images); the refresh after the first pair also
         // rewrites the other components' ghosts, so it stays a launch.
@@ -432,7 +548,10 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
         // pair before a refresh launch is not timed.  Without them every
         // pair is a span of its own.
         hipEvent_t e0, e1;
-        if (fuse_ok) {
+        if (p12) {
+          if (p == 0) e0 = e1 = nullptr;
+          else prof_launch_events(3, true, true, 1, &e0, &e1);  // pair 3
+        } else if (fuse_ok) {
           if (p == 0) e0 = e1 = nullptr;
           else prof_launch_events(3, p == 1, p == 3, 3, &e0, &e1);
         } else {
@@ -453,7 +572,7 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
         }
       }
     }
-    sor_err_stage1<<<kErrChunks, 256, 0, stream()>>>(part, 12 * nb, chunk);
+    sor_err_stage1<<<kErrChunks, 256, 0, stream()>>>(part, 12 * slotn, chunk);
     sor_err_stage2<<<1, 256, 0, stream()>>>(chunk, derr);
     FEL_CHECK(hipGetLastError());
     const double sum = read_scalars(derr, 1)[0];

@@ -122,7 +122,11 @@ def test_noalloc_equals_drop_in_on_two_calls(dt):
         want, lp0, r0 = with_drop_in(g, x, dt, 0.25)
         rc, got, lp1, r1 = noalloc(g, x, dt, 0.25)
         assert rc == 0
-        assert (lp1, r1) == (lp0, r0)
+        # the same sweeps; err/err0 is the same sum of squares grouped by the
+        # two paths' launch blocks (the drop-in runs pairs 1 and 2 as one
+        # launch), so it agrees to rounding
+        assert lp1 == lp0
+        assert abs(r1 - r0) <= 1e-13 * abs(r0), (r1, r0)
         for a, b in zip(got, want):
             assert np.array_equal(a, b), np.max(np.abs(a - b))
 
