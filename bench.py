@@ -68,6 +68,16 @@ def sor_pair_bytes(cfg) -> float:
     return 8.0 * float(cfg.cells) * 4.75
 
 
+def sor_pair12_bytes(cfg) -> float:
+    """Algorithmic HBM bytes of one launch of colour pairs 1 and 2 together
+    (the pair-2 cells' x neighbours are the pair-1 cells of the same row, so
+    one launch of whole rows does both): the three components and the
+    viscosity are read once (4 N), the sources and face densities of both
+    pairs (4 x N/4) and the swept component written at both pairs' cells
+    (2 x N/4).  Hence 5.5 N elements of 8 bytes."""
+    return 8.0 * float(cfg.cells) * 5.5
+
+
 def measured_traffic(cfg):
     """HBM bytes per colour-pair launch from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, calibrated),
@@ -328,6 +338,18 @@ def main():
     elapsed = main_run["elapsed"]
     value = cfg.cells * args.steps / elapsed / 1e6
     traffic, traffic_src = measured_traffic(cfg) if head_prec == 64 else (None, None)
+    ms12, n12 = timers.get("sor_pair12", (0.0, 0))
+    roof12 = None
+    if n12:
+        avg12 = ms12 / 1e3 / n12
+        b12 = sor_pair12_bytes(cfg) * (0.5 if head_prec == 32 else 1.0)
+        a12 = b12 / avg12 / 1e9
+        roof12 = {"bound": "hbm", "achieved": round(a12, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(a12 / HBM_PEAK_GBS, 4),
+                  "kernel": "sor_pair12_kernel (colour pairs 1 and 2 in one launch)",
+                  "bytes_per_launch": b12, "avg_launch_us": round(avg12 * 1e6, 2),
+                  "launches": n12,
+                  "pair_equivalent_frac": round(2 * b / avg12 / 1e9 / HBM_PEAK_GBS, 4)}
 
     out = {
         "metric": METRIC,
@@ -353,9 +375,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "sor_pair_kernel (colour-pair Gauss-Seidel update)",
+                     "kernel": "sor_pair_kernel (colour-pair Gauss-Seidel update: pairs 0 and 3 "
+                               "of each sweep; timed on pair 3, the same kernel with in-launch "
+                               "ghost images)",
                      "bytes_per_launch": b, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": n_sor, "traffic_source": traffic_src},
+        "roofline_pair12": roof12,
         "fp32_refine": fp32,
         "fp32_fp64_poisson": fp32_64,
         "tangent": tangent,

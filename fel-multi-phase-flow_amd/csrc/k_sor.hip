@@ -17,6 +17,7 @@
 #include <hip/hip_ext.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 #include "felgpu.hpp"
@@ -271,16 +272,18 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
 template <int COMP, bool ONFLY, class T>
 __global__ void __launch_bounds__(512) sor_pair12_kernel(Dims d, SorArgs<T> a1, SorArgs<T> a2,
                                                          int jend, double* part1,
-                                                         double* part2) {
-  __shared__ T s1[512];
+                                                         double* part2, int ntiles) {
+  __shared__ T s1_all[512];
   __shared__ double sh[2][512];
-  const int hx = (int)blockDim.x, hy = (d.nj + 1) / 2;
-  const int t = (int)threadIdx.x, L = (int)blockIdx.x;
-  const int jj = L % hy, k = 1 + L / hy;
+  const int hx = (int)blockDim.x, R = (int)blockDim.y, hy = (d.nj + 1) / 2;
+  const int gyr = (hy + R - 1) / R;  // blocks per plane
+  const int t = (int)threadIdx.x, L = xcd_tile((int)blockIdx.x, ntiles);
+  const int jj = (L % gyr) * R + (int)threadIdx.y, k = 1 + L / gyr;
+  T* s1 = s1_all + threadIdx.y * hx;
   const int q1 = (k - 2) & 1, q2 = (k - 1) & 1;  // pair 1 (1,1,2), pair 2 (1,2,1)
   const int j = 1 + (q1 & 1) + 2 * jj;           // the row of both pairs
   const int i1 = 1 + (q1 & 1) + 2 * t, i2 = 1 + (q2 & 1) + 2 * t;
-  const bool row = j <= jend;
+  const bool row = jj < hy && j <= jend;
   const int X = 1;
   T* F = COMP == 0 ? a1.un : (COMP == 1 ? a1.vn : a1.wn);
   const int ridx = r00_index(d, t, jj, k);
@@ -309,19 +312,20 @@ __global__ void __launch_bounds__(512) sor_pair12_kernel(Dims d, SorArgs<T> a1, 
     write_images<COMP>(d, a2, F, i2, j, k, nv2);
   }
   // block sums of du^2 of each pair in a fixed order
-  sh[0][t] = e1;
-  sh[1][t] = e2;
+  const int tb = (int)threadIdx.y * hx + t, nt = R * hx;
+  sh[0][tb] = e1;
+  sh[1][tb] = e2;
   __syncthreads();
   int p2 = 1;
-  while (p2 < hx) p2 <<= 1;
+  while (p2 < nt) p2 <<= 1;
   for (int st = p2 >> 1; st > 0; st >>= 1) {
-    if (t < st && t + st < hx) {
-      sh[0][t] += sh[0][t + st];
-      sh[1][t] += sh[1][t + st];
+    if (tb < st && tb + st < nt) {
+      sh[0][tb] += sh[0][tb + st];
+      sh[1][tb] += sh[1][tb + st];
     }
     __syncthreads();
   }
-  if (t == 0) {
+  if (tb == 0) {
     part1[L] = sh[0][0];
     part2[L] = sh[1][0];
   }
@@ -402,6 +406,16 @@ inline bool sor_pair12_ok(const Dims& d) {
   return d.ni % 2 == 0 && d.nk % 2 == 0 && d.ni / 2 <= 512 && d.ni >= 2;
 }
 
+// Rows per block of the pair-1/2 launch (1; 2 rows of 256 threads measured
+// 4% slower at 512x256x128 with the XCD-contiguous block order below).
+// Rows per block of the pair-1/2 launch: 1.  Measured at 512x256x128
+// (FP64): 2 rows of 256 threads 4% slower; one 64-lane wave per row with 4
+// cells per lane (no block barrier needed) 17-28% slower.
+inline int pair12_rows(const Dims& d) {
+  (void)d;
+  return 1;
+}
+
 template <int COMP, class T>
 void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, double* part1,
                    double* part2, hipEvent_t e0, hipEvent_t e1) {
@@ -412,13 +426,19 @@ void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, dou
     a1.srcd[COMP] = a.srcd[COMP] + r00_slab(d);
     a2.srcd[COMP] = a.srcd[COMP] + 2 * r00_slab(d);
   }
-  const dim3 grid(((d.nj + 1) / 2) * d.nk), block(d.ni / 2, 1, 1);
+  const int R = pair12_rows(d), hy = (d.nj + 1) / 2;
+  const int nb = ((hy + R - 1) / R) * d.nk;
+  // XCD-contiguous block order: each XCD sweeps one range of planes, whose
+  // neighbouring rows then come from its own L2 (measured at 512x256x128,
+  // FP64: 218 against 228 us per launch in dispatch order)
+  const int nt = nb;
+  const dim3 grid(nb), block(d.ni / 2, R, 1);
   if (onfly)
     hipExtLaunchKernelGGL(sor_pair12_kernel<COMP, true, T>, grid, block, 0, stream(), e0, e1, 0, d,
-                          a1, a2, jend, part1, part2);
+                          a1, a2, jend, part1, part2, nt);
   else
     hipExtLaunchKernelGGL(sor_pair12_kernel<COMP, false, T>, grid, block, 0, stream(), e0, e1, 0,
-                          d, a1, a2, jend, part1, part2);
+                          d, a1, a2, jend, part1, part2, nt);
 }
 
 // Launch blocks of one colour-pair launch.
@@ -433,7 +453,8 @@ inline int sor_blocks(const Dims& d) {
 // pair launch's and the pair-1/2 launch's block counts (the unused tail of a
 // slot stays zero).
 inline int sor_slot(const Dims& d) {
-  const int n12 = sor_pair12_ok(d) ? ((d.nj + 1) / 2) * d.nk : 0;
+  const int R = pair12_rows(d), hy = (d.nj + 1) / 2;
+  const int n12 = sor_pair12_ok(d) ? ((hy + R - 1) / R) * d.nk : 0;
   const int nb = sor_blocks(d);
   return n12 > nb ? n12 : nb;
 }
