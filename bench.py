@@ -78,16 +78,18 @@ def sor_pair12_bytes(cfg) -> float:
     return 8.0 * float(cfg.cells) * 5.5
 
 
-def measured_traffic(cfg):
-    """HBM bytes per colour-pair launch from the committed rocprofv3 PMC
-    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, calibrated),
-    or None when no summary exists for the configuration run."""
+def measured_traffic(cfg, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of the colour-pair kernel (key
+    traffic_bytes_per_launch) or of the pair-1/2 kernel
+    (traffic12_bytes_per_launch) from the committed rocprofv3 PMC summary of
+    this workload (FETCH_SIZE and WRITE_SIZE runs, calibrated), or None when
+    no summary exists for the configuration run."""
     path = os.path.join(REPO, "profiles",
                         f"hbm_traffic_pmc_{cfg.name}_{cfg.ni}x{cfg.nk}x{cfg.nj}.md")
     try:
         for line in open(path):
-            if "traffic_bytes_per_launch:" in line:
-                return float(line.split("traffic_bytes_per_launch:")[1].split("`")[0]), \
+            if key + ":" in line:
+                return float(line.split(key + ":")[1].split("`")[0]), \
                     os.path.relpath(path, REPO)
     except OSError:
         pass
@@ -344,8 +346,9 @@ def main():
         avg12 = ms12 / 1e3 / n12
         b12 = sor_pair12_bytes(cfg) * (0.5 if head_prec == 32 else 1.0)
         a12 = b12 / avg12 / 1e9
+        t12 = measured_traffic(cfg, "traffic12_bytes_per_launch")[0] if head_prec == 64 else None
         roof12 = {"bound": "hbm", "achieved": round(a12, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(a12 / HBM_PEAK_GBS, 4),
+                  "frac": round(a12 / HBM_PEAK_GBS, 4), "traffic": t12,
                   "kernel": "sor_pair12_kernel (colour pairs 1 and 2 in one launch)",
                   "bytes_per_launch": b12, "avg_launch_us": round(avg12 * 1e6, 2),
                   "launches": n12,
