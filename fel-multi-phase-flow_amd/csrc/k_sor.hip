@@ -550,9 +550,13 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
         dim3 grid(nb), block(64, rows, 1);
         double* pp = part + ps.off[slot];
         if (p12 && p == 2) continue;  // ran with pair 1
+        // The benchmark's timers stamp the launches of the first sweep only:
+        // each stamped launch leaves a gap of ~5 us after it, and every sweep
+        // runs the same launches.
+        const bool stamp = lp == 0;
         if (p12 && p == 1) {
-          hipEvent_t e0, e1;
-          prof_launch_events(4, true, true, 1, &e0, &e1);
+          hipEvent_t e0 = nullptr, e1 = nullptr;
+          if (stamp) prof_launch_events(4, true, true, 1, &e0, &e1);
           if (comp == 0) launch_pair12<0, T>(onfly, d, a, jend, pp, part + ps.off[slot + 1], e0, e1);
           if (comp == 1) launch_pair12<1, T>(onfly, d, a, jend, pp, part + ps.off[slot + 1], e0, e1);
           if (comp == 2) launch_pair12<2, T>(onfly, d, a, jend, pp, part + ps.off[slot + 1], e0, e1);
@@ -564,17 +568,16 @@ SorResult sor4(const Dims d, SorArgs<T> a, bool onfly, bool lo, bool hi, double 
         // rewrites the other components' ghosts, so it stays a launch.
         const bool fuse = fuse_ok && p > 0;
         const int nt = xcd_arg(1, nb);
-        // The benchmark's timer: with fused images, pairs 1-3 of a sweep run
-        // back to back and are timed as one span of three launches; the
-        // pair before a refresh launch is not timed.  Without them every
-        // pair is a span of its own.
-        hipEvent_t e0, e1;
-        if (p12) {
-          if (p == 0) e0 = e1 = nullptr;
-          else prof_launch_events(3, true, true, 1, &e0, &e1);  // pair 3
+        // The colour-pair timer: with pairs 1 and 2 joined, pair 3 (pair 0
+        // is followed by a refresh launch and not timed); with fused images
+        // only, pairs 1-3 back to back as one span of three launches;
+        // without them every pair is a span of its own.
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (!stamp) {
+        } else if (p12) {
+          if (p == 3) prof_launch_events(3, true, true, 1, &e0, &e1);
         } else if (fuse_ok) {
-          if (p == 0) e0 = e1 = nullptr;
-          else prof_launch_events(3, p == 1, p == 3, 3, &e0, &e1);
+          if (p > 0) prof_launch_events(3, p == 1, p == 3, 3, &e0, &e1);
         } else {
           prof_launch_events(3, true, true, 1, &e0, &e1);
         }
