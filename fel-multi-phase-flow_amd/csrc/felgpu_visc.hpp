@@ -33,6 +33,10 @@ struct SrcArgs {
   // Gauss-Seidel colour pairs (component q, pair p at
   // dense + (4 q + p) r00_slab + r00_index)
   T* dense;
+  // when not null: the face densities of the three components, pair by pair
+  // (component q, pair p at r00d[q] + p r00_slab + r00_index), the table
+  // solu_sor4_ forms for its solve
+  T* r00d[3];
 };
 
 // Layout of values stored pair by pair for the Gauss-Seidel colour pairs
@@ -68,6 +72,12 @@ struct DenseSrc {
   uint64_t key[3] = {0, 0, 0};
   int ni = 0, nj = 0, nk = 0, esize = 0;
   void* buf = nullptr;
+  // the face-density table made from these densities (valid while they
+  // carry these content numbers)
+  bool r00 = false;
+  const void* rho_host = nullptr;
+  const void* rhon_host = nullptr;
+  uint64_t rho_key = 0, rhon_key = 0;
 };
 DenseSrc& dense_src();
 

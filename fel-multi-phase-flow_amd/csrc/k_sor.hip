@@ -670,6 +670,7 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
   a.dtinv = T(1.0 / dt);
   // srcs/solu_sor4.f:150-152 (and the v, w forms): the face density of each
   // update, the same expression as the sweeps evaluate, once for the solve
+  // -- or the table cal_srcu_ formed from these very densities
   {
     T* r00[3];
     const size_t slab = r00_slab(d);
@@ -677,8 +678,14 @@ SorResult solu_sor4_t(const Dims d, double dt, const double* rho, const double* 
       r00[m] = static_cast<T*>(scratch(10 + m, sizeof(T) * 4 * slab));
       a.r00[m] = r00[m];
     }
+    const DenseSrc& ds = dense_src();
+    const bool have = ds.valid && ds.r00 && ds.ni == d.ni && ds.nj == d.nj && ds.nk == d.nk &&
+                      ds.esize == (int)sizeof(T) && ds.rho_host == (const void*)rho &&
+                      ds.rhon_host == (const void*)rhon &&
+                      ds.rho_key == content_key(rho, d.n3) &&
+                      ds.rhon_key == content_key(rhon, d.n3);
     const T *rho_d = a.rho, *rhon_d = a.rhon;
-    launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
+    if (!have) launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
       const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
       const size_t o = pair_of(i, j, k) * slab + r00_index(d, (i - 1) >> 1, (j - 1) >> 1, k);
       const T q = T(0.25);

@@ -259,6 +259,13 @@ void srcu(const Dims d, int jen, double dtinv_, double grvb_, double grvp_, SrcA
     auto ld = [&](const T* f, int o_) { return A(f[o_]); };
     const size_t slab4 = 4 * r00_slab(d);
     const size_t o = pair_of(i, j, k) * r00_slab(d) + r00_index(d, (i - 1) >> 1, (j - 1) >> 1, k);
+    if (a.r00d[0]) {
+      // solu_sor4_'s face densities, in its arithmetic (srcs/solu_sor4.f:150-152)
+      const T q4 = T(0.25);
+      a.r00d[0][o] = (a.rho[c] + a.rho[c + X] + a.rhon[c] + a.rhon[c + X]) * q4;
+      a.r00d[1][o] = (a.rho[c] + a.rho[c + Y] + a.rhon[c] + a.rhon[c + Y]) * q4;
+      a.r00d[2][o] = (a.rho[c] + a.rho[c + Z] + a.rhon[c] + a.rhon[c + Z]) * q4;
+    }
     {
       const A r = (ld(a.rho, c) + ld(a.rho, c + X) + ld(a.rhon, c) + ld(a.rhon, c + X)) * q;
       const A sv = +ld(a.u, c) * dtinv - ld(a.adv_u, c) * t15 + ld(a.adv_uo, c) * h -
@@ -523,6 +530,10 @@ void cal_srcu_t(const Dims d, int jen, double dt, double grvb, double grvp, cons
   bool reg = true;
   for (int m = 0; m < 3; m++) reg = reg && content_key(out[m], n) != 0;
   a.dense = reg ? static_cast<T*>(scratch(17, sizeof(T) * 12 * r00_slab(d))) : nullptr;
+  // and, from registered densities, solu_sor4_'s face-density table
+  const bool rreg = reg && content_key(in[3], n) != 0 && content_key(in[4], n) != 0;
+  for (int m = 0; m < 3; m++)
+    a.r00d[m] = rreg ? static_cast<T*>(scratch(10 + m, sizeof(T) * 4 * r00_slab(d))) : nullptr;
   srcu<T>(d, jen, 1.0 / dt, grvb, grvp, a);
   c.finish();
   if (reg) {
@@ -536,6 +547,13 @@ void cal_srcu_t(const Dims d, int jen, double dt, double grvb, double grvp, cons
     ds.nk = d.nk;
     ds.esize = (int)sizeof(T);
     ds.buf = a.dense;
+    ds.r00 = rreg;
+    if (rreg) {
+      ds.rho_host = in[3];
+      ds.rhon_host = in[4];
+      ds.rho_key = content_key(in[3], n);
+      ds.rhon_key = content_key(in[4], n);
+    }
   }
 }
 
