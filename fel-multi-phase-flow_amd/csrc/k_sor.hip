@@ -408,14 +408,6 @@ inline bool sor_pair12_ok(const Dims& d) {
 
 // Rows per block of the pair-1/2 launch (1; 2 rows of 256 threads measured
 // 4% slower at 512x256x128 with the XCD-contiguous block order below).
-// Rows per block of the pair-1/2 launch: 1.  Measured at 512x256x128
-// (FP64): 2 rows of 256 threads 4% slower; one 64-lane wave per row with 4
-// cells per lane (no block barrier needed) 17-28% slower.
-inline int pair12_rows(const Dims& d) {
-  (void)d;
-  return 1;
-}
-
 template <int COMP, class T>
 void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, double* part1,
                    double* part2, hipEvent_t e0, hipEvent_t e1) {
@@ -426,8 +418,11 @@ void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, dou
     a1.srcd[COMP] = a.srcd[COMP] + r00_slab(d);
     a2.srcd[COMP] = a.srcd[COMP] + 2 * r00_slab(d);
   }
-  const int R = pair12_rows(d), hy = (d.nj + 1) / 2;
-  const int nb = ((hy + R - 1) / R) * d.nk;
+  // one row per block (two rows of 256 threads measured 4% slower; one
+  // 64-lane wave per row with four cells per lane, which needs no block
+  // barrier, 17-28% slower)
+  const int R = 1, hy = (d.nj + 1) / 2;
+  const int nb = hy * d.nk;
   // XCD-contiguous block order: each XCD sweeps one range of planes, whose
   // neighbouring rows then come from its own L2 (measured at 512x256x128,
   // FP64: 218 against 228 us per launch in dispatch order)
@@ -453,8 +448,7 @@ inline int sor_blocks(const Dims& d) {
 // pair launch's and the pair-1/2 launch's block counts (the unused tail of a
 // slot stays zero).
 inline int sor_slot(const Dims& d) {
-  const int R = pair12_rows(d), hy = (d.nj + 1) / 2;
-  const int n12 = sor_pair12_ok(d) ? ((hy + R - 1) / R) * d.nk : 0;
+  const int n12 = sor_pair12_ok(d) ? ((d.nj + 1) / 2) * d.nk : 0;
   const int nb = sor_blocks(d);
   return n12 > nb ? n12 : nb;
 }
