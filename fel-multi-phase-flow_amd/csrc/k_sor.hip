@@ -272,13 +272,22 @@ __device__ __forceinline__ void sor_pair_body(const Dims& d, const SorArgs<T>& a
 template <int COMP, bool ONFLY, class T>
 __global__ void __launch_bounds__(512) sor_pair12_kernel(Dims d, SorArgs<T> a1, SorArgs<T> a2,
                                                          int jend, double* part1,
-                                                         double* part2, int ntiles) {
+                                                         double* part2, int ntiles, int band) {
   __shared__ T s1_all[512];
   __shared__ double sh[2][512];
   const int hx = (int)blockDim.x, R = (int)blockDim.y, hy = (d.nj + 1) / 2;
   const int gyr = (hy + R - 1) / R;  // blocks per plane
   const int t = (int)threadIdx.x, L = xcd_tile((int)blockIdx.x, ntiles);
-  const int jj = (L % gyr) * R + (int)threadIdx.y, k = 1 + L / gyr;
+  int jj, k;
+  if (band > 0) {  // bands of `band` rows, planes in order within a band
+    const int per = band * d.nk;
+    const int bnd = L / per, rem = L - bnd * per;
+    k = 1 + rem / band;
+    jj = bnd * band + rem % band;
+  } else {
+    jj = (L % gyr) * R + (int)threadIdx.y;
+    k = 1 + L / gyr;
+  }
   T* s1 = s1_all + threadIdx.y * hx;
   const int q1 = (k - 2) & 1, q2 = (k - 1) & 1;  // pair 1 (1,1,2), pair 2 (1,2,1)
   const int j = 1 + (q1 & 1) + 2 * jj;           // the row of both pairs
@@ -423,17 +432,22 @@ void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, dou
   // barrier, 17-28% slower)
   const int R = 1, hy = (d.nj + 1) / 2;
   const int nb = hy * d.nk;
-  // XCD-contiguous block order: each XCD sweeps one range of planes, whose
-  // neighbouring rows then come from its own L2 (measured at 512x256x128,
-  // FP64: 218 against 228 us per launch in dispatch order)
+  // Block order: bands of 8 rows, each swept through all planes before the
+  // next, the bands split over the XCDs in contiguous ranges, so a row's
+  // neighbour rows and planes are read by blocks that run together on one
+  // XCD (measured at 512x256x128, FP64: 201 us per launch, against 220 with
+  // whole planes per XCD range and 228 in dispatch order; bands of 4, 16 and
+  // 32 rows 207, 206 and 209 us; the same banding made the single-pair
+  // launch slower)
+  const int band = hy % 8 == 0 ? 8 : 0;
   const int nt = nb;
   const dim3 grid(nb), block(d.ni / 2, R, 1);
   if (onfly)
     hipExtLaunchKernelGGL(sor_pair12_kernel<COMP, true, T>, grid, block, 0, stream(), e0, e1, 0, d,
-                          a1, a2, jend, part1, part2, nt);
+                          a1, a2, jend, part1, part2, nt, band);
   else
     hipExtLaunchKernelGGL(sor_pair12_kernel<COMP, false, T>, grid, block, 0, stream(), e0, e1, 0,
-                          d, a1, a2, jend, part1, part2, nt);
+                          d, a1, a2, jend, part1, part2, nt, band);
 }
 
 // Launch blocks of one colour-pair launch.
