@@ -625,6 +625,10 @@ double* dev_scalars(int slot);
 // Copies n doubles from device to the pinned buffer and waits for them.
 const double* read_scalars(const double* dev, int n);
 
+// Aborts with a message naming `who` unless the library runs in the FP64
+// mode (the adjoint routines have no FP32 or tangent form).
+void adjoint_fp64_only(const char* who);
+
 // Finds this device's copy of the allocation-free solve's static work array
 // (k_sor.hip); once per device, with the device current.
 void sor_noalloc_setup();

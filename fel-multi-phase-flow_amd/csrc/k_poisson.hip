@@ -222,8 +222,10 @@ static void check_pow2(int n, const char* who) {
 // at centred mode indices; the scratch holds (nk, nj, ni) complex values.
 // T: element type of the modes; the divergence is in the phase type PT<T>.
 // `rhs(i,j,k)` gives the right-hand side at interior cell (i,j,k).
-template <class T, class Rhs>
-void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
+// ACC: the transform divided by `nrm` is added to the mode arrays instead of
+// stored (the adjoint of solp_fft_tdma3_, unnormalised, nrm = 1).
+template <class T, bool ACC, class Rhs>
+void fft_forward_core(const Dims d, Rhs rhs, T* pr, T* pi, T nrm) {
   using C = typename Cplx<T>::type;
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   check_pow2(ni, "solp_fft_tdma1_");
@@ -239,7 +241,6 @@ void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
   // k transforms; tile = (j, kx block), lines adjacent in kx
   const int Tw = z_tile<T>(nk, ni);
   const int tiles_per_j = ni / Tw;
-  const T nrm = T((double)(ni * nk));
   fft_z<T>(nk, nj * tiles_per_j, Tw, T(-1.0),
            [=] __device__(int tile, int l, int e) {
              const int j = tile % nj, kx = (tile / nj) * Tw + l;
@@ -249,9 +250,19 @@ void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
              const int j = tile % nj, kx = (tile / nj) * Tw + l;
              const int ic = (kx + ni / 2) % ni, kc = (e + nk / 2) % nk;
              const size_t o = (size_t)ic + (size_t)ni * ((size_t)j + (size_t)nj * kc);
-             pr[o] = v.x / nrm;
-             pi[o] = v.y / nrm;
+             if constexpr (ACC) {
+               pr[o] = pr[o] + v.x / nrm;
+               pi[o] = pi[o] + v.y / nrm;
+             } else {
+               pr[o] = v.x / nrm;
+               pi[o] = v.y / nrm;
+             }
            });
+}
+
+template <class T, class Rhs>
+void fft_forward_of(const Dims d, Rhs rhs, T* pr, T* pi) {
+  fft_forward_core<T, false>(d, rhs, pr, pi, T((double)(d.ni * d.nk)));
 }
 
 template <class T>
@@ -296,7 +307,12 @@ void fft_inverse(const Dims d, const T* pr, const T* pi, P* dp, bool add) {
 // In FP32 the last pivot of the singular (0,0) mode, zero in exact
 // arithmetic, is set to pin that mode's x(nj) to 0 exactly, which the FP64
 // expression -d/(d^2+1e-99) does when the pivot rounds to zero.
-template <class T>
+//
+// TRANS: the solve of the transposed systems (sub-diagonal an(j-1),
+// super-diagonal as(j+1)), the adjoint of the forward solve; its singular
+// mode is pinned in FP64 too (x(nj) = 0 and the leading nj-1 rows: the
+// transpose of the pinned solve).
+template <class T, bool TRANS = false>
 void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, const double* coef,
                 double dx2r, double dz2r, T* pr, T* pi) {
   T* at = static_cast<T*>(scratch(5, sizeof(T) * (size_t)ni * nj * nk));
@@ -304,17 +320,17 @@ void tdma_modes(int ni, int nj, int nk, const double* cx, const double* cz, cons
   launch_flat(ni * nk, [=] __device__(int m) {
     const int i = m % ni, k = m / ni;
     const double c1x = cx[i], c1z = cz[k];
-    const bool singular = is_f32<T> && c1x == 2.0 && c1z == 2.0;
+    const bool singular = (is_f32<T> || TRANS) && c1x == 2.0 && c1z == 2.0;
     T a_prev = T(0.0), br = T(0.0), bi = T(0.0);
     const size_t base = (size_t)i + (size_t)ni * nj * k;
     for (int j = 1; j <= nj; j++) {
       const size_t o = base + (size_t)ni * (j - 1);
       const T ap = T(ap0[j + 2] + dx2r * (2.0 - c1x) + dz2r * (2.0 - c1z));
-      const T asj = T(as[j + 2]);
+      const T asj = T(TRANS ? (j > 1 ? an[j + 1] : 0.0) : as[j + 2]);
       const T d00 = -ap + asj * a_prev;
       T apinv = -d00 / (d00 * d00 + guard<T>(1.0e-99));
       if (singular && j == nj) apinv = T(0.0);
-      a_prev = apinv * T(an[j + 2]);
+      a_prev = apinv * T(TRANS ? (j < nj ? as[j + 3] : 0.0) : an[j + 2]);
       br = apinv * (-pr[o] + asj * br);
       bi = apinv * (-pi[o] + asj * bi);
       at[o] = a_prev;
@@ -747,6 +763,133 @@ void corunp_t(const Dims d, int jen, double rho, double dxinv, double dyinv, dou
   c.finish();
 }
 
+// ---------------------------------------------------------------------------
+// Adjoints (reverse mode) of the projection.  The reference has no adjoint:
+// <name>_b_ applies the transpose of the linear map <name>_ applies to its
+// active arrays.  Every active array argument of the forward list is
+// replaced by its adjoint, of the same shape; adjoints of the forward's
+// inputs are incremented, adjoints of what it overwrites are consumed (set
+// to zero), in/out adjoints are updated in place.  Each kernel gathers what
+// one cell receives, so no two threads write one element.  FP64 mode only.
+// ---------------------------------------------------------------------------
+void adjoint_fp64_only(const char* who) {
+  if (flow_fp32() || tangent_mode()) {
+    std::fprintf(stderr, "felgpu: %s: the adjoint routines run in the FP64 mode only\n", who);
+    std::abort();
+  }
+}
+
+// srcs/caldiv.f:27-38 transposed: div(c) on [-1,n+3]^3 reads u(c) with +fx
+// and u(c-x) with -fx (fx = dxinv/dt), so u_b(c) += (div_b(c) - div_b(c+x))
+// fx over the cells of the box, likewise v, w; then div_b = 0 on the box.
+static void caldiv_adj(const Dims d, double fx, double fy, double fz, double* ub, double* vb,
+                       double* wb, double* divb) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    const bool b0 = i >= -1 && j >= -1 && k >= -1;
+    const double g0 = b0 ? divb[c] : 0.0;
+    const double gx = (j >= -1 && k >= -1 && i + 1 <= ni + 3) ? divb[c + X] : 0.0;
+    const double gy = (i >= -1 && k >= -1 && j + 1 <= nj + 3) ? divb[c + Y] : 0.0;
+    const double gz = (i >= -1 && j >= -1 && k + 1 <= nk + 3) ? divb[c + Z] : 0.0;
+    ub[c] = ub[c] + (g0 - gx) * fx;
+    vb[c] = vb[c] + (g0 - gy) * fy;
+    wb[c] = wb[c] + (g0 - gz) * fz;
+  });
+  launch_box(Box{-1, ni + 3, -1, nj + 3, -1, nk + 3},
+             [=] __device__(int i, int j, int k) { divb[d.ix(i, j, k)] = 0.0; });
+}
+
+// srcs/bnd_avzero.f:10-23 transposed: q_b(interior) -= (sum of q_b over the
+// padded array) / (ni nj nk ndiv), the sum formed on the device.
+static void avzero_adj(const Dims d, double* qb, int ndiv) {
+  double* s = dev_scalars(20);
+  const double nn = (double)(d.ni * d.nj * d.nk * ndiv);
+  Dims p = d;  // the padded array as the reduction's index range
+  p.ni = d.sx;
+  p.nj = d.sy;
+  p.nk = d.sz;
+  const int sx = d.sx, sy = d.sy;
+  interior_reduce<1>(
+      p,
+      [=] __device__(int i, int j, int k, double* o) {
+        o[0] = qb[(i - 1) + sx * ((j - 1) + sy * (k - 1))];
+      },
+      [=] __device__(const double* t, double* out) { out[0] = t[0] / nn; }, s);
+  launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
+    double* q = qb + d.ix(i, j, k);
+    *q = *q - *s;
+  });
+}
+
+// srcs/corunp_explicit.f:26-74 transposed: the velocity adjoints pass
+// through unchanged; dp_b(c) gathers +f u_b(c) - f u_b(c-x) (likewise v up
+// to jen, w) and pn_b(c) over the interior; phat_b += pn_b and pn_b = 0 on
+// the interior.
+static void corunp_adj(const Dims d, int jen, double fx, double fy, double fz, double* dpb,
+                       double* phatb, const double* unb, const double* vnb, const double* wnb,
+                       double* pnb) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  launch_box(Box{1, ni + 1, 1, nj + 1, 1, nk + 1}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    double t = 0.0;
+    if (i <= ni && j <= nj && k <= nk) {
+      t = t + unb[c] * fx;
+      if (j <= jen) t = t + vnb[c] * fy;
+      t = t + wnb[c] * fz;
+      const double pb = pnb[c];
+      t = t + pb;
+      phatb[c] = phatb[c] + pb;
+      pnb[c] = 0.0;
+    }
+    if (i >= 2 && j <= nj && k <= nk) t = t - unb[c - X] * fx;
+    if (j >= 2 && j - 1 <= jen && i <= ni && k <= nk) t = t - vnb[c - Y] * fy;
+    if (k >= 2 && i <= ni && j <= nj) t = t - wnb[c - Z] * fz;
+    dpb[c] = dpb[c] + t;
+  });
+}
+
+// Device table of the Thomas solves of solp_fft_tdma2_ (srcs/solp_fft_tdma2.f:
+// 46-87): the modal factors 2 cos(theta_x) (ni), 2 cos(theta_z) (nk) from the
+// host libm, as the reference's cos, then as, an, ap (nj+6 each).  Uploaded
+// only when it changes (the coefficients are fixed for a run), so a solve
+// has no host synchronisation here; a change counts as a new operator for
+// the FP32 warm start.
+static const double* mode_table(int ID, const int* svall, const double* as_p_fft,
+                                const double* an_p_fft, const double* ap_p_fft) {
+  const int ni = svall[0], nj = svall[1], nk = svall[2];
+  const double pi_ = std::atan(1.0) * 4.0;
+  const double pi2_ni = pi_ * 2.0 / (double)ni, pi2_nk = pi_ * 2.0 / (double)nk;
+  std::vector<double> h((size_t)ni + nk + 3 * (nj + 6));
+  for (int i = 0; i < ni; i++) h[i] = std::cos((double)(i - ni / 2) * pi2_ni) * 2.0;
+  for (int k = 0; k < nk; k++) h[ni + k] = std::cos((double)(k + 1 + ID * nk - nk / 2 - 1) * pi2_nk) * 2.0;
+  for (int j = 0; j < nj + 6; j++) {
+    h[ni + nk + j] = as_p_fft[j];
+    h[ni + nk + (nj + 6) + j] = an_p_fft[j];
+    h[ni + nk + 2 * (nj + 6) + j] = ap_p_fft[j];
+  }
+  static std::vector<double> up;  // what the device table holds
+  static double* tab = nullptr;
+  static size_t cap = 0;
+  if (h.size() > cap) {
+    if (tab) {
+      FEL_CHECK(hipStreamSynchronize(stream()));
+      FEL_CHECK(hipFree(tab));
+    }
+    FEL_CHECK(hipMalloc(&tab, sizeof(double) * h.size()));
+    cap = h.size();
+    up.clear();
+  }
+  if (h != up) {
+    up = h;
+    FEL_CHECK(hipMemcpyAsync(tab, up.data(), sizeof(double) * up.size(), hipMemcpyHostToDevice,
+                             stream()));
+    FEL_CHECK(hipStreamSynchronize(stream()));
+    prec().tab_gen++;
+  }
+  return tab;
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -819,42 +962,11 @@ void solp_fft_tdma2_(const int* ID, const int* ndiv, const int* svall, const dou
   }
   const int ni = svall[0], nj = svall[1], nk = svall[2];
   const size_t nm = (size_t)ni * nj * nk;
-  const double pi_ = std::atan(1.0) * 4.0;
-  const double pi2_ni = pi_ * 2.0 / (double)ni, pi2_nk = pi_ * 2.0 / (double)nk;
-  // mode tables (host libm, as the reference's cos)
-  std::vector<double> h((size_t)ni + nk + 3 * (nj + 6));
-  for (int i = 0; i < ni; i++) h[i] = std::cos((double)(i - ni / 2) * pi2_ni) * 2.0;
-  for (int k = 0; k < nk; k++) h[ni + k] = std::cos((double)(k + 1 + (*ID) * nk - nk / 2 - 1) * pi2_nk) * 2.0;
-  for (int j = 0; j < nj + 6; j++) {
-    h[ni + nk + j] = as_p_fft[j];
-    h[ni + nk + (nj + 6) + j] = an_p_fft[j];
-    h[ni + nk + 2 * (nj + 6) + j] = ap_p_fft[j];
-  }
-  // The device table is uploaded only when it changes (the coefficients are
-  // fixed for a run), so the step has no host synchronisation here.
-  static std::vector<double> up;  // what the device table holds
-  static double* tab = nullptr;
-  static size_t cap = 0;
-  if (h.size() > cap) {
-    if (tab) {
-      FEL_CHECK(hipStreamSynchronize(stream()));
-      FEL_CHECK(hipFree(tab));
-    }
-    FEL_CHECK(hipMalloc(&tab, sizeof(double) * h.size()));
-    cap = h.size();
-    up.clear();
-  }
-  const bool changed = h != up;
-  if (changed) {
-    up = h;
-    FEL_CHECK(hipMemcpyAsync(tab, up.data(), sizeof(double) * up.size(), hipMemcpyHostToDevice,
-                             stream()));
-    FEL_CHECK(hipStreamSynchronize(stream()));
-  }
+  const double* tab = mode_table(*ID, svall, as_p_fft, an_p_fft, ap_p_fft);
   const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
   {
     PoissonRecord& rec = prec();
-    if (changed || rec.dx2r != dx2r || rec.dz2r != dz2r) rec.tab_gen++;  // a warm start's operator
+    if (rec.dx2r != dx2r || rec.dz2r != dz2r) rec.tab_gen++;  // a warm start's operator
   }
   Call c("solp_fft_tdma2_");
   with_elem(elem_size(phiw_r), [&](auto t) {
@@ -1014,6 +1126,163 @@ void cal_coef_solp_fft_(const int* nj_, const double* as_p, const double* an_p, 
   as_fft[1] = 0.0;
   ap_fft[nj] = ap_fft[nj] - an_fft[nj];
   an_fft[nj] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint entry points (the forward's argument list with the active arrays
+// replaced by their adjoints; see the adjoint section above).
+// ---------------------------------------------------------------------------
+
+// Adjoint of caldiv_ (srcs/caldiv.f:27-38).  The RMS output is a
+// diagnostic and has no adjoint argument.
+void caldiv_b_(const int* ipara, const int* ndiv, const int* ni, const int* nj, const int* nk,
+               const double* dxinv, const double* dyinv, const double* dzinv, const double* dt,
+               double* ukb, double* vkb, double* wkb, double* divb) {
+  (void)ndiv;
+  if (*ipara != 0) {
+    std::fprintf(stderr, "felgpu: caldiv_b_: only the single-domain case is supported\n");
+    std::abort();
+  }
+  adjoint_fp64_only("caldiv_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const double dtinv = 1.0 / *dt;
+  Call c("caldiv_b_");
+  double* ub = c.io(ukb, d.n3, "ukb");
+  double* vb = c.io(vkb, d.n3, "vkb");
+  double* wb = c.io(wkb, d.n3, "wkb");
+  double* gb = c.io(divb, d.n3, "divb");
+  caldiv_adj(d, *dxinv * dtinv, *dyinv * dtinv, *dzinv * dtinv, ub, vb, wb, gb);
+  c.finish();
+}
+
+// Adjoint of solp_fft_tdma1_ (srcs/solp_fft_tdma1.f:25-47): div_b(interior)
+// += Re(unnormalised inverse DFT of phir_b) / (ni nk); phir_b = 0.
+void solp_fft_tdma1_b_(const int* ni, const int* nj, const int* nk, double* fftdata, double* divb,
+                       double* phir_rb, double* phir_ib) {
+  (void)fftdata;
+  adjoint_fp64_only("solp_fft_tdma1_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int nm = d.ni * d.nj * d.nk;
+  Call c("solp_fft_tdma1_b_");
+  double* gb = c.io(divb, d.n3, "divb");
+  double* pr = c.io(phir_rb, nm, "phir_rb");
+  double* pi = c.io(phir_ib, nm, "phir_ib");
+  const double s = 1.0 / (double)(d.ni * d.nk);  // a power of two: exact scaling
+  launch_flat(nm, [=] __device__(int m) {
+    pr[m] = pr[m] * s;
+    pi[m] = pi[m] * s;
+  });
+  fft_inverse<double, double>(d, pr, pi, gb, true);
+  launch_flat(nm, [=] __device__(int m) {
+    pr[m] = 0.0;
+    pi[m] = 0.0;
+  });
+  c.finish();
+}
+
+// Adjoints of trans_r2w_ / trans_w2r_ on one rank (a copy, none when the
+// arrays coincide): source_b += dest_b, dest_b = 0.
+void trans_r2w_b_(const int* ipara, const int* ID, const int* ndiv, const int* svall, const int* key,
+                  double* sendbuf, double* recvbuf, double* rb, double* wb) {
+  (void)ID; (void)key; (void)sendbuf; (void)recvbuf;
+  if (*ipara != 0 || *ndiv != 1) {
+    std::fprintf(stderr, "felgpu: trans_r2w_b_: only the single-domain case is supported\n");
+    std::abort();
+  }
+  adjoint_fp64_only("trans_r2w_b_");
+  if (rb == wb) return;
+  const int n = svall[0] * svall[1] * svall[2];
+  Call c("trans_r2w_b_");
+  double* a = c.io(rb, n, "rb");
+  double* b = c.io(wb, n, "wb");
+  launch_flat(n, [=] __device__(int m) {
+    a[m] = a[m] + b[m];
+    b[m] = 0.0;
+  });
+  c.finish();
+}
+
+void trans_w2r_b_(const int* ipara, const int* ID, const int* ndiv, const int* svall, const int* key,
+                  double* sendbuf, double* recvbuf, double* wb, double* rb) {
+  trans_r2w_b_(ipara, ID, ndiv, svall, key, sendbuf, recvbuf, wb, rb);
+}
+
+// Adjoint of solp_fft_tdma2_ (srcs/solp_fft_tdma2.f:46-87): the Thomas
+// solve of the transposed systems, in place on the adjoints; the singular
+// (0,0) mode solved pinned (x(nj) = 0), the transpose of the pinned solve.
+void solp_fft_tdma2_b_(const int* ID, const int* ndiv, const int* svall, const double* rho,
+                       const double* dxinv, const double* dzinv, const double* as_p_fft,
+                       const double* an_p_fft, const double* ap_p_fft, double* atdma,
+                       double* btdma_r, double* btdma_i, double* phiw_rb, double* phiw_ib) {
+  (void)atdma; (void)btdma_r; (void)btdma_i;
+  if (*ndiv != 1) {
+    std::fprintf(stderr, "felgpu: solp_fft_tdma2_b_: only the single-domain case is supported\n");
+    std::abort();
+  }
+  adjoint_fp64_only("solp_fft_tdma2_b_");
+  const int ni = svall[0], nj = svall[1], nk = svall[2];
+  const size_t nm = (size_t)ni * nj * nk;
+  const double* tab = mode_table(*ID, svall, as_p_fft, an_p_fft, ap_p_fft);
+  const double dx2r = (*dxinv * *dxinv) / *rho, dz2r = (*dzinv * *dzinv) / *rho;
+  Call c("solp_fft_tdma2_b_");
+  double* pr = c.io(phiw_rb, nm, "phiw_rb");
+  double* pi = c.io(phiw_ib, nm, "phiw_ib");
+  tdma_modes<double, true>(ni, nj, nk, tab, tab + ni, tab + ni + nk, dx2r, dz2r, pr, pi);
+  c.finish();
+}
+
+// Adjoint of solp_fft_tdma3_ (srcs/solp_fft_tdma3.f:25-44): phir_b +=
+// unnormalised forward DFT of dp_b(interior); dp_b(interior) = 0.
+void solp_fft_tdma3_b_(const int* ni, const int* nj, const int* nk, double* fftdata, double* phir_rb,
+                       double* phir_ib, double* dpb) {
+  (void)fftdata;
+  adjoint_fp64_only("solp_fft_tdma3_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const size_t nm = (size_t)d.ni * d.nj * d.nk;
+  Call c("solp_fft_tdma3_b_");
+  double* pr = c.io(phir_rb, nm, "phir_rb");
+  double* pi = c.io(phir_ib, nm, "phir_ib");
+  double* g = c.io(dpb, d.n3, "dpb");
+  fft_forward_core<double, true>(
+      d, [=] __device__(int i, int j, int k) { return g[d.ix(i, j, k)]; }, pr, pi, 1.0);
+  launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk},
+             [=] __device__(int i, int j, int k) { g[d.ix(i, j, k)] = 0.0; });
+  c.finish();
+}
+
+// Adjoint of bnd_avzero_ (srcs/bnd_avzero.f:10-23).
+void bnd_avzero_b_(const int* ipara, const int* ndiv, const int* ni, const int* nj, const int* nk,
+                   double* qb) {
+  if (*ipara != 0) {
+    std::fprintf(stderr, "felgpu: bnd_avzero_b_: only the single-domain case is supported\n");
+    std::abort();
+  }
+  adjoint_fp64_only("bnd_avzero_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("bnd_avzero_b_");
+  avzero_adj(d, c.io(qb, d.n3, "qb"), *ndiv);
+  c.finish();
+}
+
+// Adjoint of corunp_explicit_ (srcs/corunp_explicit.f:26-74).
+void corunp_explicit_b_(const int* nID, const int* ni, const int* nj, const int* nk,
+                        const double* rho, const double* dxinv, const double* dyinv,
+                        const double* dzinv, const double* dt, double* dpb, double* phatb,
+                        double* unb, double* vnb, double* wnb, double* pnb) {
+  adjoint_fp64_only("corunp_explicit_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int jen = nID[kYPlus] < 0 ? d.nj - 1 : d.nj;
+  const double fx = 1.0 / *rho * *dxinv * *dt, fy = 1.0 / *rho * *dyinv * *dt,
+               fz = 1.0 / *rho * *dzinv * *dt;
+  Call c("corunp_explicit_b_");
+  double* gdp = c.io(dpb, d.n3, "dpb");
+  double* gph = c.io(phatb, d.n3, "phatb");
+  const double* gu = c.in(unb, d.n3, "unb");
+  const double* gv = c.in(vnb, d.n3, "vnb");
+  const double* gw = c.in(wnb, d.n3, "wnb");
+  double* gpn = c.io(pnb, d.n3, "pnb");
+  corunp_adj(d, jen, fx, fy, fz, gdp, gph, gu, gv, gw, gpn);
+  c.finish();
 }
 
 }  // extern "C"

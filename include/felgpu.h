@@ -523,6 +523,101 @@ void felgpu_interface_crossings_(const int *ni, const int *nj, const int *nk, co
                                  const int *jrow, const double *dx, const double *xl,
                                  const int *first_only, double *x, int *cnt);
 
+/* ----------------------------------------------------------------------
+ * Adjoints (reverse mode) of the linear routines of the velocity-pressure
+ * update.  No counterpart in the reference, which has no adjoint: <name>_b_
+ * applies the transpose of the linear map <name>_ applies to its active
+ * arrays.  Argument list: the forward's, with every active array replaced
+ * by its adjoint (same shape and layout); arrays that only parameterise the
+ * map (densities, coefficients, slip lengths, phi) are passed as in the
+ * forward.  Adjoints of the forward's inputs are incremented, adjoints of
+ * what the forward overwrites are consumed (set to zero), in/out adjoints
+ * are updated in place.  FP64 mode only (abort otherwise).  A reverse sweep
+ * calls them in the reverse of the forward's order.
+ * ---------------------------------------------------------------------- */
+
+/* srcs/caldiv.f:27-38 transposed: uk/vk/wkb += from divb on [-1,n+3]^3,
+ * then divb = 0 there.  The RMS output has no adjoint. */
+void caldiv_b_(const int *ipara, const int *ndiv, const int *ni, const int *nj, const int *nk,
+               const double *dxinv, const double *dyinv, const double *dzinv, const double *dt,
+               double *ukb, double *vkb, double *wkb, double *divb);
+
+/* srcs/solp_fft_tdma1.f:25-47 transposed: divb(interior) += Re(unnormalised
+ * inverse DFT of phir_rb + i phir_ib) / (ni nk); phir_rb = phir_ib = 0. */
+void solp_fft_tdma1_b_(const int *ni, const int *nj, const int *nk, double *fftdata, double *divb,
+                       double *phir_rb, double *phir_ib);
+
+/* srcs/trans_r2w.f, srcs/trans_w2r.f (one rank: a copy) transposed:
+ * source_b += dest_b, dest_b = 0 (nothing when the arrays coincide). */
+void trans_r2w_b_(const int *ipara, const int *ID, const int *ndiv, const int *svall,
+                  const int *key, double *sendbuf, double *recvbuf, double *rb, double *wb);
+void trans_w2r_b_(const int *ipara, const int *ID, const int *ndiv, const int *svall,
+                  const int *key, double *sendbuf, double *recvbuf, double *wb, double *rb);
+
+/* srcs/solp_fft_tdma2.f:46-87 transposed: Thomas solves of the transposed
+ * j systems, in place on phiw_rb, phiw_ib; the singular (0,0) mode solved
+ * pinned (x(nj) = 0), the transpose of the pinned solve. */
+void solp_fft_tdma2_b_(const int *ID, const int *ndiv, const int *svall, const double *rho,
+                       const double *dxinv, const double *dzinv, const double *as_p_fft,
+                       const double *an_p_fft, const double *ap_p_fft, double *atdma,
+                       double *btdma_r, double *btdma_i, double *phiw_rb, double *phiw_ib);
+
+/* srcs/solp_fft_tdma3.f:25-44 transposed: phir_rb + i phir_ib +=
+ * unnormalised forward DFT of dpb(interior); dpb(interior) = 0. */
+void solp_fft_tdma3_b_(const int *ni, const int *nj, const int *nk, double *fftdata,
+                       double *phir_rb, double *phir_ib, double *dpb);
+
+/* srcs/bnd_avzero.f:10-23 transposed: qb(interior) -= sum(qb over the
+ * padded array) / (ni nj nk ndiv). */
+void bnd_avzero_b_(const int *ipara, const int *ndiv, const int *ni, const int *nj, const int *nk,
+                   double *qb);
+
+/* srcs/bnd_neumann.f:14-41 and srcs/bnd_periodic.f:14-43 transposed: each
+ * ghost's adjoint added to its source and cleared, in reverse order. */
+void bnd_neumann_b_(const int *nID, const int *ni, const int *nj, const int *nk, double *qb);
+void bnd_periodic_b_(const int *ni, const int *nj, const int *nk, double *qb);
+
+/* srcs/corunp_explicit.f:26-74 transposed: unb, vnb, wnb unchanged (read);
+ * dpb += the transposed gradients of them and pnb(interior); phatb +=
+ * pnb(interior); pnb(interior) = 0. */
+void corunp_explicit_b_(const int *nID, const int *ni, const int *nj, const int *nk,
+                        const double *rho, const double *dxinv, const double *dyinv,
+                        const double *dzinv, const double *dt, double *dpb, double *phatb,
+                        double *unb, double *vnb, double *wnb, double *pnb);
+
+/* srcs/bndu.f:29-158 transposed: the z fills, the x fills (shifted u and w
+ * sources included), the wall rows; ukb/vkb/wkb in place, *uwallb +=
+ * sum(-c1 ghost_b) at the bottom wall and sum(+c1 ghost_b) at the top. */
+void bndu_b_(const int *nID, const int *ni, const int *nj, const int *nk, double *ukb,
+             double *vkb, double *wkb, double *uwallb, const double *dy, const double *l1,
+             const double *l2, const double *phi);
+
+/* srcs/solu_sor4.f:131-293 transposed with the forward call's sweep count
+ * frozen: *lp is that call's loop variable on exit (felgpu_sor_last_), and
+ * min(lp+1, 21) sweeps are reversed.  src_ub/vb/wb incremented; unb, vnb,
+ * wnb hold the adjoint of the solve's final state on entry and of its
+ * initial state on return; *uwallb incremented.  Coefficients formed on the
+ * fly from the viscosity when cal_arith_coef_vis_ deferred them, as in
+ * solu_sor4_; otherwise the 45 arrays are read. */
+void solu_sor4_b_(const int *ipara, const int *ID, const int *nID, const int *ndiv, const int *ni,
+                  const int *nj, const int *nk, const int *key, double *sendjb, double *recvjb,
+                  const int *nstep, const int *imon_t, const double *dt, const double *rho,
+                  const double *rhon, double *c0, double *c1, double *c2, double *c3, double *c4,
+                  double *c5, double *c6, double *c7, double *c8, double *c9, double *c10,
+                  double *c11, double *c12, double *c13, double *c14, double *c15, double *c16,
+                  double *c17, double *c18, double *c19, double *c20, double *c21, double *c22,
+                  double *c23, double *c24, double *c25, double *c26, double *c27, double *c28,
+                  double *c29, double *c30, double *c31, double *c32, double *c33, double *c34,
+                  double *c35, double *c36, double *c37, double *c38, double *c39, double *c40,
+                  double *c41, double *c42, double *c43, double *c44, double *src_ub,
+                  double *src_vb, double *src_wb, double *unb, double *vnb, double *wnb,
+                  double *uwallb, const double *dy, const double *l1, const double *l2,
+                  const double *phi, const int *lp);
+
+/* How the last solu_sor4_b_ call obtained its coefficients: 1 formed on the
+ * fly from the viscosity, 0 read from the 45 arrays, -1 before any call. */
+int felgpu_sor_b_onfly_(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -776,14 +776,6 @@ void orc_cal_srcu(const int *nID, const int *ni_, const int *nj_, const int *nk_
 /* Semi-implicit viscous step                                                */
 /* ------------------------------------------------------------------------- */
 
-/* The 45 coefficient arrays of srcs/cal_arith_coef_vis.f, in argument order. */
-enum {
-  CU_W, CU_E, CU_S, CU_N, CU_B, CU_T, CU_P, CU_VWS, CU_VES, CU_VWN, CU_VEN, CU_WWB, CU_WEB, CU_WWT, CU_WET,
-  CV_W, CV_E, CV_S, CV_N, CV_B, CV_T, CV_P, CV_USW, CV_UNW, CV_USE, CV_UNE, CV_WSB, CV_WNB, CV_WST, CV_WNT,
-  CW_W, CW_E, CW_S, CW_N, CW_B, CW_T, CW_P, CW_UBW, CW_UTW, CW_UBE, CW_UTE, CW_VBS, CW_VTS, CW_VBN, CW_VTN,
-  NCOEF
-};
-
 /* srcs/cal_arith_coef_vis.f:1-411 with the 45 arrays gathered in a[45].
  * pinned: _ref (through the argument-list wrapper below). */
 static void coef_vis(int ni, int nj, int nk, double dxinv, double dyinv, double dzinv,

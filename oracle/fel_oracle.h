@@ -33,4 +33,13 @@
 enum { ORC_X_MINUS = 1, ORC_Y_MINUS = 2, ORC_Z_MINUS = 3,
        ORC_X_PLUS = 4, ORC_Y_PLUS = 5, ORC_Z_PLUS = 6 };
 
+/* The 45 coefficient arrays of srcs/cal_arith_coef_vis.f (and of
+ * srcs/solu_sor4.f's argument list), in argument order. */
+enum {
+  CU_W, CU_E, CU_S, CU_N, CU_B, CU_T, CU_P, CU_VWS, CU_VES, CU_VWN, CU_VEN, CU_WWB, CU_WEB, CU_WWT, CU_WET,
+  CV_W, CV_E, CV_S, CV_N, CV_B, CV_T, CV_P, CV_USW, CV_UNW, CV_USE, CV_UNE, CV_WSB, CV_WNB, CV_WST, CV_WNT,
+  CW_W, CW_E, CW_S, CW_N, CW_B, CW_T, CW_P, CW_UBW, CW_UTW, CW_UBE, CW_UTE, CW_VBS, CW_VTS, CW_VBN, CW_VTN,
+  NCOEF
+};
+
 #endif

@@ -1,0 +1,437 @@
+"""Seeded cases for the adjoint (reverse-mode) routines <name>_b.
+
+Each case is a linear map F of the forward routine <name> (its active
+arrays in, its active arrays out, passive data fixed) and the matching
+adjoint routine B.  ``forward(call, x)`` and ``adjoint(call, y)`` run them
+through a caller -- the oracle (``H.call_oracle``) or the GPU library
+(``H.call_gpu``) -- on dicts of arrays, so the same case serves the
+dot-product identity <F x, y> = <x, B y> and the GPU/oracle comparison.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+import felharness as H
+from felharness import D, I, Grid
+
+WALLS = np.array([-1, -1, -1, -1, -1, -1], dtype=np.int32)
+
+
+def dot(a: dict, b: dict) -> float:
+    return float(sum(np.dot(np.ravel(a[k]), np.ravel(b[k])) for k in a))
+
+
+def scale(a: dict, b: dict) -> float:
+    na = np.sqrt(sum(float(np.sum(np.square(a[k]))) for k in a))
+    nb = np.sqrt(sum(float(np.sum(np.square(b[k]))) for k in b))
+    return na * nb
+
+
+def box_mask(g: Grid, lo: int, hi_off: int):
+    """Mask of the padded cells in [lo, n+hi_off]^3."""
+    m = np.zeros(g.shape, bool)
+    m[lo + 2:g.nk + hi_off + 3, lo + 2:g.nj + hi_off + 3, lo + 2:g.ni + hi_off + 3] = True
+    return m
+
+
+def interior_mask(g: Grid):
+    return box_mask(g, 1, 0)
+
+
+class Case:
+    name = ""
+
+    def __init__(self, g: Grid, seed: int):
+        self.g = g
+        self.rng = np.random.default_rng(seed)
+
+    def inputs(self) -> dict:
+        raise NotImplementedError
+
+    def out_adjoints(self) -> dict:
+        raise NotImplementedError
+
+    def forward(self, call, x: dict) -> dict:
+        raise NotImplementedError
+
+    def adjoint(self, call, y: dict) -> dict:
+        raise NotImplementedError
+
+
+class Caldiv(Case):
+    name = "caldiv"
+    dt = 0.003
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("u", "v", "w")}
+
+    def out_adjoints(self):
+        return {"div": np.where(box_mask(self.g, -1, 3), self.rng.standard_normal(self.g.shape), 0.0)}
+
+    def head(self):
+        g = self.g
+        return [I(0), I(1), I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), D(self.dt)]
+
+    def forward(self, call, x):
+        div = self.g.zeros()
+        call("caldiv", *self.head(), x["u"].copy(), x["v"].copy(), x["w"].copy(), div, D(0.0))
+        return {"div": np.where(box_mask(self.g, -1, 3), div, 0.0)}
+
+    def adjoint(self, call, y):
+        ub, vb, wb = self.g.zeros(), self.g.zeros(), self.g.zeros()
+        divb = y["div"].copy()
+        call("caldiv_b", *self.head(), ub, vb, wb, divb)
+        assert not np.any(divb[box_mask(self.g, -1, 3)]), "div_b must be consumed"
+        return {"u": ub, "v": vb, "w": wb}
+
+
+class Tdma1(Case):
+    name = "solp_fft_tdma1"
+
+    def nm(self):
+        g = self.g
+        return g.ni * g.nj * g.nk
+
+    def inputs(self):
+        return {"div": np.where(interior_mask(self.g), self.rng.standard_normal(self.g.shape), 0.0)}
+
+    def out_adjoints(self):
+        return {"pr": self.rng.standard_normal(self.nm()), "pi": self.rng.standard_normal(self.nm())}
+
+    def forward(self, call, x):
+        g = self.g
+        pr, pi = np.zeros(self.nm()), np.zeros(self.nm())
+        call("solp_fft_tdma1", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * self.nm()), x["div"].copy(), pr, pi)
+        return {"pr": pr, "pi": pi}
+
+    def adjoint(self, call, y):
+        g = self.g
+        divb = g.zeros()
+        pr, pi = y["pr"].copy(), y["pi"].copy()
+        call("solp_fft_tdma1_b", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * self.nm()), divb, pr, pi)
+        assert not np.any(pr) and not np.any(pi)
+        return {"div": np.where(interior_mask(g), divb, 0.0)}
+
+
+class Tdma3(Tdma1):
+    name = "solp_fft_tdma3"
+
+    def inputs(self):
+        return {"pr": self.rng.standard_normal(self.nm()), "pi": self.rng.standard_normal(self.nm())}
+
+    def out_adjoints(self):
+        return {"dp": np.where(interior_mask(self.g), self.rng.standard_normal(self.g.shape), 0.0)}
+
+    def forward(self, call, x):
+        g = self.g
+        dp = g.zeros()
+        call("solp_fft_tdma3", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * self.nm()), x["pr"].copy(),
+             x["pi"].copy(), dp)
+        return {"dp": np.where(interior_mask(g), dp, 0.0)}
+
+    def adjoint(self, call, y):
+        g = self.g
+        pr, pi = np.zeros(self.nm()), np.zeros(self.nm())
+        dpb = y["dp"].copy()
+        call("solp_fft_tdma3_b", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * self.nm()), pr, pi, dpb)
+        assert not np.any(dpb[interior_mask(g)])
+        return {"pr": pr, "pi": pi}
+
+
+class Tdma2(Tdma1):
+    """Thomas solves per mode.  `skew` makes the j coefficients
+    non-symmetric (as(j+1) != an(j)), so that the transpose differs from the
+    forward solve; the singular (0,0) mode's entries are zero in both the
+    inputs and the output adjoints (its solve is defined up to a constant)."""
+    name = "solp_fft_tdma2"
+    rho = 0.81
+
+    def __init__(self, g, seed, skew=False):
+        super().__init__(g, seed)
+        pc = [np.zeros(g.nj + 6) for _ in range(7)]
+        H.call_oracle("cal_coef_prs", I(g.nj), D(self.rho), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), *pc)
+        fft = [np.zeros(g.nj + 6) for _ in range(3)]
+        H.call_oracle("cal_coef_solp_fft", I(g.nj), pc[2], pc[3], *fft)
+        if skew:
+            r = np.random.default_rng(seed + 1000)
+            fft[0][3:] *= r.uniform(0.5, 1.5, g.nj + 3)
+            fft[1][:-3] *= r.uniform(0.5, 1.5, g.nj + 3)
+            fft[2][:] = fft[0] + fft[1]
+            fft[0][3] = 0.0  # Neumann rows keep as(1) = an(nj) = 0
+            fft[1][g.nj + 2] = 0.0
+        self.fft = fft
+
+    def singular_zero(self, a):
+        g = self.g
+        a = a.reshape(g.nk, g.nj, g.ni).copy()
+        a[g.nk // 2, :, g.ni // 2] = 0.0
+        return a.ravel()
+
+    def inputs(self):
+        return {"pr": self.singular_zero(self.rng.standard_normal(self.nm())),
+                "pi": self.singular_zero(self.rng.standard_normal(self.nm()))}
+
+    def out_adjoints(self):
+        return self.inputs()
+
+    def head(self):
+        g = self.g
+        svall = np.array([g.ni, g.nj, g.nk], dtype=np.int32)
+        n1 = max(g.nj, g.nk) + 6
+        return [I(0), I(1), svall, D(self.rho), D(1 / g.dx), D(1 / g.dz), self.fft[0].copy(),
+                self.fft[1].copy(), self.fft[2].copy(), np.zeros(n1), np.zeros(n1), np.zeros(n1)]
+
+    def forward(self, call, x):
+        pr, pi = x["pr"].copy(), x["pi"].copy()
+        call("solp_fft_tdma2", *self.head(), pr, pi)
+        return {"pr": pr, "pi": pi}
+
+    def adjoint(self, call, y):
+        pr, pi = y["pr"].copy(), y["pi"].copy()
+        call("solp_fft_tdma2_b", *self.head(), pr, pi)
+        return {"pr": pr, "pi": pi}
+
+
+class InPlace(Case):
+    """Routines that map one padded array to itself in place."""
+
+    def inputs(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def out_adjoints(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def args(self, q):
+        g = self.g
+        return [I(g.ni), I(g.nj), I(g.nk), q]
+
+    def forward(self, call, x):
+        q = x["q"].copy()
+        call(self.name, *self.args(q))
+        return {"q": q}
+
+    def adjoint(self, call, y):
+        q = y["q"].copy()
+        call(self.name + "_b", *self.args(q))
+        return {"q": q}
+
+
+class Periodic(InPlace):
+    name = "bnd_periodic"
+
+
+class Neumann(InPlace):
+    name = "bnd_neumann"
+
+    def args(self, q):
+        g = self.g
+        return [WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), q]
+
+
+class Avzero(InPlace):
+    name = "bnd_avzero"
+
+    def args(self, q):
+        g = self.g
+        return [I(0), I(1), I(g.ni), I(g.nj), I(g.nk), q]
+
+
+class Corunp(Case):
+    name = "corunp_explicit"
+    rho, dt = 0.81, 0.003
+    keys = ("dp", "phat", "un", "vn", "wn", "pn")
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in self.keys}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("un", "vn", "wn", "pn")}
+
+    def head(self):
+        g = self.g
+        return [WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), D(self.rho), D(1 / g.dx), D(1 / g.dy),
+                D(1 / g.dz), D(self.dt)]
+
+    def forward(self, call, x):
+        a = {k: x[k].copy() for k in self.keys}
+        call("corunp_explicit", *self.head(), *[a[k] for k in self.keys])
+        return {k: a[k] for k in ("un", "vn", "wn", "pn")}
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {"dp": g.zeros(), "phat": g.zeros()}
+        b.update({k: y[k].copy() for k in ("un", "vn", "wn", "pn")})
+        call("corunp_explicit_b", *self.head(), *[b[k] for k in self.keys])
+        return b
+
+
+class Bndu(Case):
+    name = "bndu"
+    keys = ("u", "v", "w")
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 500)
+        self.l1 = r.uniform(0.01, 0.1, g.shape)
+        self.l2 = r.uniform(0.01, 0.1, g.shape)
+        self.phi = r.uniform(0, 1, g.shape)
+
+    def inputs(self):
+        x = {k: self.rng.standard_normal(self.g.shape) for k in self.keys}
+        x["uwall"] = np.array([self.rng.standard_normal()])
+        return x
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in self.keys}
+
+    def forward(self, call, x):
+        g = self.g
+        a = [x[k].copy() for k in self.keys]
+        call("bndu", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), *a, D(float(x["uwall"][0])), D(g.dy),
+             self.l1, self.l2, self.phi)
+        return dict(zip(self.keys, a))
+
+    def adjoint(self, call, y):
+        g = self.g
+        a = [y[k].copy() for k in self.keys]
+        ub = D(0.0)
+        call("bndu_b", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), *a, ub, D(g.dy), self.l1, self.l2,
+             self.phi)
+        out = dict(zip(self.keys, a))
+        out["uwall"] = np.array([ub.value])
+        return out
+
+
+def sor_lp_oracle() -> int:
+    return ctypes.c_int.in_dll(H.oracle(), "orc_sor_last_lp").value
+
+
+def sor_lp_gpu() -> int:
+    lp, ratio = ctypes.c_int(0), ctypes.c_double(0.0)
+    H.gpu().felgpu_sor_last_(ctypes.byref(lp), ctypes.byref(ratio))
+    return lp.value
+
+
+class Sor(Case):
+    """solu_sor4 with its sweep count frozen at the forward call's: linear in
+    (src_u, src_v, src_w, un, vn, wn, uwall) for fixed densities and
+    coefficients."""
+    name = "solu_sor4"
+    keys = ("src_u", "src_v", "src_w", "un", "vn", "wn")
+    dt = 0.01
+
+    def __init__(self, g, seed, smooth=True):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 700)
+        self.rmu = r.uniform(0.01, 0.2, g.shape)
+        self.coef = [g.zeros() for _ in range(45)]
+        H.call_oracle("cal_arith_coef_vis", I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), D(1 / g.dy),
+                      D(1 / g.dz), self.rmu, *self.coef)
+        self.rho = r.uniform(0.5, 1.5, g.shape)
+        self.rhon = r.uniform(0.5, 1.5, g.shape)
+        self.l1 = np.full(g.shape, 0.05)
+        self.phi = r.uniform(0, 1, g.shape)
+        self.smooth = smooth
+        self.lp = None
+
+    def inputs(self):
+        g, r = self.g, self.rng
+        x = {k: r.standard_normal(g.shape) for k in self.keys[:3]}
+        for k in self.keys[3:]:
+            x[k] = H.smooth_vel(g, r) if self.smooth else r.standard_normal(g.shape)
+        x["uwall"] = np.array([r.standard_normal()])
+        return x
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("un", "vn", "wn")}
+
+    def head(self):
+        g = self.g
+        return [I(0), I(0), WALLS.copy(), I(1), I(g.ni), I(g.nj), I(g.nk), np.zeros(8, np.int32),
+                np.zeros(8), np.zeros(8), I(1), I(0), D(self.dt), self.rho, self.rhon] + \
+            [c.copy() for c in self.coef]
+
+    def forward(self, call, x, lp_reader=None):
+        g = self.g
+        a = {k: x[k].copy() for k in self.keys}
+        call("solu_sor4", *self.head(), *[a[k] for k in self.keys], D(float(x["uwall"][0])), D(g.dy),
+             self.l1, self.l1, self.phi)
+        self.lp = (lp_reader or sor_lp_oracle)()
+        return {k: a[k] for k in ("un", "vn", "wn")}
+
+    def adjoint(self, call, y):
+        g = self.g
+        assert self.lp is not None, "run the forward first (its sweep count is frozen)"
+        b = {k: g.zeros() for k in self.keys[:3]}
+        b.update({k: y[k].copy() for k in ("un", "vn", "wn")})
+        ub = D(0.0)
+        call("solu_sor4_b", *self.head(), *[b[k] for k in self.keys], ub, D(g.dy), self.l1, self.l1,
+             self.phi, I(self.lp))
+        b["uwall"] = np.array([ub.value])
+        return b
+
+
+def projection_forward(call, g, fft, x):
+    """caldiv_ .. corunp_explicit_ (srcs/main.f90:634-650) on (un, vn, wn,
+    phat) -> (un, vn, wn, pn); the divergence, modes and dp are work arrays."""
+    rho, dt = 0.81, 0.003
+    nm = g.ni * g.nj * g.nk
+    svall = np.array([g.ni, g.nj, g.nk], dtype=np.int32)
+    u, v, w, phat = (x[k].copy() for k in ("un", "vn", "wn", "phat"))
+    div, dp, pn = g.zeros(), g.zeros(), g.zeros()
+    pr, pi = np.zeros(nm), np.zeros(nm)
+    n1 = max(g.nj, g.nk) + 6
+    call("caldiv", I(0), I(1), I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), D(dt),
+         u, v, w, div, D(0.0))
+    call("solp_fft_tdma1", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * nm), div, pr, pi)
+    call("solp_fft_tdma2", I(0), I(1), svall, D(rho), D(1 / g.dx), D(1 / g.dz), fft[0], fft[1], fft[2],
+         np.zeros(n1), np.zeros(n1), np.zeros(n1), pr, pi)
+    call("solp_fft_tdma3", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * nm), pr, pi, dp)
+    call("bnd_avzero", I(0), I(1), I(g.ni), I(g.nj), I(g.nk), dp)
+    call("bnd_neumann", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), dp)
+    call("bnd_periodic", I(g.ni), I(g.nj), I(g.nk), dp)
+    call("corunp_explicit", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), D(rho), D(1 / g.dx), D(1 / g.dy),
+         D(1 / g.dz), D(dt), dp, phat, u, v, w, pn)
+    return {"un": u, "vn": v, "wn": w, "pn": np.where(interior_mask(g), pn, 0.0)}
+
+
+def projection_adjoint(call, g, fft, y):
+    rho, dt = 0.81, 0.003
+    nm = g.ni * g.nj * g.nk
+    svall = np.array([g.ni, g.nj, g.nk], dtype=np.int32)
+    n1 = max(g.nj, g.nk) + 6
+    ub, vb, wb = (y[k].copy() for k in ("un", "vn", "wn"))
+    pnb = np.where(interior_mask(g), y["pn"], 0.0)
+    dpb, phatb, divb = g.zeros(), g.zeros(), g.zeros()
+    prb, pib = np.zeros(nm), np.zeros(nm)
+    call("corunp_explicit_b", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), D(rho), D(1 / g.dx), D(1 / g.dy),
+         D(1 / g.dz), D(dt), dpb, phatb, ub, vb, wb, pnb)
+    call("bnd_periodic_b", I(g.ni), I(g.nj), I(g.nk), dpb)
+    call("bnd_neumann_b", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), dpb)
+    call("bnd_avzero_b", I(0), I(1), I(g.ni), I(g.nj), I(g.nk), dpb)
+    call("solp_fft_tdma3_b", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * nm), prb, pib, dpb)
+    call("solp_fft_tdma2_b", I(0), I(1), svall, D(rho), D(1 / g.dx), D(1 / g.dz), fft[0], fft[1], fft[2],
+         np.zeros(n1), np.zeros(n1), np.zeros(n1), prb, pib)
+    call("solp_fft_tdma1_b", I(g.ni), I(g.nj), I(g.nk), np.zeros(2 * nm), divb, prb, pib)
+    call("caldiv_b", I(0), I(1), I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), D(dt),
+         ub, vb, wb, divb)
+    return {"un": ub, "vn": vb, "wn": wb, "phat": phatb}
+
+
+def consistent_velocity(g, rng):
+    """Random interior with the ghost images the divergence reads made
+    consistent (x and z periodic, v zero on both wall faces), so that every
+    plane's divergence sums to zero and the singular pressure mode's
+    right-hand side is compatible, as it is in a run."""
+    out = {}
+    for k in ("un", "vn", "wn"):
+        q = rng.standard_normal(g.shape)
+        if k == "vn":
+            q[:, 2, :] = 0.0          # j = 0
+            q[:, g.nj + 2, :] = 0.0   # j = nj
+        H.call_oracle("bnd_periodic", I(g.ni), I(g.nj), I(g.nk), q)
+        out[k] = q
+    out["phat"] = rng.standard_normal(g.shape)
+    return out

@@ -1,0 +1,162 @@
+"""Parity of the GPU adjoint routines <name>_b_ with the oracle's.
+
+For every routine, on the same seeded inputs: the GPU adjoint equals the
+oracle adjoint (itself pinned by the dot-product identity against the
+forward oracle, tests/test_oracle_adjoint.py) to rounding, and the GPU
+adjoint is the transpose of the GPU forward (<F x, y> = <x, F^T y>).  The
+Gauss-Seidel adjoint is checked with coefficient arrays and with the
+coefficients formed on the fly from the viscosity (registered arrays,
+cal_arith_coef_vis_ deferred), at the sweep count of the GPU forward call.
+The last test runs the whole pressure projection (caldiv_ .. corunp_explicit_,
+srcs/main.f90:634-650) forward and reversed through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import adjoint_cases as A
+import felharness as H
+from felharness import D, I, Grid
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [Grid(8, 6, 8, 1 / 8, 1 / 6, 1 / 8), Grid(32, 16, 8, 0.1, 0.05, 0.2)]
+
+
+def rel_diff(a: dict, b: dict) -> float:
+    worst = 0.0
+    for k in a:
+        x, y = np.asarray(a[k]), np.asarray(b[k])
+        sc = max(float(np.max(np.abs(x))), 1e-300)
+        worst = max(worst, float(np.max(np.abs(x - y))) / sc)
+    return worst
+
+
+def check_case(case, rtol=1e-13):
+    print(f"adjoint case {case.name} grid {case.g.ni}x{case.g.nj}x{case.g.nk}", flush=True)
+    x = case.inputs()
+    y = case.out_adjoints()
+    bo = case.adjoint(H.call_oracle, {k: v.copy() for k, v in y.items()})
+    bg = case.adjoint(H.call_gpu, {k: v.copy() for k, v in y.items()})
+    assert rel_diff(bo, bg) <= rtol, (case.name, rel_diff(bo, bg))
+    fx = case.forward(H.call_gpu, x)
+    lhs, rhs = A.dot(fx, y), A.dot(x, bg)
+    assert abs(lhs - rhs) <= 1e-12 * max(A.scale(fx, y), A.scale(x, bg)), (case.name, lhs, rhs)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.Caldiv, A.Tdma1, A.Tdma3, A.Periodic, A.Neumann, A.Avzero,
+                                 A.Corunp, A.Bndu], ids=lambda c: c.name)
+def test_adjoint_matches_oracle(cls, gi):
+    check_case(cls(GRIDS[gi], 51 + gi))
+
+
+@pytest.mark.parametrize("skew", [False, True], ids=["reference_coefficients", "skewed"])
+def test_tdma2_adjoint(skew):
+    check_case(A.Tdma2(GRIDS[1], 61, skew=skew))
+
+
+def test_trans_adjoint():
+    g = GRIDS[0]
+    n = g.ni * g.nj * g.nk
+    r = np.random.default_rng(3)
+    svall = np.array([g.ni, g.nj, g.nk], dtype=np.int32)
+    for name in ("trans_r2w_b", "trans_w2r_b"):
+        a, b = r.standard_normal(n), r.standard_normal(n)
+        ao, bo = a.copy(), b.copy()
+        H.call_oracle(name, I(0), I(0), I(1), svall, np.zeros(1, np.int32), np.zeros(1), np.zeros(1), ao, bo)
+        H.call_gpu(name, I(0), I(0), I(1), svall, np.zeros(1, np.int32), np.zeros(1), np.zeros(1), a, b)
+        assert np.array_equal(a, ao) and np.array_equal(b, bo)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_sor_adjoint_coefficient_arrays(gi):
+    g = GRIDS[gi]
+    case = A.Sor(g, 71 + gi)
+    print(f"adjoint case solu_sor4 arrays grid {g.ni}x{g.nj}x{g.nk}", flush=True)
+    x = case.inputs()
+    y = case.out_adjoints()
+    fx = case.forward(H.call_gpu, x, lp_reader=A.sor_lp_gpu)
+    bg = case.adjoint(H.call_gpu, {k: v.copy() for k, v in y.items()})
+    assert H.gpu().felgpu_sor_b_onfly_() == 0
+    bo = case.adjoint(H.call_oracle, {k: v.copy() for k, v in y.items()})
+    assert rel_diff(bo, bg) <= 1e-12, rel_diff(bo, bg)
+    lhs, rhs = A.dot(fx, y), A.dot(x, bg)
+    assert abs(lhs - rhs) <= 1e-11 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
+
+
+class Registered:
+    def __init__(self):
+        self.arrays = []
+
+    def add(self, a):
+        n, up = ctypes.c_int64(a.nbytes), ctypes.c_int(1)
+        H.gpu().felgpu_register_(ctypes.c_void_p(a.ctypes.data), ctypes.byref(n), ctypes.byref(up))
+        self.arrays.append(a)
+        return a
+
+    def download(self, a):
+        H.gpu().felgpu_download_(ctypes.c_void_p(a.ctypes.data))
+        return a
+
+    def close(self):
+        for a in self.arrays:
+            H.gpu().felgpu_unregister_(ctypes.c_void_p(a.ctypes.data))
+        self.arrays = []
+
+
+def test_sor_adjoint_on_the_fly_coefficients():
+    """Registered arrays: cal_arith_coef_vis_ defers the 45 coefficients,
+    solu_sor4_ and solu_sor4_b_ form them from the viscosity."""
+    g = GRIDS[1]
+    case = A.Sor(g, 81)
+    print(f"adjoint case solu_sor4 on-the-fly grid {g.ni}x{g.nj}x{g.nk}", flush=True)
+    x = case.inputs()
+    y = case.out_adjoints()
+    reg = Registered()
+    try:
+        rmu = reg.add(case.rmu.copy())
+        coef = [reg.add(g.zeros()) for _ in range(45)]
+        H.call_gpu("cal_arith_coef_vis", I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), D(1 / g.dy),
+                   D(1 / g.dz), rmu, *coef)
+        rho, rhon = reg.add(case.rho.copy()), reg.add(case.rhon.copy())
+        l1, phi = reg.add(case.l1.copy()), reg.add(case.phi.copy())
+        a = {k: reg.add(x[k].copy()) for k in case.keys}
+        head = [I(0), I(0), A.WALLS.copy(), I(1), I(g.ni), I(g.nj), I(g.nk), np.zeros(8, np.int32),
+                np.zeros(8), np.zeros(8), I(1), I(0), D(case.dt), rho, rhon] + coef
+        H.call_gpu("solu_sor4", *head, *[a[k] for k in case.keys], D(float(x["uwall"][0])), D(g.dy),
+                   l1, l1, phi)
+        lp = A.sor_lp_gpu()
+        fx = {k: reg.download(a[k]).copy() for k in ("un", "vn", "wn")}
+        b = {k: g.zeros() for k in case.keys[:3]}
+        b.update({k: y[k].copy() for k in ("un", "vn", "wn")})
+        ub = D(0.0)
+        H.call_gpu("solu_sor4_b", *head, *[b[k] for k in case.keys], ub, D(g.dy), l1, l1, phi, I(lp))
+        assert H.gpu().felgpu_sor_b_onfly_() == 1
+        b["uwall"] = np.array([ub.value])
+    finally:
+        reg.close()
+    # the same adjoint from the oracle with the coefficient arrays
+    case.lp = lp
+    bo = case.adjoint(H.call_oracle, {k: v.copy() for k, v in y.items()})
+    assert rel_diff(bo, b) <= 1e-12, rel_diff(bo, b)
+    lhs, rhs = A.dot(fx, y), A.dot(x, b)
+    assert abs(lhs - rhs) <= 1e-11 * max(A.scale(fx, y), A.scale(x, b)), (lhs, rhs)
+
+
+def test_projection_chain_transpose():
+    g = Grid(32, 16, 16, 1 / 32, 1 / 16, 1 / 16)
+    print("adjoint case projection chain 32x16x16", flush=True)
+    case = A.Tdma2(g, 91)
+    rng = np.random.default_rng(92)
+    x = A.consistent_velocity(g, rng)
+    y = {k: rng.standard_normal(g.shape) for k in ("un", "vn", "wn", "pn")}
+    fx = A.projection_forward(H.call_gpu, g, case.fft, x)
+    bg = A.projection_adjoint(H.call_gpu, g, case.fft, y)
+    bo = A.projection_adjoint(H.call_oracle, g, case.fft, y)
+    assert rel_diff(bo, bg) <= 1e-11, rel_diff(bo, bg)
+    lhs, rhs = A.dot(fx, y), A.dot(x, bg)
+    assert abs(lhs - rhs) <= 1e-11 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)

@@ -1,0 +1,81 @@
+"""Pins the oracle's adjoint routines (oracle/fel_oracle_adj.c).
+
+The reference has no adjoint, so the pin is the defining identity of a
+transpose: for the linear map F of each forward routine (pinned to the
+compiled reference elsewhere) and seeded random x, y,
+
+    <F x, y> = <x, F^T y>
+
+to rounding.  A dropped term, a wrong index or sign, a coefficient taken
+from the wrong cell or a reversed order of the forward's assignments each
+break it by O(1) (the last test shows it for the one place where the
+forward and its transpose differ only by the order of the coefficients).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import adjoint_cases as A
+import felharness as H
+from felharness import Grid
+
+GRIDS = [Grid(8, 6, 8, 1 / 8, 1 / 6, 1 / 8), Grid(16, 8, 4, 0.1, 0.05, 0.2)]
+
+
+def dot_check(case, call=H.call_oracle, rtol=1e-12):
+    x = case.inputs()
+    y = case.out_adjoints()
+    fx = case.forward(call, x)
+    by = case.adjoint(call, y)
+    lhs, rhs = A.dot(fx, y), A.dot(x, by)
+    sc = max(A.scale(fx, y), A.scale(x, by))
+    assert abs(lhs - rhs) <= rtol * sc, (case.name, lhs, rhs, sc)
+    return lhs, rhs, sc
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.Caldiv, A.Tdma1, A.Tdma3, A.Periodic, A.Neumann, A.Avzero,
+                                 A.Corunp, A.Bndu], ids=lambda c: c.name)
+def test_dot_product(cls, gi):
+    dot_check(cls(GRIDS[gi], 11 + gi))
+
+
+@pytest.mark.parametrize("skew", [False, True], ids=["reference_coefficients", "skewed"])
+def test_dot_product_tdma2(skew):
+    dot_check(A.Tdma2(GRIDS[0], 21, skew=skew))
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("smooth", [True, False], ids=["smooth", "random"])
+def test_dot_product_sor(gi, smooth):
+    case = A.Sor(GRIDS[gi], 31 + gi, smooth=smooth)
+    dot_check(case, rtol=1e-11)
+    assert 0 <= case.lp <= 21
+
+
+@pytest.mark.parametrize("consistent", [True, False], ids=["consistent_ghosts", "random_ghosts"])
+def test_projection_chain(consistent):
+    """caldiv .. corunp_explicit (srcs/main.f90:634-650) forward, and the
+    adjoints in reverse order."""
+    g = Grid(16, 8, 8, 1 / 16, 1 / 8, 1 / 8)
+    case = A.Tdma2(g, 93)
+    rng = np.random.default_rng(94)
+    x = A.consistent_velocity(g, rng) if consistent else \
+        {k: rng.standard_normal(g.shape) for k in ("un", "vn", "wn", "phat")}
+    y = {k: rng.standard_normal(g.shape) for k in ("un", "vn", "wn", "pn")}
+    fx = A.projection_forward(H.call_oracle, g, case.fft, x)
+    bx = A.projection_adjoint(H.call_oracle, g, case.fft, y)
+    lhs, rhs = A.dot(fx, y), A.dot(x, bx)
+    assert abs(lhs - rhs) <= 1e-12 * max(A.scale(fx, y), A.scale(x, bx)), (lhs, rhs)
+
+
+def test_planted_transpose_mistake_is_caught():
+    """The skewed Thomas systems are not symmetric: using the forward solve
+    as its own adjoint (a plausible mistake) violates the identity by O(1)."""
+    case = A.Tdma2(GRIDS[0], 22, skew=True)
+    x, y = case.inputs(), case.out_adjoints()
+    fx = case.forward(H.call_oracle, x)
+    wrong = case.forward(H.call_oracle, y)  # F used in place of F^T
+    lhs, rhs = A.dot(fx, y), A.dot(x, wrong)
+    assert abs(lhs - rhs) > 1e-6 * A.scale(fx, y)
