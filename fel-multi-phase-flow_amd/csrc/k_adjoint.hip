@@ -338,6 +338,310 @@ static void colour_adj(bool onfly, const Dims d, const SorAdj& a, int col) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Explicit momentum terms, reversed.  Every kernel gathers what one cell
+// receives; where a cell's adjoint is also read by its neighbours, the
+// consumed outputs are cleared by a second launch.
+// ---------------------------------------------------------------------------
+
+// Membership of (i,j,k) in the box [i0,i1]x[j0,j1]x[k0,k1].
+struct IBox {
+  int i0, i1, j0, j1, k0, k1;
+  __device__ __forceinline__ bool has(int i, int j, int k) const {
+    return i >= i0 && i <= i1 && j >= j0 && j <= j1 && k >= k0 && k <= k1;
+  }
+};
+
+// srcs/calphat.f:16-22 reversed (phat on i in [-1,ni+3], j, k in [-2,n+3]).
+static void calphat_adj(const Dims d, double* pb, double* pob, double* phatb) {
+  launch_box(Box{-1, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    const double g = phatb[c];
+    pb[c] = pb[c] + g * 2.0;
+    pob[c] = pob[c] - g;
+    phatb[c] = 0.0;
+  });
+}
+
+// srcs/cal_grad_prs.f:24-37 reversed (gradients on [-2,n+2]^3).
+static void grad_prs_adj(const Dims d, double dxinv, double dyinv, double dzinv, double* qb,
+                         double* gx, double* gy, double* gz) {
+  const IBox B{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2};
+  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    double t = 0.0;
+    if (B.has(i, j, k)) t = t - gx[c] * dxinv - gy[c] * dyinv - gz[c] * dzinv;
+    if (B.has(i - 1, j, k)) t = t + gx[c - X] * dxinv;
+    if (B.has(i, j - 1, k)) t = t + gy[c - Y] * dyinv;
+    if (B.has(i, j, k - 1)) t = t + gz[c - Z] * dzinv;
+    qb[c] = qb[c] + t;
+  });
+  launch_box(Box{-2, d.ni + 2, -2, d.nj + 2, -2, d.nk + 2}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    gx[c] = 0.0;
+    gy[c] = 0.0;
+    gz[c] = 0.0;
+  });
+}
+
+// Boxes of the six strain-rate components written by srcs/calsij.f:28-139
+// (the shear boxes are also those of the edge viscosities of
+// srcs/cal_arith_tau.f:40-118).
+struct SijBoxes {
+  IBox b[6];
+};
+static SijBoxes sij_boxes(const Dims& d) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  return SijBoxes{{IBox{-1, ni + 3, -2, nj + 3, -2, nk + 3}, IBox{-2, ni + 3, -1, nj + 3, -2, nk + 3},
+                   IBox{-1, ni + 3, -1, nj + 3, -1, nk + 3}, IBox{-2, ni + 2, -2, nj + 2, -2, nk + 3},
+                   IBox{-2, ni + 2, -2, nj + 3, -2, nk + 2}, IBox{-2, ni + 3, -2, nj + 2, -2, nk + 2}}};
+}
+
+// srcs/calsij.f:28-139 reversed: velocity adjoints gather the strain-rate
+// adjoints of the components that read them; then the strain-rate adjoints
+// are cleared on their boxes.
+static void calsij_adj(const Dims d, double dxinv, double dyinv, double dzinv, double* ub,
+                       double* vb, double* wb, double* sb) {
+  const SijBoxes S = sij_boxes(d);
+  const size_t n3 = (size_t)d.n3;
+  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    const double *s1 = sb, *s2 = sb + n3, *s3 = sb + 2 * n3, *s4 = sb + 3 * n3,
+                 *s5 = sb + 4 * n3, *s6 = sb + 5 * n3;
+    auto at = [&](const double* s, int l, int di, int dj, int dk, int o) {
+      return S.b[l].has(i + di, j + dj, k + dk) ? s[c + o] : 0.0;
+    };
+    const double tu = (at(s1, 0, 0, 0, 0, 0) - at(s1, 0, 1, 0, 0, X)) * dxinv +
+                      (at(s4, 3, 0, -1, 0, -Y) - at(s4, 3, 0, 0, 0, 0)) * 0.5 * dyinv +
+                      (at(s5, 4, 0, 0, -1, -Z) - at(s5, 4, 0, 0, 0, 0)) * 0.5 * dzinv;
+    const double tv = (at(s2, 1, 0, 0, 0, 0) - at(s2, 1, 0, 1, 0, Y)) * dyinv +
+                      (at(s4, 3, -1, 0, 0, -X) - at(s4, 3, 0, 0, 0, 0)) * 0.5 * dxinv +
+                      (at(s6, 5, 0, 0, -1, -Z) - at(s6, 5, 0, 0, 0, 0)) * 0.5 * dzinv;
+    const double tw = (at(s3, 2, 0, 0, 0, 0) - at(s3, 2, 0, 0, 1, Z)) * dzinv +
+                      (at(s5, 4, -1, 0, 0, -X) - at(s5, 4, 0, 0, 0, 0)) * 0.5 * dxinv +
+                      (at(s6, 5, 0, -1, 0, -Y) - at(s6, 5, 0, 0, 0, 0)) * 0.5 * dyinv;
+    ub[c] = ub[c] + tu;
+    vb[c] = vb[c] + tv;
+    wb[c] = wb[c] + tw;
+  });
+  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+      if (S.b[l].has(i, j, k)) sb[l * n3 + c] = 0.0;
+  });
+}
+
+// srcs/cal_arith_tau.f:21-118 reversed (pointwise; the viscosity fixed).
+static void arith_tau_adj(const Dims d, double* sb, const double* rmu, double* taub) {
+  const SijBoxes S = sij_boxes(d);
+  const size_t n3 = (size_t)d.n3;
+  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      sb[l * n3 + c] = sb[l * n3 + c] + taub[l * n3 + c] * rmu[c] * 2.0;
+      taub[l * n3 + c] = 0.0;
+    }
+    const int o1[3] = {X, X, Y}, o2[3] = {Y, Z, Z};
+#pragma unroll
+    for (int l = 3; l < 6; l++) {
+      if (!S.b[l].has(i, j, k)) continue;
+      const int a = o1[l - 3], b = o2[l - 3];
+      const double m = +0.25 * rmu[c] + 0.25 * rmu[c + a] + 0.25 * rmu[c + b] + 0.25 * rmu[c + a + b];
+      sb[l * n3 + c] = sb[l * n3 + c] + taub[l * n3 + c] * m * 2.0;
+      taub[l * n3 + c] = 0.0;
+    }
+  });
+}
+
+// srcs/cal_div_tensor.f:20-86 reversed: the stress adjoints gather the
+// velocity-point adjoints on the boxes the forward writes; then the vis
+// adjoints are cleared everywhere (the forward overwrites the arrays whole).
+static void div_tensor_adj(const Dims d, double dxinv, double dyinv, double dzinv, double* tb,
+                           double* ub, double* vb, double* wb) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const IBox Bu{-2, ni + 2, -1, nj + 3, -1, nk + 3}, Bv{-1, ni + 3, -2, nj + 2, -1, nk + 3},
+      Bw{-1, ni + 3, -1, nj + 3, -2, nk + 2};
+  const size_t n3 = (size_t)d.n3;
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    auto U = [&](int di, int dj, int dk, int o) { return Bu.has(i + di, j + dj, k + dk) ? ub[c + o] : 0.0; };
+    auto V = [&](int di, int dj, int dk, int o) { return Bv.has(i + di, j + dj, k + dk) ? vb[c + o] : 0.0; };
+    auto W = [&](int di, int dj, int dk, int o) { return Bw.has(i + di, j + dj, k + dk) ? wb[c + o] : 0.0; };
+    const double u0 = U(0, 0, 0, 0), v0 = V(0, 0, 0, 0), w0 = W(0, 0, 0, 0);
+    double* t = tb + c;
+    t[0] = t[0] + (U(-1, 0, 0, -X) - u0) * dxinv;
+    t[n3] = t[n3] + (V(0, -1, 0, -Y) - v0) * dyinv;
+    t[2 * n3] = t[2 * n3] + (W(0, 0, -1, -Z) - w0) * dzinv;
+    t[3 * n3] = t[3 * n3] + (u0 - U(0, 1, 0, Y)) * dyinv + (v0 - V(1, 0, 0, X)) * dxinv;
+    t[4 * n3] = t[4 * n3] + (u0 - U(0, 0, 1, Z)) * dzinv + (w0 - W(1, 0, 0, X)) * dxinv;
+    t[5 * n3] = t[5 * n3] + (v0 - V(0, 0, 1, Z)) * dzinv + (w0 - W(0, 1, 0, Y)) * dyinv;
+  });
+  launch_flat(d.n3, [=] __device__(int m) {
+    ub[m] = 0.0;
+    vb[m] = 0.0;
+    wb[m] = 0.0;
+  });
+}
+
+// srcs/cal_srcu.f:51-122 reversed (pointwise, densities fixed).
+struct SrcuAdj {
+  double* vel[3];
+  double *adv[3], *advo[3], *prs[3], *vis[3], *fst[3], *fstn[3], *src[3];
+  const double *rho, *rhon;
+};
+
+static void srcu_adj(const Dims d, int jen, double dtinv, const SrcuAdj a) {
+  launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    const int F[3] = {1, d.sx, d.sx * d.sy};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (q == 1 && j > jen) continue;
+      const double r = (a.rho[c] + a.rho[c + F[q]] + a.rhon[c] + a.rhon[c + F[q]]) * 0.25;
+      const double s = a.src[q][c];
+      a.vel[q][c] = a.vel[q][c] + s * dtinv;
+      a.adv[q][c] = a.adv[q][c] - s * 1.5;
+      a.advo[q][c] = a.advo[q][c] + s * 0.5;
+      a.prs[q][c] = a.prs[q][c] - s / r;
+      a.vis[q][c] = a.vis[q][c] + s / r * 0.5;
+      a.fst[q][c] = a.fst[q][c] + s * 0.5;
+      a.fstn[q][c] = a.fstn[q][c] + s * 0.5;
+      a.src[q][c] = 0.0;
+    }
+  });
+}
+
+// Reverse of the WENO5 derivative (srcs/weno5.h:24-60): from the six
+// differences df (dfm5, dfm3, dfm1, dfp1, dfp3, dfp5), the transport
+// velocity and the adjoint rb of the result, the adjoints of the
+// differences (db) and, returned, of the velocity; d|vel|/dvel = 0 at 0.
+__device__ __forceinline__ double weno5_adj(const double* df, double vel00, double rb, double* db) {
+  const double verysmall = 1.0e-80, d1_12 = 1.0 / 12.0;
+  const double dfm5 = df[0], dfm3 = df[1], dfm1 = df[2], dfp1 = df[3], dfp3 = df[4], dfp5 = df[5];
+  const double f[6] = {dfm5 * 2.0 - dfm3 * 7.0 + dfm1 * 11.0, -dfm3 + dfm1 * 5.0 + dfp1 * 2.0,
+                       dfm1 * 2.0 + dfp1 * 5.0 - dfp3, dfp5 * 2.0 - dfp3 * 7.0 + dfp1 * 11.0,
+                       -dfp3 + dfp1 * 5.0 + dfm1 * 2.0, dfp1 * 2.0 + dfm1 * 5.0 - dfm3};
+  double a[6], b[6], sm[6];
+  a[0] = dfm5 - dfm3 * 2.0 + dfm1; b[0] = dfm5 - dfm3 * 4.0 + dfm1 * 3.0;
+  a[1] = dfm3 - dfm1 * 2.0 + dfp1; b[1] = dfm3 - dfp1;
+  a[2] = dfm1 - dfp1 * 2.0 + dfp3; b[2] = dfm1 * 3.0 - dfp1 * 4.0 + dfp3;
+  a[3] = dfp5 - dfp3 * 2.0 + dfp1; b[3] = dfp5 - dfp3 * 4.0 + dfp1 * 3.0;
+  a[4] = dfp3 - dfp1 * 2.0 + dfm1; b[4] = dfp3 - dfm1;
+  a[5] = dfp1 - dfm1 * 2.0 + dfm3; b[5] = dfp1 * 3.0 - dfm1 * 4.0 + dfm3;
+#pragma unroll
+  for (int t = 0; t < 6; t++) sm[t] = a[t] * a[t] * 13.0 + b[t] * b[t] * 3.0;
+  const double velab = fabs(vel00);
+  const double vfac[2] = {vel00 + velab, vel00 - velab};
+  const double sg = vel00 > 0.0 ? 1.0 : (vel00 < 0.0 ? -1.0 : 0.0);
+  double velb = 0.0, smb[6], cb[6];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const double* s = sm + 3 * side;
+    const double* fc = f + 3 * side;
+    const double P1 = s[1] * s[2], P2 = s[0] * s[2], P3 = s[0] * s[1];
+    const double am1 = P1 * P1, am2 = P2 * P2 * 6.0, am3 = P3 * P3 * 3.0;
+    const double deninv = 1.0 / (am1 + am2 + am3 + verysmall);
+    const double w1 = am1 * deninv, w3 = am3 * deninv, w2 = 1.0 - w1 - w3;
+    const double R = w1 * fc[0] + w2 * fc[1] + w3 * fc[2];
+    velb = velb + R * d1_12 * rb * (side ? (1.0 - sg) : (1.0 + sg));
+    const double Rb = vfac[side] * d1_12 * rb;
+    cb[3 * side] = Rb * w1;
+    cb[3 * side + 1] = Rb * w2;
+    cb[3 * side + 2] = Rb * w3;
+    const double w1b = Rb * (fc[0] - fc[1]), w3b = Rb * (fc[2] - fc[1]);
+    const double denb = -(w1b * am1 + w3b * am3) * deninv * deninv;
+    const double am1b = w1b * deninv + denb, am2b = denb, am3b = w3b * deninv + denb;
+    const double P1b = 2.0 * P1 * am1b, P2b = 12.0 * P2 * am2b, P3b = 6.0 * P3 * am3b;
+    smb[3 * side] = P2b * s[2] + P3b * s[1];
+    smb[3 * side + 1] = P1b * s[2] + P3b * s[0];
+    smb[3 * side + 2] = P1b * s[1] + P2b * s[0];
+  }
+  // candidate stencils (dfm_k, dfp_k) and smoothness indicators
+  db[0] = cb[0] * 2.0;
+  db[1] = -cb[0] * 7.0 - cb[1] - cb[5];
+  db[2] = cb[0] * 11.0 + cb[1] * 5.0 + cb[2] * 2.0 + cb[4] * 2.0 + cb[5] * 5.0;
+  db[3] = cb[1] * 2.0 + cb[2] * 5.0 + cb[3] * 11.0 + cb[4] * 5.0 + cb[5] * 2.0;
+  db[4] = -cb[2] - cb[3] * 7.0 - cb[4];
+  db[5] = cb[3] * 2.0;
+  double ab[6], bb[6];
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    ab[t] = 26.0 * a[t] * smb[t];
+    bb[t] = 6.0 * b[t] * smb[t];
+  }
+  // d a_t / d df and d b_t / d df, row by row as a[t], b[t] above
+  db[0] += ab[0] + bb[0];
+  db[1] += -2.0 * ab[0] - 4.0 * bb[0] + ab[1] + bb[1] + ab[5] + bb[5];
+  db[2] += ab[0] + 3.0 * bb[0] - 2.0 * ab[1] + ab[2] + 3.0 * bb[2] + ab[4] - bb[4] - 2.0 * ab[5] -
+           4.0 * bb[5];
+  db[3] += ab[1] - bb[1] - 2.0 * ab[2] - 4.0 * bb[2] + ab[3] + 3.0 * bb[3] - 2.0 * ab[4] + ab[5] +
+           3.0 * bb[5];
+  db[4] += ab[2] + bb[2] - 2.0 * ab[3] - 4.0 * bb[3] + ab[4] + bb[4];
+  db[5] += ab[3] + bb[3];
+  return velb;
+}
+
+// srcs/cal_advu_weno5.f:1-257 reversed for component q and direction dd:
+// the first launch forms, at every interior cell, the adjoints of the six
+// differences and of the transport velocity (scratch P[0..5], V); the
+// second gathers them into the component's adjoint (each difference reads
+// two cells) and into the transporting component's adjoint (the velocity
+// averages four values).
+static void weno_adj_dir(const Dims d, int q, int dd, double dinv, const double* const* vel,
+                         double* const* velb, const double* advb, double* const* P) {
+  const int E[3] = {1, d.sx, d.sx * d.sy};
+  const int st = E[dd], eq = E[q];
+  const double* f = vel[q];
+  const double* tv = vel[dd];
+  double *P0 = P[0], *P1 = P[1], *P2 = P[2], *P3 = P[3], *P4 = P[4], *P5 = P[5], *V = P[6];
+  launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    const double cm = (tv[c - st] + tv[c - st + eq]) * 0.5, cp = (tv[c] + tv[c + eq]) * 0.5;
+    const double vel00 = (cm + cp) * 0.5;
+    double df[6], db[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) df[m] = -f[c + (m - 3) * st] + f[c + (m - 2) * st];
+    const double vb = weno5_adj(df, vel00, advb[c] * dinv, db);
+    P0[c] = db[0]; P1[c] = db[1]; P2[c] = db[2]; P3[c] = db[3]; P4[c] = db[4]; P5[c] = db[5];
+    V[c] = vb * 0.25;
+  });
+  double* fb = velb[q];
+  double* tb = velb[dd];
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const int di[3] = {dd == 0, dd == 1, dd == 2}, qi[3] = {q == 0, q == 1, q == 2};
+  const int sdi = di[0], sdj = di[1], sdk = di[2], sqi = qi[0], sqj = qi[1], sqk = qi[2];
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int y = d.ix(i, j, k);
+    // interior cell at y + s*(direction) + t*(component q's direction)
+    auto in_at = [=](int s, int t) {
+      const int xi = i + s * sdi + t * sqi, xj = j + s * sdj + t * sqj, xk = k + s * sdk + t * sqk;
+      return xi >= 1 && xi <= ni && xj >= 1 && xj <= nj && xk >= 1 && xk <= nk;
+    };
+    const double* Pm[6] = {P0, P1, P2, P3, P4, P5};
+    double g = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      // + in the difference of the cell at y - (m-2) st, - in that at y - (m-3) st
+      if (in_at(2 - m, 0)) g = g + Pm[m][y + (2 - m) * st];
+      if (in_at(3 - m, 0)) g = g - Pm[m][y + (3 - m) * st];
+    }
+    // transport velocity of the cells y + e_d, y + e_d - e_q, y, y - e_q
+    double t = 0.0;
+    if (in_at(1, 0)) t = t + V[y + st];
+    if (in_at(1, -1)) t = t + V[y + st - eq];
+    if (in_at(0, 0)) t = t + V[y];
+    if (in_at(0, -1)) t = t + V[y - eq];
+    if (fb == tb) {
+      fb[y] = fb[y] + (g + t);
+    } else {
+      fb[y] = fb[y] + g;
+      tb[y] = tb[y] + t;
+    }
+  });
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -473,6 +777,144 @@ void solu_sor4_b_(const int* ipara, const int* ID, const int* nID, const int* nd
         if (q == 2) colour_adj<2>(onfly, d, a, col);
       }
   add_sum(wacc, nw, uwallb);
+  c.finish();
+}
+
+// ---------------------------------------------------------------------------
+// Explicit momentum terms (adjoint entry points).
+// ---------------------------------------------------------------------------
+
+// Adjoint of calphat_ (srcs/calphat.f:16-22).
+void calphat_b_(const int* ni, const int* nj, const int* nk, double* pb, double* pob, double* phatb) {
+  adjoint_fp64_only("calphat_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calphat_b_");
+  double* a = c.io(pb, d.n3, "pb");
+  double* b = c.io(pob, d.n3, "pob");
+  double* h = c.io(phatb, d.n3, "phatb");
+  calphat_adj(d, a, b, h);
+  c.finish();
+}
+
+// Adjoint of cal_grad_prs_ (srcs/cal_grad_prs.f:24-37).
+void cal_grad_prs_b_(const int* ni, const int* nj, const int* nk, const double* dxinv,
+                     const double* dyinv, const double* dzinv, double* qb, double* qx_ub,
+                     double* qy_vb, double* qz_wb) {
+  adjoint_fp64_only("cal_grad_prs_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_grad_prs_b_");
+  double* q = c.io(qb, d.n3, "qb");
+  double* gx = c.io(qx_ub, d.n3, "qx_ub");
+  double* gy = c.io(qy_vb, d.n3, "qy_vb");
+  double* gz = c.io(qz_wb, d.n3, "qz_wb");
+  grad_prs_adj(d, *dxinv, *dyinv, *dzinv, q, gx, gy, gz);
+  c.finish();
+}
+
+// Adjoint of calsij_ (srcs/calsij.f:28-139); skb holds the six components.
+void calsij_b_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
+               const double* dzinv, double* ukb, double* vkb, double* wkb, double* skb) {
+  adjoint_fp64_only("calsij_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calsij_b_");
+  double* u = c.io(ukb, d.n3, "ukb");
+  double* v = c.io(vkb, d.n3, "vkb");
+  double* w = c.io(wkb, d.n3, "wkb");
+  double* s = c.io(skb, 6 * (size_t)d.n3, "skb");
+  calsij_adj(d, *dxinv, *dyinv, *dzinv, u, v, w, s);
+  c.finish();
+}
+
+// Adjoint of cal_arith_tau_ (srcs/cal_arith_tau.f:21-118), viscosity fixed.
+void cal_arith_tau_b_(const int* ni, const int* nj, const int* nk, double* sb, const double* rmu,
+                      double* taub) {
+  adjoint_fp64_only("cal_arith_tau_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_arith_tau_b_");
+  double* s = c.io(sb, 6 * (size_t)d.n3, "sb");
+  const double* m = c.in(rmu, d.n3, "rmu");
+  double* t = c.io(taub, 6 * (size_t)d.n3, "taub");
+  arith_tau_adj(d, s, m, t);
+  c.finish();
+}
+
+// Adjoint of cal_div_tensor_ (srcs/cal_div_tensor.f:20-86).
+void cal_div_tensor_b_(const int* ni, const int* nj, const int* nk, const double* dxinv,
+                       const double* dyinv, const double* dzinv, double* taukb, double* ukb,
+                       double* vkb, double* wkb) {
+  adjoint_fp64_only("cal_div_tensor_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_div_tensor_b_");
+  double* t = c.io(taukb, 6 * (size_t)d.n3, "taukb");
+  double* u = c.io(ukb, d.n3, "ukb");
+  double* v = c.io(vkb, d.n3, "vkb");
+  double* w = c.io(wkb, d.n3, "wkb");
+  div_tensor_adj(d, *dxinv, *dyinv, *dzinv, t, u, v, w);
+  c.finish();
+}
+
+// Adjoint of cal_srcu_ (srcs/cal_srcu.f:51-122), densities fixed; the
+// gravity terms are constants of the map.
+void cal_srcu_b_(const int* nID, const int* ni, const int* nj, const int* nk, const double* rho_av,
+                 const double* dt, const double* grvb, const double* grvp, double* ub, double* vb,
+                 double* wb, const double* rho, const double* rhon, double* adv_ub, double* adv_vb,
+                 double* adv_wb, double* adv_uob, double* adv_vob, double* adv_wob, double* prs_ub,
+                 double* prs_vb, double* prs_wb, double* vis_ub, double* vis_vb, double* vis_wb,
+                 double* fst_ub, double* fst_vb, double* fst_wb, double* fst_unb, double* fst_vnb,
+                 double* fst_wnb, double* src_ub, double* src_vb, double* src_wb) {
+  (void)rho_av; (void)grvb; (void)grvp;
+  adjoint_fp64_only("cal_srcu_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int jen = nID[kYPlus] < 0 ? d.nj - 1 : d.nj;
+  Call c("cal_srcu_b_");
+  SrcuAdj a;
+  double* hv[3] = {ub, vb, wb};
+  double* ha[3] = {adv_ub, adv_vb, adv_wb};
+  double* hao[3] = {adv_uob, adv_vob, adv_wob};
+  double* hp[3] = {prs_ub, prs_vb, prs_wb};
+  double* hvis[3] = {vis_ub, vis_vb, vis_wb};
+  double* hf[3] = {fst_ub, fst_vb, fst_wb};
+  double* hfn[3] = {fst_unb, fst_vnb, fst_wnb};
+  double* hs[3] = {src_ub, src_vb, src_wb};
+  for (int q = 0; q < 3; q++) {
+    a.vel[q] = c.io(hv[q], d.n3, "velocity adjoint");
+    a.adv[q] = c.io(ha[q], d.n3, "adv adjoint");
+    a.advo[q] = c.io(hao[q], d.n3, "adv_o adjoint");
+    a.prs[q] = c.io(hp[q], d.n3, "prs adjoint");
+    a.vis[q] = c.io(hvis[q], d.n3, "vis adjoint");
+    a.fst[q] = c.io(hf[q], d.n3, "fst adjoint");
+    a.fstn[q] = c.io(hfn[q], d.n3, "fst_n adjoint");
+    a.src[q] = c.io(hs[q], d.n3, "src adjoint");
+  }
+  a.rho = c.in(rho, d.n3, "rho");
+  a.rhon = c.in(rhon, d.n3, "rhon");
+  srcu_adj(d, jen, 1.0 / *dt, a);
+  c.finish();
+}
+
+// Adjoint of cal_advu_weno5_ (srcs/cal_advu_weno5.f:1-257).  Not linear:
+// u, v, w are the forward's inputs, each followed by its adjoint
+// (incremented); the adv adjoints are consumed.
+void cal_advu_weno5_b_(const int* ni, const int* nj, const int* nk, const double* dxinv,
+                       const double* dyinv, const double* dzinv, const double* u, double* ub,
+                       const double* v, double* vb, const double* w, double* wb, double* adv_ub,
+                       double* adv_vb, double* adv_wb) {
+  adjoint_fp64_only("cal_advu_weno5_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_advu_weno5_b_");
+  const double* vel[3] = {c.in(u, d.n3, "u"), c.in(v, d.n3, "v"), c.in(w, d.n3, "w")};
+  double* velb[3] = {c.io(ub, d.n3, "ub"), c.io(vb, d.n3, "vb"), c.io(wb, d.n3, "wb")};
+  double* advb[3] = {c.io(adv_ub, d.n3, "adv_ub"), c.io(adv_vb, d.n3, "adv_vb"),
+                     c.io(adv_wb, d.n3, "adv_wb")};
+  double* P[7];
+  for (int m = 0; m < 7; m++) P[m] = static_cast<double*>(scratch(26 + m, sizeof(double) * (size_t)d.n3));
+  const double dinv[3] = {*dxinv, *dyinv, *dzinv};
+  for (int q = 2; q >= 0; q--)
+    for (int dd = 2; dd >= 0; dd--) weno_adj_dir(d, q, dd, dinv[dd], vel, velb, advb[q], P);
+  for (int q = 0; q < 3; q++) {
+    double* a = advb[q];
+    launch_flat(d.n3, [=] __device__(int m) { a[m] = 0.0; });
+  }
   c.finish();
 }
 

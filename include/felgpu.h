@@ -614,6 +614,52 @@ void solu_sor4_b_(const int *ipara, const int *ID, const int *nID, const int *nd
                   double *uwallb, const double *dy, const double *l1, const double *l2,
                   const double *phi, const int *lp);
 
+/* srcs/calphat.f:16-22 transposed: pb += 2 phatb, pob -= phatb, phatb = 0
+ * on the box the forward writes (i >= -1). */
+void calphat_b_(const int *ni, const int *nj, const int *nk, double *pb, double *pob, double *phatb);
+
+/* srcs/cal_grad_prs.f:24-37 transposed: qb gathers the face-gradient
+ * adjoints; they are consumed on [-2,n+2]^3. */
+void cal_grad_prs_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                     const double *dyinv, const double *dzinv, double *qb, double *qx_ub,
+                     double *qy_vb, double *qz_wb);
+
+/* srcs/calsij.f:28-139 transposed: ukb, vkb, wkb gather the adjoints of
+ * the six strain-rate components skb(..,1..6), consumed on their boxes. */
+void calsij_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+               const double *dyinv, const double *dzinv, double *ukb, double *vkb, double *wkb,
+               double *skb);
+
+/* srcs/cal_arith_tau.f:21-118 transposed, viscosity rmu fixed. */
+void cal_arith_tau_b_(const int *ni, const int *nj, const int *nk, double *sb, const double *rmu,
+                      double *taub);
+
+/* srcs/cal_div_tensor.f:20-86 transposed: taukb gathers; ukb, vkb, wkb
+ * consumed (the forward overwrites them whole). */
+void cal_div_tensor_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                       const double *dyinv, const double *dzinv, double *taukb, double *ukb,
+                       double *vkb, double *wkb);
+
+/* srcs/cal_srcu.f:51-122 transposed, densities fixed: every array term's
+ * adjoint incremented from src_*b (interior, v to jen), which is consumed.
+ * The gravity terms are constants of the map. */
+void cal_srcu_b_(const int *nID, const int *ni, const int *nj, const int *nk, const double *rho_av,
+                 const double *dt, const double *grvb, const double *grvp, double *ub, double *vb,
+                 double *wb, const double *rho, const double *rhon, double *adv_ub, double *adv_vb,
+                 double *adv_wb, double *adv_uob, double *adv_vob, double *adv_wob, double *prs_ub,
+                 double *prs_vb, double *prs_wb, double *vis_ub, double *vis_vb, double *vis_wb,
+                 double *fst_ub, double *fst_vb, double *fst_wb, double *fst_unb, double *fst_vnb,
+                 double *fst_wnb, double *src_ub, double *src_vb, double *src_wb);
+
+/* srcs/cal_advu_weno5.f:1-257 reversed at the state (u, v, w), which is not
+ * a linear map: each velocity is followed by its adjoint (incremented); the
+ * adv adjoints are consumed.  The derivative of the upwind switch |vel| is
+ * taken as 0 at vel = 0. */
+void cal_advu_weno5_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                       const double *dyinv, const double *dzinv, const double *u, double *ub,
+                       const double *v, double *vb, const double *w, double *wb, double *adv_ub,
+                       double *adv_vb, double *adv_wb);
+
 /* How the last solu_sor4_b_ call obtained its coefficients: 1 formed on the
  * fly from the viscosity, 0 read from the 45 arrays, -1 before any call. */
 int felgpu_sor_b_onfly_(void);

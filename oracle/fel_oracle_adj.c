@@ -524,3 +524,363 @@ void orc_solu_sor4_b(const int *ipara, const int *ID, const int *nID, const int 
   sor4_b_impl(nID, ni, nj, nk, dt, rho, rhon, coef, src_ub, src_vb, src_wb, unb, vnb, wnb, uwallb,
               dy, l1, l2, phi, lp);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Explicit momentum terms                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* srcs/calphat.f:16-22: phat = 2p - po on i in [-1,ni+3], j, k in
+ * [-2,n+3].  Transpose: p_b += 2 phat_b, po_b -= phat_b, phat_b = 0 there. */
+void orc_calphat_b(const int *ni_, const int *nj_, const int *nk_, double *pb, double *pob,
+                   double *phatb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  for (int k = nk + 3; k >= -2; k--)
+    for (int j = nj + 3; j >= -2; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        pb[c] += phatb[c] * 2.0;
+        pob[c] -= phatb[c];
+        phatb[c] = 0.0;
+      }
+}
+
+/* srcs/cal_grad_prs.f:24-37: q_x(c) = (q(c+x) - q(c)) dxinv (likewise y, z)
+ * on [-2,n+2]^3. */
+void orc_cal_grad_prs_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                        const double *dyinv_, const double *dzinv_, double *qb, double *qx_ub,
+                        double *qy_vb, double *qz_wb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        qb[c] -= qx_ub[c] * dxinv + qy_vb[c] * dyinv + qz_wb[c] * dzinv;
+        qb[OIX(i + 1, j, k)] += qx_ub[c] * dxinv;
+        qb[OIX(i, j + 1, k)] += qy_vb[c] * dyinv;
+        qb[OIX(i, j, k + 1)] += qz_wb[c] * dzinv;
+        qx_ub[c] = 0.0;
+        qy_vb[c] = 0.0;
+        qz_wb[c] = 0.0;
+      }
+}
+
+/* srcs/calsij.f:28-139: the six strain-rate components on their boxes
+ * (orc_calsij), transposed component by component in reverse order. */
+void orc_calsij_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                  const double *dyinv_, const double *dzinv_, double *ukb, double *vkb,
+                  double *wkb, double *skb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  double *s1 = skb, *s2 = skb + n3, *s3 = skb + 2 * n3, *s4 = skb + 3 * n3, *s5 = skb + 4 * n3,
+         *s6 = skb + 5 * n3;
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 3; i >= -2; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double h = s6[c] * 0.5;
+        vkb[c] -= h * dzinv;
+        vkb[OIX(i, j, k + 1)] += h * dzinv;
+        wkb[c] -= h * dyinv;
+        wkb[OIX(i, j + 1, k)] += h * dyinv;
+        s6[c] = 0.0;
+      }
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 3; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double h = s5[c] * 0.5;
+        ukb[c] -= h * dzinv;
+        ukb[OIX(i, j, k + 1)] += h * dzinv;
+        wkb[c] -= h * dxinv;
+        wkb[OIX(i + 1, j, k)] += h * dxinv;
+        s5[c] = 0.0;
+      }
+  for (int k = nk + 3; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double h = s4[c] * 0.5;
+        ukb[c] -= h * dyinv;
+        ukb[OIX(i, j + 1, k)] += h * dyinv;
+        vkb[c] -= h * dxinv;
+        vkb[OIX(i + 1, j, k)] += h * dxinv;
+        s4[c] = 0.0;
+      }
+  for (int k = nk + 3; k >= -1; k--)
+    for (int j = nj + 3; j >= -1; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        wkb[OIX(i, j, k - 1)] -= s3[c] * dzinv;
+        wkb[c] += s3[c] * dzinv;
+        s3[c] = 0.0;
+      }
+  for (int k = nk + 3; k >= -2; k--)
+    for (int j = nj + 3; j >= -1; j--)
+      for (int i = ni + 3; i >= -2; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        vkb[OIX(i, j - 1, k)] -= s2[c] * dyinv;
+        vkb[c] += s2[c] * dyinv;
+        s2[c] = 0.0;
+      }
+  for (int k = nk + 3; k >= -2; k--)
+    for (int j = nj + 3; j >= -2; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        ukb[OIX(i - 1, j, k)] -= s1[c] * dxinv;
+        ukb[c] += s1[c] * dxinv;
+        s1[c] = 0.0;
+      }
+}
+
+/* srcs/cal_arith_tau.f:21-118: tau_l = 2 mu s_l (l = 1..3, every cell),
+ * tau_l = 2 mu_edge s_l on the shear boxes (l = 4..6), mu fixed. */
+void orc_cal_arith_tau_b(const int *ni_, const int *nj_, const int *nk_, double *sb,
+                         const double *rmu, double *taub) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 3; i >= -2; i--) {
+        const double m = +0.25 * rmu[OIX(i, j, k)] + 0.25 * rmu[OIX(i, j + 1, k)] +
+                         0.25 * rmu[OIX(i, j, k + 1)] + 0.25 * rmu[OIX(i, j + 1, k + 1)];
+        sb[5 * n3 + OIX(i, j, k)] += taub[5 * n3 + OIX(i, j, k)] * m * 2.0;
+        taub[5 * n3 + OIX(i, j, k)] = 0.0;
+      }
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 3; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const double m = +0.25 * rmu[OIX(i, j, k)] + 0.25 * rmu[OIX(i + 1, j, k)] +
+                         0.25 * rmu[OIX(i, j, k + 1)] + 0.25 * rmu[OIX(i + 1, j, k + 1)];
+        sb[4 * n3 + OIX(i, j, k)] += taub[4 * n3 + OIX(i, j, k)] * m * 2.0;
+        taub[4 * n3 + OIX(i, j, k)] = 0.0;
+      }
+  for (int k = nk + 3; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const double m = +0.25 * rmu[OIX(i, j, k)] + 0.25 * rmu[OIX(i + 1, j, k)] +
+                         0.25 * rmu[OIX(i, j + 1, k)] + 0.25 * rmu[OIX(i + 1, j + 1, k)];
+        sb[3 * n3 + OIX(i, j, k)] += taub[3 * n3 + OIX(i, j, k)] * m * 2.0;
+        taub[3 * n3 + OIX(i, j, k)] = 0.0;
+      }
+  for (int l = 2; l >= 0; l--)
+    for (ptrdiff_t m = n3 - 1; m >= 0; m--) {
+      sb[l * n3 + m] += taub[l * n3 + m] * rmu[m] * 2.0;
+      taub[l * n3 + m] = 0.0;
+    }
+}
+
+/* srcs/cal_div_tensor.f:20-86: vis = div(tau) at the u, v, w points on
+ * their boxes, zero elsewhere (the arrays are overwritten whole). */
+void orc_cal_div_tensor_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                          const double *dyinv_, const double *dzinv_, double *taukb, double *ukb,
+                          double *vkb, double *wkb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  double *t1 = taukb, *t2 = taukb + n3, *t3 = taukb + 2 * n3, *t4 = taukb + 3 * n3,
+         *t5 = taukb + 4 * n3, *t6 = taukb + 5 * n3;
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 3; j >= -1; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double g = wkb[c];
+        t5[OIX(i - 1, j, k)] -= g * dxinv;
+        t5[c] += g * dxinv;
+        t6[OIX(i, j - 1, k)] -= g * dyinv;
+        t6[c] += g * dyinv;
+        t3[c] -= g * dzinv;
+        t3[OIX(i, j, k + 1)] += g * dzinv;
+      }
+  for (int k = nk + 3; k >= -1; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double g = vkb[c];
+        t4[OIX(i - 1, j, k)] -= g * dxinv;
+        t4[c] += g * dxinv;
+        t2[c] -= g * dyinv;
+        t2[OIX(i, j + 1, k)] += g * dyinv;
+        t6[OIX(i, j, k - 1)] -= g * dzinv;
+        t6[c] += g * dzinv;
+      }
+  for (int k = nk + 3; k >= -1; k--)
+    for (int j = nj + 3; j >= -1; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double g = ukb[c];
+        t1[c] -= g * dxinv;
+        t1[OIX(i + 1, j, k)] += g * dxinv;
+        t4[OIX(i, j - 1, k)] -= g * dyinv;
+        t4[c] += g * dyinv;
+        t5[OIX(i, j, k - 1)] -= g * dzinv;
+        t5[c] += g * dzinv;
+      }
+  memset(ukb, 0, sizeof(double) * n3);
+  memset(vkb, 0, sizeof(double) * n3);
+  memset(wkb, 0, sizeof(double) * n3);
+}
+
+/* srcs/cal_srcu.f:51-122: src = u/dt - 1.5 adv + 0.5 adv_o - prs/r00 +
+ * 0.5 vis/r00 + 0.5 fst + 0.5 fst_n (+ gravity) on the interior (v to jen),
+ * densities fixed.  Gravity is a constant of the map and has no adjoint. */
+void orc_cal_srcu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_,
+                    const double *rho_av, const double *dt_, const double *grvb_,
+                    const double *grvp_, double *ub, double *vb, double *wb, const double *rho,
+                    const double *rhon, double *adv_ub, double *adv_vb, double *adv_wb,
+                    double *adv_uob, double *adv_vob, double *adv_wob, double *prs_ub,
+                    double *prs_vb, double *prs_wb, double *vis_ub, double *vis_vb,
+                    double *vis_wb, double *fst_ub, double *fst_vb, double *fst_wb,
+                    double *fst_unb, double *fst_vnb, double *fst_wnb, double *src_ub,
+                    double *src_vb, double *src_wb) {
+  (void)rho_av; (void)grvb_; (void)grvp_;
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const int jen = (nID[ORC_Y_PLUS - 1] < 0) ? nj - 1 : nj;
+  const double dtinv = 1.0 / (*dt_);
+  const ptrdiff_t F[3] = {1, sx, sx * sy};
+  double *vel[3] = {ub, vb, wb}, *adv[3] = {adv_ub, adv_vb, adv_wb},
+         *advo[3] = {adv_uob, adv_vob, adv_wob}, *prs[3] = {prs_ub, prs_vb, prs_wb},
+         *vis[3] = {vis_ub, vis_vb, vis_wb}, *fst[3] = {fst_ub, fst_vb, fst_wb},
+         *fstn[3] = {fst_unb, fst_vnb, fst_wnb}, *src[3] = {src_ub, src_vb, src_wb};
+  for (int q = 2; q >= 0; q--) {
+    const int jend = q == 1 ? jen : nj;
+    for (int k = nk; k >= 1; k--)
+      for (int j = jend; j >= 1; j--)
+        for (int i = ni; i >= 1; i--) {
+          const ptrdiff_t c = OIX(i, j, k);
+          const double r = (rho[c] + rho[c + F[q]] + rhon[c] + rhon[c + F[q]]) * 0.25;
+          const double s = src[q][c];
+          vel[q][c] += s * dtinv;
+          adv[q][c] -= s * 1.5;
+          advo[q][c] += s * 0.5;
+          prs[q][c] -= s / r;
+          vis[q][c] += s / r * 0.5;
+          fst[q][c] += s * 0.5;
+          fstn[q][c] += s * 0.5;
+          src[q][c] = 0.0;
+        }
+  }
+}
+
+/* Reverse of weno5 (srcs/weno5.h:24-60, orc weno5 in fel_oracle.c): given the
+ * six differences, the transport velocity and the adjoint rb of the result,
+ * adds the adjoints of the differences to db[6] (dfm5, dfm3, dfm1, dfp1,
+ * dfp3, dfp5) and returns the adjoint of the velocity.  The derivative of
+ * |vel| at vel = 0 is taken as 0. */
+static double weno5_b(const double *df, double vel00, double rb, double *db) {
+  const double verysmall = 1.0e-80, d1_12 = 1.0 / 12.0;
+  const double dfm5 = df[0], dfm3 = df[1], dfm1 = df[2], dfp1 = df[3], dfp3 = df[4], dfp5 = df[5];
+  /* forward */
+  const double m1 = dfm5 * 2.0 - dfm3 * 7.0 + dfm1 * 11.0, m2 = -dfm3 + dfm1 * 5.0 + dfp1 * 2.0,
+               m3 = dfm1 * 2.0 + dfp1 * 5.0 - dfp3;
+  const double p1 = dfp5 * 2.0 - dfp3 * 7.0 + dfp1 * 11.0, p2 = -dfp3 + dfp1 * 5.0 + dfm1 * 2.0,
+               p3 = dfp1 * 2.0 + dfm1 * 5.0 - dfm3;
+  /* smoothness indicators: 13 a^2 + 3 b^2 with (a, b) per stencil */
+  double a[6], b[6], sm[6];
+  a[0] = dfm5 - dfm3 * 2.0 + dfm1; b[0] = dfm5 - dfm3 * 4.0 + dfm1 * 3.0;
+  a[1] = dfm3 - dfm1 * 2.0 + dfp1; b[1] = dfm3 - dfp1;
+  a[2] = dfm1 - dfp1 * 2.0 + dfp3; b[2] = dfm1 * 3.0 - dfp1 * 4.0 + dfp3;
+  a[3] = dfp5 - dfp3 * 2.0 + dfp1; b[3] = dfp5 - dfp3 * 4.0 + dfp1 * 3.0;
+  a[4] = dfp3 - dfp1 * 2.0 + dfm1; b[4] = dfp3 - dfm1;
+  a[5] = dfp1 - dfm1 * 2.0 + dfm3; b[5] = dfp1 * 3.0 - dfm1 * 4.0 + dfm3;
+  for (int t = 0; t < 6; t++) sm[t] = a[t] * a[t] * 13.0 + b[t] * b[t] * 3.0;
+  double smb[6] = {0, 0, 0, 0, 0, 0};
+  const double velab = fabs(vel00);
+  const double velm1 = vel00 + velab, velp1 = vel00 - velab;
+  const double sg = vel00 > 0.0 ? 1.0 : (vel00 < 0.0 ? -1.0 : 0.0);
+  double velb = 0.0;
+  double dm[3], dp[3];
+  for (int side = 0; side < 2; side++) {
+    const double *s = sm + 3 * side;
+    const double f1 = side ? p1 : m1, f2 = side ? p2 : m2, f3 = side ? p3 : m3;
+    const double P1 = s[1] * s[2], P2 = s[0] * s[2], P3 = s[0] * s[1];
+    const double am1 = P1 * P1, am2 = P2 * P2 * 6.0, am3 = P3 * P3 * 3.0;
+    const double deninv = 1.0 / (am1 + am2 + am3 + verysmall);
+    const double w1 = am1 * deninv, w3 = am3 * deninv, w2 = 1.0 - w1 - w3;
+    const double R = w1 * f1 + w2 * f2 + w3 * f3;
+    const double vfac = side ? velp1 : velm1;
+    velb += R * d1_12 * rb * (side ? (1.0 - sg) : (1.0 + sg));
+    const double Rb = vfac * d1_12 * rb;
+    double *d = side ? dp : dm;
+    d[0] = Rb * w1; d[1] = Rb * w2; d[2] = Rb * w3;
+    const double w1b = Rb * (f1 - f2), w3b = Rb * (f3 - f2);
+    const double am1b0 = w1b * deninv, am3b0 = w3b * deninv;
+    const double deninvb = w1b * am1 + w3b * am3;
+    const double denb = -deninvb * deninv * deninv;
+    const double am1b = am1b0 + denb, am2b = denb, am3b = am3b0 + denb;
+    const double P1b = 2.0 * P1 * am1b, P2b = 12.0 * P2 * am2b, P3b = 6.0 * P3 * am3b;
+    double *sb = smb + 3 * side;
+    sb[0] += P2b * s[2] + P3b * s[1];
+    sb[1] += P1b * s[2] + P3b * s[0];
+    sb[2] += P1b * s[1] + P2b * s[0];
+  }
+  /* differences from the candidate stencils (dfm_k, dfp_k) */
+  db[0] += dm[0] * 2.0;
+  db[1] += -dm[0] * 7.0 - dm[1] - dp[2];
+  db[2] += dm[0] * 11.0 + dm[1] * 5.0 + dm[2] * 2.0 + dp[1] * 2.0 + dp[2] * 5.0;
+  db[3] += dm[1] * 2.0 + dm[2] * 5.0 + dp[0] * 11.0 + dp[1] * 5.0 + dp[2] * 2.0;
+  db[4] += -dm[2] - dp[0] * 7.0 - dp[1];
+  db[5] += dp[0] * 2.0;
+  /* differences from the smoothness indicators */
+  static const double ca[6][6] = {{1, -2, 1, 0, 0, 0}, {0, 1, -2, 1, 0, 0}, {0, 0, 1, -2, 1, 0},
+                                  {0, 0, 0, 1, -2, 1}, {0, 0, 1, -2, 1, 0}, {0, 1, -2, 1, 0, 0}};
+  static const double cb[6][6] = {{1, -4, 3, 0, 0, 0}, {0, 1, 0, -1, 0, 0}, {0, 0, 3, -4, 1, 0},
+                                  {0, 0, 0, 3, -4, 1}, {0, 0, -1, 0, 1, 0}, {0, 1, -4, 3, 0, 0}};
+  for (int t = 0; t < 6; t++) {
+    const double ab = 26.0 * a[t] * smb[t], bb = 6.0 * b[t] * smb[t];
+    for (int m = 0; m < 6; m++) db[m] += ab * ca[t][m] + bb * cb[t][m];
+  }
+  return velb;
+}
+
+/* srcs/cal_advu_weno5.f:1-257 reversed (orc_cal_advu_weno5): for each
+ * component, cell and direction, the adjoint of the WENO5 derivative
+ * goes to the seven cells of its differences and to the four velocities of
+ * its transport velocity.  Not linear: u, v, w are the forward's inputs;
+ * ub, vb, wb are incremented; the adv adjoints are consumed (the forward
+ * overwrites the whole arrays). */
+void orc_cal_advu_weno5_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                          const double *dyinv_, const double *dzinv_, const double *u,
+                          double *ub, const double *v, double *vb, const double *w, double *wb,
+                          double *adv_ub, double *adv_vb, double *adv_wb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dinv[3] = {*dxinv_, *dyinv_, *dzinv_};
+  ODIMS(ni, nj, nk);
+  const ptrdiff_t E[3] = {1, sx, sx * sy};
+  const double *vel[3] = {u, v, w};
+  double *velb[3] = {ub, vb, wb};
+  double *advb[3] = {adv_ub, adv_vb, adv_wb};
+  for (int q = 2; q >= 0; q--)
+    for (int k = nk; k >= 1; k--)
+      for (int j = nj; j >= 1; j--)
+        for (int i = ni; i >= 1; i--) {
+          const ptrdiff_t c = OIX(i, j, k);
+          const double ab = advb[q][c];
+          advb[q][c] = 0.0;
+          for (int dd = 2; dd >= 0; dd--) {
+            const ptrdiff_t st = E[dd];
+            /* transport velocity: component dd at c - e_d, c - e_d + e_q, c, c + e_q */
+            const double *t = vel[dd];
+            const ptrdiff_t o[4] = {c - st, c - st + E[q], c, c + E[q]};
+            const double cm = (t[o[0]] + t[o[1]]) * 0.5, cp = (t[o[2]] + t[o[3]]) * 0.5;
+            const double vel00 = (cm + cp) * 0.5;
+            const double *f = vel[q];
+            double df[6];
+            for (int m = 0; m < 6; m++) df[m] = -f[c + (m - 3) * st] + f[c + (m - 2) * st];
+            double db[6] = {0, 0, 0, 0, 0, 0};
+            const double vb00 = weno5_b(df, vel00, ab * dinv[dd], db);
+            double *fb = velb[q];
+            for (int m = 0; m < 6; m++) {
+              fb[c + (m - 3) * st] -= db[m];
+              fb[c + (m - 2) * st] += db[m];
+            }
+            for (int m = 0; m < 4; m++) velb[dd][o[m]] += vb00 * 0.25;
+          }
+        }
+  /* the forward also zeroes the ghost cells of adv: consumed everywhere */
+  for (int q = 0; q < 3; q++) memset(advb[q], 0, sizeof(double) * n3);
+}

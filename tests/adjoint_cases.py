@@ -24,8 +24,10 @@ def dot(a: dict, b: dict) -> float:
 
 
 def scale(a: dict, b: dict) -> float:
-    na = np.sqrt(sum(float(np.sum(np.square(a[k]))) for k in a))
-    nb = np.sqrt(sum(float(np.sum(np.square(b[k]))) for k in b))
+    """|a| |b| over the entries that take part in the dot product (keys
+    starting with "_" hold adjoints passed through unchanged)."""
+    na = np.sqrt(sum(float(np.sum(np.square(a[k]))) for k in a if not k.startswith("_")))
+    nb = np.sqrt(sum(float(np.sum(np.square(b[k]))) for k in b if not k.startswith("_")))
     return na * nb
 
 
@@ -435,3 +437,216 @@ def consistent_velocity(g, rng):
         out[k] = q
     out["phat"] = rng.standard_normal(g.shape)
     return out
+
+
+# --------------------------------------------------------------------------
+# Explicit momentum terms
+# --------------------------------------------------------------------------
+
+def grid_head(g: Grid):
+    return [I(g.ni), I(g.nj), I(g.nk)]
+
+
+def dxyz(g: Grid):
+    return [D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+
+
+class Calphat(Case):
+    name = "calphat"
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("p", "po")}
+
+    def out_adjoints(self):
+        return {"phat": self.rng.standard_normal(self.g.shape)}
+
+    def forward(self, call, x):
+        phat = self.g.zeros()
+        call("calphat", *grid_head(self.g), x["p"].copy(), x["po"].copy(), phat)
+        return {"phat": phat}
+
+    def adjoint(self, call, y):
+        g = self.g
+        pb, pob, phb = g.zeros(), g.zeros(), y["phat"].copy()
+        call("calphat_b", *grid_head(g), pb, pob, phb)
+        # the forward leaves i = -2 of phat unwritten: its adjoint passes through
+        return {"p": pb, "po": pob, "_phat_kept": phb}
+
+
+class GradPrs(Case):
+    name = "cal_grad_prs"
+
+    def inputs(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("qx", "qy", "qz")}
+
+    def forward(self, call, x):
+        g = self.g
+        o = [g.zeros() for _ in range(3)]
+        call("cal_grad_prs", *grid_head(g), *dxyz(g), x["q"].copy(), *o)
+        return dict(zip(("qx", "qy", "qz"), o))
+
+    def adjoint(self, call, y):
+        g = self.g
+        qb = g.zeros()
+        o = [y[k].copy() for k in ("qx", "qy", "qz")]
+        call("cal_grad_prs_b", *grid_head(g), *dxyz(g), qb, *o)
+        return {"q": qb, "_kept": o}
+
+
+class Calsij(Case):
+    name = "calsij"
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("u", "v", "w")}
+
+    def out_adjoints(self):
+        return {"s": self.rng.standard_normal((6,) + self.g.shape)}
+
+    def forward(self, call, x):
+        g = self.g
+        s = g.zeros(6)
+        call("calsij", *grid_head(g), *dxyz(g), x["u"].copy(), x["v"].copy(), x["w"].copy(), s)
+        return {"s": s}
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = [g.zeros() for _ in range(3)]
+        s = y["s"].copy()
+        call("calsij_b", *grid_head(g), *dxyz(g), *b, s)
+        return {"u": b[0], "v": b[1], "w": b[2], "_kept": s}
+
+
+class ArithTau(Case):
+    name = "cal_arith_tau"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        self.rmu = np.random.default_rng(seed + 300).uniform(0.01, 2.0, g.shape)
+
+    def inputs(self):
+        return {"s": self.rng.standard_normal((6,) + self.g.shape)}
+
+    def out_adjoints(self):
+        return {"tau": self.rng.standard_normal((6,) + self.g.shape)}
+
+    def forward(self, call, x):
+        g = self.g
+        tau = g.zeros(6)
+        call("cal_arith_tau", *grid_head(g), x["s"].copy(), self.rmu, tau)
+        return {"tau": tau}
+
+    def adjoint(self, call, y):
+        g = self.g
+        sb = g.zeros(6)
+        tb = y["tau"].copy()
+        call("cal_arith_tau_b", *grid_head(g), sb, self.rmu, tb)
+        return {"s": sb, "_kept": tb}
+
+
+class DivTensor(Case):
+    name = "cal_div_tensor"
+
+    def inputs(self):
+        return {"tau": self.rng.standard_normal((6,) + self.g.shape)}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("u", "v", "w")}
+
+    def forward(self, call, x):
+        g = self.g
+        o = [g.zeros() for _ in range(3)]
+        call("cal_div_tensor", *grid_head(g), *dxyz(g), x["tau"].copy(), *o)
+        return dict(zip(("u", "v", "w"), o))
+
+    def adjoint(self, call, y):
+        g = self.g
+        tb = g.zeros(6)
+        o = [y[k].copy() for k in ("u", "v", "w")]
+        call("cal_div_tensor_b", *grid_head(g), *dxyz(g), tb, *o)
+        assert not any(np.any(a) for a in o), "vis adjoints are consumed"
+        return {"tau": tb}
+
+
+SRCU_IN = ("u", "v", "w", "adv_u", "adv_v", "adv_w", "adv_uo", "adv_vo", "adv_wo", "prs_u", "prs_v",
+           "prs_w", "vis_u", "vis_v", "vis_w", "fst_u", "fst_v", "fst_w", "fst_un", "fst_vn", "fst_wn")
+
+
+class Srcu(Case):
+    name = "cal_srcu"
+    dt = 0.003
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 400)
+        self.rho = r.uniform(0.5, 2.0, g.shape)
+        self.rhon = r.uniform(0.5, 2.0, g.shape)
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in SRCU_IN}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("src_u", "src_v", "src_w")}
+
+    def head(self):
+        g = self.g
+        return [WALLS.copy(), *grid_head(g), D(0.9), D(self.dt), D(0.0), D(0.0)]
+
+    def forward(self, call, x):
+        g = self.g
+        a = {k: x[k].copy() for k in SRCU_IN}
+        src = [g.zeros() for _ in range(3)]
+        call("cal_srcu", *self.head(), a["u"], a["v"], a["w"], self.rho, self.rhon,
+             *[a[k] for k in SRCU_IN[3:]], *src)
+        return dict(zip(("src_u", "src_v", "src_w"), src))
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {k: g.zeros() for k in SRCU_IN}
+        src = [y[k].copy() for k in ("src_u", "src_v", "src_w")]
+        call("cal_srcu_b", *self.head(), b["u"], b["v"], b["w"], self.rho, self.rhon,
+             *[b[k] for k in SRCU_IN[3:]], *src)
+        b["_kept"] = src
+        return b
+
+
+class Weno(Case):
+    """cal_advu_weno5 is not linear: its Jacobian at x0 applied to v comes
+    from Richardson-extrapolated central differences of the forward."""
+    name = "cal_advu_weno5"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 600)
+        self.x0 = {k: H.smooth_vel(g, r) + 0.05 * r.standard_normal(g.shape) for k in ("u", "v", "w")}
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("u", "v", "w")}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("adv_u", "adv_v", "adv_w")}
+
+    def value(self, call, x):
+        g = self.g
+        o = [g.zeros() for _ in range(3)]
+        call("cal_advu_weno5", *grid_head(g), *dxyz(g), x["u"].copy(), x["v"].copy(), x["w"].copy(), *o)
+        return dict(zip(("adv_u", "adv_v", "adv_w"), o))
+
+    def forward(self, call, x, h=1e-5):
+        def central(hh):
+            p = self.value(call, {k: self.x0[k] + hh * x[k] for k in x})
+            m = self.value(call, {k: self.x0[k] - hh * x[k] for k in x})
+            return {k: (p[k] - m[k]) / (2 * hh) for k in p}
+        d1, d2 = central(h), central(2 * h)
+        return {k: (4 * d1[k] - d2[k]) / 3 for k in d1}
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {k: g.zeros() for k in ("u", "v", "w")}
+        a = [y[k].copy() for k in ("adv_u", "adv_v", "adv_w")]
+        call("cal_advu_weno5_b", *grid_head(g), *dxyz(g), self.x0["u"], b["u"], self.x0["v"], b["v"],
+             self.x0["w"], b["w"], *a)
+        assert not any(np.any(q) for q in a), "adv adjoints are consumed"
+        return b

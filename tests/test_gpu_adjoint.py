@@ -54,6 +54,28 @@ def test_adjoint_matches_oracle(cls, gi):
     check_case(cls(GRIDS[gi], 51 + gi))
 
 
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.Calphat, A.GradPrs, A.Calsij, A.ArithTau, A.DivTensor, A.Srcu],
+                         ids=lambda c: c.name)
+def test_explicit_term_adjoint_matches_oracle(cls, gi):
+    check_case(cls(GRIDS[gi], 131 + gi))
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_weno5_adjoint_matches_oracle(gi):
+    """Nonlinear: the GPU adjoint at x0 equals the oracle's to rounding, and
+    both match central differences of the GPU forward along v."""
+    case = A.Weno(GRIDS[gi], 141 + gi)
+    print(f"adjoint case cal_advu_weno5 grid {case.g.ni}x{case.g.nj}x{case.g.nk}", flush=True)
+    x, y = case.inputs(), case.out_adjoints()
+    bo = case.adjoint(H.call_oracle, {k: v.copy() for k, v in y.items()})
+    bg = case.adjoint(H.call_gpu, {k: v.copy() for k, v in y.items()})
+    assert rel_diff(bo, bg) <= 1e-12, rel_diff(bo, bg)
+    fx = case.forward(H.call_gpu, x)
+    lhs, rhs = A.dot(fx, y), A.dot(x, bg)
+    assert abs(lhs - rhs) <= 1e-8 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
+
+
 @pytest.mark.parametrize("skew", [False, True], ids=["reference_coefficients", "skewed"])
 def test_tdma2_adjoint(skew):
     check_case(A.Tdma2(GRIDS[1], 61, skew=skew))

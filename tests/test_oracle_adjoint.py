@@ -41,6 +41,22 @@ def test_dot_product(cls, gi):
     dot_check(cls(GRIDS[gi], 11 + gi))
 
 
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.Calphat, A.GradPrs, A.Calsij, A.ArithTau, A.DivTensor, A.Srcu],
+                         ids=lambda c: c.name)
+def test_dot_product_explicit_terms(cls, gi):
+    dot_check(cls(GRIDS[gi], 111 + gi))
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_weno5_adjoint_against_differences(gi):
+    """The WENO5 adjoint at x0 against Richardson central differences of the
+    forward along random directions: <(F(x0+hv)-F(x0-hv))/2h, y> = <v, J^T y>
+    to the differences' accuracy (the identity fails by O(1) for a wrong
+    weight derivative)."""
+    lhs, rhs, sc = dot_check(A.Weno(GRIDS[gi], 121 + gi), rtol=1e-8)
+
+
 @pytest.mark.parametrize("skew", [False, True], ids=["reference_coefficients", "skewed"])
 def test_dot_product_tdma2(skew):
     dot_check(A.Tdma2(GRIDS[0], 21, skew=skew))
