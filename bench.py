@@ -19,7 +19,9 @@ the same for the FP32 mode with FP64 pressure-solve modes;
 `tangent` holds the time of the tangent mode (the same step computing the
 FP64 values and their derivative along the bottom wall's contact angle:
 forward-mode differentiation, one parameter direction per run) relative to
-the FP64 step, and whether its values equal the FP64 run's bit for bit.
+the FP64 step, and whether its values equal the FP64 run's bit for bit;
+`adjoint` the time of the reverse-mode sweep of the velocity-pressure
+update at a fixed phase field against the same update's forward.
 
 The roofline object describes the dominant kernel (the colour-pair
 Gauss-Seidel update of the viscous solve): algorithmic bytes per launch
@@ -255,6 +257,67 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
     return res
 
 
+def measure_adjoint(cfg, warmup, reps, step_ms):
+    """The reverse of the velocity-pressure update at a fixed phase field
+    (felmpf.adjoint.MomentumStep, every array device-resident) against its
+    forward, both on the state and phase of a run after `warmup` steps, with
+    result reuse off so that each pass computes every routine.  The reverse
+    timing includes restoring the four output adjoints (device copies)."""
+    import felmpf
+    from felmpf import _lib, adjoint
+    gpu = _lib.gpu()
+    run = felmpf.GpuRun(cfg)
+    run.steps(warmup)
+    names = {"rho": "rho", "rhon": "rhon", "rmu": "rmu", "rmun": "rmun", "phi": "phi1",
+             "l1": "l1_array", "l2": "l2_array", "fst_u": "sum_fst_u", "fst_v": "sum_fst_v",
+             "fst_w": "sum_fst_w", "fst_un": "sum_fst_un", "fst_vn": "sum_fst_vn",
+             "fst_wn": "sum_fst_wn"}
+    fixed = {k: run.field(v) for k, v in names.items()}
+    state = {k: run.field(k) for k in adjoint.STATE}
+    run.close()
+    reuse_before = int(gpu.felgpu_result_reuse_())
+    gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(0)))
+    step = adjoint.MomentumStep(cfg, fixed, resident=True)
+    try:
+        rng = np.random.default_rng(7)
+        ybuf = {k: step.array(rng.standard_normal(cfg.shape)) for k in adjoint.OUT}
+        g3 = [adjoint.I(cfg.ni), adjoint.I(cfg.nj), adjoint.I(cfg.nk)]
+        step.set_state(state)
+        step.forward_set()
+        gpu.felgpu_sync_()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step.forward_set()
+        gpu.felgpu_sync_()
+        t1 = time.perf_counter()
+        for k in adjoint.OUT:
+            adjoint.gpu_call("cpy", *g3, ybuf[k], step.b[k])
+        step.reverse_set()
+        gpu.felgpu_sync_()
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            for k in adjoint.OUT:
+                adjoint.gpu_call("cpy", *g3, ybuf[k], step.b[k])
+            step.reverse_set()
+        gpu.felgpu_sync_()
+        t3 = time.perf_counter()
+        lp = step.lp
+    finally:
+        step.close()
+        gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(reuse_before)))
+    fwd, rev = (t1 - t0) / reps * 1e3, (t3 - t2) / reps * 1e3
+    return {"scope": "reverse of the velocity-pressure update at a fixed phase field "
+                     "(felmpf.adjoint.MomentumStep: WENO5 of u and uo, pressure predictor, "
+                     "viscous stress, momentum source, Gauss-Seidel solve, projection, ghost "
+                     "refreshes) through the *_b_ entry points",
+            "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
+            "reverse_vs_forward": round(rev / fwd, 3),
+            "reverse_vs_fp64_step": round(rev / step_ms, 3),
+            "sor_sweeps": min(lp + 1, 21), "reps": reps,
+            "held_fixed": "volume fraction and its transport (MTHINC), surface tension, "
+                          "contact line, densities and viscosities (no adjoint yet)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,7 +393,10 @@ def main():
                    "directions": 1,
                    "values_equal_fp64": bool(same),
                    "kind": "forward-mode derivative of the step (FP64 value + derivative per "
-                           "element); a reverse-mode adjoint is not built"}
+                           "element)"}
+    adjoint_leg = None
+    if both:
+        adjoint_leg = measure_adjoint(cfg, args.warmup, 3, main_run["elapsed"] / args.steps * 1e3)
 
     timers = main_run["timers"]
     ms_sor, n_sor = timers["sor_pair"]
@@ -387,6 +453,7 @@ def main():
         "fp32_refine": fp32,
         "fp32_fp64_poisson": fp32_64,
         "tangent": tangent,
+        "adjoint": adjoint_leg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -784,6 +784,21 @@ void solu_sor4_b_(const int* ipara, const int* ID, const int* nID, const int* nd
 // Explicit momentum terms (adjoint entry points).
 // ---------------------------------------------------------------------------
 
+// Adjoint of cpy_ (srcs/cpy.f:14-20): q1b += q2b, q2b = 0.
+void cpy_b_(const int* ni, const int* nj, const int* nk, double* q1b, double* q2b) {
+  adjoint_fp64_only("cpy_b_");
+  if (q1b == q2b) return;
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cpy_b_");
+  double* a = c.io(q1b, d.n3, "q1b");
+  double* b = c.io(q2b, d.n3, "q2b");
+  launch_flat(d.n3, [=] __device__(int m) {
+    a[m] = a[m] + b[m];
+    b[m] = 0.0;
+  });
+  c.finish();
+}
+
 // Adjoint of calphat_ (srcs/calphat.f:16-22).
 void calphat_b_(const int* ni, const int* nj, const int* nk, double* pb, double* pob, double* phatb) {
   adjoint_fp64_only("calphat_b_");

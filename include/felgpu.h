@@ -614,6 +614,9 @@ void solu_sor4_b_(const int *ipara, const int *ID, const int *nID, const int *nd
                   double *uwallb, const double *dy, const double *l1, const double *l2,
                   const double *phi, const int *lp);
 
+/* srcs/cpy.f:14-20 transposed: q1b += q2b, q2b = 0. */
+void cpy_b_(const int *ni, const int *nj, const int *nk, double *q1b, double *q2b);
+
 /* srcs/calphat.f:16-22 transposed: pb += 2 phatb, pob -= phatb, phatb = 0
  * on the box the forward writes (i >= -1). */
 void calphat_b_(const int *ni, const int *nj, const int *nk, double *pb, double *pob, double *phatb);

@@ -884,3 +884,14 @@ void orc_cal_advu_weno5_b(const int *ni_, const int *nj_, const int *nk_, const 
   /* the forward also zeroes the ghost cells of adv: consumed everywhere */
   for (int q = 0; q < 3; q++) memset(advb[q], 0, sizeof(double) * n3);
 }
+
+/* srcs/cpy.f:14-20: q2 = q1 on the padded array.  Transpose: q1_b += q2_b,
+ * q2_b = 0. */
+void orc_cpy_b(const int *ni_, const int *nj_, const int *nk_, double *q1b, double *q2b) {
+  ODIMS(*ni_, *nj_, *nk_);
+  if (q1b == q2b) return;
+  for (ptrdiff_t m = n3 - 1; m >= 0; m--) {
+    q1b[m] += q2b[m];
+    q2b[m] = 0.0;
+  }
+}

@@ -650,3 +650,44 @@ class Weno(Case):
              self.x0["w"], b["w"], *a)
         assert not any(np.any(q) for q in a), "adv adjoints are consumed"
         return b
+
+
+# --------------------------------------------------------------------------
+# The velocity-pressure update at a fixed phase field (felmpf.adjoint)
+# --------------------------------------------------------------------------
+
+def momentum_inputs(cfg, seed: int):
+    """Fixed fields (densities and viscosities of a droplet's two time
+    levels, phi, slip lengths, surface-tension forces) and a flow state with
+    generic (nonzero) transport velocities."""
+    r = np.random.default_rng(seed)
+    grid = Grid(cfg.ni, cfg.nj, cfg.nk, cfg.dx, cfg.dy, cfg.dz)
+    phi = H.smooth_phi(grid, r)
+    phin = np.clip(phi + 0.02 * r.standard_normal(phi.shape), 0.0, 1.0)
+    fixed = {
+        "rho": cfg.rhol * (1 - phi) + cfg.rhog * phi, "rhon": cfg.rhol * (1 - phin) + cfg.rhog * phin,
+        "rmu": cfg.rmul * (1 - phi) + cfg.rmug * phi, "rmun": cfg.rmul * (1 - phin) + cfg.rmug * phin,
+        "phi": phi, "l1": np.full(grid.shape, 0.3 * cfg.dy), "l2": np.full(grid.shape, 0.5 * cfg.dy),
+    }
+    for k in ("fst_u", "fst_v", "fst_w", "fst_un", "fst_vn", "fst_wn"):
+        fixed[k] = 0.1 * r.standard_normal(grid.shape)
+    state = {}
+    for k in ("u", "v", "w", "uo", "vo", "wo"):
+        state[k] = 0.1 * H.smooth_vel(grid, r) + 0.02 * r.standard_normal(grid.shape)
+    for k in ("p", "po"):
+        state[k] = 0.1 * r.standard_normal(grid.shape)
+    return fixed, state
+
+
+def random_like(d: dict, rng) -> dict:
+    return {k: rng.standard_normal(np.shape(v)) for k, v in d.items()}
+
+
+def richardson(fwd, x0: dict, v: dict, h: float):
+    """(4 D(h) - D(2h)) / 3 of the central differences D of fwd along v."""
+    def central(hh):
+        p = fwd({k: x0[k] + hh * v[k] for k in x0})
+        m = fwd({k: x0[k] - hh * v[k] for k in x0})
+        return {k: (p[k] - m[k]) / (2 * hh) for k in p}
+    d1, d2 = central(h), central(2 * h)
+    return {k: (4 * d1[k] - d2[k]) / 3 for k in d1}

@@ -98,7 +98,7 @@ def D(v: float):
 
 
 def _arg(a):
-    if isinstance(a, Ref):
+    if isinstance(a, Ref) or hasattr(a, "obj"):  # boxed scalars (also felmpf.adjoint's)
         return ctypes.byref(a.obj)
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"], "arrays must be contiguous"

@@ -182,3 +182,61 @@ def test_projection_chain_transpose():
     assert rel_diff(bo, bg) <= 1e-11, rel_diff(bo, bg)
     lhs, rhs = A.dot(fx, y), A.dot(x, bg)
     assert abs(lhs - rhs) <= 1e-11 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
+
+
+def momentum_cfg():
+    import dataclasses
+    from felmpf import config
+    return dataclasses.replace(config.sessile_droplet().scaled(32, 16, 16), dt=1e-3, uwall=0.05)
+
+
+@pytest.mark.parametrize("resident", [False, True], ids=["host_arrays", "registered"])
+def test_momentum_step_reverse_matches_oracle(resident):
+    """felmpf.adjoint.MomentumStep on the GPU (host arrays: coefficient
+    arrays; registered: device-resident, coefficients on the fly) against
+    the same reverse sweep through the oracle's routines."""
+    from felmpf import adjoint
+    cfg = momentum_cfg()
+    print(f"adjoint case momentum step resident={resident} 32x16x16", flush=True)
+    fixed, x0 = A.momentum_inputs(cfg, 161)
+    y = A.random_like({k: x0["u"] for k in adjoint.OUT}, np.random.default_rng(162))
+    so = adjoint.MomentumStep(cfg, fixed, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+    fo = so.forward(x0)
+    bo = so.reverse(y)
+    sg = adjoint.MomentumStep(cfg, fixed, resident=resident)
+    try:
+        fg = sg.forward(x0)
+        assert sg.lp == so.lp
+        bg = sg.reverse(y)
+        if resident:
+            assert H.gpu().felgpu_sor_b_onfly_() == 1
+    finally:
+        sg.close()
+    assert rel_diff(fo, fg) <= 1e-12, rel_diff(fo, fg)
+    assert rel_diff(bo, bg) <= 1e-11, rel_diff(bo, bg)
+
+
+def test_momentum_window_reverse_against_differences():
+    """Three steps with the state rotation, reversed on the GPU, against
+    Richardson central differences of the GPU forward."""
+    from felmpf import adjoint
+    cfg = momentum_cfg()
+    print("adjoint case momentum window 3 steps 32x16x16", flush=True)
+    fixed, x0 = A.momentum_inputs(cfg, 171)
+    step = adjoint.MomentumStep(cfg, fixed)
+    win = adjoint.MomentumWindow(step, 3)
+    lps = []
+
+    def fwd(x):
+        out = win.forward(x)
+        lps.append(step.lp)
+        return out
+
+    rng = np.random.default_rng(172)
+    v, y = A.random_like(x0, rng), A.random_like(x0, rng)
+    jv = A.richardson(fwd, x0, v, 1e-5)
+    win.forward(x0)
+    assert all(lp == step.lp for lp in lps), lps
+    bx = win.reverse(y)
+    lhs, rhs = A.dot(jv, y), A.dot(v, {k: bx[k] for k in v})
+    assert abs(lhs - rhs) <= 1e-7 * A.scale(jv, y), (lhs, rhs)

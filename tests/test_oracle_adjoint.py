@@ -86,6 +86,71 @@ def test_projection_chain(consistent):
     assert abs(lhs - rhs) <= 1e-12 * max(A.scale(fx, y), A.scale(x, bx)), (lhs, rhs)
 
 
+def momentum_cfg():
+    """A 16x8x16 droplet case with a moving wall and a step long enough for
+    the Gauss-Seidel solve to take several sweeps."""
+    import dataclasses
+    from felmpf import config
+    return dataclasses.replace(config.sessile_droplet().scaled(16, 8, 16), dt=1e-3, uwall=0.05)
+
+
+@pytest.mark.parametrize("steps", [1, 2])
+def test_momentum_window_against_differences(steps):
+    """The reverse of `steps` velocity-pressure updates at a fixed phase
+    (felmpf.adjoint.MomentumWindow through the oracle's routines) against
+    Richardson central differences of the same forward along a random
+    direction of the state; the Gauss-Seidel sweep counts of the perturbed
+    runs equal the unperturbed ones (the adjoint freezes them)."""
+    from felmpf import adjoint
+    cfg = momentum_cfg()
+    fixed, x0 = A.momentum_inputs(cfg, 151)
+    step = adjoint.MomentumStep(cfg, fixed, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+    win = adjoint.MomentumWindow(step, steps)
+    win.forward(x0)
+    lps = []
+
+    def fwd(x):
+        out = win.forward(x)
+        lps.append(step.lp)
+        return out
+
+    rng = np.random.default_rng(152)
+    v = A.random_like(x0, rng)
+    y = A.random_like(x0, rng)
+    jv = A.richardson(fwd, x0, v, 1e-5)
+    win.forward(x0)
+    base_lp = step.lp
+    assert all(lp == base_lp for lp in lps), (lps, base_lp)
+    bx = win.reverse(y)
+    lhs, rhs = A.dot(jv, y), A.dot(v, {k: bx[k] for k in v})
+    assert abs(lhs - rhs) <= 1e-7 * A.scale(jv, y), (lhs, rhs)
+
+
+def test_momentum_uwall_adjoint_against_differences():
+    """d(outputs)/d(uwall) . y against the reverse's uwall adjoint."""
+    import dataclasses
+    from felmpf import adjoint
+    cfg = momentum_cfg()
+    fixed, x0 = A.momentum_inputs(cfg, 153)
+    y = A.random_like({k: x0["u"] for k in ("un", "vn", "wn", "pn")}, np.random.default_rng(154))
+
+    def run(uw):
+        c = dataclasses.replace(cfg, uwall=cfg.uwall + uw)
+        st = adjoint.MomentumStep(c, fixed, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+        return st, st.forward(x0)
+
+    h = 1e-4
+    outs = {s: run(s * h)[1] for s in (-2, -1, 1, 2)}
+    d1 = {k: (outs[1][k] - outs[-1][k]) / (2 * h) for k in y}
+    d2 = {k: (outs[2][k] - outs[-2][k]) / (4 * h) for k in y}
+    jd = {k: (4 * d1[k] - d2[k]) / 3 for k in y}
+    st, _ = run(0.0)
+    assert st.lp >= 1
+    bx = st.reverse(y)
+    lhs, rhs = A.dot(jd, y), float(bx["uwall"][0])
+    assert abs(lhs - rhs) <= 1e-7 * max(abs(lhs), 1e-30), (lhs, rhs)
+
+
 def test_planted_transpose_mistake_is_caught():
     """The skewed Thomas systems are not symmetric: using the forward solve
     as its own adjoint (a plausible mistake) violates the identity by O(1)."""
