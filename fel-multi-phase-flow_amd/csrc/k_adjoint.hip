@@ -20,11 +20,12 @@
 //   second colour (bndu, then bnd_periodic of the component) reversed before
 //   its colour.  One colour's update x = (0.5 S/r00 + src)/den (S = sum of
 //   a_m x neighbour_m, den = dtinv + a_p/r00 0.5) is reversed in two
-//   launches: the first, over the colour's cells, adds x_b/den to src_b and
-//   stores G = x_b/den 0.5/r00; the second, over every cell that a colour
-//   cell reads, gathers sum a_m G of the colour cells that read it into its
-//   adjoint and clears the colour cells' own adjoints (a cell never reads a
-//   cell of its own colour, so the gather reads no adjoint it writes).
+//   launches, for both colours of a pair at once (they read no cell of each
+//   other): the first, over the pair's cells, adds x_b/den to src_b and
+//   stores G = x_b/den 0.5/r00; the second, over every cell that a pair
+//   cell reads, gathers sum a_m G of the pair cells that read it into its
+//   adjoint and clears the pair cells' own adjoints (a cell never reads a
+//   cell of its own pair, so the gather reads no adjoint it writes).
 //   With coefficients formed from the viscosity on the fly, the weight a
 //   colour cell x gives cell y equals the coefficient of y's own equation
 //   for x, bit for bit (cal_arith_coef_vis forms both from the same four
@@ -209,17 +210,37 @@ __device__ __forceinline__ void coef_of(const Dims& d, const VisScales<double>& 
   if (P == 2) coef_w(d, s, r, c, v);
 }
 
-// First launch of a colour: src_b += x_b/den and G = x_b/den 0.5/r00 at the
-// colour's cells of component P.
+// First launch of a colour pair: src_b += x_b/den and G = x_b/den 0.5/r00
+// at the cells of component P of both colours: the colour with base
+// parities (is,js,ks) and its complement (the pair's other colour).  The two
+// colours of a pair differ in the parity of every index, so no cell of one
+// reads a cell of the other and the forward's two updates commute: their
+// reverse is one pass.
 template <int P, bool ONFLY>
 static void colour_scale(const Dims d, const SorAdj a, int is, int js, int ks) {
   const int jend = P == 1 ? a.jen : d.nj;
-  const int hx = (d.ni - is) / 2 + 1, hy = jend >= js ? (jend - js) / 2 + 1 : 0,
-            hz = (d.nk - ks) / 2 + 1;
-  if (hy <= 0) return;
-  launch_flat(hx * hy * hz, [=] __device__(int m) {
-    const int ii = m % hx, t = m / hx, jj = t % hy, kk = t / hy;
-    const int c = d.ix(is + 2 * ii, js + 2 * jj, ks + 2 * kk);
+  int h[2][3], base[2][3], cnt[2];
+  for (int t = 0; t < 2; t++) {
+    const int ps[3] = {t ? 3 - is : is, t ? 3 - js : js, t ? 3 - ks : ks};
+    const int top[3] = {d.ni, jend, d.nk};
+    cnt[t] = 1;
+    for (int q = 0; q < 3; q++) {
+      base[t][q] = ps[q];
+      h[t][q] = top[q] >= ps[q] ? (top[q] - ps[q]) / 2 + 1 : 0;
+      cnt[t] *= h[t][q];
+    }
+  }
+  const int n0 = cnt[0];
+  const int hx0 = h[0][0], hy0 = h[0][1], hx1 = h[1][0], hy1 = h[1][1];
+  const int b00 = base[0][0], b01 = base[0][1], b02 = base[0][2];
+  const int b10 = base[1][0], b11 = base[1][1], b12 = base[1][2];
+  launch_flat(cnt[0] + cnt[1], [=] __device__(int m) {
+    const bool second = m >= n0;
+    const int mm = second ? m - n0 : m;
+    const int hx = second ? hx1 : hx0, hy = second ? hy1 : hy0;
+    const int ii = mm % hx, t = mm / hx, jj = t % hy, kk = t / hy;
+    const int c = second ? d.ix(b10 + 2 * ii, b11 + 2 * jj, b12 + 2 * kk)
+                         : d.ix(b00 + 2 * ii, b01 + 2 * jj, b02 + 2 * kk);
     const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
     const double r00 = (a.rho[c] + a.rho[c + F] + a.rhon[c] + a.rhon[c + F]) * 0.25;
     double ap;
@@ -237,19 +258,21 @@ static void colour_scale(const Dims d, const SorAdj a, int is, int js, int ks) {
   });
 }
 
-// Second launch of a colour: every cell y in [0,n+1]^3 gathers the weights
-// of the colour's cells of component P that read it, into the adjoints of
-// u, v and w; the colour cells' own adjoints are cleared.
+// Second launch of a colour pair: every cell y in [0,n+1]^3 gathers the
+// weights of the pair's cells of component P that read it, into the
+// adjoints of u, v and w; the pair cells' own adjoints are cleared.
 template <int P, bool ONFLY>
 static void colour_gather(const Dims d, const SorAdj a, int is, int js, int ks) {
   const int ni = d.ni, nk = d.nk, jend = P == 1 ? a.jen : d.nj;
   launch_box(Box{0, d.ni + 1, 0, d.nj + 1, 0, d.nk + 1}, [=] __device__(int i, int j, int k) {
     const int y = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
-    // G of the colour cell at offset (di,dj,dk), 0 when that cell is not one
+    // G of the pair cell at offset (di,dj,dk), 0 when that cell is not one:
+    // its parities all match (is,js,ks) or all differ from them
     auto in_c = [=](int di, int dj, int dk) {
       const int xi = i + di, xj = j + dj, xk = k + dk;
-      return ((xi ^ is) & 1) == 0 && ((xj ^ js) & 1) == 0 && ((xk ^ ks) & 1) == 0 && xi >= 1 &&
-             xi <= ni && xj >= 1 && xj <= jend && xk >= 1 && xk <= nk;
+      const int bi = (xi ^ is) & 1, bj = (xj ^ js) & 1, bk = (xk ^ ks) & 1;
+      return bi == bj && bj == bk && xi >= 1 && xi <= ni && xj >= 1 && xj <= jend && xk >= 1 &&
+             xk <= nk;
     };
     auto g_at = [=](int di, int dj, int dk) {
       return in_c(di, dj, dk) ? a.G[y + di * X + dj * Y + dk * Z] : 0.0;
@@ -326,9 +349,10 @@ static void colour_gather(const Dims d, const SorAdj a, int is, int js, int ks) 
   });
 }
 
+// Colour pair p (colours 2p and 2p+1 of the sweep order) of component P.
 template <int P>
-static void colour_adj(bool onfly, const Dims d, const SorAdj& a, int col) {
-  const int is = kIsor[col], js = kJsor[col], ks = kKsor[col];
+static void pair_adj(bool onfly, const Dims d, const SorAdj& a, int p) {
+  const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
   if (onfly) {
     colour_scale<P, true>(d, a, is, js, ks);
     colour_gather<P, true>(d, a, is, js, ks);
@@ -765,16 +789,15 @@ void solu_sor4_b_(const int* ipara, const int* ID, const int* nID, const int* nd
   launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
   g_last_adj_onfly = onfly ? 1 : 0;
   const int nsw = *lp < 21 ? *lp + 1 : 21;
+  // the refresh follows the second colour of each pair in the forward
   for (int s = nsw - 1; s >= 0; s--)
     for (int q = 2; q >= 0; q--)
-      for (int col = 7; col >= 0; col--) {
-        if (col % 2 == 1) {
-          periodic_adj(d, a.lb[q]);
-          bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, *dy, dl1, dl2, dphi, lo, hi);
-        }
-        if (q == 0) colour_adj<0>(onfly, d, a, col);
-        if (q == 1) colour_adj<1>(onfly, d, a, col);
-        if (q == 2) colour_adj<2>(onfly, d, a, col);
+      for (int p = 3; p >= 0; p--) {
+        periodic_adj(d, a.lb[q]);
+        bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, *dy, dl1, dl2, dphi, lo, hi);
+        if (q == 0) pair_adj<0>(onfly, d, a, p);
+        if (q == 1) pair_adj<1>(onfly, d, a, p);
+        if (q == 2) pair_adj<2>(onfly, d, a, p);
       }
   add_sum(wacc, nw, uwallb);
   c.finish();
