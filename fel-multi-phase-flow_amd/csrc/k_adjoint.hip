@@ -607,62 +607,151 @@ __device__ __forceinline__ double weno5_adj(const double* df, double vel00, doub
   return velb;
 }
 
+// The WENO5 derivative of component field f at interior cell c along the
+// direction of stride st (srcs/cal_advu_weno5.f:86-92), reversed: from the
+// adjoint rb of the scaled derivative, the adjoints of its six differences
+// (P[0..5], dfm5 .. dfp5) and a quarter of its transport velocity's
+// adjoint (V, the weight of each of the four averaged values, which lie at
+// c - st, c - st + eq, c, c + eq of the transporting component tv).
+__device__ __forceinline__ void weno_cell_adj(const double* f, const double* tv, int c, int st,
+                                              int eq, double rb, double* P, double* V) {
+  const double cm = (tv[c - st] + tv[c - st + eq]) * 0.5, cp = (tv[c] + tv[c + eq]) * 0.5;
+  const double vel00 = (cm + cp) * 0.5;
+  double df[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) df[m] = -f[c + (m - 3) * st] + f[c + (m - 2) * st];
+  *V = weno5_adj(df, vel00, rb, P) * 0.25;
+}
+
+// Difference adjoints gathered into the component's adjoint along x, one
+// block per interior (j,k) row: the six adjoints of every cell of the row go
+// to LDS, then each position y in [-2, ni+3] adds
+//   sum over cells c = y + o (o in -3..3) of P[2-o](c) - P[3-o](c)
+// (difference m of cell c reads f at c + (m-3) st with -1 and c + (m-2) st
+// with +1).  Needs 48 ni bytes of LDS.
+__global__ void __launch_bounds__(256) weno_adj_x_kernel(Dims d, const double* f, const double* tv,
+                                                         int eq, const double* advb, double dinv,
+                                                         double* fb, double* V) {
+  extern __shared__ double sh[];
+  const int ni = d.ni;
+  const int j = (int)blockIdx.x % d.nj + 1, k = (int)blockIdx.x / d.nj + 1;
+  for (int i = (int)threadIdx.x + 1; i <= ni; i += 256) {
+    const int c = d.ix(i, j, k);
+    double P[6], v;
+    weno_cell_adj(f, tv, c, 1, eq, advb[c] * dinv, P, &v);
+#pragma unroll
+    for (int m = 0; m < 6; m++) sh[m * ni + i - 1] = P[m];
+    V[c] = v;
+  }
+  __syncthreads();
+  for (int y = (int)threadIdx.x - 2; y <= ni + 3; y += 256) {
+    double g = 0.0;
+#pragma unroll
+    for (int o = -3; o <= 3; o++) {
+      const int x = y + o;
+      if (x < 1 || x > ni) continue;
+      if (2 - o >= 0 && 2 - o <= 5) g = g + sh[(2 - o) * ni + x - 1];
+      if (3 - o >= 0 && 3 - o <= 5) g = g - sh[(3 - o) * ni + x - 1];
+    }
+    double* o = fb + d.ix(y, j, k);
+    *o = *o + g;
+  }
+}
+
+// The same gather along y or z (or x for rows too long for LDS): one thread
+// per segment of kWenoAdjSeg positions of an interior line, marching along
+// it.  Cell c's adjoints are added to the running sums of the seven
+// positions c-3..c+3, after which position c-3 is complete; a segment
+// [y0, y0+kWenoAdjSeg) takes the cells c in [y0-3, y0+kWenoAdjSeg+2] (the
+// three cells either side are formed by both neighbouring segments).  Each
+// cell's difference adjoints go to its segment's thread only, and V is
+// stored by the segment that owns the cell.
+constexpr int kWenoAdjSeg = 32;
+
+__global__ void __launch_bounds__(256) weno_adj_march_kernel(Dims d, int dd, const double* f,
+                                                             const double* tv, int eq,
+                                                             const double* advb, double dinv,
+                                                             double* fb, double* V, int nseg) {
+  const int n[3] = {d.ni, d.nj, d.nk};
+  const int stv[3] = {1, d.sx, d.sx * d.sy};
+  const int a0 = dd == 0 ? 1 : 0, a1 = dd == 2 ? 1 : 2;  // the two line axes
+  const int gid = (int)(blockIdx.x * 256 + threadIdx.x);
+  const int l0 = gid % n[a0], r = gid / n[a0], seg = r % nseg, l1 = r / nseg;
+  if (l1 >= n[a1]) return;
+  int pos[3];
+  pos[a0] = l0 + 1;
+  pos[a1] = l1 + 1;
+  pos[dd] = 0;
+  const int base = d.ix(pos[0], pos[1], pos[2]);
+  const int st = stv[dd], nl = n[dd];
+  const int y0 = -2 + seg * kWenoAdjSeg;
+  const int y1 = min(y0 + kWenoAdjSeg - 1, nl + 3);
+  double acc[7];
+#pragma unroll
+  for (int t = 0; t < 7; t++) acc[t] = 0.0;
+  for (int c = y0 - 3; c <= y1 + 3; c++) {
+    if (c >= 1 && c <= nl) {
+      const int cc = base + c * st;
+      double P[6], v;
+      weno_cell_adj(f, tv, cc, st, eq, advb[cc] * dinv, P, &v);
+      if (c >= y0 && c <= y1) V[cc] = v;
+      // position c + o receives P[2+o] - P[3+o] (o = -3..3)
+      acc[0] = acc[0] - P[0];
+#pragma unroll
+      for (int o = -2; o <= 2; o++) acc[o + 3] = acc[o + 3] + (P[2 + o] - P[3 + o]);
+      acc[6] = acc[6] + P[5];
+    }
+    if (c - 3 >= y0) {
+      double* out = fb + base + (c - 3) * st;
+      *out = *out + acc[0];
+    }
+#pragma unroll
+    for (int t = 0; t < 6; t++) acc[t] = acc[t + 1];
+    acc[6] = 0.0;
+  }
+}
+
 // srcs/cal_advu_weno5.f:1-257 reversed for component q and direction dd:
-// the first launch forms, at every interior cell, the adjoints of the six
-// differences and of the transport velocity (scratch P[0..5], V); the
-// second gathers them into the component's adjoint (each difference reads
-// two cells) and into the transporting component's adjoint (the velocity
-// averages four values).
+// the difference adjoints gathered along the direction (LDS rows along x,
+// marching lines along y and z), then the transport velocity's adjoints
+// (V, one scratch array) gathered into the transporting component's
+// adjoint: cell y receives V of the cells y + e_d, y + e_d - e_q, y, y - e_q.
 static void weno_adj_dir(const Dims d, int q, int dd, double dinv, const double* const* vel,
-                         double* const* velb, const double* advb, double* const* P) {
+                         double* const* velb, const double* advb, double* V) {
   const int E[3] = {1, d.sx, d.sx * d.sy};
   const int st = E[dd], eq = E[q];
   const double* f = vel[q];
   const double* tv = vel[dd];
-  double *P0 = P[0], *P1 = P[1], *P2 = P[2], *P3 = P[3], *P4 = P[4], *P5 = P[5], *V = P[6];
-  launch_box(Box{1, d.ni, 1, d.nj, 1, d.nk}, [=] __device__(int i, int j, int k) {
-    const int c = d.ix(i, j, k);
-    const double cm = (tv[c - st] + tv[c - st + eq]) * 0.5, cp = (tv[c] + tv[c + eq]) * 0.5;
-    const double vel00 = (cm + cp) * 0.5;
-    double df[6], db[6];
-#pragma unroll
-    for (int m = 0; m < 6; m++) df[m] = -f[c + (m - 3) * st] + f[c + (m - 2) * st];
-    const double vb = weno5_adj(df, vel00, advb[c] * dinv, db);
-    P0[c] = db[0]; P1[c] = db[1]; P2[c] = db[2]; P3[c] = db[3]; P4[c] = db[4]; P5[c] = db[5];
-    V[c] = vb * 0.25;
-  });
   double* fb = velb[q];
   double* tb = velb[dd];
+  if (dd == 0 && d.ni <= 1024) {
+    weno_adj_x_kernel<<<dim3(d.nj * d.nk), dim3(256), 6 * d.ni * sizeof(double), stream()>>>(
+        d, f, tv, eq, advb, dinv, fb, V);
+  } else {
+    const int n[3] = {d.ni, d.nj, d.nk};
+    const int lines = dd == 0 ? n[1] * n[2] : (dd == 1 ? n[0] * n[2] : n[0] * n[1]);
+    const int nseg = (n[dd] + 6 + kWenoAdjSeg - 1) / kWenoAdjSeg;
+    const long long threads = (long long)lines * nseg;
+    weno_adj_march_kernel<<<dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream()>>>(
+        d, dd, f, tv, eq, advb, dinv, fb, V, nseg);
+  }
+  FEL_CHECK(hipGetLastError());
   const int ni = d.ni, nj = d.nj, nk = d.nk;
   const int di[3] = {dd == 0, dd == 1, dd == 2}, qi[3] = {q == 0, q == 1, q == 2};
   const int sdi = di[0], sdj = di[1], sdk = di[2], sqi = qi[0], sqj = qi[1], sqk = qi[2];
-  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+  launch_box(Box{-1, ni + 1, -1, nj + 1, -1, nk + 1}, [=] __device__(int i, int j, int k) {
     const int y = d.ix(i, j, k);
     // interior cell at y + s*(direction) + t*(component q's direction)
     auto in_at = [=](int s, int t) {
       const int xi = i + s * sdi + t * sqi, xj = j + s * sdj + t * sqj, xk = k + s * sdk + t * sqk;
       return xi >= 1 && xi <= ni && xj >= 1 && xj <= nj && xk >= 1 && xk <= nk;
     };
-    const double* Pm[6] = {P0, P1, P2, P3, P4, P5};
-    double g = 0.0;
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-      // + in the difference of the cell at y - (m-2) st, - in that at y - (m-3) st
-      if (in_at(2 - m, 0)) g = g + Pm[m][y + (2 - m) * st];
-      if (in_at(3 - m, 0)) g = g - Pm[m][y + (3 - m) * st];
-    }
-    // transport velocity of the cells y + e_d, y + e_d - e_q, y, y - e_q
     double t = 0.0;
     if (in_at(1, 0)) t = t + V[y + st];
     if (in_at(1, -1)) t = t + V[y + st - eq];
     if (in_at(0, 0)) t = t + V[y];
     if (in_at(0, -1)) t = t + V[y - eq];
-    if (fb == tb) {
-      fb[y] = fb[y] + (g + t);
-    } else {
-      fb[y] = fb[y] + g;
-      tb[y] = tb[y] + t;
-    }
+    tb[y] = tb[y] + t;
   });
 }
 
@@ -944,11 +1033,10 @@ void cal_advu_weno5_b_(const int* ni, const int* nj, const int* nk, const double
   double* velb[3] = {c.io(ub, d.n3, "ub"), c.io(vb, d.n3, "vb"), c.io(wb, d.n3, "wb")};
   double* advb[3] = {c.io(adv_ub, d.n3, "adv_ub"), c.io(adv_vb, d.n3, "adv_vb"),
                      c.io(adv_wb, d.n3, "adv_wb")};
-  double* P[7];
-  for (int m = 0; m < 7; m++) P[m] = static_cast<double*>(scratch(26 + m, sizeof(double) * (size_t)d.n3));
+  double* V = static_cast<double*>(scratch(26, sizeof(double) * (size_t)d.n3));
   const double dinv[3] = {*dxinv, *dyinv, *dzinv};
   for (int q = 2; q >= 0; q--)
-    for (int dd = 2; dd >= 0; dd--) weno_adj_dir(d, q, dd, dinv[dd], vel, velb, advb[q], P);
+    for (int dd = 2; dd >= 0; dd--) weno_adj_dir(d, q, dd, dinv[dd], vel, velb, advb[q], V);
   for (int q = 0; q < 3; q++) {
     double* a = advb[q];
     launch_flat(d.n3, [=] __device__(int m) { a[m] = 0.0; });
