@@ -663,6 +663,49 @@ void cal_advu_weno5_b_(const int *ni, const int *nj, const int *nk, const double
                        const double *v, double *vb, const double *w, double *wb, double *adv_ub,
                        double *adv_vb, double *adv_wb);
 
+/* srcs/cal_grad_p2a.f:30-71 transposed: qb gathers the vertex-gradient
+ * adjoints, which are consumed on [-2,n+2]^3. */
+void cal_grad_p2a_b_(const int *ID, const int *svall, const int *ni, const int *nj, const int *nk,
+                     const double *dxinv, const double *dyinv_array, const double *dzinv, double *qb,
+                     double *qxb, double *qyb, double *qzb);
+
+/* srcs/solphi_mthinc1.f:1-292 reversed at its inputs: each of phi, phix,
+ * phiy, phiz, u, v, w followed by its adjoint (incremented); the flux
+ * adjoints consumed on [-1,n+1]^3.  Branches and upwind directions are the
+ * forward's; the Newton offset is differentiated by the implicit-function
+ * theorem at the forward's iterate. */
+void solphi_mthinc1_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                       const double *dyinv, const double *dzinv, const double *dt,
+                       const double *bet_mthinc, const double *phi, double *phib, const double *phix,
+                       double *phixb, const double *phiy, double *phiyb, const double *phiz,
+                       double *phizb, const double *u, double *ub, const double *v, double *vb,
+                       const double *w, double *wb, double *flphixb, double *flphiyb,
+                       double *flphizb);
+
+/* srcs/solphi_mthinc2.f:30-41 reversed: each of u, v, w, flphix, flphiy,
+ * flphiz, phi followed by its adjoint; phinb consumed on the interior (the
+ * clamp passes the derivative where the value lies in [0,1]). */
+void solphi_mthinc2_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                       const double *dyinv, const double *dzinv, const double *dt, const double *u,
+                       double *ub, const double *v, double *vb, const double *w, double *wb,
+                       const double *flphix, double *flphixb, const double *flphiy,
+                       double *flphiyb, const double *flphiz, double *flphizb, const double *phi,
+                       double *phib, double *phinb);
+
+/* srcs/solphi_mthinc3.f:43-160 reversed: phix, phiy, phiz select the flat
+ * cells; phi followed by its adjoint; phin is the forward's input phin and
+ * phinb the adjoint of its output on entry, of its input on return. */
+void solphi_mthinc3_b_(const int *ipara, const int *ni, const int *nj, const int *nk,
+                       const double *dxinv, const double *dyinv, const double *dzinv,
+                       const double *bet_mthinc, const double *phix, const double *phiy,
+                       const double *phiz, const double *phi, double *phib, const double *phin,
+                       double *phinb);
+
+/* srcs/summation.f:27-50 reversed: qk the forward's input slices
+ * (nbub+1 padded arrays), qkb(.,0) consumed into qkb(.,1..nbub). */
+void summation_b_(const int *ni, const int *nj, const int *nk, const double *qk, double *qkb,
+                  const int *nbub);
+
 /* How the last solu_sor4_b_ call obtained its coefficients: 1 formed on the
  * fly from the viscosity, 0 read from the 45 arrays, -1 before any call. */
 int felgpu_sor_b_onfly_(void);

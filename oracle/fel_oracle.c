@@ -56,48 +56,6 @@ void orc_cal_grad_p2a(const int *ID, const int *svall, const int *ni_, const int
       }
 }
 
-/* Cell normal = mean of the 8 surrounding vertex gradients
- * (srcs/solphi_mthinc1.f:81-107, same sums in srcs/solphi_mthinc3.f:47-73). */
-static inline void cell_normal(ptrdiff_t sx, ptrdiff_t sy, const double *phix, const double *phiy,
-                               const double *phiz, int i, int j, int k, double *nx, double *ny,
-                               double *nz) {
-  const double *a[3] = {phix, phiy, phiz};
-  double r[3];
-  for (int c = 0; c < 3; c++) {
-    const double *p = a[c];
-    r[c] = (+p[OIX(i - 1, j - 1, k - 1)] + p[OIX(i, j - 1, k - 1)] + p[OIX(i - 1, j, k - 1)] +
-            p[OIX(i, j, k - 1)] + p[OIX(i - 1, j - 1, k)] + p[OIX(i, j - 1, k)] +
-            p[OIX(i - 1, j, k)] + p[OIX(i, j, k)]) *
-           0.125;
-  }
-  *nx = r[0];
-  *ny = r[1];
-  *nz = r[2];
-}
-
-/* srcs/mthinc_x.h:1-14 followed by srcs/mthinc_tanh.h:1-9: 8-point Gauss sum
- * of wght/(1+exp(-2 bet (n.x+d))). */
-static inline double mthinc_tanh_sum(double rnx, double rny, double rnz, double xg1, double xg2,
-                                     double yg1, double yg2, double zg1, double zg2, double wght,
-                                     double bet, double ddd, double xs[8]) {
-  const double x1 = rnx * xg1, x2 = rnx * xg2, y1 = rny * yg1, y2 = rny * yg2;
-  const double z1 = rnz * zg1, z2 = rnz * zg2;
-  xs[0] = x1 + y1 + z1;
-  xs[1] = x2 + y1 + z1;
-  xs[2] = x1 + y2 + z1;
-  xs[3] = x2 + y2 + z1;
-  xs[4] = x1 + y1 + z2;
-  xs[5] = x2 + y1 + z2;
-  xs[6] = x1 + y2 + z2;
-  xs[7] = x2 + y2 + z2;
-  double s = 0.0;
-  for (int m = 0; m < 8; m++) {
-    const double t = wght / (1.0 + exp(-(2.0 * bet * (xs[m] + ddd))));
-    s = (m == 0) ? t : s + t;
-  }
-  return s;
-}
-
 /* srcs/solphi_mthinc1.f:1-292 (+ mthinc_x.h, mthinc_tanh.h).  Face fluxes of
  * the MTHINC scheme, already multiplied by dt/dx.
  * pinned: _ref; symmetric d=0 case and the uniform-field upwind closed form. */

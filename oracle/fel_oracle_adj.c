@@ -895,3 +895,316 @@ void orc_cpy_b(const int *ni_, const int *nj_, const int *nk_, double *q1b, doub
     q2b[m] = 0.0;
   }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Volume-fraction transport (MTHINC)                                        */
+/* ------------------------------------------------------------------------- */
+
+/* srcs/cal_grad_p2a.f:30-71 transposed: the 8-point vertex gradients on
+ * [-2,n+2]^3 (linear). */
+void orc_cal_grad_p2a_b(const int *ID, const int *svall, const int *ni_, const int *nj_,
+                        const int *nk_, const double *dxinv_, const double *dyinv_array,
+                        const double *dzinv_, double *qb, double *qxb, double *qyb, double *qzb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  (void)svall;
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const double dyinv = dyinv_array[j + nj * (*ID) + 2];
+        const ptrdiff_t c = OIX(i, j, k);
+        const double gx = qxb[c] * 0.25 * dxinv, gy = qyb[c] * 0.25 * dyinv,
+                     gz = qzb[c] * 0.25 * dzinv;
+        for (int m = 0; m < 8; m++) {
+          const int a = m & 1, b = (m >> 1) & 1, e = (m >> 2) & 1;
+          const double sxg = a ? 1.0 : -1.0, syg = b ? 1.0 : -1.0, szg = e ? 1.0 : -1.0;
+          qb[OIX(i + a, j + b, k + e)] += sxg * gx + syg * gy + szg * gz;
+        }
+        qxb[c] = 0.0;
+        qyb[c] = 0.0;
+        qzb[c] = 0.0;
+      }
+}
+
+static inline double sig2(double bet, double z) { return 1.0 / (1.0 + exp(-(2.0 * bet * z))); }
+
+/* srcs/solphi_mthinc1.f:1-292 reversed at its inputs (orc_solphi_mthinc1).
+ * Flat cells (upwind branch): flux = u phi00 dt/dx, phi00 the truncated phi
+ * (derivative 0 where truncated).  Interface cells: flux = sum over the
+ * face sub-cell's 8 Gauss points of (vd/8) s(n.x + d), s(z) = 1/(1 +
+ * exp(-2 bet z)), n = r/|r| (r the mean of the 8 vertex gradients), the
+ * sub-cell's Gauss points moving with vd = u dt/dx; d solves
+ * sum_g s(n.x_g + d)/8 = phi over the cell's Gauss points, and its
+ * derivative is taken by the implicit-function theorem at the Newton
+ * iterate: dd = (dphi - dn.G_n)/G_d.  A face both neighbours write (flat
+ * cells, u = 0) takes the later writer's derivative, as the forward's loop
+ * order leaves its value.  Branches and upwind directions are those of the
+ * forward at its inputs.  The flux adjoints are consumed on [-1,n+1]^3. */
+void orc_solphi_mthinc1_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                          const double *dyinv_, const double *dzinv_, const double *dt_,
+                          const double *bet_mthinc, const double *phi, double *phib,
+                          const double *phix, double *phixb, const double *phiy, double *phiyb,
+                          const double *phiz, double *phizb, const double *u, double *ub,
+                          const double *v, double *vb, const double *w, double *wb,
+                          double *flphixb, double *flphiyb, double *flphizb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dinv[3] = {*dxinv_, *dyinv_, *dzinv_}, dt = *dt_;
+  ODIMS(ni, nj, nk);
+  const double dx = 1.0 / dinv[0], dy = 1.0 / dinv[1], dz = 1.0 / dinv[2];
+  const double bet = *bet_mthinc;
+  const double gp1 = -(0.5 / sqrt(3.0)), gp2 = 0.5 / sqrt(3.0);
+  const double gp[2] = {gp1, gp2};
+  const double dcellinv = 1.0 / sqrt((dx * dx + dy * dy + dz * dz) / 3.0);
+  const double ch = cosh(-(bet * 1.5));
+  const double eps_mthinc = bet / (2.0 * (ch * ch));
+  const double eps_trunc = 0.5 * (tanh(-(bet * 2.5)) + 1.0);
+  const ptrdiff_t E[3] = {1, sx, sx * sy};
+  const double *vel[3] = {u, v, w};
+  double *velb[3] = {ub, vb, wb};
+  double *flb[3] = {flphixb, flphiyb, flphizb};
+  double *gb[3] = {phixb, phiyb, phizb};
+  double xs[8];
+  for (int k = nk + 1; k >= 0; k--)
+    for (int j = nj + 1; j >= 0; j--)
+      for (int i = ni + 1; i >= 0; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        double r[3];
+        cell_normal(sx, sy, phix, phiy, phiz, i, j, k, &r[0], &r[1], &r[2]);
+        const double q00 = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        /* faces in the forward's order: minus then plus face of x, y, z;
+         * reversed here */
+        if (q00 <= eps_mthinc * dcellinv) {
+          double phi00 = phi[c];
+          double dphi = 1.0;
+          if (phi00 <= eps_trunc) { phi00 = 0.0; dphi = 0.0; }
+          if (1.0 - phi00 <= eps_trunc) { phi00 = 1.0; dphi = 0.0; }
+          for (int a = 2; a >= 0; a--) {
+            const double vm = vel[a][c - E[a]], vp = vel[a][c];
+            if (vp >= 0.0) {
+              const double fbv = flb[a][c];
+              velb[a][c] += fbv * phi00 * dt * dinv[a];
+              phib[c] += fbv * vp * dt * dinv[a] * dphi;
+              flb[a][c] = 0.0;
+            }
+            if (vm <= 0.0) {
+              const double fbv = flb[a][c - E[a]];
+              velb[a][c - E[a]] += fbv * phi00 * dt * dinv[a];
+              phib[c] += fbv * vm * dt * dinv[a] * dphi;
+              flb[a][c - E[a]] = 0.0;
+            }
+          }
+          continue;
+        }
+        const double coef = 1.0 / q00;
+        const double n[3] = {r[0] * coef, r[1] * coef, r[2] * coef};
+        const double phic = phi[c];
+        double ddd = 0.0;
+        for (int inr = 1; inr <= 10; inr++) {
+          const double wt = mthinc_tanh_sum(n[0], n[1], n[2], gp1, gp2, gp1, gp2, gp1, gp2,
+                                                0.125, bet, ddd, xs);
+          const double f = wt - phic;
+          double s = 0.0;
+          for (int m = 0; m < 8; m++) {
+            const double cc = cosh(bet * (xs[m] + ddd));
+            const double t = 0.125 / (cc * cc);
+            s = (m == 0) ? t : s + t;
+          }
+          const double df = 0.5 * bet * s;
+          ddd = ddd - f * df / (df * df + 1.0e-99);
+        }
+        /* the cell's own Gauss points: G_d, G_n */
+        double Gd = 0.0, Gn[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < 8; m++) {
+          const double X[3] = {gp[m & 1], gp[(m >> 1) & 1], gp[(m >> 2) & 1]};
+          const double z = n[0] * X[0] + n[1] * X[1] + n[2] * X[2] + ddd;
+          const double sg = sig2(bet, z);
+          const double sp = 0.125 * 2.0 * bet * sg * (1.0 - sg);
+          Gd += sp;
+          for (int q = 0; q < 3; q++) Gn[q] += sp * X[q];
+        }
+        double nb[3] = {0.0, 0.0, 0.0}, db = 0.0;
+        for (int a = 2; a >= 0; a--) {
+          for (int side = 1; side >= 0; side--) {
+            /* side 1: plus face (vp > 0), side 0: minus face (vm < 0) */
+            const double vface = side ? vel[a][c] : vel[a][c - E[a]];
+            if (side ? !(vface > 0.0) : !(vface < 0.0)) continue;
+            const ptrdiff_t fc = side ? c : c - E[a];
+            const double fbv = flb[a][fc];
+            flb[a][fc] = 0.0;
+            const double vd = vface * dt * dinv[a];
+            double st, en, dst, den_;
+            if (side) { st = 0.5 - vd; en = 0.5; dst = -1.0; den_ = 0.0; }
+            else { st = -0.5; en = -0.5 - vd; dst = 0.0; den_ = -1.0; }
+            const double g[2] = {(st + en) * 0.5 + (-st + en) * gp1, (st + en) * 0.5 + (-st + en) * gp2};
+            const double dg[2] = {(dst + den_) * 0.5 + (-dst + den_) * gp1,
+                                  (dst + den_) * 0.5 + (-dst + den_) * gp2};
+            const double W = 0.125 * vd;
+            double S = 0.0, Sp = 0.0, Spx[3] = {0.0, 0.0, 0.0}, Svd = 0.0;
+            for (int m = 0; m < 8; m++) {
+              const int ch3[3] = {m & 1, (m >> 1) & 1, (m >> 2) & 1};
+              double X[3];
+              for (int q = 0; q < 3; q++) X[q] = q == a ? g[ch3[q]] : gp[ch3[q]];
+              const double z = n[0] * X[0] + n[1] * X[1] + n[2] * X[2] + ddd;
+              const double sg = sig2(bet, z);
+              const double sp = 2.0 * bet * sg * (1.0 - sg);
+              S += sg;
+              Sp += sp;
+              for (int q = 0; q < 3; q++) Spx[q] += sp * X[q];
+              Svd += sp * n[a] * dg[ch3[a]];
+            }
+            const double vdb = fbv * (0.125 * S + W * Svd);
+            velb[a][fc] += vdb * dt * dinv[a];
+            for (int q = 0; q < 3; q++) nb[q] += fbv * W * Spx[q];
+            db += fbv * W * Sp;
+          }
+        }
+        /* through d (implicit function of n and phi) */
+        phib[c] += db / Gd;
+        for (int q = 0; q < 3; q++) nb[q] -= db * Gn[q] / Gd;
+        /* through n = r/|r| */
+        const double ndn = n[0] * nb[0] + n[1] * nb[1] + n[2] * nb[2];
+        for (int q = 0; q < 3; q++) {
+          const double rb = (nb[q] - ndn * n[q]) * coef;
+          for (int m = 0; m < 8; m++) {
+            const int a = m & 1, b = (m >> 1) & 1, e = (m >> 2) & 1;
+            gb[q][OIX(i - 1 + a, j - 1 + b, k - 1 + e)] += rb * 0.125;
+          }
+        }
+      }
+  for (int k = -1; k <= nk + 1; k++)
+    for (int j = -1; j <= nj + 1; j++)
+      for (int i = -1; i <= ni + 1; i++) {
+        flphixb[OIX(i, j, k)] = 0.0;
+        flphiyb[OIX(i, j, k)] = 0.0;
+        flphizb[OIX(i, j, k)] = 0.0;
+      }
+}
+
+/* srcs/solphi_mthinc2.f:30-41 reversed: phin = clamp(phi (1 + dt div) +
+ * flux differences) on the interior; the clamp passes the derivative where
+ * the value is in [0,1] (fmin/fmax select their first operand on ties). */
+void orc_solphi_mthinc2_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                          const double *dyinv_, const double *dzinv_, const double *dt_,
+                          const double *u, double *ub, const double *v, double *vb,
+                          const double *w, double *wb, const double *flphix, double *flphixb,
+                          const double *flphiy, double *flphiyb, const double *flphiz,
+                          double *flphizb, const double *phi, double *phib, double *phinb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_, dt = *dt_;
+  ODIMS(ni, nj, nk);
+  for (int k = nk; k >= 1; k--)
+    for (int j = nj; j >= 1; j--)
+      for (int i = ni; i >= 1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double div00 = +(-u[OIX(i - 1, j, k)] + u[c]) * dxinv +
+                             (-v[OIX(i, j - 1, k)] + v[c]) * dyinv +
+                             (-w[OIX(i, j, k - 1)] + w[c]) * dzinv;
+        const double p = phi[c] * (1.0 + dt * div00) + flphix[OIX(i - 1, j, k)] - flphix[c] +
+                         flphiy[OIX(i, j - 1, k)] - flphiy[c] + flphiz[OIX(i, j, k - 1)] - flphiz[c];
+        const double g = (p <= 1.0 && p >= 0.0) ? phinb[c] : 0.0;
+        phinb[c] = 0.0;
+        phib[c] += g * (1.0 + dt * div00);
+        const double h = g * phi[c] * dt;
+        ub[c] += h * dxinv; ub[OIX(i - 1, j, k)] -= h * dxinv;
+        vb[c] += h * dyinv; vb[OIX(i, j - 1, k)] -= h * dyinv;
+        wb[c] += h * dzinv; wb[OIX(i, j, k - 1)] -= h * dzinv;
+        flphixb[OIX(i - 1, j, k)] += g; flphixb[c] -= g;
+        flphiyb[OIX(i, j - 1, k)] += g; flphiyb[c] -= g;
+        flphizb[OIX(i, j, k - 1)] += g; flphizb[c] -= g;
+      }
+}
+
+/* srcs/solphi_mthinc3.f:43-160 reversed (orc_solphi_mthinc3): truncation of
+ * flat cells on [0,n+1]^3 (derivative 0 where it sets 0 or 1), clamp, the
+ * mass correction p + offset p^2 (1-p) with offset = (sum phi - sum p) /
+ * (sum p^2 (1-p) + 1e-99) over the interior, clamp.  phin is the input
+ * phin of the forward (in), phinb the adjoint of its output in and of its
+ * input out; phib incremented (through sum phi). */
+void orc_solphi_mthinc3_b(const int *ipara, const int *ni_, const int *nj_, const int *nk_,
+                          const double *dxinv_, const double *dyinv_, const double *dzinv_,
+                          const double *bet_mthinc, const double *phix, const double *phiy,
+                          const double *phiz, const double *phi, double *phib, const double *phin,
+                          double *phinb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  (void)ipara;
+  const double dx = 1.0 / *dxinv_, dy = 1.0 / *dyinv_, dz = 1.0 / *dzinv_;
+  const double bet = *bet_mthinc;
+  const double dcellinv = 1.0 / sqrt((dx * dx + dy * dy + dz * dz) / 3.0);
+  const double ch = cosh(-(bet * 1.5));
+  const double eps_mthinc = bet / (2.0 * (ch * ch));
+  const double eps_trunc = 0.5 * (tanh(-(bet * 2.5)) + 1.0);
+  /* forward replay: p after truncation (mask t) and clamp (mask b) */
+  double *p = malloc(sizeof(double) * n3);
+  unsigned char *mt = calloc((size_t)n3, 1), *mb = calloc((size_t)n3, 1), *md = calloc((size_t)n3, 1);
+  memcpy(p, phin, sizeof(double) * n3);
+  for (ptrdiff_t m = 0; m < n3; m++) mt[m] = 1;
+  for (int k = 0; k <= nk + 1; k++)
+    for (int j = 0; j <= nj + 1; j++)
+      for (int i = 0; i <= ni + 1; i++) {
+        double nx, ny, nz;
+        cell_normal(sx, sy, phix, phiy, phiz, i, j, k, &nx, &ny, &nz);
+        const double q00 = sqrt(nx * nx + ny * ny + nz * nz);
+        if (q00 <= eps_mthinc * dcellinv) {
+          const ptrdiff_t c = OIX(i, j, k);
+          if (p[c] <= eps_trunc) { p[c] = 0.0; mt[c] = 0; }
+          if (1.0 - p[c] <= eps_trunc) { p[c] = 1.0; mt[c] = 0; }
+        }
+      }
+  double sum1 = 0.0, sum2 = 0.0, den2 = 0.0;
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const ptrdiff_t c = OIX(i, j, k);
+        mb[c] = p[c] <= 1.0 && p[c] >= 0.0;
+        p[c] = fmax(fmin(p[c], 1.0), 0.0);
+        sum1 = sum1 + phi[c];
+        sum2 = sum2 + p[c];
+        den2 = den2 + p[c] * p[c] * (1.0 - p[c]);
+      }
+  const double den = den2 + 1.0e-99;
+  const double offset = (sum1 - sum2) / den;
+  /* reverse of the correction and its clamp */
+  double offb = 0.0;
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double q = p[c] + offset * (p[c] * p[c]) * (1.0 - p[c]);
+        md[c] = q <= 1.0 && q >= 0.0;
+        if (md[c]) offb += phinb[c] * (p[c] * p[c]) * (1.0 - p[c]);
+      }
+  const double s1b = offb / den, s2b = -offb / den, denb = -offb * offset / den;
+  for (int k = nk; k >= 1; k--)
+    for (int j = nj; j >= 1; j--)
+      for (int i = ni; i >= 1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        const double pc = p[c], gp = 2.0 * pc - 3.0 * pc * pc;
+        double l = md[c] ? phinb[c] * (1.0 + offset * gp) : 0.0;
+        l += s2b + denb * gp;
+        phib[c] += s1b;
+        phinb[c] = mb[c] ? l : 0.0;
+      }
+  for (ptrdiff_t m = 0; m < n3; m++)
+    if (!mt[m]) phinb[m] = 0.0;
+  free(p); free(mt); free(mb); free(md);
+}
+
+/* srcs/summation.f:27-50 reversed: qk(.,0) = sum of the nbub slices on
+ * the padded array, clamped to [0,1] on the interior (qk holds the forward's
+ * input slices; qkb(.,0) is consumed into qkb(.,1..nbub)). */
+void orc_summation_b(const int *ni_, const int *nj_, const int *nk_, const double *qk, double *qkb,
+                     const int *nbub_) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_, nbub = *nbub_;
+  ODIMS(ni, nj, nk);
+  for (ptrdiff_t m = 0; m < n3; m++) {
+    double s = qk[n3 + m];
+    for (int l = 2; l <= nbub; l++) s = s + qk[(ptrdiff_t)l * n3 + m];
+    const ptrdiff_t pl = m % (sx * sy), i = pl % sx - 2, j = pl / sx - 2, k = m / (sx * sy) - 2;
+    const int inner = i >= 1 && i <= ni && j >= 1 && j <= nj && k >= 1 && k <= nk;
+    const double g = (!inner || (s <= 1.0 && s >= 0.0)) ? qkb[m] : 0.0;
+    qkb[m] = 0.0;
+    for (int l = 1; l <= nbub; l++) qkb[(ptrdiff_t)l * n3 + m] += g;
+  }
+}

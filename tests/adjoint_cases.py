@@ -691,3 +691,237 @@ def richardson(fwd, x0: dict, v: dict, h: float):
         return {k: (p[k] - m[k]) / (2 * hh) for k in p}
     d1, d2 = central(h), central(2 * h)
     return {k: (4 * d1[k] - d2[k]) / 3 for k in d1}
+
+
+# --------------------------------------------------------------------------
+# Volume-fraction transport (MTHINC)
+# --------------------------------------------------------------------------
+
+class FDCase(Case):
+    """A nonlinear routine: forward(call, v) is its Jacobian at self.x0
+    applied to v, by Richardson-extrapolated central differences."""
+    h = 1e-6
+
+    def value(self, call, x):
+        raise NotImplementedError
+
+    def forward(self, call, x):
+        def central(hh):
+            p = self.value(call, {k: self.x0[k] + hh * x[k] for k in x})
+            m = self.value(call, {k: self.x0[k] - hh * x[k] for k in x})
+            return {k: (p[k] - m[k]) / (2 * hh) for k in p}
+        d1, d2 = central(self.h), central(2 * self.h)
+        return {k: (4 * d1[k] - d2[k]) / 3 for k in d1}
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(np.shape(v)) for k, v in self.x0.items()}
+
+
+def vof_grad(g: Grid, phi, call=H.call_oracle):
+    o = [g.zeros() for _ in range(3)]
+    call("cal_grad_p2a", I(0), I(g.nj), I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx),
+         np.full(g.nj + 6, 1 / g.dy), D(1 / g.dz), phi, *o)
+    return o
+
+
+def away_from_zero(r, shape, scale):
+    """Velocities bounded away from 0, so that no upwind direction flips
+    under a small perturbation."""
+    return np.where(r.uniform(size=shape) < 0.5, -1.0, 1.0) * (0.2 + r.uniform(size=shape)) * scale
+
+
+class GradP2a(Case):
+    name = "cal_grad_p2a"
+
+    def inputs(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("qx", "qy", "qz")}
+
+    def head(self):
+        g = self.g
+        return [I(0), I(g.nj), I(g.ni), I(g.nj), I(g.nk), D(1 / g.dx), np.full(g.nj + 6, 1 / g.dy),
+                D(1 / g.dz)]
+
+    def forward(self, call, x):
+        g = self.g
+        o = [g.zeros() for _ in range(3)]
+        call("cal_grad_p2a", *self.head(), x["q"].copy(), *o)
+        return dict(zip(("qx", "qy", "qz"), o))
+
+    def adjoint(self, call, y):
+        g = self.g
+        qb = g.zeros()
+        o = [y[k].copy() for k in ("qx", "qy", "qz")]
+        call("cal_grad_p2a_b", *self.head(), qb, *o)
+        return {"q": qb, "_kept": o}
+
+
+class Mthinc1(FDCase):
+    name = "solphi_mthinc1"
+    bet = 2.0
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 800)
+        phi = H.smooth_phi(g, r, beta=1.0)
+        gx, gy, gz = vof_grad(g, phi)
+        vel = {k: away_from_zero(r, g.shape, 0.5) for k in ("u", "v", "w")}
+        self.dt = 0.2 * min(g.dx, g.dy, g.dz) / 1.2
+        self.x0 = {"phi": phi, "phix": gx, "phiy": gy, "phiz": gz, **vel}
+
+    def inputs(self):
+        """Random directions, without phi components at cells within 1e-5 of
+        the truncation thresholds (the differences would cross them)."""
+        v = super().inputs()
+        et = 0.5 * (np.tanh(-self.bet * 2.5) + 1.0)
+        phi = self.x0["phi"]
+        near = (np.abs(phi - et) < 1e-5) | (np.abs(1.0 - phi - et) < 1e-5)
+        v["phi"] = np.where(near, 0.0, v["phi"])
+        return v
+
+    def out_adjoints(self):
+        m = box_mask(self.g, -1, 1)
+        return {k: np.where(m, self.rng.standard_normal(self.g.shape), 0.0) for k in ("fx", "fy", "fz")}
+
+    def head(self):
+        g = self.g
+        return [I(g.ni), I(g.nj), I(g.nk), *dxyz(g), D(self.dt), D(self.bet)]
+
+    def value(self, call, x):
+        g = self.g
+        o = [g.zeros() for _ in range(3)]
+        call("solphi_mthinc1", *self.head(), x["phi"].copy(), x["phix"].copy(), x["phiy"].copy(),
+             x["phiz"].copy(), x["u"].copy(), x["v"].copy(), x["w"].copy(), *o)
+        return dict(zip(("fx", "fy", "fz"), o))
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {k: g.zeros() for k in self.x0}
+        o = [y[k].copy() for k in ("fx", "fy", "fz")]
+        x0 = self.x0
+        call("solphi_mthinc1_b", *self.head(), x0["phi"], b["phi"], x0["phix"], b["phix"],
+             x0["phiy"], b["phiy"], x0["phiz"], b["phiz"], x0["u"], b["u"], x0["v"], b["v"],
+             x0["w"], b["w"], *o)
+        assert not any(np.any(q[box_mask(g, -1, 1)]) for q in o), "flux adjoints are consumed"
+        return b
+
+
+class Mthinc2(FDCase):
+    name = "solphi_mthinc2"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 900)
+        self.dt = 0.05 * g.dx
+        self.x0 = {"u": r.standard_normal(g.shape), "v": r.standard_normal(g.shape),
+                   "w": r.standard_normal(g.shape), "fx": 0.02 * r.standard_normal(g.shape),
+                   "fy": 0.02 * r.standard_normal(g.shape), "fz": 0.02 * r.standard_normal(g.shape),
+                   "phi": r.uniform(0.1, 0.9, g.shape)}
+        # a few cells clamped at each bound
+        self.x0["phi"][5, 4, 4] = 1.3
+        self.x0["phi"][6, 5, 7] = -0.4
+
+    def out_adjoints(self):
+        return {"phin": np.where(interior_mask(self.g), self.rng.standard_normal(self.g.shape), 0.0)}
+
+    def head(self):
+        g = self.g
+        return [I(g.ni), I(g.nj), I(g.nk), *dxyz(g), D(self.dt)]
+
+    def value(self, call, x):
+        g = self.g
+        phin = g.zeros()
+        call("solphi_mthinc2", *self.head(), x["u"].copy(), x["v"].copy(), x["w"].copy(),
+             x["fx"].copy(), x["fy"].copy(), x["fz"].copy(), x["phi"].copy(), phin)
+        return {"phin": np.where(interior_mask(g), phin, 0.0)}
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {k: g.zeros() for k in self.x0}
+        pb = y["phin"].copy()
+        x0 = self.x0
+        call("solphi_mthinc2_b", *self.head(), x0["u"], b["u"], x0["v"], b["v"], x0["w"], b["w"],
+             x0["fx"], b["fx"], x0["fy"], b["fy"], x0["fz"], b["fz"], x0["phi"], b["phi"], pb)
+        return b
+
+
+class Mthinc3(FDCase):
+    name = "solphi_mthinc3"
+    bet = 2.0
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1000)
+        phi = H.smooth_phi(g, r, beta=1.0)
+        phin = np.clip(phi + 0.02 * r.standard_normal(g.shape), 0.0, 1.0)
+        self.grad = vof_grad(g, phin)
+        self.x0 = {"phi": phi, "phin": phin}
+
+    def inputs(self):
+        """Random directions, without phin components at cells within 1e-5 of
+        the truncation thresholds or the clamp bounds."""
+        v = super().inputs()
+        et = 0.5 * (np.tanh(-self.bet * 2.5) + 1.0)
+        p = self.x0["phin"]
+        near = (np.abs(p - et) < 1e-5) | (np.abs(1.0 - p - et) < 1e-5) | (p < 1e-5) | (p > 1 - 1e-5)
+        v["phin"] = np.where(near, 0.0, v["phin"])
+        return v
+
+    def out_adjoints(self):
+        return {"phin": self.rng.standard_normal(self.g.shape)}
+
+    def head(self):
+        g = self.g
+        return [I(0), I(g.ni), I(g.nj), I(g.nk), *dxyz(g), D(self.bet)]
+
+    def value(self, call, x):
+        g = self.g
+        phin = x["phin"].copy()
+        call("solphi_mthinc3", *self.head(), *self.grad, x["phi"].copy(), phin)
+        return {"phin": phin}
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {"phi": g.zeros(), "phin": y["phin"].copy()}
+        x0 = self.x0
+        call("solphi_mthinc3_b", *self.head(), *self.grad, x0["phi"], b["phi"], x0["phin"], b["phin"])
+        return b
+
+
+class Summation(FDCase):
+    name = "summation"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1100)
+        q = np.zeros((2,) + g.shape)
+        q[1] = r.uniform(0.1, 0.9, g.shape)
+        q[1][5, 4, 4] = 1.2
+        self.x0 = {"q": q}
+
+    def inputs(self):
+        v = self.rng.standard_normal((2,) + self.g.shape)
+        v[0] = 0.0  # slice 0 is overwritten: not an input
+        return {"q": v}
+
+    def out_adjoints(self):
+        y = np.zeros((2,) + self.g.shape)
+        y[0] = self.rng.standard_normal(self.g.shape)
+        return {"q": y}
+
+    def value(self, call, x):
+        g = self.g
+        q = x["q"].copy()
+        call("summation", I(g.ni), I(g.nj), I(g.nk), q, I(1))
+        out = np.zeros_like(q)
+        out[0] = q[0]
+        return {"q": out}
+
+    def adjoint(self, call, y):
+        g = self.g
+        qb = y["q"].copy()
+        call("summation_b", I(g.ni), I(g.nj), I(g.nk), self.x0["q"], qb, I(1))
+        assert not np.any(qb[0])
+        return {"q": qb}

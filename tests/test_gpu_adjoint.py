@@ -76,6 +76,31 @@ def test_weno5_adjoint_matches_oracle(gi):
     assert abs(lhs - rhs) <= 1e-8 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
 
 
+GRID_VOF = [Grid(16, 16, 16, 1 / 16, 1 / 16, 1 / 16), Grid(32, 16, 16, 0.05, 0.1, 0.05)]
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_grad_p2a_adjoint(gi):
+    check_case(A.GradP2a(GRIDS[gi], 201 + gi))
+
+
+@pytest.mark.parametrize("gi", range(len(GRID_VOF)))
+@pytest.mark.parametrize("cls", [A.Mthinc1, A.Mthinc2, A.Mthinc3, A.Summation], ids=lambda c: c.name)
+def test_vof_adjoint_matches_oracle(cls, gi):
+    """Nonlinear: the GPU adjoint at x0 equals the oracle's to rounding
+    (transcendentals: 1e-12), and matches central differences of the GPU
+    forward along random directions."""
+    case = cls(GRID_VOF[gi], 211 + gi)
+    print(f"adjoint case {case.name} grid {case.g.ni}x{case.g.nj}x{case.g.nk}", flush=True)
+    x, y = case.inputs(), case.out_adjoints()
+    bo = case.adjoint(H.call_oracle, {k: v.copy() for k, v in y.items()})
+    bg = case.adjoint(H.call_gpu, {k: v.copy() for k, v in y.items()})
+    assert rel_diff(bo, bg) <= 1e-12, rel_diff(bo, bg)
+    fx = case.forward(H.call_gpu, x)
+    lhs, rhs = A.dot(fx, y), A.dot(x, bg)
+    assert abs(lhs - rhs) <= 1e-7 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
+
+
 @pytest.mark.parametrize("skew", [False, True], ids=["reference_coefficients", "skewed"])
 def test_tdma2_adjoint(skew):
     check_case(A.Tdma2(GRIDS[1], 61, skew=skew))

@@ -21,6 +21,7 @@ import felharness as H
 from felharness import Grid
 
 GRIDS = [Grid(8, 6, 8, 1 / 8, 1 / 6, 1 / 8), Grid(16, 8, 4, 0.1, 0.05, 0.2)]
+GRID_VOF = [Grid(16, 16, 16, 1 / 16, 1 / 16, 1 / 16), Grid(16, 8, 16, 0.05, 0.1, 0.05)]
 
 
 def dot_check(case, call=H.call_oracle, rtol=1e-12):
@@ -55,6 +56,20 @@ def test_weno5_adjoint_against_differences(gi):
     to the differences' accuracy (the identity fails by O(1) for a wrong
     weight derivative)."""
     lhs, rhs, sc = dot_check(A.Weno(GRIDS[gi], 121 + gi), rtol=1e-8)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_dot_product_grad_p2a(gi):
+    dot_check(A.GradP2a(GRIDS[gi], 181 + gi))
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.Mthinc1, A.Mthinc2, A.Mthinc3, A.Summation], ids=lambda c: c.name)
+def test_vof_adjoints_against_differences(cls, gi):
+    """The MTHINC chain's adjoints at x0 against Richardson central
+    differences along random directions (branches, upwind directions and
+    clamps fixed at x0; the Newton offset by the implicit-function theorem)."""
+    dot_check(cls(GRID_VOF[gi], 191 + gi), rtol=1e-7)
 
 
 @pytest.mark.parametrize("skew", [False, True], ids=["reference_coefficients", "skewed"])
