@@ -628,6 +628,9 @@ const double* read_scalars(const double* dev, int n);
 // Aborts with a message naming `who` unless the library runs in the FP64
 // mode (the adjoint routines have no FP32 or tangent form).
 void adjoint_fp64_only(const char* who);
+// Adds the sum of the n values of device array a to the host scalar *out
+// (a deterministic two-stage reduction; waits for it).
+void adjoint_add_sum(const double* a, int n, double* out);
 
 // Finds this device's copy of the allocation-free solve's static work array
 // (k_sor.hip); once per device, with the device current.

@@ -171,7 +171,7 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
 }
 
 // Sum of the n values of a device array, added to *out on the host.
-static void add_sum(const double* a, int n, double* out) {
+void adjoint_add_sum(const double* a, int n, double* out) {
   double* s = dev_scalars(21);
   Dims p{};
   p.ni = n;
@@ -804,7 +804,7 @@ void bndu_b_(const int* nID, const int* ni, const int* nj, const int* nk, double
   const double* dl2 = c.in(l2, d.n3, "l2");
   const double* dphi = c.in(phi, d.n3, "phi");
   bndu_adj(d, ub, vb, wb, wacc, *dy, dl1, dl2, dphi, nID[kYMinus] < 0, nID[kYPlus] < 0);
-  add_sum(wacc, nw, uwallb);
+  adjoint_add_sum(wacc, nw, uwallb);
   c.finish();
 }
 
@@ -888,7 +888,7 @@ void solu_sor4_b_(const int* ipara, const int* ID, const int* nID, const int* nd
         if (q == 1) pair_adj<1>(onfly, d, a, p);
         if (q == 2) pair_adj<2>(onfly, d, a, p);
       }
-  add_sum(wacc, nw, uwallb);
+  adjoint_add_sum(wacc, nw, uwallb);
   c.finish();
 }
 

@@ -23,6 +23,7 @@
 #include <cstdlib>
 
 #include "felgpu.hpp"
+#include "felgpu_visc.hpp"
 
 namespace fel {
 
@@ -423,6 +424,168 @@ static void mthinc3_adj(const Dims d, const VofConst m, const double* gx, const 
   });
 }
 
+// ---------------------------------------------------------------------------
+// Contact line.
+// ---------------------------------------------------------------------------
+
+// srcs/gnbc.f90:22-95 reversed, first launch, one thread per wall column of
+// each wall: the adjoints of the contact angle's inputs theta_0 and zeta
+// (its own cell) and of the column's averaged velocities ux, uz (stored:
+// plane [wall][ux|uz][column]); theta_b consumed.  The clamp and acos pass
+// the derivative only strictly inside (-1,1).
+struct GnbcAdj {
+  const double *u, *w, *th0, *zeta;
+  double *th0b, *zetab, *thb;
+  double uwall, sigma;
+  double* UV;  // scratch: 4 planes of sx*sz
+  bool lo, hi;
+};
+
+static void gnbc_adj(const Dims d, const GnbcAdj a) {
+  const int ni = d.ni, nj = d.nj, plane = d.sx * d.sz;
+  launch_flat(2 * plane, [=] __device__(int m) {
+    const int side = m / plane, col = m - side * plane;
+    double* uxb = a.UV + (2 * side) * plane;
+    double* uzb = a.UV + (2 * side + 1) * plane;
+    uxb[col] = 0.0;
+    uzb[col] = 0.0;
+    if (side == 0 ? !a.lo : !a.hi) return;
+    const int i = col % d.sx - 2, k = col / d.sx - 2;
+    const int jw = side ? nj : 1, ja = side ? nj : 0, jb = side ? nj + 1 : 1;
+    const int mi = (i + ni + 2) % (ni + 3);
+    const double pi = acos(-1.0), eps = 1.0e-12;
+    const double ux = (a.u[d.ix(mi, ja, k)] + a.u[d.ix(mi, jb, k)] + a.u[d.ix(i, ja, k)] +
+                       a.u[d.ix(i, jb, k)]) / 4.0 + (side ? -a.uwall : a.uwall);
+    const double uz = (a.w[d.ix(mi, ja, k)] + a.w[d.ix(mi, jb, k)] + a.w[d.ix(i, ja, k)] +
+                       a.w[d.ix(i, jb, k)]) / 4.0;
+    const double R = sqrt(ux * ux + uz * uz + eps);
+    double sgn = ux >= 0.0 ? 1.0 : -1.0;
+    if (i <= ni / 2) sgn = -sgn;
+    const double ucl = sgn * R;
+    const int cw = d.ix(i, jw, k);
+    const double th0 = a.th0[cw] * (pi / 180.0), zeta = a.zeta[cw];
+    const double ct = cos(th0) - zeta * ucl / a.sigma;
+    const double tb = a.thb[cw];
+    a.thb[cw] = 0.0;
+    if (!(ct > -1.0 && ct < 1.0)) return;
+    const double ctb = tb * (180.0 / pi) * (-1.0 / sqrt(1.0 - ct * ct));
+    a.th0b[cw] = a.th0b[cw] + ctb * (-sin(th0)) * (pi / 180.0);
+    a.zetab[cw] = a.zetab[cw] + ctb * (-ucl / a.sigma);
+    const double uclb = ctb * (-zeta / a.sigma);
+    uxb[col] = uclb * sgn * ux / R;
+    uzb[col] = uclb * sgn * uz / R;
+  });
+}
+
+// Second launch: each velocity of the two rows either side of a wall
+// gathers a quarter of ux_b, uz_b of the columns whose averages read it
+// (its own column i, and the columns i' with mod(i'+ni+2, ni+3) = i).
+static void gnbc_gather(const Dims d, const GnbcAdj a, double* ub, double* wb) {
+  const int ni = d.ni, nj = d.nj, sx = d.sx, plane = d.sx * d.sz;
+  launch_flat(4 * plane, [=] __device__(int m) {
+    const int side = m / (2 * plane), r = (m / plane) % 2, col = m % plane;
+    if (side == 0 ? !a.lo : !a.hi) return;
+    const int x = col % sx - 2, k = col / sx - 2;
+    const int j = side ? (r ? nj + 1 : nj) : (r ? 1 : 0);
+    const double* uxb = a.UV + (2 * side) * plane;
+    const double* uzb = a.UV + (2 * side + 1) * plane;
+    double su = uxb[col], sw = uzb[col];
+    const int cand[2] = {x + 1, x - ni - 2};
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int i = cand[t];
+      if (i < -2 || i > ni + 3 || (i + ni + 2) % (ni + 3) != x) continue;
+      const int ci = (i + 2) + sx * (k + 2);
+      su = su + uxb[ci];
+      sw = sw + uzb[ci];
+    }
+    const int c = d.ix(x, j, k);
+    ub[c] = ub[c] + su / 4.0;
+    wb[c] = wb[c] + sw / 4.0;
+  });
+}
+
+// R = n_y/(n_t + 1e-20) of the contact-angle normal and dR/dtheta (theta in
+// degrees), t = x (comp 0) or z (comp 2): n = (s sin t ax, cos t,
+// s sin t az)/sqrt(|.|^2 + 1e-20) (srcs/bnd_contact_angle.f90:25-35).
+__device__ __forceinline__ double ca_ratio(double th_deg, double sg, double ax, double az, int comp,
+                                           double* dr) {
+  const double pi = acos(-1.0), eps = 1.0e-20;
+  const double th = th_deg * pi / 180.0, ct = cos(th), st = sin(th);
+  const double rx = sg * st * ax, ry = ct, rz = sg * st * az;
+  const double N = sqrt(rx * rx + ry * ry + rz * rz + eps);
+  const double drx = sg * ct * ax, dry = -st, drz = sg * ct * az;
+  const double dN = (rx * drx + ry * dry + rz * drz) / N;
+  const double ny = ry / N, dny = (dry * N - ry * dN) / (N * N);
+  const double rt = comp == 0 ? rx : rz, drt = comp == 0 ? drx : drz;
+  const double nt = rt / N, dnt = (drt * N - rt * dN) / (N * N);
+  const double b = nt + eps;
+  *dr = (dny * b - ny * dnt) / (b * b) * (pi / 180.0);
+  return ny / b;
+}
+
+// srcs/bnd_contact_angle.f90:17-72 reversed, first launch, one thread per
+// wall column: the ghost rows' adjoints (consumed) folded into the wall
+// value's own part Q and the neighbour weights A (x) and B (z), stored per
+// column; theta_b gathered in place.  Second launch: each wall-row value
+// gathers Q of its column and A, B of its x and z neighbours (values outside
+// the padded range are read as 0 by the forward and receive nothing).
+static void contact_angle_adj(const Dims d, bool lo, bool hi, double ax, double az,
+                              const double* q, double* qb, const double* th, double* thb,
+                              double* S) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, plane = d.sx * d.sz;
+  launch_flat(2 * plane, [=] __device__(int m) {
+    const int side = m / plane, col = m - side * plane;
+    double* Q = S + (3 * side) * plane;
+    double* Aw = S + (3 * side + 1) * plane;
+    double* Bw = S + (3 * side + 2) * plane;
+    Q[col] = 0.0;
+    Aw[col] = 0.0;
+    Bw[col] = 0.0;
+    if (side == 0 ? !lo : !hi) return;
+    const int i = col % sx - 2, k = col / sx - 2;
+    const int jw = side == 0 ? 1 : nj, jg = side == 0 ? 0 : nj + 1, js = side == 0 ? -1 : 1;
+    const double sg = copysign(1.0, (double)(i - ni / 2));
+    const int cw = d.ix(i, jw, k);
+    double dRx, dRz;
+    const double Rx = ca_ratio(th[cw], sg, ax, az, 0, &dRx);
+    const double Rz = ca_ratio(th[cw], sg, ax, az, 2, &dRz);
+    const double qp = i + 1 > ni + 3 ? 0.0 : q[d.ix(i + 1, jw, k)];
+    const double qm = i - 1 < -2 ? 0.0 : q[d.ix(i - 1, jw, k)];
+    const double qt = k + 1 > nk + 3 ? 0.0 : q[d.ix(i, jw, k + 1)];
+    const double qd = k - 1 < -2 ? 0.0 : q[d.ix(i, jw, k - 1)];
+    const double Dx = (qp - qm) * 0.5 * ax, Dz = (qt - qd) * 0.5 * az;
+    const int g0 = d.ix(i, jg, k), g1 = d.ix(i, jg + js, k), g2 = d.ix(i, jg + 2 * js, k);
+    double l2 = qb[g2], l1 = qb[g1], l0 = qb[g0];
+    qb[g2] = 0.0;
+    qb[g1] = 0.0;
+    qb[g0] = 0.0;
+    l1 = l1 + 2.0 * l2;
+    l0 = l0 - l2;
+    l0 = l0 + 2.0 * l1;
+    Q[col] = l0 - l1;
+    Aw[col] = l0 * Rx * 0.5 * ax;
+    Bw[col] = l0 * Rz * 0.5 * az;
+    thb[cw] = thb[cw] + l0 * (-Dx * dRx - Dz * dRz);
+  });
+  launch_flat(2 * plane, [=] __device__(int m) {
+    const int side = m / plane, col = m - side * plane;
+    if (side == 0 ? !lo : !hi) return;
+    const double* Q = S + (3 * side) * plane;
+    const double* Aw = S + (3 * side + 1) * plane;
+    const double* Bw = S + (3 * side + 2) * plane;
+    const int x = col % sx - 2, k = col / sx - 2;
+    const int jw = side == 0 ? 1 : nj;
+    double t = Q[col];
+    if (x - 1 >= -2) t = t - Aw[col - 1];
+    if (x + 1 <= ni + 3) t = t + Aw[col + 1];
+    if (k - 1 >= -2) t = t - Bw[col - sx];
+    if (k + 1 <= nk + 3) t = t + Bw[col + sx];
+    const int c = d.ix(x, jw, k);
+    qb[c] = qb[c] + t;
+  });
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -577,6 +740,61 @@ void summation_b_(const int* ni, const int* nj, const int* nk, const double* qk,
     b[c0] = 0.0;
     for (int l = 1; l <= nb; l++) b[(size_t)l * n3 + c0] = b[(size_t)l * n3 + c0] + g;
   });
+  c.finish();
+}
+
+// Adjoint of gnbc_ (srcs/gnbc.f90:22-95): u, w, uwall, theta_0_array and
+// zeta_array each followed by its adjoint (incremented); theta_arrayb
+// consumed on the wall rows.
+void gnbc_b_(const int* nID, const int* ni, const int* nj, const int* nk, const double* uk,
+             double* ukb, const double* wk, double* wkb, const double* uwall, double* uwallb,
+             const double* theta_0_array, double* theta_0b, const double* surface_tension,
+             const double* zeta_array, double* zetab, double* theta_arrayb) {
+  adjoint_fp64_only("gnbc_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("gnbc_b_");
+  GnbcAdj a;
+  a.u = c.in(uk, d.n3, "uk");
+  a.w = c.in(wk, d.n3, "wk");
+  a.th0 = c.in(theta_0_array, d.n3, "theta_0_array");
+  a.zeta = c.in(zeta_array, d.n3, "zeta_array");
+  double* ub = c.io(ukb, d.n3, "ukb");
+  double* wb = c.io(wkb, d.n3, "wkb");
+  a.th0b = c.io(theta_0b, d.n3, "theta_0b");
+  a.zetab = c.io(zetab, d.n3, "zetab");
+  a.thb = c.io(theta_arrayb, d.n3, "theta_arrayb");
+  a.uwall = *uwall;
+  a.sigma = *surface_tension;
+  a.lo = nID[kYMinus] < 0;
+  a.hi = nID[kYPlus] < 0;
+  const int plane = d.sx * d.sz;
+  a.UV = static_cast<double*>(scratch(39, sizeof(double) * 4 * (size_t)plane));
+  gnbc_adj(d, a);
+  gnbc_gather(d, a, ub, wb);
+  // uwall_b: + the bottom wall's ux_b, - the top wall's
+  double* t = static_cast<double*>(scratch(40, sizeof(double) * (size_t)plane));
+  const double* U = a.UV;
+  launch_flat(plane, [=] __device__(int m) { t[m] = U[m] - U[2 * plane + m]; });
+  adjoint_add_sum(t, plane, uwallb);
+  c.finish();
+}
+
+// Adjoint of bnd_contact_angle_ (srcs/bnd_contact_angle.f90:17-72): qk is
+// the forward's input, qkb the adjoint of its output on entry and of its
+// input on return; theta_array followed by its adjoint (incremented).
+void bnd_contact_angle_b_(const int* nID, const int* ni, const int* nj, const int* nk,
+                          const double* qk, double* qkb, const double* theta_array,
+                          double* theta_arrayb, const double* dx, const double* dy,
+                          const double* dz) {
+  adjoint_fp64_only("bnd_contact_angle_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("bnd_contact_angle_b_");
+  const double* q = c.in(qk, d.n3, "qk");
+  double* qb = c.io(qkb, d.n3, "qkb");
+  const double* th = c.in(theta_array, d.n3, "theta_array");
+  double* thb = c.io(theta_arrayb, d.n3, "theta_arrayb");
+  double* S = static_cast<double*>(scratch(41, sizeof(double) * 6 * (size_t)d.sx * d.sz));
+  contact_angle_adj(d, nID[kYMinus] < 0, nID[kYPlus] < 0, *dy / *dx, *dy / *dz, q, qb, th, thb, S);
   c.finish();
 }
 

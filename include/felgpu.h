@@ -706,6 +706,23 @@ void solphi_mthinc3_b_(const int *ipara, const int *ni, const int *nj, const int
 void summation_b_(const int *ni, const int *nj, const int *nk, const double *qk, double *qkb,
                   const int *nbub);
 
+/* srcs/gnbc.f90:22-95 reversed: uk, wk, uwall, theta_0_array, zeta_array
+ * each followed by its adjoint (incremented); theta_arrayb consumed on the
+ * wall rows.  The clamp and acos pass the derivative strictly inside
+ * (-1,1). */
+void gnbc_b_(const int *nID, const int *ni, const int *nj, const int *nk, const double *uk,
+             double *ukb, const double *wk, double *wkb, const double *uwall, double *uwallb,
+             const double *theta_0_array, double *theta_0b, const double *surface_tension,
+             const double *zeta_array, double *zetab, double *theta_arrayb);
+
+/* srcs/bnd_contact_angle.f90:17-72 reversed: qk the forward's input, qkb the
+ * adjoint of its output on entry and of its input on return; theta_array
+ * followed by its adjoint (incremented). */
+void bnd_contact_angle_b_(const int *nID, const int *ni, const int *nj, const int *nk,
+                          const double *qk, double *qkb, const double *theta_array,
+                          double *theta_arrayb, const double *dx, const double *dy,
+                          const double *dz);
+
 /* How the last solu_sor4_b_ call obtained its coefficients: 1 formed on the
  * fly from the viscosity, 0 read from the 45 arrays, -1 before any call. */
 int felgpu_sor_b_onfly_(void);

@@ -1208,3 +1208,119 @@ void orc_summation_b(const int *ni_, const int *nj_, const int *nk_, const doubl
     for (int l = 1; l <= nbub; l++) qkb[(ptrdiff_t)l * n3 + m] += g;
   }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Contact line                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* srcs/gnbc.f90:22-95 reversed: theta = acos(clamp(cos(theta_0) -
+ * zeta u_cl/sigma)) in degrees at the wall rows, u_cl = +-sign(ux) sqrt(ux^2 +
+ * uz^2 + 1e-12) from 4-point averages of u (+-uwall) and w.  The clamp and
+ * acos pass the derivative only strictly inside (-1,1).  theta_arrayb is
+ * consumed on the two wall rows. */
+void orc_gnbc_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, const double *uk,
+                double *ukb, const double *wk, double *wkb, const double *uwall_, double *uwallb,
+                const double *theta_0_array, double *theta_0b, const double *surface_tension,
+                const double *zeta_array, double *zetab, double *theta_arrayb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const double pi = acos(-1.0), eps = 1.0e-12, sigma = *surface_tension, uwall = *uwall_;
+  for (int side = 1; side >= 0; side--) {
+    if (side == 0 && !(nID[ORC_Y_MINUS - 1] < 0)) continue;
+    if (side == 1 && !(nID[ORC_Y_PLUS - 1] < 0)) continue;
+    const int jw = side ? nj : 1, ja = side ? nj : 0, jb = side ? nj + 1 : 1;
+    const double uws = side ? -uwall : uwall;
+    for (int k = nk + 3; k >= -2; k--)
+      for (int i = ni + 3; i >= -2; i--) {
+        const int mi = (i + ni + 2) % (ni + 3);
+        const double ux = (uk[OIX(mi, ja, k)] + uk[OIX(mi, jb, k)] + uk[OIX(i, ja, k)] +
+                           uk[OIX(i, jb, k)]) / 4.0 + uws;
+        const double uz = (wk[OIX(mi, ja, k)] + wk[OIX(mi, jb, k)] + wk[OIX(i, ja, k)] +
+                           wk[OIX(i, jb, k)]) / 4.0;
+        const double R = sqrt(ux * ux + uz * uz + eps);
+        double sgn = ux >= 0.0 ? 1.0 : -1.0;
+        if (i <= ni / 2) sgn = -sgn;
+        const double ucl = sgn * R;
+        const double th0 = theta_0_array[OIX(i, jw, k)] * (pi / 180.0);
+        const double zeta = zeta_array[OIX(i, jw, k)];
+        const double ct = cos(th0) - zeta * ucl / sigma;
+        const ptrdiff_t cw = OIX(i, jw, k);
+        const double tb = theta_arrayb[cw];
+        theta_arrayb[cw] = 0.0;
+        if (!(ct > -1.0 && ct < 1.0)) continue;
+        const double ctb = tb * (180.0 / pi) * (-1.0 / sqrt(1.0 - ct * ct));
+        theta_0b[cw] += ctb * (-sin(th0)) * (pi / 180.0);
+        zetab[cw] += ctb * (-ucl / sigma);
+        const double uclb = ctb * (-zeta / sigma);
+        const double uxb = uclb * sgn * ux / R, uzb = uclb * sgn * uz / R;
+        ukb[OIX(mi, ja, k)] += uxb / 4.0; ukb[OIX(mi, jb, k)] += uxb / 4.0;
+        ukb[OIX(i, ja, k)] += uxb / 4.0; ukb[OIX(i, jb, k)] += uxb / 4.0;
+        wkb[OIX(mi, ja, k)] += uzb / 4.0; wkb[OIX(mi, jb, k)] += uzb / 4.0;
+        wkb[OIX(i, ja, k)] += uzb / 4.0; wkb[OIX(i, jb, k)] += uzb / 4.0;
+        *uwallb += side ? -uxb : uxb;
+      }
+  }
+}
+
+/* Derivative in theta (degrees) of R = n_y/(n_t + 1e-20) of the contact-angle
+ * normal n = (s sin t ax, cos t, s sin t az)/|.| (t in radians, the norm with
+ * +1e-20 inside the root), t = the tangential component x (comp 0) or z (2). */
+static double ca_ratio(double th_deg, double sg, double ax, double az, int comp, double *dr) {
+  const double pi = acos(-1.0), eps = 1.0e-20;
+  const double th = th_deg * pi / 180.0, ct = cos(th), st = sin(th);
+  const double rx = sg * st * ax, ry = ct, rz = sg * st * az;
+  const double N = sqrt(rx * rx + ry * ry + rz * rz + eps);
+  const double drx = sg * ct * ax, dry = -st, drz = sg * ct * az;
+  const double dN = (rx * drx + ry * dry + rz * drz) / N;
+  const double ny = ry / N, dny = (dry * N - ry * dN) / (N * N);
+  const double rt = comp == 0 ? rx : rz, drt = comp == 0 ? drx : drz;
+  const double nt = rt / N, dnt = (drt * N - rt * dN) / (N * N);
+  const double b = nt + eps;
+  *dr = (dny * b - ny * dnt) / (b * b) * (pi / 180.0);
+  return ny / b;
+}
+
+/* srcs/bnd_contact_angle.f90:17-72 reversed (orc_bnd_contact_angle): the
+ * three ghost rows of each wall column, from the wall row's value, its x and
+ * z neighbours (outside the padded range read as 0) and theta.  qk holds the
+ * forward's input; qkb the adjoint of its output on entry, of its input on
+ * return; theta_arrayb incremented. */
+void orc_bnd_contact_angle_b(const int *nID, const int *ni_, const int *nj_, const int *nk_,
+                             const double *qk, double *qkb, const double *theta_array,
+                             double *theta_arrayb, const double *dx_, const double *dy_,
+                             const double *dz_) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const double ax = *dy_ / *dx_, az = *dy_ / *dz_;
+  for (int side = 1; side >= 0; side--) {
+    if (side == 0 && !(nID[ORC_Y_MINUS - 1] < 0)) continue;
+    if (side == 1 && !(nID[ORC_Y_PLUS - 1] < 0)) continue;
+    const int jw = side == 0 ? 1 : nj, jg = side == 0 ? 0 : nj + 1, js = side == 0 ? -1 : 1;
+    for (int k = nk + 3; k >= -2; k--)
+      for (int i = ni + 3; i >= -2; i--) {
+        const double sg = copysign(1.0, (double)(i - ni / 2));
+        double dRx, dRz;
+        const double Rx = ca_ratio(theta_array[OIX(i, jw, k)], sg, ax, az, 0, &dRx);
+        const double Rz = ca_ratio(theta_array[OIX(i, jw, k)], sg, ax, az, 2, &dRz);
+        const double qp = (i + 1 > ni + 3) ? 0.0 : qk[OIX(i + 1, jw, k)];
+        const double qm = (i - 1 < -2) ? 0.0 : qk[OIX(i - 1, jw, k)];
+        const double qt = (k + 1 > nk + 3) ? 0.0 : qk[OIX(i, jw, k + 1)];
+        const double qb = (k - 1 < -2) ? 0.0 : qk[OIX(i, jw, k - 1)];
+        const double Dx = (qp - qm) * 0.5 * ax, Dz = (qt - qb) * 0.5 * az;
+        /* g2 = 2 g1 - g0, g1 = 2 g0 - q1, g0 = q1 - Rx Dx - Rz Dz */
+        double l2 = qkb[OIX(i, jg + 2 * js, k)], l1 = qkb[OIX(i, jg + js, k)], l0 = qkb[OIX(i, jg, k)];
+        qkb[OIX(i, jg + 2 * js, k)] = 0.0;
+        qkb[OIX(i, jg + js, k)] = 0.0;
+        qkb[OIX(i, jg, k)] = 0.0;
+        l1 += 2.0 * l2;
+        l0 -= l2;
+        l0 += 2.0 * l1;
+        qkb[OIX(i, jw, k)] += l0 - l1;
+        if (i + 1 <= ni + 3) qkb[OIX(i + 1, jw, k)] -= l0 * Rx * 0.5 * ax;
+        if (i - 1 >= -2) qkb[OIX(i - 1, jw, k)] += l0 * Rx * 0.5 * ax;
+        if (k + 1 <= nk + 3) qkb[OIX(i, jw, k + 1)] -= l0 * Rz * 0.5 * az;
+        if (k - 1 >= -2) qkb[OIX(i, jw, k - 1)] += l0 * Rz * 0.5 * az;
+        theta_arrayb[OIX(i, jw, k)] += l0 * (-Dx * dRx - Dz * dRz);
+      }
+  }
+}

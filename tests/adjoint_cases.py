@@ -925,3 +925,76 @@ class Summation(FDCase):
         call("summation_b", I(g.ni), I(g.nj), I(g.nk), self.x0["q"], qb, I(1))
         assert not np.any(qb[0])
         return {"q": qb}
+
+
+# --------------------------------------------------------------------------
+# Contact line
+# --------------------------------------------------------------------------
+
+def wall_rows_mask(g: Grid):
+    m = np.zeros(g.shape, bool)
+    m[:, 3, :] = True          # j = 1
+    m[:, g.nj + 2, :] = True   # j = nj
+    return m
+
+
+class Gnbc(FDCase):
+    name = "gnbc"
+    sigma = 5.5
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1200)
+        self.x0 = {"u": away_from_zero(r, g.shape, 0.5), "w": r.standard_normal(g.shape),
+                   "uwall": np.array([0.13]), "theta0": r.uniform(40.0, 140.0, g.shape),
+                   "zeta": r.uniform(0.5, 2.0, g.shape)}
+
+    def out_adjoints(self):
+        return {"theta": np.where(wall_rows_mask(self.g), self.rng.standard_normal(self.g.shape), 0.0)}
+
+    def value(self, call, x):
+        g = self.g
+        th = g.zeros()
+        call("gnbc", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), x["u"].copy(), x["w"].copy(),
+             D(float(x["uwall"][0])), x["theta0"].copy(), D(self.sigma), x["zeta"].copy(), th)
+        return {"theta": np.where(wall_rows_mask(g), th, 0.0)}
+
+    def adjoint(self, call, y):
+        g = self.g
+        b = {k: g.zeros() for k in ("u", "w", "theta0", "zeta")}
+        ub = D(0.0)
+        tb = y["theta"].copy()
+        x0 = self.x0
+        call("gnbc_b", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), x0["u"], b["u"], x0["w"], b["w"],
+             D(float(x0["uwall"][0])), ub, x0["theta0"], b["theta0"], D(self.sigma), x0["zeta"],
+             b["zeta"], tb)
+        assert not np.any(tb[wall_rows_mask(g)])
+        b["uwall"] = np.array([ub.value])
+        return b
+
+
+class ContactAngle(FDCase):
+    name = "bnd_contact_angle"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1300)
+        self.x0 = {"q": H.smooth_phi(g, r), "theta": r.uniform(30.0, 150.0, g.shape)}
+
+    def out_adjoints(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def value(self, call, x):
+        g = self.g
+        q = x["q"].copy()
+        call("bnd_contact_angle", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), q, x["theta"].copy(),
+             D(g.dx), D(g.dy), D(g.dz))
+        return {"q": q}
+
+    def adjoint(self, call, y):
+        g = self.g
+        qb, tb = y["q"].copy(), g.zeros()
+        x0 = self.x0
+        call("bnd_contact_angle_b", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), x0["q"], qb,
+             x0["theta"], tb, D(g.dx), D(g.dy), D(g.dz))
+        return {"q": qb, "theta": tb}

@@ -85,7 +85,8 @@ def test_grad_p2a_adjoint(gi):
 
 
 @pytest.mark.parametrize("gi", range(len(GRID_VOF)))
-@pytest.mark.parametrize("cls", [A.Mthinc1, A.Mthinc2, A.Mthinc3, A.Summation], ids=lambda c: c.name)
+@pytest.mark.parametrize("cls", [A.Mthinc1, A.Mthinc2, A.Mthinc3, A.Summation, A.Gnbc,
+                                 A.ContactAngle], ids=lambda c: c.name)
 def test_vof_adjoint_matches_oracle(cls, gi):
     """Nonlinear: the GPU adjoint at x0 equals the oracle's to rounding
     (transcendentals: 1e-12), and matches central differences of the GPU

@@ -64,7 +64,8 @@ def test_dot_product_grad_p2a(gi):
 
 
 @pytest.mark.parametrize("gi", range(len(GRIDS)))
-@pytest.mark.parametrize("cls", [A.Mthinc1, A.Mthinc2, A.Mthinc3, A.Summation], ids=lambda c: c.name)
+@pytest.mark.parametrize("cls", [A.Mthinc1, A.Mthinc2, A.Mthinc3, A.Summation, A.Gnbc,
+                                 A.ContactAngle], ids=lambda c: c.name)
 def test_vof_adjoints_against_differences(cls, gi):
     """The MTHINC chain's adjoints at x0 against Richardson central
     differences along random directions (branches, upwind directions and
