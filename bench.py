@@ -314,8 +314,63 @@ def measure_adjoint(cfg, warmup, reps, step_ms):
             "reverse_vs_forward": round(rev / fwd, 3),
             "reverse_vs_fp64_step": round(rev / step_ms, 3),
             "sor_sweeps": min(lp + 1, 21), "reps": reps,
-            "held_fixed": "volume fraction and its transport (MTHINC), surface tension, "
-                          "contact line, densities and viscosities (no adjoint yet)"}
+            "held_fixed": "volume fraction (its transport is reversed by the phase leg), "
+                          "surface tension, densities and viscosities (no adjoint yet)"}
+
+
+def measure_phase_adjoint(cfg, warmup, reps, step_ms):
+    """The reverse of the volume-fraction update at a fixed velocity
+    (felmpf.adjoint.PhaseStep, device-resident) against its forward, on the
+    state and wall data of a run after `warmup` steps, result reuse off.
+    The reverse timing includes restoring the output adjoint (device
+    copies of both slices)."""
+    import felmpf
+    from felmpf import _lib, adjoint
+    gpu = _lib.gpu()
+    run = felmpf.GpuRun(cfg)
+    run.steps(warmup)
+    params = {"u": run.field("u"), "v": run.field("v"), "w": run.field("w"),
+              "theta0": run.field("theta_0_array"), "zeta": run.field("zeta_array")}
+    phi = run.field("phi1")
+    run.close()
+    reuse_before = int(gpu.felgpu_result_reuse_())
+    gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(0)))
+    step = adjoint.PhaseStep(cfg, params, resident=True)
+    try:
+        rng = np.random.default_rng(8)
+        ybuf = step.array(rng.standard_normal((2,) + cfg.shape))
+        g3 = [adjoint.I(cfg.ni), adjoint.I(cfg.nj), adjoint.I(cfg.nk)]
+        step.set_state({"phi": phi})
+        step.forward_set()
+        gpu.felgpu_sync_()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step.forward_set()
+        gpu.felgpu_sync_()
+        t1 = time.perf_counter()
+        for l in (0, 1):
+            adjoint.gpu_call("cpy", *g3, ybuf[l], step.b["phin"][l])
+        step.reverse_set()
+        gpu.felgpu_sync_()
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            for l in (0, 1):
+                adjoint.gpu_call("cpy", *g3, ybuf[l], step.b["phin"][l])
+            step.reverse_set()
+        gpu.felgpu_sync_()
+        t3 = time.perf_counter()
+    finally:
+        step.close()
+        gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(reuse_before)))
+    fwd, rev = (t1 - t0) / reps * 1e3, (t3 - t2) / reps * 1e3
+    return {"scope": "reverse of the volume-fraction update at a fixed velocity "
+                     "(felmpf.adjoint.PhaseStep: vertex gradients, MTHINC fluxes with the "
+                     "Newton offset by the implicit-function theorem, update, truncation and "
+                     "mass correction, gnbc, contact-angle ghosts, fills, summation) through "
+                     "the *_b_ entry points; gives d/d(phi, u, v, w, theta_0, zeta, uwall)",
+            "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
+            "reverse_vs_forward": round(rev / fwd, 3),
+            "reverse_vs_fp64_step": round(rev / step_ms, 3), "reps": reps}
 
 
 def main():
@@ -396,7 +451,9 @@ def main():
                            "element)"}
     adjoint_leg = None
     if both:
-        adjoint_leg = measure_adjoint(cfg, args.warmup, 3, main_run["elapsed"] / args.steps * 1e3)
+        step_ms = main_run["elapsed"] / args.steps * 1e3
+        adjoint_leg = {"momentum": measure_adjoint(cfg, args.warmup, 3, step_ms),
+                       "phase": measure_phase_adjoint(cfg, args.warmup, 3, step_ms)}
 
     timers = main_run["timers"]
     ms_sor, n_sor = timers["sor_pair"]

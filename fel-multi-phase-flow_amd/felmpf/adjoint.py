@@ -343,3 +343,156 @@ class MomentumWindow:
             sb = nb
         sb["uwall"] = np.array([uw])
         return sb
+
+
+# ---------------------------------------------------------------------------
+# The volume-fraction transport at a fixed velocity
+# ---------------------------------------------------------------------------
+
+PHASE_PARAMS = ("u", "v", "w", "theta0", "zeta")
+
+
+class PhaseStep(MomentumStep):
+    """The phase half of one step (srcs/main.f90:489-517, one phase slice):
+    vertex gradients, MTHINC fluxes, update, truncation and mass
+    correction, contact angle (gnbc) and its ghosts, periodic fills,
+    summation into slice 0 and its fills -- forward and reversed, with the
+    velocity held fixed.  The step maps phi(.,1) (the whole padded array;
+    the forward also reads the ghost layer of phin(.,1), which holds the
+    previous step's copy of phi(.,1)) to phin(.,0:1); its adjoint gives the
+    adjoints of phi(.,1) and of the parameters u, v, w, theta_0_array,
+    zeta_array and uwall.  The third call site's contact-angle fill of
+    phi(.,0) (srcs/main.f90:509) feeds only the material properties, which
+    are held fixed, and is not part of the map."""
+
+    def __init__(self, cfg: RunConfig, params: dict, call=None, resident: bool = False):
+        self.cfg = cfg
+        self.call = call or gpu_call
+        self.resident = resident
+        self._registered = []
+        g = cfg
+        z = lambda: self.array(np.zeros(g.shape))  # noqa: E731
+        self.par = {k: self.array(np.ascontiguousarray(params[k], dtype=np.float64))
+                    for k in PHASE_PARAMS}
+        self.dyinv = np.full(g.nj + 6, 1.0 / g.dy)
+        self.a = {k: z() for k in ("phi", "gx", "gy", "gz", "hx", "hy", "hz", "fx", "fy", "fz",
+                                   "theta", "pre3", "preca")}
+        self.a["phin"] = self.array(np.zeros((2,) + g.shape))
+        self.b = {k: z() for k in ("phi", "gx", "gy", "gz", "fx", "fy", "fz", "theta") + PHASE_PARAMS}
+        self.b["phin"] = self.array(np.zeros((2,) + g.shape))
+
+    def _slice(self, arr, l):
+        return arr[l]
+
+    def set_state(self, state: dict):
+        self._set(self.a["phi"], state["phi"])
+
+    def forward_set(self):
+        c, a, p, g = self.call, self.a, self.par, self.cfg
+        gh, dd = self._g(), self._d()
+        grad_head = [I(0), I(g.nj), *gh, D(1 / g.dx), self.dyinv, D(1 / g.dz)]
+        pn1 = a["phin"][1]
+        c("cpy", *gh, a["phi"], pn1)  # the previous step's copy phi = phin
+        c("cal_grad_p2a", *grad_head, a["phi"], a["gx"], a["gy"], a["gz"])
+        c("solphi_mthinc1", *gh, *dd, D(g.dt), D(g.bet), a["phi"], a["gx"], a["gy"], a["gz"],
+          p["u"], p["v"], p["w"], a["fx"], a["fy"], a["fz"])
+        c("solphi_mthinc2", *gh, *dd, D(g.dt), p["u"], p["v"], p["w"], a["fx"], a["fy"], a["fz"],
+          a["phi"], pn1)
+        c("cal_grad_p2a", *grad_head, pn1, a["hx"], a["hy"], a["hz"])
+        c("cpy", *gh, pn1, a["pre3"])
+        c("solphi_mthinc3", I(0), *gh, *dd, D(g.bet), a["hx"], a["hy"], a["hz"], a["phi"], pn1)
+        c("gnbc", WALLS, *gh, p["u"], p["w"], D(g.uwall), p["theta0"], D(g.sigma), p["zeta"],
+          a["theta"])
+        c("cpy", *gh, pn1, a["preca"])
+        c("bnd_contact_angle", WALLS, *gh, pn1, a["theta"], D(g.dx), D(g.dy), D(g.dz))
+        c("bnd_periodic", *gh, pn1)
+        c("summation", *gh, a["phin"], I(1))
+        c("bnd_neumann", WALLS, *gh, a["phin"][0])
+        c("bnd_periodic", *gh, a["phin"][0])
+        self.lp = 0
+
+    def forward(self, state: dict, keep: bool = True) -> dict:
+        self.set_state(state)
+        self.forward_set()
+        if not keep:
+            return {}
+        return {"phin": self._get(self.a["phin"])}
+
+    def reverse(self, out_b: dict) -> dict:
+        """Adjoints of phi(.,1) and of the parameters from those of
+        phin(.,0:1), at the state of the last forward call."""
+        self._set(self.b["phin"], out_b["phin"])
+        uwallb = self.reverse_set()
+        out = {"phi": self._get(self.b["phi"])}
+        for k in PHASE_PARAMS:
+            out[k] = self._get(self.b[k])
+        out["uwall"] = np.array([uwallb])
+        return out
+
+    def reverse_set(self) -> float:
+        c, a, b, p, g = self.call, self.a, self.b, self.par, self.cfg
+        gh, dd = self._g(), self._d()
+        grad_head = [I(0), I(g.nj), *gh, D(1 / g.dx), self.dyinv, D(1 / g.dz)]
+        for k in ("phi", "theta") + PHASE_PARAMS:
+            c("init_q", *gh, b[k], b["gx"], b["gy"])
+        c("init_q", *gh, b["gz"], b["fx"], b["fy"])
+        c("init_q", *gh, b["fz"], b["gx"], b["gy"])
+        uwallb = D(0.0)
+        pn0b, pn1b = b["phin"][0], b["phin"][1]
+        c("bnd_periodic_b", *gh, pn0b)
+        c("bnd_neumann_b", WALLS, *gh, pn0b)
+        c("summation_b", *gh, a["phin"], b["phin"], I(1))
+        c("bnd_periodic_b", *gh, pn1b)
+        c("bnd_contact_angle_b", WALLS, *gh, a["preca"], pn1b, a["theta"], b["theta"], D(g.dx),
+          D(g.dy), D(g.dz))
+        c("gnbc_b", WALLS, *gh, p["u"], b["u"], p["w"], b["w"], D(g.uwall), uwallb, p["theta0"],
+          b["theta0"], D(g.sigma), p["zeta"], b["zeta"], b["theta"])
+        c("solphi_mthinc3_b", I(0), *gh, *dd, D(g.bet), a["hx"], a["hy"], a["hz"], a["phi"],
+          b["phi"], a["pre3"], pn1b)
+        c("solphi_mthinc2_b", *gh, *dd, D(g.dt), p["u"], b["u"], p["v"], b["v"], p["w"], b["w"],
+          a["fx"], b["fx"], a["fy"], b["fy"], a["fz"], b["fz"], a["phi"], b["phi"], pn1b)
+        c("solphi_mthinc1_b", *gh, *dd, D(g.dt), D(g.bet), a["phi"], b["phi"], a["gx"], b["gx"],
+          a["gy"], b["gy"], a["gz"], b["gz"], p["u"], b["u"], p["v"], b["v"], p["w"], b["w"],
+          b["fx"], b["fy"], b["fz"])
+        c("cal_grad_p2a_b", *grad_head, b["phi"], b["gx"], b["gy"], b["gz"])
+        # phin(.,1) started as the copy of phi: its unwritten (ghost) part
+        c("cpy_b", *gh, b["phi"], pn1b)
+        return uwallb.value
+
+
+class PhaseWindow:
+    """`steps` phase updates at a fixed velocity (phi(.,1) <- phin(.,1)
+    between steps) and their reverse, recomputing each step from its stored
+    input.  reverse() takes the adjoint of the last step's phin(.,0:1) and
+    returns those of the first step's phi(.,1) and of the parameters,
+    accumulated over the window."""
+
+    def __init__(self, step: PhaseStep, steps: int):
+        self.step = step
+        self.steps = steps
+        self.states = []
+
+    def forward(self, phi1: np.ndarray) -> np.ndarray:
+        st = np.array(phi1, dtype=np.float64)
+        self.states = []
+        out = None
+        for _ in range(self.steps):
+            self.states.append(st.copy())
+            out = self.step.forward({"phi": st})["phin"]
+            st = out[1].copy()
+        return out
+
+    def reverse(self, phin_b: np.ndarray) -> dict:
+        ob = np.array(phin_b, dtype=np.float64)
+        acc = {k: 0.0 for k in PHASE_PARAMS + ("uwall",)}
+        res = None
+        for n in reversed(range(self.steps)):
+            self.step.forward({"phi": self.states[n]}, keep=False)
+            res = self.step.reverse({"phin": ob})
+            for k in acc:
+                acc[k] = acc[k] + res[k]
+            ob = np.zeros_like(ob)
+            ob[1] = res["phi"]
+        out = {"phi": res["phi"]}
+        out.update(acc)
+        return out

@@ -998,3 +998,19 @@ class ContactAngle(FDCase):
         call("bnd_contact_angle_b", WALLS.copy(), I(g.ni), I(g.nj), I(g.nk), x0["q"], qb,
              x0["theta"], tb, D(g.dx), D(g.dy), D(g.dz))
         return {"q": qb, "theta": tb}
+
+
+def phase_inputs(cfg, seed: int):
+    """A smooth volume fraction strictly inside (0.15, 0.85) (no truncation
+    threshold, clamp bound or flat-cell switch near it), velocities bounded
+    away from 0, contact angles and friction of the wall rows."""
+    r = np.random.default_rng(seed)
+    grid = Grid(cfg.ni, cfg.nj, cfg.nk, cfg.dx, cfg.dy, cfg.dz)
+    x, y, z = grid.coords()
+    a = r.uniform(0, 2 * np.pi, 4)
+    phi = 0.5 + 0.35 * np.sin(2 * np.pi * x / (cfg.ni * cfg.dx) + a[0]) * \
+        np.cos(np.pi * y / (cfg.nj * cfg.dy) + a[1]) * np.cos(2 * np.pi * z / (cfg.nk * cfg.dz) + a[2])
+    params = {"u": away_from_zero(r, grid.shape, 0.3), "v": away_from_zero(r, grid.shape, 0.3),
+              "w": away_from_zero(r, grid.shape, 0.3), "theta0": r.uniform(60.0, 120.0, grid.shape),
+              "zeta": r.uniform(0.001, 0.01, grid.shape)}
+    return phi, params

@@ -167,6 +167,41 @@ def test_momentum_uwall_adjoint_against_differences():
     assert abs(lhs - rhs) <= 1e-7 * max(abs(lhs), 1e-30), (lhs, rhs)
 
 
+def phase_cfg():
+    import dataclasses
+    from felmpf import config
+    return dataclasses.replace(config.sessile_droplet().scaled(16, 8, 16), dt=2e-3, uwall=0.05)
+
+
+@pytest.mark.parametrize("steps", [1, 2])
+def test_phase_window_against_differences(steps):
+    """The reverse of `steps` volume-fraction updates at a fixed velocity
+    (felmpf.adjoint.PhaseWindow through the oracle's routines) against
+    Richardson central differences of the same forward along a random
+    direction of (phi, u, v, w, theta_0, zeta)."""
+    from felmpf import adjoint
+    cfg = phase_cfg()
+    phi0, params = A.phase_inputs(cfg, 221)
+    rng = np.random.default_rng(222)
+    keys = ("phi",) + adjoint.PHASE_PARAMS
+    x0 = {"phi": phi0, **params}
+    v = {k: rng.standard_normal(phi0.shape) for k in keys}
+    y = rng.standard_normal((2,) + phi0.shape)
+
+    def fwd(x):
+        st = adjoint.PhaseStep(cfg, {k: x[k] for k in adjoint.PHASE_PARAMS}, call=H.call_oracle)
+        return {"phin": adjoint.PhaseWindow(st, steps).forward(x["phi"])}
+
+    jv = A.richardson(fwd, x0, v, 1e-5)
+    st = adjoint.PhaseStep(cfg, params, call=H.call_oracle)
+    win = adjoint.PhaseWindow(st, steps)
+    win.forward(phi0)
+    bx = win.reverse(y)
+    lhs = float(np.sum(jv["phin"] * y))
+    rhs = sum(float(np.sum(v[k] * bx[k])) for k in keys)
+    assert abs(lhs - rhs) <= 1e-7 * abs(lhs), (lhs, rhs)
+
+
 def test_planted_transpose_mistake_is_caught():
     """The skewed Thomas systems are not symmetric: using the forward solve
     as its own adjoint (a plausible mistake) violates the identity by O(1)."""
