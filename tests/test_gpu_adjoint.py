@@ -316,3 +316,107 @@ def test_phase_window_reverse_matches_oracle(resident):
     lhs = float(np.sum(jv["phin"] * y))
     rhs = sum(float(np.sum(v[k] * bg[k])) for k in keys)
     assert abs(lhs - rhs) <= 1e-7 * abs(lhs), (lhs, rhs)
+
+
+# ---------------------------------------------------------------------------
+# Reverse mode against the library's tangent mode: <J v, y> = <v, J^T y>
+# with J v from forward-mode arithmetic (every device array a value and its
+# derivative) through the same drop-in routines.  The two derivatives are
+# written independently (the tangent mode differentiates the forward
+# kernels' arithmetic, including the ten Newton iterations of the MTHINC
+# offset, the reverse mode transposes each routine by hand), so agreement to
+# rounding pins both.
+# ---------------------------------------------------------------------------
+
+def _tangent_run(make_step, state, seeds):
+    """Forward of the step made by make_step() in the tangent mode, with
+    the derivative directions seeds {array: direction}; returns the step's
+    registered arrays' derivatives through fetch(array)."""
+    import ctypes
+    from felmpf import _lib
+    gpu = _lib.gpu()
+    gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(1)))
+    fetched = {}
+    try:
+        st = make_step()
+        try:
+            st.set_state(state)
+            for arr, dirn in seeds(st):
+                full = np.ascontiguousarray(dirn, dtype=np.float64)
+                assert full.size == arr.size
+                gpu.felgpu_tangent_seed_(ctypes.c_void_p(arr.ctypes.data),
+                                         full.ctypes.data_as(ctypes.c_void_p))
+            st.forward_set()
+            for name, arr in st.tangent_outputs().items():
+                out = np.zeros(arr.shape)
+                gpu.felgpu_tangent_fetch_(ctypes.c_void_p(arr.ctypes.data),
+                                          out.ctypes.data_as(ctypes.c_void_p))
+                fetched[name] = out
+        finally:
+            st.close()
+    finally:
+        gpu.felgpu_set_tangent_(ctypes.byref(ctypes.c_int(0)))
+    return fetched
+
+
+def test_phase_step_reverse_against_tangent_mode():
+    from felmpf import adjoint
+    cfg = phase_cfg()
+    print("adjoint case phase step vs tangent mode 32x16x16", flush=True)
+    phi0, params = A.phase_inputs(cfg, 241)
+    rng = np.random.default_rng(242)
+    keys = ("phi",) + adjoint.PHASE_PARAMS
+    v = {k: rng.standard_normal(phi0.shape) for k in keys}
+    y = rng.standard_normal((2,) + phi0.shape)
+
+    class Step(adjoint.PhaseStep):
+        def tangent_outputs(self):
+            return {"phin": self.a["phin"]}
+
+    jv = _tangent_run(lambda: Step(cfg, params, resident=True), {"phi": phi0},
+                      lambda st: [(st.a["phi"], v["phi"])] +
+                      [(st.par[k], v[k]) for k in adjoint.PHASE_PARAMS])
+    st = adjoint.PhaseStep(cfg, params, resident=True)
+    try:
+        st.forward({"phi": phi0}, keep=False)
+        bx = st.reverse({"phin": y})
+    finally:
+        st.close()
+    lhs = float(np.sum(jv["phin"] * y))
+    rhs = sum(float(np.sum(v[k] * bx[k])) for k in keys)
+    print("phase <Jv,y>", lhs, "<v,J^T y>", rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * float(np.sqrt(np.sum(jv["phin"] ** 2) * np.sum(y ** 2))), \
+        (lhs, rhs)
+
+
+def test_momentum_step_reverse_against_tangent_mode():
+    from felmpf import adjoint
+    cfg = momentum_cfg()
+    print("adjoint case momentum step vs tangent mode", cfg.shape, flush=True)
+    fixed, state = A.momentum_inputs(cfg, 251)
+    rng = np.random.default_rng(252)
+    fst = ("fst_u", "fst_v", "fst_w", "fst_un", "fst_vn", "fst_wn")
+    v = {k: rng.standard_normal(cfg.shape) for k in adjoint.STATE + fst}
+    y = {k: rng.standard_normal(cfg.shape) for k in adjoint.OUT}
+
+    class Step(adjoint.MomentumStep):
+        def tangent_outputs(self):
+            return {k: self.a[k] for k in adjoint.OUT}
+
+    jv = _tangent_run(lambda: Step(cfg, fixed, resident=True), state,
+                      lambda st: [(st.a[k], v[k]) for k in adjoint.STATE] +
+                      [(st.fx[k], v[k]) for k in fst])
+    st = adjoint.MomentumStep(cfg, fixed, resident=True)
+    try:
+        st.forward(state, keep=False)
+        bx = st.reverse(y)
+        fb = {k: st._get(st.b[k]) for k in fst}
+    finally:
+        st.close()
+    lhs = sum(float(np.sum(jv[k] * y[k])) for k in adjoint.OUT)
+    rhs = sum(float(np.sum(v[k] * bx[k])) for k in adjoint.STATE) + \
+        sum(float(np.sum(v[k] * fb[k])) for k in fst)
+    nj = np.sqrt(sum(float(np.sum(jv[k] ** 2)) for k in adjoint.OUT))
+    ny = np.sqrt(sum(float(np.sum(y[k] ** 2)) for k in adjoint.OUT))
+    print("momentum <Jv,y>", lhs, "<v,J^T y>", rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-11 * nj * ny, (lhs, rhs)
