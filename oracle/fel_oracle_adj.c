@@ -290,16 +290,63 @@ static void slip_coef(ptrdiff_t sx, ptrdiff_t sy, int ni, int i, int jw, int k, 
   *c2 = (2.0 * ls - dy) / (2.0 * ls + dy);
 }
 
+/* Adjoint of the slip coefficients of one wall cell (slip_coef) given
+ * c1_b, c2_b: ls = 1/(phi_av/l1 + (1-phi_av)/l2), c1 = 2dy/(2ls+dy),
+ * c2 = (2ls-dy)/(2ls+dy); increments l1_b, l2_b at the cell and phi_b at
+ * the two cells of phi_av. */
+static void slip_coef_b(ptrdiff_t sx, ptrdiff_t sy, int ni, int i, int jw, int k, double dy,
+                        const double *l1, const double *l2, const double *phi, double c1b,
+                        double c2b, double *l1b, double *l2b, double *phib) {
+  const ptrdiff_t m0 = OIX(i, jw, k), m1 = OIX((i + 1) % (ni + 3), jw, k);
+  const double phi_av = (phi[m0] + phi[m1]) / 2.0;
+  const double inv1 = phi_av / l1[m0], inv2 = (1.0 - phi_av) / l2[m0];
+  const double ls = 1.0 / (inv1 + inv2);
+  const double dd = 2.0 * ls + dy;
+  const double lsb = c1b * (-4.0 * dy / (dd * dd)) + c2b * (4.0 * dy / (dd * dd));
+  const double sb = -lsb * ls * ls; /* adjoint of inv1 + inv2 */
+  l1b[m0] -= sb * phi_av / (l1[m0] * l1[m0]);
+  l2b[m0] -= sb * (1.0 - phi_av) / (l2[m0] * l2[m0]);
+  const double pab = sb / l1[m0] - sb / l2[m0];
+  phib[m0] += pab / 2.0;
+  phib[m1] += pab / 2.0;
+}
+
 /* srcs/bndu.f:29-158 (orc_bndu): the j-wall rows, then the x fills, then
  * the z fills, each written as ghost = function of earlier values.
  * Transpose in reverse: z fills, x fills, wall rows; uwall_b gathers the
- * wall rows' -c1 (bottom) and +c1 (top) weights.  The slip lengths and phi
- * parameterise the map; their adjoints are not formed. */
+ * wall rows' -c1 (bottom) and +c1 (top) weights.  With uk != NULL the
+ * adjoints of the slip lengths and phi are formed too (bndu_bp): at the
+ * point where a wall row's ghost u0 = c1 (-+uwall) + c2 u1 (and w0 = c2 w1)
+ * is reversed, its adjoint A (the rows beyond folded in) gives c1_b =
+ * A (-+uwall), c2_b = A_u u1 + A_w w1 with u1, w1 the call's input values. */
+static void bndu_b_core(const int *nID, int ni, int nj, int nk, double *ukb, double *vkb,
+                        double *wkb, double *uwallb, double dy, const double *l1,
+                        const double *l2, const double *phi, const double *uk,
+                        const double *wk, double uwall, double *l1b, double *l2b,
+                        double *phib);
+
 void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, double *ukb,
                 double *vkb, double *wkb, double *uwallb, const double *dy_, const double *l1,
                 const double *l2, const double *phi) {
-  const int ni = *ni_, nj = *nj_, nk = *nk_;
-  const double dy = *dy_;
+  bndu_b_core(nID, *ni_, *nj_, *nk_, ukb, vkb, wkb, uwallb, *dy_, l1, l2, phi, NULL, NULL, 0.0,
+              NULL, NULL, NULL);
+}
+
+/* All-argument adjoint of bndu: orc_bndu_b plus the adjoints of l1, l2 and
+ * phi (incremented); uk, wk are the call's input velocities. */
+void orc_bndu_bp(const int *nID, const int *ni_, const int *nj_, const int *nk_,
+                 const double *uk, double *ukb, double *vkb, const double *wk, double *wkb,
+                 const double *uwall, double *uwallb, const double *dy_, const double *l1,
+                 double *l1b, const double *l2, double *l2b, const double *phi, double *phib) {
+  bndu_b_core(nID, *ni_, *nj_, *nk_, ukb, vkb, wkb, uwallb, *dy_, l1, l2, phi, uk, wk, *uwall,
+              l1b, l2b, phib);
+}
+
+static void bndu_b_core(const int *nID, int ni, int nj, int nk, double *ukb, double *vkb,
+                        double *wkb, double *uwallb, double dy, const double *l1,
+                        const double *l2, const double *phi, const double *uk,
+                        const double *wk, double uwall, double *l1b, double *l2b,
+                        double *phib) {
   ODIMS(ni, nj, nk);
   /* z fills (bndu.f:130-155), reversed */
   for (int i = ni + 3; i >= -2; i--)
@@ -340,6 +387,7 @@ void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, 
       for (int i = ni + 3; i >= -2; i--) {
         double c1, c2;
         slip_coef(sx, sy, ni, i, nj, k, dy, l1, l2, phi, &c1, &c2);
+        double c1b = 0.0, c2b = 0.0;
         double *q = wkb;
         q[OIX(i, nj + 2, k)] += 2.0 * q[OIX(i, nj + 3, k)];
         q[OIX(i, nj + 1, k)] -= q[OIX(i, nj + 3, k)];
@@ -348,6 +396,7 @@ void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, 
         q[OIX(i, nj, k)] -= q[OIX(i, nj + 2, k)];
         q[OIX(i, nj + 2, k)] = 0.0;
         q[OIX(i, nj, k)] += c2 * q[OIX(i, nj + 1, k)];
+        if (uk) c2b += q[OIX(i, nj + 1, k)] * wk[OIX(i, nj, k)];
         q[OIX(i, nj + 1, k)] = 0.0;
         q = vkb;
         q[OIX(i, nj - 3, k)] -= q[OIX(i, nj + 3, k)];
@@ -366,6 +415,11 @@ void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, 
         q[OIX(i, nj + 2, k)] = 0.0;
         q[OIX(i, nj, k)] += c2 * q[OIX(i, nj + 1, k)];
         ub += c1 * q[OIX(i, nj + 1, k)];
+        if (uk) {
+          c1b += q[OIX(i, nj + 1, k)] * uwall;
+          c2b += q[OIX(i, nj + 1, k)] * uk[OIX(i, nj, k)];
+          slip_coef_b(sx, sy, ni, i, nj, k, dy, l1, l2, phi, c1b, c2b, l1b, l2b, phib);
+        }
         q[OIX(i, nj + 1, k)] = 0.0;
       }
   if (nID[ORC_Y_MINUS - 1] < 0)
@@ -373,6 +427,7 @@ void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, 
       for (int i = ni + 3; i >= -2; i--) {
         double c1, c2;
         slip_coef(sx, sy, ni, i, 1, k, dy, l1, l2, phi, &c1, &c2);
+        double c1b = 0.0, c2b = 0.0;
         double *q = wkb;
         q[OIX(i, -1, k)] += 2.0 * q[OIX(i, -2, k)];
         q[OIX(i, 0, k)] -= q[OIX(i, -2, k)];
@@ -381,6 +436,7 @@ void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, 
         q[OIX(i, 1, k)] -= q[OIX(i, -1, k)];
         q[OIX(i, -1, k)] = 0.0;
         q[OIX(i, 1, k)] += c2 * q[OIX(i, 0, k)];
+        if (uk) c2b += q[OIX(i, 0, k)] * wk[OIX(i, 1, k)];
         q[OIX(i, 0, k)] = 0.0;
         q = vkb;
         q[OIX(i, 0, k)] = 0.0;
@@ -397,6 +453,11 @@ void orc_bndu_b(const int *nID, const int *ni_, const int *nj_, const int *nk_, 
         q[OIX(i, -1, k)] = 0.0;
         q[OIX(i, 1, k)] += c2 * q[OIX(i, 0, k)];
         ub -= c1 * q[OIX(i, 0, k)];
+        if (uk) {
+          c1b -= q[OIX(i, 0, k)] * uwall;
+          c2b += q[OIX(i, 0, k)] * uk[OIX(i, 1, k)];
+          slip_coef_b(sx, sy, ni, i, 1, k, dy, l1, l2, phi, c1b, c2b, l1b, l2b, phib);
+        }
         q[OIX(i, 0, k)] = 0.0;
       }
   *uwallb += ub;
@@ -1322,5 +1383,512 @@ void orc_bnd_contact_angle_b(const int *nID, const int *ni_, const int *nj_, con
         if (k - 1 >= -2) qkb[OIX(i, jw, k - 1)] += l0 * Rz * 0.5 * az;
         theta_arrayb[OIX(i, jw, k)] += l0 * (-Dx * dRx - Dz * dRz);
       }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Parameter adjoints (<name>_bp: the adjoint with respect to every real      */
+/* argument -- what <name>_b does, plus the adjoints of the arrays it takes   */
+/* as fixed, incremented; primal arrays the derivative needs come before      */
+/* their adjoints)                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* The 14 neighbour terms of the Gauss-Seidel update of component q in the
+ * order srcs/solu_sor4.f:137-152 (186-201, 236-251) sums them: coefficient
+ * index within the component's 15, component of the neighbour, offset. */
+static void sor_terms(int q, ptrdiff_t X, ptrdiff_t Y, ptrdiff_t Z, int idx[14], int comp[14],
+                      ptrdiff_t off[14]) {
+  static const int own_idx[6] = {4, 2, 0, 1, 3, 5};
+  const ptrdiff_t own_off[6] = {-Z, -Y, -X, X, Y, Z};
+  for (int t = 0; t < 6; t++) {
+    idx[t] = own_idx[t];
+    comp[t] = q;
+    off[t] = own_off[t];
+  }
+  int ci[8];
+  int cc[8];
+  ptrdiff_t co[8];
+  if (q == 0) {
+    const int i0[8] = {7, 8, 9, 10, 11, 12, 13, 14};
+    const int c0_[8] = {1, 1, 1, 1, 2, 2, 2, 2};
+    const ptrdiff_t o0[8] = {-Y, X - Y, 0, X, -Z, X - Z, 0, X};
+    memcpy(ci, i0, sizeof ci); memcpy(cc, c0_, sizeof cc); memcpy(co, o0, sizeof co);
+  } else if (q == 1) {
+    const int i1[8] = {7, 9, 8, 10, 11, 12, 13, 14};
+    const int c1_[8] = {0, 0, 0, 0, 2, 2, 2, 2};
+    const ptrdiff_t o1[8] = {-X, 0, -X + Y, Y, -Z, Y - Z, 0, Y};
+    memcpy(ci, i1, sizeof ci); memcpy(cc, c1_, sizeof cc); memcpy(co, o1, sizeof co);
+  } else {
+    const int i2[8] = {7, 9, 8, 10, 11, 13, 12, 14};
+    const int c2_[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    const ptrdiff_t o2[8] = {-X, 0, -X + Z, Z, -Y, 0, -Y + Z, Z};
+    memcpy(ci, i2, sizeof ci); memcpy(cc, c2_, sizeof cc); memcpy(co, o2, sizeof co);
+  }
+  for (int t = 0; t < 8; t++) {
+    idx[6 + t] = ci[t];
+    comp[6 + t] = cc[t];
+    off[6 + t] = co[t];
+  }
+}
+
+/* One colour's forward update of component q (srcs/solu_sor4.f:131-175),
+ * the same arithmetic as orc_solu_sor4. */
+static void colour_fwd(int q, int col, int ni, int nj, int nk, int jen, double dtinv,
+                       const double *rho, const double *rhon, double *const *cf,
+                       const double *src, double *const *x) {
+  ODIMS(ni, nj, nk);
+  const ptrdiff_t X = 1, Y = sx, Z = sx * sy;
+  const int jend = q == 1 ? jen : nj;
+  const ptrdiff_t F = q == 0 ? X : (q == 1 ? Y : Z);
+  double *const *c = cf + 15 * q;
+  int idx[14], comp[14];
+  ptrdiff_t off[14];
+  sor_terms(q, X, Y, Z, idx, comp, off);
+  for (int k = kKsor[col]; k <= nk; k += 2)
+    for (int j = kJsor[col]; j <= jend; j += 2)
+      for (int i = kIsor[col]; i <= ni; i += 2) {
+        const ptrdiff_t m = OIX(i, j, k);
+        const double r00 = (rho[m] + rho[m + F] + rhon[m] + rhon[m + F]) * 0.25;
+        double sum = 0.0;
+        for (int t = 0; t < 14; t++) sum = sum + c[idx[t]][m] * x[comp[t]][m + off[t]];
+        const double visn = sum / r00;
+        const double d = (visn * 0.5 + src[m]) / (dtinv + c[6][m] / r00 * 0.5) - x[q][m];
+        x[q][m] = x[q][m] + d;
+      }
+}
+
+/* colour_b with the parameter adjoints: for x = (0.5 S/r00 + src)/den,
+ * den = dtinv + 0.5 a_p/r00 and S = sum a_t y_t over the state y before the
+ * colour (xs): a_t_b += h y_t, a_p_b -= h x, r00_b = g (0.5 x a_p - 0.5 S)/r00^2
+ * (g = x_b/den, h = 0.5 g/r00), r00 = (rho + rho(+F) + rhon + rhon(+F))/4. */
+static void colour_bp(int q, int col, int ni, int nj, int nk, int jen, double dtinv,
+                      const double *rho, double *rhob, const double *rhon, double *rhonb,
+                      double *const *cf, double *const *cfb, const double *src, double *srcb,
+                      double *const *xs, double *const *xb) {
+  ODIMS(ni, nj, nk);
+  const ptrdiff_t X = 1, Y = sx, Z = sx * sy;
+  const int jend = q == 1 ? jen : nj;
+  const ptrdiff_t F = q == 0 ? X : (q == 1 ? Y : Z);
+  double *const *c = cf + 15 * q;
+  double *const *cb = cfb + 15 * q;
+  int idx[14], comp[14];
+  ptrdiff_t off[14];
+  sor_terms(q, X, Y, Z, idx, comp, off);
+  for (int k = kKsor[col]; k <= nk; k += 2)
+    for (int j = kJsor[col]; j <= jend; j += 2)
+      for (int i = kIsor[col]; i <= ni; i += 2) {
+        const ptrdiff_t m = OIX(i, j, k);
+        const double r00 = (rho[m] + rho[m + F] + rhon[m] + rhon[m + F]) * 0.25;
+        const double den = dtinv + c[6][m] / r00 * 0.5;
+        double sum = 0.0;
+        for (int t = 0; t < 14; t++) sum = sum + c[idx[t]][m] * xs[comp[t]][m + off[t]];
+        const double x = (sum / r00 * 0.5 + src[m]) / den;
+        const double g = xb[q][m] / den;
+        const double h = g * 0.5 / r00;
+        srcb[m] += g;
+        for (int t = 0; t < 14; t++) cb[idx[t]][m] += h * xs[comp[t]][m + off[t]];
+        cb[6][m] -= h * x;
+        const double r00b = g * (0.5 * x * c[6][m] - 0.5 * sum) / (r00 * r00);
+        rhob[m] += 0.25 * r00b;
+        rhob[m + F] += 0.25 * r00b;
+        rhonb[m] += 0.25 * r00b;
+        rhonb[m + F] += 0.25 * r00b;
+        xb[q][m] = 0.0;
+        for (int t = 0; t < 14; t++) xb[comp[t]][m + off[t]] += c[idx[t]][m] * h;
+      }
+}
+
+/* All-argument adjoint of solu_sor4 with the forward's sweep count lp
+ * frozen: the forward sweeps are recomputed from the call's input state
+ * (un, vn, wn, src_*) keeping the state before every colour and before every
+ * refresh; the reverse is sor4_b_impl's with each colour through colour_bp
+ * and each refresh's bndu through its all-argument adjoint at the kept
+ * state.  Adds the adjoints of rho, rhon, the 45 coefficients, l1, l2 and
+ * phi to what orc_solu_sor4_b gives. */
+void orc_solu_sor4_bp(const int *ipara, const int *ID, const int *nID, const int *ndiv,
+                      const int *ni_, const int *nj_, const int *nk_, const int *key,
+                      double *sendjb, double *recvjb, const int *nstep, const int *imon_t,
+                      const double *dt_, const double *rho, double *rhob, const double *rhon,
+                      double *rhonb, double *c0, double *c1, double *c2, double *c3, double *c4, double *c5, double *c6, double *c7, double *c8, double *c9, double *c10, double *c11, double *c12, double *c13, double *c14, double *c15, double *c16, double *c17, double *c18, double *c19, double *c20, double *c21, double *c22, double *c23, double *c24, double *c25, double *c26, double *c27, double *c28, double *c29, double *c30, double *c31, double *c32, double *c33, double *c34, double *c35, double *c36, double *c37, double *c38, double *c39, double *c40, double *c41, double *c42, double *c43, double *c44, double *c0b, double *c1b, double *c2b, double *c3b, double *c4b, double *c5b, double *c6b, double *c7b, double *c8b, double *c9b, double *c10b, double *c11b, double *c12b, double *c13b, double *c14b, double *c15b, double *c16b, double *c17b, double *c18b, double *c19b, double *c20b, double *c21b, double *c22b, double *c23b, double *c24b, double *c25b, double *c26b, double *c27b, double *c28b, double *c29b, double *c30b, double *c31b, double *c32b, double *c33b, double *c34b, double *c35b, double *c36b, double *c37b, double *c38b, double *c39b, double *c40b, double *c41b, double *c42b, double *c43b, double *c44b,
+                      const double *src_u, double *src_ub, const double *src_v, double *src_vb,
+                      const double *src_w, double *src_wb, const double *un, double *unb,
+                      const double *vn, double *vnb, const double *wn, double *wnb,
+                      const double *uwall, double *uwallb, const double *dy, const double *l1,
+                      double *l1b, const double *l2, double *l2b, const double *phi,
+                      double *phib, const int *lp) {
+  (void)ipara; (void)ID; (void)ndiv; (void)key; (void)sendjb; (void)recvjb; (void)nstep; (void)imon_t;
+  double *const coef[NCOEF] = {c0, c1, c2, c3, c4, c5, c6, c7, c8, c9, c10, c11, c12, c13, c14, c15, c16, c17, c18, c19, c20, c21, c22, c23, c24, c25, c26, c27, c28, c29, c30, c31, c32, c33, c34, c35, c36, c37, c38, c39, c40, c41, c42, c43, c44};
+  double *const coefb[NCOEF] = {c0b, c1b, c2b, c3b, c4b, c5b, c6b, c7b, c8b, c9b, c10b, c11b, c12b, c13b, c14b, c15b, c16b, c17b, c18b, c19b, c20b, c21b, c22b, c23b, c24b, c25b, c26b, c27b, c28b, c29b, c30b, c31b, c32b, c33b, c34b, c35b, c36b, c37b, c38b, c39b, c40b, c41b, c42b, c43b, c44b};
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const double dtinv = 1.0 / (*dt_);
+  const int jen = (nID[ORC_Y_PLUS - 1] < 0) ? nj - 1 : nj;
+  const int nsw = *lp < 21 ? *lp + 1 : 21;
+  const size_t blk = 3 * (size_t)n3;
+  double *cur = malloc(sizeof(double) * blk);
+  double *snap_c = malloc(sizeof(double) * blk * (size_t)nsw * 24);
+  double *snap_r = malloc(sizeof(double) * blk * (size_t)nsw * 12);
+  memcpy(cur, un, sizeof(double) * n3);
+  memcpy(cur + n3, vn, sizeof(double) * n3);
+  memcpy(cur + 2 * n3, wn, sizeof(double) * n3);
+  double *const x[3] = {cur, cur + n3, cur + 2 * n3};
+  const double *src[3] = {src_u, src_v, src_w};
+  double *srcb[3] = {src_ub, src_vb, src_wb};
+  double *xb[3] = {unb, vnb, wnb};
+  for (int s = 0; s < nsw; s++)
+    for (int q = 0; q < 3; q++)
+      for (int col = 0; col < 8; col++) {
+        memcpy(snap_c + blk * ((size_t)(s * 3 + q) * 8 + col), cur, sizeof(double) * blk);
+        colour_fwd(q, col, ni, nj, nk, jen, dtinv, rho, rhon, coef, src[q], x);
+        if (col % 2 == 1) {
+          memcpy(snap_r + blk * ((size_t)(s * 3 + q) * 4 + col / 2), cur, sizeof(double) * blk);
+          orc_bndu(nID, ni_, nj_, nk_, x[0], x[1], x[2], uwall, dy, l1, l2, phi);
+          orc_bnd_periodic(ni_, nj_, nk_, x[q]);
+        }
+      }
+  for (int s = nsw - 1; s >= 0; s--)
+    for (int q = 2; q >= 0; q--)
+      for (int col = 7; col >= 0; col--) {
+        if (col % 2 == 1) {
+          const double *r = snap_r + blk * ((size_t)(s * 3 + q) * 4 + col / 2);
+          orc_bnd_periodic_b(ni_, nj_, nk_, xb[q]);
+          bndu_b_core(nID, ni, nj, nk, unb, vnb, wnb, uwallb, *dy, l1, l2, phi, r, r + 2 * n3,
+                      *uwall, l1b, l2b, phib);
+        }
+        double *c0p = snap_c + blk * ((size_t)(s * 3 + q) * 8 + col);
+        double *const xs[3] = {c0p, c0p + n3, c0p + 2 * n3};
+        colour_bp(q, col, ni, nj, nk, jen, dtinv, rho, rhob, rhon, rhonb, coef, coefb, src[q],
+                  srcb[q], xs, xb);
+      }
+  free(cur);
+  free(snap_c);
+  free(snap_r);
+}
+
+/* All-argument adjoint of cal_arith_tau (orc_cal_arith_tau_b plus the
+ * viscosity's adjoint): tau = 2 s mu_face, mu_face the cell value (normal
+ * stresses) or the mean of four cells (shear stresses). */
+void orc_cal_arith_tau_bp(const int *ni_, const int *nj_, const int *nk_, const double *s,
+                          double *sb, const double *rmu, double *rmub, double *taub) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const ptrdiff_t X = 1, Y = sx, Z = sx * sy;
+  /* blocks 3..5: the shear stresses, their four-cell means and boxes */
+  const ptrdiff_t da[3] = {X, X, Y}, db[3] = {Y, Z, Z};
+  const int hi[3][3] = {{ni + 2, nj + 2, nk + 3}, {ni + 2, nj + 3, nk + 2}, {ni + 3, nj + 2, nk + 2}};
+  for (int l = 5; l >= 3; l--) {
+    const int b = l - 3;
+    for (int k = hi[b][2]; k >= -2; k--)
+      for (int j = hi[b][1]; j >= -2; j--)
+        for (int i = hi[b][0]; i >= -2; i--) {
+          const ptrdiff_t c = OIX(i, j, k);
+          const double m = +0.25 * rmu[c] + 0.25 * rmu[c + da[b]] + 0.25 * rmu[c + db[b]] +
+                           0.25 * rmu[c + da[b] + db[b]];
+          const double t = taub[l * n3 + c];
+          sb[l * n3 + c] += t * m * 2.0;
+          const double mb = t * s[l * n3 + c] * 2.0;
+          rmub[c] += 0.25 * mb;
+          rmub[c + da[b]] += 0.25 * mb;
+          rmub[c + db[b]] += 0.25 * mb;
+          rmub[c + da[b] + db[b]] += 0.25 * mb;
+          taub[l * n3 + c] = 0.0;
+        }
+  }
+  for (int l = 2; l >= 0; l--)
+    for (ptrdiff_t m = n3 - 1; m >= 0; m--) {
+      sb[l * n3 + m] += taub[l * n3 + m] * rmu[m] * 2.0;
+      rmub[m] += taub[l * n3 + m] * s[l * n3 + m] * 2.0;
+      taub[l * n3 + m] = 0.0;
+    }
+}
+
+/* All-argument adjoint of cal_srcu (orc_cal_srcu_b plus the densities'
+ * adjoints): the face density r enters as -prs/r + 0.5 vis/r, so
+ * r_b = src_b (prs - 0.5 vis)/r^2, spread over its four cells. */
+void orc_cal_srcu_bp(const int *nID, const int *ni_, const int *nj_, const int *nk_,
+                     const double *rho_av, const double *dt_, const double *grvb_,
+                     const double *grvp_, double *ub, double *vb, double *wb, const double *rho,
+                     double *rhob, const double *rhon, double *rhonb, double *adv_ub,
+                     double *adv_vb, double *adv_wb, double *adv_uob, double *adv_vob,
+                     double *adv_wob, const double *prs_u, double *prs_ub, const double *prs_v,
+                     double *prs_vb, const double *prs_w, double *prs_wb, const double *vis_u,
+                     double *vis_ub, const double *vis_v, double *vis_vb, const double *vis_w,
+                     double *vis_wb, double *fst_ub, double *fst_vb, double *fst_wb,
+                     double *fst_unb, double *fst_vnb, double *fst_wnb, double *src_ub,
+                     double *src_vb, double *src_wb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const int jen = (nID[ORC_Y_PLUS - 1] < 0) ? nj - 1 : nj;
+  const ptrdiff_t F[3] = {1, sx, sx * sy};
+  const double *prs[3] = {prs_u, prs_v, prs_w}, *vis[3] = {vis_u, vis_v, vis_w};
+  double *src[3] = {src_ub, src_vb, src_wb};
+  for (int q = 2; q >= 0; q--) {
+    const int jend = q == 1 ? jen : nj;
+    for (int k = nk; k >= 1; k--)
+      for (int j = jend; j >= 1; j--)
+        for (int i = ni; i >= 1; i--) {
+          const ptrdiff_t c = OIX(i, j, k);
+          const double r = (rho[c] + rho[c + F[q]] + rhon[c] + rhon[c + F[q]]) * 0.25;
+          const double rb = src[q][c] * (prs[q][c] - 0.5 * vis[q][c]) / (r * r);
+          rhob[c] += 0.25 * rb;
+          rhob[c + F[q]] += 0.25 * rb;
+          rhonb[c] += 0.25 * rb;
+          rhonb[c + F[q]] += 0.25 * rb;
+        }
+  }
+  orc_cal_srcu_b(nID, ni_, nj_, nk_, rho_av, dt_, grvb_, grvp_, ub, vb, wb, rho, rhon, adv_ub,
+                 adv_vb, adv_wb, adv_uob, adv_vob, adv_wob, prs_ub, prs_vb, prs_wb, vis_ub,
+                 vis_vb, vis_wb, fst_ub, fst_vb, fst_wb, fst_unb, fst_vnb, fst_wnb, src_ub,
+                 src_vb, src_wb);
+}
+
+/* srcs/cal_arith_coef_vis.f:1-411 (coef_vis in fel_oracle.c), linear in
+ * the viscosity: each coefficient of cell c is a multiple of one of six
+ * face viscosities (xm, xp, ym, yp, zm, zp: a cell value or a mean of four)
+ * and a_p their weighted sum.  Transpose: face adjoints from the 15
+ * coefficient adjoints of each component, spread to the cells of each
+ * face, all coefficient adjoints consumed (the forward overwrites every
+ * coefficient array whole). */
+void orc_cal_arith_coef_vis_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                              const double *dyinv_, const double *dzinv_, double *rmukb,
+                              double *c0b, double *c1b, double *c2b, double *c3b, double *c4b, double *c5b, double *c6b, double *c7b, double *c8b, double *c9b, double *c10b, double *c11b, double *c12b, double *c13b, double *c14b, double *c15b, double *c16b, double *c17b, double *c18b, double *c19b, double *c20b, double *c21b, double *c22b, double *c23b, double *c24b, double *c25b, double *c26b, double *c27b, double *c28b, double *c29b, double *c30b, double *c31b, double *c32b, double *c33b, double *c34b, double *c35b, double *c36b, double *c37b, double *c38b, double *c39b, double *c40b, double *c41b, double *c42b, double *c43b, double *c44b) {
+  double *const a[NCOEF] = {c0b, c1b, c2b, c3b, c4b, c5b, c6b, c7b, c8b, c9b, c10b, c11b, c12b, c13b, c14b, c15b, c16b, c17b, c18b, c19b, c20b, c21b, c22b, c23b, c24b, c25b, c26b, c27b, c28b, c29b, c30b, c31b, c32b, c33b, c34b, c35b, c36b, c37b, c38b, c39b, c40b, c41b, c42b, c43b, c44b};
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  ODIMS(ni, nj, nk);
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_;
+  const double dx2 = dxinv * dxinv, dy2 = dyinv * dyinv, dz2 = dzinv * dzinv;
+  const double dxy = dxinv * dyinv, dxz = dxinv * dzinv, dyz = dyinv * dzinv;
+#define RB(i, j, k) rmukb[OIX(i, j, k)]
+  for (int k = nk + 2; k >= -1; k--)
+    for (int j = nj + 2; j >= -1; j--)
+      for (int i = ni + 2; i >= -1; i--) {
+        const ptrdiff_t c = OIX(i, j, k);
+        /* w component (cal_arith_coef_vis.f, third block) */
+        {
+          const double P = a[CW_P][c];
+          const double bb = a[CW_B][c] + P, sb = a[CW_S][c] + P, wb = a[CW_W][c] + P;
+          const double eb = a[CW_E][c] + P, nb = a[CW_N][c] + P, tb = a[CW_T][c] + P;
+          const double xmb = dx2 * wb + dxz * (a[CW_UBW][c] - a[CW_UTW][c]);
+          const double xpb = dx2 * eb + dxz * (-a[CW_UBE][c] + a[CW_UTE][c]);
+          const double ymb = dy2 * sb + dyz * (a[CW_VBS][c] - a[CW_VTS][c]);
+          const double ypb = dy2 * nb + dyz * (-a[CW_VBN][c] + a[CW_VTN][c]);
+          const double zmb = dz2 * bb * 2.0, zpb = dz2 * tb * 2.0;
+          RB(i, j, k) += zmb;
+          RB(i, j, k + 1) += zpb;
+          RB(i - 1, j, k) += 0.25 * xmb; RB(i, j, k) += 0.25 * xmb;
+          RB(i - 1, j, k + 1) += 0.25 * xmb; RB(i, j, k + 1) += 0.25 * xmb;
+          RB(i, j, k) += 0.25 * xpb; RB(i + 1, j, k) += 0.25 * xpb;
+          RB(i, j, k + 1) += 0.25 * xpb; RB(i + 1, j, k + 1) += 0.25 * xpb;
+          RB(i, j - 1, k) += 0.25 * ymb; RB(i, j, k) += 0.25 * ymb;
+          RB(i, j - 1, k + 1) += 0.25 * ymb; RB(i, j, k + 1) += 0.25 * ymb;
+          RB(i, j, k) += 0.25 * ypb; RB(i, j + 1, k) += 0.25 * ypb;
+          RB(i, j, k + 1) += 0.25 * ypb; RB(i, j + 1, k + 1) += 0.25 * ypb;
+        }
+        /* v component */
+        {
+          const double P = a[CV_P][c];
+          const double bb = a[CV_B][c] + P, sb = a[CV_S][c] + P, wb = a[CV_W][c] + P;
+          const double eb = a[CV_E][c] + P, nb = a[CV_N][c] + P, tb = a[CV_T][c] + P;
+          const double xmb = dx2 * wb + dxy * (a[CV_USW][c] - a[CV_UNW][c]);
+          const double xpb = dx2 * eb + dxy * (-a[CV_USE][c] + a[CV_UNE][c]);
+          const double ymb = dy2 * sb * 2.0, ypb = dy2 * nb * 2.0;
+          const double zmb = dz2 * bb + dyz * (a[CV_WSB][c] - a[CV_WNB][c]);
+          const double zpb = dz2 * tb + dyz * (-a[CV_WST][c] + a[CV_WNT][c]);
+          RB(i, j, k) += ymb;
+          RB(i, j + 1, k) += ypb;
+          RB(i - 1, j, k) += 0.25 * xmb; RB(i, j, k) += 0.25 * xmb;
+          RB(i - 1, j + 1, k) += 0.25 * xmb; RB(i, j + 1, k) += 0.25 * xmb;
+          RB(i, j, k) += 0.25 * xpb; RB(i + 1, j, k) += 0.25 * xpb;
+          RB(i, j + 1, k) += 0.25 * xpb; RB(i + 1, j + 1, k) += 0.25 * xpb;
+          RB(i, j, k - 1) += 0.25 * zmb; RB(i, j + 1, k - 1) += 0.25 * zmb;
+          RB(i, j, k) += 0.25 * zmb; RB(i, j + 1, k) += 0.25 * zmb;
+          RB(i, j, k) += 0.25 * zpb; RB(i, j + 1, k) += 0.25 * zpb;
+          RB(i, j, k + 1) += 0.25 * zpb; RB(i, j + 1, k + 1) += 0.25 * zpb;
+        }
+        /* u component */
+        {
+          const double P = a[CU_P][c];
+          const double bb = a[CU_B][c] + P, sb = a[CU_S][c] + P, wb = a[CU_W][c] + P;
+          const double eb = a[CU_E][c] + P, nb = a[CU_N][c] + P, tb = a[CU_T][c] + P;
+          const double xmb = dx2 * wb * 2.0, xpb = dx2 * eb * 2.0;
+          const double ymb = dy2 * sb + dxy * (a[CU_VWS][c] - a[CU_VES][c]);
+          const double ypb = dy2 * nb + dxy * (-a[CU_VWN][c] + a[CU_VEN][c]);
+          const double zmb = dz2 * bb + dxz * (a[CU_WWB][c] - a[CU_WEB][c]);
+          const double zpb = dz2 * tb + dxz * (-a[CU_WWT][c] + a[CU_WET][c]);
+          RB(i, j, k) += xmb;
+          RB(i + 1, j, k) += xpb;
+          RB(i, j - 1, k) += 0.25 * ymb; RB(i + 1, j - 1, k) += 0.25 * ymb;
+          RB(i, j, k) += 0.25 * ymb; RB(i + 1, j, k) += 0.25 * ymb;
+          RB(i, j, k) += 0.25 * ypb; RB(i + 1, j, k) += 0.25 * ypb;
+          RB(i, j + 1, k) += 0.25 * ypb; RB(i + 1, j + 1, k) += 0.25 * ypb;
+          RB(i, j, k - 1) += 0.25 * zmb; RB(i + 1, j, k - 1) += 0.25 * zmb;
+          RB(i, j, k) += 0.25 * zmb; RB(i + 1, j, k) += 0.25 * zmb;
+          RB(i, j, k) += 0.25 * zpb; RB(i + 1, j, k) += 0.25 * zpb;
+          RB(i, j, k + 1) += 0.25 * zpb; RB(i + 1, j, k + 1) += 0.25 * zpb;
+        }
+      }
+#undef RB
+  for (int cc = 0; cc < NCOEF; cc++) memset(a[cc], 0, sizeof(double) * n3);
+}
+
+/* srcs/cal_arith_mean.f:18-26: q = ql (1-phi) + qg phi on every element.
+ * Transpose: phi_b += (qg - ql) q_b, q_b = 0. */
+void orc_cal_arith_mean_b(const int *ni_, const int *nj_, const int *nk_, double *phikb,
+                          const double *ql_, const double *qg_, double *qkb) {
+  ODIMS(*ni_, *nj_, *nk_);
+  const double ql = *ql_, qg = *qg_;
+  for (ptrdiff_t m = n3 - 1; m >= 0; m--) {
+    const double plb = ql * qkb[m], pgb = qg * qkb[m];
+    phikb[m] += -plb + pgb;
+    qkb[m] = 0.0;
+  }
+}
+
+/* srcs/calrn.f:1-95 (orc_calrn), nonlinear: rn = r/(|r| + verysmall) with r
+ * the mean of eight vertex gradients, on the box of the staggering
+ * (ip, jp, kp); zero elsewhere.  Reverse: r_b = rn_b/a - (rn_b . r) r/(a^2
+ * |r|) with a = |r| + verysmall (the |r| term taken as 0 at r = 0), spread
+ * over the eight vertices; rn_b consumed whole. */
+void orc_calrn_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                 const double *dyinv_, const double *dzinv_, const int *ip_, const int *jp_,
+                 const int *kp_, const double *phix, double *phixb, const double *phiy,
+                 double *phiyb, const double *phiz, double *phizb, double *rnb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_, ip = *ip_, jp = *jp_, kp = *kp_;
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  const double verysmall = sqrt(dxinv * dxinv + dyinv * dyinv + dzinv * dzinv) * 1.0e-30;
+  const double *a[3] = {phix, phiy, phiz};
+  double *ab[3] = {phixb, phiyb, phizb};
+  for (int k = nk + 3 - kp; k >= -2; k--)
+    for (int j = nj + 3 - jp; j >= -2; j--)
+      for (int i = ni + 3 - ip; i >= -2; i--) {
+        const int ii = i + ip, jj = j + jp, kk = k + kp;
+        const ptrdiff_t v[8] = {OIX(i, j, k), OIX(ii, j, k), OIX(i, jj, k), OIX(ii, jj, k),
+                                OIX(i, j, kk), OIX(ii, j, kk), OIX(i, jj, kk), OIX(ii, jj, kk)};
+        double r[3];
+        for (int c = 0; c < 3; c++) {
+          const double *p = a[c];
+          r[c] = (+p[v[0]] + p[v[1]] + p[v[2]] + p[v[3]] + p[v[4]] + p[v[5]] + p[v[6]] + p[v[7]]) *
+                 0.125;
+        }
+        const double nr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double inv = 1.0 / (nr + verysmall);
+        const ptrdiff_t o = OIX(ii, jj, kk);
+        const double gb[3] = {rnb[o], rnb[n3 + o], rnb[2 * n3 + o]};
+        const double invb = gb[0] * r[0] + gb[1] * r[1] + gb[2] * r[2];
+        const double nrb = -invb * inv * inv;
+        for (int c = 0; c < 3; c++) {
+          const double rb = gb[c] * inv + (nr > 0.0 ? nrb * r[c] / nr : 0.0);
+          for (int t = 0; t < 8; t++) ab[c][v[t]] += rb * 0.125;
+        }
+        rnb[o] = 0.0;
+        rnb[n3 + o] = 0.0;
+        rnb[2 * n3 + o] = 0.0;
+      }
+  memset(rnb, 0, sizeof(double) * 3 * n3);
+}
+
+/* srcs/calrkap.f:1-102 (orc_calrkap), linear: the three curvature
+ * components are differences of normal components on their boxes, zero
+ * elsewhere.  Transpose per component, rkap_b consumed whole. */
+void orc_calrkap_b(const int *ni_, const int *nj_, const int *nk_, const double *dxinv_,
+                   const double *dyinv_, const double *dzinv_, double *rn_ppb, double *rn_12b,
+                   double *rn_13b, double *rn_23b, double *rkapb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dyinv = *dyinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  const ptrdiff_t C2 = n3, C3 = 2 * n3;
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 3; j >= -1; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const double t = rkapb[C3 + OIX(i, j, k)];
+        rn_13b[OIX(i - 1, j, k)] -= t * dxinv;
+        rn_13b[OIX(i, j, k)] += t * dxinv;
+        rn_23b[C2 + OIX(i, j - 1, k)] -= t * dyinv;
+        rn_23b[C2 + OIX(i, j, k)] += t * dyinv;
+        rn_ppb[C3 + OIX(i, j, k)] -= t * dzinv;
+        rn_ppb[C3 + OIX(i, j, k + 1)] += t * dzinv;
+      }
+  for (int k = nk + 3; k >= -1; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 3; i >= -1; i--) {
+        const double t = rkapb[C2 + OIX(i, j, k)];
+        rn_12b[OIX(i - 1, j, k)] -= t * dxinv;
+        rn_12b[OIX(i, j, k)] += t * dxinv;
+        rn_ppb[C2 + OIX(i, j, k)] -= t * dyinv;
+        rn_ppb[C2 + OIX(i, j + 1, k)] += t * dyinv;
+        rn_23b[C3 + OIX(i, j, k - 1)] -= t * dzinv;
+        rn_23b[C3 + OIX(i, j, k)] += t * dzinv;
+      }
+  for (int k = nk + 3; k >= -1; k--)
+    for (int j = nj + 3; j >= -1; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const double t = rkapb[OIX(i, j, k)];
+        rn_ppb[OIX(i, j, k)] -= t * dxinv;
+        rn_ppb[OIX(i + 1, j, k)] += t * dxinv;
+        rn_12b[C2 + OIX(i, j - 1, k)] -= t * dyinv;
+        rn_12b[C2 + OIX(i, j, k)] += t * dyinv;
+        rn_13b[C3 + OIX(i, j, k - 1)] -= t * dzinv;
+        rn_13b[C3 + OIX(i, j, k)] += t * dzinv;
+      }
+  memset(rkapb, 0, sizeof(double) * 3 * n3);
+}
+
+/* srcs/cal_grad_p2uvw.f:1-47 (orc_cal_grad_p2uvw), linear: forward
+ * differences on [-2,n+2]^3; the rest of the outputs is left as it was.
+ * Transpose on the box, the outputs' adjoints consumed there. */
+void orc_cal_grad_p2uvw_b(const int *ID, const int *svall, const int *ni_, const int *nj_,
+                          const int *nk_, const double *dxinv_, const double *dyinv_array,
+                          const double *dzinv_, double *qb, double *qx_ub, double *qy_vb,
+                          double *qz_wb) {
+  const int ni = *ni_, nj = *nj_, nk = *nk_;
+  const double dxinv = *dxinv_, dzinv = *dzinv_;
+  ODIMS(ni, nj, nk);
+  (void)svall;
+  for (int k = nk + 2; k >= -2; k--)
+    for (int j = nj + 2; j >= -2; j--)
+      for (int i = ni + 2; i >= -2; i--) {
+        const double dyinv = dyinv_array[j + nj * (*ID) + 2];
+        const ptrdiff_t c = OIX(i, j, k);
+        const double gz = qz_wb[c] * dzinv, gy = qy_vb[c] * dyinv, gx = qx_ub[c] * dxinv;
+        qb[OIX(i, j, k + 1)] += gz;
+        qb[OIX(i, j + 1, k)] += gy;
+        qb[OIX(i + 1, j, k)] += gx;
+        qb[c] += -gz - gy - gx;
+        qx_ub[c] = 0.0;
+        qy_vb[c] = 0.0;
+        qz_wb[c] = 0.0;
+      }
+}
+
+/* srcs/calfst.f:1-50 (orc_calfst), bilinear: f = -(st kappa g) with st =
+ * 2 sigma/(rho_l + rho_g) on every element.  Reverse: g_b += -st kappa f_b,
+ * kappa_b += -st g f_b, f_b = 0. */
+void orc_calfst_b(const int *ni_, const int *nj_, const int *nk_, const double *rhol,
+                  const double *rhog, const double *surface_tension, const double *phix_u,
+                  double *phix_ub, const double *phiy_v, double *phiy_vb, const double *phiz_w,
+                  double *phiz_wb, const double *rkap, double *rkapb, double *fst_ub,
+                  double *fst_vb, double *fst_wb) {
+  ODIMS(*ni_, *nj_, *nk_);
+  const double st = 2.0 * (*surface_tension) / (*rhol + *rhog);
+  const double *g[3] = {phix_u, phiy_v, phiz_w};
+  double *gb[3] = {phix_ub, phiy_vb, phiz_wb}, *fb[3] = {fst_ub, fst_vb, fst_wb};
+  for (ptrdiff_t m = n3 - 1; m >= 0; m--)
+    for (int c = 2; c >= 0; c--) {
+      const double t = fb[c][m];
+      gb[c][m] += -(st * rkap[c * n3 + m] * t);
+      rkapb[c * n3 + m] += -(st * g[c][m] * t);
+      fb[c][m] = 0.0;
+    }
+}
+
+/* srcs/sum_fst.f:1-37: s = s + f.  Transpose: f_b += s_b (s_b unchanged). */
+void orc_sum_fst_b(const int *ni_, const int *nj_, const int *nk_, double *fub, double *fvb,
+                   double *fwb, const double *sub, const double *svb, const double *swb) {
+  ODIMS(*ni_, *nj_, *nk_);
+  for (ptrdiff_t m = 0; m < n3; m++) {
+    fub[m] += sub[m];
+    fvb[m] += svb[m];
+    fwb[m] += swb[m];
   }
 }

@@ -1014,3 +1014,372 @@ def phase_inputs(cfg, seed: int):
               "w": away_from_zero(r, grid.shape, 0.3), "theta0": r.uniform(60.0, 120.0, grid.shape),
               "zeta": r.uniform(0.001, 0.01, grid.shape)}
     return phi, params
+
+
+# --------------------------------------------------------------------------
+# Material properties, surface tension and the parameter adjoints (_bp) of
+# the momentum routines
+# --------------------------------------------------------------------------
+
+class ArithMean(FDCase):
+    name = "cal_arith_mean"
+    ql, qg = 1.3, 0.07
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        self.x0 = {"phi": np.random.default_rng(seed + 1400).uniform(0, 1, g.shape)}
+
+    def out_adjoints(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def value(self, call, x):
+        q = self.g.zeros()
+        call("cal_arith_mean", *grid_head(self.g), x["phi"].copy(), D(self.ql), D(self.qg), q)
+        return {"q": q}
+
+    def adjoint(self, call, y):
+        pb, qb = self.g.zeros(), y["q"].copy()
+        call("cal_arith_mean_b", *grid_head(self.g), pb, D(self.ql), D(self.qg), qb)
+        assert not np.any(qb)
+        return {"phi": pb}
+
+
+class ArithTauP(FDCase):
+    """cal_arith_tau_bp: the stress's adjoint with respect to the strain
+    rates and the viscosity."""
+    name = "cal_arith_tau_bp"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1500)
+        self.x0 = {"s": r.standard_normal((6,) + g.shape), "rmu": r.uniform(0.01, 2.0, g.shape)}
+
+    def out_adjoints(self):
+        return {"tau": self.rng.standard_normal((6,) + self.g.shape)}
+
+    def value(self, call, x):
+        tau = self.g.zeros(6)
+        call("cal_arith_tau", *grid_head(self.g), x["s"].copy(), x["rmu"].copy(), tau)
+        return {"tau": tau}
+
+    def adjoint(self, call, y):
+        g, x0 = self.g, self.x0
+        sb, mb, tb = g.zeros(6), g.zeros(), y["tau"].copy()
+        call("cal_arith_tau_bp", *grid_head(g), x0["s"], sb, x0["rmu"], mb, tb)
+        return {"s": sb, "rmu": mb}
+
+
+class SrcuP(FDCase):
+    """cal_srcu_bp: the source's adjoint including the densities."""
+    name = "cal_srcu_bp"
+    dt = 0.003
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1600)
+        self.x0 = {k: r.standard_normal(g.shape) for k in SRCU_IN}
+        self.x0["rho"] = r.uniform(0.5, 2.0, g.shape)
+        self.x0["rhon"] = r.uniform(0.5, 2.0, g.shape)
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("src_u", "src_v", "src_w")}
+
+    def head(self):
+        return [WALLS.copy(), *grid_head(self.g), D(0.9), D(self.dt), D(0.0), D(0.0)]
+
+    def value(self, call, x):
+        g = self.g
+        src = [g.zeros() for _ in range(3)]
+        call("cal_srcu", *self.head(), x["u"].copy(), x["v"].copy(), x["w"].copy(), x["rho"].copy(),
+             x["rhon"].copy(), *[x[k].copy() for k in SRCU_IN[3:]], *src)
+        return dict(zip(("src_u", "src_v", "src_w"), src))
+
+    def adjoint(self, call, y):
+        g, x0 = self.g, self.x0
+        b = {k: g.zeros() for k in SRCU_IN + ("rho", "rhon")}
+        src = [y[k].copy() for k in ("src_u", "src_v", "src_w")]
+        pv = []
+        for k in ("prs_u", "prs_v", "prs_w", "vis_u", "vis_v", "vis_w"):
+            pv += [x0[k], b[k]]
+        call("cal_srcu_bp", *self.head(), b["u"], b["v"], b["w"], x0["rho"], b["rho"], x0["rhon"],
+             b["rhon"], *[b[k] for k in ("adv_u", "adv_v", "adv_w", "adv_uo", "adv_vo", "adv_wo")],
+             *pv, *[b[k] for k in SRCU_IN[15:]], *src)
+        b["_kept"] = src
+        return b
+
+
+class CoefVis(Case):
+    """cal_arith_coef_vis_b: linear in the viscosity."""
+    name = "cal_arith_coef_vis"
+
+    def inputs(self):
+        return {"rmu": self.rng.standard_normal(self.g.shape)}
+
+    def out_adjoints(self):
+        return {f"c{i}": self.rng.standard_normal(self.g.shape) for i in range(45)}
+
+    def head(self):
+        g = self.g
+        return [*grid_head(g), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+
+    def forward(self, call, x):
+        c = [self.g.zeros() for _ in range(45)]
+        call("cal_arith_coef_vis", *self.head(), x["rmu"].copy(), *c)
+        return {f"c{i}": c[i] for i in range(45)}
+
+    def adjoint(self, call, y):
+        mb = self.g.zeros()
+        cb = [y[f"c{i}"].copy() for i in range(45)]
+        call("cal_arith_coef_vis_b", *self.head(), mb, *cb)
+        assert not any(np.any(c) for c in cb)
+        return {"rmu": mb}
+
+
+class BnduP(FDCase):
+    """bndu_bp: the wall refresh's adjoint including the slip lengths and
+    phi."""
+    name = "bndu_bp"
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1700)
+        self.x0 = {k: r.standard_normal(g.shape) for k in ("u", "v", "w")}
+        self.x0.update({"uwall": np.array([0.37]), "l1": r.uniform(0.01, 0.1, g.shape),
+                        "l2": r.uniform(0.01, 0.1, g.shape), "phi": r.uniform(0.05, 0.95, g.shape)})
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("u", "v", "w")}
+
+    def value(self, call, x):
+        g = self.g
+        a = [x[k].copy() for k in ("u", "v", "w")]
+        call("bndu", WALLS.copy(), *grid_head(g), *a, D(float(x["uwall"][0])), D(g.dy),
+             x["l1"].copy(), x["l2"].copy(), x["phi"].copy())
+        return dict(zip(("u", "v", "w"), a))
+
+    def adjoint(self, call, y):
+        g, x0 = self.g, self.x0
+        a = [y[k].copy() for k in ("u", "v", "w")]
+        pb = {k: g.zeros() for k in ("l1", "l2", "phi")}
+        ub = D(0.0)
+        call("bndu_bp", WALLS.copy(), *grid_head(g), x0["u"], a[0], a[1], x0["w"], a[2],
+             D(float(x0["uwall"][0])), ub, D(g.dy), x0["l1"], pb["l1"], x0["l2"], pb["l2"],
+             x0["phi"], pb["phi"])
+        out = dict(zip(("u", "v", "w"), a))
+        out.update(pb)
+        out["uwall"] = np.array([ub.value])
+        return out
+
+
+class SorP(FDCase):
+    """solu_sor4_bp with the sweep count frozen at the unperturbed call's:
+    the adjoint with respect to the sources, the initial state, uwall, the
+    densities, the 45 coefficients, the slip lengths and phi."""
+    name = "solu_sor4_bp"
+    dt = 0.01
+    h = 1e-6
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1800)
+        rmu = r.uniform(0.01, 0.2, g.shape)
+        coef = [g.zeros() for _ in range(45)]
+        H.call_oracle("cal_arith_coef_vis", *grid_head(g), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz),
+                      rmu, *coef)
+        self.x0 = {f"c{i}": coef[i] for i in range(45)}
+        for k in ("src_u", "src_v", "src_w"):
+            self.x0[k] = r.standard_normal(g.shape)
+        for k in ("un", "vn", "wn"):
+            self.x0[k] = H.smooth_vel(g, r)
+        self.x0.update({"uwall": np.array([0.21]), "rho": r.uniform(0.5, 1.5, g.shape),
+                        "rhon": r.uniform(0.5, 1.5, g.shape), "l1": r.uniform(0.02, 0.1, g.shape),
+                        "l2": r.uniform(0.02, 0.1, g.shape), "phi": r.uniform(0.05, 0.95, g.shape)})
+        self.lps = []
+        self.lp = None
+
+    def head(self, x):
+        g = self.g
+        return [I(0), I(0), WALLS.copy(), I(1), *grid_head(g), np.zeros(8, np.int32), np.zeros(8),
+                np.zeros(8), I(1), I(0), D(self.dt)]
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("un", "vn", "wn")}
+
+    def value(self, call, x, lp_reader=None):
+        g = self.g
+        a = {k: x[k].copy() for k in ("un", "vn", "wn")}
+        call("solu_sor4", *self.head(x), x["rho"].copy(), x["rhon"].copy(),
+             *[x[f"c{i}"].copy() for i in range(45)],
+             *[x[k].copy() for k in ("src_u", "src_v", "src_w")], a["un"], a["vn"], a["wn"],
+             D(float(x["uwall"][0])), D(g.dy), x["l1"].copy(), x["l2"].copy(), x["phi"].copy())
+        self.lps.append((lp_reader or sor_lp_oracle)())
+        return a
+
+    def adjoint(self, call, y, lp=None):
+        g, x0 = self.g, self.x0
+        if lp is None:
+            self.value(H.call_oracle, x0)
+            lp = self.lps[-1]
+        self.lp = lp
+        assert all(v == lp for v in self.lps), self.lps
+        b = {k: g.zeros() for k in x0 if k not in ("un", "vn", "wn", "uwall")}
+        b.update({k: y[k].copy() for k in ("un", "vn", "wn")})
+        ub = D(0.0)
+        cargs = [x0[f"c{i}"] for i in range(45)] + [b[f"c{i}"] for i in range(45)]
+        pairs = []
+        for k in ("src_u", "src_v", "src_w", "un", "vn", "wn"):
+            pairs += [x0[k], b[k]]
+        call("solu_sor4_bp", *self.head(x0), x0["rho"], b["rho"], x0["rhon"], b["rhon"], *cargs,
+             *pairs, D(float(x0["uwall"][0])), ub, D(g.dy), x0["l1"], b["l1"], x0["l2"], b["l2"],
+             x0["phi"], b["phi"], I(lp))
+        b["uwall"] = np.array([ub.value])
+        return b
+
+
+def vertex_gradients(g: Grid, r):
+    """Vertex gradients of a smooth volume fraction plus noise (no zero
+    mean normal on the staggered boxes)."""
+    phi = H.smooth_phi(g, r)
+    o = vof_grad(g, phi)
+    return [a + 0.3 * r.standard_normal(g.shape) for a in o]
+
+
+class Calrn(FDCase):
+    name = "calrn"
+
+    def __init__(self, g, seed, stag=(1, 1, 1)):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 1900)
+        self.stag = stag
+        gx, gy, gz = vertex_gradients(g, r)
+        self.x0 = {"phix": gx, "phiy": gy, "phiz": gz}
+
+    def out_adjoints(self):
+        return {"rn": self.rng.standard_normal((3,) + self.g.shape)}
+
+    def head(self):
+        g = self.g
+        return [*grid_head(g), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), *[I(v) for v in self.stag]]
+
+    def value(self, call, x):
+        rn = self.g.zeros(3)
+        call("calrn", *self.head(), x["phix"].copy(), x["phiy"].copy(), x["phiz"].copy(), rn)
+        return {"rn": rn}
+
+    def adjoint(self, call, y):
+        g, x0 = self.g, self.x0
+        b = {k: g.zeros() for k in ("phix", "phiy", "phiz")}
+        rb = y["rn"].copy()
+        call("calrn_b", *self.head(), x0["phix"], b["phix"], x0["phiy"], b["phiy"], x0["phiz"],
+             b["phiz"], rb)
+        assert not np.any(rb)
+        return b
+
+
+class Calrkap(Case):
+    name = "calrkap"
+    keys = ("rn_pp", "rn_12", "rn_13", "rn_23")
+
+    def inputs(self):
+        return {k: self.rng.standard_normal((3,) + self.g.shape) for k in self.keys}
+
+    def out_adjoints(self):
+        return {"rkap": self.rng.standard_normal((3,) + self.g.shape)}
+
+    def head(self):
+        g = self.g
+        return [*grid_head(g), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+
+    def forward(self, call, x):
+        rk = self.g.zeros(3)
+        call("calrkap", *self.head(), *[x[k].copy() for k in self.keys], rk)
+        return {"rkap": rk}
+
+    def adjoint(self, call, y):
+        b = {k: self.g.zeros(3) for k in self.keys}
+        rk = y["rkap"].copy()
+        call("calrkap_b", *self.head(), *[b[k] for k in self.keys], rk)
+        assert not np.any(rk)
+        return b
+
+
+class GradP2uvw(Case):
+    name = "cal_grad_p2uvw"
+
+    def inputs(self):
+        return {"q": self.rng.standard_normal(self.g.shape)}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("qx", "qy", "qz")}
+
+    def head(self):
+        g = self.g
+        return [I(0), I(g.nj), *grid_head(g), D(1 / g.dx), np.full(g.nj + 6, 1 / g.dy), D(1 / g.dz)]
+
+    def forward(self, call, x):
+        o = [self.g.zeros() for _ in range(3)]
+        call("cal_grad_p2uvw", *self.head(), x["q"].copy(), *o)
+        return dict(zip(("qx", "qy", "qz"), o))
+
+    def adjoint(self, call, y):
+        qb = self.g.zeros()
+        o = [y[k].copy() for k in ("qx", "qy", "qz")]
+        call("cal_grad_p2uvw_b", *self.head(), qb, *o)
+        return {"q": qb, "_kept": o}
+
+
+class Calfst(FDCase):
+    name = "calfst"
+    rhol, rhog, sigma = 1.0, 0.01, 0.7
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 2000)
+        self.x0 = {k: r.standard_normal(g.shape) for k in ("gx", "gy", "gz")}
+        self.x0["rkap"] = r.standard_normal((3,) + g.shape)
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("fu", "fv", "fw")}
+
+    def consts(self):
+        return [D(self.rhol), D(self.rhog), D(self.sigma)]
+
+    def value(self, call, x):
+        g = self.g
+        f = [g.zeros() for _ in range(3)]
+        call("calfst", *grid_head(g), *self.consts(), x["gx"].copy(), x["gy"].copy(), x["gz"].copy(),
+             x["rkap"].copy(), *f)
+        return dict(zip(("fu", "fv", "fw"), f))
+
+    def adjoint(self, call, y):
+        g, x0 = self.g, self.x0
+        b = {k: g.zeros() for k in ("gx", "gy", "gz")}
+        b["rkap"] = g.zeros(3)
+        f = [y[k].copy() for k in ("fu", "fv", "fw")]
+        call("calfst_b", *grid_head(g), *self.consts(), x0["gx"], b["gx"], x0["gy"], b["gy"],
+             x0["gz"], b["gz"], x0["rkap"], b["rkap"], *f)
+        assert not any(np.any(a) for a in f)
+        return b
+
+
+class SumFst(Case):
+    name = "sum_fst"
+
+    def inputs(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("fu", "fv", "fw", "su", "sv", "sw")}
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("su", "sv", "sw")}
+
+    def forward(self, call, x):
+        s = [x[k].copy() for k in ("su", "sv", "sw")]
+        call("sum_fst", *grid_head(self.g), *[x[k].copy() for k in ("fu", "fv", "fw")], *s)
+        return dict(zip(("su", "sv", "sw"), s))
+
+    def adjoint(self, call, y):
+        f = [self.g.zeros() for _ in range(3)]
+        s = [y[k].copy() for k in ("su", "sv", "sw")]
+        call("sum_fst_b", *grid_head(self.g), *f, *s)
+        out = dict(zip(("fu", "fv", "fw"), f))
+        out.update(dict(zip(("su", "sv", "sw"), s)))
+        return out

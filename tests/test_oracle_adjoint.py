@@ -211,3 +211,59 @@ def test_planted_transpose_mistake_is_caught():
     wrong = case.forward(H.call_oracle, y)  # F used in place of F^T
     lhs, rhs = A.dot(fx, y), A.dot(x, wrong)
     assert abs(lhs - rhs) > 1e-6 * A.scale(fx, y)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.CoefVis, A.Calrkap, A.GradP2uvw, A.SumFst], ids=lambda c: c.name)
+def test_dot_product_properties_and_surface_tension(cls, gi):
+    dot_check(cls(GRIDS[gi], 261 + gi))
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("cls", [A.ArithMean, A.ArithTauP, A.SrcuP, A.BnduP, A.Calfst],
+                         ids=lambda c: c.name)
+def test_parameter_adjoints_against_differences(cls, gi):
+    """All-argument adjoints (_bp) and the nonlinear property and
+    surface-tension adjoints against Richardson central differences."""
+    dot_check(cls(GRIDS[gi], 271 + gi), rtol=1e-8)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+@pytest.mark.parametrize("stag", [(1, 1, 1), (0, 0, 1), (0, 1, 0), (1, 0, 0)], ids=str)
+def test_calrn_adjoint_against_differences(gi, stag):
+    dot_check(A.Calrn(GRIDS[gi], 281 + gi, stag), rtol=1e-8)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_sor_parameter_adjoint_against_differences(gi):
+    """solu_sor4_bp (densities, coefficients, slip lengths, phi, sources,
+    state, uwall) against Richardson central differences of the solve at a
+    frozen sweep count (asserted equal in every perturbed solve)."""
+    case = A.SorP(GRIDS[gi], 291 + gi)
+    dot_check(case, rtol=1e-8)
+    assert len(set(case.lps)) == 1
+
+
+def test_sor_parameter_adjoint_reduces_to_state_adjoint():
+    """With the parameter adjoints ignored, solu_sor4_bp gives what
+    solu_sor4_b gives (same sweeps reversed, same order)."""
+    g = GRIDS[0]
+    case = A.Sor(g, 301)
+    x = case.inputs()
+    case.forward(H.call_oracle, x)
+    y = case.out_adjoints()
+    b1 = case.adjoint(H.call_oracle, y)
+    p = A.SorP(g, 302)
+    p.x0.update({k: x[k] for k in Sor_keys()})
+    p.x0["uwall"] = x["uwall"]
+    p.x0["rho"], p.x0["rhon"] = case.rho, case.rhon
+    p.x0["l1"], p.x0["l2"], p.x0["phi"] = case.l1, case.l1, case.phi
+    for i in range(45):
+        p.x0[f"c{i}"] = case.coef[i]
+    b2 = p.adjoint(H.call_oracle, y)
+    for k in Sor_keys() + ("uwall",):
+        assert np.array_equal(b1[k], b2[k]), k
+
+
+def Sor_keys():
+    return ("src_u", "src_v", "src_w", "un", "vn", "wn")
