@@ -81,12 +81,69 @@ static void neumann_adj(const Dims d, double* q, bool lo, bool hi) {
   });
 }
 
+// Parameter adjoints of bndu (bndu_bp_): the call's input velocities uk,
+// wk (their rows 1 and nj are read), uwall, and the adjoints of the slip
+// lengths and phi; pa holds the phi_av adjoint of each wall cell (bottom
+// then top, one value per (i,k) column).  l1b == nullptr: none formed.
+struct BnduParam {
+  const double *uk, *wk;
+  double uwall;
+  double *l1b, *l2b, *phib, *pa;
+};
+
+// Adjoint of the slip coefficients of wall cell (i, jw, k) given c1_b and
+// c2_b: ls = 1/(phi_av/l1 + (1-phi_av)/l2), c1 = 2dy/(2ls+dy), c2 =
+// (2ls-dy)/(2ls+dy).  l1_b, l2_b of the cell incremented; phi_av's adjoint
+// returned (it belongs to two cells, gathered by bndu_phi_gather).
+__device__ __forceinline__ double wall_coef_b(const Dims& d, int i, int jw, int k, double dy,
+                                              const double* l1, const double* l2,
+                                              const double* phi, double c1b, double c2b,
+                                              double* l1b, double* l2b) {
+  const int ip = (i + 1) % (d.ni + 3);
+  const int cw = d.ix(i, jw, k);
+  const double phi_av = (phi[cw] + phi[d.ix(ip, jw, k)]) / 2.0;
+  const double inv1 = phi_av / l1[cw], inv2 = (1.0 - phi_av) / l2[cw];
+  const double ls = 1.0 / (inv1 + inv2);
+  const double dd = 2.0 * ls + dy;
+  const double lsb = c1b * (-4.0 * dy / (dd * dd)) + c2b * (4.0 * dy / (dd * dd));
+  const double sb = -lsb * ls * ls;
+  l1b[cw] = l1b[cw] - sb * phi_av / (l1[cw] * l1[cw]);
+  l2b[cw] = l2b[cw] - sb * (1.0 - phi_av) / (l2[cw] * l2[cw]);
+  return sb / l1[cw] - sb / l2[cw];
+}
+
+// phi_b of the wall rows from the phi_av adjoints: phi_av of column i is
+// the mean of phi at i and (i+1) mod (ni+3), so cell i receives half of its
+// own column's and of the columns i' with (i'+1) mod (ni+3) = i (i-1 for
+// i in [-1, ni+2]; also ni+2 for i = 0 and ni+3 for i = 1).
+static void bndu_phi_gather(const Dims d, const BnduParam bp, bool lo, bool hi) {
+  const int ni = d.ni, nj = d.nj, sx = d.sx, nw = d.sx * d.sz;
+  launch_flat(nw, [=] __device__(int m) {
+    const int i = m % sx - 2, k = m / sx - 2;
+    for (int t = 0; t < 2; t++) {
+      if ((t == 0 && !lo) || (t == 1 && !hi)) continue;
+      const double* pa = bp.pa + t * nw;
+      double s = pa[m];
+      if (i >= -1) s = s + pa[m - 1];
+      if (i == 0) s = s + pa[(ni + 4) + (k + 2) * sx];
+      if (i == 1) s = s + pa[(ni + 5) + (k + 2) * sx];
+      const int c = d.ix(i, t == 0 ? 1 : nj, k);
+      bp.phib[c] = bp.phib[c] + 0.5 * s;
+    }
+  });
+}
+
 // srcs/bndu.f:29-158 transposed: the z fills, the x fills (with the shifted
 // sources of u in x and of w in z), then the wall rows, top first.  `wacc`
 // (one value per (i,k) wall column) gathers the uwall weights: -c1 x u_b of
-// the bottom ghost and +c1 x u_b of the top one.
+// the bottom ghost and +c1 x u_b of the top one.  With bp.l1b set, the
+// adjoints of the slip lengths and phi are formed too: a wall row's ghost
+// u0 = c1 (-+uwall) + c2 u1 (w0 = c2 w1), reversed with adjoint A (the rows
+// beyond folded in), gives c1_b = A (-+uwall) and c2_b = A_u u1 + A_w w1.
 static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc, double dy,
-                     const double* l1, const double* l2, const double* phi, bool lo, bool hi) {
+                     const double* l1, const double* l2, const double* phi, bool lo, bool hi,
+                     const BnduParam bp = BnduParam{nullptr, nullptr, 0.0, nullptr, nullptr,
+                                                    nullptr, nullptr}) {
   const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, sy = d.sy, Z = d.sx * d.sy;
   // z fills (bndu.f:130-155), every (i,j) line, statements reversed
   launch_flat(sx * sy, [=] __device__(int c) {
@@ -110,13 +167,15 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
     take(u, ix(-2), ix(ni - 1)); take(u, ix(-1), ix(ni)); take(u, ix(0), ix(ni + 1));
   });
   if (!lo && !hi) return;
+  const int nw = sx * d.sz;
   // wall rows (bndu.f:29-98), every (i,k) column, top wall first
-  launch_flat(sx * d.sz, [=] __device__(int m) {
+  launch_flat(nw, [=] __device__(int m) {
     const int i = m % sx - 2, k = m / sx - 2;
     auto jy = [=](int j) { return d.ix(i, j, k); };
     double acc = 0.0;
     if (hi) {
       const WallCoef<double> cf = wall_coef(d, i, nj, k, dy, l1, l2, phi);
+      double c1b = 0.0, c2b = 0.0;
       // u, w: g1 = c1 (+uwall) + c2 q(nj); g2 = 2 g1 - q(nj); g3 = 2 g2 - g1
       double* qs[2] = {w, u};
       for (int t = 0; t < 2; t++) {
@@ -132,8 +191,18 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
         const double b1 = q[jy(nj + 1)];
         q[jy(nj)] = q[jy(nj)] + cf.c2 * b1;
         if (t == 1) acc = acc + cf.c1 * b1;
+        if (bp.l1b) {
+          if (t == 0) {
+            c2b = c2b + b1 * bp.wk[jy(nj)];
+          } else {
+            c1b = c1b + b1 * bp.uwall;
+            c2b = c2b + b1 * bp.uk[jy(nj)];
+          }
+        }
         q[jy(nj + 1)] = 0.0;
       }
+      if (bp.l1b)
+        bp.pa[m + nw] = wall_coef_b(d, i, nj, k, dy, l1, l2, phi, c1b, c2b, bp.l1b, bp.l2b);
       // v: v(nj) = 0, v(nj+g) = -v(nj-g)
       for (int g = 3; g >= 1; g--) {
         v[jy(nj - g)] = v[jy(nj - g)] - v[jy(nj + g)];
@@ -143,6 +212,7 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
     }
     if (lo) {
       const WallCoef<double> cf = wall_coef(d, i, 1, k, dy, l1, l2, phi);
+      double c1b = 0.0, c2b = 0.0;
       double* qs[2] = {w, u};
       for (int t = 0; t < 2; t++) {
         double* q = qs[t];
@@ -157,8 +227,17 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
         const double b1 = q[jy(0)];
         q[jy(1)] = q[jy(1)] + cf.c2 * b1;
         if (t == 1) acc = acc - cf.c1 * b1;
+        if (bp.l1b) {
+          if (t == 0) {
+            c2b = c2b + b1 * bp.wk[jy(1)];
+          } else {
+            c1b = c1b - b1 * bp.uwall;
+            c2b = c2b + b1 * bp.uk[jy(1)];
+          }
+        }
         q[jy(0)] = 0.0;
       }
+      if (bp.l1b) bp.pa[m] = wall_coef_b(d, i, 1, k, dy, l1, l2, phi, c1b, c2b, bp.l1b, bp.l2b);
       // v: v(-2) = -v(2), v(-1) = -v(1), v(0) = 0
       v[jy(0)] = 0.0;
       v[jy(1)] = v[jy(1)] - v[jy(-1)];
@@ -168,6 +247,7 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
     }
     if (wacc) wacc[m] = wacc[m] + acc;
   });
+  if (bp.l1b) bndu_phi_gather(d, bp, lo, hi);
 }
 
 // Sum of the n values of a device array, added to *out on the host.
@@ -755,6 +835,407 @@ static void weno_adj_dir(const Dims d, int q, int dd, double dinv, const double*
   });
 }
 
+
+// ---------------------------------------------------------------------------
+// Parameter adjoints (<name>_bp_): what <name>_b_ does, plus the adjoints of
+// the arrays it takes as fixed (densities, viscosity, coefficients, slip
+// lengths, phi), incremented.
+// ---------------------------------------------------------------------------
+
+// Terms of component P's Gauss-Seidel update (srcs/solu_sor4.f:137-152,
+// 186-201, 236-251) in the order the forward sums them: coefficient index
+// within the component's 15, neighbour component, offset.
+template <int P>
+__device__ __forceinline__ void sor_terms(const Dims& d, int* idx, int* comp, int* off) {
+  const int X = 1, Y = d.sx, Z = d.sx * d.sy;
+  const int own_i[6] = {4, 2, 0, 1, 3, 5};
+  const int own_o[6] = {-Z, -Y, -X, X, Y, Z};
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    idx[t] = own_i[t];
+    comp[t] = P;
+    off[t] = own_o[t];
+  }
+  if (P == 0) {
+    const int i8[8] = {7, 8, 9, 10, 11, 12, 13, 14}, c8[8] = {1, 1, 1, 1, 2, 2, 2, 2};
+    const int o8[8] = {-Y, X - Y, 0, X, -Z, X - Z, 0, X};
+#pragma unroll
+    for (int t = 0; t < 8; t++) { idx[6 + t] = i8[t]; comp[6 + t] = c8[t]; off[6 + t] = o8[t]; }
+  } else if (P == 1) {
+    const int i8[8] = {7, 9, 8, 10, 11, 12, 13, 14}, c8[8] = {0, 0, 0, 0, 2, 2, 2, 2};
+    const int o8[8] = {-X, 0, -X + Y, Y, -Z, Y - Z, 0, Y};
+#pragma unroll
+    for (int t = 0; t < 8; t++) { idx[6 + t] = i8[t]; comp[6 + t] = c8[t]; off[6 + t] = o8[t]; }
+  } else {
+    const int i8[8] = {7, 9, 8, 10, 11, 13, 12, 14}, c8[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    const int o8[8] = {-X, 0, -X + Z, Z, -Y, 0, -Y + Z, Z};
+#pragma unroll
+    for (int t = 0; t < 8; t++) { idx[6 + t] = i8[t]; comp[6 + t] = c8[t]; off[6 + t] = o8[t]; }
+  }
+}
+
+// The cells of component P of colour pair p (both colours), as in
+// colour_scale: f(c) for each.
+template <int P, class F>
+static void for_pair_cells(const Dims d, int jen, int p, F f) {
+  const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
+  const int jend = P == 1 ? jen : d.nj;
+  int h[2][3], base[2][3], cnt[2];
+  for (int t = 0; t < 2; t++) {
+    const int ps[3] = {t ? 3 - is : is, t ? 3 - js : js, t ? 3 - ks : ks};
+    const int top[3] = {d.ni, jend, d.nk};
+    cnt[t] = 1;
+    for (int q = 0; q < 3; q++) {
+      base[t][q] = ps[q];
+      h[t][q] = top[q] >= ps[q] ? (top[q] - ps[q]) / 2 + 1 : 0;
+      cnt[t] *= h[t][q];
+    }
+  }
+  const int n0 = cnt[0];
+  const int hx0 = h[0][0], hy0 = h[0][1], hx1 = h[1][0], hy1 = h[1][1];
+  const int b00 = base[0][0], b01 = base[0][1], b02 = base[0][2];
+  const int b10 = base[1][0], b11 = base[1][1], b12 = base[1][2];
+  launch_flat(cnt[0] + cnt[1], [=] __device__(int m) {
+    const bool second = m >= n0;
+    const int mm = second ? m - n0 : m;
+    const int hx = second ? hx1 : hx0, hy = second ? hy1 : hy0;
+    const int ii = mm % hx, t = mm / hx, jj = t % hy, kk = t / hy;
+    const int c = second ? d.ix(b10 + 2 * ii, b11 + 2 * jj, b12 + 2 * kk)
+                         : d.ix(b00 + 2 * ii, b01 + 2 * jj, b02 + 2 * kk);
+    f(c);
+  });
+}
+
+// Arrays of the all-argument Gauss-Seidel adjoint.
+struct SorParam {
+  const double* src[3];  // primal sources
+  double* cfb[45];       // coefficient adjoints
+  double *rhob, *rhonb;
+};
+
+// Forward update of colour pair p of component P on x (srcs/solu_sor4.f:
+// 131-175 with coefficient arrays): the pair's cells read no cell of the
+// pair, so one pass.
+template <int P>
+static void pair_fwd(const Dims d, const SorAdj& a, const SorParam& sp, double* const* x, int p) {
+  const double *rho = a.rho, *rhon = a.rhon, *src = sp.src[P];
+  const double dtinv = a.dtinv;
+  double* x0 = x[0];
+  double* x1 = x[1];
+  double* x2 = x[2];
+  const SorAdj A = a;
+  for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
+    int idx[14], comp[14], off[14];
+    sor_terms<P>(d, idx, comp, off);
+    const double* xs[3] = {x0, x1, x2};
+    const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
+    const double r00 = (rho[c] + rho[c + F] + rhon[c] + rhon[c + F]) * 0.25;
+    double sum = 0.0;
+#pragma unroll
+    for (int t = 0; t < 14; t++) sum = sum + A.cf[15 * P + idx[t]][c] * xs[comp[t]][c + off[t]];
+    const double visn = sum / r00;
+    double* xp = P == 0 ? x0 : (P == 1 ? x1 : x2);
+    const double du = (visn * 0.5 + src[c]) / (dtinv + A.cf[15 * P + 6][c] / r00 * 0.5) - xp[c];
+    xp[c] = xp[c] + du;
+  });
+}
+
+// Parameter adjoints of colour pair p of component P, between colour_scale
+// and colour_gather (the pair cells' state adjoints still in place): for
+// x = (0.5 S/r00 + src)/den, den = dtinv + 0.5 a_p/r00, S = sum a_t y_t over
+// the state xs before the pair, g = x_b/den and h = 0.5 g/r00:
+// a_t_b += h y_t, a_p_b -= h x, r00_b = g (0.5 x a_p - 0.5 S)/r00^2 spread
+// over rho and rhon at c and c+F.  No two cells of a pair are F apart, so
+// the scatters do not collide.
+template <int P>
+static void pair_param(const Dims d, const SorAdj& a, const SorParam& sp,
+                       const double* const* xs, int p) {
+  const double *rho = a.rho, *rhon = a.rhon, *src = sp.src[P];
+  const double* lb = a.lb[P];
+  const double dtinv = a.dtinv;
+  const double *s0 = xs[0], *s1 = xs[1], *s2 = xs[2];
+  const SorAdj A = a;
+  const SorParam S = sp;
+  for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
+    int idx[14], comp[14], off[14];
+    sor_terms<P>(d, idx, comp, off);
+    const double* st[3] = {s0, s1, s2};
+    const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
+    const double r00 = (rho[c] + rho[c + F] + rhon[c] + rhon[c + F]) * 0.25;
+    const double ap = A.cf[15 * P + 6][c];
+    const double den = dtinv + ap / r00 * 0.5;
+    double sum = 0.0;
+#pragma unroll
+    for (int t = 0; t < 14; t++) sum = sum + A.cf[15 * P + idx[t]][c] * st[comp[t]][c + off[t]];
+    const double x = (sum / r00 * 0.5 + src[c]) / den;
+    const double g = lb[c] / den;
+    const double h = g * 0.5 / r00;
+#pragma unroll
+    for (int t = 0; t < 14; t++) {
+      double* cb = S.cfb[15 * P + idx[t]];
+      cb[c] = cb[c] + h * st[comp[t]][c + off[t]];
+    }
+    double* cbp = S.cfb[15 * P + 6];
+    cbp[c] = cbp[c] - h * x;
+    const double rb = 0.25 * g * (0.5 * x * ap - 0.5 * sum) / (r00 * r00);
+    S.rhob[c] = S.rhob[c] + rb;
+    S.rhob[c + F] = S.rhob[c + F] + rb;
+    S.rhonb[c] = S.rhonb[c] + rb;
+    S.rhonb[c + F] = S.rhonb[c + F] + rb;
+  });
+}
+
+static void copy_dev(double* dst, const double* src, int n) {
+  launch_flat(n, [=] __device__(int m) { dst[m] = src[m]; });
+}
+
+// Viscosity adjoint of the 45 coefficients (srcs/cal_arith_coef_vis.f):
+// each coefficient of component q at cell c is a multiple of one of six
+// face viscosities (face f: axis f/2, side f%2), the cell value of c or of
+// its neighbour along q's own axis, or along the other axes the mean of
+// the four cells {c+o, c+o+e_q, c+o+e_a, c+o+e_q+e_a} (o = -e_a on the
+// minus side); a_p is the weighted sum of the six.  The face adjoint of
+// (q, f) at c is d2_a (a_f + a_p) (doubled on q's own axis) plus, off the
+// own axis, dinv_q dinv_a (a_A - a_B) or (-a_A + a_B) for the n-th
+// cross pair (7+2n, 8+2n).  Every cell gathers the face adjoints of the
+// cells whose faces include it (coefficients exist on [-1, n+2]^3); the
+// coefficient adjoints are then cleared whole.
+static void coef_vis_adj(const Dims d, double dxinv, double dyinv, double dzinv, double* rmub,
+                         double* const* cb) {
+  struct Cb { const double* a[45]; };
+  Cb C;
+  for (int m = 0; m < 45; m++) C.a[m] = cb[m];
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const double dinv[3] = {dxinv, dyinv, dzinv};
+  const double di0 = dinv[0], di1 = dinv[1], di2 = dinv[2];
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const double dv[3] = {di0, di1, di2};
+    auto fa = [&](int q, int f, int ci, int cj, int ck) -> double {
+      if (ci < -1 || ci > ni + 2 || cj < -1 || cj > nj + 2 || ck < -1 || ck > nk + 2) return 0.0;
+      const int c = d.ix(ci, cj, ck);
+      const double* const* a = C.a + 15 * q;
+      const int ax = f / 2;
+      const double base = dv[ax] * dv[ax] * (a[f][c] + a[6][c]);
+      if (ax == q) return 2.0 * base;
+      const int n = f < 2 * q ? f : f - 2;
+      const int ia = 7 + 2 * n, ib = 8 + 2 * n;
+      const double cr = (n % 2 == 0) ? (a[ia][c] - a[ib][c]) : (-a[ia][c] + a[ib][c]);
+      return base + dv[q] * dv[ax] * cr;
+    };
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        const int ax = f / 2, side = f % 2;
+        int e_a[3] = {0, 0, 0}, e_q[3] = {0, 0, 0};
+        e_a[ax] = 1;
+        e_q[q] = 1;
+        if (ax == q) {
+          // single cell c + side e_a
+          acc = acc + fa(q, f, i - side * e_a[0], j - side * e_a[1], k - side * e_a[2]);
+        } else {
+          const int o[3] = {side ? 0 : -e_a[0], side ? 0 : -e_a[1], side ? 0 : -e_a[2]};
+#pragma unroll
+          for (int al = 0; al < 2; al++)
+#pragma unroll
+            for (int be = 0; be < 2; be++)
+              acc = acc + 0.25 * fa(q, f, i - o[0] - al * e_q[0] - be * e_a[0],
+                                    j - o[1] - al * e_q[1] - be * e_a[1],
+                                    k - o[2] - al * e_q[2] - be * e_a[2]);
+        }
+      }
+    const int c = d.ix(i, j, k);
+    rmub[c] = rmub[c] + acc;
+  });
+  for (int m = 0; m < 45; m++) {
+    double* a = cb[m];
+    launch_flat(d.n3, [=] __device__(int e) { a[e] = 0.0; });
+  }
+}
+
+// Viscosity adjoint of cal_arith_tau (before arith_tau_adj consumes tau_b):
+// the normal stresses read rmu at their cell, each shear stress the mean of
+// four cells; every cell gathers 2 s tau_b of the stresses that read it.
+static void arith_tau_param(const Dims d, const double* s, double* rmub, const double* taub) {
+  const SijBoxes S = sij_boxes(d);
+  const size_t n3 = (size_t)d.n3;
+  launch_box(Box{-2, d.ni + 3, -2, d.nj + 3, -2, d.nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; l++) acc = acc + taub[l * n3 + c] * s[l * n3 + c] * 2.0;
+    // shear l reads (c, c+a, c+b, c+a+b): sources c, c-a, c-b, c-a-b
+    const int ai[3] = {1, 1, 0}, aj[3] = {0, 0, 1}, ak[3] = {0, 0, 0};
+    const int bi[3] = {0, 0, 0}, bj[3] = {1, 0, 0}, bk[3] = {0, 1, 1};
+    const int ao[3] = {X, X, Y}, bo[3] = {Y, Z, Z};
+#pragma unroll
+    for (int l = 3; l < 6; l++) {
+      const int t = l - 3;
+#pragma unroll
+      for (int al = 0; al < 2; al++)
+#pragma unroll
+        for (int be = 0; be < 2; be++) {
+          const int si = i - al * ai[t] - be * bi[t], sj = j - al * aj[t] - be * bj[t],
+                    sk = k - al * ak[t] - be * bk[t];
+          if (!S.b[l].has(si, sj, sk)) continue;
+          const int sc = c - al * ao[t] - be * bo[t];
+          acc = acc + 0.25 * taub[l * n3 + sc] * s[l * n3 + sc] * 2.0;
+        }
+    }
+    rmub[c] = rmub[c] + acc;
+  });
+}
+
+// Density adjoints of cal_srcu (before srcu_adj consumes src_b): the face
+// density r = (rho + rho(+F) + rhon + rhon(+F))/4 of component q enters as
+// -prs/r + 0.5 vis/r, so r_b = src_b (prs - 0.5 vis)/r^2; every cell
+// gathers a quarter of r_b of the faces at c and c-F of each component.
+static void srcu_param(const Dims d, int jen, const double* const* prs, const double* const* vis,
+                       const double* const* srcb, const double* rho, const double* rhon,
+                       double* rhob, double* rhonb) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const double *p0 = prs[0], *p1 = prs[1], *p2 = prs[2];
+  const double *v0 = vis[0], *v1 = vis[1], *v2 = vis[2];
+  const double *s0 = srcb[0], *s1 = srcb[1], *s2 = srcb[2];
+  launch_box(Box{1, ni + 1, 1, nj + 1, 1, nk + 1}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    const int F[3] = {1, d.sx, d.sx * d.sy};
+    const double* P[3] = {p0, p1, p2};
+    const double* V[3] = {v0, v1, v2};
+    const double* Sb[3] = {s0, s1, s2};
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int jend = q == 1 ? jen : nj;
+      const int di[3] = {q == 0, q == 1, q == 2};
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int ci = i - t * di[0], cj = j - t * di[1], ck = k - t * di[2];
+        if (ci < 1 || ci > ni || cj < 1 || cj > jend || ck < 1 || ck > nk) continue;
+        const int cc = c - t * F[q];
+        const double r = (rho[cc] + rho[cc + F[q]] + rhon[cc] + rhon[cc + F[q]]) * 0.25;
+        acc = acc + 0.25 * Sb[q][cc] * (P[q][cc] - 0.5 * V[q][cc]) / (r * r);
+      }
+    }
+    rhob[c] = rhob[c] + acc;
+    rhonb[c] = rhonb[c] + acc;
+  });
+}
+
+// srcs/calrn.f:1-95 reversed (normals rn = r/(|r| + verysmall) of the mean
+// r of eight vertex gradients, on the box of staggering (ip,jp,kp), zero
+// elsewhere): first r_b per box cell into R (d|r|/dr taken as 0 at r = 0),
+// then every vertex gathers r_b/8 of the box cells whose eight vertices
+// include it (a vertex counted twice when a staggering offset is 0, as the
+// forward sums it twice); rn_b is consumed whole.
+static void calrn_adj(const Dims d, double dxinv, double dyinv, double dzinv, int ip, int jp,
+                      int kp, const double* const* g, double* const* gb, double* rnb, double* R) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const size_t n3 = (size_t)d.n3;
+  const double verysmall = sqrt(dxinv * dxinv + dyinv * dyinv + dzinv * dzinv) * 1.0e-30;
+  const double *g0 = g[0], *g1 = g[1], *g2 = g[2];
+  launch_box(Box{-2, ni + 3 - ip, -2, nj + 3 - jp, -2, nk + 3 - kp},
+             [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = ip, Y = jp * d.sx, Z = kp * d.sx * d.sy;
+    const int v[8] = {c, c + X, c + Y, c + X + Y, c + Z, c + X + Z, c + Y + Z, c + X + Y + Z};
+    const double* a[3] = {g0, g1, g2};
+    double r[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const double* p = a[q];
+      r[q] = (+p[v[0]] + p[v[1]] + p[v[2]] + p[v[3]] + p[v[4]] + p[v[5]] + p[v[6]] + p[v[7]]) *
+             0.125;
+    }
+    const double nr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double inv = 1.0 / (nr + verysmall);
+    const int o = c + X + Y + Z;
+    const double b0 = rnb[o], b1 = rnb[n3 + o], b2 = rnb[2 * n3 + o];
+    const double nrb = -(b0 * r[0] + b1 * r[1] + b2 * r[2]) * inv * inv;
+    const double gr = nr > 0.0 ? nrb / nr : 0.0;
+    R[c] = b0 * inv + gr * r[0];
+    R[n3 + c] = b1 * inv + gr * r[1];
+    R[2 * n3 + c] = b2 * inv + gr * r[2];
+  });
+  double *h0 = gb[0], *h1 = gb[1], *h2 = gb[2];
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const int si = i - a * ip, sj = j - b * jp, sk = k - e * kp;
+          if (si < -2 || si > ni + 3 - ip || sj < -2 || sj > nj + 3 - jp || sk < -2 ||
+              sk > nk + 3 - kp)
+            continue;
+          const int s = d.ix(si, sj, sk);
+#pragma unroll
+          for (int q = 0; q < 3; q++) acc[q] = acc[q] + R[q * n3 + s] * 0.125;
+        }
+    h0[c] = h0[c] + acc[0];
+    h1[c] = h1[c] + acc[1];
+    h2[c] = h2[c] + acc[2];
+    rnb[c] = 0.0;
+    rnb[n3 + c] = 0.0;
+    rnb[2 * n3 + c] = 0.0;
+  });
+}
+
+// srcs/calrkap.f:1-102 reversed: every normal component gathers the
+// curvature adjoints of the (at most two) box cells that difference it;
+// then the curvature adjoints are cleared whole.
+static void calrkap_adj(const Dims d, double dxinv, double dyinv, double dzinv, double* pp,
+                        double* r12, double* r13, double* r23, double* kb) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const size_t n3 = (size_t)d.n3;
+  const IBox B1{-2, ni + 2, -1, nj + 3, -1, nk + 3}, B2{-1, ni + 3, -2, nj + 2, -1, nk + 3},
+      B3{-1, ni + 3, -1, nj + 3, -2, nk + 2};
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    // curvature adjoint of component t at offset (di,dj,dk), 0 off its box
+    auto t1 = [&](int di, int dj, int dk, int o) { return B1.has(i + di, j + dj, k + dk) ? kb[c + o] : 0.0; };
+    auto t2 = [&](int di, int dj, int dk, int o) { return B2.has(i + di, j + dj, k + dk) ? kb[n3 + c + o] : 0.0; };
+    auto t3 = [&](int di, int dj, int dk, int o) { return B3.has(i + di, j + dj, k + dk) ? kb[2 * n3 + c + o] : 0.0; };
+    pp[c] = pp[c] + (t1(-1, 0, 0, -X) - t1(0, 0, 0, 0)) * dxinv;
+    pp[n3 + c] = pp[n3 + c] + (t2(0, -1, 0, -Y) - t2(0, 0, 0, 0)) * dyinv;
+    pp[2 * n3 + c] = pp[2 * n3 + c] + (t3(0, 0, -1, -Z) - t3(0, 0, 0, 0)) * dzinv;
+    r12[c] = r12[c] + (t2(0, 0, 0, 0) - t2(1, 0, 0, X)) * dxinv;
+    r12[n3 + c] = r12[n3 + c] + (t1(0, 0, 0, 0) - t1(0, 1, 0, Y)) * dyinv;
+    r13[c] = r13[c] + (t3(0, 0, 0, 0) - t3(1, 0, 0, X)) * dxinv;
+    r13[2 * n3 + c] = r13[2 * n3 + c] + (t1(0, 0, 0, 0) - t1(0, 0, 1, Z)) * dzinv;
+    r23[n3 + c] = r23[n3 + c] + (t3(0, 0, 0, 0) - t3(0, 1, 0, Y)) * dyinv;
+    r23[2 * n3 + c] = r23[2 * n3 + c] + (t2(0, 0, 0, 0) - t2(0, 0, 1, Z)) * dzinv;
+  });
+  launch_flat(3 * d.n3, [=] __device__(int m) { kb[m] = 0.0; });
+}
+
+// srcs/cal_grad_p2uvw.f:1-47 reversed (forward differences on [-2,n+2]^3):
+// q_b gathers the face adjoints at c (minus) and at c-X, c-Y, c-Z (plus);
+// then the face adjoints are cleared on the box.
+static void grad_p2uvw_adj(const Dims d, double dxinv, const double* dyinv_row, double dzinv,
+                           double* qb, double* gx, double* gy, double* gz) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  const IBox B{-2, ni + 2, -2, nj + 2, -2, nk + 2};
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k), X = 1, Y = d.sx, Z = d.sx * d.sy;
+    double acc = 0.0;
+    if (B.has(i, j, k)) acc = acc - gx[c] * dxinv - gy[c] * dyinv_row[j + 2] - gz[c] * dzinv;
+    if (B.has(i - 1, j, k)) acc = acc + gx[c - X] * dxinv;
+    if (B.has(i, j - 1, k)) acc = acc + gy[c - Y] * dyinv_row[j + 1];
+    if (B.has(i, j, k - 1)) acc = acc + gz[c - Z] * dzinv;
+    qb[c] = qb[c] + acc;
+  });
+  launch_box(Box{-2, ni + 2, -2, nj + 2, -2, nk + 2}, [=] __device__(int i, int j, int k) {
+    const int c = d.ix(i, j, k);
+    gx[c] = 0.0;
+    gy[c] = 0.0;
+    gz[c] = 0.0;
+  });
+}
+
 }  // namespace fel
 
 using namespace fel;
@@ -1041,6 +1522,391 @@ void cal_advu_weno5_b_(const int* ni, const int* nj, const int* nk, const double
     double* a = advb[q];
     launch_flat(d.n3, [=] __device__(int m) { a[m] = 0.0; });
   }
+  c.finish();
+}
+
+
+// ---------------------------------------------------------------------------
+// Parameter (all-argument) adjoints and the adjoints of the material
+// properties and the surface-tension chain (entry points).
+// ---------------------------------------------------------------------------
+
+// All-argument adjoint of bndu_: bndu_b_ plus the adjoints of l1, l2 and phi
+// (incremented).  uk, wk are the call's input velocities (rows 1 and nj are
+// read), each followed by its adjoint; uwall is followed by *uwallb.
+void bndu_bp_(const int* nID, const int* ni, const int* nj, const int* nk, const double* uk,
+              double* ukb, double* vkb, const double* wk, double* wkb, const double* uwall,
+              double* uwallb, const double* dy, const double* l1, double* l1b, const double* l2,
+              double* l2b, const double* phi, double* phib) {
+  adjoint_fp64_only("bndu_bp_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int nw = d.sx * d.sz;
+  double* wacc = static_cast<double*>(scratch(25, sizeof(double) * nw));
+  double* pa = static_cast<double*>(scratch(43, sizeof(double) * 2 * nw));
+  launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
+  Call c("bndu_bp_");
+  BnduParam bp;
+  bp.uk = c.in(uk, d.n3, "uk");
+  bp.wk = c.in(wk, d.n3, "wk");
+  bp.uwall = *uwall;
+  double* ub = c.io(ukb, d.n3, "ukb");
+  double* vb = c.io(vkb, d.n3, "vkb");
+  double* wb = c.io(wkb, d.n3, "wkb");
+  const double* dl1 = c.in(l1, d.n3, "l1");
+  const double* dl2 = c.in(l2, d.n3, "l2");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  bp.l1b = c.io(l1b, d.n3, "l1b");
+  bp.l2b = c.io(l2b, d.n3, "l2b");
+  bp.phib = c.io(phib, d.n3, "phib");
+  bp.pa = pa;
+  bndu_adj(d, ub, vb, wb, wacc, *dy, dl1, dl2, dphi, nID[kYMinus] < 0, nID[kYPlus] < 0, bp);
+  adjoint_add_sum(wacc, nw, uwallb);
+  c.finish();
+}
+
+// All-argument adjoint of solu_sor4_ with the forward call's sweep count
+// lp frozen: solu_sor4_b_ plus the adjoints of rho, rhon, the 45
+// coefficients, l1, l2 and phi (incremented).  Each of rho, rhon, the
+// coefficients (all 45, then their 45 adjoints), src_*, un, vn, wn (the
+// call's inputs), uwall, l1, l2, phi is followed by its adjoint.  The
+// sweeps are recomputed from the inputs, one sweep at a time before it is
+// reversed, keeping the state before each colour pair; each pair's
+// parameter adjoints are formed at that state (pair_param), and each
+// refresh's bndu is reversed with its parameter adjoints at the state
+// before the refresh (the kept state with the pair's cells updated).
+// Device work memory: 3 (nsw + 1) + 37 arrays of the padded grid.
+void solu_sor4_bp_(const int* ipara, const int* ID, const int* nID, const int* ndiv, const int* ni,
+                   const int* nj, const int* nk, const int* key, double* sendjb, double* recvjb,
+                   const int* nstep, const int* imon_t, const double* dt, const double* rho,
+                   double* rhob, const double* rhon, double* rhonb, double* c0, double* c1, double* c2, double* c3, double* c4, double* c5, double* c6, double* c7, double* c8, double* c9, double* c10, double* c11, double* c12, double* c13, double* c14, double* c15, double* c16, double* c17, double* c18, double* c19, double* c20, double* c21, double* c22, double* c23, double* c24, double* c25, double* c26, double* c27, double* c28, double* c29, double* c30, double* c31, double* c32, double* c33, double* c34, double* c35, double* c36, double* c37, double* c38, double* c39, double* c40, double* c41, double* c42, double* c43, double* c44, double* c0b, double* c1b, double* c2b, double* c3b, double* c4b, double* c5b, double* c6b, double* c7b, double* c8b, double* c9b, double* c10b, double* c11b, double* c12b, double* c13b, double* c14b, double* c15b, double* c16b, double* c17b, double* c18b, double* c19b, double* c20b, double* c21b, double* c22b, double* c23b, double* c24b, double* c25b, double* c26b, double* c27b, double* c28b, double* c29b, double* c30b, double* c31b, double* c32b, double* c33b, double* c34b, double* c35b, double* c36b, double* c37b, double* c38b, double* c39b, double* c40b, double* c41b, double* c42b, double* c43b, double* c44b,
+                   const double* src_u, double* src_ub, const double* src_v, double* src_vb,
+                   const double* src_w, double* src_wb, const double* un, double* unb,
+                   const double* vn, double* vnb, const double* wn, double* wnb,
+                   const double* uwall, double* uwallb, const double* dy, const double* l1,
+                   double* l1b, const double* l2, double* l2b, const double* phi, double* phib,
+                   const int* lp) {
+  (void)ID; (void)ndiv; (void)key; (void)sendjb; (void)recvjb; (void)nstep; (void)imon_t;
+  if (*ipara != 0) {
+    std::fprintf(stderr, "felgpu: solu_sor4_bp_: only the single-domain case is supported\n");
+    std::abort();
+  }
+  if (*lp < 0 || *lp > 21) {
+    std::fprintf(stderr, "felgpu: solu_sor4_bp_: lp = %d is not a sweep index of solu_sor4_ (0..21)\n", *lp);
+    std::abort();
+  }
+  adjoint_fp64_only("solu_sor4_bp_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  double* h[45] = {c0, c1, c2, c3, c4, c5, c6, c7, c8, c9, c10, c11, c12, c13, c14, c15, c16, c17, c18, c19, c20, c21, c22, c23, c24, c25, c26, c27, c28, c29, c30, c31, c32, c33, c34, c35, c36, c37, c38, c39, c40, c41, c42, c43, c44};
+  double* hb[45] = {c0b, c1b, c2b, c3b, c4b, c5b, c6b, c7b, c8b, c9b, c10b, c11b, c12b, c13b, c14b, c15b, c16b, c17b, c18b, c19b, c20b, c21b, c22b, c23b, c24b, c25b, c26b, c27b, c28b, c29b, c30b, c31b, c32b, c33b, c34b, c35b, c36b, c37b, c38b, c39b, c40b, c41b, c42b, c43b, c44b};
+  Call c("solu_sor4_bp_");
+  SorAdj a;
+  SorParam sp;
+  a.rho = c.in(rho, d.n3, "rho");
+  a.rhon = c.in(rhon, d.n3, "rhon");
+  sp.rhob = c.io(rhob, d.n3, "rhob");
+  sp.rhonb = c.io(rhonb, d.n3, "rhonb");
+  a.rmu = nullptr;
+  a.sc = vis_scales<double>(0.0, 0.0, 0.0);
+  for (int m = 0; m < 45; m++) a.cf[m] = c.in(h[m], d.n3, "coefficient");
+  for (int m = 0; m < 45; m++) sp.cfb[m] = c.io(hb[m], d.n3, "coefficient adjoint");
+  sp.src[0] = c.in(src_u, d.n3, "src_u");
+  sp.src[1] = c.in(src_v, d.n3, "src_v");
+  sp.src[2] = c.in(src_w, d.n3, "src_w");
+  a.srcb[0] = c.io(src_ub, d.n3, "src_ub");
+  a.srcb[1] = c.io(src_vb, d.n3, "src_vb");
+  a.srcb[2] = c.io(src_wb, d.n3, "src_wb");
+  const double* x0[3] = {c.in(un, d.n3, "un"), c.in(vn, d.n3, "vn"), c.in(wn, d.n3, "wn")};
+  a.lb[0] = c.io(unb, d.n3, "unb");
+  a.lb[1] = c.io(vnb, d.n3, "vnb");
+  a.lb[2] = c.io(wnb, d.n3, "wnb");
+  const double* dl1 = c.in(l1, d.n3, "l1");
+  const double* dl2 = c.in(l2, d.n3, "l2");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  BnduParam bp;
+  bp.uwall = *uwall;
+  bp.l1b = c.io(l1b, d.n3, "l1b");
+  bp.l2b = c.io(l2b, d.n3, "l2b");
+  bp.phib = c.io(phib, d.n3, "phib");
+  a.G = static_cast<double*>(scratch(24, sizeof(double) * (size_t)d.n3));
+  a.dtinv = 1.0 / *dt;
+  const bool lo = nID[kYMinus] < 0, hi = nID[kYPlus] < 0;
+  a.jen = hi ? d.nj - 1 : d.nj;
+  const int nw = d.sx * d.sz;
+  double* wacc = static_cast<double*>(scratch(25, sizeof(double) * nw));
+  bp.pa = static_cast<double*>(scratch(43, sizeof(double) * 2 * nw));
+  launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
+  g_last_adj_onfly = 0;
+  const int nsw = *lp < 21 ? *lp + 1 : 21;
+  const size_t n3 = (size_t)d.n3;
+  double* work = static_cast<double*>(scratch(42, sizeof(double) * n3 * (size_t)(3 * (nsw + 1) + 37)));
+  double* X[3] = {work, work + n3, work + 2 * n3};
+  double* starts = work + 3 * n3;                   // 3 per sweep
+  double* snaps = starts + 3 * n3 * (size_t)nsw;    // 3 per pair of the sweep being reversed
+  double* T = snaps + 36 * n3;                      // the pre-refresh u or w
+  const double uw = *uwall, dyv = *dy;
+  auto sweep_fwd = [&](bool keep) {
+    for (int q = 0; q < 3; q++)
+      for (int p = 0; p < 4; p++) {
+        if (keep)
+          for (int r = 0; r < 3; r++) copy_dev(snaps + n3 * (size_t)((q * 4 + p) * 3 + r), X[r], d.n3);
+        if (q == 0) pair_fwd<0>(d, a, sp, X, p);
+        if (q == 1) pair_fwd<1>(d, a, sp, X, p);
+        if (q == 2) pair_fwd<2>(d, a, sp, X, p);
+        bc_fused<double, double>(d, X[0], X[1], X[2], uw, dyv, dl1, dl2, dphi, lo, hi, q, 7);
+      }
+  };
+  for (int r = 0; r < 3; r++) copy_dev(X[r], x0[r], d.n3);
+  for (int s = 0; s < nsw; s++) {
+    for (int r = 0; r < 3; r++) copy_dev(starts + n3 * (size_t)(3 * s + r), X[r], d.n3);
+    if (s + 1 < nsw) sweep_fwd(false);
+  }
+  for (int s = nsw - 1; s >= 0; s--) {
+    for (int r = 0; r < 3; r++) copy_dev(X[r], starts + n3 * (size_t)(3 * s + r), d.n3);
+    sweep_fwd(true);
+    for (int q = 2; q >= 0; q--)
+      for (int p = 3; p >= 0; p--) {
+        double* S0 = snaps + n3 * (size_t)((q * 4 + p) * 3);
+        double* S[3] = {S0, S0 + n3, S0 + 2 * n3};
+        // the state before the refresh: the pair's cells of q updated
+        bp.uk = S[0];
+        bp.wk = S[2];
+        if (q != 1) {
+          copy_dev(T, S[q], d.n3);
+          double* Y[3] = {S[0], S[1], S[2]};
+          Y[q] = T;
+          if (q == 0) pair_fwd<0>(d, a, sp, Y, p);
+          if (q == 2) pair_fwd<2>(d, a, sp, Y, p);
+          if (q == 0) bp.uk = T; else bp.wk = T;
+        }
+        periodic_adj(d, a.lb[q]);
+        bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, dyv, dl1, dl2, dphi, lo, hi, bp);
+        const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
+        if (q == 0) {
+          colour_scale<0, false>(d, a, is, js, ks);
+          pair_param<0>(d, a, sp, S, p);
+          colour_gather<0, false>(d, a, is, js, ks);
+        } else if (q == 1) {
+          colour_scale<1, false>(d, a, is, js, ks);
+          pair_param<1>(d, a, sp, S, p);
+          colour_gather<1, false>(d, a, is, js, ks);
+        } else {
+          colour_scale<2, false>(d, a, is, js, ks);
+          pair_param<2>(d, a, sp, S, p);
+          colour_gather<2, false>(d, a, is, js, ks);
+        }
+      }
+  }
+  adjoint_add_sum(wacc, nw, uwallb);
+  c.finish();
+}
+
+// All-argument adjoint of cal_arith_tau_: cal_arith_tau_b_ plus the
+// viscosity's adjoint (incremented); s and rmu each followed by its adjoint.
+void cal_arith_tau_bp_(const int* ni, const int* nj, const int* nk, const double* s, double* sb,
+                       const double* rmu, double* rmub, double* taub) {
+  adjoint_fp64_only("cal_arith_tau_bp_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_arith_tau_bp_");
+  const double* sv = c.in(s, 6 * (size_t)d.n3, "s");
+  double* sa = c.io(sb, 6 * (size_t)d.n3, "sb");
+  const double* m = c.in(rmu, d.n3, "rmu");
+  double* mb = c.io(rmub, d.n3, "rmub");
+  double* t = c.io(taub, 6 * (size_t)d.n3, "taub");
+  arith_tau_param(d, sv, mb, t);
+  arith_tau_adj(d, sa, m, t);
+  c.finish();
+}
+
+// All-argument adjoint of cal_srcu_: cal_srcu_b_ plus the densities'
+// adjoints (incremented).  rho, rhon, prs_*, vis_* are each followed by
+// their adjoints (the primal pressure gradients and viscous terms are what
+// the density derivative needs).
+void cal_srcu_bp_(const int* nID, const int* ni, const int* nj, const int* nk, const double* rho_av,
+                  const double* dt, const double* grvb, const double* grvp, double* ub, double* vb,
+                  double* wb, const double* rho, double* rhob, const double* rhon, double* rhonb,
+                  double* adv_ub, double* adv_vb, double* adv_wb, double* adv_uob, double* adv_vob,
+                  double* adv_wob, const double* prs_u, double* prs_ub, const double* prs_v,
+                  double* prs_vb, const double* prs_w, double* prs_wb, const double* vis_u,
+                  double* vis_ub, const double* vis_v, double* vis_vb, const double* vis_w,
+                  double* vis_wb, double* fst_ub, double* fst_vb, double* fst_wb, double* fst_unb,
+                  double* fst_vnb, double* fst_wnb, double* src_ub, double* src_vb,
+                  double* src_wb) {
+  (void)rho_av; (void)grvb; (void)grvp;
+  adjoint_fp64_only("cal_srcu_bp_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int jen = nID[kYPlus] < 0 ? d.nj - 1 : d.nj;
+  Call c("cal_srcu_bp_");
+  SrcuAdj a;
+  double* hv[3] = {ub, vb, wb};
+  double* ha[3] = {adv_ub, adv_vb, adv_wb};
+  double* hao[3] = {adv_uob, adv_vob, adv_wob};
+  double* hp[3] = {prs_ub, prs_vb, prs_wb};
+  double* hvis[3] = {vis_ub, vis_vb, vis_wb};
+  double* hf[3] = {fst_ub, fst_vb, fst_wb};
+  double* hfn[3] = {fst_unb, fst_vnb, fst_wnb};
+  double* hs[3] = {src_ub, src_vb, src_wb};
+  const double* pp[3] = {prs_u, prs_v, prs_w};
+  const double* pv[3] = {vis_u, vis_v, vis_w};
+  const double *P[3], *V[3];
+  for (int q = 0; q < 3; q++) {
+    a.vel[q] = c.io(hv[q], d.n3, "velocity adjoint");
+    a.adv[q] = c.io(ha[q], d.n3, "adv adjoint");
+    a.advo[q] = c.io(hao[q], d.n3, "adv_o adjoint");
+    a.prs[q] = c.io(hp[q], d.n3, "prs adjoint");
+    a.vis[q] = c.io(hvis[q], d.n3, "vis adjoint");
+    a.fst[q] = c.io(hf[q], d.n3, "fst adjoint");
+    a.fstn[q] = c.io(hfn[q], d.n3, "fst_n adjoint");
+    a.src[q] = c.io(hs[q], d.n3, "src adjoint");
+    P[q] = c.in(pp[q], d.n3, "prs");
+    V[q] = c.in(pv[q], d.n3, "vis");
+  }
+  a.rho = c.in(rho, d.n3, "rho");
+  a.rhon = c.in(rhon, d.n3, "rhon");
+  double* rb = c.io(rhob, d.n3, "rhob");
+  double* rnb = c.io(rhonb, d.n3, "rhonb");
+  const double* sb[3] = {a.src[0], a.src[1], a.src[2]};
+  srcu_param(d, jen, P, V, sb, a.rho, a.rhon, rb, rnb);
+  srcu_adj(d, jen, 1.0 / *dt, a);
+  c.finish();
+}
+
+// Adjoint of cal_arith_coef_vis_ (srcs/cal_arith_coef_vis.f:1-411), linear
+// in the viscosity: rmukb incremented, the 45 coefficient adjoints consumed.
+void cal_arith_coef_vis_b_(const int* ni, const int* nj, const int* nk, const double* dxinv,
+                           const double* dyinv, const double* dzinv, double* rmukb, double* c0b, double* c1b, double* c2b, double* c3b, double* c4b, double* c5b, double* c6b, double* c7b, double* c8b, double* c9b, double* c10b, double* c11b, double* c12b, double* c13b, double* c14b, double* c15b, double* c16b, double* c17b, double* c18b, double* c19b, double* c20b, double* c21b, double* c22b, double* c23b, double* c24b, double* c25b, double* c26b, double* c27b, double* c28b, double* c29b, double* c30b, double* c31b, double* c32b, double* c33b, double* c34b, double* c35b, double* c36b, double* c37b, double* c38b, double* c39b, double* c40b, double* c41b, double* c42b, double* c43b, double* c44b) {
+  adjoint_fp64_only("cal_arith_coef_vis_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  double* hb[45] = {c0b, c1b, c2b, c3b, c4b, c5b, c6b, c7b, c8b, c9b, c10b, c11b, c12b, c13b, c14b, c15b, c16b, c17b, c18b, c19b, c20b, c21b, c22b, c23b, c24b, c25b, c26b, c27b, c28b, c29b, c30b, c31b, c32b, c33b, c34b, c35b, c36b, c37b, c38b, c39b, c40b, c41b, c42b, c43b, c44b};
+  Call c("cal_arith_coef_vis_b_");
+  double* mb = c.io(rmukb, d.n3, "rmukb");
+  double* cb[45];
+  for (int m = 0; m < 45; m++) cb[m] = c.io(hb[m], d.n3, "coefficient adjoint");
+  coef_vis_adj(d, *dxinv, *dyinv, *dzinv, mb, cb);
+  c.finish();
+}
+
+// Adjoint of cal_arith_mean_ (srcs/cal_arith_mean.f:18-26): phikb += (qg -
+// ql) qkb, qkb consumed.
+void cal_arith_mean_b_(const int* ni, const int* nj, const int* nk, double* phikb, const double* ql,
+                       const double* qg, double* qkb) {
+  adjoint_fp64_only("cal_arith_mean_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("cal_arith_mean_b_");
+  double* pb = c.io(phikb, d.n3, "phikb");
+  double* qb = c.io(qkb, d.n3, "qkb");
+  const double l = *ql, g = *qg;
+  launch_flat(d.n3, [=] __device__(int m) {
+    const double t = qb[m];
+    pb[m] = pb[m] + (-(l * t) + g * t);
+    qb[m] = 0.0;
+  });
+  c.finish();
+}
+
+// Adjoint of calrn_ (srcs/calrn.f:1-95), nonlinear: phix, phiy, phiz each
+// followed by its adjoint (incremented); rnb (three components) consumed.
+void calrn_b_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
+              const double* dzinv, const int* ip, const int* jp, const int* kp, const double* phix,
+              double* phixb, const double* phiy, double* phiyb, const double* phiz, double* phizb,
+              double* rnb) {
+  adjoint_fp64_only("calrn_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calrn_b_");
+  const double* g[3] = {c.in(phix, d.n3, "phix"), c.in(phiy, d.n3, "phiy"), c.in(phiz, d.n3, "phiz")};
+  double* gb[3] = {c.io(phixb, d.n3, "phixb"), c.io(phiyb, d.n3, "phiyb"), c.io(phizb, d.n3, "phizb")};
+  double* r = c.io(rnb, 3 * (size_t)d.n3, "rnb");
+  double* R = static_cast<double*>(scratch(44, sizeof(double) * 3 * (size_t)d.n3));
+  calrn_adj(d, *dxinv, *dyinv, *dzinv, *ip, *jp, *kp, g, gb, r, R);
+  c.finish();
+}
+
+// Adjoint of calrkap_ (srcs/calrkap.f:1-102), linear: the four normal
+// adjoints (three components each) incremented, rkapb consumed.
+void calrkap_b_(const int* ni, const int* nj, const int* nk, const double* dxinv, const double* dyinv,
+                const double* dzinv, double* rn_ppb, double* rn_12b, double* rn_13b, double* rn_23b,
+                double* rkapb) {
+  adjoint_fp64_only("calrkap_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calrkap_b_");
+  const size_t n = 3 * (size_t)d.n3;
+  double* pp = c.io(rn_ppb, n, "rn_ppb");
+  double* a12 = c.io(rn_12b, n, "rn_12b");
+  double* a13 = c.io(rn_13b, n, "rn_13b");
+  double* a23 = c.io(rn_23b, n, "rn_23b");
+  double* kb = c.io(rkapb, n, "rkapb");
+  calrkap_adj(d, *dxinv, *dyinv, *dzinv, pp, a12, a13, a23, kb);
+  c.finish();
+}
+
+// Adjoint of cal_grad_p2uvw_ (srcs/cal_grad_p2uvw.f:1-47), linear: qb
+// incremented, the face-gradient adjoints consumed on [-2,n+2]^3.
+void cal_grad_p2uvw_b_(const int* ID, const int* svall, const int* ni, const int* nj, const int* nk,
+                       const double* dxinv, const double* dyinv_array, const double* dzinv,
+                       double* qb, double* qx_ub, double* qy_vb, double* qz_wb) {
+  adjoint_fp64_only("cal_grad_p2uvw_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  const int count = *svall + 6;  // dyinv_array(-2:svall+3)
+  double* dyt = static_cast<double*>(scratch(35, sizeof(double) * (size_t)count));
+  FEL_CHECK(hipMemcpyAsync(dyt, dyinv_array, sizeof(double) * count, hipMemcpyHostToDevice, stream()));
+  Call c("cal_grad_p2uvw_b_");
+  double* q = c.io(qb, d.n3, "qb");
+  double* gx = c.io(qx_ub, d.n3, "qx_ub");
+  double* gy = c.io(qy_vb, d.n3, "qy_vb");
+  double* gz = c.io(qz_wb, d.n3, "qz_wb");
+  grad_p2uvw_adj(d, *dxinv, dyt + (size_t)(*nj) * (size_t)(*ID), *dzinv, q, gx, gy, gz);
+  c.finish();
+  FEL_CHECK(hipStreamSynchronize(stream()));  // dyinv_array is a host array read asynchronously
+}
+
+// Adjoint of calfst_ (srcs/calfst.f:1-50), bilinear: the face gradients and
+// the curvature (three components) each followed by its adjoint
+// (incremented); the force adjoints consumed.
+void calfst_b_(const int* ni, const int* nj, const int* nk, const double* rhol, const double* rhog,
+               const double* surface_tension, const double* phix_u, double* phix_ub,
+               const double* phiy_v, double* phiy_vb, const double* phiz_w, double* phiz_wb,
+               const double* rkap, double* rkapb, double* fst_ub, double* fst_vb, double* fst_wb) {
+  adjoint_fp64_only("calfst_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("calfst_b_");
+  const double *gx = c.in(phix_u, d.n3, "phix_u"), *gy = c.in(phiy_v, d.n3, "phiy_v"),
+               *gz = c.in(phiz_w, d.n3, "phiz_w");
+  double *gxb = c.io(phix_ub, d.n3, "phix_ub"), *gyb = c.io(phiy_vb, d.n3, "phiy_vb"),
+         *gzb = c.io(phiz_wb, d.n3, "phiz_wb");
+  const double* k = c.in(rkap, 3 * (size_t)d.n3, "rkap");
+  double* kb = c.io(rkapb, 3 * (size_t)d.n3, "rkapb");
+  double *fu = c.io(fst_ub, d.n3, "fst_ub"), *fv = c.io(fst_vb, d.n3, "fst_vb"),
+         *fw = c.io(fst_wb, d.n3, "fst_wb");
+  const double st = 2.0 * (*surface_tension) / (*rhol + *rhog);
+  const size_t n3 = (size_t)d.n3;
+  launch_flat(d.n3, [=] __device__(int m) {
+    const double* g[3] = {gx, gy, gz};
+    double* gb[3] = {gxb, gyb, gzb};
+    double* fb[3] = {fu, fv, fw};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const double t = fb[q][m];
+      gb[q][m] = gb[q][m] + -(st * k[q * n3 + m] * t);
+      kb[q * n3 + m] = kb[q * n3 + m] + -(st * g[q][m] * t);
+      fb[q][m] = 0.0;
+    }
+  });
+  c.finish();
+}
+
+// Adjoint of sum_fst_ (srcs/sum_fst.f:1-37): f_b += s_b (s_b unchanged).
+void sum_fst_b_(const int* ni, const int* nj, const int* nk, double* fub, double* fvb, double* fwb,
+                const double* sub, const double* svb, const double* swb) {
+  adjoint_fp64_only("sum_fst_b_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("sum_fst_b_");
+  double *fu = c.io(fub, d.n3, "fub"), *fv = c.io(fvb, d.n3, "fvb"), *fw = c.io(fwb, d.n3, "fwb");
+  const double *su = c.in(sub, d.n3, "sub"), *sv = c.in(svb, d.n3, "svb"),
+               *sw = c.in(swb, d.n3, "swb");
+  launch_flat(d.n3, [=] __device__(int m) {
+    fu[m] = fu[m] + su[m];
+    fv[m] = fv[m] + sv[m];
+    fw[m] = fw[m] + sw[m];
+  });
   c.finish();
 }
 

@@ -723,6 +723,127 @@ void bnd_contact_angle_b_(const int *nID, const int *ni, const int *nj, const in
                           double *theta_arrayb, const double *dx, const double *dy,
                           const double *dz);
 
+/* ---- Material properties, surface tension and parameter adjoints ----
+ * <name>_bp_ is the adjoint with respect to every real argument: what
+ * <name>_b_ does, plus the adjoints of the arrays <name>_b_ takes as fixed
+ * (densities, viscosity, coefficients, slip lengths, phi), incremented.
+ * Each array whose primal value the derivative needs is followed by its
+ * adjoint.  No counterpart in the reference (it has no adjoint). */
+
+/* srcs/bndu.f:1-161, all arguments: bndu_b_ plus l1b, l2b, phib
+ * incremented.  uk, wk: the call's input velocities (rows 1 and nj read). */
+void bndu_bp_(const int *nID, const int *ni, const int *nj, const int *nk, const double *uk,
+              double *ukb, double *vkb, const double *wk, double *wkb, const double *uwall,
+              double *uwallb, const double *dy, const double *l1, double *l1b, const double *l2,
+              double *l2b, const double *phi, double *phib);
+
+/* srcs/solu_sor4.f:1-308, all arguments, sweep count frozen at *lp (as in
+ * solu_sor4_b_): adds the adjoints of rho, rhon, the 45 coefficients (c0..c44
+ * then their adjoints c0b..c44b), l1, l2 and phi.  src_*, un, vn, wn are the
+ * call's inputs, each followed by its adjoint.  Coefficients are read from
+ * the arrays.  Device work memory: 3 (min(lp+1,21) + 1) + 37 padded arrays. */
+void solu_sor4_bp_(const int *ipara, const int *ID, const int *nID, const int *ndiv, const int *ni,
+                   const int *nj, const int *nk, const int *key, double *sendjb, double *recvjb,
+                   const int *nstep, const int *imon_t, const double *dt, const double *rho,
+                   double *rhob, const double *rhon, double *rhonb,
+                   double *c0, double *c1, double *c2, double *c3, double *c4, double *c5,
+                   double *c6, double *c7, double *c8, double *c9,
+                   double *c10, double *c11, double *c12, double *c13,
+                   double *c14, double *c15, double *c16, double *c17,
+                   double *c18, double *c19, double *c20, double *c21,
+                   double *c22, double *c23, double *c24, double *c25,
+                   double *c26, double *c27, double *c28, double *c29,
+                   double *c30, double *c31, double *c32, double *c33,
+                   double *c34, double *c35, double *c36, double *c37,
+                   double *c38, double *c39, double *c40, double *c41,
+                   double *c42, double *c43, double *c44,
+                   double *c0b, double *c1b, double *c2b, double *c3b, double *c4b, double *c5b,
+                   double *c6b, double *c7b, double *c8b, double *c9b,
+                   double *c10b, double *c11b, double *c12b, double *c13b,
+                   double *c14b, double *c15b, double *c16b, double *c17b,
+                   double *c18b, double *c19b, double *c20b, double *c21b,
+                   double *c22b, double *c23b, double *c24b, double *c25b,
+                   double *c26b, double *c27b, double *c28b, double *c29b,
+                   double *c30b, double *c31b, double *c32b, double *c33b,
+                   double *c34b, double *c35b, double *c36b, double *c37b,
+                   double *c38b, double *c39b, double *c40b, double *c41b,
+                   double *c42b, double *c43b, double *c44b,
+                   const double *src_u, double *src_ub, const double *src_v, double *src_vb,
+                   const double *src_w, double *src_wb, const double *un, double *unb,
+                   const double *vn, double *vnb, const double *wn, double *wnb,
+                   const double *uwall, double *uwallb, const double *dy, const double *l1,
+                   double *l1b, const double *l2, double *l2b, const double *phi, double *phib,
+                   const int *lp);
+
+/* srcs/cal_arith_tau.f:21-118, all arguments: cal_arith_tau_b_ plus rmub. */
+void cal_arith_tau_bp_(const int *ni, const int *nj, const int *nk, const double *s, double *sb,
+                       const double *rmu, double *rmub, double *taub);
+
+/* srcs/cal_srcu.f:51-122, all arguments: cal_srcu_b_ plus rhob, rhonb
+ * (r_b = src_b (prs - 0.5 vis)/r^2 spread over the face's four cells). */
+void cal_srcu_bp_(const int *nID, const int *ni, const int *nj, const int *nk, const double *rho_av,
+                  const double *dt, const double *grvb, const double *grvp, double *ub, double *vb,
+                  double *wb, const double *rho, double *rhob, const double *rhon, double *rhonb,
+                  double *adv_ub, double *adv_vb, double *adv_wb, double *adv_uob, double *adv_vob,
+                  double *adv_wob, const double *prs_u, double *prs_ub, const double *prs_v,
+                  double *prs_vb, const double *prs_w, double *prs_wb, const double *vis_u,
+                  double *vis_ub, const double *vis_v, double *vis_vb, const double *vis_w,
+                  double *vis_wb, double *fst_ub, double *fst_vb, double *fst_wb, double *fst_unb,
+                  double *fst_vnb, double *fst_wnb, double *src_ub, double *src_vb,
+                  double *src_wb);
+
+/* srcs/cal_arith_coef_vis.f:1-411 transposed (linear in the viscosity):
+ * rmukb incremented, the 45 coefficient adjoints consumed. */
+void cal_arith_coef_vis_b_(const int *ni, const int *nj, const int *nk, const double *dxinv,
+                           const double *dyinv, const double *dzinv, double *rmukb,
+                           double *c0b, double *c1b, double *c2b, double *c3b, double *c4b, double *c5b,
+                   double *c6b, double *c7b, double *c8b, double *c9b,
+                   double *c10b, double *c11b, double *c12b, double *c13b,
+                   double *c14b, double *c15b, double *c16b, double *c17b,
+                   double *c18b, double *c19b, double *c20b, double *c21b,
+                   double *c22b, double *c23b, double *c24b, double *c25b,
+                   double *c26b, double *c27b, double *c28b, double *c29b,
+                   double *c30b, double *c31b, double *c32b, double *c33b,
+                   double *c34b, double *c35b, double *c36b, double *c37b,
+                   double *c38b, double *c39b, double *c40b, double *c41b,
+                   double *c42b, double *c43b, double *c44b);
+
+/* srcs/cal_arith_mean.f:18-26 transposed: phikb += (qg - ql) qkb; qkb
+ * consumed. */
+void cal_arith_mean_b_(const int *ni, const int *nj, const int *nk, double *phikb, const double *ql,
+                       const double *qg, double *qkb);
+
+/* srcs/calrn.f:1-95 reversed (nonlinear): phix, phiy, phiz each followed by
+ * its adjoint (incremented); rnb (three components) consumed.  d|r|/dr is
+ * taken as 0 at r = 0. */
+void calrn_b_(const int *ni, const int *nj, const int *nk, const double *dxinv, const double *dyinv,
+              const double *dzinv, const int *ip, const int *jp, const int *kp, const double *phix,
+              double *phixb, const double *phiy, double *phiyb, const double *phiz, double *phizb,
+              double *rnb);
+
+/* srcs/calrkap.f:1-102 transposed: the four normal adjoints incremented,
+ * rkapb consumed. */
+void calrkap_b_(const int *ni, const int *nj, const int *nk, const double *dxinv, const double *dyinv,
+                const double *dzinv, double *rn_ppb, double *rn_12b, double *rn_13b, double *rn_23b,
+                double *rkapb);
+
+/* srcs/cal_grad_p2uvw.f:1-47 transposed: qb incremented, the face-gradient
+ * adjoints consumed on [-2,n+2]^3. */
+void cal_grad_p2uvw_b_(const int *ID, const int *svall, const int *ni, const int *nj, const int *nk,
+                       const double *dxinv, const double *dyinv_array, const double *dzinv,
+                       double *qb, double *qx_ub, double *qy_vb, double *qz_wb);
+
+/* srcs/calfst.f:1-50 reversed (bilinear): the face gradients and rkap each
+ * followed by its adjoint (incremented); the force adjoints consumed. */
+void calfst_b_(const int *ni, const int *nj, const int *nk, const double *rhol, const double *rhog,
+               const double *surface_tension, const double *phix_u, double *phix_ub,
+               const double *phiy_v, double *phiy_vb, const double *phiz_w, double *phiz_wb,
+               const double *rkap, double *rkapb, double *fst_ub, double *fst_vb, double *fst_wb);
+
+/* srcs/sum_fst.f:1-37 transposed: f*b += s*b (s*b unchanged). */
+void sum_fst_b_(const int *ni, const int *nj, const int *nk, double *fub, double *fvb, double *fwb,
+                const double *sub, const double *svb, const double *swb);
+
 /* How the last solu_sor4_b_ call obtained its coefficients: 1 formed on the
  * fly from the viscosity, 0 read from the 45 arrays, -1 before any call. */
 int felgpu_sor_b_onfly_(void);

@@ -1212,7 +1212,8 @@ class SorP(FDCase):
              *[x[f"c{i}"].copy() for i in range(45)],
              *[x[k].copy() for k in ("src_u", "src_v", "src_w")], a["un"], a["vn"], a["wn"],
              D(float(x["uwall"][0])), D(g.dy), x["l1"].copy(), x["l2"].copy(), x["phi"].copy())
-        self.lps.append((lp_reader or sor_lp_oracle)())
+        reader = lp_reader or (sor_lp_gpu if call is H.call_gpu else sor_lp_oracle)
+        self.lps.append(reader())
         return a
 
     def adjoint(self, call, y, lp=None):
