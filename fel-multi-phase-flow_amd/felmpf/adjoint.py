@@ -38,6 +38,9 @@ STATE = ("u", "v", "w", "uo", "vo", "wo", "p", "po")
 OUT = ("un", "vn", "wn", "pn")
 FIXED = ("rho", "rhon", "rmu", "rmun", "phi", "l1", "l2", "fst_u", "fst_v", "fst_w", "fst_un",
          "fst_vn", "fst_wn")
+# fixed fields whose adjoints MomentumStep(params=True) forms (the
+# surface-tension forces' adjoints are always formed)
+PARAMS = ("rho", "rhon", "rmu", "rmun", "phi", "l1", "l2")
 
 
 class _Ref:
@@ -90,17 +93,18 @@ class MomentumStep:
     forces of both time levels)."""
 
     def __init__(self, cfg: RunConfig, fixed: dict, call=None, lp_reader=None,
-                 resident: bool = False):
+                 resident: bool = False, params: bool = False):
         self.cfg = cfg
         self.call = call or gpu_call
         self.lp_reader = lp_reader or sor_last_lp
         self.resident = resident
+        self.params = params
         self._registered = []
         g = cfg
         self.nm = g.ni * g.nj * g.nk
         self.n1 = max(g.nj, g.nk) + 6
         z = lambda: self.array(np.zeros(g.shape))  # noqa: E731
-        self.fx = {k: self.array(np.ascontiguousarray(fixed[k], dtype=np.float64)) for k in FIXED}
+        self.fx = {k: self.array(np.array(fixed[k], dtype=np.float64)) for k in FIXED}  # own copies
         # pressure matrix and its modal form (srcs/main.f90:232-236, set-up)
         pc = [np.zeros(g.nj + 6) for _ in range(7)]
         self.call("cal_coef_prs", I(g.nj), D(g.rhog), D(1 / g.dx), D(1 / g.dy), D(1 / g.dz), *pc)
@@ -124,6 +128,14 @@ class MomentumStep:
         self.b["tau"] = self.array(np.zeros((6,) + g.shape))
         self.b["pr"] = self.array(np.zeros(self.nm))
         self.b["pi"] = self.array(np.zeros(self.nm))
+        if params:
+            # adjoints of the fixed fields (PARAMS) and of the coefficients;
+            # the refreshes' input u and w (bndu_bp_ reads their wall rows)
+            for k in PARAMS:
+                self.b[k] = z()
+            self.coefb = [z() for _ in range(45)]
+            for k in ("ur1", "wr1", "ur2", "wr2"):
+                self.a[k] = z()
         self.lp = None
 
     # -- arrays ---------------------------------------------------------
@@ -216,6 +228,7 @@ class MomentumStep:
         c("solu_sor4", *self._sor_head(), a["src_u"], a["src_v"], a["src_w"], a["un"], a["vn"],
           a["wn"], D(g.uwall), *self._bc_tail())
         self.lp = self.lp_reader()
+        self._keep_refresh_input(1)
         self._refresh(a)
         nm = self.nm
         c("caldiv", I(0), I(1), *gh, *dd, D(g.dt), a["un"], a["vn"], a["wn"], a["div"], D(0.0))
@@ -227,6 +240,7 @@ class MomentumStep:
         c("bnd_periodic", *gh, a["dp"])
         c("corunp_explicit", WALLS, *gh, D(g.rhog), *dd, D(g.dt), a["dp"], a["phat"], a["un"],
           a["vn"], a["wn"], a["pn"])
+        self._keep_refresh_input(2)
         self._refresh(a)
         c("bnd_avzero", I(0), I(1), *gh, a["pn"])
         c("bnd_neumann", WALLS, *gh, a["pn"])
@@ -239,11 +253,23 @@ class MomentumStep:
         for k in ("un", "vn", "wn"):
             c("bnd_periodic", *gh, a[k])
 
-    def _refresh_b(self, b, uwallb):
-        c, gh = self.call, self._g()
+    def _keep_refresh_input(self, n: int):
+        """Copies of un, wn before refresh n (bndu_bp_ reads their wall rows)."""
+        if self.params:
+            gh, a = self._g(), self.a
+            self.call("cpy", *gh, a["un"], a[f"ur{n}"])
+            self.call("cpy", *gh, a["wn"], a[f"wr{n}"])
+
+    def _refresh_b(self, b, uwallb, n: int = 0):
+        c, gh, g = self.call, self._g(), self.cfg
         for k in ("wn", "vn", "un"):
             c("bnd_periodic_b", *gh, b[k])
-        c("bndu_b", WALLS, *gh, b["un"], b["vn"], b["wn"], uwallb, *self._bc_tail())
+        if self.params:
+            a, f = self.a, self.fx
+            c("bndu_bp", WALLS, *gh, a[f"ur{n}"], b["un"], b["vn"], a[f"wr{n}"], b["wn"],
+              D(g.uwall), uwallb, D(g.dy), f["l1"], b["l1"], f["l2"], b["l2"], f["phi"], b["phi"])
+        else:
+            c("bndu_b", WALLS, *gh, b["un"], b["vn"], b["wn"], uwallb, *self._bc_tail())
 
     # -- reverse --------------------------------------------------------
     def reverse(self, out_b: dict) -> dict:
@@ -254,6 +280,9 @@ class MomentumStep:
         uwallb = self.reverse_set()
         out = {k: self._get(self.b[k]) for k in STATE}
         out["uwall"] = np.array([uwallb])
+        if self.params:
+            for k in FIXED:
+                out[k] = self._get(self.b[k])
         return out
 
     def reverse_set(self) -> float:
@@ -269,11 +298,16 @@ class MomentumStep:
         c("init_q", *gh, b["u"], b["v"], b["w"])
         c("init_q", *gh, b["uo"], b["vo"], b["wo"])
         c("init_q", *gh, b["p"], b["po"], b["div"])
+        if self.params:
+            for k in PARAMS:
+                c("init_q", *gh, b[k], b["div"], b["div"])
+            c("init_q", *gh, b["fst_u"], b["fst_v"], b["fst_w"])
+            c("init_q", *gh, b["fst_un"], b["fst_vn"], b["fst_wn"])
         uwallb = D(0.0)
         c("bnd_periodic_b", *gh, b["pn"])
         c("bnd_neumann_b", WALLS, *gh, b["pn"])
         c("bnd_avzero_b", I(0), I(1), *gh, b["pn"])
-        self._refresh_b(b, uwallb)
+        self._refresh_b(b, uwallb, 2)
         c("corunp_explicit_b", WALLS, *gh, D(g.rhog), *dd, D(g.dt), b["dp"], b["phat"], b["un"],
           b["vn"], b["wn"], b["pn"])
         c("bnd_periodic_b", *gh, b["dp"])
@@ -283,18 +317,41 @@ class MomentumStep:
         c("solp_fft_tdma2_b", *self._tdma2_head(), b["pr"], b["pi"])
         c("solp_fft_tdma1_b", *gh, np.zeros(2 * nm), b["div"], b["pr"], b["pi"])
         c("caldiv_b", I(0), I(1), *gh, *dd, D(g.dt), b["un"], b["vn"], b["wn"], b["div"])
-        self._refresh_b(b, uwallb)
-        c("solu_sor4_b", *self._sor_head(), b["src_u"], b["src_v"], b["src_w"], b["un"], b["vn"],
-          b["wn"], uwallb, *self._bc_tail(), I(self.lp))
-        for s, t in (("u", "un"), ("v", "vn"), ("w", "wn")):
-            c("cpy_b", *gh, b[s], b[t])
-        c("cal_srcu_b", WALLS, *gh, D(0.0), D(g.dt), D(0.0), D(g.grvp), b["u"], b["v"], b["w"],
-          f["rho"], f["rhon"], b["adv_u"], b["adv_v"], b["adv_w"], b["adv_uo"], b["adv_vo"],
-          b["adv_wo"], b["prs_u"], b["prs_v"], b["prs_w"], b["vis_u"], b["vis_v"], b["vis_w"],
-          b["fst_u"], b["fst_v"], b["fst_w"], b["fst_un"], b["fst_vn"], b["fst_wn"], b["src_u"],
-          b["src_v"], b["src_w"])
+        self._refresh_b(b, uwallb, 1)
+        if self.params:
+            pairs = []
+            for k in ("src_u", "src_v", "src_w"):
+                pairs += [a[k], b[k]]
+            for s_, t in (("u", "un"), ("v", "vn"), ("w", "wn")):
+                pairs += [a[s_], b[t]]  # the solve's input state is the copy of u, v, w
+            head = self._sor_head()
+            c("solu_sor4_bp", *head[:13], f["rho"], b["rho"], f["rhon"], b["rhon"], *self.coef,
+              *self.coefb, *pairs, D(g.uwall), uwallb, D(g.dy), f["l1"], b["l1"], f["l2"],
+              b["l2"], f["phi"], b["phi"], I(self.lp))
+        else:
+            c("solu_sor4_b", *self._sor_head(), b["src_u"], b["src_v"], b["src_w"], b["un"],
+              b["vn"], b["wn"], uwallb, *self._bc_tail(), I(self.lp))
+        for s_, t in (("u", "un"), ("v", "vn"), ("w", "wn")):
+            c("cpy_b", *gh, b[s_], b[t])
+        if self.params:
+            c("cal_arith_coef_vis_b", *gh, *dd, b["rmun"], *self.coefb)
+            c("cal_srcu_bp", WALLS, *gh, D(0.0), D(g.dt), D(0.0), D(g.grvp), b["u"], b["v"],
+              b["w"], f["rho"], b["rho"], f["rhon"], b["rhon"], b["adv_u"], b["adv_v"],
+              b["adv_w"], b["adv_uo"], b["adv_vo"], b["adv_wo"], a["prs_u"], b["prs_u"],
+              a["prs_v"], b["prs_v"], a["prs_w"], b["prs_w"], a["vis_u"], b["vis_u"], a["vis_v"],
+              b["vis_v"], a["vis_w"], b["vis_w"], b["fst_u"], b["fst_v"], b["fst_w"],
+              b["fst_un"], b["fst_vn"], b["fst_wn"], b["src_u"], b["src_v"], b["src_w"])
+        else:
+            c("cal_srcu_b", WALLS, *gh, D(0.0), D(g.dt), D(0.0), D(g.grvp), b["u"], b["v"],
+              b["w"], f["rho"], f["rhon"], b["adv_u"], b["adv_v"], b["adv_w"], b["adv_uo"],
+              b["adv_vo"], b["adv_wo"], b["prs_u"], b["prs_v"], b["prs_w"], b["vis_u"],
+              b["vis_v"], b["vis_w"], b["fst_u"], b["fst_v"], b["fst_w"], b["fst_un"],
+              b["fst_vn"], b["fst_wn"], b["src_u"], b["src_v"], b["src_w"])
         c("cal_div_tensor_b", *gh, *dd, b["tau"], b["vis_u"], b["vis_v"], b["vis_w"])
-        c("cal_arith_tau_b", *gh, b["s"], f["rmu"], b["tau"])
+        if self.params:
+            c("cal_arith_tau_bp", *gh, a["s"], b["s"], f["rmu"], b["rmu"], b["tau"])
+        else:
+            c("cal_arith_tau_b", *gh, b["s"], f["rmu"], b["tau"])
         c("calsij_b", *gh, *dd, b["u"], b["v"], b["w"], b["s"])
         c("cal_grad_prs_b", *gh, *dd, b["phat"], b["prs_u"], b["prs_v"], b["prs_w"])
         c("calphat_b", *gh, b["p"], b["po"], b["phat"])
@@ -372,7 +429,7 @@ class PhaseStep(MomentumStep):
         self._registered = []
         g = cfg
         z = lambda: self.array(np.zeros(g.shape))  # noqa: E731
-        self.par = {k: self.array(np.ascontiguousarray(params[k], dtype=np.float64))
+        self.par = {k: self.array(np.array(params[k], dtype=np.float64))  # own copies
                     for k in PHASE_PARAMS}
         self.dyinv = np.full(g.nj + 6, 1.0 / g.dy)
         self.a = {k: z() for k in ("phi", "gx", "gy", "gz", "hx", "hy", "hz", "fx", "fy", "fz",
@@ -429,7 +486,11 @@ class PhaseStep(MomentumStep):
         out["uwall"] = np.array([uwallb])
         return out
 
-    def reverse_set(self) -> float:
+    def reverse_set(self, theta_b_hook=None) -> float:
+        """reverse() on the output adjoint already in place (b["phin"]);
+        theta_b_hook(), if given, runs once the adjoints are cleared and
+        before the contact angle is reversed: it may add to b["theta"] (the
+        contact-angle fill of phi(.,0) in the full step)."""
         c, a, b, p, g = self.call, self.a, self.b, self.par, self.cfg
         gh, dd = self._g(), self._d()
         grad_head = [I(0), I(g.nj), *gh, D(1 / g.dx), self.dyinv, D(1 / g.dz)]
@@ -437,6 +498,8 @@ class PhaseStep(MomentumStep):
             c("init_q", *gh, b[k], b["gx"], b["gy"])
         c("init_q", *gh, b["gz"], b["fx"], b["fy"])
         c("init_q", *gh, b["fz"], b["gx"], b["gy"])
+        if theta_b_hook is not None:
+            theta_b_hook()
         uwallb = D(0.0)
         pn0b, pn1b = b["phin"][0], b["phin"][1]
         c("bnd_periodic_b", *gh, pn0b)
@@ -496,3 +559,245 @@ class PhaseWindow:
         out = {"phi": res["phi"]}
         out.update(acc)
         return out
+
+
+# ---------------------------------------------------------------------------
+# The whole step: phase transport, material properties, surface tension and
+# the velocity-pressure update
+# ---------------------------------------------------------------------------
+
+FULL_STATE = ("phi0", "phi1") + STATE
+FULL_PARAMS = ("theta0", "zeta", "l1", "l2")
+
+
+class FullStep:
+    """One whole timestep of the reference (srcs/main.f90:489-694 for one
+    phase slice, the order of host/fel_driver.f90:step1) through the
+    drop-in routines, and its reverse.  The step maps the state (phi(.,0),
+    phi(.,1), u, v, w, uo, vo, wo, p, po) to (phin(.,0:1), un, vn, wn, pn)
+    for the wall maps theta_0_array, zeta_array, l1_array, l2_array and
+    uwall; reverse() gives the adjoints of the state and of all five.
+
+    Forward: the phase update (PhaseStep, at the state's u, v, w); the
+    contact-angle fill of phi(.,0) with the same contact angle
+    (srcs/main.f90:509, gnbc computed once: both of its calls read the
+    same u, w); densities and viscosities of phi(.,0) and phin(.,0)
+    (cal_arith_mean x4); the surface-tension forces of phi(.,1) and
+    phin(.,1) (vertex gradients, normals on four staggerings, curvature,
+    face gradients, calfst, sum_fst); then the velocity-pressure update
+    (MomentumStep with parameter adjoints) on those fields, with bndu's
+    phi = phi(.,1).  Reverse: the momentum update with the adjoints of the
+    densities, viscosities, forces, phi, l1 and l2; the surface-tension
+    chain; the material properties; the contact-angle fill of phi(.,0)
+    (its theta adjoint joining the phase reverse's before gnbc); the
+    phase update."""
+
+    def __init__(self, cfg: RunConfig, params: dict, call=None, lp_reader=None,
+                 resident: bool = False):
+        self.cfg = cfg
+        self.call = call or gpu_call
+        g = cfg
+        zero = np.zeros(g.shape)
+        self.ph = PhaseStep(cfg, {"u": zero, "v": zero, "w": zero, "theta0": params["theta0"],
+                                  "zeta": params["zeta"]}, call=call, resident=resident)
+        fixed = {k: zero for k in FIXED}
+        fixed["l1"], fixed["l2"] = params["l1"], params["l2"]
+        self.mo = MomentumStep(cfg, fixed, call=call, lp_reader=lp_reader, resident=resident,
+                               params=True)
+        z = lambda n=None: self.mo.array(np.zeros(g.shape if n is None else (n,) + g.shape))  # noqa: E731
+        # primal arrays of the full step beyond the two parts'
+        self.a = {k: z() for k in ("phi0", "phi0c", "gx", "gy", "gz", "hx", "hy", "hz", "fgx",
+                                   "fgy", "fgz", "fhx", "fhy", "fhz", "fst_u", "fst_v", "fst_w",
+                                   "fst_un", "fst_vn", "fst_wn")}
+        for k in ("rn_pp", "rn_12", "rn_13", "rn_23", "rnn_pp", "rnn_12", "rnn_13", "rnn_23",
+                  "rkap", "rkapn"):
+            self.a[k] = z(3)
+        self.b = {k: z() for k in ("phi0c", "phi1", "gx", "gy", "gz", "fgx", "fgy", "fgz")}
+        for k in ("rn_pp", "rn_12", "rn_13", "rn_23", "rkap"):
+            self.b[k] = z(3)
+        self.dyinv = np.full(g.nj + 6, 1.0 / g.dy)
+        self.lp = None
+
+    def close(self):
+        self.ph.close()
+        self.mo.close()
+
+    # -- pieces -----------------------------------------------------------
+    def _grad_head(self):
+        g = self.cfg
+        return [I(0), I(g.nj), *self.mo._g(), D(1 / g.dx), self.dyinv, D(1 / g.dz)]
+
+    _STAG = (("pp", (1, 1, 1)), ("12", (0, 0, 1)), ("13", (0, 1, 0)), ("23", (1, 0, 0)))
+
+    def _csf(self, q, gv, rn, rkap, fg, fst):
+        """Surface-tension force of the volume fraction q (srcs/main.f90:
+        547-568): vertex gradients gv, normals rn[...], curvature rkap,
+        face gradients fg, force fst."""
+        c, a, g = self.call, self.a, self.cfg
+        gh, dd = self.mo._g(), self.mo._d()
+        c("cal_grad_p2a", *self._grad_head(), q, *[a[k] for k in gv])
+        for tag, st in self._STAG:
+            c("calrn", *gh, *dd, *[I(v) for v in st], *[a[k] for k in gv], a[rn + tag])
+        c("calrkap", *gh, *dd, *[a[rn + tag] for tag, _ in self._STAG], a[rkap])
+        c("cal_grad_p2uvw", *self._grad_head(), q, *[a[k] for k in fg])
+        c("calfst", *gh, D(g.rhol), D(g.rhog), D(g.sigma), *[a[k] for k in fg], a[rkap],
+          *[a[k] for k in fst])
+
+    def _csf_b(self, q, qb, gv, rn, rkap, fg, fst_b):
+        """Reverse of _csf for q (primal) into qb, from the force adjoints
+        fst_b (consumed)."""
+        c, a, b, g = self.call, self.a, self.b, self.cfg
+        gh, dd = self.mo._g(), self.mo._d()
+        # the gradient adjoints are shared by both levels: cleared first (the
+        # parts the forward leaves unwritten carry no derivative)
+        c("init_q", *gh, b["gx"], b["gy"], b["gz"])
+        c("init_q", *gh, b["fgx"], b["fgy"], b["fgz"])
+        c("calfst_b", *gh, D(g.rhol), D(g.rhog), D(g.sigma), a[fg[0]], b["fgx"], a[fg[1]],
+          b["fgy"], a[fg[2]], b["fgz"], a[rkap], b["rkap"], *fst_b)
+        c("cal_grad_p2uvw_b", *self._grad_head(), qb, b["fgx"], b["fgy"], b["fgz"])
+        c("calrkap_b", *gh, *dd, *[b["rn_" + tag] for tag, _ in self._STAG], b["rkap"])
+        for tag, st in reversed(self._STAG):
+            c("calrn_b", *gh, *dd, *[I(v) for v in st], a[gv[0]], b["gx"], a[gv[1]], b["gy"],
+              a[gv[2]], b["gz"], b["rn_" + tag])
+        c("cal_grad_p2a_b", *self._grad_head(), qb, b["gx"], b["gy"], b["gz"])
+
+    # -- forward ----------------------------------------------------------
+    def set_state(self, state: dict):
+        ph, mo = self.ph, self.mo
+        for k in ("u", "v", "w"):
+            ph._set(ph.par[k], state[k])
+        ph._set(ph.a["phi"], state["phi1"])
+        mo._set(self.a["phi0"], state["phi0"])
+        mo.set_state(state)
+
+    def forward_set(self):
+        c, a, g = self.call, self.a, self.cfg
+        ph, mo = self.ph, self.mo
+        gh = mo._g()
+        f = mo.fx
+        ph.forward_set()
+        pn0, pn1 = ph.a["phin"][0], ph.a["phin"][1]
+        # contact-angle fill of phi(.,0) (srcs/main.f90:509)
+        c("cpy", *gh, a["phi0"], a["phi0c"])
+        c("bnd_contact_angle", WALLS, *gh, a["phi0c"], ph.a["theta"], D(g.dx), D(g.dy), D(g.dz))
+        # material properties (srcs/main.f90:526-529)
+        c("cal_arith_mean", *gh, a["phi0c"], D(g.rhol), D(g.rhog), f["rho"])
+        c("cal_arith_mean", *gh, pn0, D(g.rhol), D(g.rhog), f["rhon"])
+        c("cal_arith_mean", *gh, a["phi0c"], D(g.rmul), D(g.rmug), f["rmu"])
+        c("cal_arith_mean", *gh, pn0, D(g.rmul), D(g.rmug), f["rmun"])
+        # surface tension (srcs/main.f90:545-573)
+        c("init_q", *gh, f["fst_u"], f["fst_v"], f["fst_w"])
+        c("init_q", *gh, f["fst_un"], f["fst_vn"], f["fst_wn"])
+        self._csf(ph.a["phi"], ("gx", "gy", "gz"), "rn_", "rkap", ("fgx", "fgy", "fgz"),
+                  ("fst_u", "fst_v", "fst_w"))
+        self._csf(pn1, ("hx", "hy", "hz"), "rnn_", "rkapn", ("fhx", "fhy", "fhz"),
+                  ("fst_un", "fst_vn", "fst_wn"))
+        c("sum_fst", *gh, a["fst_u"], a["fst_v"], a["fst_w"], f["fst_u"], f["fst_v"], f["fst_w"])
+        c("sum_fst", *gh, a["fst_un"], a["fst_vn"], a["fst_wn"], f["fst_un"], f["fst_vn"],
+          f["fst_wn"])
+        # velocity-pressure update, bndu's phi = phi(.,1)
+        c("cpy", *gh, ph.a["phi"], f["phi"])
+        mo.forward_set()
+        self.lp = mo.lp
+
+    def forward(self, state: dict, keep: bool = True) -> dict:
+        self.set_state(state)
+        self.forward_set()
+        if not keep:
+            return {}
+        out = {k: self.mo._get(self.mo.a[k]) for k in OUT}
+        out["phin"] = self.ph._get(self.ph.a["phin"])
+        return out
+
+    # -- reverse ----------------------------------------------------------
+    def reverse(self, out_b: dict) -> dict:
+        """Adjoints of the state and of the parameters from those of
+        (phin(.,0:1), un, vn, wn, pn), at the state of the last forward."""
+        c, a, b, g = self.call, self.a, self.b, self.cfg
+        ph, mo = self.ph, self.mo
+        gh = mo._g()
+        if self.lp is None:
+            raise RuntimeError("reverse() needs a forward() of the same step first")
+        for k in OUT:
+            mo._set(mo.b[k], out_b[k])
+        ph._set(ph.b["phin"], out_b["phin"])
+        pn0b, pn1b = ph.b["phin"][0], ph.b["phin"][1]
+        uw = mo.reverse_set()
+        mb = mo.b
+        c("init_q", *gh, b["phi0c"], b["phi1"], b["gx"])
+        # bndu's phi is phi(.,1); its copy into the fixed field reversed
+        c("cpy_b", *gh, b["phi1"], mb["phi"])
+        # surface tension: the sums (zeroed, then one force added: sum_fst
+        # reversed gives the force the sum's adjoint, init_q ends it) hand
+        # their adjoints to the forces, consumed by each level's chain
+        fb_n = [mb["fst_un"], mb["fst_vn"], mb["fst_wn"]]
+        fb = [mb["fst_u"], mb["fst_v"], mb["fst_w"]]
+        self._csf_b(ph.a["phin"][1], pn1b, ("hx", "hy", "hz"), "rnn_", "rkapn",
+                    ("fhx", "fhy", "fhz"), fb_n)
+        self._csf_b(ph.a["phi"], b["phi1"], ("gx", "gy", "gz"), "rn_", "rkap",
+                    ("fgx", "fgy", "fgz"), fb)
+        # material properties
+        c("cal_arith_mean_b", *gh, pn0b, D(g.rmul), D(g.rmug), mb["rmun"])
+        c("cal_arith_mean_b", *gh, b["phi0c"], D(g.rmul), D(g.rmug), mb["rmu"])
+        c("cal_arith_mean_b", *gh, pn0b, D(g.rhol), D(g.rhog), mb["rhon"])
+        c("cal_arith_mean_b", *gh, b["phi0c"], D(g.rhol), D(g.rhog), mb["rho"])
+
+        # the contact-angle fill of phi(.,0): its theta adjoint joins the
+        # phase reverse's before gnbc is reversed
+        def theta_hook():
+            c("bnd_contact_angle_b", WALLS, *gh, a["phi0"], b["phi0c"], ph.a["theta"],
+              ph.b["theta"], D(g.dx), D(g.dy), D(g.dz))
+        uw += ph.reverse_set(theta_hook)
+        out = {"phi0": mo._get(b["phi0c"])}
+        out["phi1"] = ph._get(ph.b["phi"]) + mo._get(b["phi1"])
+        for k in ("u", "v", "w"):
+            out[k] = mo._get(mb[k]) + ph._get(ph.b[k])
+        for k in ("uo", "vo", "wo", "p", "po"):
+            out[k] = mo._get(mb[k])
+        out["theta0"] = ph._get(ph.b["theta0"])
+        out["zeta"] = ph._get(ph.b["zeta"])
+        out["l1"] = mo._get(mb["l1"])
+        out["l2"] = mo._get(mb["l2"])
+        out["uwall"] = np.array([uw])
+        return out
+
+
+class FullWindow:
+    """`steps` whole timesteps with the reference's state rotation
+    (srcs/main.f90:687-698: uo = u, u = un, po = p, p = pn, phi = phin) and
+    their reverse, each step recomputed from its stored input state."""
+
+    def __init__(self, step: FullStep, steps: int):
+        self.step = step
+        self.steps = steps
+        self.states = []
+
+    def forward(self, state: dict) -> dict:
+        st = {k: np.array(state[k], dtype=np.float64) for k in FULL_STATE}
+        self.states = []
+        for _ in range(self.steps):
+            self.states.append({k: v.copy() for k, v in st.items()})
+            out = self.step.forward(st)
+            st = {"uo": st["u"], "vo": st["v"], "wo": st["w"], "po": st["p"],
+                  "u": out["un"], "v": out["vn"], "w": out["wn"], "p": out["pn"],
+                  "phi0": out["phin"][0].copy(), "phi1": out["phin"][1].copy()}
+        return st
+
+    def reverse(self, state_b: dict) -> dict:
+        """Adjoints of the window's input state and of the parameters
+        (accumulated) from those of its output state."""
+        sb = {k: np.array(state_b[k], dtype=np.float64) for k in FULL_STATE}
+        acc = {k: 0.0 for k in FULL_PARAMS + ("uwall",)}
+        for n in reversed(range(self.steps)):
+            out_b = {"un": sb["u"], "vn": sb["v"], "wn": sb["w"], "pn": sb["p"],
+                     "phin": np.stack([sb["phi0"], sb["phi1"]])}
+            self.step.forward(self.states[n], keep=False)
+            ib = self.step.reverse(out_b)
+            for k in acc:
+                acc[k] = acc[k] + ib[k]
+            nb = {k: ib[k] for k in FULL_STATE}
+            for s_, o in (("u", "uo"), ("v", "vo"), ("w", "wo"), ("p", "po")):
+                nb[s_] = nb[s_] + sb[o]
+            sb = nb
+        sb.update(acc)
+        return sb

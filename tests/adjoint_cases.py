@@ -1384,3 +1384,22 @@ class SumFst(Case):
         out = dict(zip(("fu", "fv", "fw"), f))
         out.update(dict(zip(("su", "sv", "sw"), s)))
         return out
+
+
+def full_inputs(cfg, seed: int):
+    """A state and wall maps for the whole step: the phase inputs' volume
+    fraction (phi(.,0) a small perturbation of it), velocities bounded away
+    from 0 at both time levels, pressures, contact angles, friction and
+    slip lengths of the wall rows."""
+    phi1, par = phase_inputs(cfg, seed)
+    r = np.random.default_rng(seed + 1)
+    state = {"phi1": phi1, "phi0": phi1 + 0.01 * r.standard_normal(phi1.shape),
+             "u": par["u"], "v": par["v"], "w": par["w"]}
+    for k in ("uo", "vo", "wo"):
+        state[k] = away_from_zero(r, phi1.shape, 0.3)
+    for k in ("p", "po"):
+        state[k] = 0.1 * r.standard_normal(phi1.shape)
+    params = {"theta0": par["theta0"], "zeta": par["zeta"],
+              "l1": r.uniform(0.3, 0.6, phi1.shape) * cfg.dy,
+              "l2": r.uniform(0.3, 0.6, phi1.shape) * cfg.dy}
+    return state, params
