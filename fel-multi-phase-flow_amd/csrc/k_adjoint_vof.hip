@@ -254,9 +254,14 @@ static void mthinc1_cells(const Dims d, const Mthinc1Adj A) {
         for (int b = 0; b < 3; b++) nb[b] = nb[b] + fbv * W * Spx[b];
         db = db + fbv * W * Sp;
       }
-    A.phib[c] = A.phib[c] + db / Gd;
+    // Gd = 0: every sigmoid of the cell saturated (phi at 0 or 1 to
+    // rounding), the Newton step vanished and the fluxes do not vary with
+    // d: no term through d
+    if (Gd > 0.0) {
+      A.phib[c] = A.phib[c] + db / Gd;
 #pragma unroll
-    for (int b = 0; b < 3; b++) nb[b] = nb[b] - db * Gn[b] / Gd;
+      for (int b = 0; b < 3; b++) nb[b] = nb[b] - db * Gn[b] / Gd;
+    }
     const double ndn = n[0] * nb[0] + n[1] * nb[1] + n[2] * nb[2];
 #pragma unroll
     for (int b = 0; b < 3; b++) A.rb[b][c] = (nb[b] - ndn * n[b]) * coef;

@@ -1120,9 +1120,13 @@ void orc_solphi_mthinc1_b(const int *ni_, const int *nj_, const int *nk_, const 
             db += fbv * W * Sp;
           }
         }
-        /* through d (implicit function of n and phi) */
-        phib[c] += db / Gd;
-        for (int q = 0; q < 3; q++) nb[q] -= db * Gn[q] / Gd;
+        /* through d (implicit function of n and phi).  Gd = 0: every
+         * sigmoid of the cell saturated (phi at 0 or 1 to rounding), the
+         * Newton step vanished and the fluxes do not vary with d: no term */
+        if (Gd > 0.0) {
+          phib[c] += db / Gd;
+          for (int q = 0; q < 3; q++) nb[q] -= db * Gn[q] / Gd;
+        }
         /* through n = r/|r| */
         const double ndn = n[0] * nb[0] + n[1] * nb[1] + n[2] * nb[2];
         for (int q = 0; q < 3; q++) {

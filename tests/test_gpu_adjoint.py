@@ -457,3 +457,112 @@ def test_parameter_adjoint_matches_oracle(ci, gi):
     assert abs(lhs - rhs) <= 1e-7 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
     if case.name == "solu_sor4_bp":
         assert len(set(case.lps)) == 1, case.lps
+
+
+# ---------------------------------------------------------------------------
+# The whole step
+# ---------------------------------------------------------------------------
+
+def _patterned():
+    from felmpf import config as C
+    return C.patterned_wall().scaled(32, 16, 32)
+
+
+def _driver_state(cfg, warm):
+    """State and wall maps of the drop-in host driver after `warm` steps,
+    and its fields after one more step."""
+    from felmpf.driver import GpuRun
+    run = GpuRun(cfg)
+    try:
+        run.steps(warm)
+        state = {k: run.field(k) for k in ("phi0", "phi1", "u", "v", "w", "uo", "vo", "wo", "p", "po")}
+        params = {"theta0": run.field("theta_0_array"), "zeta": run.field("zeta_array"),
+                  "l1": run.field("l1_array"), "l2": run.field("l2_array")}
+        assert np.array_equal(run.field("phin1"), state["phi1"])
+        run.steps(1)
+        after = {k: run.field(k) for k in ("phi0", "phi1", "u", "v", "w", "p")}
+    finally:
+        run.close()
+    return state, params, after
+
+
+def test_full_step_is_the_driver_step_and_reverse_matches_oracle():
+    """felmpf.adjoint.FullStep's forward through the drop-in routines gives
+    the host driver's step (host/fel_driver.f90:step1) from the driver's own
+    state, and its reverse on the GPU equals the reverse through the
+    oracle's routines."""
+    from felmpf import adjoint
+    cfg = _patterned()
+    print("adjoint case full step vs driver 32x16x32", flush=True)
+    state, params, after = _driver_state(cfg, 3)
+    st = adjoint.FullStep(cfg, params, resident=True)
+    try:
+        out = st.forward(state)
+        y = {k: np.random.default_rng(341).standard_normal(cfg.shape) for k in OUT_FULL}
+        bg = st.reverse(_out_b(y))
+        lp = st.lp
+    finally:
+        st.close()
+    for k, o in (("u", out["un"]), ("v", out["vn"]), ("w", out["wn"]), ("p", out["pn"]),
+                 ("phi0", out["phin"][0]), ("phi1", out["phin"][1])):
+        sc = max(float(np.max(np.abs(after[k]))), 1e-300)
+        assert float(np.max(np.abs(o - after[k]))) <= 1e-12 * sc, k
+    so = adjoint.FullStep(cfg, params, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+    so.forward(state, keep=False)
+    assert so.lp == lp
+    bo = so.reverse(_out_b(y))
+    for k in bo:
+        sc = max(float(np.max(np.abs(bo[k]))), 1e-300)
+        assert float(np.max(np.abs(bo[k] - bg[k]))) <= 1e-10 * sc, k
+
+
+OUT_FULL = ("phi0", "phi1", "un", "vn", "wn", "pn")
+
+
+def _out_b(y):
+    return {"un": y["un"], "vn": y["vn"], "wn": y["wn"], "pn": y["pn"],
+            "phin": np.stack([y["phi0"], y["phi1"]])}
+
+
+@pytest.mark.parametrize("which", ["angle_patch0", "angle_patch1", "slip"])
+def test_full_window_gradients_against_tangent_mode(which):
+    """Wall-parameter gradients of the state after a 3-step window on the
+    checkered wall, by the reverse sweep of the whole step (FullWindow),
+    against the library's tangent mode run through the host driver along
+    the same parameter direction: <J v, y> = <v, J^T y> for a contact-angle
+    patch group or the slip length (l1 = l2 = ls with the friction zeta
+    following it)."""
+    from felmpf import adjoint
+    from felmpf import sensitivity as S
+    from felmpf.driver import GpuRun
+    cfg = _patterned()
+    print(f"adjoint case full window vs tangent {which} 32x16x32", flush=True)
+    warm, nsteps = 2, 3
+    seeds = S.parameter_seeds(cfg)
+    seed = {"angle_patch0": seeds[0], "angle_patch1": seeds[1], "slip": seeds[-1]}[which]
+    run = GpuRun(cfg, tangent=True)
+    try:
+        run.steps(warm)
+        state = {k: run.field(k) for k in adjoint.FULL_STATE}
+        params = {"theta0": run.field("theta_0_array"), "zeta": run.field("zeta_array"),
+                  "l1": run.field("l1_array"), "l2": run.field("l2_array")}
+        for name, dirn in seed.items():
+            run.seed(name, dirn)
+        run.steps(nsteps)
+        jv = {k: run.tangent(k) for k in adjoint.FULL_STATE}
+    finally:
+        run.close()
+    y = {k: np.random.default_rng(351).standard_normal(cfg.shape) for k in adjoint.FULL_STATE}
+    st = adjoint.FullStep(cfg, params, resident=True)
+    try:
+        win = adjoint.FullWindow(st, nsteps)
+        win.forward(state)
+        bx = win.reverse(y)
+    finally:
+        st.close()
+    names = {"theta_0_array": "theta0", "zeta_array": "zeta", "l1_array": "l1", "l2_array": "l2"}
+    lhs = sum(float(np.sum(jv[k] * y[k])) for k in adjoint.FULL_STATE)
+    rhs = sum(float(np.sum(dirn * bx[names[n]])) for n, dirn in seed.items())
+    print(which, "<Jv,y>", lhs, "<v,J^T y>", rhs, flush=True)
+    assert abs(lhs) > 0.0
+    assert abs(lhs - rhs) <= 1e-8 * abs(lhs), (lhs, rhs)
