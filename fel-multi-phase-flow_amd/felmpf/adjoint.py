@@ -433,23 +433,34 @@ class PhaseStep(MomentumStep):
                     for k in PHASE_PARAMS}
         self.dyinv = np.full(g.nj + 6, 1.0 / g.dy)
         self.a = {k: z() for k in ("phi", "gx", "gy", "gz", "hx", "hy", "hz", "fx", "fy", "fz",
-                                   "theta", "pre3", "preca")}
+                                   "theta", "pre3", "preca", "phin1_in")}
         self.a["phin"] = self.array(np.zeros((2,) + g.shape))
-        self.b = {k: z() for k in ("phi", "gx", "gy", "gz", "fx", "fy", "fz", "theta") + PHASE_PARAMS}
+        self.b = {k: z() for k in ("phi", "gx", "gy", "gz", "fx", "fy", "fz", "theta",
+                                   "phin1_in") + PHASE_PARAMS}
         self.b["phin"] = self.array(np.zeros((2,) + g.shape))
+        self.own_phin = False
 
     def _slice(self, arr, l):
         return arr[l]
 
     def set_state(self, state: dict):
+        """state["phi"]: phi(.,1); state["phin1"] (optional): the contents
+        of phin(.,1) before the step, of which the update keeps the part it
+        does not write -- after a step of the reference's loop it is the
+        copy of phi(.,1) (the default), before the first step whatever the
+        start-up left there."""
         self._set(self.a["phi"], state["phi"])
+        self.own_phin = "phin1" in state
+        if self.own_phin:
+            self._set(self.a["phin1_in"], state["phin1"])
 
     def forward_set(self):
         c, a, p, g = self.call, self.a, self.par, self.cfg
         gh, dd = self._g(), self._d()
         grad_head = [I(0), I(g.nj), *gh, D(1 / g.dx), self.dyinv, D(1 / g.dz)]
         pn1 = a["phin"][1]
-        c("cpy", *gh, a["phi"], pn1)  # the previous step's copy phi = phin
+        # the previous step's copy phi = phin, or the given contents
+        c("cpy", *gh, a["phin1_in"] if self.own_phin else a["phi"], pn1)
         c("cal_grad_p2a", *grad_head, a["phi"], a["gx"], a["gy"], a["gz"])
         c("solphi_mthinc1", *gh, *dd, D(g.dt), D(g.bet), a["phi"], a["gx"], a["gy"], a["gz"],
           p["u"], p["v"], p["w"], a["fx"], a["fy"], a["fz"])
@@ -518,8 +529,13 @@ class PhaseStep(MomentumStep):
           a["gy"], b["gy"], a["gz"], b["gz"], p["u"], b["u"], p["v"], b["v"], p["w"], b["w"],
           b["fx"], b["fy"], b["fz"])
         c("cal_grad_p2a_b", *grad_head, b["phi"], b["gx"], b["gy"], b["gz"])
-        # phin(.,1) started as the copy of phi: its unwritten (ghost) part
-        c("cpy_b", *gh, b["phi"], pn1b)
+        # phin(.,1) started as the copy of phi (or the given contents): its
+        # unwritten part
+        if self.own_phin:
+            c("init_q", *gh, b["phin1_in"], b["gx"], b["gy"])
+            c("cpy_b", *gh, b["phin1_in"], pn1b)
+        else:
+            c("cpy_b", *gh, b["phi"], pn1b)
         return uwallb.value
 
 
@@ -666,7 +682,10 @@ class FullStep:
         ph, mo = self.ph, self.mo
         for k in ("u", "v", "w"):
             ph._set(ph.par[k], state[k])
-        ph._set(ph.a["phi"], state["phi1"])
+        ps = {"phi": state["phi1"]}
+        if "phin1" in state:
+            ps["phin1"] = state["phin1"]
+        ph.set_state(ps)
         mo._set(self.a["phi0"], state["phi0"])
         mo.set_state(state)
 
@@ -713,14 +732,34 @@ class FullStep:
     def reverse(self, out_b: dict) -> dict:
         """Adjoints of the state and of the parameters from those of
         (phin(.,0:1), un, vn, wn, pn), at the state of the last forward."""
+        ph, mo, b = self.ph, self.mo, self.b
+        for k in OUT:
+            mo._set(mo.b[k], out_b[k])
+        ph._set(ph.b["phin"], out_b["phin"])
+        uw = self.reverse_set()
+        mb = mo.b
+        out = {"phi0": mo._get(b["phi0c"])}
+        out["phi1"] = ph._get(ph.b["phi"]) + mo._get(b["phi1"])
+        for k in ("u", "v", "w"):
+            out[k] = mo._get(mb[k]) + ph._get(ph.b[k])
+        for k in ("uo", "vo", "wo", "p", "po"):
+            out[k] = mo._get(mb[k])
+        out["theta0"] = ph._get(ph.b["theta0"])
+        out["zeta"] = ph._get(ph.b["zeta"])
+        out["l1"] = mo._get(mb["l1"])
+        out["l2"] = mo._get(mb["l2"])
+        out["uwall"] = np.array([uw])
+        return out
+
+    def reverse_set(self) -> float:
+        """reverse() on the output adjoints already in place (mo.b[un, vn,
+        wn, pn], ph.b["phin"]), without transfers: the adjoints are left in
+        the arrays reverse() reads; returns uwall's."""
         c, a, b, g = self.call, self.a, self.b, self.cfg
         ph, mo = self.ph, self.mo
         gh = mo._g()
         if self.lp is None:
             raise RuntimeError("reverse() needs a forward() of the same step first")
-        for k in OUT:
-            mo._set(mo.b[k], out_b[k])
-        ph._set(ph.b["phin"], out_b["phin"])
         pn0b, pn1b = ph.b["phin"][0], ph.b["phin"][1]
         uw = mo.reverse_set()
         mb = mo.b
@@ -748,18 +787,7 @@ class FullStep:
             c("bnd_contact_angle_b", WALLS, *gh, a["phi0"], b["phi0c"], ph.a["theta"],
               ph.b["theta"], D(g.dx), D(g.dy), D(g.dz))
         uw += ph.reverse_set(theta_hook)
-        out = {"phi0": mo._get(b["phi0c"])}
-        out["phi1"] = ph._get(ph.b["phi"]) + mo._get(b["phi1"])
-        for k in ("u", "v", "w"):
-            out[k] = mo._get(mb[k]) + ph._get(ph.b[k])
-        for k in ("uo", "vo", "wo", "p", "po"):
-            out[k] = mo._get(mb[k])
-        out["theta0"] = ph._get(ph.b["theta0"])
-        out["zeta"] = ph._get(ph.b["zeta"])
-        out["l1"] = mo._get(mb["l1"])
-        out["l2"] = mo._get(mb["l2"])
-        out["uwall"] = np.array([uw])
-        return out
+        return uw
 
 
 class FullWindow:
@@ -773,7 +801,11 @@ class FullWindow:
         self.states = []
 
     def forward(self, state: dict) -> dict:
+        """state: FULL_STATE, and optionally "phin1", the contents of
+        phin(.,1) before the first step (PhaseStep.set_state)."""
         st = {k: np.array(state[k], dtype=np.float64) for k in FULL_STATE}
+        if "phin1" in state:
+            st["phin1"] = np.array(state["phin1"], dtype=np.float64)
         self.states = []
         for _ in range(self.steps):
             self.states.append({k: v.copy() for k, v in st.items()})

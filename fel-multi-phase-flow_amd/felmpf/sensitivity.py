@@ -20,6 +20,9 @@ forward(params) runs one tangent run per parameter (each run also gives the
 values) and returns the final liquid fraction of the interior; backward
 contracts the incoming gradient with the stored tangents.  The cost is one
 tangent run per parameter; the runs start from the same state.
+AdjointWindow has the same contract with the gradient from one reverse
+sweep of the whole step (felmpf.adjoint.FullWindow), whatever the number of
+parameters.
 """
 from __future__ import annotations
 
@@ -155,6 +158,79 @@ def _make_function():
             return g, None, None, None, None
 
     return StepWindow
+
+
+_MAP_KEYS = {"theta_0_array": "theta0", "zeta_array": "zeta", "l1_array": "l1",
+             "l2_array": "l2"}
+
+
+def driver_start(cfg: RunConfig, state=None):
+    """The host driver's state after its start-up (fills of the initial
+    fields; phin(.,1) as the start-up left it) and its wall maps, as
+    FullWindow takes them."""
+    from .adjoint import FULL_STATE
+    run = GpuRun(cfg, state=state)
+    try:
+        st = {k: run.field(k) for k in FULL_STATE + ("phin1",)}
+        maps = {v: run.field(k) for k, v in _MAP_KEYS.items()}
+    finally:
+        run.close()
+    return st, maps
+
+
+def _make_adjoint_function():
+    import torch
+
+    class AdjointWindow(torch.autograd.Function):
+        """phi_final = AdjointWindow.apply(params, cfg, nsteps, seeds, state).
+
+        The same contract as StepWindow, with the gradient from one reverse
+        sweep of the whole step (felmpf.adjoint.FullWindow: every step
+        recomputed from its stored input, then reversed through the
+        library's adjoint entry points) instead of one tangent run per
+        parameter: backward costs one window's reverse whatever the number
+        of parameters.
+        """
+
+        @staticmethod
+        def forward(ctx, params, cfg, nsteps, seeds, state=None):
+            from .adjoint import FullStep, FullWindow
+            if params.dim() != 1 or params.numel() != len(seeds):
+                raise ValueError("one parameter per seed")
+            if torch.any(params != 0):
+                raise ValueError("AdjointWindow differentiates at the configuration's "
+                                 "parameters: pass zero offsets")
+            st, maps = driver_start(cfg, state)
+            # host arrays (nothing registered): backward may run once per
+            # torch.autograd.grad call on the same graph
+            win = FullWindow(FullStep(cfg, maps), nsteps)
+            out = win.forward(st)
+            ctx.window, ctx.seeds, ctx.cfg = win, seeds, cfg
+            return torch.from_numpy(_interior(cfg, out["phi1"]).copy())
+
+        @staticmethod
+        def backward(ctx, grad_phi):
+            from .adjoint import FULL_STATE
+            cfg = ctx.cfg
+            y = {k: np.zeros(cfg.shape) for k in FULL_STATE}
+            _interior(cfg, y["phi1"])[...] = grad_phi.detach().cpu().numpy()
+            bx = ctx.window.reverse(y)
+            g = [sum(float(np.sum(d * bx[_MAP_KEYS[n]])) for n, d in seed.items())
+                 for seed in ctx.seeds]
+            return torch.tensor(g, dtype=torch.float64), None, None, None, None
+
+    return AdjointWindow
+
+
+def adjoint_window():
+    """The reverse-sweep torch.autograd.Function (torch imported on first
+    use)."""
+    global _ADJOINT_WINDOW
+    try:
+        return _ADJOINT_WINDOW
+    except NameError:
+        _ADJOINT_WINDOW = _make_adjoint_function()
+        return _ADJOINT_WINDOW
 
 
 def step_window():
