@@ -312,10 +312,10 @@ def measure_adjoint(cfg, warmup, reps, step_ms):
                      "refreshes) through the *_b_ entry points",
             "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
             "reverse_vs_forward": round(rev / fwd, 3),
-            "reverse_vs_fp64_step": round(rev / step_ms, 3),
+            "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None,
             "sor_sweeps": min(lp + 1, 21), "reps": reps,
-            "held_fixed": "volume fraction (its transport is reversed by the phase leg), "
-                          "surface tension, densities and viscosities (no adjoint yet)"}
+            "held_fixed": "volume fraction, densities, viscosities and surface-tension "
+                          "forces (the full_step leg reverses their dependence too)"}
 
 
 def measure_phase_adjoint(cfg, warmup, reps, step_ms):
@@ -370,7 +370,72 @@ def measure_phase_adjoint(cfg, warmup, reps, step_ms):
                      "the *_b_ entry points; gives d/d(phi, u, v, w, theta_0, zeta, uwall)",
             "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
             "reverse_vs_forward": round(rev / fwd, 3),
-            "reverse_vs_fp64_step": round(rev / step_ms, 3), "reps": reps}
+            "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None, "reps": reps}
+
+
+def measure_full_adjoint(cfg, warmup, reps, step_ms):
+    """The reverse of the whole step (felmpf.adjoint.FullStep,
+    device-resident) against its forward through the same drop-in routines,
+    on the state and wall maps of a run after `warmup` steps, result reuse
+    off.  The reverse gives the adjoints of the whole state and of the wall
+    maps theta_0, zeta, l1, l2 and uwall; its timing includes restoring the
+    output adjoints (device copies)."""
+    import felmpf
+    from felmpf import _lib, adjoint
+    gpu = _lib.gpu()
+    run = felmpf.GpuRun(cfg)
+    run.steps(warmup)
+    state = {k: run.field(k) for k in adjoint.FULL_STATE}
+    maps = {"theta0": run.field("theta_0_array"), "zeta": run.field("zeta_array"),
+            "l1": run.field("l1_array"), "l2": run.field("l2_array")}
+    run.close()
+    reuse_before = int(gpu.felgpu_result_reuse_())
+    gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(0)))
+    step = adjoint.FullStep(cfg, maps, resident=True)
+    try:
+        mo, ph = step.mo, step.ph
+        rng = np.random.default_rng(9)
+        ybuf = {k: mo.array(rng.standard_normal(cfg.shape)) for k in adjoint.OUT}
+        ybuf["phin"] = mo.array(rng.standard_normal((2,) + cfg.shape))
+        g3 = [adjoint.I(cfg.ni), adjoint.I(cfg.nj), adjoint.I(cfg.nk)]
+
+        def restore():
+            for k in adjoint.OUT:
+                adjoint.gpu_call("cpy", *g3, ybuf[k], mo.b[k])
+            for l in (0, 1):
+                adjoint.gpu_call("cpy", *g3, ybuf["phin"][l], ph.b["phin"][l])
+
+        step.set_state(state)
+        step.forward_set()
+        gpu.felgpu_sync_()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step.forward_set()
+        gpu.felgpu_sync_()
+        t1 = time.perf_counter()
+        restore()
+        step.reverse_set()
+        gpu.felgpu_sync_()
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            restore()
+            step.reverse_set()
+        gpu.felgpu_sync_()
+        t3 = time.perf_counter()
+        lp = step.lp
+    finally:
+        step.close()
+        gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(reuse_before)))
+    fwd, rev = (t1 - t0) / reps * 1e3, (t3 - t2) / reps * 1e3
+    return {"scope": "reverse of the whole step (felmpf.adjoint.FullStep: phase update, "
+                     "contact-angle fill of phi(.,0), material properties, surface tension, "
+                     "velocity-pressure update with the density, viscosity, force, phi and "
+                     "slip-length adjoints) through the *_b_/_bp_ entry points; gives "
+                     "d/d(state, theta_0, zeta, l1, l2, uwall)",
+            "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
+            "reverse_vs_forward": round(rev / fwd, 3),
+            "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None,
+            "sor_sweeps": min(lp + 1, 21), "reps": reps}
 
 
 def main():
@@ -384,7 +449,18 @@ def main():
     ap.add_argument("--precision", type=int, default=0, choices=(0, 32, 64),
                     help="64 or 32: time that path only; default: FP64, FP32 and tangent "
                          "(value = FP64)")
+    ap.add_argument("--adjoint", choices=("momentum", "phase", "full_step"), default=None,
+                    help="time one reverse-mode leg only (after --warmup steps, --steps "
+                         "repetitions) and print its JSON object instead of the bench line")
     args = ap.parse_args()
+    if args.adjoint is not None:
+        import felmpf
+        cfg = felmpf.CONFIGS[args.config]()
+        fn = {"momentum": measure_adjoint, "phase": measure_phase_adjoint,
+              "full_step": measure_full_adjoint}[args.adjoint]
+        res = fn(cfg, args.warmup, args.steps, float("nan"))
+        print(json.dumps({"adjoint": args.adjoint, "config": args.config, **res}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
@@ -454,6 +530,10 @@ def main():
         step_ms = main_run["elapsed"] / args.steps * 1e3
         adjoint_leg = {"momentum": measure_adjoint(cfg, args.warmup, 3, step_ms),
                        "phase": measure_phase_adjoint(cfg, args.warmup, 3, step_ms)}
+        try:
+            adjoint_leg["full_step"] = measure_full_adjoint(cfg, args.warmup, 3, step_ms)
+        except Exception as e:  # noqa: BLE001 - reported, the main line stands
+            adjoint_leg["full_step"] = {"error": f"{type(e).__name__}: {e}"}
 
     timers = main_run["timers"]
     ms_sor, n_sor = timers["sor_pair"]

@@ -81,12 +81,14 @@ static void neumann_adj(const Dims d, double* q, bool lo, bool hi) {
   });
 }
 
-// Parameter adjoints of bndu (bndu_bp_): the call's input velocities uk,
-// wk (their rows 1 and nj are read), uwall, and the adjoints of the slip
-// lengths and phi; pa holds the phi_av adjoint of each wall cell (bottom
-// then top, one value per (i,k) column).  l1b == nullptr: none formed.
+// Parameter adjoints of bndu (bndu_bp_): the call's input velocities on
+// the rows next to the walls (rows: u at j = 1, u at j = nj, w at j = 1, w
+// at j = nj, one value per (i,k) column each), uwall, and the adjoints of
+// the slip lengths and phi; pa holds the phi_av adjoint of each wall cell
+// (bottom then top, one value per (i,k) column).  l1b == nullptr: none
+// formed.
 struct BnduParam {
-  const double *uk, *wk;
+  const double* rows;
   double uwall;
   double *l1b, *l2b, *phib, *pa;
 };
@@ -142,8 +144,8 @@ static void bndu_phi_gather(const Dims d, const BnduParam bp, bool lo, bool hi) 
 // beyond folded in), gives c1_b = A (-+uwall) and c2_b = A_u u1 + A_w w1.
 static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc, double dy,
                      const double* l1, const double* l2, const double* phi, bool lo, bool hi,
-                     const BnduParam bp = BnduParam{nullptr, nullptr, 0.0, nullptr, nullptr,
-                                                    nullptr, nullptr}) {
+                     const BnduParam bp = BnduParam{nullptr, 0.0, nullptr, nullptr, nullptr,
+                                                    nullptr}) {
   const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, sy = d.sy, Z = d.sx * d.sy;
   // z fills (bndu.f:130-155), every (i,j) line, statements reversed
   launch_flat(sx * sy, [=] __device__(int c) {
@@ -193,10 +195,10 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
         if (t == 1) acc = acc + cf.c1 * b1;
         if (bp.l1b) {
           if (t == 0) {
-            c2b = c2b + b1 * bp.wk[jy(nj)];
+            c2b = c2b + b1 * bp.rows[3 * nw + m];
           } else {
             c1b = c1b + b1 * bp.uwall;
-            c2b = c2b + b1 * bp.uk[jy(nj)];
+            c2b = c2b + b1 * bp.rows[nw + m];
           }
         }
         q[jy(nj + 1)] = 0.0;
@@ -229,10 +231,10 @@ static void bndu_adj(const Dims d, double* u, double* v, double* w, double* wacc
         if (t == 1) acc = acc - cf.c1 * b1;
         if (bp.l1b) {
           if (t == 0) {
-            c2b = c2b + b1 * bp.wk[jy(1)];
+            c2b = c2b + b1 * bp.rows[2 * nw + m];
           } else {
             c1b = c1b - b1 * bp.uwall;
-            c2b = c2b + b1 * bp.uk[jy(1)];
+            c2b = c2b + b1 * bp.rows[m];
           }
         }
         q[jy(0)] = 0.0;
@@ -906,6 +908,19 @@ static void for_pair_cells(const Dims d, int jen, int p, F f) {
   });
 }
 
+// The rows next to the walls of u and w (j = 1 and j = nj, every (i,k)
+// column) into rows[4 nw], the layout BnduParam reads.
+static void wall_rows(const Dims d, const double* u, const double* w, double* rows) {
+  const int nj = d.nj, sx = d.sx, nw = d.sx * d.sz;
+  launch_flat(nw, [=] __device__(int m) {
+    const int i = m % sx - 2, k = m / sx - 2;
+    rows[m] = u[d.ix(i, 1, k)];
+    rows[nw + m] = u[d.ix(i, nj, k)];
+    rows[2 * nw + m] = w[d.ix(i, 1, k)];
+    rows[3 * nw + m] = w[d.ix(i, nj, k)];
+  });
+}
+
 // Arrays of the all-argument Gauss-Seidel adjoint.
 struct SorParam {
   const double* src[3];  // primal sources
@@ -913,30 +928,70 @@ struct SorParam {
   double *rhob, *rhonb;
 };
 
-// Forward update of colour pair p of component P on x (srcs/solu_sor4.f:
-// 131-175 with coefficient arrays): the pair's cells read no cell of the
-// pair, so one pass.
+// The value the Gauss-Seidel update (srcs/solu_sor4.f:131-175, coefficient
+// arrays) gives cell c of component P from the state x0, x1, x2.
+template <int P>
+__device__ __forceinline__ double pair_value(const Dims& d, const SorAdj& A, const double* src,
+                                             const double* x0, const double* x1,
+                                             const double* x2, int c) {
+  int idx[14], comp[14], off[14];
+  sor_terms<P>(d, idx, comp, off);
+  const double* xs[3] = {x0, x1, x2};
+  const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
+  const double r00 = (A.rho[c] + A.rho[c + F] + A.rhon[c] + A.rhon[c + F]) * 0.25;
+  double sum = 0.0;
+#pragma unroll
+  for (int t = 0; t < 14; t++) sum = sum + A.cf[15 * P + idx[t]][c] * xs[comp[t]][c + off[t]];
+  const double visn = sum / r00;
+  const double xo = xs[P][c];
+  const double du = (visn * 0.5 + src[c]) / (A.dtinv + A.cf[15 * P + 6][c] / r00 * 0.5) - xo;
+  return xo + du;
+}
+
+// Forward update of colour pair p of component P on x: the pair's cells
+// read no cell of the pair, so one pass.
 template <int P>
 static void pair_fwd(const Dims d, const SorAdj& a, const SorParam& sp, double* const* x, int p) {
-  const double *rho = a.rho, *rhon = a.rhon, *src = sp.src[P];
-  const double dtinv = a.dtinv;
+  const double* src = sp.src[P];
   double* x0 = x[0];
   double* x1 = x[1];
   double* x2 = x[2];
   const SorAdj A = a;
   for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
-    int idx[14], comp[14], off[14];
-    sor_terms<P>(d, idx, comp, off);
-    const double* xs[3] = {x0, x1, x2};
-    const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
-    const double r00 = (rho[c] + rho[c + F] + rhon[c] + rhon[c + F]) * 0.25;
-    double sum = 0.0;
-#pragma unroll
-    for (int t = 0; t < 14; t++) sum = sum + A.cf[15 * P + idx[t]][c] * xs[comp[t]][c + off[t]];
-    const double visn = sum / r00;
+    const double v = pair_value<P>(d, A, src, x0, x1, x2, c);
     double* xp = P == 0 ? x0 : (P == 1 ? x1 : x2);
-    const double du = (visn * 0.5 + src[c]) / (dtinv + A.cf[15 * P + 6][c] / r00 * 0.5) - xp[c];
-    xp[c] = xp[c] + du;
+    xp[c] = v;
+  });
+}
+
+// wall_rows of the state before the refresh that follows colour pair p of
+// component P: the kept state xs before the pair, with the pair's cells of
+// u (P = 0) or w (P = 2) on those rows updated (pair_value, the forward's
+// arithmetic).
+template <int P>
+static void pair_rows(const Dims d, const SorAdj& a, const SorParam& sp, const double* const* xs,
+                      int p, double* rows) {
+  const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, nw = d.sx * d.sz;
+  const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
+  const int jend = P == 1 ? a.jen : d.nj;
+  const double* src = sp.src[P];
+  const double *x0 = xs[0], *x1 = xs[1], *x2 = xs[2];
+  const SorAdj A = a;
+  launch_flat(nw, [=] __device__(int m) {
+    const int i = m % sx - 2, k = m / sx - 2;
+    const int jr[2] = {1, nj};
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int j = jr[t], c = d.ix(i, j, k);
+      const int bi = (i ^ is) & 1, bj = (j ^ js) & 1, bk = (k ^ ks) & 1;
+      const bool in_pair = bi == bj && bj == bk && i >= 1 && i <= ni && j >= 1 && j <= jend &&
+                           k >= 1 && k <= nk;
+      double uu = x0[c], ww = x2[c];
+      if (in_pair && P == 0) uu = pair_value<0>(d, A, src, x0, x1, x2, c);
+      if (in_pair && P == 2) ww = pair_value<2>(d, A, src, x0, x1, x2, c);
+      rows[t * nw + m] = uu;
+      rows[(2 + t) * nw + m] = ww;
+    }
   });
 }
 
@@ -1546,8 +1601,9 @@ void bndu_bp_(const int* nID, const int* ni, const int* nj, const int* nk, const
   launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
   Call c("bndu_bp_");
   BnduParam bp;
-  bp.uk = c.in(uk, d.n3, "uk");
-  bp.wk = c.in(wk, d.n3, "wk");
+  double* rows = static_cast<double*>(scratch(45, sizeof(double) * 4 * nw));
+  wall_rows(d, c.in(uk, d.n3, "uk"), c.in(wk, d.n3, "wk"), rows);
+  bp.rows = rows;
   bp.uwall = *uwall;
   double* ub = c.io(ukb, d.n3, "ukb");
   double* vb = c.io(vkb, d.n3, "vkb");
@@ -1573,8 +1629,10 @@ void bndu_bp_(const int* nID, const int* ni, const int* nj, const int* nk, const
 // reversed, keeping the state before each colour pair; each pair's
 // parameter adjoints are formed at that state (pair_param), and each
 // refresh's bndu is reversed with its parameter adjoints at the state
-// before the refresh (the kept state with the pair's cells updated).
-// Device work memory: 3 (nsw + 1) + 37 arrays of the padded grid.
+// before the refresh (the kept state with the pair's cells updated, on the
+// wall rows).  Device work memory, nsw = min(lp+1, 21) sweeps: 3 + 36 nsw
+// arrays of the padded grid for nsw <= 4 (one forward pass keeps every
+// pair's start), else 3 (nsw + 1) + 36 (each sweep recomputed).
 void solu_sor4_bp_(const int* ipara, const int* ID, const int* nID, const int* ndiv, const int* ni,
                    const int* nj, const int* nk, const int* key, double* sendjb, double* recvjb,
                    const int* nstep, const int* imon_t, const double* dt, const double* rho,
@@ -1638,17 +1696,22 @@ void solu_sor4_bp_(const int* ipara, const int* ID, const int* nID, const int* n
   g_last_adj_onfly = 0;
   const int nsw = *lp < 21 ? *lp + 1 : 21;
   const size_t n3 = (size_t)d.n3;
-  double* work = static_cast<double*>(scratch(42, sizeof(double) * n3 * (size_t)(3 * (nsw + 1) + 37)));
+  // up to 4 sweeps: every pair's starting state kept in one forward pass;
+  // more: each sweep's start kept and the sweep recomputed before reversal
+  const bool keep_all = nsw <= 4;
+  const size_t arrays = keep_all ? 3 + 36 * (size_t)nsw : 3 + 3 * (size_t)nsw + 36;
+  double* work = static_cast<double*>(scratch(42, sizeof(double) * n3 * arrays));
   double* X[3] = {work, work + n3, work + 2 * n3};
-  double* starts = work + 3 * n3;                   // 3 per sweep
-  double* snaps = starts + 3 * n3 * (size_t)nsw;    // 3 per pair of the sweep being reversed
-  double* T = snaps + 36 * n3;                      // the pre-refresh u or w
+  double* starts = work + 3 * n3;                                        // 3 per sweep
+  double* snaps = keep_all ? work + 3 * n3 : starts + 3 * n3 * (size_t)nsw;  // 36 per sweep
+  double* rows = static_cast<double*>(scratch(45, sizeof(double) * 4 * nw));
+  bp.rows = rows;
   const double uw = *uwall, dyv = *dy;
-  auto sweep_fwd = [&](bool keep) {
+  auto sweep_fwd = [&](double* keep) {
     for (int q = 0; q < 3; q++)
       for (int p = 0; p < 4; p++) {
         if (keep)
-          for (int r = 0; r < 3; r++) copy_dev(snaps + n3 * (size_t)((q * 4 + p) * 3 + r), X[r], d.n3);
+          for (int r = 0; r < 3; r++) copy_dev(keep + n3 * (size_t)((q * 4 + p) * 3 + r), X[r], d.n3);
         if (q == 0) pair_fwd<0>(d, a, sp, X, p);
         if (q == 1) pair_fwd<1>(d, a, sp, X, p);
         if (q == 2) pair_fwd<2>(d, a, sp, X, p);
@@ -1656,28 +1719,30 @@ void solu_sor4_bp_(const int* ipara, const int* ID, const int* nID, const int* n
       }
   };
   for (int r = 0; r < 3; r++) copy_dev(X[r], x0[r], d.n3);
-  for (int s = 0; s < nsw; s++) {
-    for (int r = 0; r < 3; r++) copy_dev(starts + n3 * (size_t)(3 * s + r), X[r], d.n3);
-    if (s + 1 < nsw) sweep_fwd(false);
+  if (keep_all) {
+    for (int s = 0; s < nsw; s++) sweep_fwd(snaps + 36 * n3 * (size_t)s);
+  } else {
+    for (int s = 0; s < nsw; s++) {
+      for (int r = 0; r < 3; r++) copy_dev(starts + n3 * (size_t)(3 * s + r), X[r], d.n3);
+      if (s + 1 < nsw) sweep_fwd(nullptr);
+    }
   }
   for (int s = nsw - 1; s >= 0; s--) {
-    for (int r = 0; r < 3; r++) copy_dev(X[r], starts + n3 * (size_t)(3 * s + r), d.n3);
-    sweep_fwd(true);
+    double* base = snaps;
+    if (keep_all) {
+      base = snaps + 36 * n3 * (size_t)s;
+    } else {
+      for (int r = 0; r < 3; r++) copy_dev(X[r], starts + n3 * (size_t)(3 * s + r), d.n3);
+      sweep_fwd(snaps);
+    }
     for (int q = 2; q >= 0; q--)
       for (int p = 3; p >= 0; p--) {
-        double* S0 = snaps + n3 * (size_t)((q * 4 + p) * 3);
+        double* S0 = base + n3 * (size_t)((q * 4 + p) * 3);
         double* S[3] = {S0, S0 + n3, S0 + 2 * n3};
         // the state before the refresh: the pair's cells of q updated
-        bp.uk = S[0];
-        bp.wk = S[2];
-        if (q != 1) {
-          copy_dev(T, S[q], d.n3);
-          double* Y[3] = {S[0], S[1], S[2]};
-          Y[q] = T;
-          if (q == 0) pair_fwd<0>(d, a, sp, Y, p);
-          if (q == 2) pair_fwd<2>(d, a, sp, Y, p);
-          if (q == 0) bp.uk = T; else bp.wk = T;
-        }
+        if (q == 0) pair_rows<0>(d, a, sp, S, p, rows);
+        if (q == 1) pair_rows<1>(d, a, sp, S, p, rows);
+        if (q == 2) pair_rows<2>(d, a, sp, S, p, rows);
         periodic_adj(d, a.lb[q]);
         bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, dyv, dl1, dl2, dphi, lo, hi, bp);
         const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
