@@ -793,12 +793,31 @@ class FullStep:
 class FullWindow:
     """`steps` whole timesteps with the reference's state rotation
     (srcs/main.f90:687-698: uo = u, u = un, po = p, p = pn, phi = phin) and
-    their reverse, each step recomputed from its stored input state."""
+    their reverse, each step recomputed from its input state before it is
+    reversed.  checkpoint_every = K keeps the input state of every K-th step
+    only (host memory for ceil(steps/K) + K states instead of `steps`): the
+    reverse recomputes each segment of K steps from its checkpoint, keeping
+    its K input states, and reverses them, so the forward runs about twice
+    in all.  K = None (or 1) keeps every input state."""
 
-    def __init__(self, step: FullStep, steps: int):
+    def __init__(self, step: FullStep, steps: int, checkpoint_every: int | None = None):
         self.step = step
         self.steps = steps
-        self.states = []
+        self.every = checkpoint_every if checkpoint_every else 1
+        if self.every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.checkpoints = {}
+        self.forward_steps = 0  # steps run forward, recomputation included
+
+    @staticmethod
+    def _rotate(st: dict, out: dict) -> dict:
+        return {"uo": st["u"], "vo": st["v"], "wo": st["w"], "po": st["p"],
+                "u": out["un"], "v": out["vn"], "w": out["wn"], "p": out["pn"],
+                "phi0": out["phin"][0].copy(), "phi1": out["phin"][1].copy()}
+
+    def _advance(self, st: dict) -> dict:
+        self.forward_steps += 1
+        return self._rotate(st, self.step.forward(st))
 
     def forward(self, state: dict) -> dict:
         """state: FULL_STATE, and optionally "phin1", the contents of
@@ -806,13 +825,12 @@ class FullWindow:
         st = {k: np.array(state[k], dtype=np.float64) for k in FULL_STATE}
         if "phin1" in state:
             st["phin1"] = np.array(state["phin1"], dtype=np.float64)
-        self.states = []
-        for _ in range(self.steps):
-            self.states.append({k: v.copy() for k, v in st.items()})
-            out = self.step.forward(st)
-            st = {"uo": st["u"], "vo": st["v"], "wo": st["w"], "po": st["p"],
-                  "u": out["un"], "v": out["vn"], "w": out["wn"], "p": out["pn"],
-                  "phi0": out["phin"][0].copy(), "phi1": out["phin"][1].copy()}
+        self.checkpoints = {}
+        self.forward_steps = 0
+        for n in range(self.steps):
+            if n % self.every == 0:
+                self.checkpoints[n] = {k: v.copy() for k, v in st.items()}
+            st = self._advance(st)
         return st
 
     def reverse(self, state_b: dict) -> dict:
@@ -820,16 +838,22 @@ class FullWindow:
         (accumulated) from those of its output state."""
         sb = {k: np.array(state_b[k], dtype=np.float64) for k in FULL_STATE}
         acc = {k: 0.0 for k in FULL_PARAMS + ("uwall",)}
-        for n in reversed(range(self.steps)):
-            out_b = {"un": sb["u"], "vn": sb["v"], "wn": sb["w"], "pn": sb["p"],
-                     "phin": np.stack([sb["phi0"], sb["phi1"]])}
-            self.step.forward(self.states[n], keep=False)
-            ib = self.step.reverse(out_b)
-            for k in acc:
-                acc[k] = acc[k] + ib[k]
-            nb = {k: ib[k] for k in FULL_STATE}
-            for s_, o in (("u", "uo"), ("v", "vo"), ("w", "wo"), ("p", "po")):
-                nb[s_] = nb[s_] + sb[o]
-            sb = nb
+        for c0 in sorted(self.checkpoints, reverse=True):
+            # the segment's input states, recomputed from its checkpoint
+            seg = [self.checkpoints[c0]]
+            for _ in range(c0 + 1, min(c0 + self.every, self.steps)):
+                seg.append(self._advance(seg[-1]))
+            for st in reversed(seg):
+                out_b = {"un": sb["u"], "vn": sb["v"], "wn": sb["w"], "pn": sb["p"],
+                         "phin": np.stack([sb["phi0"], sb["phi1"]])}
+                self.step.forward(st, keep=False)
+                self.forward_steps += 1
+                ib = self.step.reverse(out_b)
+                for k in acc:
+                    acc[k] = acc[k] + ib[k]
+                nb = {k: ib[k] for k in FULL_STATE}
+                for s_, o in (("u", "uo"), ("v", "vo"), ("w", "wo"), ("p", "po")):
+                    nb[s_] = nb[s_] + sb[o]
+                sb = nb
         sb.update(acc)
         return sb

@@ -308,3 +308,34 @@ def test_full_window_against_differences(steps):
     lhs = sum(float(np.sum(jv[k] * y[k])) for k in adjoint.FULL_STATE)
     rhs = sum(float(np.sum(v[k] * bx[k])) for k in x0)
     assert abs(lhs - rhs) <= 1e-7 * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.parametrize("every", [1, 2, 3])
+def test_full_window_checkpointing_gives_the_same_adjoint(every):
+    """FullWindow with checkpoints every K steps (the segment's states
+    recomputed in the reverse) gives exactly the adjoint of the window that
+    keeps every state, and runs the forward steps + recomputations it
+    should."""
+    from felmpf import adjoint
+    cfg = phase_cfg()
+    state, params = A.full_inputs(cfg, 361)
+    rng = np.random.default_rng(362)
+    y = {k: rng.standard_normal(cfg.shape) for k in adjoint.FULL_STATE}
+    steps = 4
+    out = {}
+    for K in (None, every):
+        st = adjoint.FullStep(cfg, params, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+        win = adjoint.FullWindow(st, steps, checkpoint_every=K)
+        fwd = win.forward(state)
+        out[K] = (fwd, win.reverse(y), win.forward_steps)
+    f0, b0, n0 = out[None]
+    f1, b1, n1 = out[every]
+    for k in f0:
+        assert np.array_equal(f0[k], f1[k]), k
+    for k in b0:
+        assert np.array_equal(b0[k], b1[k]), k
+    # forward: steps; reverse: one recompute per reversed step plus the
+    # segments' recomputed states (steps - number of checkpoints)
+    ncp = -(-steps // every)
+    assert n0 == 2 * steps
+    assert n1 == 2 * steps + (steps - ncp)
