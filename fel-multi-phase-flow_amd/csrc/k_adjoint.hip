@@ -924,33 +924,50 @@ static void wall_rows(const Dims d, const double* u, const double* w, double* ro
 // Arrays of the all-argument Gauss-Seidel adjoint.
 struct SorParam {
   const double* src[3];  // primal sources
-  double* cfb[45];       // coefficient adjoints
+  double* cfb[45];       // coefficient adjoints (coefficient arrays)
+  double* fb[18];        // face-viscosity adjoints, 6 per component (viscosity form)
   double *rhob, *rhonb;
+  double dinv[3];        // dxinv, dyinv, dzinv (viscosity form)
 };
 
-// The value the Gauss-Seidel update (srcs/solu_sor4.f:131-175, coefficient
-// arrays) gives cell c of component P from the state x0, x1, x2.
-template <int P>
+// The 15 coefficients of component P at cell c: read from the arrays, or
+// (V) formed from the viscosity with the expressions of
+// cal_arith_coef_vis (the same bits).
+template <int P, bool V>
+__device__ __forceinline__ void coefs_at(const Dims& d, const SorAdj& A, int c, double* cf) {
+  if (V) {
+    coef_of<P>(d, A.sc, A.rmu, c, cf);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 15; t++) cf[t] = A.cf[15 * P + t][c];
+  }
+}
+
+// The value the Gauss-Seidel update (srcs/solu_sor4.f:131-175) gives cell c
+// of component P from the state x0, x1, x2.
+template <int P, bool V>
 __device__ __forceinline__ double pair_value(const Dims& d, const SorAdj& A, const double* src,
                                              const double* x0, const double* x1,
                                              const double* x2, int c) {
   int idx[14], comp[14], off[14];
   sor_terms<P>(d, idx, comp, off);
+  double cf[15];
+  coefs_at<P, V>(d, A, c, cf);
   const double* xs[3] = {x0, x1, x2};
   const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
   const double r00 = (A.rho[c] + A.rho[c + F] + A.rhon[c] + A.rhon[c + F]) * 0.25;
   double sum = 0.0;
 #pragma unroll
-  for (int t = 0; t < 14; t++) sum = sum + A.cf[15 * P + idx[t]][c] * xs[comp[t]][c + off[t]];
+  for (int t = 0; t < 14; t++) sum = sum + cf[idx[t]] * xs[comp[t]][c + off[t]];
   const double visn = sum / r00;
   const double xo = xs[P][c];
-  const double du = (visn * 0.5 + src[c]) / (A.dtinv + A.cf[15 * P + 6][c] / r00 * 0.5) - xo;
+  const double du = (visn * 0.5 + src[c]) / (A.dtinv + cf[6] / r00 * 0.5) - xo;
   return xo + du;
 }
 
 // Forward update of colour pair p of component P on x: the pair's cells
 // read no cell of the pair, so one pass.
-template <int P>
+template <int P, bool V>
 static void pair_fwd(const Dims d, const SorAdj& a, const SorParam& sp, double* const* x, int p) {
   const double* src = sp.src[P];
   double* x0 = x[0];
@@ -958,7 +975,7 @@ static void pair_fwd(const Dims d, const SorAdj& a, const SorParam& sp, double* 
   double* x2 = x[2];
   const SorAdj A = a;
   for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
-    const double v = pair_value<P>(d, A, src, x0, x1, x2, c);
+    const double v = pair_value<P, V>(d, A, src, x0, x1, x2, c);
     double* xp = P == 0 ? x0 : (P == 1 ? x1 : x2);
     xp[c] = v;
   });
@@ -968,7 +985,7 @@ static void pair_fwd(const Dims d, const SorAdj& a, const SorParam& sp, double* 
 // component P: the kept state xs before the pair, with the pair's cells of
 // u (P = 0) or w (P = 2) on those rows updated (pair_value, the forward's
 // arithmetic).
-template <int P>
+template <int P, bool V>
 static void pair_rows(const Dims d, const SorAdj& a, const SorParam& sp, const double* const* xs,
                       int p, double* rows) {
   const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, nw = d.sx * d.sz;
@@ -987,12 +1004,33 @@ static void pair_rows(const Dims d, const SorAdj& a, const SorParam& sp, const d
       const bool in_pair = bi == bj && bj == bk && i >= 1 && i <= ni && j >= 1 && j <= jend &&
                            k >= 1 && k <= nk;
       double uu = x0[c], ww = x2[c];
-      if (in_pair && P == 0) uu = pair_value<0>(d, A, src, x0, x1, x2, c);
-      if (in_pair && P == 2) ww = pair_value<2>(d, A, src, x0, x1, x2, c);
+      if (in_pair && P == 0) uu = pair_value<0, V>(d, A, src, x0, x1, x2, c);
+      if (in_pair && P == 2) ww = pair_value<2, V>(d, A, src, x0, x1, x2, c);
       rows[t * nw + m] = uu;
       rows[(2 + t) * nw + m] = ww;
     }
   });
+}
+
+// The adjoints of component P's six face viscosities (f = xm, xp, ym, yp,
+// zm, zp) from its 15 coefficient adjoints cb, the per-cell map
+// coef_vis_adj applies: d2_a (cb_f + cb_p), doubled on P's own axis, plus
+// off it dinv_P dinv_a (cb_A - cb_B) or (-cb_A + cb_B) for the n-th cross
+// pair (7+2n, 8+2n).
+template <int P>
+__device__ __forceinline__ void face_adj_of(const double* dv, const double* cb, double* fa) {
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    const int ax = f / 2;
+    const double base = dv[ax] * dv[ax] * (cb[f] + cb[6]);
+    if (ax == P) {
+      fa[f] = 2.0 * base;
+    } else {
+      const int n = f < 2 * P ? f : f - 2;
+      const double cr = (n % 2 == 0) ? (cb[7 + 2 * n] - cb[8 + 2 * n]) : (-cb[7 + 2 * n] + cb[8 + 2 * n]);
+      fa[f] = base + dv[P] * dv[ax] * cr;
+    }
+  }
 }
 
 // Parameter adjoints of colour pair p of component P, between colour_scale
@@ -1000,9 +1038,11 @@ static void pair_rows(const Dims d, const SorAdj& a, const SorParam& sp, const d
 // x = (0.5 S/r00 + src)/den, den = dtinv + 0.5 a_p/r00, S = sum a_t y_t over
 // the state xs before the pair, g = x_b/den and h = 0.5 g/r00:
 // a_t_b += h y_t, a_p_b -= h x, r00_b = g (0.5 x a_p - 0.5 S)/r00^2 spread
-// over rho and rhon at c and c+F.  No two cells of a pair are F apart, so
-// the scatters do not collide.
-template <int P>
+// over rho and rhon at c and c+F.  V: the coefficient adjoints go straight
+// into the cell's six face-viscosity adjoints (face_adj_of) instead of the
+// 15 coefficient arrays.  No two cells of a pair are F apart, so the
+// scatters do not collide.
+template <int P, bool V>
 static void pair_param(const Dims d, const SorAdj& a, const SorParam& sp,
                        const double* const* xs, int p) {
   const double *rho = a.rho, *rhon = a.rhon, *src = sp.src[P];
@@ -1014,29 +1054,90 @@ static void pair_param(const Dims d, const SorAdj& a, const SorParam& sp,
   for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
     int idx[14], comp[14], off[14];
     sor_terms<P>(d, idx, comp, off);
+    double cf[15];
+    coefs_at<P, V>(d, A, c, cf);
     const double* st[3] = {s0, s1, s2};
     const int F = P == 0 ? 1 : (P == 1 ? d.sx : d.sx * d.sy);
     const double r00 = (rho[c] + rho[c + F] + rhon[c] + rhon[c + F]) * 0.25;
-    const double ap = A.cf[15 * P + 6][c];
+    const double ap = cf[6];
     const double den = dtinv + ap / r00 * 0.5;
+    double y[14];
     double sum = 0.0;
 #pragma unroll
-    for (int t = 0; t < 14; t++) sum = sum + A.cf[15 * P + idx[t]][c] * st[comp[t]][c + off[t]];
+    for (int t = 0; t < 14; t++) {
+      y[t] = st[comp[t]][c + off[t]];
+      sum = sum + cf[idx[t]] * y[t];
+    }
     const double x = (sum / r00 * 0.5 + src[c]) / den;
     const double g = lb[c] / den;
     const double h = g * 0.5 / r00;
+    if (V) {
+      double cb[15];
 #pragma unroll
-    for (int t = 0; t < 14; t++) {
-      double* cb = S.cfb[15 * P + idx[t]];
-      cb[c] = cb[c] + h * st[comp[t]][c + off[t]];
+      for (int t = 0; t < 14; t++) cb[idx[t]] = h * y[t];
+      cb[6] = -(h * x);
+      double fa[6];
+      face_adj_of<P>(S.dinv, cb, fa);
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        double* fb = S.fb[6 * P + f];
+        fb[c] = fb[c] + fa[f];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 14; t++) {
+        double* cb = S.cfb[15 * P + idx[t]];
+        cb[c] = cb[c] + h * y[t];
+      }
+      double* cbp = S.cfb[15 * P + 6];
+      cbp[c] = cbp[c] - h * x;
     }
-    double* cbp = S.cfb[15 * P + 6];
-    cbp[c] = cbp[c] - h * x;
     const double rb = 0.25 * g * (0.5 * x * ap - 0.5 * sum) / (r00 * r00);
     S.rhob[c] = S.rhob[c] + rb;
     S.rhob[c + F] = S.rhob[c + F] + rb;
     S.rhonb[c] = S.rhonb[c] + rb;
     S.rhonb[c + F] = S.rhonb[c + F] + rb;
+  });
+}
+
+// The viscosity adjoint from face-viscosity adjoints fb (6 per component,
+// nonzero only where the coefficients exist): each cell gathers the face
+// adjoints of the cells whose faces include it (the geometry of
+// coef_vis_adj).
+static void face_gather(const Dims d, const double* const* fb, double* rmub) {
+  struct Fb { const double* a[18]; };
+  Fb B;
+  for (int m = 0; m < 18; m++) B.a[m] = fb[m];
+  const int ni = d.ni, nj = d.nj, nk = d.nk;
+  launch_box(Box{-2, ni + 3, -2, nj + 3, -2, nk + 3}, [=] __device__(int i, int j, int k) {
+    auto fa = [&](int q, int f, int ci, int cj, int ck) -> double {
+      if (ci < -1 || ci > ni + 2 || cj < -1 || cj > nj + 2 || ck < -1 || ck > nk + 2) return 0.0;
+      return B.a[6 * q + f][d.ix(ci, cj, ck)];
+    };
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        const int ax = f / 2, side = f % 2;
+        int e_a[3] = {0, 0, 0}, e_q[3] = {0, 0, 0};
+        e_a[ax] = 1;
+        e_q[q] = 1;
+        if (ax == q) {
+          acc = acc + fa(q, f, i - side * e_a[0], j - side * e_a[1], k - side * e_a[2]);
+        } else {
+          const int o[3] = {side ? 0 : -e_a[0], side ? 0 : -e_a[1], side ? 0 : -e_a[2]};
+#pragma unroll
+          for (int al = 0; al < 2; al++)
+#pragma unroll
+            for (int be = 0; be < 2; be++)
+              acc = acc + 0.25 * fa(q, f, i - o[0] - al * e_q[0] - be * e_a[0],
+                                    j - o[1] - al * e_q[1] - be * e_a[1],
+                                    k - o[2] - al * e_q[2] - be * e_a[2]);
+        }
+      }
+    const int c = d.ix(i, j, k);
+    rmub[c] = rmub[c] + acc;
   });
 }
 
@@ -1289,6 +1390,90 @@ static void grad_p2uvw_adj(const Dims d, double dxinv, const double* dyinv_row, 
     gy[c] = 0.0;
     gz[c] = 0.0;
   });
+}
+
+// The reverse sweeps of solu_sor4_bp_ (V false: coefficient arrays and
+// their adjoints) and solu_sor4_bpv_ (V true: coefficients from the
+// viscosity, face-viscosity adjoints), a, sp and bp set up by the entry.
+template <bool V>
+static void sor_bp_run(const Dims d, const int* nID, SorAdj& a, SorParam& sp, BnduParam& bp,
+                       const double* const* x0, const double* dl1, const double* dl2,
+                       const double* dphi, double uwall, double dy, int lp, double* uwallb) {
+  a.G = static_cast<double*>(scratch(24, sizeof(double) * (size_t)d.n3));
+  const bool lo = nID[kYMinus] < 0, hi = nID[kYPlus] < 0;
+  a.jen = hi ? d.nj - 1 : d.nj;
+  const int nw = d.sx * d.sz;
+  double* wacc = static_cast<double*>(scratch(25, sizeof(double) * nw));
+  bp.pa = static_cast<double*>(scratch(43, sizeof(double) * 2 * nw));
+  launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
+  const int nsw = lp < 21 ? lp + 1 : 21;
+  const size_t n3 = (size_t)d.n3;
+  // up to 4 sweeps: every pair's starting state kept in one forward pass;
+  // more: each sweep's start kept and the sweep recomputed before reversal
+  const bool keep_all = nsw <= 4;
+  const size_t arrays = keep_all ? 3 + 36 * (size_t)nsw : 3 + 3 * (size_t)nsw + 36;
+  double* work = static_cast<double*>(scratch(42, sizeof(double) * n3 * arrays));
+  double* X[3] = {work, work + n3, work + 2 * n3};
+  double* starts = work + 3 * n3;                                        // 3 per sweep
+  double* snaps = keep_all ? work + 3 * n3 : starts + 3 * n3 * (size_t)nsw;  // 36 per sweep
+  double* rows = static_cast<double*>(scratch(45, sizeof(double) * 4 * nw));
+  bp.rows = rows;
+  const double uw = uwall, dyv = dy;
+  auto sweep_fwd = [&](double* keep) {
+    for (int q = 0; q < 3; q++)
+      for (int p = 0; p < 4; p++) {
+        if (keep)
+          for (int r = 0; r < 3; r++) copy_dev(keep + n3 * (size_t)((q * 4 + p) * 3 + r), X[r], d.n3);
+        if (q == 0) pair_fwd<0, V>(d, a, sp, X, p);
+        if (q == 1) pair_fwd<1, V>(d, a, sp, X, p);
+        if (q == 2) pair_fwd<2, V>(d, a, sp, X, p);
+        bc_fused<double, double>(d, X[0], X[1], X[2], uw, dyv, dl1, dl2, dphi, lo, hi, q, 7);
+      }
+  };
+  for (int r = 0; r < 3; r++) copy_dev(X[r], x0[r], d.n3);
+  if (keep_all) {
+    for (int s = 0; s < nsw; s++) sweep_fwd(snaps + 36 * n3 * (size_t)s);
+  } else {
+    for (int s = 0; s < nsw; s++) {
+      for (int r = 0; r < 3; r++) copy_dev(starts + n3 * (size_t)(3 * s + r), X[r], d.n3);
+      if (s + 1 < nsw) sweep_fwd(nullptr);
+    }
+  }
+  for (int s = nsw - 1; s >= 0; s--) {
+    double* base = snaps;
+    if (keep_all) {
+      base = snaps + 36 * n3 * (size_t)s;
+    } else {
+      for (int r = 0; r < 3; r++) copy_dev(X[r], starts + n3 * (size_t)(3 * s + r), d.n3);
+      sweep_fwd(snaps);
+    }
+    for (int q = 2; q >= 0; q--)
+      for (int p = 3; p >= 0; p--) {
+        double* S0 = base + n3 * (size_t)((q * 4 + p) * 3);
+        double* S[3] = {S0, S0 + n3, S0 + 2 * n3};
+        // the state before the refresh: the pair's cells of q updated
+        if (q == 0) pair_rows<0, V>(d, a, sp, S, p, rows);
+        if (q == 1) pair_rows<1, V>(d, a, sp, S, p, rows);
+        if (q == 2) pair_rows<2, V>(d, a, sp, S, p, rows);
+        periodic_adj(d, a.lb[q]);
+        bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, dyv, dl1, dl2, dphi, lo, hi, bp);
+        const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
+        if (q == 0) {
+          colour_scale<0, V>(d, a, is, js, ks);
+          pair_param<0, V>(d, a, sp, S, p);
+          colour_gather<0, V>(d, a, is, js, ks);
+        } else if (q == 1) {
+          colour_scale<1, V>(d, a, is, js, ks);
+          pair_param<1, V>(d, a, sp, S, p);
+          colour_gather<1, V>(d, a, is, js, ks);
+        } else {
+          colour_scale<2, V>(d, a, is, js, ks);
+          pair_param<2, V>(d, a, sp, S, p);
+          colour_gather<2, V>(d, a, is, js, ks);
+        }
+      }
+  }
+  adjoint_add_sum(wacc, nw, uwallb);
 }
 
 }  // namespace fel
@@ -1685,83 +1870,86 @@ void solu_sor4_bp_(const int* ipara, const int* ID, const int* nID, const int* n
   bp.l1b = c.io(l1b, d.n3, "l1b");
   bp.l2b = c.io(l2b, d.n3, "l2b");
   bp.phib = c.io(phib, d.n3, "phib");
-  a.G = static_cast<double*>(scratch(24, sizeof(double) * (size_t)d.n3));
   a.dtinv = 1.0 / *dt;
-  const bool lo = nID[kYMinus] < 0, hi = nID[kYPlus] < 0;
-  a.jen = hi ? d.nj - 1 : d.nj;
-  const int nw = d.sx * d.sz;
-  double* wacc = static_cast<double*>(scratch(25, sizeof(double) * nw));
-  bp.pa = static_cast<double*>(scratch(43, sizeof(double) * 2 * nw));
-  launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
   g_last_adj_onfly = 0;
-  const int nsw = *lp < 21 ? *lp + 1 : 21;
-  const size_t n3 = (size_t)d.n3;
-  // up to 4 sweeps: every pair's starting state kept in one forward pass;
-  // more: each sweep's start kept and the sweep recomputed before reversal
-  const bool keep_all = nsw <= 4;
-  const size_t arrays = keep_all ? 3 + 36 * (size_t)nsw : 3 + 3 * (size_t)nsw + 36;
-  double* work = static_cast<double*>(scratch(42, sizeof(double) * n3 * arrays));
-  double* X[3] = {work, work + n3, work + 2 * n3};
-  double* starts = work + 3 * n3;                                        // 3 per sweep
-  double* snaps = keep_all ? work + 3 * n3 : starts + 3 * n3 * (size_t)nsw;  // 36 per sweep
-  double* rows = static_cast<double*>(scratch(45, sizeof(double) * 4 * nw));
-  bp.rows = rows;
-  const double uw = *uwall, dyv = *dy;
-  auto sweep_fwd = [&](double* keep) {
-    for (int q = 0; q < 3; q++)
-      for (int p = 0; p < 4; p++) {
-        if (keep)
-          for (int r = 0; r < 3; r++) copy_dev(keep + n3 * (size_t)((q * 4 + p) * 3 + r), X[r], d.n3);
-        if (q == 0) pair_fwd<0>(d, a, sp, X, p);
-        if (q == 1) pair_fwd<1>(d, a, sp, X, p);
-        if (q == 2) pair_fwd<2>(d, a, sp, X, p);
-        bc_fused<double, double>(d, X[0], X[1], X[2], uw, dyv, dl1, dl2, dphi, lo, hi, q, 7);
-      }
-  };
-  for (int r = 0; r < 3; r++) copy_dev(X[r], x0[r], d.n3);
-  if (keep_all) {
-    for (int s = 0; s < nsw; s++) sweep_fwd(snaps + 36 * n3 * (size_t)s);
-  } else {
-    for (int s = 0; s < nsw; s++) {
-      for (int r = 0; r < 3; r++) copy_dev(starts + n3 * (size_t)(3 * s + r), X[r], d.n3);
-      if (s + 1 < nsw) sweep_fwd(nullptr);
-    }
+  sor_bp_run<false>(d, nID, a, sp, bp, x0, dl1, dl2, dphi, *uwall, *dy, *lp, uwallb);
+  c.finish();
+}
+
+// All-argument adjoint of cal_arith_coef_vis_ followed by solu_sor4_ (the
+// 45 coefficients formed from the viscosity rmu with dxinv, dyinv, dzinv),
+// sweep count frozen at *lp: what solu_sor4_bp_ and then
+// cal_arith_coef_vis_b_ give, without the coefficient arrays -- the
+// coefficients are formed on the fly (the same bits) and each pair cell's
+// coefficient adjoints go straight into its six face-viscosity adjoints,
+// gathered into rmub at the end.  The forward's argument list with the 45
+// coefficients replaced by dxinv, dyinv, dzinv, rmu, rmub; each of rho,
+// rhon, rmu, src_*, un, vn, wn, uwall, l1, l2, phi followed by its adjoint.
+// Device work memory as solu_sor4_bp_'s plus 18 arrays of the padded grid.
+void solu_sor4_bpv_(const int* ipara, const int* ID, const int* nID, const int* ndiv, const int* ni,
+                    const int* nj, const int* nk, const int* key, double* sendjb, double* recvjb,
+                    const int* nstep, const int* imon_t, const double* dt, const double* rho,
+                    double* rhob, const double* rhon, double* rhonb, const double* dxinv,
+                    const double* dyinv, const double* dzinv, const double* rmu, double* rmub,
+                    const double* src_u, double* src_ub, const double* src_v, double* src_vb,
+                    const double* src_w, double* src_wb, const double* un, double* unb,
+                    const double* vn, double* vnb, const double* wn, double* wnb,
+                    const double* uwall, double* uwallb, const double* dy, const double* l1,
+                    double* l1b, const double* l2, double* l2b, const double* phi, double* phib,
+                    const int* lp) {
+  (void)ID; (void)ndiv; (void)key; (void)sendjb; (void)recvjb; (void)nstep; (void)imon_t;
+  if (*ipara != 0) {
+    std::fprintf(stderr, "felgpu: solu_sor4_bpv_: only the single-domain case is supported\n");
+    std::abort();
   }
-  for (int s = nsw - 1; s >= 0; s--) {
-    double* base = snaps;
-    if (keep_all) {
-      base = snaps + 36 * n3 * (size_t)s;
-    } else {
-      for (int r = 0; r < 3; r++) copy_dev(X[r], starts + n3 * (size_t)(3 * s + r), d.n3);
-      sweep_fwd(snaps);
-    }
-    for (int q = 2; q >= 0; q--)
-      for (int p = 3; p >= 0; p--) {
-        double* S0 = base + n3 * (size_t)((q * 4 + p) * 3);
-        double* S[3] = {S0, S0 + n3, S0 + 2 * n3};
-        // the state before the refresh: the pair's cells of q updated
-        if (q == 0) pair_rows<0>(d, a, sp, S, p, rows);
-        if (q == 1) pair_rows<1>(d, a, sp, S, p, rows);
-        if (q == 2) pair_rows<2>(d, a, sp, S, p, rows);
-        periodic_adj(d, a.lb[q]);
-        bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, dyv, dl1, dl2, dphi, lo, hi, bp);
-        const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
-        if (q == 0) {
-          colour_scale<0, false>(d, a, is, js, ks);
-          pair_param<0>(d, a, sp, S, p);
-          colour_gather<0, false>(d, a, is, js, ks);
-        } else if (q == 1) {
-          colour_scale<1, false>(d, a, is, js, ks);
-          pair_param<1>(d, a, sp, S, p);
-          colour_gather<1, false>(d, a, is, js, ks);
-        } else {
-          colour_scale<2, false>(d, a, is, js, ks);
-          pair_param<2>(d, a, sp, S, p);
-          colour_gather<2, false>(d, a, is, js, ks);
-        }
-      }
+  if (*lp < 0 || *lp > 21) {
+    std::fprintf(stderr, "felgpu: solu_sor4_bpv_: lp = %d is not a sweep index of solu_sor4_ (0..21)\n", *lp);
+    std::abort();
   }
-  adjoint_add_sum(wacc, nw, uwallb);
+  adjoint_fp64_only("solu_sor4_bpv_");
+  const Dims d = make_dims(*ni, *nj, *nk);
+  Call c("solu_sor4_bpv_");
+  SorAdj a;
+  SorParam sp;
+  a.rho = c.in(rho, d.n3, "rho");
+  a.rhon = c.in(rhon, d.n3, "rhon");
+  sp.rhob = c.io(rhob, d.n3, "rhob");
+  sp.rhonb = c.io(rhonb, d.n3, "rhonb");
+  a.rmu = c.in(rmu, d.n3, "rmu");
+  double* mb = c.io(rmub, d.n3, "rmub");
+  a.sc = vis_scales<double>(*dxinv, *dyinv, *dzinv);
+  for (int m = 0; m < 45; m++) a.cf[m] = nullptr;
+  for (int m = 0; m < 45; m++) sp.cfb[m] = nullptr;
+  sp.dinv[0] = *dxinv;
+  sp.dinv[1] = *dyinv;
+  sp.dinv[2] = *dzinv;
+  double* faces = static_cast<double*>(scratch(46, sizeof(double) * 18 * (size_t)d.n3));
+  for (int m = 0; m < 18; m++) sp.fb[m] = faces + (size_t)m * d.n3;
+  launch_flat(18 * d.n3, [=] __device__(int m) { faces[m] = 0.0; });
+  sp.src[0] = c.in(src_u, d.n3, "src_u");
+  sp.src[1] = c.in(src_v, d.n3, "src_v");
+  sp.src[2] = c.in(src_w, d.n3, "src_w");
+  a.srcb[0] = c.io(src_ub, d.n3, "src_ub");
+  a.srcb[1] = c.io(src_vb, d.n3, "src_vb");
+  a.srcb[2] = c.io(src_wb, d.n3, "src_wb");
+  const double* x0[3] = {c.in(un, d.n3, "un"), c.in(vn, d.n3, "vn"), c.in(wn, d.n3, "wn")};
+  a.lb[0] = c.io(unb, d.n3, "unb");
+  a.lb[1] = c.io(vnb, d.n3, "vnb");
+  a.lb[2] = c.io(wnb, d.n3, "wnb");
+  const double* dl1 = c.in(l1, d.n3, "l1");
+  const double* dl2 = c.in(l2, d.n3, "l2");
+  const double* dphi = c.in(phi, d.n3, "phi");
+  BnduParam bp;
+  bp.uwall = *uwall;
+  bp.l1b = c.io(l1b, d.n3, "l1b");
+  bp.l2b = c.io(l2b, d.n3, "l2b");
+  bp.phib = c.io(phib, d.n3, "phib");
+  a.dtinv = 1.0 / *dt;
+  g_last_adj_onfly = 1;
+  sor_bp_run<true>(d, nID, a, sp, bp, x0, dl1, dl2, dphi, *uwall, *dy, *lp, uwallb);
+  const double* fb[18];
+  for (int m = 0; m < 18; m++) fb[m] = sp.fb[m];
+  face_gather(d, fb, mb);
   c.finish();
 }
 

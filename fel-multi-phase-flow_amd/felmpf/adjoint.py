@@ -129,11 +129,10 @@ class MomentumStep:
         self.b["pr"] = self.array(np.zeros(self.nm))
         self.b["pi"] = self.array(np.zeros(self.nm))
         if params:
-            # adjoints of the fixed fields (PARAMS) and of the coefficients;
-            # the refreshes' input u and w (bndu_bp_ reads their wall rows)
+            # adjoints of the fixed fields (PARAMS); the refreshes' input u
+            # and w (bndu_bp_ reads their wall rows)
             for k in PARAMS:
                 self.b[k] = z()
-            self.coefb = [z() for _ in range(45)]
             for k in ("ur1", "wr1", "ur2", "wr2"):
                 self.a[k] = z()
         self.lp = None
@@ -324,17 +323,18 @@ class MomentumStep:
                 pairs += [a[k], b[k]]
             for s_, t in (("u", "un"), ("v", "vn"), ("w", "wn")):
                 pairs += [a[s_], b[t]]  # the solve's input state is the copy of u, v, w
+            # the solve and cal_arith_coef_vis reversed together: the
+            # coefficients are those of the viscosity rmun
             head = self._sor_head()
-            c("solu_sor4_bp", *head[:13], f["rho"], b["rho"], f["rhon"], b["rhon"], *self.coef,
-              *self.coefb, *pairs, D(g.uwall), uwallb, D(g.dy), f["l1"], b["l1"], f["l2"],
-              b["l2"], f["phi"], b["phi"], I(self.lp))
+            c("solu_sor4_bpv", *head[:13], f["rho"], b["rho"], f["rhon"], b["rhon"], *dd,
+              f["rmun"], b["rmun"], *pairs, D(g.uwall), uwallb, D(g.dy), f["l1"], b["l1"],
+              f["l2"], b["l2"], f["phi"], b["phi"], I(self.lp))
         else:
             c("solu_sor4_b", *self._sor_head(), b["src_u"], b["src_v"], b["src_w"], b["un"],
               b["vn"], b["wn"], uwallb, *self._bc_tail(), I(self.lp))
         for s_, t in (("u", "un"), ("v", "vn"), ("w", "wn")):
             c("cpy_b", *gh, b[s_], b[t])
         if self.params:
-            c("cal_arith_coef_vis_b", *gh, *dd, b["rmun"], *self.coefb)
             c("cal_srcu_bp", WALLS, *gh, D(0.0), D(g.dt), D(0.0), D(g.grvp), b["u"], b["v"],
               b["w"], f["rho"], b["rho"], f["rhon"], b["rhon"], b["adv_u"], b["adv_v"],
               b["adv_w"], b["adv_uo"], b["adv_vo"], b["adv_wo"], a["prs_u"], b["prs_u"],

@@ -776,6 +776,22 @@ void solu_sor4_bp_(const int *ipara, const int *ID, const int *nID, const int *n
                    double *l1b, const double *l2, double *l2b, const double *phi, double *phib,
                    const int *lp);
 
+/* srcs/cal_arith_coef_vis.f then srcs/solu_sor4.f, all arguments, the 45
+ * coefficients formed from the viscosity rmu (dxinv, dyinv, dzinv): what
+ * solu_sor4_bp_ and then cal_arith_coef_vis_b_ give, without coefficient
+ * arrays; rmub incremented.  Work memory: solu_sor4_bp_'s plus 18 arrays. */
+void solu_sor4_bpv_(const int *ipara, const int *ID, const int *nID, const int *ndiv, const int *ni,
+                    const int *nj, const int *nk, const int *key, double *sendjb, double *recvjb,
+                    const int *nstep, const int *imon_t, const double *dt, const double *rho,
+                    double *rhob, const double *rhon, double *rhonb, const double *dxinv,
+                    const double *dyinv, const double *dzinv, const double *rmu, double *rmub,
+                    const double *src_u, double *src_ub, const double *src_v, double *src_vb,
+                    const double *src_w, double *src_wb, const double *un, double *unb,
+                    const double *vn, double *vnb, const double *wn, double *wnb,
+                    const double *uwall, double *uwallb, const double *dy, const double *l1,
+                    double *l1b, const double *l2, double *l2b, const double *phi, double *phib,
+                    const int *lp);
+
 /* srcs/cal_arith_tau.f:21-118, all arguments: cal_arith_tau_b_ plus rmub. */
 void cal_arith_tau_bp_(const int *ni, const int *nj, const int *nk, const double *s, double *sb,
                        const double *rmu, double *rmub, double *taub);

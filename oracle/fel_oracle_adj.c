@@ -1896,3 +1896,38 @@ void orc_sum_fst_b(const int *ni_, const int *nj_, const int *nk_, double *fub, 
     fwb[m] += swb[m];
   }
 }
+
+/* All-argument adjoint of cal_arith_coef_vis followed by solu_sor4 (the
+ * coefficients formed from the viscosity rmu): the composition of
+ * orc_solu_sor4_bp and orc_cal_arith_coef_vis_b, with the forward's 45
+ * coefficients from orc_cal_arith_coef_vis.  rmub incremented. */
+void orc_solu_sor4_bpv(const int *ipara, const int *ID, const int *nID, const int *ndiv,
+                       const int *ni_, const int *nj_, const int *nk_, const int *key,
+                       double *sendjb, double *recvjb, const int *nstep, const int *imon_t,
+                       const double *dt_, const double *rho, double *rhob, const double *rhon,
+                       double *rhonb, const double *dxinv, const double *dyinv,
+                       const double *dzinv, const double *rmu, double *rmub,
+                       const double *src_u, double *src_ub, const double *src_v, double *src_vb,
+                       const double *src_w, double *src_wb, const double *un, double *unb,
+                       const double *vn, double *vnb, const double *wn, double *wnb,
+                       const double *uwall, double *uwallb, const double *dy, const double *l1,
+                       double *l1b, const double *l2, double *l2b, const double *phi,
+                       double *phib, const int *lp) {
+  ODIMS(*ni_, *nj_, *nk_);
+  double *cf[NCOEF], *cb[NCOEF];
+  for (int m = 0; m < NCOEF; m++) {
+    cf[m] = calloc((size_t)n3, sizeof(double));
+    cb[m] = calloc((size_t)n3, sizeof(double));
+  }
+  orc_cal_arith_coef_vis(ni_, nj_, nk_, dxinv, dyinv, dzinv, rmu, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], cf[6], cf[7], cf[8], cf[9], cf[10], cf[11], cf[12], cf[13], cf[14], cf[15], cf[16], cf[17], cf[18], cf[19], cf[20], cf[21], cf[22], cf[23], cf[24], cf[25], cf[26], cf[27], cf[28], cf[29], cf[30], cf[31], cf[32], cf[33], cf[34], cf[35], cf[36], cf[37], cf[38], cf[39], cf[40], cf[41], cf[42], cf[43], cf[44]);
+  orc_solu_sor4_bp(ipara, ID, nID, ndiv, ni_, nj_, nk_, key, sendjb, recvjb, nstep, imon_t, dt_,
+                   rho, rhob, rhon, rhonb, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], cf[6], cf[7], cf[8], cf[9], cf[10], cf[11], cf[12], cf[13], cf[14], cf[15], cf[16], cf[17], cf[18], cf[19], cf[20], cf[21], cf[22], cf[23], cf[24], cf[25], cf[26], cf[27], cf[28], cf[29], cf[30], cf[31], cf[32], cf[33], cf[34], cf[35], cf[36], cf[37], cf[38], cf[39], cf[40], cf[41], cf[42], cf[43], cf[44],
+                   cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], cb[6], cb[7], cb[8], cb[9], cb[10], cb[11], cb[12], cb[13], cb[14], cb[15], cb[16], cb[17], cb[18], cb[19], cb[20], cb[21], cb[22], cb[23], cb[24], cb[25], cb[26], cb[27], cb[28], cb[29], cb[30], cb[31], cb[32], cb[33], cb[34], cb[35], cb[36], cb[37], cb[38], cb[39], cb[40], cb[41], cb[42], cb[43], cb[44],
+                   src_u, src_ub, src_v, src_vb, src_w, src_wb, un, unb, vn, vnb, wn, wnb, uwall,
+                   uwallb, dy, l1, l1b, l2, l2b, phi, phib, lp);
+  orc_cal_arith_coef_vis_b(ni_, nj_, nk_, dxinv, dyinv, dzinv, rmub, cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], cb[6], cb[7], cb[8], cb[9], cb[10], cb[11], cb[12], cb[13], cb[14], cb[15], cb[16], cb[17], cb[18], cb[19], cb[20], cb[21], cb[22], cb[23], cb[24], cb[25], cb[26], cb[27], cb[28], cb[29], cb[30], cb[31], cb[32], cb[33], cb[34], cb[35], cb[36], cb[37], cb[38], cb[39], cb[40], cb[41], cb[42], cb[43], cb[44]);
+  for (int m = 0; m < NCOEF; m++) {
+    free(cf[m]);
+    free(cb[m]);
+  }
+}

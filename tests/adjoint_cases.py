@@ -1403,3 +1403,67 @@ def full_inputs(cfg, seed: int):
               "l1": r.uniform(0.3, 0.6, phi1.shape) * cfg.dy,
               "l2": r.uniform(0.3, 0.6, phi1.shape) * cfg.dy}
     return state, params
+
+
+class SorPV(FDCase):
+    """solu_sor4_bpv: the adjoint of cal_arith_coef_vis followed by
+    solu_sor4 (coefficients from the viscosity), with respect to the
+    sources, state, uwall, densities, viscosity, slip lengths and phi; sweep
+    count frozen at the unperturbed call's."""
+    name = "solu_sor4_bpv"
+    dt = 0.01
+    h = 1e-6
+
+    def __init__(self, g, seed):
+        super().__init__(g, seed)
+        r = np.random.default_rng(seed + 2100)
+        self.x0 = {"rmu": r.uniform(0.01, 0.2, g.shape)}
+        for k in ("src_u", "src_v", "src_w"):
+            self.x0[k] = r.standard_normal(g.shape)
+        for k in ("un", "vn", "wn"):
+            self.x0[k] = H.smooth_vel(g, r)
+        self.x0.update({"uwall": np.array([0.21]), "rho": r.uniform(0.5, 1.5, g.shape),
+                        "rhon": r.uniform(0.5, 1.5, g.shape), "l1": r.uniform(0.02, 0.1, g.shape),
+                        "l2": r.uniform(0.02, 0.1, g.shape), "phi": r.uniform(0.05, 0.95, g.shape)})
+        self.lps = []
+
+    def head(self):
+        g = self.g
+        return [I(0), I(0), WALLS.copy(), I(1), *grid_head(g), np.zeros(8, np.int32), np.zeros(8),
+                np.zeros(8), I(1), I(0), D(self.dt)]
+
+    def dinv(self):
+        g = self.g
+        return [D(1 / g.dx), D(1 / g.dy), D(1 / g.dz)]
+
+    def out_adjoints(self):
+        return {k: self.rng.standard_normal(self.g.shape) for k in ("un", "vn", "wn")}
+
+    def value(self, call, x):
+        g = self.g
+        coef = [g.zeros() for _ in range(45)]
+        call("cal_arith_coef_vis", *grid_head(g), *self.dinv(), x["rmu"].copy(), *coef)
+        a = {k: x[k].copy() for k in ("un", "vn", "wn")}
+        call("solu_sor4", *self.head(), x["rho"].copy(), x["rhon"].copy(), *coef,
+             *[x[k].copy() for k in ("src_u", "src_v", "src_w")], a["un"], a["vn"], a["wn"],
+             D(float(x["uwall"][0])), D(g.dy), x["l1"].copy(), x["l2"].copy(), x["phi"].copy())
+        reader = sor_lp_gpu if call is H.call_gpu else sor_lp_oracle
+        self.lps.append(reader())
+        return a
+
+    def adjoint(self, call, y):
+        g, x0 = self.g, self.x0
+        self.value(H.call_oracle, x0)
+        lp = self.lps[-1]
+        assert all(v == lp for v in self.lps), self.lps
+        b = {k: g.zeros() for k in x0 if k not in ("un", "vn", "wn", "uwall")}
+        b.update({k: y[k].copy() for k in ("un", "vn", "wn")})
+        ub = D(0.0)
+        pairs = []
+        for k in ("src_u", "src_v", "src_w", "un", "vn", "wn"):
+            pairs += [x0[k], b[k]]
+        call("solu_sor4_bpv", *self.head(), x0["rho"], b["rho"], x0["rhon"], b["rhon"], *self.dinv(),
+             x0["rmu"], b["rmu"], *pairs, D(float(x0["uwall"][0])), ub, D(g.dy), x0["l1"], b["l1"],
+             x0["l2"], b["l2"], x0["phi"], b["phi"], I(lp))
+        b["uwall"] = np.array([ub.value])
+        return b

@@ -433,14 +433,14 @@ def test_property_and_surface_tension_adjoint_matches_oracle(cls, gi):
     check_case(cls(GRIDS[gi], 311 + gi))
 
 
-PARAM_CASES = [A.ArithMean, A.ArithTauP, A.SrcuP, A.BnduP, A.Calfst, A.SorP,
+PARAM_CASES = [A.ArithMean, A.ArithTauP, A.SrcuP, A.BnduP, A.Calfst, A.SorP, A.SorPV,
                lambda g, s: A.Calrn(g, s, (1, 1, 1)), lambda g, s: A.Calrn(g, s, (0, 1, 0))]
 
 
 @pytest.mark.parametrize("gi", range(len(GRIDS)))
 @pytest.mark.parametrize("ci", range(len(PARAM_CASES)),
                          ids=["cal_arith_mean", "cal_arith_tau_bp", "cal_srcu_bp", "bndu_bp", "calfst",
-                              "solu_sor4_bp", "calrn_111", "calrn_010"])
+                              "solu_sor4_bp", "solu_sor4_bpv", "calrn_111", "calrn_010"])
 def test_parameter_adjoint_matches_oracle(ci, gi):
     """The GPU adjoint at x0 equals the oracle's to rounding, and matches
     Richardson central differences of the GPU forward along random
@@ -450,12 +450,12 @@ def test_parameter_adjoint_matches_oracle(ci, gi):
     x, y = case.inputs(), case.out_adjoints()
     bo = case.adjoint(H.call_oracle, {k: v.copy() for k, v in y.items()})
     bg = case.adjoint(H.call_gpu, {k: v.copy() for k, v in y.items()})
-    tol = 1e-11 if case.name == "solu_sor4_bp" else 1e-12
+    tol = 1e-11 if case.name.startswith("solu_sor4") else 1e-12
     assert rel_diff({k: bo[k] for k in x}, {k: bg[k] for k in x}) <= tol
     fx = case.forward(H.call_gpu, x)
     lhs, rhs = A.dot(fx, y), A.dot(x, bg)
     assert abs(lhs - rhs) <= 1e-7 * max(A.scale(fx, y), A.scale(x, bg)), (lhs, rhs)
-    if case.name == "solu_sor4_bp":
+    if case.name.startswith("solu_sor4"):
         assert len(set(case.lps)) == 1, case.lps
 
 

@@ -339,3 +339,13 @@ def test_full_window_checkpointing_gives_the_same_adjoint(every):
     ncp = -(-steps // every)
     assert n0 == 2 * steps
     assert n1 == 2 * steps + (steps - ncp)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_sor_viscosity_adjoint_against_differences(gi):
+    """solu_sor4_bpv (the solve with its coefficients from the viscosity)
+    against Richardson central differences of cal_arith_coef_vis then
+    solu_sor4, at a frozen sweep count."""
+    case = A.SorPV(GRIDS[gi], 371 + gi)
+    dot_check(case, rtol=1e-8)
+    assert len(set(case.lps)) == 1
