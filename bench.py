@@ -463,7 +463,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the multi-GPU decomposition is a design only, not built)", file=sys.stderr)
         sys.exit(2)
 
     import felmpf

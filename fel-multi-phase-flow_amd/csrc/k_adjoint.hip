@@ -1033,8 +1033,9 @@ __device__ __forceinline__ void face_adj_of(const double* dv, const double* cb, 
   }
 }
 
-// Parameter adjoints of colour pair p of component P, between colour_scale
-// and colour_gather (the pair cells' state adjoints still in place): for
+// First launch of colour pair p of component P in the all-argument
+// reverse: colour_scale's src_b += x_b/den and G = x_b/den 0.5/r00, and the
+// parameter adjoints (the pair cells' state adjoints still in place): for
 // x = (0.5 S/r00 + src)/den, den = dtinv + 0.5 a_p/r00, S = sum a_t y_t over
 // the state xs before the pair, g = x_b/den and h = 0.5 g/r00:
 // a_t_b += h y_t, a_p_b -= h x, r00_b = g (0.5 x a_p - 0.5 S)/r00^2 spread
@@ -1071,6 +1072,9 @@ static void pair_param(const Dims d, const SorAdj& a, const SorParam& sp,
     const double x = (sum / r00 * 0.5 + src[c]) / den;
     const double g = lb[c] / den;
     const double h = g * 0.5 / r00;
+    // colour_scale's work, fused: src_b += g, G = h
+    A.srcb[P][c] = A.srcb[P][c] + g;
+    A.G[c] = h;
     if (V) {
       double cb[15];
 #pragma unroll
@@ -1459,15 +1463,12 @@ static void sor_bp_run(const Dims d, const int* nID, SorAdj& a, SorParam& sp, Bn
         bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, dyv, dl1, dl2, dphi, lo, hi, bp);
         const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
         if (q == 0) {
-          colour_scale<0, V>(d, a, is, js, ks);
           pair_param<0, V>(d, a, sp, S, p);
           colour_gather<0, V>(d, a, is, js, ks);
         } else if (q == 1) {
-          colour_scale<1, V>(d, a, is, js, ks);
           pair_param<1, V>(d, a, sp, S, p);
           colour_gather<1, V>(d, a, is, js, ks);
         } else {
-          colour_scale<2, V>(d, a, is, js, ks);
           pair_param<2, V>(d, a, sp, S, p);
           colour_gather<2, V>(d, a, is, js, ks);
         }
