@@ -877,7 +877,8 @@ __device__ __forceinline__ void sor_terms(const Dims& d, int* idx, int* comp, in
 }
 
 // The cells of component P of colour pair p (both colours), as in
-// colour_scale: f(c) for each.
+// colour_scale: f(m, c) for each, m the cell's place in the pair's
+// enumeration (0 .. count-1, the same in every launch).
 template <int P, class F>
 static void for_pair_cells(const Dims d, int jen, int p, F f) {
   const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
@@ -904,7 +905,7 @@ static void for_pair_cells(const Dims d, int jen, int p, F f) {
     const int ii = mm % hx, t = mm / hx, jj = t % hy, kk = t / hy;
     const int c = second ? d.ix(b10 + 2 * ii, b11 + 2 * jj, b12 + 2 * kk)
                          : d.ix(b00 + 2 * ii, b01 + 2 * jj, b02 + 2 * kk);
-    f(c);
+    f(m, c);
   });
 }
 
@@ -974,41 +975,67 @@ static void pair_fwd(const Dims d, const SorAdj& a, const SorParam& sp, double* 
   double* x1 = x[1];
   double* x2 = x[2];
   const SorAdj A = a;
-  for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
+  for_pair_cells<P>(d, a.jen, p, [=] __device__(int, int c) {
     const double v = pair_value<P, V>(d, A, src, x0, x1, x2, c);
     double* xp = P == 0 ? x0 : (P == 1 ? x1 : x2);
     xp[c] = v;
   });
 }
 
-// wall_rows of the state before the refresh that follows colour pair p of
-// component P: the kept state xs before the pair, with the pair's cells of
-// u (P = 0) or w (P = 2) on those rows updated (pair_value, the forward's
-// arithmetic).
-template <int P, bool V>
-static void pair_rows(const Dims d, const SorAdj& a, const SorParam& sp, const double* const* xs,
-                      int p, double* rows) {
-  const int ni = d.ni, nj = d.nj, nk = d.nk, sx = d.sx, nw = d.sx * d.sz;
-  const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
-  const int jend = P == 1 ? a.jen : d.nj;
-  const double* src = sp.src[P];
-  const double *x0 = xs[0], *x1 = xs[1], *x2 = xs[2];
-  const SorAdj A = a;
-  launch_flat(nw, [=] __device__(int m) {
-    const int i = m % sx - 2, k = m / sx - 2;
-    const int jr[2] = {1, nj};
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int j = jr[t], c = d.ix(i, j, k);
-      const int bi = (i ^ is) & 1, bj = (j ^ js) & 1, bk = (k ^ ks) & 1;
-      const bool in_pair = bi == bj && bj == bk && i >= 1 && i <= ni && j >= 1 && j <= jend &&
-                           k >= 1 && k <= nk;
-      double uu = x0[c], ww = x2[c];
-      if (in_pair && P == 0) uu = pair_value<0, V>(d, A, src, x0, x1, x2, c);
-      if (in_pair && P == 2) ww = pair_value<2, V>(d, A, src, x0, x1, x2, c);
-      rows[t * nw + m] = uu;
-      rows[(2 + t) * nw + m] = ww;
-    }
+// The undo log of the all-argument solve adjoint: the forward pass keeps,
+// for every colour pair, the values its update overwrites (the pair's cells
+// of the swept component, in for_pair_cells order) and the values the
+// refresh after it overwrites (every cell outside [1,ni] x [1,nj-1] x
+// [1,nk] of u, v and w: the ghosts, and v's top wall row); the reverse
+// restores them in reverse order, so the working state is, in turn, the
+// state before each refresh and before each pair -- exactly, bit for bit.
+template <int P>
+static void pair_save(const Dims d, int jen, int p, const double* x, double* buf) {
+  for_pair_cells<P>(d, jen, p, [=] __device__(int m, int c) { buf[m] = x[c]; });
+}
+template <int P>
+static void pair_restore(const Dims d, int jen, int p, double* x, const double* buf) {
+  for_pair_cells<P>(d, jen, p, [=] __device__(int m, int c) { x[c] = buf[m]; });
+}
+
+// Offsets of the cells outside [1,ni] x [1,nj-1] x [1,nk] (every cell a
+// refresh writes, and a few it does not), on the device; count in *n.
+static const int* shell_cells(const Dims& d, int* n) {
+  static int cached[3] = {-1, -1, -1};
+  static int count = 0;
+  int* dev = static_cast<int*>(scratch(47, sizeof(int) * (size_t)d.n3));
+  if (cached[0] != d.ni || cached[1] != d.nj || cached[2] != d.nk) {
+    std::vector<int> h;
+    h.reserve((size_t)d.n3 - (size_t)d.ni * (size_t)(d.nj - 1) * (size_t)d.nk);
+    for (int k = -2; k <= d.nk + 3; k++)
+      for (int j = -2; j <= d.nj + 3; j++)
+        for (int i = -2; i <= d.ni + 3; i++)
+          if (i < 1 || i > d.ni || j < 1 || j > d.nj - 1 || k < 1 || k > d.nk) h.push_back(d.ix(i, j, k));
+    count = (int)h.size();
+    FEL_CHECK(hipMemcpyAsync(dev, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, stream()));
+    FEL_CHECK(hipStreamSynchronize(stream()));
+    cached[0] = d.ni;
+    cached[1] = d.nj;
+    cached[2] = d.nk;
+  }
+  *n = count;
+  return dev;
+}
+
+static void shell_save(const int* cells, int n, double* const* x, double* buf) {
+  double *x0 = x[0], *x1 = x[1], *x2 = x[2];
+  launch_flat(3 * n, [=] __device__(int m) {
+    const int q = m / n, t = m - q * n;
+    const double* xq = q == 0 ? x0 : (q == 1 ? x1 : x2);
+    buf[m] = xq[cells[t]];
+  });
+}
+static void shell_restore(const int* cells, int n, double* const* x, const double* buf) {
+  double *x0 = x[0], *x1 = x[1], *x2 = x[2];
+  launch_flat(3 * n, [=] __device__(int m) {
+    const int q = m / n, t = m - q * n;
+    double* xq = q == 0 ? x0 : (q == 1 ? x1 : x2);
+    xq[cells[t]] = buf[m];
   });
 }
 
@@ -1052,7 +1079,7 @@ static void pair_param(const Dims d, const SorAdj& a, const SorParam& sp,
   const double *s0 = xs[0], *s1 = xs[1], *s2 = xs[2];
   const SorAdj A = a;
   const SorParam S = sp;
-  for_pair_cells<P>(d, a.jen, p, [=] __device__(int c) {
+  for_pair_cells<P>(d, a.jen, p, [=] __device__(int, int c) {
     int idx[14], comp[14], off[14];
     sor_terms<P>(d, idx, comp, off);
     double cf[15];
@@ -1412,68 +1439,53 @@ static void sor_bp_run(const Dims d, const int* nID, SorAdj& a, SorParam& sp, Bn
   launch_flat(nw, [=] __device__(int m) { wacc[m] = 0.0; });
   const int nsw = lp < 21 ? lp + 1 : 21;
   const size_t n3 = (size_t)d.n3;
-  // up to 4 sweeps: every pair's starting state kept in one forward pass;
-  // more: each sweep's start kept and the sweep recomputed before reversal
-  const bool keep_all = nsw <= 4;
-  const size_t arrays = keep_all ? 3 + 36 * (size_t)nsw : 3 + 3 * (size_t)nsw + 36;
-  double* work = static_cast<double*>(scratch(42, sizeof(double) * n3 * arrays));
+  int ns = 0;
+  const int* shell = shell_cells(d, &ns);
+  // undo-log slot of one pair: its cells' old values, then the shell's
+  const size_t np = 2 * (size_t)((d.ni + 1) / 2) * (size_t)((d.nj + 1) / 2) * (size_t)((d.nk + 1) / 2);
+  const size_t slot = np + 3 * (size_t)ns;
+  double* work = static_cast<double*>(scratch(42, sizeof(double) * (3 * n3 + slot * 12 * (size_t)nsw)));
   double* X[3] = {work, work + n3, work + 2 * n3};
-  double* starts = work + 3 * n3;                                        // 3 per sweep
-  double* snaps = keep_all ? work + 3 * n3 : starts + 3 * n3 * (size_t)nsw;  // 36 per sweep
+  double* undo = work + 3 * n3;
   double* rows = static_cast<double*>(scratch(45, sizeof(double) * 4 * nw));
   bp.rows = rows;
   const double uw = uwall, dyv = dy;
-  auto sweep_fwd = [&](double* keep) {
+  const int jen = a.jen;
+  for (int r = 0; r < 3; r++) copy_dev(X[r], x0[r], d.n3);
+  for (int s = 0; s < nsw; s++)
     for (int q = 0; q < 3; q++)
       for (int p = 0; p < 4; p++) {
-        if (keep)
-          for (int r = 0; r < 3; r++) copy_dev(keep + n3 * (size_t)((q * 4 + p) * 3 + r), X[r], d.n3);
-        if (q == 0) pair_fwd<0, V>(d, a, sp, X, p);
-        if (q == 1) pair_fwd<1, V>(d, a, sp, X, p);
-        if (q == 2) pair_fwd<2, V>(d, a, sp, X, p);
+        double* u = undo + slot * (size_t)((s * 3 + q) * 4 + p);
+        if (q == 0) { pair_save<0>(d, jen, p, X[0], u); pair_fwd<0, V>(d, a, sp, X, p); }
+        if (q == 1) { pair_save<1>(d, jen, p, X[1], u); pair_fwd<1, V>(d, a, sp, X, p); }
+        if (q == 2) { pair_save<2>(d, jen, p, X[2], u); pair_fwd<2, V>(d, a, sp, X, p); }
+        shell_save(shell, ns, X, u + np);
         bc_fused<double, double>(d, X[0], X[1], X[2], uw, dyv, dl1, dl2, dphi, lo, hi, q, 7);
       }
-  };
-  for (int r = 0; r < 3; r++) copy_dev(X[r], x0[r], d.n3);
-  if (keep_all) {
-    for (int s = 0; s < nsw; s++) sweep_fwd(snaps + 36 * n3 * (size_t)s);
-  } else {
-    for (int s = 0; s < nsw; s++) {
-      for (int r = 0; r < 3; r++) copy_dev(starts + n3 * (size_t)(3 * s + r), X[r], d.n3);
-      if (s + 1 < nsw) sweep_fwd(nullptr);
-    }
-  }
-  for (int s = nsw - 1; s >= 0; s--) {
-    double* base = snaps;
-    if (keep_all) {
-      base = snaps + 36 * n3 * (size_t)s;
-    } else {
-      for (int r = 0; r < 3; r++) copy_dev(X[r], starts + n3 * (size_t)(3 * s + r), d.n3);
-      sweep_fwd(snaps);
-    }
+  for (int s = nsw - 1; s >= 0; s--)
     for (int q = 2; q >= 0; q--)
       for (int p = 3; p >= 0; p--) {
-        double* S0 = base + n3 * (size_t)((q * 4 + p) * 3);
-        double* S[3] = {S0, S0 + n3, S0 + 2 * n3};
-        // the state before the refresh: the pair's cells of q updated
-        if (q == 0) pair_rows<0, V>(d, a, sp, S, p, rows);
-        if (q == 1) pair_rows<1, V>(d, a, sp, S, p, rows);
-        if (q == 2) pair_rows<2, V>(d, a, sp, S, p, rows);
+        double* u = undo + slot * (size_t)((s * 3 + q) * 4 + p);
+        // X: the state before the refresh, then before the pair
+        shell_restore(shell, ns, X, u + np);
+        wall_rows(d, X[0], X[2], rows);
         periodic_adj(d, a.lb[q]);
         bndu_adj(d, a.lb[0], a.lb[1], a.lb[2], wacc, dyv, dl1, dl2, dphi, lo, hi, bp);
         const int is = kIsor[2 * p], js = kJsor[2 * p], ks = kKsor[2 * p];
         if (q == 0) {
-          pair_param<0, V>(d, a, sp, S, p);
+          pair_restore<0>(d, jen, p, X[0], u);
+          pair_param<0, V>(d, a, sp, X, p);
           colour_gather<0, V>(d, a, is, js, ks);
         } else if (q == 1) {
-          pair_param<1, V>(d, a, sp, S, p);
+          pair_restore<1>(d, jen, p, X[1], u);
+          pair_param<1, V>(d, a, sp, X, p);
           colour_gather<1, V>(d, a, is, js, ks);
         } else {
-          pair_param<2, V>(d, a, sp, S, p);
+          pair_restore<2>(d, jen, p, X[2], u);
+          pair_param<2, V>(d, a, sp, X, p);
           colour_gather<2, V>(d, a, is, js, ks);
         }
       }
-  }
   adjoint_add_sum(wacc, nw, uwallb);
 }
 
@@ -1812,13 +1824,13 @@ void bndu_bp_(const int* nID, const int* ni, const int* nj, const int* nk, const
 // coefficients (all 45, then their 45 adjoints), src_*, un, vn, wn (the
 // call's inputs), uwall, l1, l2, phi is followed by its adjoint.  The
 // sweeps are recomputed from the inputs, one sweep at a time before it is
-// reversed, keeping the state before each colour pair; each pair's
-// parameter adjoints are formed at that state (pair_param), and each
+// reversed from an undo log of the forward pass (the values each pair and
+// each refresh overwrite), restored in reverse order: each pair's parameter
+// adjoints are formed at the state before it (pair_param), and each
 // refresh's bndu is reversed with its parameter adjoints at the state
-// before the refresh (the kept state with the pair's cells updated, on the
-// wall rows).  Device work memory, nsw = min(lp+1, 21) sweeps: 3 + 36 nsw
-// arrays of the padded grid for nsw <= 4 (one forward pass keeps every
-// pair's start), else 3 (nsw + 1) + 36 (each sweep recomputed).
+// before the refresh.  Device work memory, nsw = min(lp+1, 21) sweeps:
+// 3 padded arrays plus 12 nsw undo slots of a quarter array and three
+// padded arrays' ghost shells each.
 void solu_sor4_bp_(const int* ipara, const int* ID, const int* nID, const int* ndiv, const int* ni,
                    const int* nj, const int* nk, const int* key, double* sendjb, double* recvjb,
                    const int* nstep, const int* imon_t, const double* dt, const double* rho,

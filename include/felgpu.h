@@ -741,8 +741,8 @@ void bndu_bp_(const int *nID, const int *ni, const int *nj, const int *nk, const
  * solu_sor4_b_): adds the adjoints of rho, rhon, the 45 coefficients (c0..c44
  * then their adjoints c0b..c44b), l1, l2 and phi.  src_*, un, vn, wn are the
  * call's inputs, each followed by its adjoint.  Coefficients are read from
- * the arrays.  Device work memory, nsw = min(lp+1,21): 3 + 36 nsw padded
- * arrays for nsw <= 4, else 3 (nsw + 1) + 36. */
+ * the arrays.  Device work memory, nsw = min(lp+1,21): 3 padded arrays plus
+ * an undo log of 12 nsw (a quarter array + three ghost shells). */
 void solu_sor4_bp_(const int *ipara, const int *ID, const int *nID, const int *ndiv, const int *ni,
                    const int *nj, const int *nk, const int *key, double *sendjb, double *recvjb,
                    const int *nstep, const int *imon_t, const double *dt, const double *rho,
