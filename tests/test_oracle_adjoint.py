@@ -349,3 +349,51 @@ def test_sor_viscosity_adjoint_against_differences(gi):
     case = A.SorPV(GRIDS[gi], 371 + gi)
     dot_check(case, rtol=1e-8)
     assert len(set(case.lps)) == 1
+
+
+def test_wall_gradients_on_a_developed_state_against_differences():
+    """The checkered wall of the patterned-wall case (32x16x32), advanced 2
+    steps by the oracle's own timestep: the gradients of a random function
+    of the state after 3 more steps with respect to each contact-angle
+    patch group and the slip length (l1 = l2 = ls with zeta following it),
+    from one reverse sweep (FullWindow through the oracle's routines),
+    against Richardson central differences of the same window (h = 0.005
+    deg and 1% of the slip length, as for the tangent mode)."""
+    from felmpf import adjoint
+    from felmpf import config as C
+    from felmpf import sensitivity as S
+    from oracle_run import OracleRun
+    cfg = C.patterned_wall().scaled(32, 16, 32)
+    o = OracleRun(cfg)
+    o.steps(2)
+    state = {k: o.field(k) for k in adjoint.FULL_STATE}
+    maps = {"theta0": o.field("theta_0_array"), "zeta": o.field("zeta_array"),
+            "l1": o.field("l1_array"), "l2": o.field("l2_array")}
+    n = 3
+    rng = np.random.default_rng(381)
+    y = {k: rng.standard_normal(cfg.shape) for k in adjoint.FULL_STATE}
+    step = adjoint.FullStep(cfg, maps, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+    win = adjoint.FullWindow(step, n)
+    win.forward(state)
+    bx = win.reverse(y)
+    names = {"theta_0_array": "theta0", "zeta_array": "zeta", "l1_array": "l1", "l2_array": "l2"}
+    seeds = S.parameter_seeds(cfg)
+    lps = []
+
+    def f(seed, t):
+        p = {k: v.copy() for k, v in maps.items()}
+        for name, dirn in seed.items():
+            p[names[name]] = p[names[name]] + t * dirn
+        st = adjoint.FullStep(cfg, p, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+        out = adjoint.FullWindow(st, n).forward(state)
+        lps.append(st.lp)
+        return sum(float(np.sum(out[k] * y[k])) for k in adjoint.FULL_STATE)
+
+    for seed in seeds:
+        h = 0.005 if "theta_0_array" in seed else 0.01 * cfg.bottom.slip
+        d1 = (f(seed, h) - f(seed, -h)) / (2 * h)
+        d2 = (f(seed, 2 * h) - f(seed, -2 * h)) / (4 * h)
+        fd = (4 * d1 - d2) / 3
+        g = sum(float(np.sum(dirn * bx[names[name]])) for name, dirn in seed.items())
+        assert abs(fd - g) <= 1e-6 * abs(fd), (list(seed), fd, g)
+    assert len(set(lps)) == 1, lps
