@@ -257,12 +257,25 @@ def measure(cfg, precision, steps, warmup, keep_fields=False, tangent=False, fp3
     return res
 
 
+def _rep_times(fn, reps, sync):
+    """Per-repetition wall times (ms) of fn() followed by a device sync."""
+    out = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        sync()
+        out.append((time.perf_counter() - t0) * 1e3)
+    return out
+
+
 def measure_adjoint(cfg, warmup, reps, step_ms):
     """The reverse of the velocity-pressure update at a fixed phase field
     (felmpf.adjoint.MomentumStep, every array device-resident) against its
     forward, both on the state and phase of a run after `warmup` steps, with
     result reuse off so that each pass computes every routine.  The reverse
-    timing includes restoring the four output adjoints (device copies)."""
+    timing includes restoring the four output adjoints (device copies).
+    Each figure is the median of `reps` synchronised repetitions (all
+    reported)."""
     import felmpf
     from felmpf import _lib, adjoint
     gpu = _lib.gpu()
@@ -282,30 +295,23 @@ def measure_adjoint(cfg, warmup, reps, step_ms):
         rng = np.random.default_rng(7)
         ybuf = {k: step.array(rng.standard_normal(cfg.shape)) for k in adjoint.OUT}
         g3 = [adjoint.I(cfg.ni), adjoint.I(cfg.nj), adjoint.I(cfg.nk)]
-        step.set_state(state)
-        step.forward_set()
-        gpu.felgpu_sync_()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            step.forward_set()
-        gpu.felgpu_sync_()
-        t1 = time.perf_counter()
-        for k in adjoint.OUT:
-            adjoint.gpu_call("cpy", *g3, ybuf[k], step.b[k])
-        step.reverse_set()
-        gpu.felgpu_sync_()
-        t2 = time.perf_counter()
-        for _ in range(reps):
+        def rev_once():
             for k in adjoint.OUT:
                 adjoint.gpu_call("cpy", *g3, ybuf[k], step.b[k])
             step.reverse_set()
+
+        step.set_state(state)
+        step.forward_set()
         gpu.felgpu_sync_()
-        t3 = time.perf_counter()
+        fwd_t = _rep_times(step.forward_set, reps, gpu.felgpu_sync_)
+        rev_once()
+        gpu.felgpu_sync_()
+        rev_t = _rep_times(rev_once, reps, gpu.felgpu_sync_)
         lp = step.lp
     finally:
         step.close()
         gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(reuse_before)))
-    fwd, rev = (t1 - t0) / reps * 1e3, (t3 - t2) / reps * 1e3
+    fwd, rev = float(np.median(fwd_t)), float(np.median(rev_t))
     return {"scope": "reverse of the velocity-pressure update at a fixed phase field "
                      "(felmpf.adjoint.MomentumStep: WENO5 of u and uo, pressure predictor, "
                      "viscous stress, momentum source, Gauss-Seidel solve, projection, ghost "
@@ -314,6 +320,8 @@ def measure_adjoint(cfg, warmup, reps, step_ms):
             "reverse_vs_forward": round(rev / fwd, 3),
             "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None,
             "sor_sweeps": min(lp + 1, 21), "reps": reps,
+            "forward_ms_reps": [round(t, 3) for t in fwd_t],
+            "reverse_ms_reps": [round(t, 3) for t in rev_t],
             "held_fixed": "volume fraction, densities, viscosities and surface-tension "
                           "forces (the full_step leg reverses their dependence too)"}
 
@@ -340,29 +348,22 @@ def measure_phase_adjoint(cfg, warmup, reps, step_ms):
         rng = np.random.default_rng(8)
         ybuf = step.array(rng.standard_normal((2,) + cfg.shape))
         g3 = [adjoint.I(cfg.ni), adjoint.I(cfg.nj), adjoint.I(cfg.nk)]
-        step.set_state({"phi": phi})
-        step.forward_set()
-        gpu.felgpu_sync_()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            step.forward_set()
-        gpu.felgpu_sync_()
-        t1 = time.perf_counter()
-        for l in (0, 1):
-            adjoint.gpu_call("cpy", *g3, ybuf[l], step.b["phin"][l])
-        step.reverse_set()
-        gpu.felgpu_sync_()
-        t2 = time.perf_counter()
-        for _ in range(reps):
+        def rev_once():
             for l in (0, 1):
                 adjoint.gpu_call("cpy", *g3, ybuf[l], step.b["phin"][l])
             step.reverse_set()
+
+        step.set_state({"phi": phi})
+        step.forward_set()
         gpu.felgpu_sync_()
-        t3 = time.perf_counter()
+        fwd_t = _rep_times(step.forward_set, reps, gpu.felgpu_sync_)
+        rev_once()
+        gpu.felgpu_sync_()
+        rev_t = _rep_times(rev_once, reps, gpu.felgpu_sync_)
     finally:
         step.close()
         gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(reuse_before)))
-    fwd, rev = (t1 - t0) / reps * 1e3, (t3 - t2) / reps * 1e3
+    fwd, rev = float(np.median(fwd_t)), float(np.median(rev_t))
     return {"scope": "reverse of the volume-fraction update at a fixed velocity "
                      "(felmpf.adjoint.PhaseStep: vertex gradients, MTHINC fluxes with the "
                      "Newton offset by the implicit-function theorem, update, truncation and "
@@ -370,7 +371,9 @@ def measure_phase_adjoint(cfg, warmup, reps, step_ms):
                      "the *_b_ entry points; gives d/d(phi, u, v, w, theta_0, zeta, uwall)",
             "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
             "reverse_vs_forward": round(rev / fwd, 3),
-            "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None, "reps": reps}
+            "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None, "reps": reps,
+            "forward_ms_reps": [round(t, 3) for t in fwd_t],
+            "reverse_ms_reps": [round(t, 3) for t in rev_t]}
 
 
 def measure_full_adjoint(cfg, warmup, reps, step_ms):
@@ -405,28 +408,22 @@ def measure_full_adjoint(cfg, warmup, reps, step_ms):
             for l in (0, 1):
                 adjoint.gpu_call("cpy", *g3, ybuf["phin"][l], ph.b["phin"][l])
 
+        def rev_once():
+            restore()
+            step.reverse_set()
+
         step.set_state(state)
         step.forward_set()
         gpu.felgpu_sync_()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            step.forward_set()
+        fwd_t = _rep_times(step.forward_set, reps, gpu.felgpu_sync_)
+        rev_once()
         gpu.felgpu_sync_()
-        t1 = time.perf_counter()
-        restore()
-        step.reverse_set()
-        gpu.felgpu_sync_()
-        t2 = time.perf_counter()
-        for _ in range(reps):
-            restore()
-            step.reverse_set()
-        gpu.felgpu_sync_()
-        t3 = time.perf_counter()
+        rev_t = _rep_times(rev_once, reps, gpu.felgpu_sync_)
         lp = step.lp
     finally:
         step.close()
         gpu.felgpu_set_result_reuse_(ctypes.byref(ctypes.c_int(reuse_before)))
-    fwd, rev = (t1 - t0) / reps * 1e3, (t3 - t2) / reps * 1e3
+    fwd, rev = float(np.median(fwd_t)), float(np.median(rev_t))
     return {"scope": "reverse of the whole step (felmpf.adjoint.FullStep: phase update, "
                      "contact-angle fill of phi(.,0), material properties, surface tension, "
                      "velocity-pressure update with the density, viscosity, force, phi and "
@@ -435,7 +432,9 @@ def measure_full_adjoint(cfg, warmup, reps, step_ms):
             "forward_ms": round(fwd, 3), "reverse_ms": round(rev, 3),
             "reverse_vs_forward": round(rev / fwd, 3),
             "reverse_vs_fp64_step": round(rev / step_ms, 3) if step_ms == step_ms else None,
-            "sor_sweeps": min(lp + 1, 21), "reps": reps}
+            "sor_sweeps": min(lp + 1, 21), "reps": reps,
+            "forward_ms_reps": [round(t, 3) for t in fwd_t],
+            "reverse_ms_reps": [round(t, 3) for t in rev_t]}
 
 
 def main():
@@ -528,8 +527,8 @@ def main():
     adjoint_leg = None
     if both:
         step_ms = main_run["elapsed"] / args.steps * 1e3
-        adjoint_leg = {"momentum": measure_adjoint(cfg, args.warmup, 3, step_ms),
-                       "phase": measure_phase_adjoint(cfg, args.warmup, 3, step_ms)}
+        adjoint_leg = {"momentum": measure_adjoint(cfg, args.warmup, 5, step_ms),
+                       "phase": measure_phase_adjoint(cfg, args.warmup, 5, step_ms)}
         try:
             adjoint_leg["full_step"] = measure_full_adjoint(cfg, args.warmup, 3, step_ms)
         except Exception as e:  # noqa: BLE001 - reported, the main line stands
