@@ -1937,8 +1937,11 @@ void solu_sor4_bpv_(const int* ipara, const int* ID, const int* nID, const int* 
   sp.dinv[1] = *dyinv;
   sp.dinv[2] = *dzinv;
   double* faces = static_cast<double*>(scratch(46, sizeof(double) * 18 * (size_t)d.n3));
-  for (int m = 0; m < 18; m++) sp.fb[m] = faces + (size_t)m * d.n3;
-  launch_flat(18 * d.n3, [=] __device__(int m) { faces[m] = 0.0; });
+  for (int m = 0; m < 18; m++) {
+    double* f = faces + (size_t)m * d.n3;
+    sp.fb[m] = f;
+    launch_flat(d.n3, [=] __device__(int e) { f[e] = 0.0; });
+  }
   sp.src[0] = c.in(src_u, d.n3, "src_u");
   sp.src[1] = c.in(src_v, d.n3, "src_v");
   sp.src[2] = c.in(src_w, d.n3, "src_w");
