@@ -437,6 +437,74 @@ def measure_full_adjoint(cfg, warmup, reps, step_ms):
             "reverse_ms_reps": [round(t, 3) for t in rev_t]}
 
 
+def adjoint_window_run(nsteps, every):
+    """The scale run's FP64 adjoint: the scale-run setup (32 droplets,
+    x-stripes of 60/100 deg, slip 0.02) on 256x256x128 (reference axes), dt
+    frozen at the CFL step of srcs/caldt.f at the start, a window of nsteps
+    whole steps reversed with checkpoints every `every` steps; gradients of
+    the wetted area and of the liquid centroid height with respect to the
+    two stripe angles and the slip length (one reverse sweep each), each
+    against the tangent mode's derivative along the same direction (one
+    tangent run per parameter)."""
+    import dataclasses
+    import felmpf
+    from felmpf import sensitivity as S
+    base = felmpf.CONFIGS["scale_run"]().scaled(256, 128, 256)
+    run = felmpf.GpuRun(base)
+    run.steps(1)
+    dt = run.state()["dt"]
+    run.close()
+    cfg = dataclasses.replace(base, cfl=0.0, dt=dt)
+    seeds = S.parameter_seeds(cfg)
+    theta = S.initial_state(cfg)["theta_0_array"]
+    names = [("theta %g deg" % float(np.max(s["theta_0_array"] * theta)))
+             if "theta_0_array" in s else "slip length" for s in seeds]
+
+    def progress(phase, n):
+        if n % 20 == 0:
+            print(f"adjoint window: {phase} step {n}", file=sys.stderr, flush=True)
+
+    obs = {}
+    phi_end = None
+    for which in ("area", "height"):
+        r = S.observable_gradients(cfg, nsteps, seeds, which, checkpoint_every=every,
+                                   on_step=progress)
+        phi_end = r.pop("phi1")
+        obs[which] = r
+    tangents = {which: [] for which in obs}
+    t_s = []
+    values_equal = True
+    for seed in seeds:
+        t0 = time.perf_counter()
+        tr = felmpf.GpuRun(cfg, tangent=True)
+        try:
+            for name, dirn in seed.items():
+                tr.seed(name, dirn)
+            tr.steps(nsteps)
+            tp = tr.tangent("phi1")
+            values_equal = values_equal and bool(np.allclose(tr.field("phi1"), phi_end, rtol=0,
+                                                             atol=1e-9))
+        finally:
+            tr.close()
+        t_s.append(round(time.perf_counter() - t0, 2))
+        for which, r in obs.items():
+            tangents[which].append(float(np.sum(r["weights"] * tp)))
+    out = {"adjoint_window": "scale_run",
+           "grid": "256x256x128 (ni, nj, nk = %d, %d, %d)" % (cfg.ni, cfg.nj, cfg.nk),
+           "steps": nsteps, "checkpoint_every": every, "dt": dt, "parameters": names,
+           "values_equal_driver": values_equal, "tangent_run_s": t_s}
+    for which, r in obs.items():
+        sc = max(abs(t) for t in tangents[which])
+        out[which] = {"value": r["value"], "gradients_reverse": r["gradients"],
+                      "gradients_tangent": tangents[which],
+                      "abs_diff_over_max": [abs(a - b) / sc for a, b in
+                                            zip(r["gradients"], tangents[which])],
+                      "forward_window_s": round(r["forward_s"], 2),
+                      "reverse_window_s": round(r["reverse_s"], 2),
+                      "forward_steps_run": r["forward_steps_run"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,7 +519,18 @@ def main():
     ap.add_argument("--adjoint", choices=("momentum", "phase", "full_step"), default=None,
                     help="time one reverse-mode leg only (after --warmup steps, --steps "
                          "repetitions) and print its JSON object instead of the bench line")
+    ap.add_argument("--adjoint-window", action="store_true",
+                    help="the scale run's FP64 adjoint: 256x256x128, a --window-steps window "
+                         "with checkpoints every --checkpoint steps, gradients of the wetted "
+                         "area and the centroid height with respect to the two stripe angles "
+                         "and the slip length, each checked against a tangent-mode run; "
+                         "prints one JSON object")
+    ap.add_argument("--window-steps", type=int, default=200)
+    ap.add_argument("--checkpoint", type=int, default=20)
     args = ap.parse_args()
+    if args.adjoint_window:
+        print(json.dumps(adjoint_window_run(args.window_steps, args.checkpoint)))
+        return
     if args.adjoint is not None:
         import felmpf
         cfg = felmpf.CONFIGS[args.config]()

@@ -800,7 +800,8 @@ class FullWindow:
     its K input states, and reverses them, so the forward runs about twice
     in all.  K = None (or 1) keeps every input state."""
 
-    def __init__(self, step: FullStep, steps: int, checkpoint_every: int | None = None):
+    def __init__(self, step: FullStep, steps: int, checkpoint_every: int | None = None,
+                 on_step=None):
         self.step = step
         self.steps = steps
         self.every = checkpoint_every if checkpoint_every else 1
@@ -808,6 +809,7 @@ class FullWindow:
             raise ValueError("checkpoint_every must be >= 1")
         self.checkpoints = {}
         self.forward_steps = 0  # steps run forward, recomputation included
+        self.on_step = on_step  # on_step("forward" | "reverse", n) after each step
 
     @staticmethod
     def _rotate(st: dict, out: dict) -> dict:
@@ -831,6 +833,8 @@ class FullWindow:
             if n % self.every == 0:
                 self.checkpoints[n] = {k: v.copy() for k, v in st.items()}
             st = self._advance(st)
+            if self.on_step:
+                self.on_step("forward", n)
         return st
 
     def reverse(self, state_b: dict) -> dict:
@@ -843,7 +847,7 @@ class FullWindow:
             seg = [self.checkpoints[c0]]
             for _ in range(c0 + 1, min(c0 + self.every, self.steps)):
                 seg.append(self._advance(seg[-1]))
-            for st in reversed(seg):
+            for m, st in reversed(list(enumerate(seg))):
                 out_b = {"un": sb["u"], "vn": sb["v"], "wn": sb["w"], "pn": sb["p"],
                          "phin": np.stack([sb["phi0"], sb["phi1"]])}
                 self.step.forward(st, keep=False)
@@ -855,5 +859,7 @@ class FullWindow:
                 for s_, o in (("u", "uo"), ("v", "vo"), ("w", "wo"), ("p", "po")):
                     nb[s_] = nb[s_] + sb[o]
                 sb = nb
+                if self.on_step:
+                    self.on_step("reverse", c0 + m)
         sb.update(acc)
         return sb

@@ -178,6 +178,56 @@ def driver_start(cfg: RunConfig, state=None):
     return st, maps
 
 
+def observable_weights(cfg: RunConfig, phi: np.ndarray, which: str):
+    """Value and gradient (full padded shape, nonzero on the interior) of a
+    droplet observable of the volume fraction phi (liquid is phi = 0):
+    "area", the wetted area sum over the bottom wall row of (1 - phi) dx dz;
+    "height", the liquid centroid height sum (1 - phi) y / sum (1 - phi)."""
+    g = np.zeros(cfg.shape)
+    if which == "area":
+        da = cfg.dx * cfg.dz
+        g[3:cfg.nk + 3, 3, 3:cfg.ni + 3] = -da
+        return float(np.sum(1.0 - phi[3:cfg.nk + 3, 3, 3:cfg.ni + 3]) * da), g
+    if which == "height":
+        liq = 1.0 - _interior(cfg, phi)
+        y = (np.arange(1, cfg.nj + 1) - 0.5) * cfg.dy
+        vol = float(np.sum(liq))
+        yc = float(np.sum(liq * y[None, :, None])) / vol
+        _interior(cfg, g)[...] = -(y[None, :, None] - yc) / vol
+        return yc, g
+    raise ValueError(which)
+
+
+def observable_gradients(cfg: RunConfig, nsteps: int, seeds, which: str = "area",
+                         checkpoint_every=None, state=None, resident: bool = True,
+                         on_step=None) -> dict:
+    """Gradient of a droplet observable (observable_weights) after `nsteps`
+    steps from the driver's start state, with respect to each parameter
+    direction in `seeds` (parameter_seeds), from one reverse sweep of the
+    whole step (FullWindow, checkpoints every `checkpoint_every` steps).
+    Returns the observable, the gradients and the host wall times of the
+    forward window and of the reverse."""
+    import time
+    from .adjoint import FULL_STATE, FullStep, FullWindow
+    st, maps = driver_start(cfg, state)
+    step = FullStep(cfg, maps, resident=resident)
+    try:
+        win = FullWindow(step, nsteps, checkpoint_every=checkpoint_every, on_step=on_step)
+        t0 = time.perf_counter()
+        out = win.forward(st)
+        t1 = time.perf_counter()
+        value, w = observable_weights(cfg, out["phi1"], which)
+        y = {k: np.zeros(cfg.shape) for k in FULL_STATE}
+        y["phi1"] = w
+        bx = win.reverse(y)
+        t2 = time.perf_counter()
+    finally:
+        step.close()
+    grads = [sum(float(np.sum(d * bx[_MAP_KEYS[n]])) for n, d in seed.items()) for seed in seeds]
+    return {"value": value, "gradients": grads, "forward_s": t1 - t0, "reverse_s": t2 - t1,
+            "forward_steps_run": win.forward_steps, "phi1": out["phi1"], "weights": w}
+
+
 def _make_adjoint_function():
     import torch
 

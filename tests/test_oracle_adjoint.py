@@ -397,3 +397,26 @@ def test_wall_gradients_on_a_developed_state_against_differences():
         g = sum(float(np.sum(dirn * bx[names[name]])) for name, dirn in seed.items())
         assert abs(fd - g) <= 1e-6 * abs(fd), (list(seed), fd, g)
     assert len(set(lps)) == 1, lps
+
+
+@pytest.mark.parametrize("which", ["area", "height"])
+def test_observable_weights_are_gradients(which):
+    """felmpf.sensitivity.observable_weights: the value equals the droplet
+    observables' (wetted area, centroid height), and the weights are its
+    gradient (central differences along a random direction; the area is
+    linear, the height a ratio)."""
+    torch = pytest.importorskip("torch")
+    from felmpf import config as C
+    from felmpf import sensitivity as S
+    cfg = C.patterned_wall().scaled(16, 8, 16)
+    rng = np.random.default_rng(391)
+    phi = rng.uniform(0.0, 1.0, cfg.shape)
+    val, w = S.observable_weights(cfg, phi, which)
+    obs = S.observables(cfg, torch.from_numpy(np.ascontiguousarray(
+        phi[3:cfg.nk + 3, 3:cfg.nj + 3, 3:cfg.ni + 3]))).numpy()
+    assert abs(val - obs[3 if which == "area" else 1]) <= 1e-13 * abs(val)
+    v = rng.standard_normal(cfg.shape)
+    h = 1e-6
+    fd = (S.observable_weights(cfg, phi + h * v, which)[0] -
+          S.observable_weights(cfg, phi - h * v, which)[0]) / (2 * h)
+    assert abs(fd - float(np.sum(w * v))) <= 1e-7 * abs(fd)
