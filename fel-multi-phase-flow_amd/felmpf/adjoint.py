@@ -138,11 +138,12 @@ class MomentumStep:
         self.lp = None
 
     # -- arrays ---------------------------------------------------------
-    def array(self, a: np.ndarray) -> np.ndarray:
-        """Returns a (registered with the library when resident)."""
+    def array(self, a: np.ndarray, upload: bool = True) -> np.ndarray:
+        """Returns a (registered with the library when resident; upload=False
+        leaves the device copy zero-filled instead of copying a's values)."""
         if self.resident:
             lib = _lib.gpu()
-            n, up = ctypes.c_int64(a.nbytes), ctypes.c_int(1)
+            n, up = ctypes.c_int64(a.nbytes), ctypes.c_int(1 if upload else 0)
             lib.felgpu_register_(ctypes.c_void_p(a.ctypes.data), ctypes.byref(n), ctypes.byref(up))
             self._registered.append(a)
         return a
@@ -863,3 +864,162 @@ class FullWindow:
                     self.on_step("reverse", c0 + m)
         sb.update(acc)
         return sb
+
+
+class DeviceFullWindow:
+    """FullWindow with every state, checkpoint and adjoint in device memory
+    (with FullStep(resident=True); with host arrays it runs the same
+    sequence on them): states move between steps by the library's cpy_
+    (device copies), adjoints are summed by cpy_b_ (q1b += q2b), and only
+    the window's input state, the output adjoint and the results cross to
+    the host.  checkpoint_every = K keeps every K-th input state and
+    recomputes the segments in the reverse, as FullWindow does; the slots
+    (one state each: ten padded arrays) are reused, ceil(steps/K) + K + 3
+    at most.  Results are those of FullWindow, bit for bit."""
+
+    def __init__(self, step: FullStep, steps: int, checkpoint_every: int | None = None,
+                 on_step=None):
+        self.step = step
+        self.steps = steps
+        self.every = checkpoint_every if checkpoint_every else 1
+        if self.every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.on_step = on_step
+        self.pool = []
+        self.slots = 0
+        self.checkpoints = {}
+        self.forward_steps = 0
+        g = step.cfg
+        self.g3 = [I(g.ni), I(g.nj), I(g.nk)]
+        self.phin1 = None
+
+    # -- slots ----------------------------------------------------------------
+    def _take(self) -> dict:
+        if self.pool:
+            return self.pool.pop()
+        self.slots += 1
+        return {k: self.step.mo.array(np.zeros(self.step.cfg.shape), upload=False)
+                for k in FULL_STATE}
+
+    def _give(self, slot: dict):
+        self.pool.append(slot)
+
+    def _cp(self, src, dst):
+        self.step.call("cpy", *self.g3, src, dst)
+
+    def _add(self, dst, src):
+        """dst += src (src consumed)."""
+        self.step.call("cpy_b", *self.g3, dst, src)
+
+    # -- forward ----------------------------------------------------------------
+    def _load(self, slot: dict, first: bool):
+        st, ph, mo = self.step, self.step.ph, self.step.mo
+        for k in ("u", "v", "w"):
+            self._cp(slot[k], ph.par[k])
+        self._cp(slot["phi1"], ph.a["phi"])
+        ph.own_phin = first and self.phin1 is not None
+        if ph.own_phin:
+            self._cp(self.phin1, ph.a["phin1_in"])
+        self._cp(slot["phi0"], st.a["phi0"])
+        for k in STATE:
+            self._cp(slot[k], mo.a[k])
+
+    def _advance(self, slot: dict, first: bool) -> dict:
+        """The step from the state in `slot`; returns a new slot with the
+        next state (the reference's rotation)."""
+        st, ph, mo = self.step, self.step.ph, self.step.mo
+        self._load(slot, first)
+        st.forward_set()
+        self.forward_steps += 1
+        nxt = self._take()
+        for s_, o in (("u", "uo"), ("v", "vo"), ("w", "wo"), ("p", "po")):
+            self._cp(slot[s_], nxt[o])
+        for s_, o in (("un", "u"), ("vn", "v"), ("wn", "w"), ("pn", "p")):
+            self._cp(mo.a[s_], nxt[o])
+        self._cp(ph.a["phin"][0], nxt["phi0"])
+        self._cp(ph.a["phin"][1], nxt["phi1"])
+        return nxt
+
+    def forward(self, state: dict) -> dict:
+        """state: FULL_STATE, and optionally "phin1" (PhaseStep.set_state);
+        returns the window's output state (host copies)."""
+        mo = self.step.mo
+        for c in self.checkpoints.values():
+            self._give(c)
+        self.checkpoints = {}
+        self.forward_steps = 0
+        cur = self._take()
+        for k in FULL_STATE:
+            mo._set(cur[k], np.asarray(state[k], dtype=np.float64))
+        self.phin1 = None
+        if "phin1" in state:
+            self.phin1 = mo.array(np.array(state["phin1"], dtype=np.float64))
+        for n in range(self.steps):
+            if n % self.every == 0:
+                self.checkpoints[n] = cur
+            nxt = self._advance(cur, n == 0)
+            if n % self.every != 0:
+                self._give(cur)
+            cur = nxt
+            if self.on_step:
+                self.on_step("forward", n)
+        out = {k: mo._get(cur[k]) for k in FULL_STATE}
+        self._give(cur)
+        return out
+
+    # -- reverse ----------------------------------------------------------------
+    def reverse(self, state_b: dict) -> dict:
+        """Adjoints of the window's input state and of the wall maps
+        (accumulated) from those of its output state (host arrays in, host
+        arrays out)."""
+        st, ph, mo = self.step, self.step.ph, self.step.mo
+        sb = self._take()
+        for k in FULL_STATE:
+            mo._set(sb[k], np.asarray(state_b[k], dtype=np.float64))
+        acc = self._take()  # theta0, zeta, l1, l2 accumulate in four of its arrays
+        for k in FULL_STATE:
+            mo._set(acc[k], np.zeros(st.cfg.shape))
+        slots = {"theta0": "phi0", "zeta": "phi1", "l1": "u", "l2": "v"}
+        uw = 0.0
+        for c0 in sorted(self.checkpoints, reverse=True):
+            seg = [self.checkpoints[c0]]
+            for m in range(c0 + 1, min(c0 + self.every, self.steps)):
+                seg.append(self._advance(seg[-1], m - 1 == 0))
+            for m in reversed(range(len(seg))):
+                n = c0 + m
+                self._load(seg[m], n == 0)
+                st.forward_set()
+                self.forward_steps += 1
+                for s_, o in (("u", "un"), ("v", "vn"), ("w", "wn"), ("p", "pn")):
+                    self._cp(sb[s_], mo.b[o])
+                self._cp(sb["phi0"], ph.b["phin"][0])
+                self._cp(sb["phi1"], ph.b["phin"][1])
+                uw += st.reverse_set()
+                nb = self._take()
+                self._cp(st.b["phi0c"], nb["phi0"])
+                self._cp(ph.b["phi"], nb["phi1"])
+                self._add(nb["phi1"], st.b["phi1"])
+                for k in ("u", "v", "w"):
+                    self._cp(mo.b[k], nb[k])
+                    self._add(nb[k], ph.b[k])
+                for k in ("uo", "vo", "wo", "p", "po"):
+                    self._cp(mo.b[k], nb[k])
+                for s_, o in (("u", "uo"), ("v", "vo"), ("w", "wo"), ("p", "po")):
+                    self._add(nb[s_], sb[o])
+                self._add(acc[slots["theta0"]], ph.b["theta0"])
+                self._add(acc[slots["zeta"]], ph.b["zeta"])
+                self._add(acc[slots["l1"]], mo.b["l1"])
+                self._add(acc[slots["l2"]], mo.b["l2"])
+                self._give(sb)
+                sb = nb
+                if m > 0:
+                    self._give(seg[m])
+                if self.on_step:
+                    self.on_step("reverse", n)
+        out = {k: mo._get(sb[k]) for k in FULL_STATE}
+        for k, a in slots.items():
+            out[k] = mo._get(acc[a])
+        out["uwall"] = np.array([uw])
+        self._give(sb)
+        self._give(acc)
+        return out

@@ -208,11 +208,13 @@ def observable_gradients(cfg: RunConfig, nsteps: int, seeds, which: str = "area"
     Returns the observable, the gradients and the host wall times of the
     forward window and of the reverse."""
     import time
-    from .adjoint import FULL_STATE, FullStep, FullWindow
+    from .adjoint import FULL_STATE, DeviceFullWindow, FullStep, FullWindow
     st, maps = driver_start(cfg, state)
     step = FullStep(cfg, maps, resident=resident)
     try:
-        win = FullWindow(step, nsteps, checkpoint_every=checkpoint_every, on_step=on_step)
+        # device-resident: the states and adjoints stay in device memory
+        cls = DeviceFullWindow if resident else FullWindow
+        win = cls(step, nsteps, checkpoint_every=checkpoint_every, on_step=on_step)
         t0 = time.perf_counter()
         out = win.forward(st)
         t1 = time.perf_counter()

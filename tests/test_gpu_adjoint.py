@@ -608,3 +608,33 @@ def test_observable_gradients_reverse_equals_tangent():
         err = float(np.max(np.abs(t[m] - r[m])))
         print("observable", m, "max |grad|", sc, "max diff", err, flush=True)
         assert err <= 1e-5 * sc + floor, (m, t[m], r[m])
+
+
+def test_device_window_equals_host_window_on_the_gpu():
+    """DeviceFullWindow (every state, checkpoint and adjoint in device
+    memory) against FullWindow (states through host memory), both
+    device-resident on the checkered wall from the driver's start state:
+    the same forward and adjoint, bit for bit."""
+    from felmpf import adjoint
+    from felmpf import sensitivity as S
+    cfg = _patterned()
+    print("adjoint case device window vs host window 32x16x32", flush=True)
+    st, maps = S.driver_start(cfg)
+    rng = np.random.default_rng(411)
+    y = {k: rng.standard_normal(cfg.shape) for k in adjoint.FULL_STATE}
+    res = {}
+    for cls in (adjoint.FullWindow, adjoint.DeviceFullWindow):
+        step = adjoint.FullStep(cfg, maps, resident=True)
+        try:
+            win = cls(step, 4, checkpoint_every=3)
+            f = win.forward(st)
+            b = win.reverse(y)
+        finally:
+            step.close()
+        res[cls.__name__] = (f, b)
+    f0, b0 = res["FullWindow"]
+    f1, b1 = res["DeviceFullWindow"]
+    for k in f0:
+        assert np.array_equal(f0[k], f1[k]), k
+    for k in b0:
+        assert np.array_equal(b0[k], b1[k]), k

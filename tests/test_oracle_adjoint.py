@@ -420,3 +420,31 @@ def test_observable_weights_are_gradients(which):
     fd = (S.observable_weights(cfg, phi + h * v, which)[0] -
           S.observable_weights(cfg, phi - h * v, which)[0]) / (2 * h)
     assert abs(fd - float(np.sum(w * v))) <= 1e-7 * abs(fd)
+
+
+@pytest.mark.parametrize("every", [None, 2])
+def test_device_window_equals_host_window(every):
+    """DeviceFullWindow (states, checkpoints and adjoints moved by cpy_ and
+    summed by cpy_b_, here on host arrays through the oracle's routines)
+    gives FullWindow's forward and adjoint bit for bit, with a start-up
+    phin(.,1) and with checkpoints."""
+    from felmpf import adjoint
+    cfg = phase_cfg()
+    state, params = A.full_inputs(cfg, 401)
+    state["phin1"] = state["phi1"] + 0.01 * np.random.default_rng(402).standard_normal(cfg.shape)
+    rng = np.random.default_rng(403)
+    y = {k: rng.standard_normal(cfg.shape) for k in adjoint.FULL_STATE}
+    res = {}
+    for cls in (adjoint.FullWindow, adjoint.DeviceFullWindow):
+        st = adjoint.FullStep(cfg, params, call=H.call_oracle, lp_reader=A.sor_lp_oracle)
+        win = cls(st, 3, checkpoint_every=every)
+        f = win.forward(state)
+        b = win.reverse(y)
+        res[cls.__name__] = (f, b, win.forward_steps)
+    f0, b0, n0 = res["FullWindow"]
+    f1, b1, n1 = res["DeviceFullWindow"]
+    for k in f0:
+        assert np.array_equal(f0[k], f1[k]), k
+    for k in b0:
+        assert np.array_equal(b0[k], b1[k]), k
+    assert n0 == n1
