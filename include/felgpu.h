@@ -524,16 +524,19 @@ void felgpu_interface_crossings_(const int *ni, const int *nj, const int *nk, co
                                  const int *first_only, double *x, int *cnt);
 
 /* ----------------------------------------------------------------------
- * Adjoints (reverse mode) of the linear routines of the velocity-pressure
- * update.  No counterpart in the reference, which has no adjoint: <name>_b_
- * applies the transpose of the linear map <name>_ applies to its active
- * arrays.  Argument list: the forward's, with every active array replaced
- * by its adjoint (same shape and layout); arrays that only parameterise the
- * map (densities, coefficients, slip lengths, phi) are passed as in the
- * forward.  Adjoints of the forward's inputs are incremented, adjoints of
- * what the forward overwrites are consumed (set to zero), in/out adjoints
- * are updated in place.  FP64 mode only (abort otherwise).  A reverse sweep
- * calls them in the reverse of the forward's order.
+ * Adjoints (reverse mode) of the routines of the timestep.  No counterpart
+ * in the reference, which has no adjoint: <name>_b_ applies the transpose
+ * of the derivative of <name>_ with respect to its active arrays.  Argument
+ * list: the forward's, with every active array replaced by its adjoint
+ * (same shape and layout); arrays that only parameterise the map
+ * (densities, coefficients, slip lengths, phi) are passed as in the
+ * forward, and nonlinear routines take each primal array the derivative
+ * needs followed by its adjoint.  Adjoints of the forward's inputs are
+ * incremented, adjoints of what the forward overwrites are consumed (set to
+ * zero), in/out adjoints are updated in place.  <name>_bp_ (further below)
+ * also forms the adjoints of the parameterising arrays.  FP64 mode only
+ * (abort otherwise).  A reverse sweep calls them in the reverse of the
+ * forward's order.
  * ---------------------------------------------------------------------- */
 
 /* srcs/caldiv.f:27-38 transposed: uk/vk/wkb += from divb on [-1,n+3]^3,
