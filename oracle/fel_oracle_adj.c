@@ -1,7 +1,7 @@
 /*
  * fel_oracle_adj.c -- CPU restatement (plain C, FP64, serial) of the adjoints
- * (reverse-mode derivatives) of the linear routines of the velocity-pressure
- * update of toruneko1129/fel-multi-phase-flow.
+ * (reverse-mode derivatives) of the routines of the timestep of
+ * toruneko1129/fel-multi-phase-flow.
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP library's *_b_ entry
  * points, never part of the product.
@@ -20,7 +20,12 @@
  * z_b += b*x_b, x_b = 0), so it is the exact transpose of the forward's
  * sequence.  pinned: the dot-product identity <J v, w> = <v, J^T w> against
  * the forward oracle on seeded random v, w (tests/test_oracle_adjoint.py);
- * any dropped term, wrong index or transposed coefficient breaks it.
+ * any dropped term, wrong index or transposed coefficient breaks it.  The
+ * nonlinear routines (WENO5, MTHINC, gnbc, the contact angle, the normals,
+ * calfst) and the all-argument adjoints (_bp: the parameterising arrays'
+ * adjoints too) are pinned the same way against Richardson central
+ * differences of the forward oracle, and the reverse of whole windows of
+ * steps likewise.
  */
 #include <math.h>
 #include <stdlib.h>
