@@ -1,5 +1,9 @@
-// k_adjoint.hip -- adjoints (reverse mode) of the ghost fills and of the
-// semi-implicit viscous solve.
+// k_adjoint.hip -- adjoints (reverse mode) of the ghost fills, the
+// semi-implicit viscous solve, the explicit momentum terms, WENO5
+// advection, the material properties and the surface-tension chain, and
+// the all-argument adjoints (<name>_bp_) that also form the adjoints of the
+// densities, viscosity, coefficients, slip lengths and phi.  (The phase
+// transport's are in k_adjoint_vof.hip, the projection's in k_poisson.hip.)
 //
 // The reference has no adjoint.  <name>_b_ applies the transpose of the
 // linear map the drop-in routine <name>_ applies to its active arrays: the
@@ -32,6 +36,12 @@
 //   viscosities in the same order), so the gather evaluates each cell's
 //   own three coefficient sets once; with coefficient arrays it reads the
 //   colour cells' coefficients.
+// * solu_sor4_bp_ / solu_sor4_bpv_: the same reverse with the parameter
+//   adjoints: the sweeps are recomputed once into an undo log (the values
+//   each colour pair and each refresh overwrite), restored pair by pair in
+//   the reverse, so each pair's coefficient and face-density adjoints and
+//   each refresh's slip-length and phi adjoints are formed at the state
+//   the forward saw.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
