@@ -1,22 +1,27 @@
-"""Reverse mode (adjoint) of the velocity-pressure update at a fixed phase field.
+"""Reverse mode (adjoint) of the timestep, through the library's drop-in
+routines and their adjoint entry points (<name>_b_, <name>_bp_).
 
-One timestep of the reference (srcs/main.f90:526-667) maps the flow state
-(u, v, w, uo, vo, wo, p, po) to (un, vn, wn, pn) through WENO5 advection of
-u and uo, the pressure predictor, the explicit viscous stress, the momentum
-source, the Crank-Nicolson Gauss-Seidel solve, the projection and the ghost
-refreshes; the interface enters through the densities, viscosities, the
-surface-tension forces, phi and the slip lengths.  With those held fixed
-(the phase field frozen), MomentumStep runs that sequence through the
-library's drop-in routines and reverses it through their adjoint
-entry points (<name>_b_), in the reverse order:
+* MomentumStep: the velocity-pressure update (srcs/main.f90:526-667) with
+  the phase fields fixed -- WENO5 advection of u and uo, the pressure
+  predictor, the explicit viscous stress, the momentum source, the
+  Crank-Nicolson Gauss-Seidel solve, the projection and the ghost
+  refreshes.  params=True also gives the adjoints of the densities,
+  viscosities, surface-tension forces, phi and slip lengths.
+* PhaseStep: the volume-fraction update (srcs/main.f90:489-517) at a fixed
+  velocity -- vertex gradients, MTHINC fluxes, update, truncation and mass
+  correction, contact angle and its ghosts, summation.
+* FullStep: the whole step (srcs/main.f90:489-694): PhaseStep, the
+  contact-angle fill of phi(.,0), the material properties, the surface
+  tension and MomentumStep(params=True); its reverse gives the adjoints of
+  the state and of the wall maps theta_0, zeta, l1, l2 and uwall.
 
-    forward(state)   -> outputs, recording the primal velocities the WENO5
-                        adjoint needs and the Gauss-Seidel sweep count
-    reverse(out_b)   -> adjoints of the state and of uwall
-
-MomentumWindow chains steps with the reference's state rotation
-(srcs/main.f90:687-694: uo = u, u = un, po = p, p = pn) and reverses a
-window of them, storing each step's input state (the checkpoints).
+Each runs the reference's call sequence forward (forward(state), which
+records what the reverse needs: primal velocities, sweep counts, the
+refreshes' inputs) and reverses it (reverse(out_b)).  The *Window classes
+chain steps with the reference's state rotation (srcs/main.f90:687-698)
+and reverse a window, recomputing each step from its stored input state
+(checkpoint and recompute); DeviceFullWindow keeps every state in device
+memory.
 
 The routines are FP64 (the library must be in its FP64 mode).  `call` is
 the function that runs a drop-in routine by name -- by default the GPU
