@@ -278,12 +278,13 @@ __global__ void __launch_bounds__(512) sor_pair12_kernel(Dims d, SorArgs<T> a1, 
   const int hx = (int)blockDim.x, R = (int)blockDim.y, hy = (d.nj + 1) / 2;
   const int gyr = (hy + R - 1) / R;  // blocks per plane
   const int t = (int)threadIdx.x, L = xcd_tile((int)blockIdx.x, ntiles);
-  int jj, k;
-  if (band > 0) {  // bands of `band` rows, planes in order within a band
+  int jj, k, slot = L;
+  if (band > 0) {  // bands of `band` rows, planes in descending order within a band
     const int per = band * d.nk;
     const int bnd = L / per, rem = L - bnd * per;
-    k = 1 + rem / band;
+    k = d.nk - rem / band;
     jj = bnd * band + rem % band;
+    slot = bnd * per + (k - 1) * band + rem % band;  // the ascending order's slot
   } else {
     jj = (L % gyr) * R + (int)threadIdx.y;
     k = 1 + L / gyr;
@@ -335,8 +336,8 @@ __global__ void __launch_bounds__(512) sor_pair12_kernel(Dims d, SorArgs<T> a1, 
     __syncthreads();
   }
   if (tb == 0) {
-    part1[L] = sh[0][0];
-    part2[L] = sh[1][0];
+    part1[slot] = sh[0][0];
+    part2[slot] = sh[1][0];
   }
 }
 
@@ -438,7 +439,11 @@ void launch_pair12(bool onfly, const Dims& d, const SorArgs<T>& a, int jend, dou
   // XCD (measured at 512x256x128, FP64: 201 us per launch, against 220 with
   // whole planes per XCD range and 228 in dispatch order; bands of 4, 16 and
   // 32 rows 207, 206 and 209 us; the same banding made the single-pair
-  // launch slower)
+  // launch slower).  The planes of a band run in descending order, so the
+  // launch ends on the low planes where the pair-3 launch that follows
+  // starts (same box: pair-3 launch 134 -> 126 us, this launch 202 -> 196
+  // us); each block's partial keeps the ascending order's slot, so the
+  // residual sum is unchanged.
   const int band = hy % 8 == 0 ? 8 : 0;
   const int nt = nb;
   const dim3 grid(nb), block(d.ni / 2, R, 1);
